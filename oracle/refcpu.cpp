@@ -1,0 +1,1704 @@
+// refcpu -- FP64 CPU restatement of the MDcraft force-and-integration path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the correctness oracle and the
+// CPU-baseline leg of bench.py.  Only tests/, __graft_entry__.smoke() and
+// bench.py's cpu_baseline leg may load it, and only as the checker.  The
+// engine in mdcraft_amd/ never links, loads or calls it.
+//
+// It restates the reference semantics (MDcraft, C++17 + Eigen) from the
+// reference source read as text; every block cites the reference file:line
+// it follows.  It keeps the reference's data layout (208-byte AoS Atom
+// records), its full Verlet lists that include self, its linked-cell build,
+// its single-image Periodic ghost rule, its spline quirks and its
+// std::thread chunking rule, so that it can also serve as the timed CPU
+// baseline.  Eigen is absent from this image, so vector algebra is written
+// out by hand with the same evaluation order where the reference spells it
+// out in scalar code.
+//
+// Pinning (see tests/test_oracle_*.py):
+//   * domain / grid / Verlet-list rules: the reference unit-test expectations
+//     (test/cxx/domain/test_domain.cxx, test/cxx/neibs/test_grid.cxx,
+//     test/cxx/neibs/test_verlet_list.cxx) and brute-force O(N^2) pair sets;
+//   * LJsCut: closed forms of test/cxx/solver/test_ljs_cut.cxx:14-118 and
+//     the Ar-ball goldens of test/cxx/solver/test_argon_ball_md.cxx:58-82;
+//   * EAM + Periodic + Verlet: the Al goldens of
+//     test/cxx/solver/test_aluminum_bulk_eam.cxx:74-87, analytic tables
+//     (spline converges to the analytic functions), energy/force finite
+//     differences;
+//   * BPNN (maise_bpnn): parity unpinned against the reference (no reference
+//     test exists and the reference cannot be built here); pinned by
+//     energy/force finite differences, translation/rotation/permutation
+//     invariance and closed-form dimer/trimer descriptor values.
+//
+// Build: see oracle/Makefile (g++ -O2 -std=c++17 -ffp-contract=off).
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <condition_variable>
+#include <cstdint>
+#include <cstring>
+#include <fstream>
+#include <functional>
+#include <limits>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace refcpu {
+
+// ---------------------------------------------------------------------------
+// data::Atom (mdcraft/data/atom.h:21-38): 208-byte record.  V is an Eigen
+// column-major 3x3, stored here as V[col*3 + row].
+// ---------------------------------------------------------------------------
+struct Atom {
+    double r[3];
+    double v[3];
+    double f[3];
+    double V[9];
+    double m;
+    double rcut;
+    double rns;
+    double Ep;
+    double Em;
+    double nc;
+    uint16_t bc;
+    uint16_t kind;
+    uint32_t uid;
+    uint16_t tag;
+    uint16_t pad_[3];
+};
+static_assert(sizeof(Atom) == 208, "Atom must be 208 bytes");
+
+static thread_local std::string g_err;
+
+// ---------------------------------------------------------------------------
+// lattice::Domain (mdcraft/lattice/domain.cxx:9-66)
+// ---------------------------------------------------------------------------
+struct Domain {
+    double bmin[3], bmax[3], size[3];
+    bool periodic[3];
+    int dim;
+
+    Domain(const double lo[3], const double hi[3], const int32_t per[3]) {
+        for (int i = 0; i < 3; ++i) {
+            bmin[i] = lo[i];
+            bmax[i] = hi[i];
+            size[i] = hi[i] - lo[i];
+            periodic[i] = per[i] != 0;
+        }
+        // domain.cxx:19-20
+        dim = std::fabs(lo[2] - hi[2]) > 1e-15 ? 3 : std::fabs(lo[1] - hi[1]) > 1e-15 ? 2 : 1;
+    }
+    // domain.cxx:41-47: single shift, strict comparisons
+    void fit_in_period(double* v) const {
+        for (int i = 0; i < 3; i++) {
+            if (periodic[i])
+                v[i] = v[i] < bmin[i] ? v[i] + size[i] : v[i] > bmax[i] ? v[i] - size[i] : v[i];
+        }
+    }
+    // domain.cxx:49-57
+    double nearest_distance(const double* v, int axis) const {
+        double dist = std::numeric_limits<double>::max();
+        if (periodic[axis]) {
+            double dmin = std::fabs(v[axis] - bmin[axis]);
+            double dmax = std::fabs(v[axis] - bmax[axis]);
+            dist = std::min(dmin, dmax);
+        }
+        return dist;
+    }
+    // domain.cxx:59-66: +L then -L, each only when strictly shorter
+    void shortest(double* v) const {
+        for (int i = 0; i < 3; ++i) {
+            if (!periodic[i]) continue;
+            v[i] = std::fabs(v[i] + size[i]) < std::fabs(v[i]) ? v[i] + size[i] : v[i];
+            v[i] = std::fabs(v[i] - size[i]) < std::fabs(v[i]) ? v[i] - size[i] : v[i];
+        }
+    }
+};
+
+static inline double sqnorm(const double* a) { return a[0] * a[0] + a[1] * a[1] + a[2] * a[2]; }
+
+// ---------------------------------------------------------------------------
+// tools::Threads chunking rule (mdcraft/tools/threads.h:215-220, :280-299):
+// serial when fewer than 2 threads; otherwise n_tasks =
+// max(1, min(10*threads, n/500)) contiguous chunks on a persistent pool.
+// ---------------------------------------------------------------------------
+class Pool {
+public:
+    explicit Pool(int count) : count_(count) {
+        if (count_ >= 2)
+            for (int i = 0; i < count_; ++i) workers_.emplace_back([this] { loop(); });
+    }
+    ~Pool() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto& t : workers_) t.join();
+    }
+    int count() const { return count_; }
+
+    template <class F>
+    void for_n(size_t n, F&& func) {
+        if (count_ < 2) {
+            for (size_t i = 0; i < n; ++i) func(i);
+            return;
+        }
+        if (n < 1) return;
+        int n_tasks = std::max(1, std::min(10 * count_, int(n / 500)));
+        size_t bin = n / n_tasks;
+        std::atomic<int> left(n_tasks);
+        std::mutex dm;
+        std::condition_variable dcv;
+        size_t from = 0;
+        for (int t = 0; t < n_tasks; ++t) {
+            size_t to = (t == n_tasks - 1) ? n : from + bin;
+            push([&, from, to] {
+                for (size_t i = from; i < to; ++i) func(i);
+                if (left.fetch_sub(1) == 1) {
+                    std::lock_guard<std::mutex> lk(dm);
+                    dcv.notify_all();
+                }
+            });
+            from = to;
+        }
+        std::unique_lock<std::mutex> lk(dm);
+        dcv.wait(lk, [&] { return left.load() == 0; });
+    }
+
+private:
+    void push(std::function<void()> f) {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            q_.push_back(std::move(f));
+        }
+        cv_.notify_one();
+    }
+    void loop() {
+        for (;;) {
+            std::function<void()> f;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [this] { return stop_ || !q_.empty(); });
+                if (stop_ && q_.empty()) return;
+                f = std::move(q_.front());
+                q_.erase(q_.begin());
+            }
+            f();
+        }
+    }
+    int count_;
+    std::vector<std::thread> workers_;
+    std::vector<std::function<void()>> q_;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    bool stop_ = false;
+};
+
+// ---------------------------------------------------------------------------
+// neibs::Grid (mdcraft/neibs/grid.cxx)
+// ---------------------------------------------------------------------------
+struct Grid {
+    const Domain* dom = nullptr;
+    int domain_num_cells[3] = {1, 1, 1};
+    double cell_sizes[3] = {0, 0, 0};
+    int grid_offset[3] = {0, 0, 0};
+    int num_cells[3] = {1, 1, 1};
+
+    // grid.cxx:209-243 find_bounds + grid.cxx:43-95 build
+    void build(const Atom* atoms, size_t n) {
+        double max_rns = 0.0;
+        double mn[3] = {std::numeric_limits<double>::max(), std::numeric_limits<double>::max(),
+                        std::numeric_limits<double>::max()};
+        double mx[3] = {std::numeric_limits<double>::lowest(), std::numeric_limits<double>::lowest(),
+                        std::numeric_limits<double>::lowest()};
+        for (size_t i = 0; i < n; ++i) {
+            max_rns = std::max(max_rns, atoms[i].rns);
+            for (int k = 0; k < 3; ++k) {
+                mn[k] = std::min(mn[k], atoms[i].r[k]);
+                mx[k] = std::max(mx[k], atoms[i].r[k]);
+            }
+        }
+        double bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
+        for (int i = 0; i < dom->dim; ++i) {
+            bmin[i] = mn[i];
+            bmax[i] = mx[i];
+        }
+        for (int i = 0; i < 3; ++i) domain_num_cells[i] = 1;
+        for (int i = 0; i < dom->dim; ++i)
+            domain_num_cells[i] = std::max(1, static_cast<int>(std::floor(dom->size[i] / max_rns)));
+        for (int i = 0; i < 3; ++i) cell_sizes[i] = dom->size[i];
+        for (int i = 0; i < dom->dim; ++i) cell_sizes[i] /= domain_num_cells[i];
+        global_index3D(bmin, grid_offset);
+        int upper[3];
+        global_index3D(bmax, upper);
+        for (int i = 0; i < dom->dim; ++i) upper[i] = std::min(upper[i] + 1, domain_num_cells[i]);
+        for (int i = 0; i < 3; ++i) num_cells[i] = 1;
+        for (int i = 0; i < dom->dim; ++i) num_cells[i] = upper[i] - grid_offset[i];
+    }
+    // grid.cxx:107-112
+    void global_index3D(const double* v, int* c) const {
+        c[0] = c[1] = c[2] = 0;
+        for (int i = 0; i < dom->dim; ++i)
+            c[i] = static_cast<int>(std::floor((v[i] - dom->bmin[i]) / cell_sizes[i]));
+    }
+    void local_index3D(const double* v, int* c) const {
+        global_index3D(v, c);
+        for (int i = 0; i < dom->dim; ++i) c[i] -= grid_offset[i];
+    }
+    int cell_index(const int* c) const { return c[0] + num_cells[0] * (c[1] + num_cells[1] * c[2]); }
+    int cells_number() const { return num_cells[0] * num_cells[1] * num_cells[2]; }
+    double cell_size() const { return std::max(cell_sizes[0], std::max(cell_sizes[1], cell_sizes[2])); }
+
+    // grid.cxx:138-207: offsets x, y, z in {-1,0,1}; dedupe for 1 or 2
+    // periodic cells; is_edge from the query's global index vs the local
+    // cell count.
+    int adjacent_cells(const double* r, int* list, bool* is_edge) const {
+        int cell[3];
+        global_index3D(r, cell);
+        const int nx = domain_num_cells[0], ny = domain_num_cells[1], nz = domain_num_cells[2];
+        int sz = 0;
+        for (int x = -1; x <= 1; ++x) {
+            if (dom->periodic[0]) {
+                if (nx == 1 && x != 0) continue;
+                if (nx == 2 && x == 1) continue;
+            }
+            for (int y = -1; y <= 1; ++y) {
+                if (dom->periodic[1]) {
+                    if (ny == 1 && y != 0) continue;
+                    if (ny == 2 && y == 1) continue;
+                }
+                for (int z = -1; z <= 1; ++z) {
+                    if (dom->periodic[2]) {
+                        if (nz == 1 && z != 0) continue;
+                        if (nz == 2 && z == 1) continue;
+                    }
+                    int nb[3] = {cell[0] + x, cell[1] + y, cell[2] + z};
+                    if (dom->periodic[0]) nb[0] = (nb[0] + nx) % nx;
+                    if (dom->periodic[1]) nb[1] = (nb[1] + ny) % ny;
+                    if (dom->periodic[2]) nb[2] = (nb[2] + nz) % nz;
+                    nb[0] -= grid_offset[0];
+                    nb[1] -= grid_offset[1];
+                    nb[2] -= grid_offset[2];
+                    if (0 <= nb[0] && nb[0] < num_cells[0] && 0 <= nb[1] && nb[1] < num_cells[1] && 0 <= nb[2] &&
+                        nb[2] < num_cells[2])
+                        list[sz++] = cell_index(nb);
+                }
+            }
+        }
+        is_edge[0] = is_edge[1] = is_edge[2] = false;
+        for (int i = 0; i < dom->dim; ++i) is_edge[i] = cell[i] == 0 || cell[i] == num_cells[i] - 1;
+        return sz;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// neibs::VerletList (mdcraft/neibs/verlet-list.cxx)
+// ---------------------------------------------------------------------------
+struct Saved {
+    double r[3];
+    double rcut, rns;
+    uint32_t uid;
+};
+
+struct VerletList {
+    Pool* pool;
+    bool half;
+    std::vector<std::vector<size_t>> nlist;
+    std::vector<Saved> saved;
+    Grid grid;
+    std::vector<long> head, next;
+
+    // verlet-list.cxx:374-400 are_neibs: d^2 <= max(rns_a, rns_b)^2 with a
+    // single shift on axes where a's cell is an edge cell and the axis is
+    // periodic; the sum is evaluated left to right without contraction.
+    static bool are_neibs(const Domain& d, const Atom& a, const Atom& b, const bool* is_edge) {
+        double rr = std::max(a.rns, b.rns);
+        double d2 = rr * rr;
+        double drx = a.r[0] - b.r[0];
+        double dry = a.r[1] - b.r[1];
+        double drz = a.r[2] - b.r[2];
+        const double xs = d.size[0], ys = d.size[1], zs = d.size[2];
+        if (is_edge[0] && d.periodic[0] && (2 * drx < -xs)) drx += xs;
+        if (is_edge[0] && d.periodic[0] && (2 * drx > xs)) drx -= xs;
+        if (is_edge[1] && d.periodic[1] && (2 * dry < -ys)) dry += ys;
+        if (is_edge[1] && d.periodic[1] && (2 * dry > ys)) dry -= ys;
+        if (is_edge[2] && d.periodic[2] && (2 * drz < -zs)) drz += zs;
+        if (is_edge[2] && d.periodic[2] && (2 * drz > zs)) drz -= zs;
+        return drx * drx + dry * dry + drz * drz <= d2;
+    }
+
+    // verlet-list.cxx:72-92
+    void optimize_search_radius(Atom* atoms, size_t n, double kbuf) {
+        pool->for_n(n, [&](size_t i) {
+            double hmax = 0.0;
+            if (i < nlist.size())
+                for (size_t k = 0; k < nlist[i].size(); ++k) {
+                    const Atom& nb = atoms[nlist[i][k]];
+                    double h = atoms[i].rcut + nb.rcut;
+                    hmax = std::max(h, hmax);
+                }
+            atoms[i].rns = kbuf * hmax;
+        });
+    }
+
+    // verlet-list.cxx:306-355 build_linked_cells
+    void build_linked_cells(Atom* atoms, size_t n, const Domain& dom) {
+        if (n == 0) return;
+        pool->for_n(n, [&](size_t i) { dom.fit_in_period(atoms[i].r); });
+        grid.dom = &dom;
+        grid.build(atoms, n);
+        const size_t n_cells = grid.cells_number();
+        if (n_cells == 0) return;
+        next.assign(n, -1);
+        head.assign(n_cells, -1);
+        for (size_t i = 0; i < n; ++i) {
+            int c[3];
+            grid.local_index3D(atoms[i].r, c);
+            long cid = grid.cell_index(c);
+            next[i] = head[cid];
+            head[cid] = (long)i;
+        }
+        pool->for_n(n, [&](size_t i) {
+            int adj[27];
+            bool is_edge[3];
+            int na = grid.adjacent_cells(atoms[i].r, adj, is_edge);
+            for (int a = 0; a < na; ++a) {
+                for (long j = head[adj[a]]; j != -1; j = next[j]) {
+                    if (half && (size_t)j < i) continue;
+                    if (are_neibs(dom, atoms[i], atoms[j], is_edge)) nlist[i].push_back((size_t)j);
+                }
+            }
+        });
+    }
+
+    // verlet-list.cxx:160-175
+    void snapshot(const Atom* atoms, size_t n) {
+        saved.resize(n);
+        for (size_t i = 0; i < n; ++i) {
+            std::memcpy(saved[i].r, atoms[i].r, sizeof(saved[i].r));
+            saved[i].rcut = atoms[i].rcut;
+            saved[i].rns = atoms[i].rns;
+            saved[i].uid = atoms[i].uid;
+        }
+    }
+
+    // verlet-list.cxx:41-55
+    void update(Atom* atoms, size_t n, const Domain& dom, double kbuf) {
+        if (n == 0) {
+            nlist.assign(0, {});
+            snapshot(atoms, n);
+            return;
+        }
+        if (kbuf > 0.0) optimize_search_radius(atoms, n, kbuf);
+        nlist.assign(n, {});
+        for (auto& l : nlist) l.reserve(128);
+        build_linked_cells(atoms, n, dom);
+        snapshot(atoms, n);
+    }
+
+    // verlet-list.cxx:110-158
+    bool decide(const Atom* atoms, size_t n, const Domain& dom) const {
+        if (n != saved.size()) return true;
+        if (n == 0) return false;
+        std::atomic<bool> need(false);
+        pool->for_n(n, [&](size_t i) {
+            if (need.load(std::memory_order_relaxed)) return;
+            const Atom& a = atoms[i];
+            const Saved& s = saved[i];
+            if (a.uid != s.uid || a.rcut != s.rcut || a.rns != s.rns) {
+                need.store(true);
+                return;
+            }
+            const double skin = s.rns - s.rcut;
+            if (skin <= 0.0) {
+                need.store(true);
+                return;
+            }
+            double dr[3] = {a.r[0] - s.r[0], a.r[1] - s.r[1], a.r[2] - s.r[2]};
+            dom.shortest(dr);
+            if (sqnorm(dr) > 0.25 * skin * skin) need.store(true);
+        });
+        return need.load();
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Potentials (mdcraft/solver/potential/base.h:15-65)
+// ---------------------------------------------------------------------------
+enum PotType { PT_PAIR = 1, PT_EAM = 2, PT_TERNARY = 4 };
+
+struct Potential {
+    int type;
+    double R1cut = 0, R2cut = 0;
+    bool half = false;  // the list the per-atom calls receive is a half list
+    explicit Potential(int t) : type(t) {}
+    virtual ~Potential() {}
+    virtual void force(Atom& atom, Atom* neibs, const size_t* list, size_t nl) {}
+    virtual void virial(Atom& atom, Atom* neibs, const size_t* list, size_t nl) {}
+    virtual void density(Atom& atom, Atom* neibs, const size_t* list, size_t nl) {}
+};
+
+// LJsCut (mdcraft/solver/potential/LJs-cut.cxx:8-131)
+struct LJsCut : Potential {
+    double a_, s_, s6_, lj1_, lj2_, factor_lj_;
+    LJsCut(double a, double s, double rcut) : Potential(PT_PAIR) {
+        R1cut = rcut;
+        R2cut = rcut * rcut;
+        a_ = a;
+        s_ = s;
+        s6_ = std::pow(s, 6);
+        lj1_ = 2 * s6_;
+        lj2_ = 1.;
+        factor_lj_ = 48. * a_ * s6_;
+    }
+    void force(Atom& atom, Atom* neibs, const size_t* list, size_t nl) override {
+        if (nl == 0) return;
+        const double x = atom.r[0], y = atom.r[1], z = atom.r[2];
+        double fx = 0, fy = 0, fz = 0;
+        for (size_t k = 0; k < nl; ++k) {
+            Atom& nb = neibs[list[k]];
+            double drx = nb.r[0] - x, dry = nb.r[1] - y, drz = nb.r[2] - z;
+            double r2 = drx * drx + dry * dry + drz * drz;
+            if (r2 > R2cut || r2 < 1e-15) continue;
+            double r2inv = 1.0 / r2;
+            double r6inv = r2inv * r2inv * r2inv;
+            double prefact = -0.5 * factor_lj_ * (lj1_ * r6inv - lj2_) * r6inv * r2inv;
+            fx += drx * prefact;
+            fy += dry * prefact;
+            fz += drz * prefact;
+            if (half) {
+                nb.f[0] -= drx * prefact;
+                nb.f[1] -= dry * prefact;
+                nb.f[2] -= drz * prefact;
+            }
+        }
+        atom.f[0] += fx;
+        atom.f[1] += fy;
+        atom.f[2] += fz;
+    }
+    void virial(Atom& atom, Atom* neibs, const size_t* list, size_t nl) override {
+        if (nl == 0) return;
+        const double x = atom.r[0], y = atom.r[1], z = atom.r[2];
+        double Vxx = 0, Vyy = 0, Vzz = 0, Vxy = 0, Vxz = 0, Vyz = 0;
+        for (size_t k = 0; k < nl; ++k) {
+            Atom& nb = neibs[list[k]];
+            double drx = nb.r[0] - x, dry = nb.r[1] - y, drz = nb.r[2] - z;
+            double r2 = drx * drx + dry * dry + drz * drz;
+            if (r2 > R2cut || r2 < 1e-15) continue;
+            double r2inv = 1.0 / r2;
+            double r6inv = r2inv * r2inv * r2inv;
+            double prefact = -0.5 * factor_lj_ * (lj1_ * r6inv - lj2_) * r6inv * r2inv;
+            double fx = prefact * drx, fy = prefact * dry, fz = prefact * drz;
+            Vxx += fx * drx;
+            Vyy += fy * dry;
+            Vzz += fz * drz;
+            Vxy += fx * dry;
+            Vxz += fx * drz;
+            Vyz += fy * drz;
+            double tmp = s6_ * r6inv;
+            double add_Ep = 0.5 * 4 * a_ * tmp * (tmp - 1);
+            atom.Ep += add_Ep;
+            if (half) {
+                // LJs-cut.cxx:113-121 (rows f, columns dr)
+                double fr[3] = {fx, fy, fz}, dd[3] = {drx, dry, drz};
+                for (int row = 0; row < 3; ++row)
+                    for (int col = 0; col < 3; ++col) nb.V[col * 3 + row] += 0.5 * (fr[row] * dd[col]);
+                nb.Ep += add_Ep;
+            }
+        }
+        // LJs-cut.cxx:126-130: symmetric matrix
+        double M[9] = {Vxx, Vxy, Vxz, Vxy, Vyy, Vyz, Vxz, Vyz, Vzz};
+        for (int k = 0; k < 9; ++k) atom.V[k] += 0.5 * M[k];
+    }
+};
+
+// Physical constants of mdcraft/solver/potential/EAM.cxx:8-13
+static const double kNa = 6.02214076e+23;
+static const double kQe = 1.602176634e-19;
+static const double kUnMass = 1e-3 / kNa;
+static const double kUnLeng = 1e-9;
+static const double kUnTime = 1e-12;
+static const double kUnEnrg = kUnMass * (kUnLeng / kUnTime) * (kUnLeng / kUnTime);
+
+// EAM (mdcraft/solver/potential/EAM.cxx)
+struct EAM : Potential {
+    int Nr1 = 0, Nrh1 = 0;
+    double dr = 0, drh = 0, dr1 = 0, drh1 = 0;
+    // Tables carry one guard element so that the reference's read of
+    // [N1+1] at N1 = N-1 (EAM.cxx:152-157) stays defined; the guard repeats
+    // the last knot.
+    std::vector<double> Femb, dFemb, Rho, dRho, Vpar, dVpar;
+
+    EAM() : Potential(PT_EAM) {}
+
+    void finish() {
+        dr1 = 1.0 / dr;
+        drh1 = 1.0 / drh;
+        dVpar.assign(Nr1, 0.0);
+        dRho.assign(Nr1, 0.0);
+        dFemb.assign(Nrh1, 0.0);
+        init_spline(Vpar.data(), dVpar.data(), dr1, Nr1, 1);
+        init_spline(Rho.data(), dRho.data(), dr1, Nr1, 1);
+        init_spline(Femb.data(), dFemb.data(), drh1, Nrh1, 2);
+        auto guard = [](std::vector<double>& t) { t.push_back(t.back()); };
+        guard(Vpar);
+        guard(dVpar);
+        guard(Rho);
+        guard(dRho);
+        guard(Femb);
+        guard(dFemb);
+    }
+
+    // EAM.cxx:15-53 (table constructor; tables in reference units)
+    static EAM* from_tables(double rc, const double* F, const double* rho, const double* rphi, int nr, double dr_,
+                            int nrho, double drho) {
+        auto* p = new EAM();
+        p->R1cut = rc;
+        p->R2cut = rc * rc;
+        p->Nr1 = nr;
+        p->Nrh1 = nrho;
+        p->dr = dr_;
+        p->drh = drho;
+        p->Rho.assign(rho, rho + nr);
+        p->Vpar.assign(rphi, rphi + nr);
+        p->Femb.assign(F, F + nrho);
+        p->finish();
+        return p;
+    }
+
+    // EAM.cxx:55-87 (single-element setfl reader with unit conversion)
+    static EAM* from_file(const char* path) {
+        std::ifstream file(path);
+        if (!file.is_open()) throw std::runtime_error(std::string("Failed to open file: ") + path);
+        auto* p = new EAM();
+        std::string line, lattice;
+        for (int i = 0; i < 4; ++i) std::getline(file, line);
+        int Z;
+        double M, Ecoh;
+        file >> p->Nrh1 >> p->drh >> p->Nr1 >> p->dr >> p->R1cut;
+        file >> Z >> M >> Ecoh >> lattice;
+        if (!file || p->Nrh1 < 2 || p->Nr1 < 2) {
+            delete p;
+            throw std::runtime_error(std::string("malformed setfl header: ") + path);
+        }
+        p->dr /= 10;
+        p->R1cut /= 10;
+        p->R2cut = p->R1cut * p->R1cut;
+        p->Femb.resize(p->Nrh1);
+        p->Rho.resize(p->Nr1);
+        p->Vpar.resize(p->Nr1);
+        for (int i = 0; i < p->Nrh1; ++i) {
+            file >> p->Femb[i];
+            p->Femb[i] *= kQe / kUnEnrg;
+        }
+        for (int i = 0; i < p->Nr1; ++i) file >> p->Rho[i];
+        for (int i = 0; i < p->Nr1; ++i) {
+            file >> p->Vpar[i];
+            p->Vpar[i] *= kQe / (10 * kUnEnrg);
+        }
+        if (!file) {
+            delete p;
+            throw std::runtime_error(std::string("truncated setfl tables: ") + path);
+        }
+        p->finish();
+        return p;
+    }
+
+    // EAM.cxx:603-686, including the back-substitution z[i] = s[i]*z[i] + t[i]
+    // with z[i] = 0 (the vectors are zero-initialised), i.e. z[i] = t[i].
+    static void init_spline(const double* y, double* z, double H1, size_t N, size_t ib) {
+        double rp;
+        std::vector<double> a(N), b(N), c(N), d(N);
+        std::vector<double> s(N + 1), t(N + 1), u(N + 1), v(N + 1), w(N + 1);
+        std::fill(a.begin(), a.end(), 2.0);
+        b[0] = 1.0;
+        c[0] = 0.0;
+        d[0] = 3.0 * H1 * (y[1] - y[0]);
+        b[N - 1] = 0.0;
+        if (ib == 1) {
+            c[N - 1] = 0.0;
+            d[N - 1] = 0.0;
+        } else if (ib == 2) {
+            c[N - 1] = 1.0;
+            d[N - 1] = 3.0 * H1 * (y[N - 1] - y[N - 2]);
+        }
+        for (size_t i = 1; i < N - 1; i++) {
+            c[i] = 0.5;
+            b[i] = 0.5;
+            d[i] = 1.5 * H1 * (y[i + 1] - y[i - 1]);
+        }
+        u[0] = 0.0;
+        v[0] = 0.0;
+        w[0] = 1.0;
+        for (size_t i = 0; i < N; i++) {
+            rp = -1.0 / (a[i] + c[i] * v[i]);
+            v[i + 1] = rp * b[i];
+            u[i + 1] = rp * (c[i] * u[i] - d[i]);
+            w[i + 1] = rp * c[i] * w[i];
+        }
+        s[N - 1] = 1.0;
+        t[N - 1] = 0.0;
+        for (long i = (long)N - 2; i >= 0; i--) {
+            s[i] = v[i + 1] * s[i + 1] + w[i + 1];
+            t[i] = v[i + 1] * t[i + 1] + u[i + 1];
+        }
+        z[N - 1] = (d[N - 1] - b[N - 1] * t[0] - c[N - 1] * t[N - 2]) / (a[N - 1] + b[N - 1] * s[0] + c[N - 1] * s[N - 2]);
+        for (size_t i = 0; i < N - 1; i++) z[i] = s[i] * z[i] + t[i];
+    }
+
+    // EAM.cxx:499-531 density(r2[], size)
+    double density_sum(const double* r2, size_t size) const {
+        double SD = 0.0;
+        for (size_t k = 0; k < size; k++) {
+            double R1 = std::sqrt(r2[k]);
+            double RN = R1 * dr1;
+            int N1 = (int)std::trunc(RN);
+            if (N1 < Nr1) {
+                double VR = N1;
+                double DV = RN - VR;
+                RN = R1 - VR * dr;
+                VR = (Rho[N1 + 1] - Rho[N1]) * dr1;
+                double aa = (dRho[N1 + 1] - VR) - (VR - dRho[N1]);
+                double bb = aa - (VR - dRho[N1]);
+                double nr = Rho[N1] + RN * (dRho[N1] + DV * (DV * aa - bb));
+                SD = SD + nr;
+            }
+        }
+        return SD;
+    }
+
+    // EAM.cxx:562-601 embeddingEnergy (only the in-range branch is defined;
+    // out-of-range densities are reported as errors by the callers' inputs)
+    void embedding(const double* nr, double* EF, double* dF, size_t size) const {
+        for (size_t k = 0; k < size; k++) {
+            double RN = nr[k] * drh1;
+            int N1 = (int)std::trunc(RN);
+            if (RN >= 0.0 && N1 < Nrh1) {
+                double VR = N1;
+                double DV = RN - VR;
+                RN = nr[k] - VR * drh;
+                VR = (Femb[N1 + 1] - Femb[N1]) * drh1;
+                double aa = (dFemb[N1 + 1] - VR) - (VR - dFemb[N1]);
+                double bb = aa - (VR - dFemb[N1]);
+                aa = aa * DV;
+                EF[k] = Femb[N1] + RN * (dFemb[N1] + DV * (aa - bb));
+                dF[k] = dFemb[N1] + 2.0 * DV * (1.5 * aa - bb);
+            } else {
+                // EAM.cxx:590-599 reads past the table and uses the output
+                // array as input; NaN marks the undefined result.
+                EF[k] = std::numeric_limits<double>::quiet_NaN();
+                dF[k] = std::numeric_limits<double>::quiet_NaN();
+            }
+        }
+    }
+
+    // shared per-pair spline body of EAM.cxx:367-394: returns FF (force/r)
+    // and the pair energy rphi/r
+    inline void pair_terms(double R2, double AdF, double& FF, double& phi) const {
+        double R1 = std::sqrt(R2);
+        double RN = R1 * dr1;
+        int N1 = (int)std::trunc(RN);
+        double VR, DV, aa, bb;
+        double nd;
+        if (N1 < Nr1) {
+            VR = N1;
+            DV = RN - VR;
+            RN = R1 - VR * dr;
+            VR = (Rho[N1 + 1] - Rho[N1]) * dr1;
+            aa = (dRho[N1 + 1] - VR) - (VR - dRho[N1]);
+            bb = aa - (VR - dRho[N1]);
+            nd = dRho[N1] + 2.0 * DV * (1.5 * DV * aa - bb);
+            VR = (Vpar[N1 + 1] - Vpar[N1]) * dr1;
+            aa = (dVpar[N1 + 1] - VR) - (VR - dVpar[N1]);
+            bb = aa - (VR - dVpar[N1]);
+            aa = aa * DV;
+            VR = Vpar[N1] + RN * (dVpar[N1] + DV * (aa - bb));
+            DV = dVpar[N1] + 2.0 * DV * (1.5 * aa - bb);
+        } else {
+            nd = 0.0;
+            VR = 0.0;
+            DV = 0.0;
+        }
+        FF = (AdF * nd + (DV * R1 - VR) / R2) / R1;
+        phi = VR / R1;
+    }
+
+    // EAM.cxx:532-560
+    void density(Atom& atom, Atom* neibs, const size_t* list, size_t nl) override {
+        if (nl == 0) return;
+        std::vector<double> r2(nl);
+        for (size_t i = 0; i < nl; ++i) {
+            const Atom& nb = neibs[list[i]];
+            double d[3] = {nb.r[0] - atom.r[0], nb.r[1] - atom.r[1], nb.r[2] - atom.r[2]};
+            r2[i] = sqnorm(d);
+        }
+        size_t na = 0;
+        for (size_t i = 0; i < nl; ++i) {
+            if (r2[i] > R2cut) continue;
+            if (r2[i] < 1e-15) continue;
+            r2[na++] = r2[i];
+        }
+        atom.nc += density_sum(r2.data(), na);
+    }
+
+    // EAM.cxx:453-497 force and :306-354 virial share the gather
+    void gather(Atom& atom, Atom* neibs, const size_t* list, size_t nl, bool is_virial) {
+        if (nl == 0) return;
+        std::vector<double> rall(3 * nl), r2(nl), nCharge(nl + 1), dF(nl + 1), EF(nl + 1);
+        for (size_t i = 0; i < nl; ++i) {
+            const Atom& nb = neibs[list[i]];
+            for (int c = 0; c < 3; ++c) rall[3 * i + c] = nb.r[c] - atom.r[c];
+            r2[i] = sqnorm(&rall[3 * i]);
+            nCharge[i] = nb.nc;
+        }
+        nCharge[nl] = atom.nc;
+        embedding(nCharge.data(), EF.data(), dF.data(), nl + 1);
+        if (is_virial) atom.Em = EF[nl];
+        double F[3] = {0, 0, 0}, Vt[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, Ut = 0.0;
+        for (size_t i = 0; i < nl; ++i) {
+            if (r2[i] > R2cut) continue;
+            if (r2[i] < 1e-15) continue;
+            double AdF = dF[i] + dF[nl];
+            double FF, phi;
+            pair_terms(r2[i], AdF, FF, phi);
+            const double* rk = &rall[3 * i];
+            if (!is_virial) {
+                for (int c = 0; c < 3; ++c) F[c] += FF * rk[c];
+            } else {
+                double s = 0.5 * FF;
+                for (int col = 0; col < 3; ++col)
+                    for (int row = 0; row < 3; ++row) Vt[col * 3 + row] += (s * rk[row]) * rk[col];
+                Ut += 0.5 * phi;
+            }
+        }
+        if (!is_virial) {
+            for (int c = 0; c < 3; ++c) atom.f[c] += F[c];
+        } else {
+            for (int k = 0; k < 9; ++k) atom.V[k] += Vt[k];
+            atom.Ep += Ut;
+        }
+    }
+    void force(Atom& atom, Atom* neibs, const size_t* list, size_t nl) override {
+        gather(atom, neibs, list, nl, false);
+    }
+    void virial(Atom& atom, Atom* neibs, const size_t* list, size_t nl) override {
+        gather(atom, neibs, list, nl, true);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// MAISE model parser (mdcraft/tools/parser/maise_model.cxx:29-129) and the
+// BPNN descriptor/ANN setup (mdcraft/tools/bpnn/maise_model.cxx:19-131,
+// maise_ann.cxx:8-55).
+// ---------------------------------------------------------------------------
+static std::vector<std::string> split_ws(const std::string& s) {
+    std::vector<std::string> out;
+    std::istringstream is(s);
+    std::string t;
+    while (is >> t) out.push_back(t);
+    return out;
+}
+
+struct MaiseModel {
+    int nspc = 0;
+    std::vector<int> NN;
+    int nw = 0;
+    std::vector<std::vector<double>> vRmin, vRmax, range;
+    std::vector<double> weights;
+    std::vector<double> b2a, scaling;
+    std::vector<std::vector<std::vector<int>>> maps;
+    std::vector<std::vector<std::vector<double>>> vals;
+
+    static bool getl(std::ifstream& f, std::string& line) {
+        if (!std::getline(f, line)) throw std::runtime_error("unexpected end of MAISE model file");
+        return true;
+    }
+
+    void read(const char* path) {
+        static const std::string ICR =
+            "|                          input component range                            |";
+        std::ifstream f(path);
+        if (!f.is_open()) throw std::runtime_error(std::string("Failed to open file: ") + path);
+        std::string line;
+        for (int i = 0; i < 3; ++i) getl(f, line);
+        auto text = [&]() {
+            getl(f, line);
+            size_t p = line.find("|", 1);
+            if (p == std::string::npos) throw std::runtime_error("malformed MAISE head line: " + line);
+            return line.substr(p + 1, p + 10);
+        };
+        (void)std::stoul(text(), nullptr, 16);  // model id
+        nspc = std::stoi(text());
+        (void)text();  // species types
+        (void)text();  // species names
+        (void)std::stoi(text());  // number of layers
+        for (auto& s : split_ws(text())) NN.push_back(std::stoi(s));
+        nw = std::stoi(text());
+        while (line != ICR) getl(f, line);
+        getl(f, line);
+        getl(f, line);
+        int inDim = NN.at(0);
+        vRmin.assign(nspc, std::vector<double>(inDim));
+        vRmax.assign(nspc, std::vector<double>(inDim));
+        range.assign(nspc, std::vector<double>(inDim));
+        for (int s = 0; s < nspc; ++s)
+            for (int i = 0; i < inDim; ++i) {
+                getl(f, line);
+                auto t = split_ws(line);
+                if (t.size() < 3) throw std::runtime_error("malformed MAISE input range row");
+                vRmin[s][i] = std::stod(t[0]);
+                vRmax[s][i] = std::stod(t[1]);
+                range[s][i] = std::stod(t[2]);
+            }
+        for (int i = 0; i < 3; ++i) getl(f, line);
+        weights.resize(nw);
+        for (int i = 0; i < nw; ++i) {
+            getl(f, line);
+            weights[i] = std::stod(line);
+        }
+        b2a.resize(nspc);
+        scaling.resize(nspc);
+        maps.resize(nspc);
+        vals.resize(nspc);
+        for (int s = 0; s < nspc; ++s) {
+            if (f.peek() == EOF) {
+                b2a[s] = b2a[0];
+                scaling[s] = scaling[0];
+                maps[s] = maps[0];
+                vals[s] = vals[0];
+                continue;
+            }
+            for (int i = 0; i < 3; ++i) getl(f, line);
+            auto bs = split_ws(line);
+            b2a[s] = std::stod(bs.at(0));
+            scaling[s] = std::stod(bs.at(1));
+            for (int i = 0; i < 3; ++i) getl(f, line);
+            getl(f, line);
+            while (!line.empty()) {
+                std::vector<int> row;
+                for (auto& t : split_ws(line)) row.push_back(std::stoi(t));
+                maps[s].push_back(row);
+                getl(f, line);
+            }
+            for (int i = 0; i < 7; ++i) {
+                getl(f, line);
+                auto t = split_ws(line);
+                if (t.size() < 2 || std::stoi(t[1]) != (int)t.size() - 2)
+                    throw std::runtime_error("malformed MAISE descriptor value row: " + line);
+                std::vector<double> v;
+                for (size_t k = 2; k < t.size(); ++k) v.push_back(std::stod(t[k]));
+                vals[s].push_back(v);
+            }
+        }
+    }
+};
+
+struct G4Par {
+    double eta, dz, lam;
+};
+
+// MaiseBPNNPotential (mdcraft/solver/potential/maise_bpnn.cxx)
+struct BPNN : Potential {
+    Domain dom;
+    double Rc = 0, Rs = 0;
+    std::vector<double> g2eta;
+    std::vector<G4Par> g4;
+    int nin = 0, h1 = 0, h2 = 0;
+    // per kind: W01 (h1 x nin, row-major, columns scaled by range), b01,
+    // W12 (h2 x h1), b12, W23 (1 x h2), b23
+    struct Ann {
+        std::vector<double> W01, b01, W12, b12, W23;
+        double b23;
+    };
+    std::vector<Ann> anns;
+    unsigned nradial = 0, nangular = 0;
+
+    struct TInfo {
+        std::vector<double> g, dedg;
+        std::vector<double> dgdr, dgdr_X_r;  // [slot][l][c]
+        long self_slot = -1;
+    };
+    std::vector<TInfo> tinfo;
+    const std::vector<std::vector<size_t>>* nl = nullptr;
+
+    BPNN(const char* path, const Domain& d) : Potential(PT_TERNARY), dom(d) {
+        MaiseModel m;
+        m.read(path);
+        if (m.NN.size() != 4) throw std::runtime_error("MaiseBPNN: only 4-layer networks are supported");
+        if (m.nspc != 1) throw std::runtime_error("MaiseBPNN restatement supports one species");
+        nin = m.NN[0];
+        h1 = m.NN[1];
+        h2 = m.NN[2];
+        if (m.NN[3] != 1) throw std::runtime_error("MaiseBPNN: output layer must have one neuron");
+        // maise_model.cxx:34-50
+        double lin = m.b2a[0] * m.scaling[0] * 0.1;
+        double inv_sq = 1.0 / lin / lin;
+        const auto& V = m.vals[0];
+        Rc = V.at(0).at(0) * lin;
+        Rs = V.at(1).at(0) * lin;
+        std::vector<double> eta1 = V.at(2), eta2 = V.at(3);
+        for (auto& e : eta1) e *= inv_sq;
+        for (auto& e : eta2) e *= inv_sq;
+        for (auto& row : m.maps[0]) {
+            if (row.size() != 9) throw std::runtime_error("MAISE descriptor map rows need 9 columns");
+            int order = row[1];
+            if (order == 2) {
+                g2eta.push_back(eta1.at(row[4]));
+                ++nradial;
+            } else if (order == 4) {
+                g4.push_back({eta2.at(row[5]), V.at(5).at(row[7]), V.at(6).at(row[8])});
+                ++nangular;
+            } else {
+                throw std::runtime_error("only G2 radial and G4 angular descriptors are supported");
+            }
+        }
+        if ((int)(nradial + nangular) < nin) throw std::runtime_error("fewer descriptors than network inputs");
+        R1cut = Rc;
+        R2cut = Rc * Rc;
+        // maise_ann.cxx:23-51 weight order: per kind, biases of every layer,
+        // then weights of every layer, rows = next-layer neuron
+        size_t w = 0;
+        anns.resize(m.nspc);
+        for (int k = 0; k < m.nspc; ++k) {
+            Ann& a = anns[k];
+            a.b01.assign(m.weights.begin() + w, m.weights.begin() + w + h1);
+            w += h1;
+            a.b12.assign(m.weights.begin() + w, m.weights.begin() + w + h2);
+            w += h2;
+            a.b23 = m.weights.at(w);
+            w += 1;
+            a.W01.assign(m.weights.begin() + w, m.weights.begin() + w + (size_t)h1 * nin);
+            w += (size_t)h1 * nin;
+            a.W12.assign(m.weights.begin() + w, m.weights.begin() + w + (size_t)h2 * h1);
+            w += (size_t)h2 * h1;
+            a.W23.assign(m.weights.begin() + w, m.weights.begin() + w + h2);
+            w += h2;
+            for (int in = 0; in < nin; ++in)
+                for (int q = 0; q < h1; ++q) a.W01[(size_t)q * nin + in] *= m.range[k][in];
+        }
+        if ((int)w != m.nw) throw std::runtime_error("MAISE weight count does not match the architecture");
+    }
+
+    // tools/bpnn/base.h:95-190 descriptor patterns
+    double fc(double r) const { return 0.5 * (std::cos(M_PI * r / Rc) + 1.0); }
+    double dfc(double r) const { return (-0.5) * std::sin(M_PI * r / Rc) * M_PI / Rc; }
+    double G2(size_t o, double r) const { return std::exp(-g2eta[o] * (r - Rs) * (r - Rs)) * fc(r); }
+    double dG2(size_t o, double r) const {
+        double eta = g2eta[o];
+        return std::exp(-eta * (r - Rs) * (r - Rs)) * (-2.0 * eta * (r - Rs) * fc(r) + dfc(r));
+    }
+    // base.cxx:116-128: angular value carries the factor 2
+    double G4v(size_t o, double Rij, double Rik, double Rjk, double cs) const {
+        const G4Par& p = g4[o];
+        return std::pow(2.0, 1.0 - p.dz) * std::pow(1.0 + p.lam * cs, p.dz) *
+               std::exp(-p.eta * (Rij * Rij + Rik * Rik + Rjk * Rjk)) * 2.0 * fc(Rij) * fc(Rik) * fc(Rjk);
+    }
+    // base.h:133-155 pattern_dG4_
+    double dG4(size_t o, double Rij, double Rik, double Rjk, double cs, double dcs, double dRij, double dRik) const {
+        const G4Par& p = g4[o];
+        const double dz = p.dz, lam = p.lam, eta = p.eta;
+        return fc(Rjk) * std::exp(-eta * (Rij * Rij + Rik * Rik + Rjk * Rjk)) * std::pow(2.0, 1.0 - dz) *
+               std::pow(1.0 + lam * cs, dz - 1.0) *
+               ((dz * lam * dcs + (1.0 + lam * cs) * (-2.0 * eta) * (Rij * dRij + Rik * dRik)) * fc(Rij) * fc(Rik) +
+                (1.0 + lam * cs) * (dRij * dfc(Rij) * fc(Rik) + dRik * dfc(Rik) * fc(Rij)));
+    }
+    // base.h:157-181 pattern_dG4_s_
+    double dG4s(size_t o, double Rij, double Rik, double Rjk, double cs, double R_ij, double R_ik) const {
+        const G4Par& p = g4[o];
+        const double dz = p.dz, lam = p.lam, eta = p.eta;
+        double R_kj = R_ij - R_ik;
+        return std::exp(-eta * (Rij * Rij + Rik * Rik + Rjk * Rjk)) * std::pow(2.0, 1.0 - dz) *
+               std::pow(1.0 + lam * cs, dz - 1.0) *
+               (fc(Rik) * fc(Rjk) *
+                    ((-2 * eta) * fc(Rij) * (1.0 + lam * cs) - lam * dz * cs * fc(Rij) / Rij / Rij +
+                     (1 + lam * cs) * dfc(Rij) / Rij) *
+                    R_ij * R_ij +
+                fc(Rij) * fc(Rjk) *
+                    ((-2 * eta) * fc(Rik) * (1.0 + lam * cs) - lam * dz * cs * fc(Rik) / Rik / Rik +
+                     (1 + lam * cs) * dfc(Rik) / Rik) *
+                    R_ik * R_ik +
+                fc(Rik) * fc(Rij) * ((-2 * eta) * fc(Rjk) * (1.0 + lam * cs) + (1 + lam * cs) * dfc(Rjk) / Rjk) *
+                    R_kj * R_kj +
+                2.0 * lam * dz * fc(Rij) * fc(Rjk) * fc(Rik) / Rij / Rik * R_ij * R_ik);
+    }
+
+    void reset_natoms(size_t n) { tinfo.resize(n); }
+
+    // maise_bpnn.cxx:122-272 accumulate_inner_grads, with dgdr indexed by
+    // the slot of the neighbour in nlist_i instead of its global id (the
+    // reference's [300,3,512] tensor limits N to 512 atoms).
+    void accumulate(size_t ii, Atom* atoms) {
+        const auto& L = *nl;
+        const auto& nli = L[ii];
+        TInfo& T = tinfo[ii];
+        const size_t NS = nli.size();
+        const unsigned IN = nradial + nangular;
+        T.g.assign(IN, 0.0);
+        T.dedg.assign(IN, 0.0);
+        T.dgdr.assign((NS + 1) * IN * 3, 0.0);
+        T.dgdr_X_r.assign((NS + 1) * IN * 3, 0.0);
+        T.self_slot = (long)NS;  // dummy slot when self is absent
+        for (size_t j = 0; j < NS; ++j)
+            if (nli[j] == ii) T.self_slot = (long)j;
+        auto D = [&](size_t slot, unsigned pos, int c) -> double& { return T.dgdr[(slot * IN + pos) * 3 + c]; };
+        auto DX = [&](size_t slot, unsigned pos, int c) -> double& { return T.dgdr_X_r[(slot * IN + pos) * 3 + c]; };
+        const size_t ss = (size_t)T.self_slot;
+        const Atom& ai = atoms[ii];
+        for (size_t j = 0; j < NS; ++j) {
+            size_t jj = nli[j];
+            const Atom& aj = atoms[jj];
+            const auto& nlj = L[jj];
+            double rij_v[3] = {aj.r[0] - ai.r[0], aj.r[1] - ai.r[1], aj.r[2] - ai.r[2]};
+            dom.shortest(rij_v);
+            double r2ij = sqnorm(rij_v);
+            if (r2ij < 1e-15) continue;
+            if (!(r2ij < R2cut)) continue;
+            double rij = std::sqrt(r2ij);
+            double drij[3] = {rij_v[0] / rij, rij_v[1] / rij, rij_v[2] / rij};
+            // radial part
+            for (unsigned ir = 0; ir < nradial; ++ir) {
+                unsigned pos = ir;
+                T.g[pos] += G2(ir, rij);
+                for (int c = 0; c < 3; ++c) {
+                    double add = dG2(ir, rij) * drij[c];
+                    D(j, pos, c) += add;
+                    D(ss, pos, c) += add;
+                    DX(j, pos, c) -= add * rij_v[c];
+                }
+            }
+            // angular part 1: centre j, k in nlist_j
+            for (size_t k = 0; k < nlj.size(); ++k) {
+                const Atom& ak = atoms[nlj[k]];
+                double rjk_v[3] = {ak.r[0] - aj.r[0], ak.r[1] - aj.r[1], ak.r[2] - aj.r[2]};
+                dom.shortest(rjk_v);
+                double r2jk = sqnorm(rjk_v);
+                if (r2jk < 1e-15) continue;
+                double rjk = std::sqrt(r2jk);
+                double drjk[3] = {rjk_v[0] / rjk, rjk_v[1] / rjk, rjk_v[2] / rjk};
+                if (!(r2jk < R2cut)) continue;
+                double rik_v[3] = {rij_v[0] + rjk_v[0], rij_v[1] + rjk_v[1], rij_v[2] + rjk_v[2]};
+                double r2ik = sqnorm(rik_v);
+                if (r2ik < 1e-15) continue;
+                if (!(r2ik < R2cut)) continue;
+                double rik = std::sqrt(r2ik);
+                double drik[3] = {rik_v[0] / rik, rik_v[1] / rik, rik_v[2] / rik};
+                double cos_j = -(drij[0] * drjk[0] + drij[1] * drjk[1] + drij[2] * drjk[2]);
+                for (unsigned ia = 0; ia < nangular; ++ia) {
+                    unsigned pos = ia + nradial;
+                    for (int c = 0; c < 3; ++c) {
+                        double dcos = rij_v[c] * cos_j / rij / rij + rjk_v[c] / rij / rjk;
+                        D(j, pos, c) -= 2.0 * dG4(ia, rij, rik, rjk, cos_j, dcos, -drij[c], -drik[c]);
+                    }
+                }
+            }
+            // angular part 2: centre i, k later in nlist_i
+            for (size_t j1 = j; j1 < NS; ++j1) {
+                const Atom& ak = atoms[nli[j1]];
+                double rjk_v[3] = {ak.r[0] - aj.r[0], ak.r[1] - aj.r[1], ak.r[2] - aj.r[2]};
+                dom.shortest(rjk_v);
+                double r2jk = sqnorm(rjk_v);
+                if (r2jk < 1e-15) continue;
+                double rjk = std::sqrt(r2jk);
+                if (!(r2jk < R2cut)) continue;
+                double rik_v[3] = {rij_v[0] + rjk_v[0], rij_v[1] + rjk_v[1], rij_v[2] + rjk_v[2]};
+                double r2ik = sqnorm(rik_v);
+                if (r2ik < 1e-15) continue;
+                if (!(r2ik < R2cut)) continue;
+                double rik = std::sqrt(r2ik);
+                double drik[3] = {rik_v[0] / rik, rik_v[1] / rik, rik_v[2] / rik};
+                double cos_i = drij[0] * drik[0] + drij[1] * drik[1] + drij[2] * drik[2];
+                for (unsigned ia = 0; ia < nangular; ++ia) {
+                    unsigned pos = ia + nradial;
+                    T.g[pos] += G4v(ia, rij, rik, rjk, cos_i);
+                    for (int c = 0; c < 3; ++c) {
+                        double dcos = cos_i * (drij[c] / rij + drik[c] / rik) - drij[c] / rik - drik[c] / rij;
+                        D(ss, pos, c) -= 2.0 * dG4(ia, rij, rik, rjk, cos_i, dcos, -drij[c], -drik[c]);
+                        DX(ss, pos, c) -= 2.0 * dG4s(ia, rij, rik, rjk, cos_i, rij_v[c], rik_v[c]);
+                    }
+                }
+            }
+        }
+    }
+
+    // maise_bpnn.cxx:275-325 dEdG: tanh, tanh, linear; the reference takes
+    // dE/dG by forward-mode autodiff, restated here as the equivalent
+    // reverse pass.  Energies in eV are scaled by 10*eV*Na to kJ/mol.
+    void dEdG(size_t ii, Atom& atom) {
+        static const double eV_Na = 1.602176634 * 6.02214076;
+        TInfo& T = tinfo[ii];
+        const Ann& a = anns.at(atom.kind);
+        std::vector<double> z1(h1), y1(h1), z2(h2), y2(h2);
+        for (int q = 0; q < h1; ++q) {
+            double s = 0.0;
+            for (int in = 0; in < nin; ++in) s += a.W01[(size_t)q * nin + in] * T.g[in];
+            z1[q] = s + a.b01[q];
+            y1[q] = std::tanh(z1[q]);
+        }
+        for (int q = 0; q < h2; ++q) {
+            double s = 0.0;
+            for (int p = 0; p < h1; ++p) s += a.W12[(size_t)q * h1 + p] * y1[p];
+            z2[q] = s + a.b12[q];
+            y2[q] = std::tanh(z2[q]);
+        }
+        double E = a.b23;
+        for (int q = 0; q < h2; ++q) E += a.W23[q] * y2[q];
+        std::vector<double> d2(h2), d1(h1);
+        for (int q = 0; q < h2; ++q) d2[q] = a.W23[q] * (1.0 - y2[q] * y2[q]);
+        for (int p = 0; p < h1; ++p) {
+            double s = 0.0;
+            for (int q = 0; q < h2; ++q) s += a.W12[(size_t)q * h1 + p] * d2[q];
+            d1[p] = s * (1.0 - y1[p] * y1[p]);
+        }
+        for (int in = 0; in < nin; ++in) {
+            double s = 0.0;
+            for (int q = 0; q < h1; ++q) s += a.W01[(size_t)q * nin + in] * d1[q];
+            T.dedg[in] = s * 10.0 * eV_Na;
+        }
+        atom.Ep = E * 10.0 * eV_Na;
+    }
+
+    // maise_bpnn.cxx:61-96
+    void force_one(size_t ii, Atom& atom) {
+        const auto& nli = (*nl)[ii];
+        const TInfo& T = tinfo[ii];
+        const unsigned IN = nradial + nangular;
+        for (size_t j = 0; j < nli.size(); ++j) {
+            const TInfo& Tj = tinfo[nli[j]];
+            for (unsigned l = 0; l < IN; ++l)
+                for (int c = 0; c < 3; ++c) atom.f[c] += Tj.dedg[l] * T.dgdr[(j * IN + l) * 3 + c];
+        }
+    }
+    // maise_bpnn.cxx:98-120 (diagonal only)
+    void virial_one(size_t ii, Atom& atom) {
+        const auto& nli = (*nl)[ii];
+        const TInfo& T = tinfo[ii];
+        const unsigned IN = nradial + nangular;
+        for (size_t j = 0; j < nli.size(); ++j) {
+            const TInfo& Tj = tinfo[nli[j]];
+            for (unsigned l = 0; l < IN; ++l)
+                for (int c = 0; c < 3; ++c) atom.V[c * 3 + c] -= Tj.dedg[l] * T.dgdr_X_r[(j * IN + l) * 3 + c];
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Solvers: Single/Pair with the Periodic ghost pass (solver/pair.cxx:16-127,
+// solver/single.cxx:11-83, solver/boundary/periodic.cxx:9-102) and Ternary
+// (solver/ternary.cxx:19-67).
+// ---------------------------------------------------------------------------
+struct Engine {
+    Potential* pot;
+    Pool* pool;
+    Atom* atoms;
+    size_t n;
+    const Domain* dom;
+    VerletList* vl;
+    bool periodic_bc;  // solver built with boundary::Periodic(domain)
+
+    // periodic.cxx:36-60
+    void ghosts_of(size_t i, std::vector<Atom>& g) {
+        const Atom& a = atoms[i];
+        const auto& nli = vl->nlist[i];
+        g.clear();
+        for (size_t j : nli) {
+            const Atom& nb = atoms[j];
+            double dist[3] = {nb.r[0] - a.r[0], nb.r[1] - a.r[1], nb.r[2] - a.r[2]};
+            double r2rns = std::max(nb.rns, a.rns);
+            r2rns *= r2rns;
+            if (sqnorm(dist) < r2rns) continue;
+            dom->shortest(dist);
+            if (sqnorm(dist) > r2rns) continue;
+            g.push_back(nb);
+            for (int c = 0; c < 3; ++c) g.back().r[c] = a.r[c] + dist[c];
+        }
+    }
+
+    template <class F>
+    void with_ghosts(size_t i, F&& body) {
+        if (!(atoms[i].bc & 0x1)) return;
+        std::vector<Atom> g;
+        ghosts_of(i, g);
+        if (g.empty()) return;
+        std::vector<size_t> idx(g.size());
+        for (size_t k = 0; k < g.size(); ++k) idx[k] = k;
+        body(g.data(), idx.data(), g.size());
+    }
+
+    void prepare() {
+        if (pot->type == PT_TERNARY) {
+            auto* b = static_cast<BPNN*>(pot);
+            for (size_t i = 0; i < n; ++i)
+                if (atoms[i].tag != 0 && atoms[i].kind >= b->anns.size())
+                    throw std::runtime_error("atom kind has no network in the MAISE model");
+            b->reset_natoms(n);
+            b->nl = &vl->nlist;
+            pool->for_n(n, [&](size_t i) {
+                Atom& a = atoms[i];
+                if (a.tag == 0) return;
+                std::memset(a.f, 0, sizeof(a.f));
+                std::memset(a.V, 0, sizeof(a.V));
+                a.Ep = 0.0;
+                a.rcut = pot->R1cut;
+                b->accumulate(i, atoms);
+                b->dEdG(i, a);
+            });
+            return;
+        }
+        pool->for_n(n, [&](size_t i) {
+            Atom& a = atoms[i];
+            if (a.tag != 0) {
+                std::memset(a.f, 0, sizeof(a.f));
+                std::memset(a.V, 0, sizeof(a.V));
+                a.Ep = 0.0;
+                if (pot->type == PT_EAM) {
+                    a.Em = 0.0;
+                    a.nc = 0.0;
+                }
+                a.rcut = pot->R1cut;
+            }
+            a.bc = 0x0;
+            if (periodic_bc && a.tag != 0) {
+                bool need = false;
+                for (int d = 0; d < 3; ++d) need = need || dom->nearest_distance(a.r, d) < a.rns;
+                a.bc += need ? 0x1 : 0x0;
+            }
+        });
+    }
+
+    const size_t* lp(size_t i) { return vl->nlist[i].data(); }
+    size_t ls(size_t i) { return vl->nlist[i].size(); }
+
+    void charge() {
+        pool->for_n(n, [&](size_t i) {
+            Atom& a = atoms[i];
+            if (a.tag != 0) pot->density(a, atoms, lp(i), ls(i));
+            if (a.bc)
+                with_ghosts(i, [&](Atom* g, const size_t* idx, size_t ng) {
+                    if (a.tag != 0) pot->density(a, g, idx, ng);
+                });
+        });
+    }
+
+    void forces() {
+        if (pot->type == PT_TERNARY) {
+            auto* b = static_cast<BPNN*>(pot);
+            pool->for_n(n, [&](size_t i) {
+                if (atoms[i].tag != 0) b->force_one(i, atoms[i]);
+            });
+            return;
+        }
+        pool->for_n(n, [&](size_t i) {
+            Atom& a = atoms[i];
+            if (a.tag != 0) pot->force(a, atoms, lp(i), ls(i));
+            if (a.bc)
+                with_ghosts(i, [&](Atom* g, const size_t* idx, size_t ng) {
+                    if (a.tag != 0) pot->force(a, g, idx, ng);
+                });
+        });
+    }
+
+    void virials() {
+        if (pot->type == PT_TERNARY) {
+            auto* b = static_cast<BPNN*>(pot);
+            pool->for_n(n, [&](size_t i) {
+                if (atoms[i].tag != 0) b->virial_one(i, atoms[i]);
+            });
+            return;
+        }
+        pool->for_n(n, [&](size_t i) {
+            Atom& a = atoms[i];
+            if (a.tag != 0) pot->virial(a, atoms, lp(i), ls(i));
+            if (a.bc)
+                with_ghosts(i, [&](Atom* g, const size_t* idx, size_t ng) {
+                    if (a.tag != 0) pot->virial(a, g, idx, ng);
+                });
+        });
+    }
+
+    // single.cxx:23-35 stages: pair -> [forces]; eam -> [charge, forces];
+    // ternary.cxx:19-24 -> [forces]
+    void stages() {
+        if (pot->type == PT_EAM) charge();
+        forces();
+    }
+
+    // stepper/verlet.cxx:36-48, :71-85
+    void make_step(double dt) {
+        pool->for_n(n, [&](size_t i) {
+            Atom& a = atoms[i];
+            if (a.tag == 0) return;
+            for (int c = 0; c < 3; ++c) {
+                double acc = a.f[c] / a.m;
+                double vh = a.v[c] + acc * dt * 0.5;
+                a.r[c] += vh * dt;
+                a.v[c] = vh;
+            }
+        });
+        prepare();
+        stages();
+        pool->for_n(n, [&](size_t i) {
+            Atom& a = atoms[i];
+            if (a.tag == 0) return;
+            for (int c = 0; c < 3; ++c) {
+                double acc = a.f[c] / a.m;
+                a.v[c] += acc * 0.5 * dt;
+            }
+        });
+    }
+};
+
+// test/cxx/support/md_test_utils.h:214-239 measure_state
+// out: [Ep, Em, KE, Etot, W, comx, comy, comz]
+static void measure_state(const Atom* atoms, size_t n, double* out) {
+    double Ep = 0, Em = 0, W = 0, com[3] = {0, 0, 0}, M = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const Atom& a = atoms[i];
+        Ep += a.Ep;
+        Em += a.Em;
+        W += a.V[0] + a.V[4] + a.V[8];
+        for (int c = 0; c < 3; ++c) com[c] += a.m * a.v[c];
+        M += a.m;
+    }
+    if (M > 0.0)
+        for (int c = 0; c < 3; ++c) com[c] /= M;
+    else
+        com[0] = com[1] = com[2] = 0.0;
+    double KE = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        const Atom& a = atoms[i];
+        double rv[3] = {a.v[0] - com[0], a.v[1] - com[1], a.v[2] - com[2]};
+        KE += 0.5 * a.m * sqnorm(rv);
+    }
+    out[0] = Ep;
+    out[1] = Em;
+    out[2] = KE;
+    out[3] = Ep + Em + KE;
+    out[4] = W;
+    out[5] = com[0];
+    out[6] = com[1];
+    out[7] = com[2];
+}
+
+}  // namespace refcpu
+
+using namespace refcpu;
+
+// ---------------------------------------------------------------------------
+// C ABI for ctypes (tests and the CPU-baseline leg of bench.py)
+// ---------------------------------------------------------------------------
+extern "C" {
+
+typedef struct {
+    double lo[3];
+    double hi[3];
+    int32_t periodic[3];
+    int32_t pad;
+} ref_domain;
+
+struct ref_nlist {
+    VerletList vl;
+};
+
+#define REF_TRY(body)                     \
+    try {                                 \
+        body;                             \
+    } catch (const std::exception& e) {   \
+        g_err = e.what();                 \
+        return -1;                        \
+    }
+
+const char* ref_last_error() { return g_err.c_str(); }
+int ref_atom_size() { return (int)sizeof(Atom); }
+
+void* ref_pool_create(int threads) { return new Pool(threads); }
+void ref_pool_free(void* p) { delete static_cast<Pool*>(p); }
+
+void* ref_lj_create(double a, double s, double rcut) { return new LJsCut(a, s, rcut); }
+
+void* ref_eam_create(double rc, const double* F, const double* rho, const double* rphi, int nr, double dr,
+                     int nrho, double drho) {
+    if (nr < 3 || nrho < 3) {
+        g_err = "EAM tables need at least 3 knots";
+        return nullptr;
+    }
+    return EAM::from_tables(rc, F, rho, rphi, nr, dr, nrho, drho);
+}
+
+void* ref_eam_create_file(const char* path) {
+    try {
+        return EAM::from_file(path);
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
+void* ref_bpnn_create(const char* path, const ref_domain* d) {
+    try {
+        Domain dom(d->lo, d->hi, d->periodic);
+        return new BPNN(path, dom);
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
+void ref_pot_free(void* p) { delete static_cast<Potential*>(p); }
+double ref_pot_rcut(void* p) { return static_cast<Potential*>(p)->R1cut; }
+int ref_pot_type(void* p) { return static_cast<Potential*>(p)->type; }
+
+// Spline tables after init_spline: out_meta = [rc, dr, drho, Nr, Nrho];
+// each table pointer may be null.
+int ref_eam_tables(void* p, double* meta, double* F, double* dF, double* rho, double* drho, double* rphi,
+                   double* drphi) {
+    auto* e = dynamic_cast<EAM*>(static_cast<Potential*>(p));
+    if (!e) {
+        g_err = "not an EAM potential";
+        return -1;
+    }
+    meta[0] = e->R1cut;
+    meta[1] = e->dr;
+    meta[2] = e->drh;
+    meta[3] = e->Nr1;
+    meta[4] = e->Nrh1;
+    auto cp = [](double* dst, const std::vector<double>& src, int n) {
+        if (dst) std::memcpy(dst, src.data(), sizeof(double) * n);
+    };
+    cp(F, e->Femb, e->Nrh1);
+    cp(dF, e->dFemb, e->Nrh1);
+    cp(rho, e->Rho, e->Nr1);
+    cp(drho, e->dRho, e->Nr1);
+    cp(rphi, e->Vpar, e->Nr1);
+    cp(drphi, e->dVpar, e->Nr1);
+    return 0;
+}
+
+// BPNN per-atom descriptors G (n x nin) and dE/dG (n x nin) from the last
+// prepare; nin returned when out pointers are null.
+int ref_bpnn_descriptors(void* p, size_t n, double* G, double* dEdG) {
+    auto* b = dynamic_cast<BPNN*>(static_cast<Potential*>(p));
+    if (!b) {
+        g_err = "not a BPNN potential";
+        return -1;
+    }
+    if (!G && !dEdG) return b->nin;
+    if (b->tinfo.size() < n) {
+        g_err = "descriptors not computed for this many atoms";
+        return -1;
+    }
+    for (size_t i = 0; i < n; ++i)
+        for (int l = 0; l < b->nin; ++l) {
+            if (G) G[i * b->nin + l] = b->tinfo[i].g[l];
+            if (dEdG) dEdG[i * b->nin + l] = b->tinfo[i].dedg[l];
+        }
+    return b->nin;
+}
+
+void* ref_nlist_create(void* pool, int half) {
+    auto* p = new ref_nlist();
+    p->vl.pool = static_cast<Pool*>(pool);
+    p->vl.half = half != 0;
+    if (p->vl.half && p->vl.pool->count() >= 2) {
+        // verlet-list.cxx:27-32
+        delete p;
+        g_err =
+            "Half Verlet lists are currently incompatible with multithreaded pair-force evaluation. "
+            "Disable threads or use a full neighbor list.";
+        return nullptr;
+    }
+    return p;
+}
+void ref_nlist_free(void* p) { delete static_cast<ref_nlist*>(p); }
+
+int ref_nlist_update(void* nl, void* atoms, size_t n, const ref_domain* d, double kbuf) {
+    REF_TRY({
+        Domain dom(d->lo, d->hi, d->periodic);
+        static_cast<ref_nlist*>(nl)->vl.update(static_cast<Atom*>(atoms), n, dom, kbuf);
+    });
+    return 0;
+}
+
+int ref_nlist_decide(void* nl, const void* atoms, size_t n, const ref_domain* d) {
+    Domain dom(d->lo, d->hi, d->periodic);
+    return static_cast<ref_nlist*>(nl)->vl.decide(static_cast<const Atom*>(atoms), n, dom) ? 1 : 0;
+}
+
+size_t ref_nlist_total(void* nl) {
+    size_t t = 0;
+    for (auto& l : static_cast<ref_nlist*>(nl)->vl.nlist) t += l.size();
+    return t;
+}
+
+void ref_nlist_export(void* nl, uint64_t* offsets, uint64_t* ids) {
+    auto& L = static_cast<ref_nlist*>(nl)->vl.nlist;
+    uint64_t o = 0;
+    for (size_t i = 0; i < L.size(); ++i) {
+        offsets[i] = o;
+        for (size_t j : L[i]) ids[o++] = j;
+    }
+    offsets[L.size()] = o;
+}
+
+// Grid queries for the reference grid unit tests: out = [cell_size, cells,
+// local ijk of each atom (3n), cell_index of each atom (n)]
+int ref_grid_build(const void* atoms, size_t n, const ref_domain* d, double* cell_size, int* ncells, int* ijk,
+                   int* cidx) {
+    Domain dom(d->lo, d->hi, d->periodic);
+    Grid g;
+    g.dom = &dom;
+    const Atom* a = static_cast<const Atom*>(atoms);
+    g.build(a, n);
+    *cell_size = g.cell_size();
+    *ncells = g.cells_number();
+    for (size_t i = 0; i < n; ++i) {
+        int c[3];
+        g.local_index3D(a[i].r, c);
+        for (int k = 0; k < 3; ++k) ijk[3 * i + k] = c[k];
+        cidx[i] = g.cell_index(c);
+    }
+    return 0;
+}
+
+int ref_grid_adjacent(const void* atoms, size_t n, const ref_domain* d, const double* q, int* list, int* is_edge) {
+    Domain dom(d->lo, d->hi, d->periodic);
+    Grid g;
+    g.dom = &dom;
+    g.build(static_cast<const Atom*>(atoms), n);
+    bool e[3];
+    int na = g.adjacent_cells(q, list, e);
+    for (int k = 0; k < 3; ++k) is_edge[k] = e[k];
+    return na;
+}
+
+int ref_domain_op(const ref_domain* d, int op, double* v, int axis, double* out) {
+    Domain dom(d->lo, d->hi, d->periodic);
+    if (op == 0) dom.fit_in_period(v);
+    else if (op == 1) dom.shortest(v);
+    else if (op == 2) *out = dom.nearest_distance(v, axis);
+    return 0;
+}
+
+static Engine make_engine(void* pot, void* pool, void* atoms, size_t n, const Domain& dom, void* nl, int boundary) {
+    Engine e;
+    e.pot = static_cast<Potential*>(pot);
+    e.pool = static_cast<Pool*>(pool);
+    e.atoms = static_cast<Atom*>(atoms);
+    e.n = n;
+    e.dom = &dom;
+    e.vl = &static_cast<ref_nlist*>(nl)->vl;
+    e.periodic_bc = boundary != 0;
+    e.pot->half = e.vl->half;
+    return e;
+}
+
+static int check_list(void* nl, size_t n) {
+    if (static_cast<ref_nlist*>(nl)->vl.nlist.size() != n) {
+        g_err = "neighbour list was built for a different atom count";
+        return -1;
+    }
+    return 0;
+}
+
+int ref_prepare(void* pot, void* pool, void* atoms, size_t n, const ref_domain* d, void* nl, int boundary) {
+    if (check_list(nl, n)) return -1;
+    REF_TRY({
+        Domain dom(d->lo, d->hi, d->periodic);
+        make_engine(pot, pool, atoms, n, dom, nl, boundary).prepare();
+    });
+    return 0;
+}
+
+int ref_stages(void* pot, void* pool, void* atoms, size_t n, const ref_domain* d, void* nl, int boundary) {
+    if (check_list(nl, n)) return -1;
+    REF_TRY({
+        Domain dom(d->lo, d->hi, d->periodic);
+        make_engine(pot, pool, atoms, n, dom, nl, boundary).stages();
+    });
+    return 0;
+}
+
+int ref_virials(void* pot, void* pool, void* atoms, size_t n, const ref_domain* d, void* nl, int boundary) {
+    if (check_list(nl, n)) return -1;
+    REF_TRY({
+        Domain dom(d->lo, d->hi, d->periodic);
+        make_engine(pot, pool, atoms, n, dom, nl, boundary).virials();
+    });
+    return 0;
+}
+
+int ref_make_step(void* pot, void* pool, void* atoms, size_t n, const ref_domain* d, void* nl, int boundary,
+                  double dt) {
+    if (check_list(nl, n)) return -1;
+    REF_TRY({
+        Domain dom(d->lo, d->hi, d->periodic);
+        make_engine(pot, pool, atoms, n, dom, nl, boundary).make_step(dt);
+    });
+    return 0;
+}
+
+void ref_measure_state(const void* atoms, size_t n, double* out) {
+    measure_state(static_cast<const Atom*>(atoms), n, out);
+}
+
+// md_test_utils.h:241-279 run_md_steps: update(-1), recompute_state, then per
+// step make_step, update every `interval` steps, recompute_state.
+// history receives 8 doubles per state (measure_state layout).
+int ref_run_md_steps(void* pot, void* pool, void* atoms, size_t n, const ref_domain* d, void* nl, int boundary,
+                     double dt, int steps, int interval, double* history) {
+    REF_TRY({
+        Domain dom(d->lo, d->hi, d->periodic);
+        auto& vl = static_cast<ref_nlist*>(nl)->vl;
+        Atom* a = static_cast<Atom*>(atoms);
+        vl.update(a, n, dom, -1.0);
+        Engine e = make_engine(pot, pool, atoms, n, dom, nl, boundary);
+        auto recompute = [&](int k) {
+            e.prepare();
+            e.stages();
+            e.virials();
+            measure_state(a, n, history + 8 * k);
+        };
+        recompute(0);
+        for (int s = 1; s <= steps; ++s) {
+            e.make_step(dt);
+            if (interval > 0 && (s % interval) == 0) vl.update(a, n, dom, -1.0);
+            recompute(s);
+        }
+    });
+    return 0;
+}
+
+// Production NVE loop of the reference scripts (problems/*/run.py,
+// test/python/solver/run_ball_ar.py:241-246): initial prepare+stages, then
+// per step make_step and, when decide() says so, update().  Returns the
+// number of list rebuilds; used as the timed CPU baseline.
+int ref_run_nve(void* pot, void* pool, void* atoms, size_t n, const ref_domain* d, void* nl, int boundary, double dt,
+                int steps, int initial_forces) {
+    int rebuilds = 0;
+    REF_TRY({
+        Domain dom(d->lo, d->hi, d->periodic);
+        auto& vl = static_cast<ref_nlist*>(nl)->vl;
+        Atom* a = static_cast<Atom*>(atoms);
+        Engine e = make_engine(pot, pool, atoms, n, dom, nl, boundary);
+        if (initial_forces) {
+            e.prepare();
+            e.stages();
+        }
+        for (int s = 0; s < steps; ++s) {
+            e.make_step(dt);
+            if (vl.decide(a, n, dom)) {
+                vl.update(a, n, dom, -1.0);
+                ++rebuilds;
+            }
+        }
+    });
+    return rebuilds;
+}
+
+}  // extern "C"
