@@ -1,0 +1,128 @@
+/* mdc.h -- C ABI of the MI355X force-and-integration engine.
+ *
+ * Plain pointers and sizes only; every call returns an mdc_status (0 = ok)
+ * and leaves a message retrievable with mdc_last_error() (per thread) or
+ * mdc_ctx_last_error().  Units are the reference's: nm, ps, amu, kJ/mol.
+ * Atom records are the reference's 208-byte data::Atom
+ * (mdcraft/data/atom.h:21-38), read and written through a field mask.
+ *
+ * The entry points replace, at whole-array granularity, the per-atom
+ * interfaces the reference dispatches through (file:line in the reference):
+ *   neibs::VerletList::update / decide / get      neibs/verlet-list.h:177-235
+ *   solver::ISolver::prepare + stages             solver/isolver.h:19-32
+ *   solver::Pair::forces / charge / virials       solver/pair.h:28-126
+ *   potential::Base force/density/virial          solver/potential/base.h:30-65
+ *   potential::BaseTernary prepare/force/virial   solver/potential/base.h:105-131
+ *   stepper::Base::make_step                      solver/stepper/base.h:78-82
+ */
+#ifndef MDC_H
+#define MDC_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mdc_ctx mdc_ctx;
+typedef struct mdc_pot mdc_pot;
+typedef struct mdc_sys mdc_sys;
+typedef int mdc_status;
+
+enum {
+    MDC_OK = 0,
+    MDC_EINVAL = 1,  /* bad argument */
+    MDC_EHIP = 2,    /* HIP runtime error */
+    MDC_ENOMEM = 3,  /* device allocation failed */
+    MDC_EFILE = 4,   /* file missing or malformed */
+    MDC_ESTATE = 5,  /* call out of order (e.g. step before any force) */
+    MDC_EBUDGET = 6, /* request exceeds the memory budget */
+    MDC_ERANGE = 7   /* value outside a table or non-finite result */
+};
+
+/* precision modes */
+enum { MDC_FP64 = 0, MDC_MIXED = 1 };
+/* what to compute */
+enum { MDC_FORCE = 1, MDC_ENERGY = 2, MDC_VIRIAL = 4 };
+/* atom record fields for upload / download */
+enum {
+    MDC_F_R = 1 << 0, MDC_F_V = 1 << 1, MDC_F_F = 1 << 2, MDC_F_VIR = 1 << 3,
+    MDC_F_M = 1 << 4, MDC_F_RCUT = 1 << 5, MDC_F_RNS = 1 << 6, MDC_F_EP = 1 << 7,
+    MDC_F_EM = 1 << 8, MDC_F_NC = 1 << 9, MDC_F_BC = 1 << 10, MDC_F_KIND = 1 << 11,
+    MDC_F_UID = 1 << 12, MDC_F_TAG = 1 << 13, MDC_F_ALL = (1 << 14) - 1
+};
+
+/* ---- context: one device, one ordered stream ---------------------------- */
+mdc_status mdc_ctx_create(int device, mdc_ctx** out);
+mdc_status mdc_ctx_destroy(mdc_ctx* ctx);
+const char* mdc_ctx_last_error(const mdc_ctx* ctx);
+const char* mdc_last_error(void);
+int mdc_atom_record_size(void); /* 208 */
+
+/* ---- potentials (immutable after create; may be shared by systems) ------ */
+/* LJsCut(a, s, rcut): solver/potential/LJs-cut.cxx:8-20 */
+mdc_status mdc_pot_lj_create(mdc_ctx* ctx, double epsilon, double sigma, double rcut, mdc_pot** out);
+/* EAM(Rcutoff, Femb, Rho, Vpar, Nr1, dr, Nrh1, drh): solver/potential/EAM.cxx:15-53
+ * (tables in reference units; rphi is r*phi) */
+mdc_status mdc_pot_eam_create(mdc_ctx* ctx, double rcut, const double* F, const double* rho, const double* rphi,
+                              int nr, double dr, int nrho, double drho, mdc_pot** out);
+/* EAM(filename), single-element setfl with eV/Angstrom conversion:
+ * solver/potential/EAM.cxx:55-87 */
+mdc_status mdc_pot_eam_create_file(mdc_ctx* ctx, const char* setfl_path, mdc_pot** out);
+/* setfl-alloy EAM for nkinds elements (the per-kind potential routing of
+ * solver/multi.cxx:26-151 generalised to standard alloy semantics):
+ * F[k*nrho + i], rho[k*nr + i], rphi[p*nr + i] with p = a*(a+1)/2 + b for
+ * kinds a >= b. Atom kind selects the element. */
+mdc_status mdc_pot_eam_alloy_create(mdc_ctx* ctx, int nkinds, double rcut, const double* F, const double* rho,
+                                    const double* rphi, int nr, double dr, int nrho, double drho, mdc_pot** out);
+/* MaiseBPNNPotential(filename, domain): solver/potential/maise_bpnn.cxx:21-30;
+ * G2 radial + G4 angular descriptors, one species, 4-layer tanh network */
+mdc_status mdc_pot_bpnn_create(mdc_ctx* ctx, const char* maise_path, mdc_pot** out);
+mdc_status mdc_pot_rcut(const mdc_pot* pot, double* rcut);
+mdc_status mdc_pot_destroy(mdc_pot* pot);
+
+/* ---- systems: nsys independent boxes resident in HBM --------------------- */
+/* box_lo/box_hi: 3 doubles per system; periodic: 3 flags per system
+ * (lattice::Domain, lattice/domain.h:14-126).  periodic_boundary: 1 attaches
+ * the reference's boundary::Periodic to the solver (solver/boundary/periodic.h),
+ * 0 the null boundary.  Atoms of system s are records
+ * [off_s, off_s + natoms[s]) of every upload/download buffer. */
+mdc_status mdc_sys_create(mdc_ctx* ctx, int nsys, const uint64_t* natoms, const double* box_lo, const double* box_hi,
+                          const uint8_t* periodic, int periodic_boundary, mdc_sys** out);
+mdc_status mdc_sys_destroy(mdc_sys* sys);
+mdc_status mdc_sys_upload(mdc_sys* sys, const void* atoms, uint32_t fields);
+mdc_status mdc_sys_download(mdc_sys* sys, void* atoms, uint32_t fields);
+mdc_status mdc_sys_natoms(const mdc_sys* sys, uint64_t* n);
+/* lanes per atom (1, 2, 4, 8, 16) of the force kernels; changes the list layout */
+mdc_status mdc_sys_set_lanes(mdc_sys* sys, int lanes_per_atom);
+
+/* ---- neighbour list (neibs::VerletList) --------------------------------- */
+/* update(kbuf = -1): wraps positions into periodic boxes, bins, rebuilds */
+mdc_status mdc_nlist_build(mdc_sys* sys);
+/* decide(): 1 when any atom moved more than half the skin (rns - rcut) */
+mdc_status mdc_nlist_decide(mdc_sys* sys, double rcut, int* need);
+/* full list including self in the reference's form: CSR over original atom
+ * order; offsets has n+1 entries; ids NULL queries the size via *total */
+mdc_status mdc_nlist_export(mdc_sys* sys, uint64_t* offsets, uint64_t* ids, uint64_t* total);
+mdc_status mdc_nlist_stats(mdc_sys* sys, double* mean_neighbours, int* max_neighbours, int* capacity, int* rebuilds);
+
+/* ---- force evaluation and integration ----------------------------------- */
+/* prepare + stages (+ virials when MDC_ENERGY | MDC_VIRIAL) */
+mdc_status mdc_compute(mdc_sys* sys, const mdc_pot* pot, uint32_t what, int precision);
+/* nsteps of velocity Verlet (make_step), list rebuilt on the half-skin
+ * trigger; *rebuilds receives the number of rebuilds (may be NULL) */
+mdc_status mdc_step(mdc_sys* sys, const mdc_pot* pot, double dt, int nsteps, int precision, int* rebuilds);
+/* per system: Ep, Em, KE, Etot, W (sum of virial traces), com velocity[3] */
+mdc_status mdc_thermo(mdc_sys* sys, double* out);
+mdc_status mdc_sync(mdc_sys* sys);
+
+/* ---- instrumentation ---------------------------------------------------- */
+/* kernel timing with hipEvents on the launch stream: kind 0 pair force,
+ * 1 EAM density, 2 pre-drift, 3 BPNN descriptors, 4 BPNN MLP, 5 BPNN force,
+ * 6 BPNN gather, 7 rebuild */
+mdc_status mdc_timing(mdc_sys* sys, int enable, double* total_ms, int64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

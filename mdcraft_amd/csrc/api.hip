@@ -1,0 +1,880 @@
+// C ABI of the engine (include/mdc.h): contexts, potentials, resident
+// systems, upload/download of the reference's 208-byte atom records, and the
+// entry points that replace VerletList::update/decide, Solver prepare/stages/
+// virials and Stepper::make_step at whole-array granularity.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <numeric>
+#include <sstream>
+#include "engine.h"
+#include "../../include/mdc.h"
+
+using namespace mdc;
+
+struct mdc_ctx {
+    Ctx c;
+};
+struct mdc_pot {
+    Pot p;
+};
+struct mdc_sys {
+    Batch b;
+};
+
+namespace {
+
+thread_local std::string g_last;
+
+// 208-byte record (mdcraft/data/atom.h:21-38)
+struct AtomRec {
+    double r[3], v[3], f[3], V[9];
+    double m, rcut, rns, Ep, Em, nc;
+    uint16_t bc, kind;
+    uint32_t uid;
+    uint16_t tag, pad[3];
+};
+static_assert(sizeof(AtomRec) == 208, "atom record must be 208 bytes");
+
+template <class F>
+int guard(Ctx* c, F&& f) {
+    try {
+        f();
+        if (c) c->err.clear();
+        return 0;
+    } catch (const Error& e) {
+        g_last = e.what();
+        if (c) c->err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_last = "host allocation failed";
+        if (c) c->err = g_last;
+        return E_NOMEM;
+    } catch (const std::exception& e) {
+        g_last = e.what();
+        if (c) c->err = e.what();
+        return E_INVAL;
+    }
+}
+
+template <class T>
+T* dalloc(size_t count) {
+    if (count == 0) count = 1;
+    void* p = nullptr;
+    if (hipMalloc(&p, count * sizeof(T)) != hipSuccess)
+        throw Error(E_NOMEM, "out of device memory (" + std::to_string(count * sizeof(T)) + " bytes)");
+    return static_cast<T*>(p);
+}
+
+template <class T>
+void dfree(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+// Cubic spline slopes of the reference's tables (EAM.cxx:603-686): sweep
+// solve of the tridiagonal system with end condition ib (1: z[N-1] = 0,
+// 2: second-derivative end), including the reference's back-substitution
+// z[i] = t[i] for i < N-1.
+void spline_slopes(const double* y, double* z, double H1, size_t N, int ib) {
+    std::vector<double> a(N, 2.0), b(N), c(N), d(N), s(N + 1), t(N + 1), u(N + 1), v(N + 1), w(N + 1);
+    b[0] = 1.0;
+    c[0] = 0.0;
+    d[0] = 3.0 * H1 * (y[1] - y[0]);
+    b[N - 1] = 0.0;
+    if (ib == 1) {
+        c[N - 1] = 0.0;
+        d[N - 1] = 0.0;
+    } else {
+        c[N - 1] = 1.0;
+        d[N - 1] = 3.0 * H1 * (y[N - 1] - y[N - 2]);
+    }
+    for (size_t i = 1; i + 1 < N; ++i) {
+        c[i] = 0.5;
+        b[i] = 0.5;
+        d[i] = 1.5 * H1 * (y[i + 1] - y[i - 1]);
+    }
+    u[0] = 0.0;
+    v[0] = 0.0;
+    w[0] = 1.0;
+    for (size_t i = 0; i < N; ++i) {
+        double rp = -1.0 / (a[i] + c[i] * v[i]);
+        v[i + 1] = rp * b[i];
+        u[i + 1] = rp * (c[i] * u[i] - d[i]);
+        w[i + 1] = rp * c[i] * w[i];
+    }
+    s[N - 1] = 1.0;
+    t[N - 1] = 0.0;
+    for (long i = (long)N - 2; i >= 0; --i) {
+        s[i] = v[i + 1] * s[i + 1] + w[i + 1];
+        t[i] = v[i + 1] * t[i + 1] + u[i + 1];
+    }
+    z[N - 1] = (d[N - 1] - b[N - 1] * t[0] - c[N - 1] * t[N - 2]) / (a[N - 1] + b[N - 1] * s[0] + c[N - 1] * s[N - 2]);
+    for (size_t i = 0; i + 1 < N; ++i) z[i] = t[i];
+}
+
+// interval records (y0, z0, y1, z1); the last record repeats the last knot
+void records(const double* y, const double* z, size_t N, std::vector<double>& out) {
+    for (size_t k = 0; k < N; ++k) {
+        size_t k1 = k + 1 < N ? k + 1 : k;
+        out.push_back(y[k]);
+        out.push_back(z[k]);
+        out.push_back(y[k1]);
+        out.push_back(z[k1]);
+    }
+}
+
+Pot* make_eam(Ctx* ctx, int nk, double rcut, const double* F, const double* rho, const double* rphi, int nr, double dr,
+              int nrho, double drho) {
+    if (nk < 1 || nk > kMaxKinds) throw Error(E_INVAL, "EAM kinds must be 1..8");
+    if (nr < 3 || nrho < 3) throw Error(E_INVAL, "EAM tables need at least 3 knots");
+    if (!(dr > 0) || !(drho > 0) || !(rcut > 0)) throw Error(E_INVAL, "EAM steps and cutoff must be positive");
+    auto p = std::make_unique<Pot>();
+    p->ctx = ctx;
+    p->type = PT_EAM;
+    p->rcut = rcut;
+    EamTables& T = p->eam;
+    T.nk = nk;
+    T.nr = nr;
+    T.nrho = nrho;
+    T.dr = dr;
+    T.drho = drho;
+    T.rc = rcut;
+    const int npair = nk * (nk + 1) / 2;
+    std::vector<double> rec64, F64, z(std::max(nr, nrho));
+    for (int k = 0; k < nk; ++k) {
+        spline_slopes(rho + (size_t)k * nr, z.data(), 1.0 / dr, nr, 1);
+        records(rho + (size_t)k * nr, z.data(), nr, rec64);
+    }
+    for (int q = 0; q < npair; ++q) {
+        spline_slopes(rphi + (size_t)q * nr, z.data(), 1.0 / dr, nr, 1);
+        records(rphi + (size_t)q * nr, z.data(), nr, rec64);
+    }
+    for (int k = 0; k < nk; ++k) {
+        spline_slopes(F + (size_t)k * nrho, z.data(), 1.0 / drho, nrho, 2);
+        records(F + (size_t)k * nrho, z.data(), nrho, F64);
+    }
+    for (double x : rec64)
+        if (!std::isfinite(x)) throw Error(E_RANGE, "non-finite value in the EAM r tables");
+    std::vector<float> rec32(rec64.begin(), rec64.end());
+    const size_t nrec = rec64.size() / 4;
+    T.r64 = dalloc<double4>(nrec);
+    T.r32 = dalloc<float4>(nrec);
+    T.F64 = dalloc<double4>(F64.size() / 4);
+    HIPCHECK(hipMemcpy(T.r64, rec64.data(), rec64.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(T.r32, rec32.data(), rec32.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(T.F64, F64.data(), F64.size() * sizeof(double), hipMemcpyHostToDevice));
+    return p.release();
+}
+
+// Single-element setfl reader with the reference's unit conversion
+// (EAM.cxx:55-87): skip 4 lines; Nrho drho Nr dr rc; Z M a lattice; F (eV),
+// rho, r*phi (eV*Angstrom); r, rc from Angstrom to nm.
+Pot* make_eam_file(Ctx* ctx, const char* path) {
+    std::ifstream f(path);
+    if (!f.is_open()) throw Error(E_FILE, std::string("Failed to open file: ") + path);
+    std::string line, lattice;
+    for (int i = 0; i < 4; ++i) std::getline(f, line);
+    int nrho = 0, nr = 0, Z = 0;
+    double drho = 0, dr = 0, rc = 0, M = 0, a0 = 0;
+    f >> nrho >> drho >> nr >> dr >> rc;
+    f >> Z >> M >> a0 >> lattice;
+    if (!f || nrho < 3 || nr < 3) throw Error(E_FILE, std::string("malformed setfl header: ") + path);
+    const double Na = 6.02214076e+23, Qe = 1.602176634e-19;
+    const double UnMass = 1e-3 / Na, UnLeng = 1e-9, UnTime = 1e-12;
+    const double UnEnrg = UnMass * (UnLeng / UnTime) * (UnLeng / UnTime);
+    std::vector<double> F(nrho), rho(nr), rphi(nr);
+    for (auto& x : F) {
+        f >> x;
+        x *= Qe / UnEnrg;
+    }
+    for (auto& x : rho) f >> x;
+    for (auto& x : rphi) {
+        f >> x;
+        x *= Qe / (10 * UnEnrg);
+    }
+    if (!f) throw Error(E_FILE, std::string("truncated setfl tables: ") + path);
+    return make_eam(ctx, 1, rc / 10, F.data(), rho.data(), rphi.data(), nr, dr / 10, nrho, drho);
+}
+
+void free_pot(Pot* p) {
+    if (!p) return;
+    if (p->type == PT_BPNN) {
+        bpnn_destroy(p);
+        return;
+    }
+    dfree(p->eam.r64);
+    dfree(p->eam.r32);
+    dfree(p->eam.F64);
+    dfree(p->eam.knots32);
+    delete p;
+}
+
+// ---- systems -------------------------------------------------------------
+
+__global__ void k_scatter_in(const double* __restrict__ stage, const double4* __restrict__ spos,
+                             const int* __restrict__ where, double4* __restrict__ pos, double* __restrict__ d,
+                             float* __restrict__ fl, uint32_t dmask, int set_r, int set_f32, int n) {
+    int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= n) return;
+    int p = where[o];
+    double4 q = pos[p];
+    double4 s = spos[o];
+    if (set_r) {
+        q.x = s.x;
+        q.y = s.y;
+        q.z = s.z;
+    }
+    q.w = s.w;
+    pos[p] = q;
+    for (int f = 0; f < D_NFIELDS; ++f)
+        if (dmask & (1u << f)) d[(size_t)f * n + p] = stage[(size_t)f * n + o];
+    if (set_f32)
+        for (int c = 0; c < 3; ++c) fl[(size_t)c * n + p] = (float)stage[(size_t)(D_FX + c) * n + o];
+}
+
+__global__ void k_gather_out(double* __restrict__ stage, double4* __restrict__ spos, const int* __restrict__ where,
+                             const double4* __restrict__ pos, const double* __restrict__ d, const float* __restrict__ fl,
+                             int f_mixed, int n) {
+    int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= n) return;
+    int p = where[o];
+    spos[o] = pos[p];
+    for (int f = 0; f < D_NFIELDS; ++f) stage[(size_t)f * n + o] = d[(size_t)f * n + p];
+    if (f_mixed)
+        for (int c = 0; c < 3; ++c) stage[(size_t)(D_FX + c) * n + o] = (double)fl[(size_t)c * n + p];
+}
+
+void free_batch(Batch& b) {
+    dfree(b.pos[0]);
+    dfree(b.pos[1]);
+    dfree(b.dst[0]);
+    dfree(b.dst[1]);
+    dfree(b.fst[0]);
+    dfree(b.fst[1]);
+    dfree(b.orig[0]);
+    dfree(b.orig[1]);
+    dfree(b.rsave);
+    dfree(b.where);
+    dfree(b.cell_count);
+    dfree(b.cell_start);
+    dfree(b.cell_fill);
+    dfree(b.atom_cell);
+    dfree(b.sorted_src);
+    dfree(b.nl);
+    dfree(b.nn);
+    dfree(b.misc);
+    dfree(b.d_desc);
+    dfree(b.thermo);
+    if (b.bp_scratch) (void)hipFree(b.bp_scratch);
+    b.bp_scratch = nullptr;
+    for (auto& t : b.ev_pending) {
+        b.ev_free.push_back(t.a);
+        b.ev_free.push_back(t.b);
+    }
+    for (auto e : b.ev_free) (void)hipEventDestroy(e);
+    b.ev_free.clear();
+    b.ev_pending.clear();
+}
+
+// Cell grid of each system: cells of edge >= max rns over the box (capped
+// per axis), cell offsets into the global cell arrays.
+void layout_cells(Batch& b) {
+    int off = 0;
+    for (int s = 0; s < b.nsys; ++s) {
+        SysDesc& d = b.desc[s];
+        const double rmax = b.rns_max[s];
+        for (int a = 0; a < 3; ++a) {
+            int n = 1;
+            if (rmax > 0 && d.L[a] > 0) {
+                double q = std::floor(d.L[a] / (rmax * (1.0 + 1e-9)));
+                n = (int)std::max(1.0, std::min(q, 256.0));
+            }
+            d.nc[a] = n;
+            d.csize[a] = d.L[a] > 0 ? d.L[a] / n : 1e300;
+        }
+        d.rns2 = rmax * rmax;
+        d.cell_off = off;
+        off += d.nc[0] * d.nc[1] * d.nc[2];
+    }
+    if (off != b.ncells) {
+        dfree(b.cell_count);
+        dfree(b.cell_start);
+        dfree(b.cell_fill);
+        b.ncells = off;
+        b.cell_count = dalloc<int>(off + 1);
+        b.cell_start = dalloc<int>(off + 1);
+        b.cell_fill = dalloc<int>(std::max(off + 1, b.n));
+    }
+    HIPCHECK(hipMemcpyAsync(b.d_desc, b.desc.data(), sizeof(SysDesc) * b.nsys, hipMemcpyHostToDevice, b.ctx->stream));
+}
+
+void size_list(Batch& b) {
+    // initial capacity from the number density: 1.3 x (4/3) pi rns^3 n + 16
+    double est = 16;
+    for (int s = 0; s < b.nsys; ++s) {
+        const SysDesc& d = b.desc[s];
+        double vol = 1.0;
+        for (int a = 0; a < 3; ++a) vol *= d.L[a] > 0 ? d.L[a] : 1.0;
+        double nat = b.atom_off[s + 1] - b.atom_off[s];
+        double r = b.rns_max[s];
+        est = std::max(est, 1.3 * nat / vol * 4.18879 * r * r * r + 16);
+    }
+    est = std::min(est, 4096.0);
+    int cap = (((int)est + b.tpa - 1) / b.tpa) * b.tpa;
+    const int A = kWave / b.tpa;
+    size_t waves = (size_t)(b.n + A - 1) / A;
+    size_t elems = waves * (size_t)(cap / b.tpa) * kWave;
+    if (elems > b.nl_elems || !b.nl) {
+        dfree(b.nl);
+        b.nl = dalloc<int>(elems);
+        b.nl_elems = elems;
+    }
+    b.cap = cap;
+    b.M = cap / b.tpa;
+}
+
+void sync_where(Batch& b, std::vector<int>& where) {
+    where.resize(b.n);
+    HIPCHECK(hipMemcpyAsync(where.data(), b.where, sizeof(int) * b.n, hipMemcpyDeviceToHost, b.ctx->stream));
+    HIPCHECK(hipStreamSynchronize(b.ctx->stream));
+}
+
+void upload(Batch& b, const AtomRec* a, uint32_t fields) {
+    const int n = b.n;
+    if (n == 0) return;
+    hipStream_t st = b.ctx->stream;
+    std::vector<double> stage((size_t)D_NFIELDS * n, 0.0);
+    std::vector<double4> spos(n);
+    uint32_t dmask = 0;
+    auto put = [&](int f, int o, double v) { stage[(size_t)f * n + o] = v; };
+    bool rns_changed = false;
+    for (int s = 0; s < b.nsys; ++s) {
+        for (int o = b.atom_off[s]; o < b.atom_off[s + 1]; ++o) {
+            const AtomRec& r = a[o];
+            if (fields & MDC_F_KIND) {
+                if (r.kind > 255) throw Error(E_INVAL, "atom kind above 255 is not supported");
+                b.kind[o] = r.kind;
+            }
+            if (fields & MDC_F_TAG) b.tag[o] = r.tag;
+            if (fields & MDC_F_UID) b.uid[o] = r.uid;
+            if (fields & MDC_F_RCUT) b.rcut_host[o] = r.rcut;
+            spos[o] = make_double4(r.r[0], r.r[1], r.r[2], pack_w(b.kind[o], b.tag[o] == 0 ? 1 : 0, s));
+            if (fields & MDC_F_V)
+                for (int c = 0; c < 3; ++c) put(D_VX + c, o, r.v[c]);
+            if (fields & MDC_F_F)
+                for (int c = 0; c < 3; ++c) put(D_FX + c, o, r.f[c]);
+            if (fields & MDC_F_VIR) {
+                // column-major V[col*3 + row]
+                put(D_VXX, o, r.V[0]);
+                put(D_VYY, o, r.V[4]);
+                put(D_VZZ, o, r.V[8]);
+                put(D_VXY, o, r.V[3]);
+                put(D_VXZ, o, r.V[6]);
+                put(D_VYZ, o, r.V[7]);
+            }
+            if (fields & MDC_F_M) put(D_M, o, r.m);
+            if (fields & MDC_F_RNS) {
+                if (!(r.rns >= 0) || !std::isfinite(r.rns)) throw Error(E_INVAL, "rns must be finite and >= 0");
+                put(D_RNS, o, r.rns);
+            }
+            if (fields & MDC_F_EP) put(D_EP, o, r.Ep);
+            if (fields & MDC_F_EM) put(D_EM, o, r.Em);
+            if (fields & MDC_F_NC) put(D_NC, o, r.nc);
+        }
+    }
+    if (fields & MDC_F_V) dmask |= 7u << D_VX;
+    if (fields & MDC_F_F) dmask |= 7u << D_FX;
+    if (fields & MDC_F_VIR) dmask |= 63u << D_VXX;
+    if (fields & MDC_F_M) dmask |= 1u << D_M;
+    if (fields & MDC_F_RNS) dmask |= 1u << D_RNS;
+    if (fields & MDC_F_EP) dmask |= 1u << D_EP;
+    if (fields & MDC_F_EM) dmask |= 1u << D_EM;
+    if (fields & MDC_F_NC) dmask |= 1u << D_NC;
+    if (fields & MDC_F_RNS) {
+        for (int s = 0; s < b.nsys; ++s) {
+            double mn = 1e300, mx = 0;
+            for (int o = b.atom_off[s]; o < b.atom_off[s + 1]; ++o) {
+                mn = std::min(mn, a[o].rns);
+                mx = std::max(mx, a[o].rns);
+            }
+            if (b.atom_off[s + 1] == b.atom_off[s]) mn = mx = 0;
+            if (mn != b.rns_min[s] || mx != b.rns_max[s]) rns_changed = true;
+            b.rns_min[s] = mn;
+            b.rns_max[s] = mx;
+        }
+    }
+    double* d_stage = dalloc<double>((size_t)D_NFIELDS * n);
+    double4* d_spos = dalloc<double4>(n);
+    try {
+        HIPCHECK(hipMemcpyAsync(d_stage, stage.data(), stage.size() * sizeof(double), hipMemcpyHostToDevice, st));
+        HIPCHECK(hipMemcpyAsync(d_spos, spos.data(), spos.size() * sizeof(double4), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_scatter_in, dim3((n + 255) / 256), dim3(256), 0, st, d_stage, d_spos, b.where, b.pos[b.cur],
+                           b.dst[b.sc], b.fst[b.sc], dmask, (fields & MDC_F_R) ? 1 : 0,
+                           (fields & MDC_F_F) ? 1 : 0, n);
+        HIPCHECK(hipStreamSynchronize(st));
+    } catch (...) {
+        dfree(d_stage);
+        dfree(d_spos);
+        throw;
+    }
+    dfree(d_stage);
+    dfree(d_spos);
+    if (fields & MDC_F_F) b.f_prec = P_FP64;
+    if (rns_changed) {
+        layout_cells(b);
+        size_list(b);
+        b.list_valid = false;
+    }
+}
+
+void download(Batch& b, AtomRec* a, uint32_t fields) {
+    const int n = b.n;
+    if (n == 0) return;
+    hipStream_t st = b.ctx->stream;
+    double* d_stage = dalloc<double>((size_t)D_NFIELDS * n);
+    double4* d_spos = dalloc<double4>(n);
+    std::vector<double> stage((size_t)D_NFIELDS * n);
+    std::vector<double4> spos(n);
+    try {
+        hipLaunchKernelGGL(k_gather_out, dim3((n + 255) / 256), dim3(256), 0, st, d_stage, d_spos, b.where,
+                           b.pos[b.cur], b.dst[b.sc], b.fst[b.sc], b.f_prec == P_MIXED ? 1 : 0, n);
+        HIPCHECK(hipMemcpyAsync(stage.data(), d_stage, stage.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipMemcpyAsync(spos.data(), d_spos, spos.size() * sizeof(double4), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+    } catch (...) {
+        dfree(d_stage);
+        dfree(d_spos);
+        throw;
+    }
+    dfree(d_stage);
+    dfree(d_spos);
+    auto get = [&](int f, int o) { return stage[(size_t)f * n + o]; };
+    for (int s = 0; s < b.nsys; ++s) {
+        const SysDesc& d = b.desc[s];
+        for (int o = b.atom_off[s]; o < b.atom_off[s + 1]; ++o) {
+            AtomRec& r = a[o];
+            if (fields & MDC_F_R) {
+                r.r[0] = spos[o].x;
+                r.r[1] = spos[o].y;
+                r.r[2] = spos[o].z;
+            }
+            if (fields & MDC_F_V)
+                for (int c = 0; c < 3; ++c) r.v[c] = get(D_VX + c, o);
+            if (fields & MDC_F_F)
+                for (int c = 0; c < 3; ++c) r.f[c] = get(D_FX + c, o);
+            if (fields & MDC_F_VIR) {
+                double xx = get(D_VXX, o), yy = get(D_VYY, o), zz = get(D_VZZ, o);
+                double xy = get(D_VXY, o), xz = get(D_VXZ, o), yz = get(D_VYZ, o);
+                double V[9] = {xx, xy, xz, xy, yy, yz, xz, yz, zz};
+                std::memcpy(r.V, V, sizeof(V));
+            }
+            if (fields & MDC_F_M) r.m = get(D_M, o);
+            if (fields & MDC_F_RCUT) r.rcut = b.last_rcut >= 0 ? b.last_rcut : b.rcut_host[o];
+            if (fields & MDC_F_RNS) r.rns = get(D_RNS, o);
+            if (fields & MDC_F_EP) r.Ep = get(D_EP, o);
+            if (fields & MDC_F_EM) r.Em = get(D_EM, o);
+            if (fields & MDC_F_NC) r.nc = get(D_NC, o);
+            if (fields & MDC_F_KIND) r.kind = b.kind[o];
+            if (fields & MDC_F_UID) r.uid = b.uid[o];
+            if (fields & MDC_F_TAG) r.tag = b.tag[o];
+            if (fields & MDC_F_BC) {
+                // boundary::Periodic::need_apply (periodic.cxx:9-21)
+                bool need = false;
+                if (b.periodic_bc[s] && b.tag[o] != 0) {
+                    const double rr[3] = {spos[o].x, spos[o].y, spos[o].z};
+                    for (int ax = 0; ax < 3; ++ax)
+                        if (d.per[ax]) {
+                            double dist = std::min(std::fabs(rr[ax] - d.lo[ax]), std::fabs(rr[ax] - d.hi[ax]));
+                            need = need || dist < get(D_RNS, o);
+                        }
+                }
+                r.bc = need ? 1 : 0;
+            }
+        }
+    }
+}
+
+void check_kinds(const Batch& b, const Pot& p) {
+    if (p.type == PT_EAM && p.eam.nk > 1)
+        for (int o = 0; o < b.n; ++o)
+            if (b.kind[o] >= p.eam.nk) throw Error(E_INVAL, "atom kind outside the alloy EAM's elements");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mdc_last_error(void) { return g_last.c_str(); }
+const char* mdc_ctx_last_error(const mdc_ctx* ctx) { return ctx ? ctx->c.err.c_str() : g_last.c_str(); }
+int mdc_atom_record_size(void) { return (int)sizeof(AtomRec); }
+
+mdc_status mdc_ctx_create(int device, mdc_ctx** out) {
+    return guard(nullptr, [&] {
+        if (!out) throw Error(E_INVAL, "null output pointer");
+        int ndev = 0;
+        HIPCHECK(hipGetDeviceCount(&ndev));
+        if (device < 0 || device >= ndev) throw Error(E_INVAL, "no such device");
+        HIPCHECK(hipSetDevice(device));
+        auto c = std::make_unique<mdc_ctx>();
+        c->c.device = device;
+        HIPCHECK(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));
+        void* pin = nullptr;
+        HIPCHECK(hipHostMalloc(&pin, sizeof(int) * M_COUNT, hipHostMallocDefault));
+        c->c.h_pin = static_cast<int*>(pin);
+        std::memset(c->c.h_pin, 0, sizeof(int) * M_COUNT);
+        hipDeviceProp_t prop;
+        HIPCHECK(hipGetDeviceProperties(&prop, device));
+        c->c.cu_count = prop.multiProcessorCount;
+        *out = c.release();
+    });
+}
+
+mdc_status mdc_ctx_destroy(mdc_ctx* ctx) {
+    if (!ctx) return 0;
+    (void)hipSetDevice(ctx->c.device);
+    if (ctx->c.stream) (void)hipStreamDestroy(ctx->c.stream);
+    if (ctx->c.h_pin) (void)hipHostFree(ctx->c.h_pin);
+    delete ctx;
+    return 0;
+}
+
+mdc_status mdc_pot_lj_create(mdc_ctx* ctx, double a, double s, double rcut, mdc_pot** out) {
+    return guard(ctx ? &ctx->c : nullptr, [&] {
+        if (!ctx || !out) throw Error(E_INVAL, "null argument");
+        if (!(rcut > 0) || !(s > 0)) throw Error(E_INVAL, "LJ sigma and cutoff must be positive");
+        auto p = std::make_unique<Pot>();
+        p->ctx = &ctx->c;
+        p->type = PT_LJ;
+        p->rcut = rcut;
+        // LJs-cut.cxx:8-20
+        LJParams& L = p->lj;
+        L.a = a;
+        L.s = s;
+        L.s6 = std::pow(s, 6);
+        L.lj1 = 2 * L.s6;
+        L.lj2 = 1.0;
+        L.factor = 48.0 * a * L.s6;
+        L.rc = rcut;
+        L.rc2 = rcut * rcut;
+        *out = reinterpret_cast<mdc_pot*>(p.release());
+    });
+}
+
+mdc_status mdc_pot_eam_create(mdc_ctx* ctx, double rcut, const double* F, const double* rho, const double* rphi,
+                              int nr, double dr, int nrho, double drho, mdc_pot** out) {
+    return guard(ctx ? &ctx->c : nullptr, [&] {
+        if (!ctx || !out || !F || !rho || !rphi) throw Error(E_INVAL, "null argument");
+        HIPCHECK(hipSetDevice(ctx->c.device));
+        *out = reinterpret_cast<mdc_pot*>(make_eam(&ctx->c, 1, rcut, F, rho, rphi, nr, dr, nrho, drho));
+    });
+}
+
+mdc_status mdc_pot_eam_alloy_create(mdc_ctx* ctx, int nk, double rcut, const double* F, const double* rho,
+                                    const double* rphi, int nr, double dr, int nrho, double drho, mdc_pot** out) {
+    return guard(ctx ? &ctx->c : nullptr, [&] {
+        if (!ctx || !out || !F || !rho || !rphi) throw Error(E_INVAL, "null argument");
+        HIPCHECK(hipSetDevice(ctx->c.device));
+        *out = reinterpret_cast<mdc_pot*>(make_eam(&ctx->c, nk, rcut, F, rho, rphi, nr, dr, nrho, drho));
+    });
+}
+
+mdc_status mdc_pot_eam_create_file(mdc_ctx* ctx, const char* path, mdc_pot** out) {
+    return guard(ctx ? &ctx->c : nullptr, [&] {
+        if (!ctx || !out || !path) throw Error(E_INVAL, "null argument");
+        HIPCHECK(hipSetDevice(ctx->c.device));
+        *out = reinterpret_cast<mdc_pot*>(make_eam_file(&ctx->c, path));
+    });
+}
+
+mdc_status mdc_pot_bpnn_create(mdc_ctx* ctx, const char* path, mdc_pot** out) {
+    return guard(ctx ? &ctx->c : nullptr, [&] {
+        if (!ctx || !out || !path) throw Error(E_INVAL, "null argument");
+        HIPCHECK(hipSetDevice(ctx->c.device));
+        *out = reinterpret_cast<mdc_pot*>(bpnn_create(&ctx->c, path));
+    });
+}
+
+mdc_status mdc_pot_rcut(const mdc_pot* pot, double* rcut) {
+    if (!pot || !rcut) return E_INVAL;
+    *rcut = pot->p.rcut;
+    return 0;
+}
+
+mdc_status mdc_pot_destroy(mdc_pot* pot) {
+    if (!pot) return 0;
+    (void)hipSetDevice(pot->p.ctx->device);
+    free_pot(&pot->p);
+    return 0;
+}
+
+mdc_status mdc_sys_create(mdc_ctx* ctx, int nsys, const uint64_t* natoms, const double* lo, const double* hi,
+                          const uint8_t* periodic, int periodic_boundary, mdc_sys** out) {
+    return guard(ctx ? &ctx->c : nullptr, [&] {
+        if (!ctx || !out || nsys < 1 || !natoms || !lo || !hi || !periodic) throw Error(E_INVAL, "bad argument");
+        if (nsys > (1 << 20)) throw Error(E_INVAL, "too many systems in one batch");
+        HIPCHECK(hipSetDevice(ctx->c.device));
+        auto s = std::make_unique<mdc_sys>();
+        Batch& b = s->b;
+        b.ctx = &ctx->c;
+        b.nsys = nsys;
+        b.atom_off.assign(nsys + 1, 0);
+        for (int i = 0; i < nsys; ++i) {
+            b.atom_off[i + 1] = b.atom_off[i] + (int)natoms[i];
+            if ((uint64_t)b.atom_off[i + 1] > (1ull << 30)) throw Error(E_INVAL, "too many atoms");
+        }
+        b.n = b.atom_off[nsys];
+        b.desc.resize(nsys);
+        b.periodic_bc.assign(nsys, periodic_boundary ? 1 : 0);
+        for (int i = 0; i < nsys; ++i) {
+            SysDesc& d = b.desc[i];
+            std::memset(&d, 0, sizeof(d));
+            d.any_per = 0;
+            for (int a = 0; a < 3; ++a) {
+                d.lo[a] = lo[3 * i + a];
+                d.hi[a] = hi[3 * i + a];
+                d.L[a] = d.hi[a] - d.lo[a];
+                // the image rule of the ghost pass applies only when the solver
+                // carries the Periodic boundary
+                d.per[a] = periodic[3 * i + a] ? 1 : 0;
+                if (d.per[a] && !(d.L[a] > 0)) throw Error(E_INVAL, "periodic axis with non-positive length");
+                d.any_per |= (d.per[a] && periodic_boundary) ? 1 : 0;
+            }
+            d.atom_off = b.atom_off[i];
+            d.natoms = b.atom_off[i + 1] - b.atom_off[i];
+            d.trig2 = -1.0;
+        }
+        b.rns_min.assign(nsys, 0.0);
+        b.rns_max.assign(nsys, 0.0);
+        b.kind.assign(b.n, 0);
+        b.tag.assign(b.n, 65535);
+        b.uid.assign(b.n, 0);
+        b.rcut_host.assign(b.n, 0.0);
+        const int n = b.n;
+        try {
+            b.d_desc = dalloc<SysDesc>(nsys);
+            b.pos[0] = dalloc<double4>(n);
+            b.pos[1] = dalloc<double4>(n);
+            for (int k = 0; k < 2; ++k) {
+                b.dst[k] = dalloc<double>((size_t)D_NFIELDS * n);
+                b.fst[k] = dalloc<float>((size_t)F_NFIELDS * n);
+                b.orig[k] = dalloc<int>(n);
+                HIPCHECK(hipMemset(b.dst[k], 0, sizeof(double) * D_NFIELDS * std::max(n, 1)));
+                HIPCHECK(hipMemset(b.fst[k], 0, sizeof(float) * F_NFIELDS * std::max(n, 1)));
+            }
+            b.rsave = dalloc<double4>(n);
+            b.where = dalloc<int>(n);
+            b.atom_cell = dalloc<int>(n);
+            b.sorted_src = dalloc<int>(n);
+            b.nn = dalloc<int>(n);
+            b.misc = dalloc<int>(M_COUNT);
+            b.thermo = dalloc<double>((size_t)9 * nsys + 2);
+            HIPCHECK(hipMemset(b.misc, 0, sizeof(int) * M_COUNT));
+            HIPCHECK(hipMemset(b.pos[0], 0, sizeof(double4) * std::max(n, 1)));
+            HIPCHECK(hipMemset(b.rsave, 0, sizeof(double4) * std::max(n, 1)));
+            HIPCHECK(hipMemset(b.nn, 0, sizeof(int) * std::max(n, 1)));
+            std::vector<int> iota(n);
+            std::iota(iota.begin(), iota.end(), 0);
+            HIPCHECK(hipMemcpy(b.where, iota.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+            HIPCHECK(hipMemcpy(b.orig[0], iota.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+            b.tpa = n >= 200000 ? 4 : 8;
+            layout_cells(b);
+            size_list(b);
+        } catch (...) {
+            free_batch(b);
+            throw;
+        }
+        *out = s.release();
+    });
+}
+
+mdc_status mdc_sys_destroy(mdc_sys* sys) {
+    if (!sys) return 0;
+    (void)hipSetDevice(sys->b.ctx->device);
+    (void)hipStreamSynchronize(sys->b.ctx->stream);
+    free_batch(sys->b);
+    delete sys;
+    return 0;
+}
+
+mdc_status mdc_sys_upload(mdc_sys* sys, const void* atoms, uint32_t fields) {
+    return guard(sys ? sys->b.ctx : nullptr, [&] {
+        if (!sys || !atoms) throw Error(E_INVAL, "null argument");
+        HIPCHECK(hipSetDevice(sys->b.ctx->device));
+        upload(sys->b, static_cast<const AtomRec*>(atoms), fields);
+    });
+}
+
+mdc_status mdc_sys_download(mdc_sys* sys, void* atoms, uint32_t fields) {
+    return guard(sys ? sys->b.ctx : nullptr, [&] {
+        if (!sys || !atoms) throw Error(E_INVAL, "null argument");
+        HIPCHECK(hipSetDevice(sys->b.ctx->device));
+        download(sys->b, static_cast<AtomRec*>(atoms), fields);
+    });
+}
+
+mdc_status mdc_sys_natoms(const mdc_sys* sys, uint64_t* n) {
+    if (!sys || !n) return E_INVAL;
+    *n = (uint64_t)sys->b.n;
+    return 0;
+}
+
+mdc_status mdc_sys_set_lanes(mdc_sys* sys, int lanes) {
+    return guard(sys ? sys->b.ctx : nullptr, [&] {
+        if (!sys) throw Error(E_INVAL, "null argument");
+        if (lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16)
+            throw Error(E_INVAL, "lanes per atom must be 1, 2, 4, 8 or 16");
+        Batch& b = sys->b;
+        if (lanes == b.tpa) return;
+        HIPCHECK(hipSetDevice(b.ctx->device));
+        int maxcap = b.cap;
+        b.tpa = lanes;
+        size_list(b);
+        if (maxcap > b.cap) {
+            // keep the grown capacity
+            int cap = ((maxcap + b.tpa - 1) / b.tpa) * b.tpa;
+            const int A = kWave / b.tpa;
+            size_t elems = (size_t)((b.n + A - 1) / A) * (size_t)(cap / b.tpa) * kWave;
+            if (elems > b.nl_elems) {
+                dfree(b.nl);
+                b.nl = dalloc<int>(elems);
+                b.nl_elems = elems;
+            }
+            b.cap = cap;
+            b.M = cap / b.tpa;
+        }
+        b.list_valid = false;
+    });
+}
+
+mdc_status mdc_nlist_build(mdc_sys* sys) {
+    return guard(sys ? sys->b.ctx : nullptr, [&] {
+        if (!sys) throw Error(E_INVAL, "null argument");
+        HIPCHECK(hipSetDevice(sys->b.ctx->device));
+        hipEvent_t t0 = timer_begin(sys->b);
+        nlist_rebuild(sys->b);
+        timer_end(sys->b, t0, K_REBUILD);
+    });
+}
+
+mdc_status mdc_nlist_decide(mdc_sys* sys, double rcut, int* need) {
+    return guard(sys ? sys->b.ctx : nullptr, [&] {
+        if (!sys || !need) throw Error(E_INVAL, "null argument");
+        HIPCHECK(hipSetDevice(sys->b.ctx->device));
+        *need = nlist_decide(sys->b, rcut);
+    });
+}
+
+mdc_status mdc_nlist_export(mdc_sys* sys, uint64_t* offsets, uint64_t* ids, uint64_t* total) {
+    return guard(sys ? sys->b.ctx : nullptr, [&] {
+        if (!sys) throw Error(E_INVAL, "null argument");
+        Batch& b = sys->b;
+        if (!b.list_valid) throw Error(E_STATE, "no neighbour list: call mdc_nlist_build first");
+        HIPCHECK(hipSetDevice(b.ctx->device));
+        const int n = b.n;
+        std::vector<int> nn(n), orig(n), where(n);
+        std::vector<int> nl(b.nl_elems);
+        hipStream_t st = b.ctx->stream;
+        HIPCHECK(hipMemcpyAsync(nn.data(), b.nn, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipMemcpyAsync(orig.data(), b.orig[b.sc], sizeof(int) * n, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipMemcpyAsync(where.data(), b.where, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipMemcpyAsync(nl.data(), b.nl, sizeof(int) * b.nl_elems, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        uint64_t tot = 0;
+        for (int i = 0; i < n; ++i) tot += (uint64_t)std::min(nn[i], b.cap) + 1;
+        if (total) *total = tot;
+        if (!offsets || !ids) return;
+        uint64_t o = 0;
+        for (int a = 0; a < n; ++a) {
+            offsets[a] = o;
+            const int p = where[a];
+            ids[o++] = (uint64_t)a;  // the reference's list includes self
+            const int c = std::min(nn[p], b.cap);
+            for (int k = 0; k < c; ++k) ids[o++] = (uint64_t)orig[nl[list_index(p, k, b.tpa, b.M)]];
+        }
+        offsets[n] = o;
+    });
+}
+
+mdc_status mdc_nlist_stats(mdc_sys* sys, double* mean, int* maxn, int* capacity, int* rebuilds) {
+    return guard(sys ? sys->b.ctx : nullptr, [&] {
+        if (!sys) throw Error(E_INVAL, "null argument");
+        Batch& b = sys->b;
+        HIPCHECK(hipSetDevice(b.ctx->device));
+        std::vector<int> nn(b.n);
+        HIPCHECK(hipMemcpyAsync(nn.data(), b.nn, sizeof(int) * b.n, hipMemcpyDeviceToHost, b.ctx->stream));
+        HIPCHECK(hipStreamSynchronize(b.ctx->stream));
+        double s = 0;
+        int mx = 0;
+        for (int x : nn) {
+            s += x;
+            mx = std::max(mx, x);
+        }
+        if (mean) *mean = b.n ? s / b.n : 0.0;
+        if (maxn) *maxn = mx;
+        if (capacity) *capacity = b.cap;
+        if (rebuilds) *rebuilds = b.rebuilds;
+    });
+}
+
+mdc_status mdc_compute(mdc_sys* sys, const mdc_pot* pot, uint32_t what, int precision) {
+    return guard(sys ? sys->b.ctx : nullptr, [&] {
+        if (!sys || !pot) throw Error(E_INVAL, "null argument");
+        if (precision != MDC_FP64 && precision != MDC_MIXED) throw Error(E_INVAL, "precision must be FP64 or MIXED");
+        HIPCHECK(hipSetDevice(sys->b.ctx->device));
+        check_kinds(sys->b, pot->p);
+        force_compute(sys->b, pot->p, (int)what, precision);
+    });
+}
+
+mdc_status mdc_step(mdc_sys* sys, const mdc_pot* pot, double dt, int nsteps, int precision, int* rebuilds) {
+    return guard(sys ? sys->b.ctx : nullptr, [&] {
+        if (!sys || !pot) throw Error(E_INVAL, "null argument");
+        if (precision != MDC_FP64 && precision != MDC_MIXED) throw Error(E_INVAL, "precision must be FP64 or MIXED");
+        if (!(dt > 0) || nsteps < 0) throw Error(E_INVAL, "dt must be positive and nsteps >= 0");
+        HIPCHECK(hipSetDevice(sys->b.ctx->device));
+        check_kinds(sys->b, pot->p);
+        int r0 = sys->b.rebuilds;
+        run_steps(sys->b, pot->p, dt, nsteps, precision, 16);
+        if (rebuilds) *rebuilds = sys->b.rebuilds - r0;
+    });
+}
+
+mdc_status mdc_thermo(mdc_sys* sys, double* out) {
+    return guard(sys ? sys->b.ctx : nullptr, [&] {
+        if (!sys || !out) throw Error(E_INVAL, "null argument");
+        HIPCHECK(hipSetDevice(sys->b.ctx->device));
+        thermo(sys->b, out);
+    });
+}
+
+mdc_status mdc_sync(mdc_sys* sys) {
+    return guard(sys ? sys->b.ctx : nullptr, [&] {
+        if (!sys) throw Error(E_INVAL, "null argument");
+        HIPCHECK(hipStreamSynchronize(sys->b.ctx->stream));
+    });
+}
+
+mdc_status mdc_timing(mdc_sys* sys, int enable, double* total_ms, int64_t* launches) {
+    return guard(sys ? sys->b.ctx : nullptr, [&] {
+        if (!sys) throw Error(E_INVAL, "null argument");
+        Batch& b = sys->b;
+        HIPCHECK(hipStreamSynchronize(b.ctx->stream));
+        timer_drain(b);
+        for (int k = 0; k < 8; ++k) {
+            if (total_ms) total_ms[k] = b.kt_ms[k];
+            if (launches) launches[k] = b.kt_n[k];
+        }
+        if (enable >= 0) {
+            b.timing = enable != 0;
+            for (int k = 0; k < 8; ++k) {
+                b.kt_ms[k] = 0;
+                b.kt_n[k] = 0;
+            }
+        }
+    });
+}
+
+}  // extern "C"

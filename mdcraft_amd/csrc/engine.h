@@ -1,0 +1,160 @@
+// Host-side engine state shared by the kernel launchers and the C ABI.
+#pragma once
+#include <string>
+#include <vector>
+#include <stdexcept>
+#include "common.h"
+
+namespace mdc {
+
+enum ErrCode { E_OK = 0, E_INVAL = 1, E_HIP = 2, E_NOMEM = 3, E_FILE = 4, E_STATE = 5, E_BUDGET = 6, E_RANGE = 7 };
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHECK(x)                                                                                  \
+    do {                                                                                             \
+        hipError_t e_ = (x);                                                                         \
+        if (e_ != hipSuccess) throw ::mdc::Error(::mdc::E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Per-atom double fields, SoA with stride N (dstate[f*N + i]).
+enum DField {
+    D_VX, D_VY, D_VZ,        // velocity (holds the half-step velocity inside a step sequence)
+    D_FX, D_FY, D_FZ,        // FP64 force
+    D_M, D_RNS,              // mass, neighbour search radius
+    D_EP, D_EM, D_NC, D_FP,  // pair energy, embedding energy, EAM density, F'(density)
+    D_VXX, D_VYY, D_VZZ, D_VXY, D_VXZ, D_VYZ,  // per-atom virial (symmetric part)
+    D_NFIELDS
+};
+// Per-atom float fields (mixed-precision force)
+enum FField { F_FX, F_FY, F_FZ, F_NFIELDS };
+
+enum Prec { P_FP64 = 0, P_MIXED = 1 };
+enum What { W_FORCE = 1, W_ENERGY = 2, W_VIRIAL = 4 };
+enum PotType { PT_LJ = 1, PT_EAM = 2, PT_BPNN = 4 };
+// misc device counters
+enum Misc { M_MAXNN = 0, M_ABORT = 1, M_ERR = 2, M_DECIDE = 3, M_COUNT = 8 };
+enum ErrBits { EB_RHO_RANGE = 1, EB_NONFINITE = 2, EB_R_RANGE = 4 };
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int* h_pin = nullptr;  // pinned host mirror of the misc counters
+    int cu_count = 0;
+};
+
+struct LJParams {
+    double a, s, s6, lj1, lj2, factor, rc, rc2;
+};
+
+// Spline tables of an (alloy) EAM potential.  Interval record k of a table
+// holds (y[k], z[k], y[k+1], z[k+1]); the record past the end repeats the
+// last knot, as the reference's [N1+1] read at N1 = N-1 assumes a value.
+struct EamTables {
+    int nk = 1;
+    int nr = 0, nrho = 0;
+    double dr = 0, drho = 0, rc = 0;
+    // device: rho tables [nk][nr], then pair tables [npair][nr] of r*phi;
+    // embedding tables [nk][nrho]
+    double4* r64 = nullptr;
+    float4* r32 = nullptr;
+    double4* F64 = nullptr;
+    float2* knots32 = nullptr;  // (y, z) knots of every r table for LDS staging: [ntab][nr+1]
+};
+
+struct BpnnParams;
+
+struct Pot {
+    Ctx* ctx = nullptr;
+    int type = 0;
+    double rcut = 0;
+    LJParams lj{};
+    EamTables eam;
+    BpnnParams* bp = nullptr;
+};
+
+struct Batch {
+    Ctx* ctx = nullptr;
+    int nsys = 0;
+    int n = 0;
+    std::vector<SysDesc> desc;
+    SysDesc* d_desc = nullptr;
+    std::vector<int> atom_off;  // [nsys+1]
+    // state
+    double4* pos[2] = {nullptr, nullptr};
+    int cur = 0;
+    double* dst[2] = {nullptr, nullptr};
+    float* fst[2] = {nullptr, nullptr};
+    int* orig[2] = {nullptr, nullptr};
+    int sc = 0;  // which dst/fst/orig copy is current
+    double4* rsave = nullptr;
+    int* where = nullptr;  // original index -> sorted index
+    // cells
+    int ncells = 0;
+    int* cell_count = nullptr;
+    int* cell_start = nullptr;
+    int* cell_fill = nullptr;
+    int* atom_cell = nullptr;
+    int* sorted_src = nullptr;
+    // list
+    int tpa = 4;
+    int cap = 0;   // slots per atom (multiple of tpa)
+    int M = 0;     // cap / tpa
+    size_t nl_elems = 0;
+    int* nl = nullptr;
+    int* nn = nullptr;
+    int* misc = nullptr;
+    bool list_valid = false;
+    int f_prec = -1;      // which force array holds the current force
+    double last_rcut = -1;
+    int rebuilds = 0;
+    // host copies of the integer fields the device does not need
+    std::vector<uint16_t> kind, tag;
+    std::vector<uint32_t> uid;
+    std::vector<double> rcut_host;
+    std::vector<uint8_t> periodic_bc;  // per system: solver built with the Periodic boundary
+    // scratch for BPNN
+    void* bp_scratch = nullptr;
+    size_t bp_scratch_bytes = 0;
+    // reduction output
+    double* thermo = nullptr;
+    // per-system extent of rns (host), for the displacement trigger
+    std::vector<double> rns_min, rns_max;
+    // optional kernel timing: hipEvents recorded on the launch stream around
+    // each launch of the timed kernels, drained at the next sync point
+    bool timing = false;
+    std::vector<hipEvent_t> ev_free;
+    struct Timed { hipEvent_t a, b; int kind; };
+    std::vector<Timed> ev_pending;
+    double kt_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    long kt_n[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+
+    double* d(int s, int f) const { return dst[s] + (size_t)f * n; }
+    float* fl(int s, int f) const { return fst[s] + (size_t)f * n; }
+};
+
+// timed kernel kinds
+enum KKind { K_FORCE = 0, K_DENSITY = 1, K_PREDRIFT = 2, K_BP_DESC = 3, K_BP_MLP = 4, K_BP_FORCE = 5, K_BP_GATHER = 6, K_REBUILD = 7 };
+hipEvent_t timer_begin(Batch& b);
+void timer_end(Batch& b, hipEvent_t a, int kind);
+void timer_drain(Batch& b);
+
+// launchers (nlist.hip)
+void nlist_rebuild(Batch& b);
+int nlist_decide(Batch& b, double rcut);
+// launchers (forces.hip)
+void force_compute(Batch& b, const Pot& p, int what, int prec);
+void run_steps(Batch& b, const Pot& p, double dt, int nsteps, int prec, int chunk);
+void thermo(Batch& b, double* out);
+void set_trigger(Batch& b, double rcut);
+void check_errors(Batch& b);
+// launchers (bpnn.hip)
+void bpnn_compute(Batch& b, const Pot& p, int what, int prec, int mode, double dt, int step);
+Pot* bpnn_create(Ctx* ctx, const char* path);
+void bpnn_destroy(Pot* p);
+
+}  // namespace mdc

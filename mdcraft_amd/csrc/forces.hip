@@ -1,0 +1,712 @@
+// Pair (LJsCut) and two-pass EAM force kernels with the velocity-Verlet
+// update fused in, the standalone pre-drift kernel, the FP64 thermo
+// reduction and the multi-step driver.
+//
+// Semantics restated from the reference:
+//   LJsCut force/virial  solver/potential/LJs-cut.cxx:22-131
+//   EAM density          solver/potential/EAM.cxx:499-560
+//   EAM embedding        solver/potential/EAM.cxx:562-601 (once per atom here;
+//                        the reference re-evaluates F'(rho_j) per pair)
+//   EAM force/virial     solver/potential/EAM.cxx:306-497
+//   Periodic ghost pass  solver/boundary/periodic.cxx:36-102 (pair_image)
+//   velocity Verlet      solver/stepper/verlet.cxx:36-85
+//   thermo               test/cxx/support/md_test_utils.h:214-239
+//
+// Precision: in the mixed mode the FP64 minimum-image difference is rounded
+// to FP32 and the pair arithmetic and per-atom force accumulation run in
+// FP32; energy and virial accumulate in FP64, and the EAM density sum,
+// embedding, integration and displacement check stay FP64.
+#include <climits>
+#include <cmath>
+#include "engine.h"
+
+namespace mdc {
+
+enum Mode { MODE_COMPUTE = 0, MODE_FINAL = 1, MODE_FINAL_DRIFT = 2 };
+
+__device__ inline float rcp_(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline double rcp_(double x) { return 1.0 / x; }
+
+template <typename R>
+struct Vec4T;
+template <>
+struct Vec4T<double> { using T = double4; };
+template <>
+struct Vec4T<float> { using T = float4; };
+
+// Hermite interval of the reference tables (EAM.cxx:367-394).
+// rec = (y0, z0, y1, z1); returns value and derivative at R1.
+template <typename R>
+__device__ inline void spline_vd(const typename Vec4T<R>::T rec, R R1, R RN, int N1, R dr, R dr1, R& val, R& der) {
+    R VR = (R)N1;
+    R DV = RN - VR;
+    R h = R1 - VR * dr;
+    R s = (rec.z - rec.x) * dr1;
+    R aa = (rec.w - s) - (s - rec.y);
+    R bb = aa - (s - rec.y);
+    val = rec.x + h * (rec.y + DV * (DV * aa - bb));
+    der = rec.y + (R)2 * DV * ((R)1.5 * DV * aa - bb);
+}
+
+template <typename R>
+__device__ inline R spline_v(const typename Vec4T<R>::T rec, R R1, R RN, int N1, R dr, R dr1) {
+    R VR = (R)N1;
+    R DV = RN - VR;
+    R h = R1 - VR * dr;
+    R s = (rec.z - rec.x) * dr1;
+    R aa = (rec.w - s) - (s - rec.y);
+    R bb = aa - (s - rec.y);
+    return rec.x + h * (rec.y + DV * (DV * aa - bb));
+}
+
+struct StepArgs {
+    double dt;
+    int step;  // index of this step inside the running sequence (-1: plain compute)
+    int mode;
+    int flags;  // W_ENERGY | W_VIRIAL
+    int store_aux;
+};
+
+struct AtomArrays {
+    double* d;   // current double fields base (stride n)
+    float* f;    // current float fields base (stride n)
+    int n;
+};
+
+// Finalise one atom after its force is known: store force, and for step
+// modes apply the closing half kick (verlet.cxx:80-85) and, when another
+// step follows, the opening half kick and drift (verlet.cxx:71-78) into the
+// other position buffer together with the displacement trigger of
+// verlet-list.cxx:149-154.
+template <typename R>
+__device__ inline void finalize_atom(int i, const double4& pi, double fx, double fy, double fz, const AtomArrays& A,
+                                     const StepArgs& S, double4* __restrict__ pos_next,
+                                     const double4* __restrict__ rsave, const SysDesc& s, int* __restrict__ misc) {
+    const int n = A.n;
+    if (sizeof(R) == 4) {
+        A.f[(size_t)F_FX * n + i] = (float)fx;
+        A.f[(size_t)F_FY * n + i] = (float)fy;
+        A.f[(size_t)F_FZ * n + i] = (float)fz;
+    } else {
+        A.d[(size_t)D_FX * n + i] = fx;
+        A.d[(size_t)D_FY * n + i] = fy;
+        A.d[(size_t)D_FZ * n + i] = fz;
+    }
+    if (S.mode == MODE_COMPUTE) return;
+    const double m = A.d[(size_t)D_M * n + i];
+    double v[3] = {A.d[(size_t)D_VX * n + i], A.d[(size_t)D_VY * n + i], A.d[(size_t)D_VZ * n + i]};
+    const double F[3] = {fx, fy, fz};
+    for (int c = 0; c < 3; ++c) {
+        double a = F[c] / m;
+        v[c] += a * 0.5 * S.dt;
+    }
+    if (S.mode == MODE_FINAL_DRIFT) {
+        double r[3] = {pi.x, pi.y, pi.z};
+        for (int c = 0; c < 3; ++c) {
+            double a = F[c] / m;
+            v[c] = v[c] + a * S.dt * 0.5;
+            r[c] += v[c] * S.dt;
+        }
+        pos_next[i] = make_double4(r[0], r[1], r[2], pi.w);
+        const double4 q = rsave[i];
+        double dx = r[0] - q.x, dy = r[1] - q.y, dz = r[2] - q.z;
+        shortest3(s, dx, dy, dz);
+        if (sq3_exact(dx, dy, dz) > s.trig2) atomicMin(&misc[M_ABORT], S.step + 1);
+        if (!(fabs(r[0]) < 1e300 && fabs(r[1]) < 1e300 && fabs(r[2]) < 1e300)) atomicOr(&misc[M_ERR], EB_NONFINITE);
+    }
+    A.d[(size_t)D_VX * n + i] = v[0];
+    A.d[(size_t)D_VY * n + i] = v[1];
+    A.d[(size_t)D_VZ * n + i] = v[2];
+}
+
+__device__ inline bool skip_step(const StepArgs& S, const int* misc) {
+    return S.mode != MODE_COMPUTE && *(volatile const int*)&misc[M_ABORT] <= S.step;
+}
+
+// --------------------------------------------------------------------------
+// LJsCut
+// --------------------------------------------------------------------------
+struct LJK {
+    double a, s6, lj1, lj2, factor, rc2;
+};
+
+template <typename R, int TPA>
+__global__ void __launch_bounds__(256) k_lj(const double4* __restrict__ pos, double4* __restrict__ pos_next,
+                                            const int* __restrict__ nl, const int* __restrict__ nn, int M, int cap,
+                                            const SysDesc* __restrict__ desc, LJK P, AtomArrays A, StepArgs S,
+                                            const double4* __restrict__ rsave, int* __restrict__ misc) {
+    if (skip_step(S, misc)) return;
+    const AtomSlots as = atom_slots<TPA>(M);
+    const int i = as.i;
+    const bool valid = i < A.n;
+    const double4 pi = valid ? pos[i] : make_double4(0, 0, 0, 0);
+    const bool active = valid && !w_inactive(pi.w);
+    const SysDesc& s = desc[valid ? w_sys(pi.w) : 0];
+    const int cnt = active ? min(nn[i], cap) : 0;
+    const R c_pref = (R)(-0.5 * P.factor), c_lj1 = (R)P.lj1, c_lj2 = (R)P.lj2, c_s6 = (R)P.s6, c_e = (R)(0.5 * 4 * P.a);
+    R fx = 0, fy = 0, fz = 0;
+    double e = 0, vxx = 0, vyy = 0, vzz = 0, vxy = 0, vxz = 0, vyz = 0;
+    for (int m = 0;; ++m) {
+        const int k = m * TPA + as.sub;
+        if (k >= cnt) break;
+        const int j = nl[as.base + (long long)m * kWave];
+        const double4 pj = pos[j];
+        double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z, d2;
+        if (!pair_image(s, dx, dy, dz, d2)) continue;
+        if (d2 > P.rc2 || d2 < 1e-15) continue;
+        const R r2 = (R)d2, rx = (R)dx, ry = (R)dy, rz = (R)dz;
+        const R r2inv = rcp_(r2);
+        const R r6inv = r2inv * r2inv * r2inv;
+        const R pref = c_pref * (c_lj1 * r6inv - c_lj2) * r6inv * r2inv;
+        fx += rx * pref;
+        fy += ry * pref;
+        fz += rz * pref;
+        if (S.flags & W_ENERGY) {
+            const R t = c_s6 * r6inv;
+            e += (double)(c_e * t * (t - (R)1));
+        }
+        if (S.flags & W_VIRIAL) {
+            const double px = (double)(pref * rx), py = (double)(pref * ry), pz = (double)(pref * rz);
+            vxx += px * (double)rx;
+            vyy += py * (double)ry;
+            vzz += pz * (double)rz;
+            vxy += px * (double)ry;
+            vxz += px * (double)rz;
+            vyz += py * (double)rz;
+        }
+    }
+    fx = wave_sum_group(fx, TPA);
+    fy = wave_sum_group(fy, TPA);
+    fz = wave_sum_group(fz, TPA);
+    if (S.flags & W_ENERGY) e = wave_sum_group(e, TPA);
+    if (S.flags & W_VIRIAL) {
+        vxx = wave_sum_group(vxx, TPA);
+        vyy = wave_sum_group(vyy, TPA);
+        vzz = wave_sum_group(vzz, TPA);
+        vxy = wave_sum_group(vxy, TPA);
+        vxz = wave_sum_group(vxz, TPA);
+        vyz = wave_sum_group(vyz, TPA);
+    }
+    if (!valid || as.sub != 0) return;
+    if (!active) {
+        if (S.mode == MODE_FINAL_DRIFT) pos_next[i] = pi;
+        return;
+    }
+    const int n = A.n;
+    if (S.flags & W_ENERGY) A.d[(size_t)D_EP * n + i] = e;
+    if (S.flags & W_VIRIAL) {
+        A.d[(size_t)D_VXX * n + i] = 0.5 * vxx;
+        A.d[(size_t)D_VYY * n + i] = 0.5 * vyy;
+        A.d[(size_t)D_VZZ * n + i] = 0.5 * vzz;
+        A.d[(size_t)D_VXY * n + i] = 0.5 * vxy;
+        A.d[(size_t)D_VXZ * n + i] = 0.5 * vxz;
+        A.d[(size_t)D_VYZ * n + i] = 0.5 * vyz;
+    }
+    finalize_atom<R>(i, pi, (double)fx, (double)fy, (double)fz, A, S, pos_next, rsave, s, misc);
+}
+
+// --------------------------------------------------------------------------
+// EAM (single element, or setfl-alloy with per-kind density/embedding and a
+// symmetric pair table)
+// --------------------------------------------------------------------------
+struct EamK {
+    int nk, nr, nrho;
+    double dr, dr1, drho, drho1, rc2;
+    const double4* r64;
+    const float4* r32;
+    const double4* F64;
+};
+
+template <typename R>
+__device__ inline const typename Vec4T<R>::T* rtab(const EamK& P);
+template <>
+__device__ inline const double4* rtab<double>(const EamK& P) { return P.r64; }
+template <>
+__device__ inline const float4* rtab<float>(const EamK& P) { return P.r32; }
+
+__device__ inline int pair_tab(int nk, int a, int b) {
+    int hi = a > b ? a : b, lo = a > b ? b : a;
+    return nk + hi * (hi + 1) / 2 + lo;
+}
+
+// EAM.cxx:562-589 embedding function and derivative (FP64); flags a density
+// outside the table, where the reference's branch is undefined.
+__device__ inline void embed(const EamK& P, int kind, double rho, double& F, double& dF, int* misc) {
+    double RN = rho * P.drho1;
+    int N1 = (int)trunc(RN);
+    if (!(RN >= 0.0 && N1 < P.nrho)) {
+        atomicOr(&misc[M_ERR], EB_RHO_RANGE);
+        F = dF = nan("");
+        return;
+    }
+    const double4 c = P.F64[(size_t)kind * P.nrho + N1];
+    double VR = N1;
+    double DV = RN - VR;
+    RN = rho - VR * P.drho;
+    VR = (c.z - c.x) * P.drho1;
+    double aa = (c.w - VR) - (VR - c.y);
+    double bb = aa - (VR - c.y);
+    aa = aa * DV;
+    F = c.x + RN * (c.y + DV * (aa - bb));
+    dF = c.y + 2.0 * DV * (1.5 * aa - bb);
+}
+
+template <typename R, int TPA>
+__global__ void __launch_bounds__(256) k_eam_rho(const double4* __restrict__ pos, const int* __restrict__ nl,
+                                                 const int* __restrict__ nn, int M, int cap,
+                                                 const SysDesc* __restrict__ desc, EamK P, AtomArrays A, StepArgs S,
+                                                 int* __restrict__ misc) {
+    if (skip_step(S, misc)) return;
+    const AtomSlots as = atom_slots<TPA>(M);
+    const int i = as.i;
+    const bool valid = i < A.n;
+    const double4 pi = valid ? pos[i] : make_double4(0, 0, 0, 0);
+    const bool active = valid && !w_inactive(pi.w);
+    const SysDesc& s = desc[valid ? w_sys(pi.w) : 0];
+    const int cnt = active ? min(nn[i], cap) : 0;
+    const auto* tab = rtab<R>(P);
+    const R dr = (R)P.dr, dr1 = (R)P.dr1;
+    double rho = 0.0;
+    for (int m = 0;; ++m) {
+        const int k = m * TPA + as.sub;
+        if (k >= cnt) break;
+        const int j = nl[as.base + (long long)m * kWave];
+        const double4 pj = pos[j];
+        double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z, d2;
+        if (!pair_image(s, dx, dy, dz, d2)) continue;
+        if (d2 > P.rc2 || d2 < 1e-15) continue;
+        const R R1 = sqrt((R)d2);
+        const R RN = R1 * dr1;
+        const int N1 = (int)RN;
+        if (N1 >= P.nr) continue;
+        const int kj = P.nk > 1 ? w_kind(pj.w) : 0;
+        rho += (double)spline_v<R>(tab[(size_t)kj * P.nr + N1], R1, RN, N1, dr, dr1);
+    }
+    rho = wave_sum_group(rho, TPA);
+    if (!valid || as.sub != 0 || !active) return;
+    const int n = A.n;
+    const int ki = P.nk > 1 ? w_kind(pi.w) : 0;
+    double F, dF;
+    embed(P, ki, rho, F, dF, misc);
+    A.d[(size_t)D_FP * n + i] = dF;
+    if (S.store_aux) {
+        A.d[(size_t)D_NC * n + i] = rho;
+        if (S.flags & W_ENERGY) A.d[(size_t)D_EM * n + i] = F;
+    }
+}
+
+template <typename R, int TPA>
+__global__ void __launch_bounds__(256) k_eam_force(const double4* __restrict__ pos, double4* __restrict__ pos_next,
+                                                   const int* __restrict__ nl, const int* __restrict__ nn, int M,
+                                                   int cap, const SysDesc* __restrict__ desc, EamK P, AtomArrays A,
+                                                   StepArgs S, const double4* __restrict__ rsave,
+                                                   int* __restrict__ misc) {
+    if (skip_step(S, misc)) return;
+    const AtomSlots as = atom_slots<TPA>(M);
+    const int i = as.i;
+    const bool valid = i < A.n;
+    const double4 pi = valid ? pos[i] : make_double4(0, 0, 0, 0);
+    const bool active = valid && !w_inactive(pi.w);
+    const SysDesc& s = desc[valid ? w_sys(pi.w) : 0];
+    const int cnt = active ? min(nn[i], cap) : 0;
+    const int n = A.n;
+    const double* __restrict__ Fp = A.d + (size_t)D_FP * n;
+    const double Fpi = active ? Fp[i] : 0.0;
+    const auto* tab = rtab<R>(P);
+    const R dr = (R)P.dr, dr1 = (R)P.dr1;
+    const int ki = P.nk > 1 ? w_kind(pi.w) : 0;
+    R fx = 0, fy = 0, fz = 0;
+    double e = 0, vxx = 0, vyy = 0, vzz = 0, vxy = 0, vxz = 0, vyz = 0;
+    for (int m = 0;; ++m) {
+        const int k = m * TPA + as.sub;
+        if (k >= cnt) break;
+        const int j = nl[as.base + (long long)m * kWave];
+        const double4 pj = pos[j];
+        double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z, d2;
+        if (!pair_image(s, dx, dy, dz, d2)) continue;
+        if (d2 > P.rc2 || d2 < 1e-15) continue;
+        const R R2 = (R)d2;
+        const R R1 = sqrt(R2);
+        const R RN = R1 * dr1;
+        const int N1 = (int)RN;
+        R FF = 0, VR = 0;
+        if (N1 < P.nr) {
+            const R Fpj = (R)Fp[j];
+            R nd, rv, phv, phd;
+            if (P.nk == 1) {
+                spline_vd<R>(tab[N1], R1, RN, N1, dr, dr1, rv, nd);
+                spline_vd<R>(tab[(size_t)P.nr + N1], R1, RN, N1, dr, dr1, phv, phd);
+                const R AdF = (R)Fpi + Fpj;
+                FF = (AdF * nd + (phd * R1 - phv) / R2) / R1;
+            } else {
+                const int kj = w_kind(pj.w);
+                R ndj, ndi;
+                spline_vd<R>(tab[(size_t)kj * P.nr + N1], R1, RN, N1, dr, dr1, rv, ndj);
+                spline_vd<R>(tab[(size_t)ki * P.nr + N1], R1, RN, N1, dr, dr1, rv, ndi);
+                spline_vd<R>(tab[(size_t)pair_tab(P.nk, ki, kj) * P.nr + N1], R1, RN, N1, dr, dr1, phv, phd);
+                FF = ((R)Fpi * ndj + Fpj * ndi + (phd * R1 - phv) / R2) / R1;
+            }
+            VR = phv;
+            if (S.flags & W_ENERGY) e += 0.5 * (double)(VR / R1);
+        }
+        const R rx = (R)dx, ry = (R)dy, rz = (R)dz;
+        fx += FF * rx;
+        fy += FF * ry;
+        fz += FF * rz;
+        if (S.flags & W_VIRIAL) {
+            const double h = 0.5 * (double)FF;
+            const double ax = h * dx, ay = h * dy, az = h * dz;
+            vxx += ax * dx;
+            vyy += ay * dy;
+            vzz += az * dz;
+            vxy += ax * dy;
+            vxz += ax * dz;
+            vyz += ay * dz;
+        }
+    }
+    fx = wave_sum_group(fx, TPA);
+    fy = wave_sum_group(fy, TPA);
+    fz = wave_sum_group(fz, TPA);
+    if (S.flags & W_ENERGY) e = wave_sum_group(e, TPA);
+    if (S.flags & W_VIRIAL) {
+        vxx = wave_sum_group(vxx, TPA);
+        vyy = wave_sum_group(vyy, TPA);
+        vzz = wave_sum_group(vzz, TPA);
+        vxy = wave_sum_group(vxy, TPA);
+        vxz = wave_sum_group(vxz, TPA);
+        vyz = wave_sum_group(vyz, TPA);
+    }
+    if (!valid || as.sub != 0) return;
+    if (!active) {
+        if (S.mode == MODE_FINAL_DRIFT) pos_next[i] = pi;
+        return;
+    }
+    if (S.flags & W_ENERGY) A.d[(size_t)D_EP * n + i] = e;
+    if (S.flags & W_VIRIAL) {
+        A.d[(size_t)D_VXX * n + i] = vxx;
+        A.d[(size_t)D_VYY * n + i] = vyy;
+        A.d[(size_t)D_VZZ * n + i] = vzz;
+        A.d[(size_t)D_VXY * n + i] = vxy;
+        A.d[(size_t)D_VXZ * n + i] = vxz;
+        A.d[(size_t)D_VYZ * n + i] = vyz;
+    }
+    finalize_atom<R>(i, pi, (double)fx, (double)fy, (double)fz, A, S, pos_next, rsave, s, misc);
+}
+
+// Opening half kick + drift of the first step of a sequence from the stored
+// force (verlet.cxx:71-78); the trigger marks step 0 as needing a rebuild.
+__global__ void k_predrift(const double4* __restrict__ pos, double4* __restrict__ pos_next, AtomArrays A, int f_prec,
+                           double dt, const SysDesc* __restrict__ desc, const double4* __restrict__ rsave,
+                           int* __restrict__ misc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = A.n;
+    if (i >= n) return;
+    const double4 pi = pos[i];
+    if (w_inactive(pi.w)) {
+        pos_next[i] = pi;
+        return;
+    }
+    double F[3];
+    if (f_prec == P_MIXED) {
+        F[0] = A.f[(size_t)F_FX * n + i];
+        F[1] = A.f[(size_t)F_FY * n + i];
+        F[2] = A.f[(size_t)F_FZ * n + i];
+    } else {
+        F[0] = A.d[(size_t)D_FX * n + i];
+        F[1] = A.d[(size_t)D_FY * n + i];
+        F[2] = A.d[(size_t)D_FZ * n + i];
+    }
+    const double m = A.d[(size_t)D_M * n + i];
+    double r[3] = {pi.x, pi.y, pi.z};
+    for (int c = 0; c < 3; ++c) {
+        double a = F[c] / m;
+        double v = A.d[(size_t)(D_VX + c) * n + i] + a * dt * 0.5;
+        r[c] += v * dt;
+        A.d[(size_t)(D_VX + c) * n + i] = v;
+    }
+    pos_next[i] = make_double4(r[0], r[1], r[2], pi.w);
+    const SysDesc& s = desc[w_sys(pi.w)];
+    const double4 q = rsave[i];
+    double dx = r[0] - q.x, dy = r[1] - q.y, dz = r[2] - q.z;
+    shortest3(s, dx, dy, dz);
+    if (sq3_exact(dx, dy, dz) > s.trig2) atomicMin(&misc[M_ABORT], 0);
+}
+
+// Per-system FP64 reduction in a fixed order (one workgroup per system):
+// out[s] = [Ep, Em, KE, Etot, W, com_x, com_y, com_z].
+__global__ void __launch_bounds__(256) k_thermo(const double* __restrict__ d, int n, const int* __restrict__ off,
+                                                double* __restrict__ out) {
+    const int sys = blockIdx.x;
+    const int b = off[sys], e = off[sys + 1];
+    __shared__ double red[8][256];
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // Ep, Em, W, px, py, pz, M
+    for (int i = b + threadIdx.x; i < e; i += 256) {
+        acc[0] += d[(size_t)D_EP * n + i];
+        acc[1] += d[(size_t)D_EM * n + i];
+        acc[2] += d[(size_t)D_VXX * n + i] + d[(size_t)D_VYY * n + i] + d[(size_t)D_VZZ * n + i];
+        const double m = d[(size_t)D_M * n + i];
+        acc[3] += m * d[(size_t)D_VX * n + i];
+        acc[4] += m * d[(size_t)D_VY * n + i];
+        acc[5] += m * d[(size_t)D_VZ * n + i];
+        acc[6] += m;
+    }
+    for (int k = 0; k < 7; ++k) red[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            for (int k = 0; k < 7; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + o];
+        __syncthreads();
+    }
+    const double M = red[6][0];
+    double com[3] = {0, 0, 0};
+    if (M > 0.0)
+        for (int c = 0; c < 3; ++c) com[c] = red[3 + c][0] / M;
+    __syncthreads();
+    double ke = 0.0;
+    for (int i = b + threadIdx.x; i < e; i += 256) {
+        const double m = d[(size_t)D_M * n + i];
+        double rx = d[(size_t)D_VX * n + i] - com[0], ry = d[(size_t)D_VY * n + i] - com[1],
+               rz = d[(size_t)D_VZ * n + i] - com[2];
+        ke += 0.5 * m * (rx * rx + ry * ry + rz * rz);
+    }
+    red[7][threadIdx.x] = ke;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[7][threadIdx.x] += red[7][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double* o = out + 8 * sys;
+        o[0] = red[0][0];
+        o[1] = red[1][0];
+        o[2] = red[7][0];
+        o[3] = red[0][0] + red[1][0] + red[7][0];
+        o[4] = red[2][0];
+        o[5] = com[0];
+        o[6] = com[1];
+        o[7] = com[2];
+    }
+}
+
+// --------------------------------------------------------------------------
+// host side
+// --------------------------------------------------------------------------
+static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
+
+hipEvent_t timer_begin(Batch& b) {
+    if (!b.timing) return nullptr;
+    hipEvent_t e;
+    if (!b.ev_free.empty()) {
+        e = b.ev_free.back();
+        b.ev_free.pop_back();
+    } else {
+        HIPCHECK(hipEventCreate(&e));
+    }
+    HIPCHECK(hipEventRecord(e, b.ctx->stream));
+    return e;
+}
+
+void timer_end(Batch& b, hipEvent_t a, int kind) {
+    if (!b.timing || !a) return;
+    hipEvent_t e;
+    if (!b.ev_free.empty()) {
+        e = b.ev_free.back();
+        b.ev_free.pop_back();
+    } else {
+        HIPCHECK(hipEventCreate(&e));
+    }
+    HIPCHECK(hipEventRecord(e, b.ctx->stream));
+    b.ev_pending.push_back({a, e, kind});
+}
+
+void timer_drain(Batch& b) {
+    for (auto& t : b.ev_pending) {
+        float ms = 0;
+        HIPCHECK(hipEventSynchronize(t.b));
+        HIPCHECK(hipEventElapsedTime(&ms, t.a, t.b));
+        b.kt_ms[t.kind] += ms;
+        b.kt_n[t.kind] += 1;
+        b.ev_free.push_back(t.a);
+        b.ev_free.push_back(t.b);
+    }
+    b.ev_pending.clear();
+}
+
+void set_trigger(Batch& b, double rcut) {
+    bool changed = false;
+    for (int s = 0; s < b.nsys; ++s) {
+        double skin = b.rns_min[s] - rcut;
+        double t = skin > 0.0 ? 0.25 * skin * skin : -1.0;
+        if (b.desc[s].trig2 != t) {
+            b.desc[s].trig2 = t;
+            changed = true;
+        }
+    }
+    if (changed)
+        HIPCHECK(hipMemcpyAsync(b.d_desc, b.desc.data(), sizeof(SysDesc) * b.nsys, hipMemcpyHostToDevice,
+                                b.ctx->stream));
+}
+
+static EamK eam_args(const Pot& p) {
+    EamK k;
+    k.nk = p.eam.nk;
+    k.nr = p.eam.nr;
+    k.nrho = p.eam.nrho;
+    k.dr = p.eam.dr;
+    k.dr1 = 1.0 / p.eam.dr;
+    k.drho = p.eam.drho;
+    k.drho1 = 1.0 / p.eam.drho;
+    k.rc2 = p.eam.rc * p.eam.rc;
+    k.r64 = p.eam.r64;
+    k.r32 = p.eam.r32;
+    k.F64 = p.eam.F64;
+    return k;
+}
+
+static LJK lj_args(const Pot& p) {
+    return LJK{p.lj.a, p.lj.s6, p.lj.lj1, p.lj.lj2, p.lj.factor, p.lj.rc2};
+}
+
+template <typename R, int TPA>
+static void launch_pair_t(Batch& b, const Pot& p, const StepArgs& S, const double4* pos, double4* pos_next) {
+    hipStream_t st = b.ctx->stream;
+    const int A = kWave / TPA;
+    const long long waves = (b.n + A - 1) / A;
+    const int grid = nblk(waves * kWave, 256);
+    AtomArrays AA{b.dst[b.sc], b.fst[b.sc], b.n};
+    if (p.type == PT_LJ) {
+        hipEvent_t t0 = timer_begin(b);
+        hipLaunchKernelGGL((k_lj<R, TPA>), dim3(grid), dim3(256), 0, st, pos, pos_next, b.nl, b.nn, b.M, b.cap,
+                           b.d_desc, lj_args(p), AA, S, b.rsave, b.misc);
+        timer_end(b, t0, K_FORCE);
+    } else {
+        EamK E = eam_args(p);
+        hipEvent_t t0 = timer_begin(b);
+        hipLaunchKernelGGL((k_eam_rho<R, TPA>), dim3(grid), dim3(256), 0, st, pos, b.nl, b.nn, b.M, b.cap, b.d_desc, E,
+                           AA, S, b.misc);
+        timer_end(b, t0, K_DENSITY);
+        hipEvent_t t1 = timer_begin(b);
+        hipLaunchKernelGGL((k_eam_force<R, TPA>), dim3(grid), dim3(256), 0, st, pos, pos_next, b.nl, b.nn, b.M, b.cap,
+                           b.d_desc, E, AA, S, b.rsave, b.misc);
+        timer_end(b, t1, K_FORCE);
+    }
+}
+
+template <typename R>
+static void launch_pair_r(Batch& b, const Pot& p, const StepArgs& S, const double4* pos, double4* pos_next) {
+    switch (b.tpa) {
+        case 1: launch_pair_t<R, 1>(b, p, S, pos, pos_next); break;
+        case 2: launch_pair_t<R, 2>(b, p, S, pos, pos_next); break;
+        case 4: launch_pair_t<R, 4>(b, p, S, pos, pos_next); break;
+        case 8: launch_pair_t<R, 8>(b, p, S, pos, pos_next); break;
+        case 16: launch_pair_t<R, 16>(b, p, S, pos, pos_next); break;
+        default: throw Error(E_INVAL, "lanes per atom must be 1, 2, 4, 8 or 16");
+    }
+}
+
+static void launch_step_kernels(Batch& b, const Pot& p, const StepArgs& S, int prec, const double4* pos,
+                                double4* pos_next) {
+    if (p.type == PT_BPNN) {
+        bpnn_compute(b, p, S.flags, prec, S.mode, S.dt, S.step);
+        return;
+    }
+    if (prec == P_MIXED) launch_pair_r<float>(b, p, S, pos, pos_next);
+    else launch_pair_r<double>(b, p, S, pos, pos_next);
+}
+
+void check_errors(Batch& b) {
+    int e = b.ctx->h_pin[M_ERR];
+    if (!e) return;
+    HIPCHECK(hipMemsetAsync(b.misc + M_ERR, 0, sizeof(int), b.ctx->stream));
+    if (e & EB_RHO_RANGE)
+        throw Error(E_RANGE, "EAM density outside the embedding table (the reference's branch for it is undefined)");
+    if (e & EB_NONFINITE) throw Error(E_RANGE, "non-finite position after integration");
+    throw Error(E_RANGE, "device error flag set");
+}
+
+static void read_misc(Batch& b) {
+    HIPCHECK(hipMemcpyAsync(b.ctx->h_pin, b.misc, sizeof(int) * M_COUNT, hipMemcpyDeviceToHost, b.ctx->stream));
+    HIPCHECK(hipStreamSynchronize(b.ctx->stream));
+    timer_drain(b);
+}
+
+void force_compute(Batch& b, const Pot& p, int what, int prec) {
+    if (b.n == 0) return;
+    set_trigger(b, p.rcut);
+    if (!b.list_valid) nlist_rebuild(b);
+    StepArgs S{0.0, -1, MODE_COMPUTE, what & (W_ENERGY | W_VIRIAL), 1};
+    // the reference's prepare zeroes Ep, Em and V and only virials() fills
+    // them (solver/single.cxx:38-53)
+    if (p.type != PT_BPNN && !(what & W_ENERGY)) {
+        HIPCHECK(hipMemsetAsync(b.d(b.sc, D_EP), 0, sizeof(double) * b.n, b.ctx->stream));
+        HIPCHECK(hipMemsetAsync(b.d(b.sc, D_EM), 0, sizeof(double) * b.n, b.ctx->stream));
+    }
+    if (!(what & W_VIRIAL)) HIPCHECK(hipMemsetAsync(b.d(b.sc, D_VXX), 0, sizeof(double) * 6 * b.n, b.ctx->stream));
+    launch_step_kernels(b, p, S, prec, b.pos[b.cur], nullptr);
+    b.f_prec = prec;
+    b.last_rcut = p.rcut;
+    read_misc(b);
+    check_errors(b);
+}
+
+void run_steps(Batch& b, const Pot& p, double dt, int nsteps, int prec, int chunk) {
+    if (b.n == 0 || nsteps <= 0) return;
+    hipStream_t st = b.ctx->stream;
+    set_trigger(b, p.rcut);
+    if (!b.list_valid) nlist_rebuild(b);
+    if (b.f_prec < 0) throw Error(E_STATE, "no force to start from: compute forces once before stepping");
+    const int grid = nblk(b.n, 256);
+    int abort_init = INT_MAX;
+    HIPCHECK(hipMemcpyAsync(b.misc + M_ABORT, &abort_init, sizeof(int), hipMemcpyHostToDevice, st));
+    int c0 = b.cur;
+    {
+        AtomArrays AA{b.dst[b.sc], b.fst[b.sc], b.n};
+        hipEvent_t t0 = timer_begin(b);
+        hipLaunchKernelGGL(k_predrift, dim3(grid), dim3(256), 0, st, b.pos[c0], b.pos[c0 ^ 1], AA, b.f_prec, dt,
+                           b.d_desc, b.rsave, b.misc);
+        timer_end(b, t0, K_PREDRIFT);
+    }
+    auto buf = [&](int t) { return c0 ^ 1 ^ (t & 1); };
+    int s = 0;
+    while (s < nsteps) {
+        const int e = std::min(nsteps, s + chunk);
+        for (int t = s; t < e; ++t) {
+            const bool last = t == nsteps - 1;
+            StepArgs S{dt, t, last ? MODE_FINAL : MODE_FINAL_DRIFT, 0, last ? 1 : 0};
+            launch_step_kernels(b, p, S, prec, b.pos[buf(t)], last ? nullptr : b.pos[buf(t) ^ 1]);
+        }
+        read_misc(b);
+        check_errors(b);
+        const int ab = b.ctx->h_pin[M_ABORT];
+        if (ab < e) {
+            // steps [s, ab) ran; positions of step ab wait in buf(ab)
+            b.cur = buf(ab);
+            nlist_rebuild(b);
+            c0 = b.cur ^ 1 ^ (ab & 1);
+            HIPCHECK(hipMemcpyAsync(b.misc + M_ABORT, &abort_init, sizeof(int), hipMemcpyHostToDevice, st));
+            s = ab;
+        } else {
+            s = e;
+        }
+    }
+    b.cur = buf(nsteps - 1);
+    // after make_step the reference holds Ep = Em = 0 and V = 0 (prepare
+    // zeroes them; only virials() fills them); BPNN sets Ep in prepare
+    if (p.type != PT_BPNN) {
+        HIPCHECK(hipMemsetAsync(b.d(b.sc, D_EP), 0, sizeof(double) * 2 * b.n, st));
+    }
+    HIPCHECK(hipMemsetAsync(b.d(b.sc, D_VXX), 0, sizeof(double) * 6 * b.n, st));
+    b.f_prec = (p.type == PT_BPNN) ? P_MIXED : prec;
+    b.last_rcut = p.rcut;
+}
+
+void thermo(Batch& b, double* out) {
+    hipStream_t st = b.ctx->stream;
+    int* d_off = reinterpret_cast<int*>(b.thermo + 8 * b.nsys);
+    HIPCHECK(hipMemcpyAsync(d_off, b.atom_off.data(), sizeof(int) * (b.nsys + 1), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_thermo, dim3(b.nsys), dim3(256), 0, st, b.dst[b.sc], b.n, d_off, b.thermo);
+    HIPCHECK(hipMemcpyAsync(out, b.thermo, sizeof(double) * 8 * b.nsys, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+}
+
+}  // namespace mdc

@@ -1,0 +1,258 @@
+// Neighbour maintenance on the device: wrap + linked-cell binning as an
+// atomic-count / prefix-scan counting sort, a deterministic within-cell
+// order, a physical reorder of every per-atom array into cell order, and the
+// Verlet-list build in the wavefront-packed layout.
+//
+// Replaces neibs::VerletList::update / build_linked_cells / are_neibs and
+// decide (mdcraft/neibs/verlet-list.cxx:41-55, :110-158, :306-400) and the
+// grid of mdcraft/neibs/grid.cxx.  The pair set is the reference's: all j
+// (self excluded here, added back on export) with
+// |d|^2 <= max(rns_i, rns_j)^2 where d = r_i - r_j gets one shift of the box
+// length on a periodic axis when 2|d| exceeds it, summed in FP64 without
+// contraction.  The reference applies that shift only for atoms in edge
+// cells; for three or more cells a shift from an interior cell can never
+// produce a pair and for one or two cells every cell is an edge, so the sets
+// are the same.
+#include "engine.h"
+
+namespace mdc {
+
+// lattice::Domain::fit_in_period (domain.cxx:41-47) and the cell of an atom
+// in its system's grid (cells of edge >= max rns spanning the box).
+__global__ void k_wrap_count(double4* __restrict__ pos, const SysDesc* __restrict__ desc, int* __restrict__ atom_cell,
+                             int* __restrict__ cell_count, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double4 p = pos[i];
+    const SysDesc& s = desc[w_sys(p.w)];
+    double r[3] = {p.x, p.y, p.z};
+    int c[3];
+    for (int a = 0; a < 3; ++a) {
+        if (s.per[a]) r[a] = r[a] < s.lo[a] ? r[a] + s.L[a] : r[a] > s.hi[a] ? r[a] - s.L[a] : r[a];
+        double q = floor((r[a] - s.lo[a]) / s.csize[a]);
+        int ci = q < 0.0 ? 0 : (q >= (double)s.nc[a] ? s.nc[a] - 1 : (int)q);
+        c[a] = ci;
+    }
+    p.x = r[0];
+    p.y = r[1];
+    p.z = r[2];
+    pos[i] = p;
+    int cid = s.cell_off + c[0] + s.nc[0] * (c[1] + s.nc[1] * c[2]);
+    atom_cell[i] = cid;
+    atomicAdd(&cell_count[cid], 1);
+}
+
+// Exclusive scan of the per-cell counts with one workgroup; also clears the
+// fill counters.  start has ncells + 1 entries.
+__global__ void __launch_bounds__(1024) k_scan_cells(const int* __restrict__ count, int* __restrict__ start,
+                                                     int* __restrict__ fill, int ncells) {
+    __shared__ int warp_tot[16];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int base = 0; base < ncells; base += 1024) {
+        int idx = base + threadIdx.x;
+        int v = idx < ncells ? count[idx] : 0;
+        if (idx < ncells) fill[idx] = 0;
+        int x = v;
+        for (int o = 1; o < 64; o <<= 1) {
+            int y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) warp_tot[wv] = x;
+        __syncthreads();
+        if (wv == 0) {
+            int t = lane < 16 ? warp_tot[lane] : 0;
+            for (int o = 1; o < 16; o <<= 1) {
+                int y = __shfl_up(t, o, 64);
+                if (lane >= o) t += y;
+            }
+            if (lane < 16) warp_tot[lane] = t;
+        }
+        __syncthreads();
+        int before = carry + (wv > 0 ? warp_tot[wv - 1] : 0) + x - v;
+        if (idx < ncells) start[idx] = before;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry = before + v;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) start[ncells] = carry;
+}
+
+__global__ void k_scatter(const int* __restrict__ atom_cell, const int* __restrict__ start, int* __restrict__ fill,
+                          int* __restrict__ src, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int c = atom_cell[i];
+    int slot = start[c] + atomicAdd(&fill[c], 1);
+    src[slot] = i;
+}
+
+// Deterministic order inside each cell: ascending previous index.
+__global__ void k_cell_sort(const int* __restrict__ start, int* __restrict__ src, int ncells) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    int b = start[c], e = start[c + 1];
+    for (int k = b + 1; k < e; ++k) {
+        int v = src[k];
+        int q = k - 1;
+        while (q >= b && src[q] > v) {
+            src[q + 1] = src[q];
+            --q;
+        }
+        src[q + 1] = v;
+    }
+}
+
+// Move every per-atom array into the new (cell) order; record the cell of
+// each sorted atom and the displacement reference positions.
+__global__ void k_permute(const int* __restrict__ src, const double4* __restrict__ pos_in, double4* __restrict__ pos_out,
+                          const double* __restrict__ din, double* __restrict__ dout, const float* __restrict__ fin,
+                          float* __restrict__ fout, const int* __restrict__ oin, int* __restrict__ oout,
+                          const int* __restrict__ cell_in, int* __restrict__ cell_out, double4* __restrict__ rsave,
+                          int* __restrict__ where, int n) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    int s = src[p];
+    double4 q = pos_in[s];
+    pos_out[p] = q;
+    rsave[p] = q;
+    for (int f = 0; f < D_NFIELDS; ++f) dout[(size_t)f * n + p] = din[(size_t)f * n + s];
+    for (int f = 0; f < F_NFIELDS; ++f) fout[(size_t)f * n + p] = fin[(size_t)f * n + s];
+    int o = oin[s];
+    oout[p] = o;
+    where[o] = p;
+    cell_out[p] = cell_in[s];
+}
+
+// One thread per atom: 27-cell stencil (with the reference's de-duplication
+// for one or two periodic cells, grid.cxx:150-166), FP64 test of
+// verlet-list.cxx:374-400, slots written in stencil order into the packed
+// layout.  Counts beyond the capacity are still counted so the host can grow
+// the list.
+__global__ void k_build(const double4* __restrict__ pos, const double* __restrict__ rns, const int* __restrict__ cell_of,
+                        const int* __restrict__ start, const SysDesc* __restrict__ desc, int* __restrict__ nl,
+                        int* __restrict__ nn, int* __restrict__ misc, int n, int tpa, int M, int cap) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double4 pi = pos[i];
+    const SysDesc& s = desc[w_sys(pi.w)];
+    const double ri = rns[i];
+    const int lc = cell_of[i] - s.cell_off;
+    const int nx = s.nc[0], ny = s.nc[1], nz = s.nc[2];
+    const int cx = lc % nx, cy = (lc / nx) % ny, cz = lc / (nx * ny);
+    int cnt = 0;
+    for (int ox = -1; ox <= 1; ++ox) {
+        if (s.per[0] && ((nx == 1 && ox != 0) || (nx == 2 && ox == 1))) continue;
+        int x = cx + ox;
+        if (s.per[0]) x = (x + nx) % nx;
+        else if (x < 0 || x >= nx) continue;
+        for (int oy = -1; oy <= 1; ++oy) {
+            if (s.per[1] && ((ny == 1 && oy != 0) || (ny == 2 && oy == 1))) continue;
+            int y = cy + oy;
+            if (s.per[1]) y = (y + ny) % ny;
+            else if (y < 0 || y >= ny) continue;
+            for (int oz = -1; oz <= 1; ++oz) {
+                if (s.per[2] && ((nz == 1 && oz != 0) || (nz == 2 && oz == 1))) continue;
+                int z = cz + oz;
+                if (s.per[2]) z = (z + nz) % nz;
+                else if (z < 0 || z >= nz) continue;
+                const int c = s.cell_off + x + nx * (y + ny * z);
+                const int b = start[c], e = start[c + 1];
+                for (int j = b; j < e; ++j) {
+                    if (j == i) continue;
+                    const double4 pj = pos[j];
+                    double rr = fmax(ri, rns[j]);
+                    double d2c = rr * rr;
+                    double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                    if (s.per[0] && (2 * dx < -s.L[0])) dx += s.L[0];
+                    if (s.per[0] && (2 * dx > s.L[0])) dx -= s.L[0];
+                    if (s.per[1] && (2 * dy < -s.L[1])) dy += s.L[1];
+                    if (s.per[1] && (2 * dy > s.L[1])) dy -= s.L[1];
+                    if (s.per[2] && (2 * dz < -s.L[2])) dz += s.L[2];
+                    if (s.per[2] && (2 * dz > s.L[2])) dz -= s.L[2];
+                    if (sq3_exact(dx, dy, dz) <= d2c) {
+                        if (cnt < cap) nl[list_index(i, cnt, tpa, M)] = j;
+                        ++cnt;
+                    }
+                }
+            }
+        }
+    }
+    nn[i] = cnt;
+    atomicMax(&misc[M_MAXNN], cnt);
+}
+
+// verlet-list.cxx:110-158 decide(): |shortest(r - r_saved)|^2 > (skin/2)^2
+__global__ void k_decide(const double4* __restrict__ pos, const double4* __restrict__ rsave,
+                         const SysDesc* __restrict__ desc, int* __restrict__ misc, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double4 p = pos[i], q = rsave[i];
+    const SysDesc& s = desc[w_sys(p.w)];
+    double dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+    shortest3(s, dx, dy, dz);
+    if (sq3_exact(dx, dy, dz) > s.trig2) atomicOr(&misc[M_DECIDE], 1);
+}
+
+static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
+
+void nlist_rebuild(Batch& b) {
+    hipStream_t st = b.ctx->stream;
+    const int n = b.n;
+    if (n == 0) {
+        b.list_valid = true;
+        return;
+    }
+    HIPCHECK(hipMemsetAsync(b.cell_count, 0, sizeof(int) * (b.ncells + 1), st));
+    hipLaunchKernelGGL(k_wrap_count, dim3(nblk(n, 256)), dim3(256), 0, st, b.pos[b.cur], b.d_desc, b.atom_cell,
+                       b.cell_count, n);
+    hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, st, b.cell_count, b.cell_start, b.cell_fill, b.ncells);
+    hipLaunchKernelGGL(k_scatter, dim3(nblk(n, 256)), dim3(256), 0, st, b.atom_cell, b.cell_start, b.cell_fill,
+                       b.sorted_src, n);
+    hipLaunchKernelGGL(k_cell_sort, dim3(nblk(b.ncells, 256)), dim3(256), 0, st, b.cell_start, b.sorted_src, b.ncells);
+    const int s0 = b.sc, s1 = b.sc ^ 1;
+    // cell_fill is free after the scatter: reuse it for the sorted cell ids
+    hipLaunchKernelGGL(k_permute, dim3(nblk(n, 256)), dim3(256), 0, st, b.sorted_src, b.pos[b.cur], b.pos[b.cur ^ 1],
+                       b.dst[s0], b.dst[s1], b.fst[s0], b.fst[s1], b.orig[s0], b.orig[s1], b.atom_cell, b.cell_fill,
+                       b.rsave, b.where, n);
+    b.cur ^= 1;
+    b.sc = s1;
+    for (int attempt = 0; attempt < 3; ++attempt) {
+        HIPCHECK(hipMemsetAsync(b.misc + M_MAXNN, 0, sizeof(int), st));
+        hipLaunchKernelGGL(k_build, dim3(nblk(n, 128)), dim3(128), 0, st, b.pos[b.cur], b.d(b.sc, D_RNS), b.cell_fill,
+                           b.cell_start, b.d_desc, b.nl, b.nn, b.misc, n, b.tpa, b.M, b.cap);
+        HIPCHECK(hipMemcpyAsync(b.ctx->h_pin, b.misc, sizeof(int) * M_COUNT, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        int maxnn = b.ctx->h_pin[M_MAXNN];
+        if (maxnn <= b.cap) break;
+        // grow the list and rebuild it
+        int cap = ((maxnn + maxnn / 4 + 8 + b.tpa - 1) / b.tpa) * b.tpa;
+        int A = kWave / b.tpa;
+        size_t waves = (size_t)(n + A - 1) / A;
+        size_t elems = waves * (size_t)(cap / b.tpa) * kWave;
+        if (b.nl) HIPCHECK(hipFree(b.nl));
+        b.nl = nullptr;
+        if (hipMalloc(&b.nl, elems * sizeof(int)) != hipSuccess)
+            throw Error(E_NOMEM, "out of device memory growing the neighbour list");
+        b.cap = cap;
+        b.M = cap / b.tpa;
+        b.nl_elems = elems;
+    }
+    b.list_valid = true;
+    b.rebuilds++;
+}
+
+int nlist_decide(Batch& b, double rcut) {
+    hipStream_t st = b.ctx->stream;
+    if (!b.list_valid) return 1;
+    set_trigger(b, rcut);
+    HIPCHECK(hipMemsetAsync(b.misc + M_DECIDE, 0, sizeof(int), st));
+    hipLaunchKernelGGL(k_decide, dim3(nblk(b.n, 256)), dim3(256), 0, st, b.pos[b.cur], b.rsave, b.d_desc, b.misc, b.n);
+    HIPCHECK(hipMemcpyAsync(b.ctx->h_pin, b.misc, sizeof(int) * M_COUNT, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    return b.ctx->h_pin[M_DECIDE] != 0;
+}
+
+}  // namespace mdc

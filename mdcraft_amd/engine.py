@@ -1,0 +1,272 @@
+"""Python host layer over the engine's C ABI (include/mdc.h, libmdc.so).
+
+The classes mirror the reference's names and argument meanings where the
+path crosses into the engine: ``LJsCut(a, s, rcut)`` and ``EAM(...)``
+(python/mdcraft/solver/potential/_potential.cxx), ``MaiseBPNNPotential``,
+a ``VerletList``-style update/decide/export on a resident ``System``, and
+``Verlet``-style stepping.  Errors raise ``RuntimeError`` with the engine's
+message, as the reference's pybind11 layer turns C++ exceptions into
+``RuntimeError``.
+
+The HIP library is the only compute path: importing this module fails when
+``libmdc.so`` is missing, and there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .atoms import ATOM_DTYPE
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmdc.so")
+
+FP64, MIXED = 0, 1
+FORCE, ENERGY, VIRIAL = 1, 2, 4
+F_R, F_V, F_F, F_VIR, F_M, F_RCUT, F_RNS, F_EP, F_EM, F_NC, F_BC, F_KIND, F_UID, F_TAG = (1 << k for k in range(14))
+F_ALL = (1 << 14) - 1
+
+EXPORTS = [
+    "mdc_ctx_create", "mdc_ctx_destroy", "mdc_ctx_last_error", "mdc_last_error", "mdc_atom_record_size",
+    "mdc_pot_lj_create", "mdc_pot_eam_create", "mdc_pot_eam_create_file", "mdc_pot_eam_alloy_create",
+    "mdc_pot_bpnn_create", "mdc_pot_rcut", "mdc_pot_destroy",
+    "mdc_sys_create", "mdc_sys_destroy", "mdc_sys_upload", "mdc_sys_download", "mdc_sys_natoms",
+    "mdc_sys_set_lanes", "mdc_nlist_build", "mdc_nlist_decide", "mdc_nlist_export", "mdc_nlist_stats",
+    "mdc_compute", "mdc_step", "mdc_thermo", "mdc_sync", "mdc_timing",
+]
+
+
+def load_library(path=LIB_PATH):
+    if not os.path.exists(path):
+        raise ImportError(
+            "libmdc.so is not built (run __graft_entry__.build() or make -C mdcraft_amd/csrc); "
+            "the engine has no CPU fallback")
+    lib = C.CDLL(path)
+    P, I, D, U32, U64 = C.c_void_p, C.c_int, C.c_double, C.c_uint32, C.c_uint64
+    PP = C.POINTER(C.c_void_p)
+    DP = C.POINTER(C.c_double)
+    sig = {
+        "mdc_ctx_create": (I, [I, PP]),
+        "mdc_ctx_destroy": (I, [P]),
+        "mdc_ctx_last_error": (C.c_char_p, [P]),
+        "mdc_last_error": (C.c_char_p, []),
+        "mdc_atom_record_size": (I, []),
+        "mdc_pot_lj_create": (I, [P, D, D, D, PP]),
+        "mdc_pot_eam_create": (I, [P, D, DP, DP, DP, I, D, I, D, PP]),
+        "mdc_pot_eam_create_file": (I, [P, C.c_char_p, PP]),
+        "mdc_pot_eam_alloy_create": (I, [P, I, D, DP, DP, DP, I, D, I, D, PP]),
+        "mdc_pot_bpnn_create": (I, [P, C.c_char_p, PP]),
+        "mdc_pot_rcut": (I, [P, DP]),
+        "mdc_pot_destroy": (I, [P]),
+        "mdc_sys_create": (I, [P, I, C.POINTER(U64), DP, DP, C.POINTER(C.c_uint8), I, PP]),
+        "mdc_sys_destroy": (I, [P]),
+        "mdc_sys_upload": (I, [P, P, U32]),
+        "mdc_sys_download": (I, [P, P, U32]),
+        "mdc_sys_natoms": (I, [P, C.POINTER(U64)]),
+        "mdc_sys_set_lanes": (I, [P, I]),
+        "mdc_nlist_build": (I, [P]),
+        "mdc_nlist_decide": (I, [P, D, C.POINTER(I)]),
+        "mdc_nlist_export": (I, [P, C.POINTER(U64), C.POINTER(U64), C.POINTER(U64)]),
+        "mdc_nlist_stats": (I, [P, DP, C.POINTER(I), C.POINTER(I), C.POINTER(I)]),
+        "mdc_compute": (I, [P, P, U32, I]),
+        "mdc_step": (I, [P, P, D, I, I, C.POINTER(I)]),
+        "mdc_thermo": (I, [P, DP]),
+        "mdc_sync": (I, [P]),
+        "mdc_timing": (I, [P, I, DP, C.POINTER(C.c_int64)]),
+    }
+    for name in EXPORTS:
+        fn = getattr(lib, name)
+        fn.restype, fn.argtypes = sig[name]
+    if lib.mdc_atom_record_size() != ATOM_DTYPE.itemsize:
+        raise ImportError("atom record size mismatch")
+    return lib
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = load_library()
+    return _lib
+
+
+def _check(status):
+    if status != 0:
+        raise RuntimeError(lib().mdc_last_error().decode() or f"mdc error {status}")
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Context:
+    """One device and one ordered HIP stream."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        _check(lib().mdc_ctx_create(device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mdc_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Potential:
+    def __init__(self, ctx, h, kind):
+        self.ctx, self.h, self.kind = ctx, h, kind
+
+    @property
+    def rcut(self):
+        r = C.c_double()
+        _check(lib().mdc_pot_rcut(self.h, C.byref(r)))
+        return r.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mdc_pot_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def LJsCut(ctx, a, s, rcut):
+    """Truncated LJ without energy shift, LJsCut(a, s, rcut)."""
+    h = C.c_void_p()
+    _check(lib().mdc_pot_lj_create(ctx.h, a, s, rcut, C.byref(h)))
+    return Potential(ctx, h, "pair")
+
+
+def EAM(ctx, rcut=None, F=None, rho=None, rphi=None, dr=None, drho=None, path=None):
+    """Tabulated EAM: EAM(Rcutoff, Femb, Rho, Vpar, Nr1, dr, Nrh1, drh) or
+    EAM(setfl filename)."""
+    h = C.c_void_p()
+    if path is not None:
+        _check(lib().mdc_pot_eam_create_file(ctx.h, path.encode(), C.byref(h)))
+    else:
+        F, rho, rphi = (np.ascontiguousarray(x, dtype=np.float64) for x in (F, rho, rphi))
+        _check(lib().mdc_pot_eam_create(ctx.h, rcut, _dp(F), _dp(rho), _dp(rphi), len(rho), dr, len(F), drho,
+                                        C.byref(h)))
+    return Potential(ctx, h, "eam")
+
+
+def EAMAlloy(ctx, rcut, F, rho, rphi, dr, drho):
+    """Alloy EAM: F [nk, nrho], rho [nk, nr], rphi [nk(nk+1)/2, nr] (pair
+    index a(a+1)/2 + b for kinds a >= b)."""
+    F, rho, rphi = (np.ascontiguousarray(x, dtype=np.float64) for x in (F, rho, rphi))
+    nk = F.shape[0]
+    h = C.c_void_p()
+    _check(lib().mdc_pot_eam_alloy_create(ctx.h, nk, rcut, _dp(F), _dp(rho), _dp(rphi), rho.shape[1], dr,
+                                          F.shape[1], drho, C.byref(h)))
+    return Potential(ctx, h, "eam")
+
+
+def MaiseBPNNPotential(ctx, path):
+    h = C.c_void_p()
+    _check(lib().mdc_pot_bpnn_create(ctx.h, path.encode(), C.byref(h)))
+    return Potential(ctx, h, "ternary")
+
+
+class System:
+    """One or more independent boxes resident in HBM.
+
+    ``boxes`` is a list of (lo[3], hi[3], periodic[3]); ``natoms`` the atom
+    count of each box; atoms of box s are the records
+    [sum(natoms[:s]), sum(natoms[:s+1])) of every atoms array passed in.
+    ``periodic_boundary`` attaches the reference's boundary::Periodic ghost
+    pass to the force evaluation.
+    """
+
+    def __init__(self, ctx, natoms, boxes, periodic_boundary=True, lanes=None):
+        self.ctx = ctx
+        natoms = np.asarray(natoms, dtype=np.uint64).reshape(-1)
+        ns = len(natoms)
+        lo = np.zeros((ns, 3))
+        hi = np.zeros((ns, 3))
+        per = np.zeros((ns, 3), dtype=np.uint8)
+        for s, (l, h, p) in enumerate(boxes):
+            lo[s], hi[s], per[s] = l, h, [1 if x else 0 for x in p]
+        self.natoms = natoms
+        self.nsys = ns
+        self.n = int(natoms.sum())
+        h = C.c_void_p()
+        _check(lib().mdc_sys_create(ctx.h, ns, natoms.ctypes.data_as(C.POINTER(C.c_uint64)), _dp(lo), _dp(hi),
+                                    per.ctypes.data_as(C.POINTER(C.c_uint8)), int(periodic_boundary), C.byref(h)))
+        self.h = h
+        if lanes is not None:
+            self.set_lanes(lanes)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mdc_sys_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_lanes(self, lanes):
+        _check(lib().mdc_sys_set_lanes(self.h, lanes))
+
+    def upload(self, atoms, fields=F_ALL):
+        atoms = np.ascontiguousarray(atoms, dtype=ATOM_DTYPE)
+        assert len(atoms) == self.n
+        _check(lib().mdc_sys_upload(self.h, atoms.ctypes.data, fields))
+
+    def download(self, atoms=None, fields=F_ALL):
+        if atoms is None:
+            atoms = np.zeros(self.n, dtype=ATOM_DTYPE)
+        assert atoms.dtype == ATOM_DTYPE and atoms.flags.c_contiguous and len(atoms) == self.n
+        _check(lib().mdc_sys_download(self.h, atoms.ctypes.data, fields))
+        return atoms
+
+    def build(self):
+        _check(lib().mdc_nlist_build(self.h))
+
+    def decide(self, rcut):
+        need = C.c_int()
+        _check(lib().mdc_nlist_decide(self.h, rcut, C.byref(need)))
+        return bool(need.value)
+
+    def export(self):
+        tot = C.c_uint64()
+        _check(lib().mdc_nlist_export(self.h, None, None, C.byref(tot)))
+        off = np.zeros(self.n + 1, dtype=np.uint64)
+        ids = np.zeros(max(tot.value, 1), dtype=np.uint64)
+        _check(lib().mdc_nlist_export(self.h, off.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                      ids.ctypes.data_as(C.POINTER(C.c_uint64)), None))
+        return off.astype(np.int64), ids[:tot.value].astype(np.int64)
+
+    def stats(self):
+        mean, mx, cap, reb = C.c_double(), C.c_int(), C.c_int(), C.c_int()
+        _check(lib().mdc_nlist_stats(self.h, C.byref(mean), C.byref(mx), C.byref(cap), C.byref(reb)))
+        return {"mean": mean.value, "max": mx.value, "capacity": cap.value, "rebuilds": reb.value}
+
+    def compute(self, pot, what=FORCE | ENERGY | VIRIAL, precision=FP64):
+        _check(lib().mdc_compute(self.h, pot.h, what, precision))
+
+    def step(self, pot, dt, nsteps=1, precision=FP64):
+        reb = C.c_int()
+        _check(lib().mdc_step(self.h, pot.h, dt, nsteps, precision, C.byref(reb)))
+        return reb.value
+
+    def thermo(self):
+        """Per system: [Ep, Em, KE, Etot, W, com_x, com_y, com_z]."""
+        out = np.zeros((self.nsys, 8))
+        _check(lib().mdc_thermo(self.h, _dp(out)))
+        return out
+
+    def sync(self):
+        _check(lib().mdc_sync(self.h))
+
+    def timing(self, enable=-1):
+        ms = np.zeros(8)
+        n = np.zeros(8, dtype=np.int64)
+        _check(lib().mdc_timing(self.h, enable, _dp(ms), n.ctypes.data_as(C.POINTER(C.c_int64))))
+        return ms, n

@@ -1,0 +1,328 @@
+"""Seeded synthetic workloads in the reference's units (nm, ps, amu, kJ/mol).
+
+Random numbers come from a counter-based SplitMix64 stream with Box-Muller
+normals, so every structure is reproducible from (base seed, workload id,
+index) on any platform, independent of libstdc++ or numpy generators.
+
+Workloads (sizes as benchmarked):
+  lj_argon      fcc Ar a = 5.26 A, 10^3 cells = 4000 atoms, LJ eps = 0.0104 eV,
+                sigma = 3.405 A, rc = 2.5 sigma, skin 0.3 A, 100 K, dt = 2 fs
+  eam_cu        fcc Cu a = 3.615 A, 20^3 cells = 32000 atoms, rattle 0.03 A,
+                300 K, analytic Morse/exponential/sqrt EAM tabulated on
+                5000-point grids to 5.0 A, dt = 1 fs
+  bpnn_cu       fcc Cu 15^3 cells = 13500 atoms, rattle 0.05 A, G2/G4 + a
+                16-30-30-1 tanh network with Xavier weights, dt = 1 fs
+  nial_batch    512 B2 NiAl cells (a = 2.88 A), atom counts log-uniform in
+                [32, 4096], normal strain up to 2 %, rattle 0.05 A, alloy EAM
+"""
+import math
+import os
+
+import numpy as np
+
+from .atoms import EV, KB, make_atoms, set_vec
+
+BASE_SEED = 20261203
+ANG = 0.1  # nm per Angstrom
+FS = 1e-3  # ps per fs
+_GAMMA = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def _mix(z):
+    z = (z ^ (z >> np.uint64(30))) * _M1
+    z = (z ^ (z >> np.uint64(27))) * _M2
+    return z ^ (z >> np.uint64(31))
+
+
+def splitmix64(x):
+    """One SplitMix64 output for state x (Python int)."""
+    with np.errstate(over="ignore"):
+        return int(_mix(np.uint64((int(x) + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF)))
+
+
+class Rng:
+    """Counter-based SplitMix64 stream."""
+
+    def __init__(self, seed):
+        self.state = np.uint64(int(seed) & 0xFFFFFFFFFFFFFFFF)
+        self.counter = 0
+
+    def bits(self, n):
+        with np.errstate(over="ignore"):
+            k = np.arange(self.counter + 1, self.counter + 1 + n, dtype=np.uint64)
+            z = self.state + k * _GAMMA
+            self.counter += n
+            return _mix(z)
+
+    def uniform(self, n):
+        """Uniform doubles in (0, 1)."""
+        return ((self.bits(n) >> np.uint64(11)).astype(np.float64) + 0.5) * (1.0 / 9007199254740992.0)
+
+    def normal(self, n):
+        m = (n + 1) // 2
+        u1 = self.uniform(m)
+        u2 = self.uniform(m)
+        r = np.sqrt(-2.0 * np.log(u1))
+        z = np.concatenate([r * np.cos(2 * np.pi * u2), r * np.sin(2 * np.pi * u2)])
+        return z[:n]
+
+
+def structure_seed(workload_id, index, base=BASE_SEED):
+    return splitmix64(base ^ (workload_id * 0x100000001B3) ^ (index * 0x9E3779B1))
+
+
+# --------------------------------------------------------------------------
+# lattices
+# --------------------------------------------------------------------------
+FCC = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+B2 = np.array([[0, 0, 0], [0.5, 0.5, 0.5]])
+
+
+def lattice(basis, reps, a):
+    """Positions (nm) of a replicated cubic cell with origin at 0; returns
+    positions and the basis index of every site."""
+    nx, ny, nz = reps
+    idx = np.stack(np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij"), -1).reshape(-1, 3)
+    # z slowest, then y, then x, as md_test_utils.h make_fcc_bulk orders sites
+    idx = idx[np.lexsort((idx[:, 0], idx[:, 1], idx[:, 2]))]
+    pos = (idx[:, None, :] + basis[None, :, :]).reshape(-1, 3) * a
+    site = np.tile(np.arange(len(basis)), len(idx))
+    return pos, site
+
+
+def maxwell_boltzmann(rng, mass, T):
+    """Velocities (nm/ps) at temperature T with the net momentum removed."""
+    sig = np.sqrt(KB * T / mass)
+    v = rng.normal(3 * len(mass)).reshape(3, -1).T * sig[:, None]
+    p = (mass[:, None] * v).sum(0)
+    return v - p / mass.sum()
+
+
+def _box(L, periodic=(True, True, True)):
+    return (np.zeros(3), np.asarray(L, dtype=np.float64), tuple(periodic))
+
+
+# --------------------------------------------------------------------------
+# analytic EAM tables (Cu-like), reference units
+# --------------------------------------------------------------------------
+EAM_RC = 5.0  # Angstrom
+
+
+def taper(r, rc=EAM_RC, h=0.5):
+    """Smooth cutoff x^4/(1+x^4), x = (r - rc)/h for r < rc, zero beyond."""
+    x = (np.asarray(r, dtype=np.float64) - rc) / h
+    s = x ** 4 / (1.0 + x ** 4)
+    return np.where(r < rc, s, 0.0)
+
+
+def morse(r, D0=0.2, alpha=1.6, r0=2.55):
+    e = np.exp(-alpha * (r - r0))
+    return D0 * (e * e - 2.0 * e)
+
+
+def density(r):
+    return np.exp(-3.0 * (r / 2.556 - 1.0))
+
+
+def eam_cu_tables(nr=5000, nrho=5000, rho_max=50.0, scale_rho=1.0, scale_F=1.0):
+    """Tables on 5000-point grids: r step 5.01/4999 A so (nr-1)*dr exceeds
+    the 5.0 A cutoff; rho to 50.  Returns dict in reference units:
+    F (kJ/mol), rho (-), rphi (kJ nm/mol), dr (nm), drho, rc (nm)."""
+    dr_a = 5.01 / (nr - 1)
+    r = np.arange(nr) * dr_a
+    t = taper(r)
+    phi_ev = morse(r) * t
+    rho = density(r) * t * scale_rho
+    drho = rho_max / (nrho - 1)
+    rh = np.arange(nrho) * drho
+    F_ev = -1.2 * np.sqrt(rh) * scale_F
+    return {
+        "F": F_ev * EV,
+        "rho": rho,
+        "rphi": (r * ANG) * phi_ev * EV,
+        "dr": dr_a * ANG,
+        "drho": drho,
+        "rc": EAM_RC * ANG,
+    }
+
+
+def eam_analytic(r_nm, rhobar):
+    """Analytic phi (kJ/mol), rho, F (kJ/mol) of the tabulated functions."""
+    r = np.asarray(r_nm) / ANG
+    return morse(r) * taper(r) * EV, density(r) * taper(r), -1.2 * np.sqrt(rhobar) * EV
+
+
+# --------------------------------------------------------------------------
+# workloads
+# --------------------------------------------------------------------------
+def lj_argon(reps=10, seed_index=0, T=100.0):
+    a = 5.26 * ANG
+    pos, _ = lattice(FCC, (reps,) * 3, a)
+    n = len(pos)
+    rng = Rng(structure_seed(1, seed_index))
+    at = make_atoms(n)
+    m = np.full(n, 39.948)
+    set_vec(at, "r", pos)
+    set_vec(at, "v", maxwell_boltzmann(rng, m, T))
+    sigma = 3.405 * ANG
+    rc = 2.5 * sigma
+    at["m"] = m
+    at["rcut"] = rc
+    at["rns"] = rc + 0.3 * ANG
+    at["uid"] = np.arange(1, n + 1)
+    pot = {"kind": "lj", "a": 0.0104 * EV, "s": sigma, "rc": rc}
+    return {"atoms": at, "box": _box([reps * a] * 3), "pot": pot, "dt": 2 * FS, "steps": 1000, "name": "lj_argon"}
+
+
+def eam_cu(reps=20, seed_index=0, T=300.0, rattle=0.03):
+    a = 3.615 * ANG
+    pos, _ = lattice(FCC, (reps,) * 3, a)
+    n = len(pos)
+    rng = Rng(structure_seed(2, seed_index))
+    L = reps * a
+    pos = np.mod(pos + rng.normal(3 * n).reshape(n, 3) * rattle * ANG, L)
+    at = make_atoms(n)
+    m = np.full(n, 63.546)
+    set_vec(at, "r", pos)
+    set_vec(at, "v", maxwell_boltzmann(rng, m, T))
+    tab = eam_cu_tables()
+    at["m"] = m
+    at["rcut"] = tab["rc"]
+    at["rns"] = tab["rc"] + 0.3 * ANG
+    at["uid"] = np.arange(1, n + 1)
+    pot = dict(kind="eam", **tab)
+    return {"atoms": at, "box": _box([L] * 3), "pot": pot, "dt": 1 * FS, "steps": 500, "name": "eam_cu"}
+
+
+def write_maise_model(path, seed_index=0, hidden=(30, 30)):
+    """MAISE-format model file for the BPNN workload: 8 G2 (eta log-spaced
+    0.003-0.5 A^-2, Rs = 0) and 8 G4 (eta in {0.003, 0.03}, zeta in {1, 4},
+    lambda = +-1), Rc = 6 A, input range 1, Xavier-uniform weights and zero
+    biases.  Layout follows the reference parser
+    (mdcraft/tools/parser/maise_model.cxx:29-129)."""
+    rng = Rng(structure_seed(3, 1000 + seed_index))
+    nin = 16
+    arch = [nin, hidden[0], hidden[1], 1]
+    weights = []
+    for l in range(3):
+        weights.append(np.zeros(arch[l + 1]))
+    mats = []
+    for l in range(3):
+        lim = math.sqrt(6.0 / (arch[l] + arch[l + 1]))
+        mats.append((rng.uniform(arch[l] * arch[l + 1]) * 2 - 1) * lim)
+    flat = np.concatenate(weights + mats)
+    bar = "-" * 77
+
+    def head(label, value):
+        return "|  " + label.ljust(21) + "|  " + str(value).ljust(49) + "|"
+
+    lines = [bar, "|" + "neural network general information".center(75) + "|", bar,
+             head("model unique ID", "0000BEEF"), head("number of species", 1), head("species types", 29),
+             head("species names", "Cu"), head("number of layers", 4), head("architecture", " ".join(map(str, arch))),
+             head("number of weights", len(flat)), head("reference", "synthetic"), bar,
+             "|                          input component range                            |", bar,
+             "        minimum                    maximum                     range"]
+    for _ in range(nin):
+        lines.append("   0.0000000000000000      100.0000000000000000         1.0000000000000000")
+    lines += [bar, "|                        neural network weights                             |", bar]
+    lines += ["  %.16f" % w for w in flat]
+    lines += [bar, "  B2A          Scaling", "  1.0  1.0", "", "  N  GN  Rc  Rs   n1  n2  k   z   l", ""]
+    row = 1
+    for i in range(8):
+        lines.append("%3d   2   0   0   %d   0   0   0   0" % (row, i))
+        row += 1
+    for ie in range(2):
+        for iz in range(2):
+            for il in range(2):
+                lines.append("%3d   4   0   0   0   %d   0   %d   %d" % (row, ie, iz, il))
+                row += 1
+    lines.append("")
+    eta2 = np.exp(np.linspace(np.log(0.003), np.log(0.5), 8))
+    lines += [" Rc   1   6.0", " Rs   1   0.0", " n1   8   " + "   ".join("%.10f" % e for e in eta2),
+              " n2   2   0.0030000000   0.0300000000", " k    1   1.0000", " z    2   1.0000   4.0000",
+              " l    2  -1.0000   1.0000"]
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    return path
+
+
+def bpnn_cu(reps=15, seed_index=0, T=300.0, rattle=0.05, model_dir=None):
+    a = 3.615 * ANG
+    pos, _ = lattice(FCC, (reps,) * 3, a)
+    n = len(pos)
+    rng = Rng(structure_seed(3, seed_index))
+    L = reps * a
+    pos = np.mod(pos + rng.normal(3 * n).reshape(n, 3) * rattle * ANG, L)
+    at = make_atoms(n)
+    m = np.full(n, 63.546)
+    set_vec(at, "r", pos)
+    set_vec(at, "v", maxwell_boltzmann(rng, m, T))
+    at["m"] = m
+    at["rcut"] = 0.6
+    at["rns"] = 0.6 + 0.3 * ANG
+    at["kind"] = 0
+    at["uid"] = np.arange(1, n + 1)
+    model_dir = model_dir or os.path.join(os.environ.get("TMPDIR", "/tmp"), "mdc_models")
+    os.makedirs(model_dir, exist_ok=True)
+    path = write_maise_model(os.path.join(model_dir, "bpnn_cu_%d.maise" % seed_index))
+    pot = {"kind": "bpnn", "path": path, "rc": 0.6}
+    return {"atoms": at, "box": _box([L] * 3), "pot": pot, "dt": 1 * FS, "steps": 200, "name": "bpnn_cu"}
+
+
+def nial_cell(index, target=None):
+    """One strained, rattled B2 NiAl cell; kind 0 = Ni, 1 = Al."""
+    rng = Rng(structure_seed(4, index))
+    if target is None:
+        target = math.exp(math.log(32) + rng.uniform(1)[0] * (math.log(4096) - math.log(32)))
+    cells = max(1, int(round(target / 2)))
+    best = None
+    for nx in range(1, cells + 1):
+        for ny in range(nx, cells + 1):
+            if nx * ny > cells:
+                break
+            nz = max(ny, int(round(cells / (nx * ny))))
+            c = nx * ny * nz
+            score = (abs(c - cells), nz - nx)
+            if best is None or score < best[0]:
+                best = (score, (nx, ny, nz))
+    reps = best[1]
+    a = 2.88 * ANG
+    pos, site = lattice(B2, reps, a)
+    n = len(pos)
+    strain = 1.0 + (rng.uniform(3) * 2 - 1) * 0.02
+    L = np.array(reps) * a * strain
+    pos = pos * strain
+    pos = np.mod(pos + rng.normal(3 * n).reshape(n, 3) * 0.05 * ANG, L)
+    at = make_atoms(n)
+    set_vec(at, "r", pos)
+    at["kind"] = site
+    m = np.where(site == 0, 58.6934, 26.9815385)
+    set_vec(at, "v", maxwell_boltzmann(rng, m, 300.0))
+    at["m"] = m
+    at["rcut"] = EAM_RC * ANG
+    at["rns"] = EAM_RC * ANG + 0.3 * ANG
+    at["uid"] = np.arange(1, n + 1)
+    return at, _box(L)
+
+
+def nial_tables():
+    """Alloy tables: Ni (kind 0) density/embedding x1.1, Al (kind 1) x0.9,
+    one Morse pair for Ni-Ni and Al-Al and their mean for Ni-Al."""
+    ni = eam_cu_tables(scale_rho=1.1, scale_F=1.1)
+    al = eam_cu_tables(scale_rho=0.9, scale_F=0.9)
+    rphi = ni["rphi"]
+    return {
+        "F": np.stack([ni["F"], al["F"]]),
+        "rho": np.stack([ni["rho"], al["rho"]]),
+        "rphi": np.stack([rphi, 0.5 * (rphi + al["rphi"]), al["rphi"]]),
+        "dr": ni["dr"],
+        "drho": ni["drho"],
+        "rc": ni["rc"],
+    }
+
+
+def nial_batch(count=512):
+    cells = [nial_cell(i) for i in range(count)]
+    return {"cells": cells, "pot": dict(kind="eam_alloy", **nial_tables()), "dt": 1 * FS, "name": "nial_batch"}

@@ -1,0 +1,79 @@
+"""Shared helpers: run the same seeded input through the oracle
+(oracle/librefcpu.so, the checker) and through the engine (libmdc.so)."""
+import os
+
+import numpy as np
+
+from mdcraft_amd.atoms import vec, mat, trace_V
+
+
+def oracle_mod():
+    from oracle import refcpu
+    return refcpu
+
+
+def oracle_potential(R, pot, box=None):
+    k = pot["kind"]
+    if k == "lj":
+        return R.LJsCut(pot["a"], pot["s"], pot["rc"])
+    if k == "eam":
+        if "path" in pot:
+            return R.EAM(path=pot["path"])
+        return R.EAM(rc=pot["rc"], F=pot["F"], rho=pot["rho"], rphi=pot["rphi"], dr=pot["dr"], drho=pot["drho"])
+    if k == "bpnn":
+        return R.MaiseBPNN(pot["path"], R.Domain(*box))
+    raise ValueError(k)
+
+
+def engine_potential(E, ctx, pot):
+    k = pot["kind"]
+    if k == "lj":
+        return E.LJsCut(ctx, pot["a"], pot["s"], pot["rc"])
+    if k == "eam":
+        if "path" in pot:
+            return E.EAM(ctx, path=pot["path"])
+        return E.EAM(ctx, rcut=pot["rc"], F=pot["F"], rho=pot["rho"], rphi=pot["rphi"], dr=pot["dr"],
+                     drho=pot["drho"])
+    if k == "eam_alloy":
+        return E.EAMAlloy(ctx, pot["rc"], pot["F"], pot["rho"], pot["rphi"], pot["dr"], pot["drho"])
+    if k == "bpnn":
+        return E.MaiseBPNNPotential(ctx, pot["path"])
+    raise ValueError(k)
+
+
+def oracle_compute(atoms, box, pot, periodic_boundary=True, threads=1):
+    """prepare + stages + virials on the oracle; returns (atoms, lists)."""
+    R = oracle_mod()
+    a = atoms.copy()
+    dom = R.Domain(*box)
+    pool = R.Pool(threads)
+    p = oracle_potential(R, pot, box)
+    a["rcut"] = p.rcut
+    nl = R.VerletList(pool)
+    nl.update(a, dom)
+    s = R.Solver(p, pool, periodic_boundary)
+    s.compute(a, dom, nl)
+    return a, nl.export(len(a))
+
+
+def pair_set(off, ids):
+    n = len(off) - 1
+    i = np.repeat(np.arange(n), np.diff(off))
+    return set(zip(i.tolist(), ids.tolist()))
+
+
+def force_rms_rel(f, fref):
+    return float(np.sqrt(np.mean(np.sum((f - fref) ** 2, 1))) / np.sqrt(np.mean(np.sum(fref ** 2, 1))))
+
+
+def max_rel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    scale = max(np.max(np.abs(b)), 1e-300)
+    return float(np.max(np.abs(a - b)) / scale)
+
+
+REFERENCE = "/root/reference"
+
+
+def have_reference():
+    return os.path.isdir(REFERENCE)
