@@ -117,6 +117,10 @@ mdc_status mdc_thermo(mdc_sys* sys, double* out);
 mdc_status mdc_sync(mdc_sys* sys);
 
 /* ---- instrumentation ---------------------------------------------------- */
+/* BPNN descriptors G and dE/dG (kJ/mol per descriptor unit) of the last
+ * evaluation, [n][nin] in original atom order (the per-atom g and dedg of
+ * maise_bpnn.h:40-47); G or dEdG may be NULL; *nin receives the width */
+mdc_status mdc_bpnn_inspect(mdc_sys* sys, const mdc_pot* pot, double* G, double* dEdG, int* nin);
 /* kernel timing with hipEvents on the launch stream: kind 0 pair force,
  * 1 EAM density, 2 pre-drift, 3 BPNN descriptors, 4 BPNN MLP, 5 BPNN force,
  * 6 BPNN gather, 7 rebuild */

@@ -158,7 +158,18 @@ Pot* make_eam(Ctx* ctx, int nk, double rcut, const double* F, const double* rho,
     }
     for (double x : rec64)
         if (!std::isfinite(x)) throw Error(E_RANGE, "non-finite value in the EAM r tables");
-    std::vector<float> rec32(rec64.begin(), rec64.end());
+    // FP32 records: the interval cubic as coefficients in t = r/dr - N1
+    std::vector<float> rec32(rec64.size());
+    for (size_t q = 0; q < rec64.size(); q += 4) {
+        const double y0 = rec64[q], z0 = rec64[q + 1], y1 = rec64[q + 2], z1 = rec64[q + 3];
+        const double sl = (y1 - y0) / dr;
+        const double aa = (z1 - sl) - (sl - z0);
+        const double bb = aa - (sl - z0);
+        rec32[q] = (float)y0;
+        rec32[q + 1] = (float)(dr * z0);
+        rec32[q + 2] = (float)(-dr * bb);
+        rec32[q + 3] = (float)(dr * aa);
+    }
     const size_t nrec = rec64.size() / 4;
     T.r64 = dalloc<double4>(nrec);
     T.r32 = dalloc<float4>(nrec);
@@ -268,8 +279,7 @@ void free_batch(Batch& b) {
     dfree(b.misc);
     dfree(b.d_desc);
     dfree(b.thermo);
-    if (b.bp_scratch) (void)hipFree(b.bp_scratch);
-    b.bp_scratch = nullptr;
+    bpnn_release_work(b);
     for (auto& t : b.ev_pending) {
         b.ev_free.push_back(t.a);
         b.ev_free.push_back(t.b);
@@ -854,6 +864,14 @@ mdc_status mdc_sync(mdc_sys* sys) {
     return guard(sys ? sys->b.ctx : nullptr, [&] {
         if (!sys) throw Error(E_INVAL, "null argument");
         HIPCHECK(hipStreamSynchronize(sys->b.ctx->stream));
+    });
+}
+
+mdc_status mdc_bpnn_inspect(mdc_sys* sys, const mdc_pot* pot, double* G, double* dEdG, int* nin) {
+    return guard(sys ? sys->b.ctx : nullptr, [&] {
+        if (!sys || !pot || pot->p.type != PT_BPNN) throw Error(E_INVAL, "needs a system and a BPNN potential");
+        if (nin) *nin = bpnn_inputs(pot->p);
+        if (G || dEdG) bpnn_inspect(sys->b, pot->p, G, dEdG);
     });
 }
 

@@ -1,7 +1,917 @@
-// placeholder: BPNN kernels are added in the next change
-#include "engine.h"
+// Behler-Parrinello network (MAISE model format) on the device.
+//
+// Reference semantics (maise_bpnn.cxx:122-325, tools/bpnn/base.h:95-190,
+// base.cxx:116-128, maise_ann.cxx:8-55, parser/maise_model.cxx:29-129):
+//   G2_r(i)   = sum_j exp(-eta_r (r_ij - Rs)^2) fc(r_ij)
+//   G4_a(i)   = sum_{j<k} 2 * 2^(1-zeta) (1 + lambda cos_ijk)^zeta
+//                 exp(-eta (r_ij^2 + r_ik^2 + r_jk^2)) fc(r_ij) fc(r_ik) fc(r_jk)
+//   fc(r)     = (cos(pi r / Rc) + 1) / 2, pairs and the opposite edge < Rc,
+//               minimum image from Domain::shortest
+//   E_i       = W23 tanh(W12 tanh(W01 (G o range) + b01) + b12) + b23, in eV,
+//               scaled by 10 eV Na to kJ/mol
+//   f_m       = -sum_i dE_i/dG_i . dG_i/dr_m
+//
+// Device formulation: the reference gathers, for every atom, the derivatives
+// of all neighbours' descriptors (about 1.5 n^2 triplets per atom).  Here
+// each centre i is one wavefront that stages its in-range neighbours in LDS,
+// and
+//   pass 1 (k_bp_desc)   sums G_i over pairs j<k (n^2/2 triplets);
+//   pass 2 (k_bp_mlp)    runs the per-element MLP forward and backward for
+//                        dE_i/dG_i on MFMA (FP16 inputs split into hi + lo
+//                        halves, three products per tile, FP32 accumulation),
+//                        or in FP64 on the vector units in FP64 mode;
+//   pass 3 (k_bp_grad)   has lane j sum d(w_i . G_i)/dr_j over all k (ordered
+//                        pairs, no atomics), giving the force that centre i
+//                        puts on each neighbour slot, and the centre's own
+//                        share by translation invariance;
+//   pass 4 (k_bp_gather) adds, for every atom, its own share and the shares
+//                        its neighbours put on it (through a reverse-slot
+//                        index built at list rebuilds), then runs the fused
+//                        velocity-Verlet update.
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <functional>
+#include <sstream>
+#include <string>
+#include <vector>
+#include "kern.h"
+
 namespace mdc {
-void bpnn_compute(Batch&, const Pot&, int, int, int, double, int) { throw Error(E_STATE, "BPNN not built yet"); }
-Pot* bpnn_create(Ctx*, const char*) { throw Error(E_STATE, "BPNN not built yet"); }
-void bpnn_destroy(Pot* p) { delete p; }
+
+constexpr int kBpMaxR = 16, kBpMaxA = 16, kBpMaxEta = 4, kBpMaxHidden = 64, kBpMaxIn = 32;
+
+struct BpDesc {
+    int nr, na, nin, h1, h2, ne;
+    double Rc, Rc2, Rs, pi_rc, escale;
+    double eta_r[kBpMaxR];
+    double eta_e[kBpMaxEta];
+    int a_e[kBpMaxA];
+    double a_coef[kBpMaxA], a_lam[kBpMaxA], a_zeta[kBpMaxA];
+    int a_iz[kBpMaxA];
+};
+
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+struct BpnnParams {
+    BpDesc h{};
+    BpDesc* d = nullptr;
+    double* w64 = nullptr;  // W01 [h1][nin] (range-scaled), b01, W12 [h2][h1], b12, W23 [h2], b23
+    float* w32 = nullptr;   // b01, b12, W23, b23 as float for the MFMA epilogues
+    half4_t* bsplit = nullptr;
+    int boff[4] = {0, 0, 0, 0};
+    int nt[4] = {0, 0, 0, 0}, kt[4] = {0, 0, 0, 0};
+    int nin_pad = 0, h1_pad = 0, h2_pad = 0;
+};
+
+// per-batch BPNN work arrays (in Batch::bp_scratch)
+struct BpWork {
+    size_t bytes = 0;
+    int cap = 0, n = 0, prec = -1;
+    void* G = nullptr;
+    void* w = nullptr;
+    void* P = nullptr;
+    void* own = nullptr;
+    int* rev = nullptr;
+    bool rev_valid = false;
+    int rev_builds = -1;
+};
+
+// --------------------------------------------------------------------------
+// device helpers
+// --------------------------------------------------------------------------
+__device__ inline float dcos_(float x) { return __cosf(x); }
+__device__ inline double dcos_(double x) { return cos(x); }
+__device__ inline float dsin_(float x) { return __sinf(x); }
+__device__ inline double dsin_(double x) { return sin(x); }
+__device__ inline float dexp_(float x) { return __expf(x); }
+__device__ inline double dexp_(double x) { return exp(x); }
+__device__ inline float dsqrt_(float x) { return sqrtf(x); }
+__device__ inline double dsqrt_(double x) { return sqrt(x); }
+
+template <typename R>
+__device__ inline R powi_(R b, int iz, R z) {
+    if (iz >= 0) {
+        R p = (R)1;
+        for (int q = 0; q < iz; ++q) p *= b;
+        return p;
+    }
+    return (R)pow((double)b, (double)z);
 }
+
+template <typename R>
+__device__ inline R wave_sum(R v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+template <typename R>
+struct Stage {
+    R *x, *y, *z, *d, *fc, *dfc, *E;  // E: [ne][cap]
+    int* slot;
+    int cap;
+};
+
+template <typename R>
+__device__ inline Stage<R> carve(char* base, int cap, int ne) {
+    Stage<R> S;
+    R* p = reinterpret_cast<R*>(base);
+    S.x = p;
+    S.y = p + cap;
+    S.z = p + 2 * cap;
+    S.d = p + 3 * cap;
+    S.fc = p + 4 * cap;
+    S.dfc = p + 5 * cap;
+    S.E = p + 6 * cap;
+    S.slot = reinterpret_cast<int*>(p + (6 + ne) * cap);
+    S.cap = cap;
+    return S;
+}
+
+__host__ __device__ inline size_t stage_bytes(int cap, int ne, int rsize) {
+    return (size_t)cap * ((6 + ne) * rsize + 4);
+}
+
+// Stage the in-range neighbours of centre i (1e-15 <= r^2 < Rc^2 under
+// Domain::shortest) into LDS, compacted in list order; out-of-range slots
+// get zero force shares in P when P is given.
+template <typename R>
+__device__ int bp_stage(int i, const double4& pi, const SysDesc& s, const int* __restrict__ nl, int cnt, int tpa,
+                        int M, const double4* __restrict__ pos, const BpDesc& P, const Stage<R>& S,
+                        R* __restrict__ Pzero, int cap) {
+    const int lane = threadIdx.x & 63;
+    int base = 0;
+    for (int k0 = 0; k0 < cnt; k0 += kWave) {
+        const int k = k0 + lane;
+        bool ok = false;
+        double dx = 0, dy = 0, dz = 0, d2 = 0;
+        if (k < cnt) {
+            const int j = nl[list_index(i, k, tpa, M)];
+            const double4 pj = pos[j];
+            dx = pj.x - pi.x;
+            dy = pj.y - pi.y;
+            dz = pj.z - pi.z;
+            shortest3(s, dx, dy, dz);
+            d2 = sq3_exact(dx, dy, dz);
+            ok = d2 >= 1e-15 && d2 < P.Rc2;
+            if (!ok && Pzero) {
+                R* q = Pzero + ((size_t)i * cap + k) * 3;
+                q[0] = q[1] = q[2] = (R)0;
+            }
+        }
+        const unsigned long long m = __ballot(ok);
+        if (ok) {
+            const int at = base + __popcll(m & ((1ull << lane) - 1ull));
+            const double d = sqrt(d2);
+            S.x[at] = (R)dx;
+            S.y[at] = (R)dy;
+            S.z[at] = (R)dz;
+            S.d[at] = (R)d;
+            S.fc[at] = (R)(0.5 * (cos(P.pi_rc * d) + 1.0));
+            S.dfc[at] = (R)(-0.5 * sin(P.pi_rc * d) * P.pi_rc);
+            for (int e = 0; e < P.ne; ++e) S.E[e * S.cap + at] = (R)exp(-P.eta_e[e] * d2);
+            S.slot[at] = k;
+        }
+        base += __popcll(m);
+    }
+    // LDS traffic of one wavefront is processed in order; the barrier only
+    // stops the compiler from moving the readers above the staging stores
+    __builtin_amdgcn_wave_barrier();
+    return base;
+}
+
+// pass 1: descriptors
+template <typename R>
+__global__ void __launch_bounds__(256) k_bp_desc(const double4* __restrict__ pos, const int* __restrict__ nl,
+                                                 const int* __restrict__ nn, int tpa, int M, int cap,
+                                                 const SysDesc* __restrict__ desc, const BpDesc* __restrict__ Pd,
+                                                 R* __restrict__ G, int ldg, int n, StepArgs SA,
+                                                 int* __restrict__ misc) {
+    if (skip_step(SA, misc)) return;
+    extern __shared__ char smem[];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = blockIdx.x * (blockDim.x >> 6) + wv;
+    if (i >= n) return;
+    const BpDesc& P = *Pd;
+    const Stage<R> S = carve<R>(smem + wv * stage_bytes(cap, P.ne, sizeof(R)), cap, P.ne);
+    const double4 pi = pos[i];
+    const SysDesc& s = desc[w_sys(pi.w)];
+    const int cnt = min(nn[i], cap);
+    const int nc = bp_stage<R>(i, pi, s, nl, cnt, tpa, M, pos, P, S, (R*)nullptr, cap);
+    R acc[kBpMaxR + kBpMaxA];
+#pragma unroll
+    for (int q = 0; q < kBpMaxR + kBpMaxA; ++q) acc[q] = (R)0;
+    const R Rs = (R)P.Rs;
+    for (int a = lane; a < nc; a += kWave) {
+        const R d = S.d[a], fc = S.fc[a];
+#pragma unroll
+        for (int r = 0; r < kBpMaxR; ++r)
+            if (r < P.nr) acc[r] += dexp_(-(R)P.eta_r[r] * (d - Rs) * (d - Rs)) * fc;
+    }
+    // pairs a < b over the staged list, flattened and strided over lanes
+    const R Rc2 = (R)P.Rc2, pirc = (R)P.pi_rc;
+    int a = 0, t = lane;
+    while (a < nc - 1 && t >= nc - 1 - a) {
+        t -= nc - 1 - a;
+        ++a;
+    }
+    while (a < nc - 1) {
+        const int b = a + 1 + t;
+        const R cx = S.x[b] - S.x[a], cy = S.y[b] - S.y[a], cz = S.z[b] - S.z[a];
+        const R d2c = cx * cx + cy * cy + cz * cz;
+        if (d2c >= (R)1e-15 && d2c < Rc2) {
+            const R dc = dsqrt_(d2c);
+            const R fcc = (R)0.5 * (dcos_(pirc * dc) + (R)1);
+            const R cs = (S.x[a] * S.x[b] + S.y[a] * S.y[b] + S.z[a] * S.z[b]) / (S.d[a] * S.d[b]);
+            const R F = S.fc[a] * S.fc[b] * fcc;
+            R Ex[kBpMaxEta];
+#pragma unroll
+            for (int e = 0; e < kBpMaxEta; ++e)
+                if (e < P.ne) Ex[e] = S.E[e * cap + a] * S.E[e * cap + b] * dexp_(-(R)P.eta_e[e] * d2c);
+#pragma unroll
+            for (int q = 0; q < kBpMaxA; ++q)
+                if (q < P.na) {
+                    const R base = (R)1 + (R)P.a_lam[q] * cs;
+                    acc[kBpMaxR + q] += (R)P.a_coef[q] * powi_(base, P.a_iz[q], (R)P.a_zeta[q]) * Ex[P.a_e[q]] * F;
+                }
+        }
+        t += kWave;
+        while (a < nc - 1 && t >= nc - 1 - a) {
+            t -= nc - 1 - a;
+            ++a;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kBpMaxR + kBpMaxA; ++q) acc[q] = wave_sum(acc[q]);
+    if (lane == 0) {
+        R* g = G + (size_t)i * ldg;
+        for (int r = 0; r < P.nr; ++r) g[r] = acc[r];
+        for (int q = 0; q < P.na; ++q) g[P.nr + q] = acc[kBpMaxR + q];
+        for (int q = P.nr + P.na; q < ldg; ++q) g[q] = (R)0;
+    }
+}
+
+// pass 2 (FP64): one thread per atom, vector units
+__global__ void __launch_bounds__(128) k_bp_mlp64(const double* __restrict__ G, int ldg, const double* __restrict__ W,
+                                                  const BpDesc* __restrict__ Pd, const double4* __restrict__ pos,
+                                                  double* __restrict__ wout, double* __restrict__ Ep, int n,
+                                                  StepArgs SA, int* __restrict__ misc) {
+    if (skip_step(SA, misc)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const BpDesc& P = *Pd;
+    const int nin = P.nin, h1 = P.h1, h2 = P.h2;
+    const double* W01 = W;
+    const double* b01 = W01 + h1 * nin;
+    const double* W12 = b01 + h1;
+    const double* b12 = W12 + h2 * h1;
+    const double* W23 = b12 + h2;
+    const double b23 = W23[h2];
+    double* wo = wout + (size_t)i * ldg;
+    if (w_inactive(pos[i].w)) {
+        for (int q = 0; q < ldg; ++q) wo[q] = 0.0;
+        return;
+    }
+    const double* g = G + (size_t)i * ldg;
+    double y1[kBpMaxHidden], y2[kBpMaxHidden], d1[kBpMaxHidden];
+    for (int q = 0; q < h1; ++q) {
+        double sacc = 0.0;
+        for (int k = 0; k < nin; ++k) sacc += W01[q * nin + k] * g[k];
+        y1[q] = tanh(sacc + b01[q]);
+    }
+    double E = b23;
+    for (int q = 0; q < h2; ++q) {
+        double sacc = 0.0;
+        for (int p = 0; p < h1; ++p) sacc += W12[q * h1 + p] * y1[p];
+        y2[q] = tanh(sacc + b12[q]);
+        E += W23[q] * y2[q];
+    }
+    for (int q = 0; q < h2; ++q) y2[q] = W23[q] * (1.0 - y2[q] * y2[q]);
+    for (int p = 0; p < h1; ++p) {
+        double sacc = 0.0;
+        for (int q = 0; q < h2; ++q) sacc += W12[q * h1 + p] * y2[q];
+        d1[p] = sacc * (1.0 - y1[p] * y1[p]);
+    }
+    for (int k = 0; k < nin; ++k) {
+        double sacc = 0.0;
+        for (int q = 0; q < h1; ++q) sacc += W01[q * nin + k] * d1[q];
+        wo[k] = sacc * P.escale;
+    }
+    for (int k = nin; k < ldg; ++k) wo[k] = 0.0;
+    Ep[i] = E * P.escale;
+}
+
+// pass 2 (mixed): one wavefront per 16 atoms on v_mfma_f32_16x16x16_f16.
+// Operand layout of the 16x16x16 instruction: lane l holds A[l%16][4(l/16)+c]
+// and B[4(l/16)+c][l%16]; the result D[4(l/16)+c][l%16].  Each FP32 operand
+// x is split into hi = fp16(x), lo = fp16(x - hi), and a tile product is
+// A_hi B_hi + A_hi B_lo + A_lo B_hi.
+__device__ inline void split_h(float x, _Float16& hi, _Float16& lo) {
+    hi = (_Float16)x;
+    lo = (_Float16)(x - (float)hi);
+}
+
+// C[16 x nt*16] = A[16 x kt*16] (LDS, row stride lda) * B (pre-split lane
+// layout) ; result tile nt written to LDS out[row][nt*16 + col]
+__device__ inline void mfma_gemm(const float* As, int lda, const half4_t* __restrict__ B, int nt, int kt, float* out,
+                                 int ldo, const float* bias, int act_tanh, int ncols) {
+    const int lane = threadIdx.x & 63;
+    const int r = lane & 15, kq = (lane >> 4) * 4;
+    for (int t = 0; t < nt; ++t) {
+        float4_t acc = {0.f, 0.f, 0.f, 0.f};
+        for (int k = 0; k < kt; ++k) {
+            half4_t ahi, alo;
+            for (int c = 0; c < 4; ++c) {
+                _Float16 h, l;
+                split_h(As[r * lda + k * 16 + kq + c], h, l);
+                ahi[c] = h;
+                alo[c] = l;
+            }
+            const half4_t bhi = B[((size_t)(t * kt + k) * 2 + 0) * 64 + lane];
+            const half4_t blo = B[((size_t)(t * kt + k) * 2 + 1) * 64 + lane];
+            acc = __builtin_amdgcn_mfma_f32_16x16x16f16(ahi, bhi, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16f16(ahi, blo, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16f16(alo, bhi, acc, 0, 0, 0);
+        }
+        const int col = t * 16 + (lane & 15);
+        for (int c = 0; c < 4; ++c) {
+            const int row = (lane >> 4) * 4 + c;
+            float v = acc[c];
+            if (bias) v += col < ncols ? bias[col] : 0.f;
+            if (act_tanh) v = col < ncols ? tanhf(v) : 0.f;
+            out[row * ldo + col] = v;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(64) k_bp_mlp_mfma(const float* __restrict__ G, int ldg, const half4_t* __restrict__ B,
+                                                    int4 boff, int4 nts, int4 kts, const float* __restrict__ w32,
+                                                    const BpDesc* __restrict__ Pd, const double4* __restrict__ pos,
+                                                    float* __restrict__ wout, double* __restrict__ Ep, int n,
+                                                    StepArgs SA, int* __restrict__ misc) {
+    if (skip_step(SA, misc)) return;
+    __shared__ float X[16 * kBpMaxHidden], H1[16 * kBpMaxHidden], H2[16 * kBpMaxHidden], T[16 * kBpMaxHidden];
+    const BpDesc& P = *Pd;
+    const int lane = threadIdx.x;
+    const int a0 = blockIdx.x * 16;
+    const int nin = P.nin, h1 = P.h1, h2 = P.h2;
+    const int inp = kts.x * 16, h1p = nts.x * 16, h2p = nts.y * 16;
+    const float* b01 = w32;
+    const float* b12 = b01 + h1;
+    const float* W23 = b12 + h2;
+    const float b23 = W23[h2];
+    for (int e = lane; e < 16 * inp; e += 64) {
+        const int r = e / inp, c = e % inp;
+        const int i = a0 + r;
+        X[r * kBpMaxHidden + c] = (i < n && c < nin) ? G[(size_t)i * ldg + c] : 0.f;
+    }
+    __syncthreads();
+    // forward
+    mfma_gemm(X, kBpMaxHidden, B + boff.x, nts.x, kts.x, H1, kBpMaxHidden, b01, 1, h1);
+    __syncthreads();
+    mfma_gemm(H1, kBpMaxHidden, B + boff.y, nts.y, kts.y, H2, kBpMaxHidden, b12, 1, h2);
+    __syncthreads();
+    if (lane < 16) {
+        const int i = a0 + lane;
+        float E = b23;
+        for (int q = 0; q < h2; ++q) E += W23[q] * H2[lane * kBpMaxHidden + q];
+        if (i < n && !w_inactive(pos[i].w)) Ep[i] = (double)E * P.escale;
+    }
+    // backward: D2 = W23 o (1 - h2^2)
+    for (int e = lane; e < 16 * h2p; e += 64) {
+        const int r = e / h2p, c = e % h2p;
+        const float y = H2[r * kBpMaxHidden + c];
+        T[r * kBpMaxHidden + c] = c < h2 ? W23[c] * (1.f - y * y) : 0.f;
+    }
+    __syncthreads();
+    // D1 = (D2 W12) o (1 - h1^2): B[k][n] = W12[k][n]
+    mfma_gemm(T, kBpMaxHidden, B + boff.z, nts.z, kts.z, H2, kBpMaxHidden, nullptr, 0, h1);
+    __syncthreads();
+    for (int e = lane; e < 16 * h1p; e += 64) {
+        const int r = e / h1p, c = e % h1p;
+        const float y = H1[r * kBpMaxHidden + c];
+        T[r * kBpMaxHidden + c] = c < h1 ? H2[r * kBpMaxHidden + c] * (1.f - y * y) : 0.f;
+    }
+    __syncthreads();
+    // dE/dG = D1 W01: B[k][n] = W01[k][n]
+    mfma_gemm(T, kBpMaxHidden, B + boff.w, nts.w, kts.w, X, kBpMaxHidden, nullptr, 0, nin);
+    __syncthreads();
+    for (int e = lane; e < 16 * ldg; e += 64) {
+        const int r = e / ldg, c = e % ldg;
+        const int i = a0 + r;
+        if (i < n) {
+            const bool act = !w_inactive(pos[i].w);
+            wout[(size_t)i * ldg + c] = (act && c < nin) ? (float)(X[r * kBpMaxHidden + c] * P.escale) : 0.f;
+        }
+    }
+}
+
+// pass 3: force shares of centre i on its neighbour slots and on itself
+template <typename R>
+__global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos, const int* __restrict__ nl,
+                                                 const int* __restrict__ nn, int tpa, int M, int cap,
+                                                 const SysDesc* __restrict__ desc, const BpDesc* __restrict__ Pd,
+                                                 const R* __restrict__ wgt, int ldg, R* __restrict__ Pout,
+                                                 R* __restrict__ own, int n, StepArgs SA, int* __restrict__ misc) {
+    if (skip_step(SA, misc)) return;
+    extern __shared__ char smem[];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = blockIdx.x * (blockDim.x >> 6) + wv;
+    if (i >= n) return;
+    const BpDesc& P = *Pd;
+    const Stage<R> S = carve<R>(smem + wv * stage_bytes(cap, P.ne, sizeof(R)), cap, P.ne);
+    const double4 pi = pos[i];
+    const SysDesc& s = desc[w_sys(pi.w)];
+    const int cnt = min(nn[i], cap);
+    const int nc = bp_stage<R>(i, pi, s, nl, cnt, tpa, M, pos, P, S, Pout, cap);
+    R w[kBpMaxR + kBpMaxA];
+#pragma unroll
+    for (int q = 0; q < kBpMaxR; ++q) w[q] = q < P.nr ? wgt[(size_t)i * ldg + q] : (R)0;
+#pragma unroll
+    for (int q = 0; q < kBpMaxA; ++q) w[kBpMaxR + q] = q < P.na ? wgt[(size_t)i * ldg + P.nr + q] : (R)0;
+    const R Rs = (R)P.Rs, Rc2 = (R)P.Rc2, pirc = (R)P.pi_rc;
+    R ox = 0, oy = 0, oz = 0;
+    for (int a = lane; a < nc; a += kWave) {
+        const R ax = S.x[a], ay = S.y[a], az = S.z[a], da = S.d[a], fa = S.fc[a], dfa = S.dfc[a];
+        const R ux = ax / da, uy = ay / da, uz = az / da;
+        // radial: sum_r w_r g_r'(d) along u_a
+        R sr = 0;
+#pragma unroll
+        for (int r = 0; r < kBpMaxR; ++r)
+            if (r < P.nr) {
+                const R eta = (R)P.eta_r[r];
+                sr += w[r] * dexp_(-eta * (da - Rs) * (da - Rs)) * ((R)-2 * eta * (da - Rs) * fa + dfa);
+            }
+        R gx = sr * ux, gy = sr * uy, gz = sr * uz;
+        R Ea[kBpMaxEta];
+#pragma unroll
+        for (int e = 0; e < kBpMaxEta; ++e)
+            if (e < P.ne) Ea[e] = S.E[e * cap + a];
+        for (int b = 0; b < nc; ++b) {
+            if (b == a) continue;
+            const R bx = S.x[b], by = S.y[b], bz = S.z[b], db = S.d[b];
+            const R cx = bx - ax, cy = by - ay, cz = bz - az;
+            const R d2c = cx * cx + cy * cy + cz * cz;
+            if (!(d2c >= (R)1e-15 && d2c < Rc2)) continue;
+            const R dc = dsqrt_(d2c);
+            const R fcc = (R)0.5 * (dcos_(pirc * dc) + (R)1);
+            const R dfcc = (R)-0.5 * dsin_(pirc * dc) * pirc;
+            const R fb = S.fc[b];
+            const R cs = (ax * bx + ay * by + az * bz) / (da * db);
+            const R F = fa * fb * fcc;
+            R Ex[kBpMaxEta];
+#pragma unroll
+            for (int e = 0; e < kBpMaxEta; ++e)
+                if (e < P.ne) Ex[e] = Ea[e] * S.E[e * cap + b] * dexp_(-(R)P.eta_e[e] * d2c);
+            R ca = 0, cb = 0, cc = 0;
+#pragma unroll
+            for (int q = 0; q < kBpMaxA; ++q)
+                if (q < P.na) {
+                    const R lam = (R)P.a_lam[q], zeta = (R)P.a_zeta[q], eta = (R)P.eta_e[P.a_e[q]];
+                    const R p1 = (R)1 + lam * cs;
+                    const R pz1 = P.a_iz[q] >= 1 ? powi_(p1, P.a_iz[q] - 1, zeta - (R)1)
+                                                 : (R)pow((double)p1, (double)(zeta - (R)1));
+                    const R pz = pz1 * p1;
+                    const R cm = w[kBpMaxR + q] * (R)P.a_coef[q] * Ex[P.a_e[q]];
+                    const R zl = zeta * lam * pz1 * F / da;
+                    cb += cm * zl;
+                    ca += cm * (-zl * cs + pz * ((R)-2 * eta * F * da + fb * dfa * fcc));
+                    cc += cm * pz * ((R)2 * eta * F * dc - fb * fa * dfcc);
+                }
+            const R ubx = bx / db, uby = by / db, ubz = bz / db;
+            const R ucx = cx / dc, ucy = cy / dc, ucz = cz / dc;
+            gx += ca * ux + cb * ubx + cc * ucx;
+            gy += ca * uy + cb * uby + cc * ucy;
+            gz += ca * uz + cb * ubz + cc * ucz;
+        }
+        R* q = Pout + ((size_t)i * cap + S.slot[a]) * 3;
+        q[0] = -gx;
+        q[1] = -gy;
+        q[2] = -gz;
+        ox += gx;
+        oy += gy;
+        oz += gz;
+    }
+    ox = wave_sum(ox);
+    oy = wave_sum(oy);
+    oz = wave_sum(oz);
+    if (lane == 0) {
+        own[(size_t)i * 3 + 0] = ox;
+        own[(size_t)i * 3 + 1] = oy;
+        own[(size_t)i * 3 + 2] = oz;
+    }
+}
+
+// reverse slot index: rev(m, k) = slot of m in the list of its k-th neighbour
+__global__ void k_bp_rev(const int* __restrict__ nl, const int* __restrict__ nn, int* __restrict__ rev, int tpa, int M,
+                         int cap, int n) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    const int cm = min(nn[m], cap);
+    for (int k = 0; k < cm; ++k) {
+        const int i = nl[list_index(m, k, tpa, M)];
+        const int ci = min(nn[i], cap);
+        int r = -1;
+        for (int q = 0; q < ci; ++q)
+            if (nl[list_index(i, q, tpa, M)] == m) {
+                r = q;
+                break;
+            }
+        rev[list_index(m, k, tpa, M)] = r;
+    }
+}
+
+// pass 4: gather the force shares, then the fused velocity-Verlet update
+template <typename R>
+__global__ void __launch_bounds__(256) k_bp_gather(const double4* __restrict__ pos, double4* __restrict__ pos_next,
+                                                   const int* __restrict__ nl, const int* __restrict__ nn,
+                                                   const int* __restrict__ rev, int tpa, int M, int cap,
+                                                   const SysDesc* __restrict__ desc, const R* __restrict__ Pin,
+                                                   const R* __restrict__ own, AtomArrays A, StepArgs SA,
+                                                   const double4* __restrict__ rsave, int* __restrict__ misc) {
+    if (skip_step(SA, misc)) return;
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= A.n) return;
+    const double4 pm = pos[m];
+    if (w_inactive(pm.w)) {
+        if (SA.mode == MODE_FINAL_DRIFT) pos_next[m] = pm;
+        return;
+    }
+    R fx = own[(size_t)m * 3 + 0], fy = own[(size_t)m * 3 + 1], fz = own[(size_t)m * 3 + 2];
+    const int cm = min(nn[m], cap);
+    for (int k = 0; k < cm; ++k) {
+        const long long li = list_index(m, k, tpa, M);
+        const int i = nl[li];
+        const int r = rev[li];
+        if (r < 0) continue;
+        const R* q = Pin + ((size_t)i * cap + r) * 3;
+        fx += q[0];
+        fy += q[1];
+        fz += q[2];
+    }
+    const SysDesc& s = desc[w_sys(pm.w)];
+    finalize_atom<R>(m, pm, (double)fx, (double)fy, (double)fz, A, SA, pos_next, rsave, s, misc);
+}
+
+// --------------------------------------------------------------------------
+// host: MAISE model file (parser/maise_model.cxx:29-129 layout) and setup
+// --------------------------------------------------------------------------
+namespace {
+
+std::vector<std::string> words(const std::string& s) {
+    std::vector<std::string> out;
+    std::istringstream is(s);
+    std::string t;
+    while (is >> t) out.push_back(t);
+    return out;
+}
+
+struct Maise {
+    int nspc = 0;
+    std::vector<int> NN;
+    int nw = 0;
+    std::vector<double> range;  // species 0
+    std::vector<double> weights;
+    double b2a = 1, scaling = 1;
+    std::vector<std::vector<int>> map;
+    std::vector<std::vector<double>> vals;
+};
+
+Maise read_maise(const char* path) {
+    std::ifstream f(path);
+    if (!f.is_open()) throw Error(E_FILE, std::string("Failed to open file: ") + path);
+    Maise m;
+    std::string line;
+    auto next = [&]() {
+        if (!std::getline(f, line)) throw Error(E_FILE, std::string("unexpected end of MAISE model file: ") + path);
+    };
+    auto field = [&]() {
+        next();
+        size_t p = line.find('|', 1);
+        if (p == std::string::npos) throw Error(E_FILE, "malformed MAISE head line: " + line);
+        return line.substr(p + 1, p + 10);
+    };
+    for (int i = 0; i < 3; ++i) next();
+    (void)field();  // model id
+    m.nspc = std::stoi(field());
+    (void)field();  // species types
+    (void)field();  // species names
+    (void)field();  // number of layers
+    for (auto& t : words(field())) m.NN.push_back(std::stoi(t));
+    m.nw = std::stoi(field());
+    const std::string ICR = "|                          input component range                            |";
+    do next(); while (line != ICR);
+    next();
+    next();
+    if (m.NN.empty()) throw Error(E_FILE, "MAISE architecture missing");
+    for (int s = 0; s < m.nspc; ++s)
+        for (int k = 0; k < m.NN[0]; ++k) {
+            next();
+            auto t = words(line);
+            if (t.size() < 3) throw Error(E_FILE, "malformed MAISE input range row");
+            if (s == 0) m.range.push_back(std::stod(t[2]));
+        }
+    for (int i = 0; i < 3; ++i) next();
+    for (int i = 0; i < m.nw; ++i) {
+        next();
+        m.weights.push_back(std::stod(line));
+    }
+    for (int i = 0; i < 3; ++i) next();
+    auto bs = words(line);
+    if (bs.size() < 2) throw Error(E_FILE, "malformed MAISE B2A/Scaling row");
+    m.b2a = std::stod(bs[0]);
+    m.scaling = std::stod(bs[1]);
+    for (int i = 0; i < 3; ++i) next();
+    next();
+    while (!line.empty()) {
+        std::vector<int> row;
+        for (auto& t : words(line)) row.push_back(std::stoi(t));
+        m.map.push_back(row);
+        next();
+    }
+    for (int i = 0; i < 7; ++i) {
+        next();
+        auto t = words(line);
+        if (t.size() < 2 || std::stoi(t[1]) != (int)t.size() - 2)
+            throw Error(E_FILE, "malformed MAISE descriptor value row: " + line);
+        std::vector<double> v;
+        for (size_t k = 2; k < t.size(); ++k) v.push_back(std::stod(t[k]));
+        m.vals.push_back(v);
+    }
+    return m;
+}
+
+template <class T>
+T* dnew(size_t count) {
+    void* p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess)
+        throw Error(E_NOMEM, "out of device memory for the BPNN model");
+    return static_cast<T*>(p);
+}
+
+BpWork& work(Batch& b) {
+    if (!b.bp_scratch) {
+        b.bp_scratch = new BpWork();
+    }
+    return *static_cast<BpWork*>(b.bp_scratch);
+}
+
+void free_work(BpWork& w) {
+    for (void* p : {w.G, w.w, w.P, w.own, (void*)w.rev})
+        if (p) (void)hipFree(p);
+    w.G = w.w = w.P = w.own = nullptr;
+    w.rev = nullptr;
+    w.bytes = 0;
+}
+
+}  // namespace
+
+void bpnn_release_work(Batch& b) {
+    if (!b.bp_scratch) return;
+    BpWork* w = static_cast<BpWork*>(b.bp_scratch);
+    free_work(*w);
+    delete w;
+    b.bp_scratch = nullptr;
+}
+
+Pot* bpnn_create(Ctx* ctx, const char* path) {
+    Maise m = read_maise(path);
+    if (m.NN.size() != 4) throw Error(E_FILE, "MaiseBPNN: only 4-layer networks are supported");
+    if (m.nspc != 1) throw Error(E_INVAL, "MaiseBPNN: one species is supported");
+    if (m.NN[3] != 1) throw Error(E_FILE, "MaiseBPNN: the output layer must have one neuron");
+    auto* bp = new BpnnParams();
+    BpDesc& D = bp->h;
+    std::memset(&D, 0, sizeof(D));
+    const int nin = m.NN[0], h1 = m.NN[1], h2 = m.NN[2];
+    if (h1 > kBpMaxHidden || h2 > kBpMaxHidden || nin > kBpMaxIn)
+        throw Error(E_INVAL, "MaiseBPNN: at most 32 inputs and 64 neurons per hidden layer are supported");
+    // maise_model.cxx:34-50 unit scaling
+    const double lin = m.b2a * m.scaling * 0.1, isq = 1.0 / lin / lin;
+    if (m.vals.size() < 7) throw Error(E_FILE, "MAISE descriptor table incomplete");
+    D.Rc = m.vals[0].at(0) * lin;
+    D.Rs = m.vals[1].at(0) * lin;
+    std::vector<double> eta1 = m.vals[2], eta2 = m.vals[3];
+    for (auto& e : eta1) e *= isq;
+    for (auto& e : eta2) e *= isq;
+    std::vector<double> etas;
+    for (auto& row : m.map) {
+        if (row.size() != 9) throw Error(E_FILE, "MAISE descriptor map rows need 9 columns");
+        if (row[1] == 2) {
+            if (D.nr >= kBpMaxR) throw Error(E_INVAL, "at most 16 radial descriptors are supported");
+            D.eta_r[D.nr++] = eta1.at(row[4]);
+        } else if (row[1] == 4) {
+            if (D.na >= kBpMaxA) throw Error(E_INVAL, "at most 16 angular descriptors are supported");
+            double eta = eta2.at(row[5]), z = m.vals[5].at(row[7]), l = m.vals[6].at(row[8]);
+            int e = -1;
+            for (size_t q = 0; q < etas.size(); ++q)
+                if (etas[q] == eta) e = (int)q;
+            if (e < 0) {
+                if ((int)etas.size() >= kBpMaxEta) throw Error(E_INVAL, "at most 4 distinct G4 eta values are supported");
+                etas.push_back(eta);
+                e = (int)etas.size() - 1;
+            }
+            const int a = D.na++;
+            D.a_e[a] = e;
+            D.a_zeta[a] = z;
+            D.a_lam[a] = l;
+            D.a_coef[a] = std::pow(2.0, 1.0 - z) * 2.0;  // base.cxx:120 factor 2
+            D.a_iz[a] = (z == std::floor(z) && z >= 0 && z <= 32) ? (int)z : -1;
+        } else {
+            throw Error(E_INVAL, "only G2 radial and G4 angular descriptors are supported");
+        }
+    }
+    if (D.nr + D.na < nin) throw Error(E_FILE, "fewer descriptors than network inputs");
+    if (D.nr + D.na != nin) throw Error(E_INVAL, "descriptor count must equal the network input width");
+    D.ne = (int)etas.size();
+    for (int e = 0; e < D.ne; ++e) D.eta_e[e] = etas[e];
+    D.Rc2 = D.Rc * D.Rc;
+    D.pi_rc = M_PI / D.Rc;
+    D.nin = nin;
+    D.h1 = h1;
+    D.h2 = h2;
+    D.escale = 10.0 * 1.602176634 * 6.02214076;  // maise_bpnn.cxx:317
+    // maise_ann.cxx:23-51 weight order, columns of W01 scaled by range
+    size_t w = 0;
+    auto take = [&](size_t k) {
+        if (w + k > m.weights.size()) throw Error(E_FILE, "MAISE weight count does not match the architecture");
+        std::vector<double> v(m.weights.begin() + w, m.weights.begin() + w + k);
+        w += k;
+        return v;
+    };
+    auto b01 = take(h1), b12 = take(h2), b23 = take(1);
+    auto W01 = take((size_t)h1 * nin), W12 = take((size_t)h2 * h1), W23 = take(h2);
+    if ((int)w != m.nw) throw Error(E_FILE, "MAISE weight count does not match the architecture");
+    for (int k = 0; k < nin; ++k)
+        for (int q = 0; q < h1; ++q) W01[(size_t)q * nin + k] *= m.range.at(k);
+    std::vector<double> W64;
+    for (auto* v : {&W01, &b01, &W12, &b12, &W23}) W64.insert(W64.end(), v->begin(), v->end());
+    W64.push_back(b23[0]);
+    std::vector<float> W32;
+    for (auto* v : {&b01, &b12, &W23}) W32.insert(W32.end(), v->begin(), v->end());
+    W32.push_back((float)b23[0]);
+    // B operands of the four GEMMs in the MFMA lane layout, split hi/lo
+    auto pad16 = [](int x) { return (x + 15) / 16 * 16; };
+    bp->nin_pad = pad16(nin);
+    bp->h1_pad = pad16(h1);
+    bp->h2_pad = pad16(h2);
+    struct Gm {
+        int K, N;
+        std::function<double(int, int)> at;
+    };
+    std::vector<Gm> gm = {
+        {nin, h1, [&](int k, int n) { return W01[(size_t)n * nin + k]; }},  // X W01^T
+        {h1, h2, [&](int k, int n) { return W12[(size_t)n * h1 + k]; }},    // H1 W12^T
+        {h2, h1, [&](int k, int n) { return W12[(size_t)k * h1 + n]; }},    // D2 W12
+        {h1, nin, [&](int k, int n) { return W01[(size_t)k * nin + n]; }},  // D1 W01
+    };
+    std::vector<half4_t> B;
+    for (int g = 0; g < 4; ++g) {
+        const int kt = pad16(gm[g].K) / 16, nt = pad16(gm[g].N) / 16;
+        bp->boff[g] = (int)B.size();
+        bp->kt[g] = kt;
+        bp->nt[g] = nt;
+        for (int t = 0; t < nt; ++t)
+            for (int k = 0; k < kt; ++k)
+                for (int part = 0; part < 2; ++part)
+                    for (int lane = 0; lane < 64; ++lane) {
+                        half4_t v;
+                        for (int c = 0; c < 4; ++c) {
+                            const int kk = k * 16 + (lane >> 4) * 4 + c, nn = t * 16 + (lane & 15);
+                            const double x = (kk < gm[g].K && nn < gm[g].N) ? gm[g].at(kk, nn) : 0.0;
+                            const _Float16 hi = (_Float16)(float)x;
+                            const _Float16 lo = (_Float16)((float)x - (float)hi);
+                            v[c] = part == 0 ? hi : lo;
+                        }
+                        B.push_back(v);
+                    }
+    }
+    bp->d = dnew<BpDesc>(1);
+    bp->w64 = dnew<double>(W64.size());
+    bp->w32 = dnew<float>(W32.size());
+    bp->bsplit = dnew<half4_t>(B.size());
+    HIPCHECK(hipMemcpy(bp->d, &D, sizeof(D), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(bp->w64, W64.data(), W64.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(bp->w32, W32.data(), W32.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(bp->bsplit, B.data(), B.size() * sizeof(half4_t), hipMemcpyHostToDevice));
+    auto* p = new Pot();
+    p->ctx = ctx;
+    p->type = PT_BPNN;
+    p->rcut = D.Rc;
+    p->bp = bp;
+    return p;
+}
+
+void bpnn_destroy(Pot* p) {
+    if (p->bp) {
+        BpnnParams* bp = p->bp;
+        for (void* q : {(void*)bp->d, (void*)bp->w64, (void*)bp->w32, (void*)bp->bsplit})
+            if (q) (void)hipFree(q);
+        delete bp;
+    }
+    delete p;
+}
+
+static void ensure_work(Batch& b, const BpnnParams& bp, int prec) {
+    BpWork& w = work(b);
+    const size_t rs = prec == P_FP64 ? 8 : 4;
+    const int ldg = bp.nin_pad;
+    if (w.cap == b.cap && w.n == b.n && w.prec == prec && w.G) return;
+    free_work(w);
+    w.G = dnew<char>((size_t)b.n * ldg * rs);
+    w.w = dnew<char>((size_t)b.n * ldg * rs);
+    w.P = dnew<char>((size_t)b.n * b.cap * 3 * rs);
+    w.own = dnew<char>((size_t)b.n * 3 * rs);
+    w.rev = dnew<int>(b.nl_elems);
+    w.cap = b.cap;
+    w.n = b.n;
+    w.prec = prec;
+    w.rev_valid = false;
+}
+
+void bpnn_compute(Batch& b, const Pot& p, int what, int prec, int mode, double dt, int step, const double4* pos,
+                  double4* pos_next) {
+    const BpnnParams& bp = *p.bp;
+    for (int s = 0; s < b.nsys; ++s)
+        for (int a = 0; a < 3; ++a)
+            if (b.desc[s].per[a] && b.desc[s].L[a] < 2.0 * std::sqrt(b.desc[s].rns2))
+                throw Error(E_INVAL, "BPNN needs periodic box lengths of at least twice the search radius");
+    if (b.cap > 1024) throw Error(E_INVAL, "BPNN neighbour capacity above 1024");
+    ensure_work(b, bp, prec);
+    BpWork& w = work(b);
+    hipStream_t st = b.ctx->stream;
+    const int n = b.n;
+    StepArgs S{dt, step, mode, what & (W_ENERGY | W_VIRIAL), mode == 0 ? 1 : 0};
+    if (!w.rev_valid || w.rev_builds != b.rebuilds) {
+        hipLaunchKernelGGL(k_bp_rev, dim3((n + 255) / 256), dim3(256), 0, st, b.nl, b.nn, w.rev, b.tpa, b.M, b.cap, n);
+        w.rev_valid = true;
+        w.rev_builds = b.rebuilds;
+    }
+    const int waves = 4;
+    const dim3 gw((n + waves - 1) / waves), bw(64 * waves);
+    const int ldg = bp.nin_pad;
+    AtomArrays AA{b.dst[b.sc], b.fst[b.sc], n};
+    double* Ep = b.d(b.sc, D_EP);
+    const int4 boff = make_int4(bp.boff[0], bp.boff[1], bp.boff[2], bp.boff[3]);
+    const int4 nts = make_int4(bp.nt[0], bp.nt[1], bp.nt[2], bp.nt[3]);
+    const int4 kts = make_int4(bp.kt[0], bp.kt[1], bp.kt[2], bp.kt[3]);
+    auto run = [&](auto rtag) {
+        using R = decltype(rtag);
+        const size_t lds = waves * stage_bytes(b.cap, bp.h.ne, sizeof(R));
+        if (lds > 160 * 1024) throw Error(E_INVAL, "BPNN neighbour staging exceeds the LDS");
+        hipEvent_t t0 = timer_begin(b);
+        hipLaunchKernelGGL((k_bp_desc<R>), gw, bw, lds, st, pos, b.nl, b.nn, b.tpa, b.M, b.cap, b.d_desc, bp.d,
+                           (R*)w.G, ldg, n, S, b.misc);
+        timer_end(b, t0, K_BP_DESC);
+        hipEvent_t t1 = timer_begin(b);
+        if (sizeof(R) == 8)
+            hipLaunchKernelGGL(k_bp_mlp64, dim3((n + 127) / 128), dim3(128), 0, st, (const double*)w.G, ldg, bp.w64,
+                               bp.d, pos, (double*)w.w, Ep, n, S, b.misc);
+        else
+            hipLaunchKernelGGL(k_bp_mlp_mfma, dim3((n + 15) / 16), dim3(64), 0, st, (const float*)w.G, ldg, bp.bsplit,
+                               boff, nts, kts, bp.w32, bp.d, pos, (float*)w.w, Ep, n, S, b.misc);
+        timer_end(b, t1, K_BP_MLP);
+        hipEvent_t t2 = timer_begin(b);
+        hipLaunchKernelGGL((k_bp_grad<R>), gw, bw, lds, st, pos, b.nl, b.nn, b.tpa, b.M, b.cap, b.d_desc, bp.d,
+                           (const R*)w.w, ldg, (R*)w.P, (R*)w.own, n, S, b.misc);
+        timer_end(b, t2, K_BP_FORCE);
+        hipEvent_t t3 = timer_begin(b);
+        hipLaunchKernelGGL((k_bp_gather<R>), dim3((n + 255) / 256), dim3(256), 0, st, pos, pos_next, b.nl, b.nn, w.rev, b.tpa, b.M,
+                           b.cap, b.d_desc, (const R*)w.P, (const R*)w.own, AA, S, b.rsave, b.misc);
+        timer_end(b, t3, K_BP_GATHER);
+    };
+    if (prec == P_FP64) run(double(0));
+    else run(float(0));
+}
+
+// G and dE/dG of the last evaluation, original atom order, as doubles
+void bpnn_inspect(Batch& b, const Pot& p, double* G, double* dEdG) {
+    if (!b.bp_scratch) throw Error(E_STATE, "no BPNN evaluation to inspect");
+    BpWork& w = work(b);
+    const int ldg = p.bp->nin_pad, nin = p.bp->h.nin, n = b.n;
+    const size_t rs = w.prec == P_FP64 ? 8 : 4;
+    std::vector<char> g((size_t)n * ldg * rs), e((size_t)n * ldg * rs);
+    std::vector<int> where(n);
+    hipStream_t st = b.ctx->stream;
+    HIPCHECK(hipMemcpyAsync(g.data(), w.G, g.size(), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(e.data(), w.w, e.size(), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(where.data(), b.where, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    for (int o = 0; o < n; ++o) {
+        const int q = where[o];
+        for (int k = 0; k < nin; ++k) {
+            const size_t idx = (size_t)q * ldg + k;
+            if (rs == 8) {
+                if (G) G[(size_t)o * nin + k] = reinterpret_cast<double*>(g.data())[idx];
+                if (dEdG) dEdG[(size_t)o * nin + k] = reinterpret_cast<double*>(e.data())[idx];
+            } else {
+                if (G) G[(size_t)o * nin + k] = reinterpret_cast<float*>(g.data())[idx];
+                if (dEdG) dEdG[(size_t)o * nin + k] = reinterpret_cast<float*>(e.data())[idx];
+            }
+        }
+    }
+}
+
+int bpnn_inputs(const Pot& p) { return p.bp ? p.bp->h.nin : 0; }
+
+}  // namespace mdc

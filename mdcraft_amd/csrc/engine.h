@@ -153,7 +153,11 @@ void thermo(Batch& b, double* out);
 void set_trigger(Batch& b, double rcut);
 void check_errors(Batch& b);
 // launchers (bpnn.hip)
-void bpnn_compute(Batch& b, const Pot& p, int what, int prec, int mode, double dt, int step);
+void bpnn_compute(Batch& b, const Pot& p, int what, int prec, int mode, double dt, int step, const double4* pos,
+                  double4* pos_next);
+void bpnn_release_work(Batch& b);
+void bpnn_inspect(Batch& b, const Pot& p, double* G, double* dEdG);
+int bpnn_inputs(const Pot& p);
 Pot* bpnn_create(Ctx* ctx, const char* path);
 void bpnn_destroy(Pot* p);
 

@@ -18,110 +18,9 @@
 // embedding, integration and displacement check stay FP64.
 #include <climits>
 #include <cmath>
-#include "engine.h"
+#include "kern.h"
 
 namespace mdc {
-
-enum Mode { MODE_COMPUTE = 0, MODE_FINAL = 1, MODE_FINAL_DRIFT = 2 };
-
-__device__ inline float rcp_(float x) { return __builtin_amdgcn_rcpf(x); }
-__device__ inline double rcp_(double x) { return 1.0 / x; }
-
-template <typename R>
-struct Vec4T;
-template <>
-struct Vec4T<double> { using T = double4; };
-template <>
-struct Vec4T<float> { using T = float4; };
-
-// Hermite interval of the reference tables (EAM.cxx:367-394).
-// rec = (y0, z0, y1, z1); returns value and derivative at R1.
-template <typename R>
-__device__ inline void spline_vd(const typename Vec4T<R>::T rec, R R1, R RN, int N1, R dr, R dr1, R& val, R& der) {
-    R VR = (R)N1;
-    R DV = RN - VR;
-    R h = R1 - VR * dr;
-    R s = (rec.z - rec.x) * dr1;
-    R aa = (rec.w - s) - (s - rec.y);
-    R bb = aa - (s - rec.y);
-    val = rec.x + h * (rec.y + DV * (DV * aa - bb));
-    der = rec.y + (R)2 * DV * ((R)1.5 * DV * aa - bb);
-}
-
-template <typename R>
-__device__ inline R spline_v(const typename Vec4T<R>::T rec, R R1, R RN, int N1, R dr, R dr1) {
-    R VR = (R)N1;
-    R DV = RN - VR;
-    R h = R1 - VR * dr;
-    R s = (rec.z - rec.x) * dr1;
-    R aa = (rec.w - s) - (s - rec.y);
-    R bb = aa - (s - rec.y);
-    return rec.x + h * (rec.y + DV * (DV * aa - bb));
-}
-
-struct StepArgs {
-    double dt;
-    int step;  // index of this step inside the running sequence (-1: plain compute)
-    int mode;
-    int flags;  // W_ENERGY | W_VIRIAL
-    int store_aux;
-};
-
-struct AtomArrays {
-    double* d;   // current double fields base (stride n)
-    float* f;    // current float fields base (stride n)
-    int n;
-};
-
-// Finalise one atom after its force is known: store force, and for step
-// modes apply the closing half kick (verlet.cxx:80-85) and, when another
-// step follows, the opening half kick and drift (verlet.cxx:71-78) into the
-// other position buffer together with the displacement trigger of
-// verlet-list.cxx:149-154.
-template <typename R>
-__device__ inline void finalize_atom(int i, const double4& pi, double fx, double fy, double fz, const AtomArrays& A,
-                                     const StepArgs& S, double4* __restrict__ pos_next,
-                                     const double4* __restrict__ rsave, const SysDesc& s, int* __restrict__ misc) {
-    const int n = A.n;
-    if (sizeof(R) == 4) {
-        A.f[(size_t)F_FX * n + i] = (float)fx;
-        A.f[(size_t)F_FY * n + i] = (float)fy;
-        A.f[(size_t)F_FZ * n + i] = (float)fz;
-    } else {
-        A.d[(size_t)D_FX * n + i] = fx;
-        A.d[(size_t)D_FY * n + i] = fy;
-        A.d[(size_t)D_FZ * n + i] = fz;
-    }
-    if (S.mode == MODE_COMPUTE) return;
-    const double m = A.d[(size_t)D_M * n + i];
-    double v[3] = {A.d[(size_t)D_VX * n + i], A.d[(size_t)D_VY * n + i], A.d[(size_t)D_VZ * n + i]};
-    const double F[3] = {fx, fy, fz};
-    for (int c = 0; c < 3; ++c) {
-        double a = F[c] / m;
-        v[c] += a * 0.5 * S.dt;
-    }
-    if (S.mode == MODE_FINAL_DRIFT) {
-        double r[3] = {pi.x, pi.y, pi.z};
-        for (int c = 0; c < 3; ++c) {
-            double a = F[c] / m;
-            v[c] = v[c] + a * S.dt * 0.5;
-            r[c] += v[c] * S.dt;
-        }
-        pos_next[i] = make_double4(r[0], r[1], r[2], pi.w);
-        const double4 q = rsave[i];
-        double dx = r[0] - q.x, dy = r[1] - q.y, dz = r[2] - q.z;
-        shortest3(s, dx, dy, dz);
-        if (sq3_exact(dx, dy, dz) > s.trig2) atomicMin(&misc[M_ABORT], S.step + 1);
-        if (!(fabs(r[0]) < 1e300 && fabs(r[1]) < 1e300 && fabs(r[2]) < 1e300)) atomicOr(&misc[M_ERR], EB_NONFINITE);
-    }
-    A.d[(size_t)D_VX * n + i] = v[0];
-    A.d[(size_t)D_VY * n + i] = v[1];
-    A.d[(size_t)D_VZ * n + i] = v[2];
-}
-
-__device__ inline bool skip_step(const StepArgs& S, const int* misc) {
-    return S.mode != MODE_COMPUTE && *(volatile const int*)&misc[M_ABORT] <= S.step;
-}
 
 // --------------------------------------------------------------------------
 // LJsCut
@@ -283,10 +182,18 @@ __global__ void __launch_bounds__(256) k_eam_rho(const double4* __restrict__ pos
         rho += (double)spline_v<R>(tab[(size_t)kj * P.nr + N1], R1, RN, N1, dr, dr1);
     }
     rho = wave_sum_group(rho, TPA);
-    if (!valid || as.sub != 0 || !active) return;
+    if (!valid || as.sub != 0) return;
     const int n = A.n;
     const int ki = P.nk > 1 ? w_kind(pi.w) : 0;
     double F, dF;
+    if (!active) {
+        // tag == 0: the reference skips the atom's prepare and charge, yet
+        // neighbours still evaluate F'(nc) of its stored density
+        // (single.cxx:44-50, EAM.cxx:467-474)
+        embed(P, ki, A.d[(size_t)D_NC * n + i], F, dF, misc);
+        A.d[(size_t)D_FP * n + i] = dF;
+        return;
+    }
     embed(P, ki, rho, F, dF, misc);
     A.d[(size_t)D_FP * n + i] = dF;
     if (S.store_aux) {
@@ -607,7 +514,7 @@ static void launch_pair_r(Batch& b, const Pot& p, const StepArgs& S, const doubl
 static void launch_step_kernels(Batch& b, const Pot& p, const StepArgs& S, int prec, const double4* pos,
                                 double4* pos_next) {
     if (p.type == PT_BPNN) {
-        bpnn_compute(b, p, S.flags, prec, S.mode, S.dt, S.step);
+        bpnn_compute(b, p, S.flags, prec, S.mode, S.dt, S.step, pos, pos_next);
         return;
     }
     if (prec == P_MIXED) launch_pair_r<float>(b, p, S, pos, pos_next);
@@ -696,7 +603,7 @@ void run_steps(Batch& b, const Pot& p, double dt, int nsteps, int prec, int chun
         HIPCHECK(hipMemsetAsync(b.d(b.sc, D_EP), 0, sizeof(double) * 2 * b.n, st));
     }
     HIPCHECK(hipMemsetAsync(b.d(b.sc, D_VXX), 0, sizeof(double) * 6 * b.n, st));
-    b.f_prec = (p.type == PT_BPNN) ? P_MIXED : prec;
+    b.f_prec = prec;
     b.last_rcut = p.rcut;
 }
 

@@ -1,0 +1,132 @@
+// Device helpers shared by the pair, EAM and BPNN kernels: step arguments,
+// the abort check of a step sequence and the fused velocity-Verlet finish.
+#pragma once
+#include "engine.h"
+
+namespace mdc {
+
+enum Mode { MODE_COMPUTE = 0, MODE_FINAL = 1, MODE_FINAL_DRIFT = 2 };
+
+__device__ inline float rcp_(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline double rcp_(double x) { return 1.0 / x; }
+
+template <typename R>
+struct Vec4T;
+template <>
+struct Vec4T<double> { using T = double4; };
+template <>
+struct Vec4T<float> { using T = float4; };
+
+// Hermite interval of the reference tables (EAM.cxx:367-394).
+// FP64 records are (y0, z0, y1, z1) and use the reference's formula.  FP32
+// records hold the same cubic as coefficients in the interval fraction t,
+// c0 + t (c1 + t (c2 + t c3)), computed in FP64 on the host: forming the
+// slope (y1 - y0)/dr from FP32 knots cancels catastrophically.
+template <typename R>
+__device__ inline void spline_vd(const typename Vec4T<R>::T rec, R R1, R RN, int N1, R dr, R dr1, R& val, R& der);
+template <typename R>
+__device__ inline R spline_v(const typename Vec4T<R>::T rec, R R1, R RN, int N1, R dr, R dr1);
+
+template <>
+__device__ inline void spline_vd<double>(const double4 rec, double R1, double RN, int N1, double dr, double dr1,
+                                         double& val, double& der) {
+    double VR = (double)N1;
+    double DV = RN - VR;
+    double h = R1 - VR * dr;
+    double s = (rec.z - rec.x) * dr1;
+    double aa = (rec.w - s) - (s - rec.y);
+    double bb = aa - (s - rec.y);
+    val = rec.x + h * (rec.y + DV * (DV * aa - bb));
+    der = rec.y + 2.0 * DV * (1.5 * DV * aa - bb);
+}
+
+template <>
+__device__ inline double spline_v<double>(const double4 rec, double R1, double RN, int N1, double dr, double dr1) {
+    double VR = (double)N1;
+    double DV = RN - VR;
+    double h = R1 - VR * dr;
+    double s = (rec.z - rec.x) * dr1;
+    double aa = (rec.w - s) - (s - rec.y);
+    double bb = aa - (s - rec.y);
+    return rec.x + h * (rec.y + DV * (DV * aa - bb));
+}
+
+template <>
+__device__ inline void spline_vd<float>(const float4 c, float R1, float RN, int N1, float dr, float dr1, float& val,
+                                        float& der) {
+    const float t = RN - (float)N1;
+    val = c.x + t * (c.y + t * (c.z + t * c.w));
+    der = (c.y + t * (2.f * c.z + 3.f * t * c.w)) * dr1;
+}
+
+template <>
+__device__ inline float spline_v<float>(const float4 c, float R1, float RN, int N1, float dr, float dr1) {
+    const float t = RN - (float)N1;
+    return c.x + t * (c.y + t * (c.z + t * c.w));
+}
+
+struct StepArgs {
+    double dt;
+    int step;  // index of this step inside the running sequence (-1: plain compute)
+    int mode;
+    int flags;  // W_ENERGY | W_VIRIAL
+    int store_aux;
+};
+
+struct AtomArrays {
+    double* d;   // current double fields base (stride n)
+    float* f;    // current float fields base (stride n)
+    int n;
+};
+
+// Finalise one atom after its force is known: store force, and for step
+// modes apply the closing half kick (verlet.cxx:80-85) and, when another
+// step follows, the opening half kick and drift (verlet.cxx:71-78) into the
+// other position buffer together with the displacement trigger of
+// verlet-list.cxx:149-154.
+template <typename R>
+__device__ inline void finalize_atom(int i, const double4& pi, double fx, double fy, double fz, const AtomArrays& A,
+                                     const StepArgs& S, double4* __restrict__ pos_next,
+                                     const double4* __restrict__ rsave, const SysDesc& s, int* __restrict__ misc) {
+    const int n = A.n;
+    if (sizeof(R) == 4) {
+        A.f[(size_t)F_FX * n + i] = (float)fx;
+        A.f[(size_t)F_FY * n + i] = (float)fy;
+        A.f[(size_t)F_FZ * n + i] = (float)fz;
+    } else {
+        A.d[(size_t)D_FX * n + i] = fx;
+        A.d[(size_t)D_FY * n + i] = fy;
+        A.d[(size_t)D_FZ * n + i] = fz;
+    }
+    if (S.mode == MODE_COMPUTE) return;
+    const double m = A.d[(size_t)D_M * n + i];
+    double v[3] = {A.d[(size_t)D_VX * n + i], A.d[(size_t)D_VY * n + i], A.d[(size_t)D_VZ * n + i]};
+    const double F[3] = {fx, fy, fz};
+    for (int c = 0; c < 3; ++c) {
+        double a = F[c] / m;
+        v[c] += a * 0.5 * S.dt;
+    }
+    if (S.mode == MODE_FINAL_DRIFT) {
+        double r[3] = {pi.x, pi.y, pi.z};
+        for (int c = 0; c < 3; ++c) {
+            double a = F[c] / m;
+            v[c] = v[c] + a * S.dt * 0.5;
+            r[c] += v[c] * S.dt;
+        }
+        pos_next[i] = make_double4(r[0], r[1], r[2], pi.w);
+        const double4 q = rsave[i];
+        double dx = r[0] - q.x, dy = r[1] - q.y, dz = r[2] - q.z;
+        shortest3(s, dx, dy, dz);
+        if (sq3_exact(dx, dy, dz) > s.trig2) atomicMin(&misc[M_ABORT], S.step + 1);
+        if (!(fabs(r[0]) < 1e300 && fabs(r[1]) < 1e300 && fabs(r[2]) < 1e300)) atomicOr(&misc[M_ERR], EB_NONFINITE);
+    }
+    A.d[(size_t)D_VX * n + i] = v[0];
+    A.d[(size_t)D_VY * n + i] = v[1];
+    A.d[(size_t)D_VZ * n + i] = v[2];
+}
+
+__device__ inline bool skip_step(const StepArgs& S, const int* misc) {
+    return S.mode != MODE_COMPUTE && *(volatile const int*)&misc[M_ABORT] <= S.step;
+}
+
+}  // namespace mdc

@@ -32,7 +32,7 @@ EXPORTS = [
     "mdc_pot_bpnn_create", "mdc_pot_rcut", "mdc_pot_destroy",
     "mdc_sys_create", "mdc_sys_destroy", "mdc_sys_upload", "mdc_sys_download", "mdc_sys_natoms",
     "mdc_sys_set_lanes", "mdc_nlist_build", "mdc_nlist_decide", "mdc_nlist_export", "mdc_nlist_stats",
-    "mdc_compute", "mdc_step", "mdc_thermo", "mdc_sync", "mdc_timing",
+    "mdc_compute", "mdc_step", "mdc_thermo", "mdc_sync", "mdc_timing", "mdc_bpnn_inspect",
 ]
 
 
@@ -73,6 +73,7 @@ def load_library(path=LIB_PATH):
         "mdc_thermo": (I, [P, DP]),
         "mdc_sync": (I, [P]),
         "mdc_timing": (I, [P, I, DP, C.POINTER(C.c_int64)]),
+        "mdc_bpnn_inspect": (I, [P, P, DP, DP, C.POINTER(I)]),
     }
     for name in EXPORTS:
         fn = getattr(lib, name)
@@ -264,6 +265,15 @@ class System:
 
     def sync(self):
         _check(lib().mdc_sync(self.h))
+
+    def bpnn_inspect(self, pot):
+        """(G, dE/dG) of the last BPNN evaluation, [n, nin] each."""
+        nin = C.c_int()
+        _check(lib().mdc_bpnn_inspect(self.h, pot.h, None, None, C.byref(nin)))
+        G = np.zeros((self.n, nin.value))
+        dE = np.zeros((self.n, nin.value))
+        _check(lib().mdc_bpnn_inspect(self.h, pot.h, _dp(G), _dp(dE), None))
+        return G, dE
 
     def timing(self, enable=-1):
         ms = np.zeros(8)

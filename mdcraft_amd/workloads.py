@@ -157,11 +157,15 @@ def eam_analytic(r_nm, rhobar):
 # --------------------------------------------------------------------------
 # workloads
 # --------------------------------------------------------------------------
-def lj_argon(reps=10, seed_index=0, T=100.0):
+def lj_argon(reps=10, seed_index=0, T=100.0, rattle=0.0):
+    """rattle (A) displaces the lattice sites; the benchmark workload uses
+    none (forces vanish by symmetry at step 0), parity tests a small one."""
     a = 5.26 * ANG
     pos, _ = lattice(FCC, (reps,) * 3, a)
     n = len(pos)
     rng = Rng(structure_seed(1, seed_index))
+    if rattle:
+        pos = np.mod(pos + rng.normal(3 * n).reshape(n, 3) * rattle * ANG, reps * a)
     at = make_atoms(n)
     m = np.full(n, 39.948)
     set_vec(at, "r", pos)

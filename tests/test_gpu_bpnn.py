@@ -1,0 +1,75 @@
+"""BPNN on the engine against the oracle (FP64: 1e-12; mixed: FP32
+descriptors and an FP16-split MFMA network, force RMS within 1e-5)."""
+import numpy as np
+import pytest
+
+from mdcraft_amd import workloads as W
+from mdcraft_amd.atoms import vec
+from tests.helpers import force_rms_rel, max_rel, oracle_mod
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle(w, threads=8):
+    R = oracle_mod()
+    a = w["atoms"].copy()
+    dom = R.Domain(*w["box"])
+    pool = R.Pool(threads)
+    p = R.MaiseBPNN(w["pot"]["path"], dom)
+    a["rcut"] = p.rcut
+    nl = R.VerletList(pool)
+    nl.update(a, dom)
+    s = R.Solver(p, pool, True)
+    s.compute(a, dom, nl)
+    G, dE = R.bpnn_descriptors(p, len(a))
+    return a, G, dE, (s, nl, dom, p)
+
+
+def engine(ctx, w, precision, steps=0):
+    from mdcraft_amd import engine as E
+    s = E.System(ctx, [len(w["atoms"])], [w["box"]], lanes=16)
+    s.upload(w["atoms"])
+    p = E.MaiseBPNNPotential(ctx, w["pot"]["path"])
+    s.build()
+    s.compute(p, 7, precision)
+    if steps:
+        s.step(p, w["dt"], steps, precision)
+        s.compute(p, 7, precision)
+    G, dE = s.bpnn_inspect(p)
+    return s.download(), G, dE
+
+
+@pytest.fixture(scope="module")
+def small(tmp_path_factory):
+    return W.bpnn_cu(reps=4, model_dir=str(tmp_path_factory.mktemp("bp")))
+
+
+def test_bpnn_fp64_parity(ctx, small):
+    ref, G, dE, _ = oracle(small)
+    got, g, de = engine(ctx, small, 0)
+    assert max_rel(g, G) < 1e-12
+    assert max_rel(de, dE) < 1e-11
+    assert max_rel(got["Ep"], ref["Ep"]) < 1e-12
+    assert max_rel(vec(got, "f"), vec(ref, "f")) < 1e-12
+
+
+def test_bpnn_mixed_accuracy(ctx, small):
+    ref, G, dE, _ = oracle(small)
+    got, g, de = engine(ctx, small, 1)
+    assert max_rel(g, G) < 1e-5
+    assert max_rel(de, dE) < 1e-4
+    e, er = got["Ep"].sum(), ref["Ep"].sum()
+    assert abs(e - er) < 1e-5 * abs(er)
+    assert force_rms_rel(vec(got, "f"), vec(ref, "f")) < 1e-5
+
+
+def test_bpnn_fp64_short_nve(ctx, small):
+    ref, _, _, (s, nl, dom, p) = oracle(small)
+    s.run_nve(ref, dom, nl, small["dt"], 5, initial_forces=False)
+    s.compute(ref, dom, nl)
+    got, _, _ = engine(ctx, small, 0, steps=5)
+    L = small["box"][1] - small["box"][0]
+    d = vec(got, "r") - vec(ref, "r")
+    d -= np.round(d / L) * L
+    assert np.max(np.abs(d)) < 1e-11
+    assert max_rel(vec(got, "f"), vec(ref, "f")) < 1e-10

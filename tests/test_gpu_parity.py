@@ -56,7 +56,7 @@ def small_cases():
     rules: bulk, a box shorter than 2 rns (single image per pair), open
     boundaries, mixed periodicity and per-atom search radii."""
     cases = []
-    w = W.lj_argon(reps=4)
+    w = W.lj_argon(reps=4, rattle=0.05)
     cases.append(("lj_bulk_256", w["atoms"], w["box"], w["pot"]))
     w = W.eam_cu(reps=6)
     cases.append(("eam_bulk_864", w["atoms"], w["box"], w["pot"]))
@@ -141,7 +141,7 @@ def test_forces_energy_virial(ctx, case, precision):
 @pytest.mark.parametrize("which", ["lj", "eam"])
 @pytest.mark.parametrize("precision", [0, 1])
 def test_full_size_workloads(ctx, which, precision):
-    w = W.lj_argon() if which == "lj" else W.eam_cu()
+    w = W.lj_argon(rattle=0.05) if which == "lj" else W.eam_cu()
     ref, _ = oracle_compute(w["atoms"], w["box"], w["pot"], threads=8)
     got, _, _ = run_engine(ctx, w["atoms"], w["box"], w["pot"], precision)
     compare(got, ref, precision, which)
