@@ -74,20 +74,26 @@ def run_gpu(E, ctx, w, precision, steps, warmup, lanes=None):
     if warmup:
         s.step(p, w["dt"], warmup, precision)
     s.sync()
-    s.timing(1)
     t0 = time.perf_counter()
     reb = s.step(p, w["dt"], steps, precision)
     s.sync()
     t1 = time.perf_counter()
-    ms, nl = s.timing(0)
     st = s.stats()
+    # kernel durations from hipEvents on the launch stream, in a separate
+    # run so the events do not perturb the timed region
+    s.timing(1)
+    s.step(p, w["dt"], min(steps, 200), precision)
+    s.sync()
+    ms, nl = s.timing(0)
     s.compute(p, E.FORCE | E.ENERGY | E.VIRIAL, precision)
     e1 = s.thermo()[0]
     n = len(at)
     t_ps = (steps + warmup) * w["dt"]
     drift = (e1[3] - e0[3]) / n / EV_KJMOL / (t_ps * 1e-3)  # eV/atom/ns
+    kinds = ["force", "density", "predrift", "bp_desc", "bp_mlp", "bp_force", "bp_gather", "rebuild"]
+    per_kernel = {kinds[k]: round(ms[k] / nl[k] * 1e3, 2) for k in range(8) if nl[k] > 0}
     return {
-        "n": n, "steps": steps, "seconds": t1 - t0, "value": n * steps / (t1 - t0),
+        "n": n, "steps": steps, "seconds": t1 - t0, "value": n * steps / (t1 - t0), "kernel_us": per_kernel,
         "ms_per_step": 1e3 * (t1 - t0) / steps, "kernel_ms": ms.tolist(), "kernel_launches": nl.tolist(),
         "nbar": st["mean"], "rebuilds": reb, "drift_eV_atom_ns": drift, "lanes": lanes,
     }
@@ -154,16 +160,15 @@ def main():
             "algorithmic_bytes_per_launch": int(per_launch)}
     k = args.steps / max(head["rebuilds"], 1)
     step_gbs = step_bytes("eam", nbar, k) * head["value"] / 1e9
-    extra = {"eam_cu_mixed": {kk: head[kk] for kk in ("value", "ms_per_step", "nbar", "rebuilds",
-                                                      "drift_eV_atom_ns")},
+    keys = ("value", "ms_per_step", "nbar", "rebuilds", "drift_eV_atom_ns", "kernel_us")
+    extra = {"eam_cu_mixed": {kk: head[kk] for kk in keys},
              "eam_cu_mixed_step_GBs": round(step_gbs, 1)}
     if not args.no_extra:
         r = run_gpu(E, ctx, w, E.FP64, args.steps, args.warmup, args.lanes)
-        extra["eam_cu_fp64"] = {kk: r[kk] for kk in ("value", "ms_per_step", "nbar", "rebuilds", "drift_eV_atom_ns")}
+        extra["eam_cu_fp64"] = {kk: r[kk] for kk in keys}
         wl = W.lj_argon()
         r = run_gpu(E, ctx, wl, E.FP64, 1000, args.warmup, args.lanes)
-        extra["lj_argon_fp64"] = {kk: r[kk] for kk in ("value", "ms_per_step", "nbar", "rebuilds",
-                                                       "drift_eV_atom_ns")}
+        extra["lj_argon_fp64"] = {kk: r[kk] for kk in keys}
     cpu = None if args.no_cpu else cpu_baseline(w, max_steps=500)
     line = {
         "metric": METRIC, "value": round(head["value"], 1), "unit": "atom-steps/s", "n_gpus": 1,

@@ -636,6 +636,7 @@ mdc_status mdc_sys_create(mdc_ctx* ctx, int nsys, const uint64_t* natoms, const 
             if ((uint64_t)b.atom_off[i + 1] > (1ull << 30)) throw Error(E_INVAL, "too many atoms");
         }
         b.n = b.atom_off[nsys];
+        if (b.n >= kMaxAtomsShift) throw Error(E_INVAL, "at most 2^26 - 1 atoms per batch");
         b.desc.resize(nsys);
         b.periodic_bc.assign(nsys, periodic_boundary ? 1 : 0);
         for (int i = 0; i < nsys; ++i) {
@@ -802,7 +803,7 @@ mdc_status mdc_nlist_export(mdc_sys* sys, uint64_t* offsets, uint64_t* ids, uint
             const int p = where[a];
             ids[o++] = (uint64_t)a;  // the reference's list includes self
             const int c = std::min(nn[p], b.cap);
-            for (int k = 0; k < c; ++k) ids[o++] = (uint64_t)orig[nl[list_index(p, k, b.tpa, b.M)]];
+            for (int k = 0; k < c; ++k) ids[o++] = (uint64_t)orig[ent_j(nl[list_index(p, k, b.tpa, b.M)])];
         }
         offsets[n] = o;
     });

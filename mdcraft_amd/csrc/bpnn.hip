@@ -147,7 +147,7 @@ __device__ int bp_stage(int i, const double4& pi, const SysDesc& s, const int* _
         bool ok = false;
         double dx = 0, dy = 0, dz = 0, d2 = 0;
         if (k < cnt) {
-            const int j = nl[list_index(i, k, tpa, M)];
+            const int j = ent_j(nl[list_index(i, k, tpa, M)]);
             const double4 pj = pos[j];
             dx = pj.x - pi.x;
             dy = pj.y - pi.y;
@@ -510,11 +510,11 @@ __global__ void k_bp_rev(const int* __restrict__ nl, const int* __restrict__ nn,
     if (m >= n) return;
     const int cm = min(nn[m], cap);
     for (int k = 0; k < cm; ++k) {
-        const int i = nl[list_index(m, k, tpa, M)];
+        const int i = ent_j(nl[list_index(m, k, tpa, M)]);
         const int ci = min(nn[i], cap);
         int r = -1;
         for (int q = 0; q < ci; ++q)
-            if (nl[list_index(i, q, tpa, M)] == m) {
+            if (ent_j(nl[list_index(i, q, tpa, M)]) == m) {
                 r = q;
                 break;
             }
@@ -533,6 +533,7 @@ __global__ void __launch_bounds__(256) k_bp_gather(const double4* __restrict__ p
     if (skip_step(SA, misc)) return;
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= A.n) return;
+    const FinIn fin = load_fin(m, A, SA, rsave, true);
     const double4 pm = pos[m];
     if (w_inactive(pm.w)) {
         if (SA.mode == MODE_FINAL_DRIFT) pos_next[m] = pm;
@@ -542,7 +543,7 @@ __global__ void __launch_bounds__(256) k_bp_gather(const double4* __restrict__ p
     const int cm = min(nn[m], cap);
     for (int k = 0; k < cm; ++k) {
         const long long li = list_index(m, k, tpa, M);
-        const int i = nl[li];
+        const int i = ent_j(nl[li]);
         const int r = rev[li];
         if (r < 0) continue;
         const R* q = Pin + ((size_t)i * cap + r) * 3;
@@ -551,7 +552,7 @@ __global__ void __launch_bounds__(256) k_bp_gather(const double4* __restrict__ p
         fz += q[2];
     }
     const SysDesc& s = desc[w_sys(pm.w)];
-    finalize_atom<R>(m, pm, (double)fx, (double)fy, (double)fz, A, SA, pos_next, rsave, s, misc);
+    finalize_atom<R>(m, pm, (double)fx, (double)fy, (double)fz, A, SA, pos_next, fin, s, misc);
 }
 
 // --------------------------------------------------------------------------

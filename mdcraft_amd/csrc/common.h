@@ -34,7 +34,42 @@ struct SysDesc {
     int atom_off;
     int natoms;
     int any_per;
+    // 1 when every periodic edge exceeds 2 (rns + skin): the minimum image
+    // of a listed pair is then unique and fixed between rebuilds, and the
+    // image shift stored in the list entry gives the reference's image rule
+    int shift_ok;
+    int pad_;
 };
+
+// A list entry holds the neighbour index in bits 0-25 and, per axis, the
+// image shift (-1, 0, +1) + 1 of r_j - r_i in bits 26-27, 28-29, 30-31.
+constexpr unsigned kIdxMask = (1u << 26) - 1u;
+constexpr int kMaxAtomsShift = 1 << 26;
+__host__ __device__ inline int ent_j(int e) { return (int)((unsigned)e & kIdxMask); }
+__host__ __device__ inline int ent_s(int e, int axis) { return (int)(((unsigned)e >> (26 + 2 * axis)) & 3u) - 1; }
+
+// raw buffer loads: 32-bit offsets, out-of-range reads return zero
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
+typedef double d2_t __attribute__((ext_vector_type(2)));
+__device__ inline __amdgpu_buffer_rsrc_t make_buf(const void* p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ inline double4 buf_d4(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    const d2_t a = __builtin_bit_cast(d2_t, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+    const d2_t b = __builtin_bit_cast(d2_t, __builtin_amdgcn_raw_buffer_load_b128(r, off + 16, 0, 0));
+    return make_double4(a[0], a[1], b[0], b[1]);
+}
+// The whole 128-bit vector is reinterpreted at once: per-element casts of
+// the loaded vector have been narrowed to a single dword by the compiler.
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+__device__ inline float4 buf_f4(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    const f32x4_t a = __builtin_bit_cast(f32x4_t, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+    return make_float4(a[0], a[1], a[2], a[3]);
+}
+__device__ inline double buf_d1(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
 
 // tag word packed into pos.w
 __host__ __device__ inline double pack_w(int kind, int inactive, int sys) {

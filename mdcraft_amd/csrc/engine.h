@@ -112,6 +112,10 @@ struct Batch {
     int f_prec = -1;      // which force array holds the current force
     double last_rcut = -1;
     int rebuilds = 0;
+    // steps between rebuilds (running estimate) and steps since the last one,
+    // used to size the launch chunks of a step sequence
+    double pred_interval = 0.0;
+    int since_rebuild = 0;
     // host copies of the integer fields the device does not need
     std::vector<uint16_t> kind, tag;
     std::vector<uint32_t> uid;
@@ -150,7 +154,7 @@ int nlist_decide(Batch& b, double rcut);
 void force_compute(Batch& b, const Pot& p, int what, int prec);
 void run_steps(Batch& b, const Pot& p, double dt, int nsteps, int prec, int chunk);
 void thermo(Batch& b, double* out);
-void set_trigger(Batch& b, double rcut);
+void set_trigger(Batch& b, double rcut, bool allow_shift);
 void check_errors(Batch& b);
 // launchers (bpnn.hip)
 void bpnn_compute(Batch& b, const Pot& p, int what, int prec, int mode, double dt, int step, const double4* pos,

@@ -34,25 +34,26 @@ __global__ void __launch_bounds__(256) k_lj(const double4* __restrict__ pos, dou
                                             const int* __restrict__ nl, const int* __restrict__ nn, int M, int cap,
                                             const SysDesc* __restrict__ desc, LJK P, AtomArrays A, StepArgs S,
                                             const double4* __restrict__ rsave, int* __restrict__ misc) {
-    if (skip_step(S, misc)) return;
     const AtomSlots as = atom_slots<TPA>(M);
     const int i = as.i;
     const bool valid = i < A.n;
+    const int ab = abort_flag(S, misc);
     const double4 pi = valid ? pos[i] : make_double4(0, 0, 0, 0);
+    const int nraw = valid ? nn[i] : 0;
+    int j0[4];
+    first_group<TPA, 4>(as, valid, M, nl, j0);
+    if (ab <= S.step) return;
     const bool active = valid && !w_inactive(pi.w);
     const SysDesc& s = desc[valid ? w_sys(pi.w) : 0];
-    const int cnt = active ? min(nn[i], cap) : 0;
+    const int cnt = active ? min(nraw, cap) : 0;
     const R c_pref = (R)(-0.5 * P.factor), c_lj1 = (R)P.lj1, c_lj2 = (R)P.lj2, c_s6 = (R)P.s6, c_e = (R)(0.5 * 4 * P.a);
     R fx = 0, fy = 0, fz = 0;
     double e = 0, vxx = 0, vyy = 0, vzz = 0, vxy = 0, vxz = 0, vyz = 0;
-    for (int m = 0;; ++m) {
-        const int k = m * TPA + as.sub;
-        if (k >= cnt) break;
-        const int j = nl[as.base + (long long)m * kWave];
-        const double4 pj = pos[j];
-        double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z, d2;
-        if (!pair_image(s, dx, dy, dz, d2)) continue;
-        if (d2 > P.rc2 || d2 < 1e-15) continue;
+    const __amdgpu_buffer_rsrc_t posb = make_buf(pos, (unsigned)A.n * 32u);
+    for_neighbours<TPA, 4>(as, cnt, nl, posb, j0, [&](int ent, const double4& pj) {
+        double dx, dy, dz, d2;
+        if (!pair_delta(s, pi, pj, ent, dx, dy, dz, d2)) return;
+        if (d2 > P.rc2 || d2 < 1e-15) return;
         const R r2 = (R)d2, rx = (R)dx, ry = (R)dy, rz = (R)dz;
         const R r2inv = rcp_(r2);
         const R r6inv = r2inv * r2inv * r2inv;
@@ -73,7 +74,7 @@ __global__ void __launch_bounds__(256) k_lj(const double4* __restrict__ pos, dou
             vxz += px * (double)rz;
             vyz += py * (double)rz;
         }
-    }
+    });
     fx = wave_sum_group(fx, TPA);
     fy = wave_sum_group(fy, TPA);
     fz = wave_sum_group(fz, TPA);
@@ -101,7 +102,8 @@ __global__ void __launch_bounds__(256) k_lj(const double4* __restrict__ pos, dou
         A.d[(size_t)D_VXZ * n + i] = 0.5 * vxz;
         A.d[(size_t)D_VYZ * n + i] = 0.5 * vyz;
     }
-    finalize_atom<R>(i, pi, (double)fx, (double)fy, (double)fz, A, S, pos_next, rsave, s, misc);
+    const FinIn fin = load_fin(i, A, S, rsave, true);
+    finalize_atom<R>(i, pi, (double)fx, (double)fy, (double)fz, A, S, pos_next, fin, s, misc);
 }
 
 // --------------------------------------------------------------------------
@@ -109,12 +111,19 @@ __global__ void __launch_bounds__(256) k_lj(const double4* __restrict__ pos, dou
 // symmetric pair table)
 // --------------------------------------------------------------------------
 struct EamK {
-    int nk, nr, nrho;
+    int nk, nr, nrho, ntab;
     double dr, dr1, drho, drho1, rc2;
     const double4* r64;
     const float4* r32;
     const double4* F64;
 };
+
+template <typename R>
+__device__ inline typename Vec4T<R>::T load_rec(__amdgpu_buffer_rsrc_t b, int rec);
+template <>
+__device__ inline double4 load_rec<double>(__amdgpu_buffer_rsrc_t b, int rec) { return buf_d4(b, (unsigned)rec * 32u); }
+template <>
+__device__ inline float4 load_rec<float>(__amdgpu_buffer_rsrc_t b, int rec) { return buf_f4(b, (unsigned)rec * 16u); }
 
 template <typename R>
 __device__ inline const typename Vec4T<R>::T* rtab(const EamK& P);
@@ -150,36 +159,82 @@ __device__ inline void embed(const EamK& P, int kind, double rho, double& F, dou
     dF = c.y + 2.0 * DV * (1.5 * aa - bb);
 }
 
+// Pairs of one lane are processed U at a time in predicated phases
+// (distances, then every table and F' read, then arithmetic), so the U
+// dependent table reads are in flight together.
+constexpr int kU = 4;
+
+template <typename R>
+struct PairGeo {
+    double dx, dy, dz;
+    R R1, RN, R2;
+    int N1;
+    bool ok;
+};
+
+template <typename R>
+__device__ inline PairGeo<R> pair_geo(const SysDesc& s, const double4& pi, const double4& pj, int e, bool valid,
+                                      double rc2, R dr1, int nr) {
+    PairGeo<R> g;
+    double d2;
+    g.ok = valid && pair_delta(s, pi, pj, e, g.dx, g.dy, g.dz, d2) && !(d2 > rc2 || d2 < 1e-15);
+    g.R2 = g.ok ? (R)d2 : (R)1;
+    g.R1 = sqrt(g.R2);
+    g.RN = g.R1 * dr1;
+    g.N1 = g.ok ? (int)g.RN : 0;
+    if (g.N1 >= nr) {
+        g.ok = false;
+        g.N1 = 0;
+    }
+    return g;
+}
+
 template <typename R, int TPA>
 __global__ void __launch_bounds__(256) k_eam_rho(const double4* __restrict__ pos, const int* __restrict__ nl,
                                                  const int* __restrict__ nn, int M, int cap,
                                                  const SysDesc* __restrict__ desc, EamK P, AtomArrays A, StepArgs S,
                                                  int* __restrict__ misc) {
-    if (skip_step(S, misc)) return;
     const AtomSlots as = atom_slots<TPA>(M);
     const int i = as.i;
     const bool valid = i < A.n;
+    const int ab = abort_flag(S, misc);
     const double4 pi = valid ? pos[i] : make_double4(0, 0, 0, 0);
+    const int nraw = valid ? nn[i] : 0;
+    int j0[kU];
+    first_group<TPA, kU>(as, valid, M, nl, j0);
+    if (ab <= S.step) return;
     const bool active = valid && !w_inactive(pi.w);
     const SysDesc& s = desc[valid ? w_sys(pi.w) : 0];
-    const int cnt = active ? min(nn[i], cap) : 0;
+    const int cnt = active ? min(nraw, cap) : 0;
     const auto* tab = rtab<R>(P);
     const R dr = (R)P.dr, dr1 = (R)P.dr1;
     double rho = 0.0;
-    for (int m = 0;; ++m) {
-        const int k = m * TPA + as.sub;
-        if (k >= cnt) break;
-        const int j = nl[as.base + (long long)m * kWave];
-        const double4 pj = pos[j];
-        double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z, d2;
-        if (!pair_image(s, dx, dy, dz, d2)) continue;
-        if (d2 > P.rc2 || d2 < 1e-15) continue;
-        const R R1 = sqrt((R)d2);
-        const R RN = R1 * dr1;
-        const int N1 = (int)RN;
-        if (N1 >= P.nr) continue;
-        const int kj = P.nk > 1 ? w_kind(pj.w) : 0;
-        rho += (double)spline_v<R>(tab[(size_t)kj * P.nr + N1], R1, RN, N1, dr, dr1);
+    const __amdgpu_buffer_rsrc_t posb = make_buf(pos, (unsigned)A.n * 32u);
+    const __amdgpu_buffer_rsrc_t tabb = make_buf(tab, (unsigned)(P.ntab * P.nr) * (unsigned)sizeof(*tab));
+    for (int m0 = 0; m0 * TPA + as.sub < cnt; m0 += kU) {
+        int e[kU];
+        bool okk[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int k = (m0 + u) * TPA + as.sub;
+            okk[u] = k < cnt;
+            e[u] = okk[u] ? (m0 == 0 ? j0[u] : nl[as.base + (long long)(m0 + u) * kWave]) : 0;
+        }
+        double4 pj[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) pj[u] = buf_d4(posb, (unsigned)ent_j(e[u]) * 32u);
+        PairGeo<R> g[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) g[u] = pair_geo<R>(s, pi, pj[u], e[u], okk[u], P.rc2, dr1, P.nr);
+        typename Vec4T<R>::T rec[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int kj = P.nk > 1 ? w_kind(pj[u].w) : 0;
+            rec[u] = load_rec<R>(tabb, kj * P.nr + g[u].N1);
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+            if (g[u].ok) rho += (double)spline_v<R>(rec[u], g[u].R1, g[u].RN, g[u].N1, dr, dr1);
     }
     rho = wave_sum_group(rho, TPA);
     if (!valid || as.sub != 0) return;
@@ -202,79 +257,107 @@ __global__ void __launch_bounds__(256) k_eam_rho(const double4* __restrict__ pos
     }
 }
 
-template <typename R, int TPA>
+template <typename R, int TPA, bool ALLOY>
 __global__ void __launch_bounds__(256) k_eam_force(const double4* __restrict__ pos, double4* __restrict__ pos_next,
                                                    const int* __restrict__ nl, const int* __restrict__ nn, int M,
                                                    int cap, const SysDesc* __restrict__ desc, EamK P, AtomArrays A,
                                                    StepArgs S, const double4* __restrict__ rsave,
                                                    int* __restrict__ misc) {
-    if (skip_step(S, misc)) return;
     const AtomSlots as = atom_slots<TPA>(M);
     const int i = as.i;
     const bool valid = i < A.n;
-    const double4 pi = valid ? pos[i] : make_double4(0, 0, 0, 0);
-    const bool active = valid && !w_inactive(pi.w);
-    const SysDesc& s = desc[valid ? w_sys(pi.w) : 0];
-    const int cnt = active ? min(nn[i], cap) : 0;
     const int n = A.n;
     const double* __restrict__ Fp = A.d + (size_t)D_FP * n;
-    const double Fpi = active ? Fp[i] : 0.0;
+    const int ab = abort_flag(S, misc);
+    const double4 pi = valid ? pos[i] : make_double4(0, 0, 0, 0);
+    const int nraw = valid ? nn[i] : 0;
+    const double Fpi0 = valid ? Fp[i] : 0.0;
+    int j0[kU];
+    first_group<TPA, kU>(as, valid, M, nl, j0);
+    if (ab <= S.step) return;
+    const bool active = valid && !w_inactive(pi.w);
+    const SysDesc& s = desc[valid ? w_sys(pi.w) : 0];
+    const int cnt = active ? min(nraw, cap) : 0;
+    const double Fpi = active ? Fpi0 : 0.0;
     const auto* tab = rtab<R>(P);
     const R dr = (R)P.dr, dr1 = (R)P.dr1;
     const int ki = P.nk > 1 ? w_kind(pi.w) : 0;
     R fx = 0, fy = 0, fz = 0;
-    double e = 0, vxx = 0, vyy = 0, vzz = 0, vxy = 0, vxz = 0, vyz = 0;
-    for (int m = 0;; ++m) {
-        const int k = m * TPA + as.sub;
-        if (k >= cnt) break;
-        const int j = nl[as.base + (long long)m * kWave];
-        const double4 pj = pos[j];
-        double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z, d2;
-        if (!pair_image(s, dx, dy, dz, d2)) continue;
-        if (d2 > P.rc2 || d2 < 1e-15) continue;
-        const R R2 = (R)d2;
-        const R R1 = sqrt(R2);
-        const R RN = R1 * dr1;
-        const int N1 = (int)RN;
-        R FF = 0, VR = 0;
-        if (N1 < P.nr) {
-            const R Fpj = (R)Fp[j];
-            R nd, rv, phv, phd;
-            if (P.nk == 1) {
-                spline_vd<R>(tab[N1], R1, RN, N1, dr, dr1, rv, nd);
-                spline_vd<R>(tab[(size_t)P.nr + N1], R1, RN, N1, dr, dr1, phv, phd);
-                const R AdF = (R)Fpi + Fpj;
-                FF = (AdF * nd + (phd * R1 - phv) / R2) / R1;
-            } else {
-                const int kj = w_kind(pj.w);
-                R ndj, ndi;
-                spline_vd<R>(tab[(size_t)kj * P.nr + N1], R1, RN, N1, dr, dr1, rv, ndj);
-                spline_vd<R>(tab[(size_t)ki * P.nr + N1], R1, RN, N1, dr, dr1, rv, ndi);
-                spline_vd<R>(tab[(size_t)pair_tab(P.nk, ki, kj) * P.nr + N1], R1, RN, N1, dr, dr1, phv, phd);
-                FF = ((R)Fpi * ndj + Fpj * ndi + (phd * R1 - phv) / R2) / R1;
-            }
-            VR = phv;
-            if (S.flags & W_ENERGY) e += 0.5 * (double)(VR / R1);
+    double en = 0, vxx = 0, vyy = 0, vzz = 0, vxy = 0, vxz = 0, vyz = 0;
+    const __amdgpu_buffer_rsrc_t posb = make_buf(pos, (unsigned)n * 32u);
+    const __amdgpu_buffer_rsrc_t fpb = make_buf(Fp, (unsigned)n * 8u);
+    const __amdgpu_buffer_rsrc_t tabb = make_buf(tab, (unsigned)(P.ntab * P.nr) * (unsigned)sizeof(*tab));
+    for (int m0 = 0; m0 * TPA + as.sub < cnt; m0 += kU) {
+        int e[kU];
+        bool okk[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int k = (m0 + u) * TPA + as.sub;
+            okk[u] = k < cnt;
+            e[u] = okk[u] ? (m0 == 0 ? j0[u] : nl[as.base + (long long)(m0 + u) * kWave]) : 0;
         }
-        const R rx = (R)dx, ry = (R)dy, rz = (R)dz;
-        fx += FF * rx;
-        fy += FF * ry;
-        fz += FF * rz;
-        if (S.flags & W_VIRIAL) {
-            const double h = 0.5 * (double)FF;
-            const double ax = h * dx, ay = h * dy, az = h * dz;
-            vxx += ax * dx;
-            vyy += ay * dy;
-            vzz += az * dz;
-            vxy += ax * dy;
-            vxz += ax * dz;
-            vyz += ay * dz;
+        double4 pj[kU];
+        R Fpj[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            pj[u] = buf_d4(posb, (unsigned)ent_j(e[u]) * 32u);
+            Fpj[u] = (R)buf_d1(fpb, (unsigned)ent_j(e[u]) * 8u);
+        }
+        PairGeo<R> g[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) g[u] = pair_geo<R>(s, pi, pj[u], e[u], okk[u], P.rc2, dr1, P.nr);
+        typename Vec4T<R>::T rr[kU], rp[kU], rq[ALLOY ? kU : 1];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            if (!ALLOY) {
+                rr[u] = load_rec<R>(tabb, g[u].N1);
+                rp[u] = load_rec<R>(tabb, P.nr + g[u].N1);
+            } else {
+                const int kj = w_kind(pj[u].w);
+                rr[u] = load_rec<R>(tabb, kj * P.nr + g[u].N1);
+                rq[ALLOY ? u : 0] = load_rec<R>(tabb, ki * P.nr + g[u].N1);
+                rp[u] = load_rec<R>(tabb, pair_tab(P.nk, ki, kj) * P.nr + g[u].N1);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            if (!g[u].ok) continue;
+            const R R1 = g[u].R1, RN = g[u].RN, R2 = g[u].R2;
+            const int N1 = g[u].N1;
+            R nd, rv, phv, phd, FF;
+            spline_vd<R>(rp[u], R1, RN, N1, dr, dr1, phv, phd);
+            spline_vd<R>(rr[u], R1, RN, N1, dr, dr1, rv, nd);
+            // FF = (AdF rho'(r) + phi'(r)) / r with phi' = ((r phi)' r - r phi) / r^2
+            const R i1 = rcp_(R1), i2 = rcp_(R2);
+            if (!ALLOY) {
+                const R AdF = (R)Fpi + Fpj[u];
+                FF = (AdF * nd + (phd * R1 - phv) * i2) * i1;
+            } else {
+                R ndi;
+                spline_vd<R>(rq[ALLOY ? u : 0], R1, RN, N1, dr, dr1, rv, ndi);
+                FF = ((R)Fpi * nd + Fpj[u] * ndi + (phd * R1 - phv) * i2) * i1;
+            }
+            if (S.flags & W_ENERGY) en += 0.5 * (double)(phv * i1);
+            const R rx = (R)g[u].dx, ry = (R)g[u].dy, rz = (R)g[u].dz;
+            fx += FF * rx;
+            fy += FF * ry;
+            fz += FF * rz;
+            if (S.flags & W_VIRIAL) {
+                const double h = 0.5 * (double)FF;
+                const double ax = h * g[u].dx, ay = h * g[u].dy, az = h * g[u].dz;
+                vxx += ax * g[u].dx;
+                vyy += ay * g[u].dy;
+                vzz += az * g[u].dz;
+                vxy += ax * g[u].dy;
+                vxz += ax * g[u].dz;
+                vyz += ay * g[u].dz;
+            }
         }
     }
     fx = wave_sum_group(fx, TPA);
     fy = wave_sum_group(fy, TPA);
     fz = wave_sum_group(fz, TPA);
-    if (S.flags & W_ENERGY) e = wave_sum_group(e, TPA);
+    if (S.flags & W_ENERGY) en = wave_sum_group(en, TPA);
     if (S.flags & W_VIRIAL) {
         vxx = wave_sum_group(vxx, TPA);
         vyy = wave_sum_group(vyy, TPA);
@@ -288,7 +371,7 @@ __global__ void __launch_bounds__(256) k_eam_force(const double4* __restrict__ p
         if (S.mode == MODE_FINAL_DRIFT) pos_next[i] = pi;
         return;
     }
-    if (S.flags & W_ENERGY) A.d[(size_t)D_EP * n + i] = e;
+    if (S.flags & W_ENERGY) A.d[(size_t)D_EP * n + i] = en;
     if (S.flags & W_VIRIAL) {
         A.d[(size_t)D_VXX * n + i] = vxx;
         A.d[(size_t)D_VYY * n + i] = vyy;
@@ -297,7 +380,8 @@ __global__ void __launch_bounds__(256) k_eam_force(const double4* __restrict__ p
         A.d[(size_t)D_VXZ * n + i] = vxz;
         A.d[(size_t)D_VYZ * n + i] = vyz;
     }
-    finalize_atom<R>(i, pi, (double)fx, (double)fy, (double)fz, A, S, pos_next, rsave, s, misc);
+    const FinIn fin = load_fin(i, A, S, rsave, true);
+    finalize_atom<R>(i, pi, (double)fx, (double)fy, (double)fz, A, S, pos_next, fin, s, misc);
 }
 
 // Opening half kick + drift of the first step of a sequence from the stored
@@ -439,13 +523,22 @@ void timer_drain(Batch& b) {
     b.ev_pending.clear();
 }
 
-void set_trigger(Batch& b, double rcut) {
+void set_trigger(Batch& b, double rcut, bool allow_shift) {
     bool changed = false;
     for (int s = 0; s < b.nsys; ++s) {
+        SysDesc& d = b.desc[s];
         double skin = b.rns_min[s] - rcut;
         double t = skin > 0.0 ? 0.25 * skin * skin : -1.0;
-        if (b.desc[s].trig2 != t) {
-            b.desc[s].trig2 = t;
+        // within a step sequence every atom stays within skin/2 of its
+        // position at the last build, so listed pairs stay within
+        // rns_max + skin; their minimum image is unique while that is below
+        // half of every periodic edge
+        int sh = allow_shift && skin > 0.0 ? 1 : 0;
+        for (int a = 0; a < 3; ++a)
+            if (d.per[a] && !(0.5 * d.L[a] > b.rns_max[s] + skin)) sh = 0;
+        if (d.trig2 != t || d.shift_ok != sh) {
+            d.trig2 = t;
+            d.shift_ok = sh;
             changed = true;
         }
     }
@@ -459,6 +552,7 @@ static EamK eam_args(const Pot& p) {
     k.nk = p.eam.nk;
     k.nr = p.eam.nr;
     k.nrho = p.eam.nrho;
+    k.ntab = p.eam.nk + p.eam.nk * (p.eam.nk + 1) / 2;
     k.dr = p.eam.dr;
     k.dr1 = 1.0 / p.eam.dr;
     k.drho = p.eam.drho;
@@ -493,8 +587,12 @@ static void launch_pair_t(Batch& b, const Pot& p, const StepArgs& S, const doubl
                            AA, S, b.misc);
         timer_end(b, t0, K_DENSITY);
         hipEvent_t t1 = timer_begin(b);
-        hipLaunchKernelGGL((k_eam_force<R, TPA>), dim3(grid), dim3(256), 0, st, pos, pos_next, b.nl, b.nn, b.M, b.cap,
-                           b.d_desc, E, AA, S, b.rsave, b.misc);
+        if (E.nk > 1)
+            hipLaunchKernelGGL((k_eam_force<R, TPA, true>), dim3(grid), dim3(256), 0, st, pos, pos_next, b.nl, b.nn,
+                               b.M, b.cap, b.d_desc, E, AA, S, b.rsave, b.misc);
+        else
+            hipLaunchKernelGGL((k_eam_force<R, TPA, false>), dim3(grid), dim3(256), 0, st, pos, pos_next, b.nl, b.nn,
+                               b.M, b.cap, b.d_desc, E, AA, S, b.rsave, b.misc);
         timer_end(b, t1, K_FORCE);
     }
 }
@@ -539,7 +637,7 @@ static void read_misc(Batch& b) {
 
 void force_compute(Batch& b, const Pot& p, int what, int prec) {
     if (b.n == 0) return;
-    set_trigger(b, p.rcut);
+    set_trigger(b, p.rcut, false);
     if (!b.list_valid) nlist_rebuild(b);
     StepArgs S{0.0, -1, MODE_COMPUTE, what & (W_ENERGY | W_VIRIAL), 1};
     // the reference's prepare zeroes Ep, Em and V and only virials() fills
@@ -559,7 +657,7 @@ void force_compute(Batch& b, const Pot& p, int what, int prec) {
 void run_steps(Batch& b, const Pot& p, double dt, int nsteps, int prec, int chunk) {
     if (b.n == 0 || nsteps <= 0) return;
     hipStream_t st = b.ctx->stream;
-    set_trigger(b, p.rcut);
+    set_trigger(b, p.rcut, true);
     if (!b.list_valid) nlist_rebuild(b);
     if (b.f_prec < 0) throw Error(E_STATE, "no force to start from: compute forces once before stepping");
     const int grid = nblk(b.n, 256);
@@ -576,7 +674,16 @@ void run_steps(Batch& b, const Pot& p, double dt, int nsteps, int prec, int chun
     auto buf = [&](int t) { return c0 ^ 1 ^ (t & 1); };
     int s = 0;
     while (s < nsteps) {
-        const int e = std::min(nsteps, s + chunk);
+        // Launch up to the predicted next rebuild in one chunk, then in
+        // short chunks: kernels after a raised trigger exit at once, and each
+        // chunk ends with one host read of the trigger.
+        int c = chunk;
+        if (b.pred_interval > 0) {
+            const int left = (int)(0.8 * b.pred_interval) - b.since_rebuild;
+            c = left > 2 ? left : 2;
+        }
+        c = std::max(1, std::min(c, 64));
+        const int e = std::min(nsteps, s + c);
         for (int t = s; t < e; ++t) {
             const bool last = t == nsteps - 1;
             StepArgs S{dt, t, last ? MODE_FINAL : MODE_FINAL_DRIFT, 0, last ? 1 : 0};
@@ -587,12 +694,17 @@ void run_steps(Batch& b, const Pot& p, double dt, int nsteps, int prec, int chun
         const int ab = b.ctx->h_pin[M_ABORT];
         if (ab < e) {
             // steps [s, ab) ran; positions of step ab wait in buf(ab)
+            b.since_rebuild += ab - s;
+            b.pred_interval = b.pred_interval > 0 ? 0.5 * b.pred_interval + 0.5 * b.since_rebuild
+                                                  : (double)b.since_rebuild;
+            b.since_rebuild = 0;
             b.cur = buf(ab);
             nlist_rebuild(b);
             c0 = b.cur ^ 1 ^ (ab & 1);
             HIPCHECK(hipMemcpyAsync(b.misc + M_ABORT, &abort_init, sizeof(int), hipMemcpyHostToDevice, st));
             s = ab;
         } else {
+            b.since_rebuild += e - s;
             s = e;
         }
     }

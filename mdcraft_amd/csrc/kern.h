@@ -79,6 +79,32 @@ struct AtomArrays {
     int n;
 };
 
+// Per-atom inputs of the velocity-Verlet finish, loaded in the kernel
+// prologue so their latency overlaps the neighbour loop.
+struct FinIn {
+    double v[3];
+    double m;
+    double4 rs;
+};
+
+__device__ inline FinIn load_fin(int i, const AtomArrays& A, const StepArgs& S, const double4* __restrict__ rsave,
+                                 bool need) {
+    FinIn f;
+    if (need && S.mode != MODE_COMPUTE) {
+        const int n = A.n;
+        f.v[0] = A.d[(size_t)D_VX * n + i];
+        f.v[1] = A.d[(size_t)D_VY * n + i];
+        f.v[2] = A.d[(size_t)D_VZ * n + i];
+        f.m = A.d[(size_t)D_M * n + i];
+        f.rs = S.mode == MODE_FINAL_DRIFT ? rsave[i] : make_double4(0, 0, 0, 0);
+    } else {
+        f.v[0] = f.v[1] = f.v[2] = 0.0;
+        f.m = 1.0;
+        f.rs = make_double4(0, 0, 0, 0);
+    }
+    return f;
+}
+
 // Finalise one atom after its force is known: store force, and for step
 // modes apply the closing half kick (verlet.cxx:80-85) and, when another
 // step follows, the opening half kick and drift (verlet.cxx:71-78) into the
@@ -86,8 +112,8 @@ struct AtomArrays {
 // verlet-list.cxx:149-154.
 template <typename R>
 __device__ inline void finalize_atom(int i, const double4& pi, double fx, double fy, double fz, const AtomArrays& A,
-                                     const StepArgs& S, double4* __restrict__ pos_next,
-                                     const double4* __restrict__ rsave, const SysDesc& s, int* __restrict__ misc) {
+                                     const StepArgs& S, double4* __restrict__ pos_next, const FinIn& in,
+                                     const SysDesc& s, int* __restrict__ misc) {
     const int n = A.n;
     if (sizeof(R) == 4) {
         A.f[(size_t)F_FX * n + i] = (float)fx;
@@ -99,8 +125,8 @@ __device__ inline void finalize_atom(int i, const double4& pi, double fx, double
         A.d[(size_t)D_FZ * n + i] = fz;
     }
     if (S.mode == MODE_COMPUTE) return;
-    const double m = A.d[(size_t)D_M * n + i];
-    double v[3] = {A.d[(size_t)D_VX * n + i], A.d[(size_t)D_VY * n + i], A.d[(size_t)D_VZ * n + i]};
+    const double m = in.m;
+    double v[3] = {in.v[0], in.v[1], in.v[2]};
     const double F[3] = {fx, fy, fz};
     for (int c = 0; c < 3; ++c) {
         double a = F[c] / m;
@@ -114,7 +140,7 @@ __device__ inline void finalize_atom(int i, const double4& pi, double fx, double
             r[c] += v[c] * S.dt;
         }
         pos_next[i] = make_double4(r[0], r[1], r[2], pi.w);
-        const double4 q = rsave[i];
+        const double4 q = in.rs;
         double dx = r[0] - q.x, dy = r[1] - q.y, dz = r[2] - q.z;
         shortest3(s, dx, dy, dz);
         if (sq3_exact(dx, dy, dz) > s.trig2) atomicMin(&misc[M_ABORT], S.step + 1);
@@ -123,6 +149,62 @@ __device__ inline void finalize_atom(int i, const double4& pi, double fx, double
     A.d[(size_t)D_VX * n + i] = v[0];
     A.d[(size_t)D_VY * n + i] = v[1];
     A.d[(size_t)D_VZ * n + i] = v[2];
+}
+
+// The abort flag of a step sequence, read without ordering so the load
+// overlaps the other prologue loads.
+__device__ inline int abort_flag(const StepArgs& S, const int* misc) {
+    return S.mode != MODE_COMPUTE ? __atomic_load_n(&misc[M_ABORT], __ATOMIC_RELAXED) : 0x7fffffff;
+}
+
+// Neighbour slots of this lane are visited in groups of U: the U list
+// reads, then the U position gathers (32-bit-offset buffer loads), are
+// issued back to back so their latencies overlap.  j0 holds the first
+// group's list entries, read in the kernel prologue (the list stores M chunks
+// per atom, so those reads stay inside it).
+template <int TPA, int U>
+__device__ inline void first_group(const AtomSlots& as, bool valid, int M, const int* __restrict__ nl, int* j0) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) j0[u] = (valid && u < M) ? nl[as.base + (long long)u * kWave] : 0;
+}
+
+// Difference r_j - r_i of a listed pair under the reference's image rule
+// (pair_image); when the system's boxes make the minimum image unique
+// between rebuilds, the shift recorded at build time is applied instead.
+__device__ inline bool pair_delta(const SysDesc& s, const double4& pi, const double4& pj, int e, double& dx,
+                                  double& dy, double& dz, double& d2) {
+    dx = pj.x - pi.x;
+    dy = pj.y - pi.y;
+    dz = pj.z - pi.z;
+    if (s.shift_ok) {
+        dx += (double)ent_s(e, 0) * s.L[0];
+        dy += (double)ent_s(e, 1) * s.L[1];
+        dz += (double)ent_s(e, 2) * s.L[2];
+        d2 = sq3_exact(dx, dy, dz);
+        return true;
+    }
+    return pair_image(s, dx, dy, dz, d2);
+}
+
+template <int TPA, int U, class Body>
+__device__ inline void for_neighbours(const AtomSlots& as, int cnt, const int* __restrict__ nl,
+                                      __amdgpu_buffer_rsrc_t posb, const int* j0, Body&& body) {
+    for (int m0 = 0; m0 * TPA + as.sub < cnt; m0 += U) {
+        int e[U];
+        bool ok[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = (m0 + u) * TPA + as.sub;
+            ok[u] = k < cnt;
+            e[u] = ok[u] ? (m0 == 0 ? j0[u] : nl[as.base + (long long)(m0 + u) * kWave]) : 0;
+        }
+        double4 pj[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) pj[u] = buf_d4(posb, (unsigned)ent_j(e[u]) * 32u);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (ok[u]) body(e[u], pj[u]);
+    }
 }
 
 __device__ inline bool skip_step(const StepArgs& S, const int* misc) {

@@ -89,11 +89,26 @@ __global__ void k_scatter(const int* __restrict__ atom_cell, const int* __restri
     src[slot] = i;
 }
 
-// Deterministic order inside each cell: ascending previous index.
-__global__ void k_cell_sort(const int* __restrict__ start, int* __restrict__ src, int ncells) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
+// Deterministic order inside each cell: ascending previous index.  One
+// wavefront per cell ranks up to 64 entries with shuffles; larger cells
+// fall back to an insertion sort by the first lane.
+__global__ void __launch_bounds__(256) k_cell_sort(const int* __restrict__ start, int* __restrict__ src, int ncells) {
+    const int lane = threadIdx.x & 63;
+    const int c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (c >= ncells) return;
-    int b = start[c], e = start[c + 1];
+    const int b = start[c], e = start[c + 1], m = e - b;
+    if (m <= 1) return;
+    if (m <= kWave) {
+        const int v = lane < m ? src[b + lane] : 0x7fffffff;
+        int rank = 0;
+        for (int q = 0; q < kWave; ++q) {
+            const int o = __shfl(v, q, kWave);
+            rank += (q < m && o < v) ? 1 : 0;
+        }
+        if (lane < m) src[b + rank] = v;
+        return;
+    }
+    if (lane != 0) return;
     for (int k = b + 1; k < e; ++k) {
         int v = src[k];
         int q = k - 1;
@@ -126,15 +141,20 @@ __global__ void k_permute(const int* __restrict__ src, const double4* __restrict
     cell_out[p] = cell_in[s];
 }
 
-// One thread per atom: 27-cell stencil (with the reference's de-duplication
-// for one or two periodic cells, grid.cxx:150-166), FP64 test of
-// verlet-list.cxx:374-400, slots written in stencil order into the packed
-// layout.  Counts beyond the capacity are still counted so the host can grow
-// the list.
-__global__ void k_build(const double4* __restrict__ pos, const double* __restrict__ rns, const int* __restrict__ cell_of,
-                        const int* __restrict__ start, const SysDesc* __restrict__ desc, int* __restrict__ nl,
-                        int* __restrict__ nn, int* __restrict__ misc, int n, int tpa, int M, int cap) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+// One wavefront per atom: the lanes walk the concatenated candidate ranges
+// of the 27-cell stencil (with the reference's de-duplication for one or two
+// periodic cells, grid.cxx:150-166) 64 at a time, apply the FP64 test of
+// verlet-list.cxx:374-400, and compact the hits with a ballot so the slots
+// keep stencil order, then cell order.  Counts beyond the capacity are still
+// counted so the host can grow the list.
+__global__ void __launch_bounds__(256) k_build(const double4* __restrict__ pos, const double* __restrict__ rns,
+                                               const int* __restrict__ cell_of, const int* __restrict__ start,
+                                               const SysDesc* __restrict__ desc, int* __restrict__ nl,
+                                               int* __restrict__ nn, int* __restrict__ misc, int n, int tpa, int M,
+                                               int cap) {
+    __shared__ int sh_beg[4][32], sh_pre[4][32];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int i = blockIdx.x * 4 + wv;
     if (i >= n) return;
     const double4 pi = pos[i];
     const SysDesc& s = desc[w_sys(pi.w)];
@@ -142,46 +162,81 @@ __global__ void k_build(const double4* __restrict__ pos, const double* __restric
     const int lc = cell_of[i] - s.cell_off;
     const int nx = s.nc[0], ny = s.nc[1], nz = s.nc[2];
     const int cx = lc % nx, cy = (lc / nx) % ny, cz = lc / (nx * ny);
-    int cnt = 0;
-    for (int ox = -1; ox <= 1; ++ox) {
-        if (s.per[0] && ((nx == 1 && ox != 0) || (nx == 2 && ox == 1))) continue;
-        int x = cx + ox;
-        if (s.per[0]) x = (x + nx) % nx;
-        else if (x < 0 || x >= nx) continue;
-        for (int oy = -1; oy <= 1; ++oy) {
-            if (s.per[1] && ((ny == 1 && oy != 0) || (ny == 2 && oy == 1))) continue;
-            int y = cy + oy;
-            if (s.per[1]) y = (y + ny) % ny;
-            else if (y < 0 || y >= ny) continue;
-            for (int oz = -1; oz <= 1; ++oz) {
-                if (s.per[2] && ((nz == 1 && oz != 0) || (nz == 2 && oz == 1))) continue;
-                int z = cz + oz;
-                if (s.per[2]) z = (z + nz) % nz;
-                else if (z < 0 || z >= nz) continue;
-                const int c = s.cell_off + x + nx * (y + ny * z);
-                const int b = start[c], e = start[c + 1];
-                for (int j = b; j < e; ++j) {
-                    if (j == i) continue;
-                    const double4 pj = pos[j];
-                    double rr = fmax(ri, rns[j]);
-                    double d2c = rr * rr;
-                    double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                    if (s.per[0] && (2 * dx < -s.L[0])) dx += s.L[0];
-                    if (s.per[0] && (2 * dx > s.L[0])) dx -= s.L[0];
-                    if (s.per[1] && (2 * dy < -s.L[1])) dy += s.L[1];
-                    if (s.per[1] && (2 * dy > s.L[1])) dy -= s.L[1];
-                    if (s.per[2] && (2 * dz < -s.L[2])) dz += s.L[2];
-                    if (s.per[2] && (2 * dz > s.L[2])) dz -= s.L[2];
-                    if (sq3_exact(dx, dy, dz) <= d2c) {
-                        if (cnt < cap) nl[list_index(i, cnt, tpa, M)] = j;
-                        ++cnt;
-                    }
-                }
-            }
+    // stencil cell of this lane (lanes 0..26), in x, y, z offset order
+    int cnt_c = 0, beg_c = 0;
+    if (lane < 27) {
+        const int ox = lane / 9 - 1, oy = (lane / 3) % 3 - 1, oz = lane % 3 - 1;
+        bool ok = true;
+        int x = cx + ox, y = cy + oy, z = cz + oz;
+        if (s.per[0]) {
+            ok = ok && !((nx == 1 && ox != 0) || (nx == 2 && ox == 1));
+            x = (x + nx) % nx;
+        } else ok = ok && x >= 0 && x < nx;
+        if (s.per[1]) {
+            ok = ok && !((ny == 1 && oy != 0) || (ny == 2 && oy == 1));
+            y = (y + ny) % ny;
+        } else ok = ok && y >= 0 && y < ny;
+        if (s.per[2]) {
+            ok = ok && !((nz == 1 && oz != 0) || (nz == 2 && oz == 1));
+            z = (z + nz) % nz;
+        } else ok = ok && z >= 0 && z < nz;
+        if (ok) {
+            const int c = s.cell_off + x + nx * (y + ny * z);
+            beg_c = start[c];
+            cnt_c = start[c + 1] - beg_c;
         }
     }
-    nn[i] = cnt;
-    atomicMax(&misc[M_MAXNN], cnt);
+    int pre = cnt_c;
+    for (int o = 1; o < 32; o <<= 1) {
+        const int y = __shfl_up(pre, o, kWave);
+        if (lane >= o) pre += y;
+    }
+    if (lane < 27) {
+        sh_beg[wv][lane] = beg_c;
+        sh_pre[wv][lane] = pre;  // inclusive
+    }
+    const int total = __shfl(pre, 26, kWave);
+    __builtin_amdgcn_wave_barrier();
+    int cnt = 0;
+    int c = 0;  // stencil cell of this lane's candidate; only moves forward
+    for (int base = 0; base < total; base += kWave) {
+        const int idx = base + lane;
+        bool hit = false;
+        int j = -1;
+        if (idx < total) {
+            while (sh_pre[wv][c] <= idx) ++c;
+            const int first = c > 0 ? sh_pre[wv][c - 1] : 0;
+            j = sh_beg[wv][c] + (idx - first);
+            if (j != i) {  // j < 2^26 (checked on the host)
+                const double4 pj = pos[j];
+                const double rr = fmax(ri, rns[j]);
+                const double d2c = rr * rr;
+                double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                int sx = 0, sy = 0, sz = 0;
+                if (s.per[0] && (2 * dx < -s.L[0])) dx += s.L[0], sx = 1;
+                if (s.per[0] && (2 * dx > s.L[0])) dx -= s.L[0], sx = -1;
+                if (s.per[1] && (2 * dy < -s.L[1])) dy += s.L[1], sy = 1;
+                if (s.per[1] && (2 * dy > s.L[1])) dy -= s.L[1], sy = -1;
+                if (s.per[2] && (2 * dz < -s.L[2])) dz += s.L[2], sz = 1;
+                if (s.per[2] && (2 * dz > s.L[2])) dz -= s.L[2], sz = -1;
+                hit = sq3_exact(dx, dy, dz) <= d2c;
+                // the image of r_j - r_i is the negated shift of r_i - r_j
+                j = (int)((unsigned)j | ((unsigned)(1 - sx) << 26) | ((unsigned)(1 - sy) << 28) |
+                          ((unsigned)(1 - sz) << 30));
+            }
+        }
+        const unsigned long long m = __ballot(hit);
+        if (hit) {
+            const int at = cnt + __popcll(m & ((1ull << lane) - 1ull));
+            if (at < cap) nl[list_index(i, at, tpa, M)] = j;
+        }
+        cnt += __popcll(m);
+    }
+    if (lane == 0) {
+        nn[i] = cnt;
+        // the host needs the maximum only when the list overflowed
+        if (cnt > cap) atomicMax(&misc[M_MAXNN], cnt);
+    }
 }
 
 // verlet-list.cxx:110-158 decide(): |shortest(r - r_saved)|^2 > (skin/2)^2
@@ -211,7 +266,7 @@ void nlist_rebuild(Batch& b) {
     hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, st, b.cell_count, b.cell_start, b.cell_fill, b.ncells);
     hipLaunchKernelGGL(k_scatter, dim3(nblk(n, 256)), dim3(256), 0, st, b.atom_cell, b.cell_start, b.cell_fill,
                        b.sorted_src, n);
-    hipLaunchKernelGGL(k_cell_sort, dim3(nblk(b.ncells, 256)), dim3(256), 0, st, b.cell_start, b.sorted_src, b.ncells);
+    hipLaunchKernelGGL(k_cell_sort, dim3(nblk(b.ncells, 4)), dim3(256), 0, st, b.cell_start, b.sorted_src, b.ncells);
     const int s0 = b.sc, s1 = b.sc ^ 1;
     // cell_fill is free after the scatter: reuse it for the sorted cell ids
     hipLaunchKernelGGL(k_permute, dim3(nblk(n, 256)), dim3(256), 0, st, b.sorted_src, b.pos[b.cur], b.pos[b.cur ^ 1],
@@ -221,12 +276,12 @@ void nlist_rebuild(Batch& b) {
     b.sc = s1;
     for (int attempt = 0; attempt < 3; ++attempt) {
         HIPCHECK(hipMemsetAsync(b.misc + M_MAXNN, 0, sizeof(int), st));
-        hipLaunchKernelGGL(k_build, dim3(nblk(n, 128)), dim3(128), 0, st, b.pos[b.cur], b.d(b.sc, D_RNS), b.cell_fill,
+        hipLaunchKernelGGL(k_build, dim3(nblk(n, 4)), dim3(256), 0, st, b.pos[b.cur], b.d(b.sc, D_RNS), b.cell_fill,
                            b.cell_start, b.d_desc, b.nl, b.nn, b.misc, n, b.tpa, b.M, b.cap);
         HIPCHECK(hipMemcpyAsync(b.ctx->h_pin, b.misc, sizeof(int) * M_COUNT, hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
         int maxnn = b.ctx->h_pin[M_MAXNN];
-        if (maxnn <= b.cap) break;
+        if (maxnn <= b.cap) break;  // zero unless some atom overflowed
         // grow the list and rebuild it
         int cap = ((maxnn + maxnn / 4 + 8 + b.tpa - 1) / b.tpa) * b.tpa;
         int A = kWave / b.tpa;
@@ -242,12 +297,13 @@ void nlist_rebuild(Batch& b) {
     }
     b.list_valid = true;
     b.rebuilds++;
+    b.since_rebuild = 0;
 }
 
 int nlist_decide(Batch& b, double rcut) {
     hipStream_t st = b.ctx->stream;
     if (!b.list_valid) return 1;
-    set_trigger(b, rcut);
+    set_trigger(b, rcut, false);
     HIPCHECK(hipMemsetAsync(b.misc + M_DECIDE, 0, sizeof(int), st));
     hipLaunchKernelGGL(k_decide, dim3(nblk(b.n, 256)), dim3(256), 0, st, b.pos[b.cur], b.rsave, b.d_desc, b.misc, b.n);
     HIPCHECK(hipMemcpyAsync(b.ctx->h_pin, b.misc, sizeof(int) * M_COUNT, hipMemcpyDeviceToHost, st));
