@@ -59,16 +59,25 @@ def read_traffic(kernel_tag):
     return vals.get("hbm_bytes_per_launch")
 
 
+def make_pot(E, ctx, pot):
+    if pot["kind"] == "lj":
+        return E.LJsCut(ctx, pot["a"], pot["s"], pot["rc"])
+    if pot["kind"] == "eam":
+        return E.EAM(ctx, rcut=pot["rc"], F=pot["F"], rho=pot["rho"], rphi=pot["rphi"], dr=pot["dr"],
+                     drho=pot["drho"])
+    raise ValueError(pot["kind"])
+
+
 def run_gpu(E, ctx, w, precision, steps, warmup, lanes=None):
     pot = w["pot"]
     at = w["atoms"]
     s = E.System(ctx, [len(at)], [w["box"]], True, lanes=lanes)
     s.upload(at)
-    if pot["kind"] == "lj":
-        p = E.LJsCut(ctx, pot["a"], pot["s"], pot["rc"])
-    else:
-        p = E.EAM(ctx, rcut=pot["rc"], F=pot["F"], rho=pot["rho"], rphi=pot["rphi"], dr=pot["dr"], drho=pot["drho"])
+    p = make_pot(E, ctx, pot)
     s.build()
+    tune = None
+    if lanes is None:
+        lanes, tune = s.autotune(p, precision, reps=20)
     s.compute(p, E.FORCE | E.ENERGY | E.VIRIAL, precision)
     e0 = s.thermo()[0]
     if warmup:
@@ -85,6 +94,8 @@ def run_gpu(E, ctx, w, precision, steps, warmup, lanes=None):
     s.step(p, w["dt"], min(steps, 200), precision)
     s.sync()
     ms, nl = s.timing(0)
+    s.build()  # a fresh list, as after any rebuild: no atom past the trigger
+    b2b = s.time_kernels(p, precision, w["dt"], reps=200)
     s.compute(p, E.FORCE | E.ENERGY | E.VIRIAL, precision)
     e1 = s.thermo()[0]
     n = len(at)
@@ -96,7 +107,90 @@ def run_gpu(E, ctx, w, precision, steps, warmup, lanes=None):
         "n": n, "steps": steps, "seconds": t1 - t0, "value": n * steps / (t1 - t0), "kernel_us": per_kernel,
         "ms_per_step": 1e3 * (t1 - t0) / steps, "kernel_ms": ms.tolist(), "kernel_launches": nl.tolist(),
         "nbar": st["mean"], "rebuilds": reb, "drift_eV_atom_ns": drift, "lanes": lanes,
+        "autotune_us": tune, "b2b_us": {"density": round(b2b[0], 2), "force": round(b2b[1], 2)},
     }
+
+
+COPY_MIB = 256
+
+
+def copy_calibration_buffers():
+    """Two 256 MiB device buffers for the calibration kernel, allocated
+    before the engine touches the device (torch initialised after another
+    HIP user has crashed under rocprofv3 --pmc)."""
+    import torch
+    a = torch.ones(COPY_MIB << 18, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    return a, torch.empty_like(a)
+
+
+def copy_calibration(bufs):
+    """One plain elementwise kernel over 256 MiB (reads 256 MiB, writes
+    256 MiB), run three times: what FETCH_SIZE / WRITE_SIZE read for it
+    under rocprofv3 --pmc scales the counters of the engine's kernels."""
+    import torch
+    a, b = bufs
+    for _ in range(3):
+        torch.mul(a, 2.0, out=b)
+    torch.cuda.synchronize()
+
+
+def _csv_rows(d, suffix):
+    import csv
+    import glob
+    rows = []
+    for f in sorted(glob.glob(os.path.join(d, "**", "*" + suffix), recursive=True)):
+        rows += list(csv.DictReader(open(f)))
+    if not rows:
+        raise SystemExit("no %s under %s" % (suffix, d))
+    return rows
+
+
+def summarise_pmc(fetch_dir, write_dir, kernel):
+    """Per-launch HBM bytes of `kernel` from two rocprofv3 --pmc runs
+    (FETCH_SIZE, WRITE_SIZE; KiB per dispatch), each scaled by the
+    counter's reading for the 256 MiB calibration kernel."""
+    out = []
+    res = {}
+    for tag, d, ctr in (("fetch", fetch_dir, "FETCH_SIZE"), ("write", write_dir, "WRITE_SIZE")):
+        rows = [r for r in _csv_rows(d, "counter_collection.csv") if r["Counter_Name"] == ctr]
+        mine = sorted(float(r["Counter_Value"]) for r in rows if kernel in r["Kernel_Name"])
+        cal = [float(r["Counter_Value"]) for r in rows if "MulFunctor" in r["Kernel_Name"]]
+        if not mine or not cal:
+            raise SystemExit("missing %s rows for %s or the calibration kernel" % (ctr, kernel))
+        # full-work dispatches only (launches queued ahead of a rebuild exit at once)
+        med = mine[len(mine) // 2]
+        mine = [v for v in mine if v >= 0.5 * med]
+        kib, cal_kib = sum(mine) / len(mine), max(cal)
+        factor = COPY_MIB * 1024.0 / cal_kib
+        res[tag] = kib * factor * 1024.0
+        out += ["%s_launches: %d" % (tag, len(mine)), "%s_kib_per_launch: %.2f" % (tag, kib),
+                "%s_calibration_kib: %.1f (expected %d)" % (tag, cal_kib, COPY_MIB * 1024),
+                "%s_factor: %.5f" % (tag, factor)]
+    out.append("hbm_bytes_per_launch: %.0f" % (res["fetch"] + res["write"]))
+    return "\n".join(out)
+
+
+def summarise_stats(d, kernel):
+    """Kernel table from rocprofv3 --kernel-trace --stats (CSV output), and
+    for `kernel` the mean over full-work dispatches: launches queued ahead
+    of a list rebuild exit at once (a few microseconds) and are counted
+    apart."""
+    import numpy as np
+    tr = [r for r in _csv_rows(d, "kernel_trace.csv") if kernel in r["Kernel_Name"]]
+    dur = np.array([(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3 for r in tr])
+    full = dur[dur >= 0.5 * np.median(dur)] if len(dur) else dur
+    head = ["%s: %d dispatches, mean %.2f us; %d full-work dispatches, mean %.2f us; %d early exits, mean %.2f us"
+            % (kernel, len(dur), dur.mean(), len(full), full.mean(), len(dur) - len(full),
+               dur[dur < 0.5 * np.median(dur)].mean() if len(dur) > len(full) else 0.0), ""]
+    rows = _csv_rows(d, "kernel_stats.csv")
+    lines = head + ["%-60s %8s %12s %12s %7s" % ("kernel", "calls", "avg_us", "total_us", "pct")]
+    for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"])):
+        name = r["Name"].replace("HIP_vector_type<double, 4u>", "double4")
+        name = name.split("(")[0] if name.startswith(("mdc::", "void mdc::")) else name[:60]
+        lines.append("%-60s %8s %12.2f %12.1f %7.2f" % (name[:60], r["Calls"], float(r["AverageNs"]) / 1e3,
+                                                     float(r["TotalDurationNs"]) / 1e3, float(r["Percentage"])))
+    return "\n".join(lines)
 
 
 def cpu_baseline(w, max_seconds=20.0, max_steps=None):
@@ -139,11 +233,23 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--lanes", type=int, default=None)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run the 256 MiB calibration kernel (for rocprofv3 --pmc runs)")
+    ap.add_argument("--summarise-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
+    ap.add_argument("--summarise-stats", metavar="STATS_DIR")
+    ap.add_argument("--kernel", default="k_eam_force<float")
     args = ap.parse_args()
+    if args.summarise_pmc:
+        print(summarise_pmc(*args.summarise_pmc, args.kernel))
+        return
+    if args.summarise_stats:
+        print(summarise_stats(args.summarise_stats, args.kernel))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU holding one domain; multi-GPU runs are not part of this build "
               "(see the sharding notes in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    cal = copy_calibration_buffers() if args.copy_calibration else None
     from mdcraft_amd import engine as E
     from mdcraft_amd import workloads as W
 
@@ -151,16 +257,19 @@ def main():
     w = W.eam_cu()
     head = run_gpu(E, ctx, w, E.MIXED, args.steps, args.warmup, args.lanes)
     nbar = head["nbar"]
-    avg_ms = head["kernel_ms"][0] / max(head["kernel_launches"][0], 1)
+    # mean duration of the step-mode force kernel: 200 back-to-back launches
+    # between two HIP events on the engine's stream
+    avg_ms = head["b2b_us"]["force"] * 1e-3
     per_launch = algorithmic_bytes("eam_mixed", nbar) * head["n"]
     achieved = per_launch / (avg_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic("eam_force_mixed"),
             "kernel": "k_eam_force<float>", "avg_launch_us": round(avg_ms * 1e3, 2),
+            "avg_launch_us_in_steps": round(head["kernel_ms"][0] / max(head["kernel_launches"][0], 1) * 1e3, 2),
             "algorithmic_bytes_per_launch": int(per_launch)}
     k = args.steps / max(head["rebuilds"], 1)
     step_gbs = step_bytes("eam", nbar, k) * head["value"] / 1e9
-    keys = ("value", "ms_per_step", "nbar", "rebuilds", "drift_eV_atom_ns", "kernel_us")
+    keys = ("value", "ms_per_step", "nbar", "rebuilds", "drift_eV_atom_ns", "kernel_us", "b2b_us", "lanes")
     extra = {"eam_cu_mixed": {kk: head[kk] for kk in keys},
              "eam_cu_mixed_step_GBs": round(step_gbs, 1)}
     if not args.no_extra:
@@ -181,6 +290,8 @@ def main():
         "roofline": roof, "cpu_baseline": cpu, "workloads": extra,
     }
     print(json.dumps(line), flush=True)
+    if cal is not None:
+        copy_calibration(cal)
 
 
 if __name__ == "__main__":

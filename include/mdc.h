@@ -95,6 +95,13 @@ mdc_status mdc_sys_download(mdc_sys* sys, void* atoms, uint32_t fields);
 mdc_status mdc_sys_natoms(const mdc_sys* sys, uint64_t* n);
 /* lanes per atom (1, 2, 4, 8, 16) of the force kernels; changes the list layout */
 mdc_status mdc_sys_set_lanes(mdc_sys* sys, int lanes_per_atom);
+/* Times one force evaluation (MDC_FORCE at the given precision, reps
+ * repetitions after a warm-up) with 1, 2, 4, 8 and 16 lanes per atom,
+ * keeps the fastest and leaves a fresh list and forces for it.
+ * us_per_eval (may be NULL) receives the 5 mean times in microseconds.
+ * No reference counterpart: the reference has no device launch shape. */
+mdc_status mdc_sys_autotune(mdc_sys* sys, const mdc_pot* pot, int precision, int reps, int* best_lanes,
+                            double* us_per_eval);
 
 /* ---- neighbour list (neibs::VerletList) --------------------------------- */
 /* update(kbuf = -1): wraps positions into periodic boxes, bins, rebuilds */
@@ -109,6 +116,13 @@ mdc_status mdc_nlist_stats(mdc_sys* sys, double* mean_neighbours, int* max_neigh
 /* ---- force evaluation and integration ----------------------------------- */
 /* prepare + stages (+ virials when MDC_ENERGY | MDC_VIRIAL) */
 mdc_status mdc_compute(mdc_sys* sys, const mdc_pot* pot, uint32_t what, int precision);
+/* Mean duration in microseconds of the step-mode density (us_out[0], 0 for
+ * LJ) and force (us_out[1]) kernels, each launched reps times back to back
+ * between two events on the context stream.  Leaves positions, velocities,
+ * forces and the list as they were.  LJ and EAM only.
+ * No reference counterpart: a measurement hook. */
+mdc_status mdc_sys_time_kernels(mdc_sys* sys, const mdc_pot* pot, int precision, double dt, int reps,
+                                double* us_out);
 /* nsteps of velocity Verlet (make_step), list rebuilt on the half-skin
  * trigger; *rebuilds receives the number of rebuilds (may be NULL) */
 mdc_status mdc_step(mdc_sys* sys, const mdc_pot* pot, double dt, int nsteps, int precision, int* rebuilds);

@@ -732,31 +732,34 @@ mdc_status mdc_sys_natoms(const mdc_sys* sys, uint64_t* n) {
     return 0;
 }
 
+static void set_lanes(Batch& b, int lanes) {
+    if (lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16)
+        throw Error(E_INVAL, "lanes per atom must be 1, 2, 4, 8 or 16");
+    if (lanes == b.tpa) return;
+    HIPCHECK(hipSetDevice(b.ctx->device));
+    int maxcap = b.cap;
+    b.tpa = lanes;
+    size_list(b);
+    if (maxcap > b.cap) {
+        // keep the grown capacity
+        int cap = ((maxcap + b.tpa - 1) / b.tpa) * b.tpa;
+        const int A = kWave / b.tpa;
+        size_t elems = (size_t)((b.n + A - 1) / A) * (size_t)(cap / b.tpa) * kWave;
+        if (elems > b.nl_elems) {
+            dfree(b.nl);
+            b.nl = dalloc<int>(elems);
+            b.nl_elems = elems;
+        }
+        b.cap = cap;
+        b.M = cap / b.tpa;
+    }
+    b.list_valid = false;
+}
+
 mdc_status mdc_sys_set_lanes(mdc_sys* sys, int lanes) {
     return guard(sys ? sys->b.ctx : nullptr, [&] {
         if (!sys) throw Error(E_INVAL, "null argument");
-        if (lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16)
-            throw Error(E_INVAL, "lanes per atom must be 1, 2, 4, 8 or 16");
-        Batch& b = sys->b;
-        if (lanes == b.tpa) return;
-        HIPCHECK(hipSetDevice(b.ctx->device));
-        int maxcap = b.cap;
-        b.tpa = lanes;
-        size_list(b);
-        if (maxcap > b.cap) {
-            // keep the grown capacity
-            int cap = ((maxcap + b.tpa - 1) / b.tpa) * b.tpa;
-            const int A = kWave / b.tpa;
-            size_t elems = (size_t)((b.n + A - 1) / A) * (size_t)(cap / b.tpa) * kWave;
-            if (elems > b.nl_elems) {
-                dfree(b.nl);
-                b.nl = dalloc<int>(elems);
-                b.nl_elems = elems;
-            }
-            b.cap = cap;
-            b.M = cap / b.tpa;
-        }
-        b.list_valid = false;
+        set_lanes(sys->b, lanes);
     });
 }
 
@@ -837,6 +840,67 @@ mdc_status mdc_compute(mdc_sys* sys, const mdc_pot* pot, uint32_t what, int prec
         HIPCHECK(hipSetDevice(sys->b.ctx->device));
         check_kinds(sys->b, pot->p);
         force_compute(sys->b, pot->p, (int)what, precision);
+    });
+}
+
+mdc_status mdc_sys_autotune(mdc_sys* sys, const mdc_pot* pot, int precision, int reps, int* best_lanes,
+                            double* us_per_eval) {
+    return guard(sys ? sys->b.ctx : nullptr, [&] {
+        if (!sys || !pot || !best_lanes) throw Error(E_INVAL, "null argument");
+        if (precision != MDC_FP64 && precision != MDC_MIXED) throw Error(E_INVAL, "precision must be FP64 or MIXED");
+        if (reps < 1) throw Error(E_INVAL, "reps must be >= 1");
+        Batch& b = sys->b;
+        HIPCHECK(hipSetDevice(b.ctx->device));
+        check_kinds(b, pot->p);
+        const int cand[5] = {1, 2, 4, 8, 16};
+        hipStream_t st = b.ctx->stream;
+        hipEvent_t e0, e1;
+        HIPCHECK(hipEventCreate(&e0));
+        HIPCHECK(hipEventCreate(&e1));
+        double best = 1e300;
+        int best_l = b.tpa;
+        try {
+            for (int c = 0; c < 5; ++c) {
+                const int l = cand[c];
+                set_lanes(b, l);
+                nlist_rebuild(b);
+                force_compute(b, pot->p, MDC_FORCE, precision);  // warm-up
+                HIPCHECK(hipEventRecord(e0, st));
+                for (int r = 0; r < reps; ++r) force_compute(b, pot->p, MDC_FORCE, precision);
+                HIPCHECK(hipEventRecord(e1, st));
+                HIPCHECK(hipEventSynchronize(e1));
+                float ms = 0.f;
+                HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+                const double us = 1e3 * ms / reps;
+                if (us_per_eval) us_per_eval[c] = us;
+                if (us < best) {
+                    best = us;
+                    best_l = l;
+                }
+            }
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        HIPCHECK(hipEventDestroy(e0));
+        HIPCHECK(hipEventDestroy(e1));
+        set_lanes(b, best_l);
+        nlist_rebuild(b);
+        force_compute(b, pot->p, MDC_FORCE, precision);
+        *best_lanes = best_l;
+    });
+}
+
+mdc_status mdc_sys_time_kernels(mdc_sys* sys, const mdc_pot* pot, int precision, double dt, int reps,
+                                double* us_out) {
+    return guard(sys ? sys->b.ctx : nullptr, [&] {
+        if (!sys || !pot || !us_out) throw Error(E_INVAL, "null argument");
+        if (precision != MDC_FP64 && precision != MDC_MIXED) throw Error(E_INVAL, "precision must be FP64 or MIXED");
+        if (!(dt > 0) || reps < 1) throw Error(E_INVAL, "dt must be positive and reps >= 1");
+        HIPCHECK(hipSetDevice(sys->b.ctx->device));
+        check_kinds(sys->b, pot->p);
+        time_step_kernels(sys->b, pot->p, precision, dt, reps, us_out);
     });
 }
 

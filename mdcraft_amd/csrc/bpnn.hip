@@ -191,7 +191,7 @@ __global__ void __launch_bounds__(256) k_bp_desc(const double4* __restrict__ pos
     if (skip_step(SA, misc)) return;
     extern __shared__ char smem[];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int i = blockIdx.x * (blockDim.x >> 6) + wv;
+    const int i = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + wv;
     if (i >= n) return;
     const BpDesc& P = *Pd;
     const Stage<R> S = carve<R>(smem + wv * stage_bytes(cap, P.ne, sizeof(R)), cap, P.ne);
@@ -417,7 +417,7 @@ __global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos
     if (skip_step(SA, misc)) return;
     extern __shared__ char smem[];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int i = blockIdx.x * (blockDim.x >> 6) + wv;
+    const int i = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + wv;
     if (i >= n) return;
     const BpDesc& P = *Pd;
     const Stage<R> S = carve<R>(smem + wv * stage_bytes(cap, P.ne, sizeof(R)), cap, P.ne);
@@ -531,7 +531,7 @@ __global__ void __launch_bounds__(256) k_bp_gather(const double4* __restrict__ p
                                                    const R* __restrict__ own, AtomArrays A, StepArgs SA,
                                                    const double4* __restrict__ rsave, int* __restrict__ misc) {
     if (skip_step(SA, misc)) return;
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (m >= A.n) return;
     const FinIn fin = load_fin(m, A, SA, rsave, true);
     const double4 pm = pos[m];
@@ -860,26 +860,18 @@ void bpnn_compute(Batch& b, const Pot& p, int what, int prec, int mode, double d
         using R = decltype(rtag);
         const size_t lds = waves * stage_bytes(b.cap, bp.h.ne, sizeof(R));
         if (lds > 160 * 1024) throw Error(E_INVAL, "BPNN neighbour staging exceeds the LDS");
-        hipEvent_t t0 = timer_begin(b);
-        hipLaunchKernelGGL((k_bp_desc<R>), gw, bw, lds, st, pos, b.nl, b.nn, b.tpa, b.M, b.cap, b.d_desc, bp.d,
-                           (R*)w.G, ldg, n, S, b.misc);
-        timer_end(b, t0, K_BP_DESC);
-        hipEvent_t t1 = timer_begin(b);
+        launch_k(b, K_BP_DESC, k_bp_desc<R>, gw, bw, (unsigned)lds, pos, b.nl, b.nn, b.tpa, b.M, b.cap, b.d_desc,
+                 bp.d, (R*)w.G, ldg, n, S, b.misc);
         if (sizeof(R) == 8)
-            hipLaunchKernelGGL(k_bp_mlp64, dim3((n + 127) / 128), dim3(128), 0, st, (const double*)w.G, ldg, bp.w64,
-                               bp.d, pos, (double*)w.w, Ep, n, S, b.misc);
+            launch_k(b, K_BP_MLP, k_bp_mlp64, dim3((n + 127) / 128), dim3(128), 0, (const double*)w.G, ldg, bp.w64,
+                     bp.d, pos, (double*)w.w, Ep, n, S, b.misc);
         else
-            hipLaunchKernelGGL(k_bp_mlp_mfma, dim3((n + 15) / 16), dim3(64), 0, st, (const float*)w.G, ldg, bp.bsplit,
-                               boff, nts, kts, bp.w32, bp.d, pos, (float*)w.w, Ep, n, S, b.misc);
-        timer_end(b, t1, K_BP_MLP);
-        hipEvent_t t2 = timer_begin(b);
-        hipLaunchKernelGGL((k_bp_grad<R>), gw, bw, lds, st, pos, b.nl, b.nn, b.tpa, b.M, b.cap, b.d_desc, bp.d,
-                           (const R*)w.w, ldg, (R*)w.P, (R*)w.own, n, S, b.misc);
-        timer_end(b, t2, K_BP_FORCE);
-        hipEvent_t t3 = timer_begin(b);
-        hipLaunchKernelGGL((k_bp_gather<R>), dim3((n + 255) / 256), dim3(256), 0, st, pos, pos_next, b.nl, b.nn, w.rev, b.tpa, b.M,
-                           b.cap, b.d_desc, (const R*)w.P, (const R*)w.own, AA, S, b.rsave, b.misc);
-        timer_end(b, t3, K_BP_GATHER);
+            launch_k(b, K_BP_MLP, k_bp_mlp_mfma, dim3((n + 15) / 16), dim3(64), 0, (const float*)w.G, ldg, bp.bsplit,
+                     boff, nts, kts, bp.w32, bp.d, pos, (float*)w.w, Ep, n, S, b.misc);
+        launch_k(b, K_BP_FORCE, k_bp_grad<R>, gw, bw, (unsigned)lds, pos, b.nl, b.nn, b.tpa, b.M, b.cap, b.d_desc,
+                 bp.d, (const R*)w.w, ldg, (R*)w.P, (R*)w.own, n, S, b.misc);
+        launch_k(b, K_BP_GATHER, k_bp_gather<R>, dim3((n + 255) / 256), dim3(256), 0, pos, pos_next, b.nl, b.nn,
+                 w.rev, b.tpa, b.M, b.cap, b.d_desc, (const R*)w.P, (const R*)w.own, AA, S, b.rsave, b.misc);
     };
     if (prec == P_FP64) run(double(0));
     else run(float(0));

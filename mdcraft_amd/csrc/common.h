@@ -79,8 +79,30 @@ __device__ inline int w_kind(double w) { return (int)((long long)w & 0xff); }
 __device__ inline int w_inactive(double w) { return (int)(((long long)w >> 8) & 1); }
 __device__ inline int w_sys(double w) { return (int)((long long)w >> 9); }
 
+// The part of a system's descriptor that the pair loops read, copied into
+// registers once per atom so it is not reloaded per neighbour.
+struct Frame {
+    double L[3];
+    double rns2;
+    int per[3];
+    int any_per;
+    int shift_ok;
+};
+__device__ inline Frame frame_of(const SysDesc& s) {
+    Frame f;
+    for (int c = 0; c < 3; ++c) {
+        f.L[c] = s.L[c];
+        f.per[c] = s.per[c];
+    }
+    f.rns2 = s.rns2;
+    f.any_per = s.any_per;
+    f.shift_ok = s.shift_ok;
+    return f;
+}
+
 // lattice::Domain::shortest (mdcraft/lattice/domain.cxx:59-66)
-__device__ inline void shortest3(const SysDesc& s, double& x, double& y, double& z) {
+template <class S>
+__device__ inline void shortest3(const S& s, double& x, double& y, double& z) {
     if (s.per[0]) {
         x = fabs(x + s.L[0]) < fabs(x) ? x + s.L[0] : x;
         x = fabs(x - s.L[0]) < fabs(x) ? x - s.L[0] : x;
@@ -113,12 +135,37 @@ __device__ inline T wave_sum_group(T v, int tpa) {
 // the raw difference is used when |raw|^2 < max(rns)^2, otherwise the
 // single-shift minimum image, dropped when that is beyond max(rns).
 // Returns false when the pair has no image.
-__device__ inline bool pair_image(const SysDesc& s, double& dx, double& dy, double& dz, double& d2) {
+template <class S>
+__device__ inline bool pair_image(const S& s, double& dx, double& dy, double& dz, double& d2) {
     d2 = sq3_exact(dx, dy, dz);
     if (!s.any_per || d2 < s.rns2) return true;
     shortest3(s, dx, dy, dz);
     d2 = sq3_exact(dx, dy, dz);
     return !(d2 > s.rns2);
+}
+
+// Raise a flag word once per wavefront: every lane with p set would store
+// the same value, so the lowest such lane does it for all (atomics from many
+// lanes to one address serialise).
+__device__ inline void wave_atomic_min(bool p, int* addr, int v) {
+    const unsigned long long m = __ballot(p);
+    if (p && (int)__lane_id() == __builtin_ctzll(m)) atomicMin(addr, v);
+}
+__device__ inline void wave_atomic_or(bool p, int* addr, int v) {
+    const unsigned long long m = __ballot(p);
+    if (p && (int)__lane_id() == __builtin_ctzll(m)) atomicOr(addr, v);
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  xcd_block maps the hardware workgroup index to a logical one so that
+// each XCD runs one contiguous range of logical workgroups: for kernels over
+// cell-ordered atoms that is one spatial slab, whose neighbours then stay in
+// that XCD's L2.  The map is a bijection of [0, G).
+constexpr int kXcd = 8;
+__device__ inline int xcd_block(int b, int G) {
+    const int x = b % kXcd, k = b / kXcd;
+    const int per = G / kXcd, rem = G % kXcd;
+    return x < rem ? x * (per + 1) + k : rem * (per + 1) + (x - rem) * per + k;
 }
 
 struct AtomSlots {
@@ -130,7 +177,8 @@ struct AtomSlots {
 template <int TPA>
 __device__ inline AtomSlots atom_slots(int M) {
     const int lane = threadIdx.x & (kWave - 1);
-    const long long gw = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const long long gw = ((long long)blk * blockDim.x + threadIdx.x) / kWave;
     AtomSlots a;
     a.i = (int)(gw * (kWave / TPA) + lane / TPA);
     a.sub = lane % TPA;

@@ -1,8 +1,10 @@
 // Host-side engine state shared by the kernel launchers and the C ABI.
 #pragma once
 #include <string>
+#include <tuple>
 #include <vector>
 #include <stdexcept>
+#include <hip/hip_ext.h>
 #include "common.h"
 
 namespace mdc {
@@ -146,6 +148,30 @@ enum KKind { K_FORCE = 0, K_DENSITY = 1, K_PREDRIFT = 2, K_BP_DESC = 3, K_BP_MLP
 hipEvent_t timer_begin(Batch& b);
 void timer_end(Batch& b, hipEvent_t a, int kind);
 void timer_drain(Batch& b);
+hipEvent_t timer_event(Batch& b);
+
+// Launches kernel k on the context's stream.  While timing is on, the
+// start/stop events travel inside the kernel's own dispatch
+// (hipExtLaunchKernel), so the recorded time is the kernel's execution
+// without the cost of separate marker packets.
+template <typename... P, typename... A>
+void launch_k(Batch& b, int kind, void (*k)(P...), dim3 grid, dim3 block, unsigned lds, A... args) {
+    static_assert(sizeof...(P) == sizeof...(A), "kernel argument count");
+    std::tuple<P...> t{static_cast<P>(args)...};
+    void* argv[sizeof...(P) > 0 ? sizeof...(P) : 1];
+    std::apply([&](auto&... x) {
+        int i = 0;
+        ((argv[i++] = (void*)&x), ...);
+    }, t);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (b.timing && kind >= 0) {
+        e0 = timer_event(b);
+        e1 = timer_event(b);
+    }
+    const hipError_t err = hipExtLaunchKernel((const void*)k, grid, block, argv, lds, b.ctx->stream, e0, e1, 0);
+    if (err != hipSuccess) throw Error(E_HIP, std::string("kernel launch: ") + hipGetErrorString(err));
+    if (e0) b.ev_pending.push_back({e0, e1, kind});
+}
 
 // launchers (nlist.hip)
 void nlist_rebuild(Batch& b);
@@ -153,6 +179,7 @@ int nlist_decide(Batch& b, double rcut);
 // launchers (forces.hip)
 void force_compute(Batch& b, const Pot& p, int what, int prec);
 void run_steps(Batch& b, const Pot& p, double dt, int nsteps, int prec, int chunk);
+void time_step_kernels(Batch& b, const Pot& p, int prec, double dt, int reps, double* us);
 void thermo(Batch& b, double* out);
 void set_trigger(Batch& b, double rcut, bool allow_shift);
 void check_errors(Batch& b);

@@ -45,6 +45,7 @@ __global__ void __launch_bounds__(256) k_lj(const double4* __restrict__ pos, dou
     if (ab <= S.step) return;
     const bool active = valid && !w_inactive(pi.w);
     const SysDesc& s = desc[valid ? w_sys(pi.w) : 0];
+    const Frame fr = frame_of(s);
     const int cnt = active ? min(nraw, cap) : 0;
     const R c_pref = (R)(-0.5 * P.factor), c_lj1 = (R)P.lj1, c_lj2 = (R)P.lj2, c_s6 = (R)P.s6, c_e = (R)(0.5 * 4 * P.a);
     R fx = 0, fy = 0, fz = 0;
@@ -52,7 +53,7 @@ __global__ void __launch_bounds__(256) k_lj(const double4* __restrict__ pos, dou
     const __amdgpu_buffer_rsrc_t posb = make_buf(pos, (unsigned)A.n * 32u);
     for_neighbours<TPA, 4>(as, cnt, nl, posb, j0, [&](int ent, const double4& pj) {
         double dx, dy, dz, d2;
-        if (!pair_delta(s, pi, pj, ent, dx, dy, dz, d2)) return;
+        if (!pair_delta(fr, pi, pj, ent, dx, dy, dz, d2)) return;
         if (d2 > P.rc2 || d2 < 1e-15) return;
         const R r2 = (R)d2, rx = (R)dx, ry = (R)dy, rz = (R)dz;
         const R r2inv = rcp_(r2);
@@ -173,7 +174,7 @@ struct PairGeo {
 };
 
 template <typename R>
-__device__ inline PairGeo<R> pair_geo(const SysDesc& s, const double4& pi, const double4& pj, int e, bool valid,
+__device__ inline PairGeo<R> pair_geo(const Frame& s, const double4& pi, const double4& pj, int e, bool valid,
                                       double rc2, R dr1, int nr) {
     PairGeo<R> g;
     double d2;
@@ -205,6 +206,7 @@ __global__ void __launch_bounds__(256) k_eam_rho(const double4* __restrict__ pos
     if (ab <= S.step) return;
     const bool active = valid && !w_inactive(pi.w);
     const SysDesc& s = desc[valid ? w_sys(pi.w) : 0];
+    const Frame fr = frame_of(s);
     const int cnt = active ? min(nraw, cap) : 0;
     const auto* tab = rtab<R>(P);
     const R dr = (R)P.dr, dr1 = (R)P.dr1;
@@ -225,7 +227,7 @@ __global__ void __launch_bounds__(256) k_eam_rho(const double4* __restrict__ pos
         for (int u = 0; u < kU; ++u) pj[u] = buf_d4(posb, (unsigned)ent_j(e[u]) * 32u);
         PairGeo<R> g[kU];
 #pragma unroll
-        for (int u = 0; u < kU; ++u) g[u] = pair_geo<R>(s, pi, pj[u], e[u], okk[u], P.rc2, dr1, P.nr);
+        for (int u = 0; u < kU; ++u) g[u] = pair_geo<R>(fr, pi, pj[u], e[u], okk[u], P.rc2, dr1, P.nr);
         typename Vec4T<R>::T rec[kU];
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
@@ -277,6 +279,7 @@ __global__ void __launch_bounds__(256) k_eam_force(const double4* __restrict__ p
     if (ab <= S.step) return;
     const bool active = valid && !w_inactive(pi.w);
     const SysDesc& s = desc[valid ? w_sys(pi.w) : 0];
+    const Frame fr = frame_of(s);
     const int cnt = active ? min(nraw, cap) : 0;
     const double Fpi = active ? Fpi0 : 0.0;
     const auto* tab = rtab<R>(P);
@@ -305,7 +308,7 @@ __global__ void __launch_bounds__(256) k_eam_force(const double4* __restrict__ p
         }
         PairGeo<R> g[kU];
 #pragma unroll
-        for (int u = 0; u < kU; ++u) g[u] = pair_geo<R>(s, pi, pj[u], e[u], okk[u], P.rc2, dr1, P.nr);
+        for (int u = 0; u < kU; ++u) g[u] = pair_geo<R>(fr, pi, pj[u], e[u], okk[u], P.rc2, dr1, P.nr);
         typename Vec4T<R>::T rr[kU], rp[kU], rq[ALLOY ? kU : 1];
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
@@ -420,7 +423,7 @@ __global__ void k_predrift(const double4* __restrict__ pos, double4* __restrict_
     const double4 q = rsave[i];
     double dx = r[0] - q.x, dy = r[1] - q.y, dz = r[2] - q.z;
     shortest3(s, dx, dy, dz);
-    if (sq3_exact(dx, dy, dz) > s.trig2) atomicMin(&misc[M_ABORT], 0);
+    wave_atomic_min(sq3_exact(dx, dy, dz) > s.trig2, &misc[M_ABORT], 0);
 }
 
 // Per-system FP64 reduction in a fixed order (one workgroup per system):
@@ -484,8 +487,7 @@ __global__ void __launch_bounds__(256) k_thermo(const double* __restrict__ d, in
 // --------------------------------------------------------------------------
 static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
 
-hipEvent_t timer_begin(Batch& b) {
-    if (!b.timing) return nullptr;
+hipEvent_t timer_event(Batch& b) {
     hipEvent_t e;
     if (!b.ev_free.empty()) {
         e = b.ev_free.back();
@@ -493,19 +495,20 @@ hipEvent_t timer_begin(Batch& b) {
     } else {
         HIPCHECK(hipEventCreate(&e));
     }
+    return e;
+}
+
+// Marker-event timing of a sequence of launches (the list rebuild).
+hipEvent_t timer_begin(Batch& b) {
+    if (!b.timing) return nullptr;
+    hipEvent_t e = timer_event(b);
     HIPCHECK(hipEventRecord(e, b.ctx->stream));
     return e;
 }
 
 void timer_end(Batch& b, hipEvent_t a, int kind) {
     if (!b.timing || !a) return;
-    hipEvent_t e;
-    if (!b.ev_free.empty()) {
-        e = b.ev_free.back();
-        b.ev_free.pop_back();
-    } else {
-        HIPCHECK(hipEventCreate(&e));
-    }
+    hipEvent_t e = timer_event(b);
     HIPCHECK(hipEventRecord(e, b.ctx->stream));
     b.ev_pending.push_back({a, e, kind});
 }
@@ -568,43 +571,43 @@ static LJK lj_args(const Pot& p) {
     return LJK{p.lj.a, p.lj.s6, p.lj.lj1, p.lj.lj2, p.lj.factor, p.lj.rc2};
 }
 
+// which: 1 = density pass (EAM), 2 = force pass, 3 = both
 template <typename R, int TPA>
-static void launch_pair_t(Batch& b, const Pot& p, const StepArgs& S, const double4* pos, double4* pos_next) {
+static void launch_pair_t(Batch& b, const Pot& p, const StepArgs& S, const double4* pos, double4* pos_next,
+                          int which) {
     hipStream_t st = b.ctx->stream;
     const int A = kWave / TPA;
     const long long waves = (b.n + A - 1) / A;
     const int grid = nblk(waves * kWave, 256);
     AtomArrays AA{b.dst[b.sc], b.fst[b.sc], b.n};
     if (p.type == PT_LJ) {
-        hipEvent_t t0 = timer_begin(b);
-        hipLaunchKernelGGL((k_lj<R, TPA>), dim3(grid), dim3(256), 0, st, pos, pos_next, b.nl, b.nn, b.M, b.cap,
-                           b.d_desc, lj_args(p), AA, S, b.rsave, b.misc);
-        timer_end(b, t0, K_FORCE);
+        if (!(which & 2)) return;
+        launch_k(b, K_FORCE, k_lj<R, TPA>, dim3(grid), dim3(256), 0, pos, pos_next, b.nl, b.nn, b.M, b.cap, b.d_desc,
+                 lj_args(p), AA, S, b.rsave, b.misc);
     } else {
         EamK E = eam_args(p);
-        hipEvent_t t0 = timer_begin(b);
-        hipLaunchKernelGGL((k_eam_rho<R, TPA>), dim3(grid), dim3(256), 0, st, pos, b.nl, b.nn, b.M, b.cap, b.d_desc, E,
-                           AA, S, b.misc);
-        timer_end(b, t0, K_DENSITY);
-        hipEvent_t t1 = timer_begin(b);
+        if (which & 1)
+            launch_k(b, K_DENSITY, k_eam_rho<R, TPA>, dim3(grid), dim3(256), 0, pos, b.nl, b.nn, b.M, b.cap, b.d_desc,
+                     E, AA, S, b.misc);
+        if (!(which & 2)) return;
         if (E.nk > 1)
-            hipLaunchKernelGGL((k_eam_force<R, TPA, true>), dim3(grid), dim3(256), 0, st, pos, pos_next, b.nl, b.nn,
-                               b.M, b.cap, b.d_desc, E, AA, S, b.rsave, b.misc);
+            launch_k(b, K_FORCE, k_eam_force<R, TPA, true>, dim3(grid), dim3(256), 0, pos, pos_next, b.nl, b.nn, b.M,
+                     b.cap, b.d_desc, E, AA, S, b.rsave, b.misc);
         else
-            hipLaunchKernelGGL((k_eam_force<R, TPA, false>), dim3(grid), dim3(256), 0, st, pos, pos_next, b.nl, b.nn,
-                               b.M, b.cap, b.d_desc, E, AA, S, b.rsave, b.misc);
-        timer_end(b, t1, K_FORCE);
+            launch_k(b, K_FORCE, k_eam_force<R, TPA, false>, dim3(grid), dim3(256), 0, pos, pos_next, b.nl, b.nn, b.M,
+                     b.cap, b.d_desc, E, AA, S, b.rsave, b.misc);
     }
 }
 
 template <typename R>
-static void launch_pair_r(Batch& b, const Pot& p, const StepArgs& S, const double4* pos, double4* pos_next) {
+static void launch_pair_r(Batch& b, const Pot& p, const StepArgs& S, const double4* pos, double4* pos_next,
+                          int which = 3) {
     switch (b.tpa) {
-        case 1: launch_pair_t<R, 1>(b, p, S, pos, pos_next); break;
-        case 2: launch_pair_t<R, 2>(b, p, S, pos, pos_next); break;
-        case 4: launch_pair_t<R, 4>(b, p, S, pos, pos_next); break;
-        case 8: launch_pair_t<R, 8>(b, p, S, pos, pos_next); break;
-        case 16: launch_pair_t<R, 16>(b, p, S, pos, pos_next); break;
+        case 1: launch_pair_t<R, 1>(b, p, S, pos, pos_next, which); break;
+        case 2: launch_pair_t<R, 2>(b, p, S, pos, pos_next, which); break;
+        case 4: launch_pair_t<R, 4>(b, p, S, pos, pos_next, which); break;
+        case 8: launch_pair_t<R, 8>(b, p, S, pos, pos_next, which); break;
+        case 16: launch_pair_t<R, 16>(b, p, S, pos, pos_next, which); break;
         default: throw Error(E_INVAL, "lanes per atom must be 1, 2, 4, 8 or 16");
     }
 }
@@ -666,10 +669,8 @@ void run_steps(Batch& b, const Pot& p, double dt, int nsteps, int prec, int chun
     int c0 = b.cur;
     {
         AtomArrays AA{b.dst[b.sc], b.fst[b.sc], b.n};
-        hipEvent_t t0 = timer_begin(b);
-        hipLaunchKernelGGL(k_predrift, dim3(grid), dim3(256), 0, st, b.pos[c0], b.pos[c0 ^ 1], AA, b.f_prec, dt,
-                           b.d_desc, b.rsave, b.misc);
-        timer_end(b, t0, K_PREDRIFT);
+        launch_k(b, K_PREDRIFT, k_predrift, dim3(grid), dim3(256), 0, b.pos[c0], b.pos[c0 ^ 1], AA, b.f_prec, dt,
+                 b.d_desc, b.rsave, b.misc);
     }
     auto buf = [&](int t) { return c0 ^ 1 ^ (t & 1); };
     int s = 0;
@@ -717,6 +718,63 @@ void run_steps(Batch& b, const Pot& p, double dt, int nsteps, int prec, int chun
     HIPCHECK(hipMemsetAsync(b.d(b.sc, D_VXX), 0, sizeof(double) * 6 * b.n, st));
     b.f_prec = prec;
     b.last_rcut = p.rcut;
+}
+
+// Mean duration of the step-mode density and force kernels (the
+// configuration of every step but the last in a sequence), each launched
+// reps times back to back between two events on the context stream.  The
+// velocities, which the force kernel updates in place, are saved in the
+// idle half of the double-buffered state and restored; forces and F' are
+// rewritten with the values they already hold.
+void time_step_kernels(Batch& b, const Pot& p, int prec, double dt, int reps, double* us) {
+    us[0] = us[1] = 0.0;
+    if (b.n == 0) return;
+    if (p.type == PT_BPNN) throw Error(E_INVAL, "kernel timing covers the LJ and EAM kernels");
+    if (b.f_prec < 0) throw Error(E_STATE, "no force to start from: compute forces once before timing");
+    hipStream_t st = b.ctx->stream;
+    set_trigger(b, p.rcut, true);
+    if (!b.list_valid) nlist_rebuild(b);
+    const int idle = b.sc ^ 1;
+    const size_t vbytes = sizeof(double) * 3 * (size_t)b.n;
+    HIPCHECK(hipMemcpyAsync(b.d(idle, D_VX), b.d(b.sc, D_VX), vbytes, hipMemcpyDeviceToDevice, st));
+    int abort_init = INT_MAX;
+    HIPCHECK(hipMemcpyAsync(b.misc + M_ABORT, &abort_init, sizeof(int), hipMemcpyHostToDevice, st));
+    const StepArgs S{dt, 0, MODE_FINAL_DRIFT, 0, 0};
+    const double4* pos = b.pos[b.cur];
+    double4* pos_next = b.pos[b.cur ^ 1];
+    hipEvent_t e[3];
+    for (auto& x : e) HIPCHECK(hipEventCreate(&x));
+    const bool timing = b.timing;
+    b.timing = false;
+    auto run = [&](int which) {
+        if (prec == P_MIXED) launch_pair_r<float>(b, p, S, pos, pos_next, which);
+        else launch_pair_r<double>(b, p, S, pos, pos_next, which);
+    };
+    try {
+        run(3);  // warm-up
+        HIPCHECK(hipEventRecord(e[0], st));
+        if (p.type == PT_EAM)
+            for (int r = 0; r < reps; ++r) run(1);
+        HIPCHECK(hipEventRecord(e[1], st));
+        for (int r = 0; r < reps; ++r) run(2);
+        HIPCHECK(hipEventRecord(e[2], st));
+        HIPCHECK(hipEventSynchronize(e[2]));
+    } catch (...) {
+        b.timing = timing;
+        for (auto& x : e) (void)hipEventDestroy(x);
+        throw;
+    }
+    b.timing = timing;
+    float m0 = 0.f, m1 = 0.f;
+    HIPCHECK(hipEventElapsedTime(&m0, e[0], e[1]));
+    HIPCHECK(hipEventElapsedTime(&m1, e[1], e[2]));
+    for (auto& x : e) HIPCHECK(hipEventDestroy(x));
+    us[0] = p.type == PT_EAM ? 1e3 * m0 / reps : 0.0;
+    us[1] = 1e3 * m1 / reps;
+    HIPCHECK(hipMemcpyAsync(b.d(b.sc, D_VX), b.d(idle, D_VX), vbytes, hipMemcpyDeviceToDevice, st));
+    HIPCHECK(hipMemcpyAsync(b.misc + M_ABORT, &abort_init, sizeof(int), hipMemcpyHostToDevice, st));
+    read_misc(b);
+    check_errors(b);
 }
 
 void thermo(Batch& b, double* out) {

@@ -143,8 +143,9 @@ __device__ inline void finalize_atom(int i, const double4& pi, double fx, double
         const double4 q = in.rs;
         double dx = r[0] - q.x, dy = r[1] - q.y, dz = r[2] - q.z;
         shortest3(s, dx, dy, dz);
-        if (sq3_exact(dx, dy, dz) > s.trig2) atomicMin(&misc[M_ABORT], S.step + 1);
-        if (!(fabs(r[0]) < 1e300 && fabs(r[1]) < 1e300 && fabs(r[2]) < 1e300)) atomicOr(&misc[M_ERR], EB_NONFINITE);
+        wave_atomic_min(sq3_exact(dx, dy, dz) > s.trig2, &misc[M_ABORT], S.step + 1);
+        wave_atomic_or(!(fabs(r[0]) < 1e300 && fabs(r[1]) < 1e300 && fabs(r[2]) < 1e300), &misc[M_ERR],
+                       EB_NONFINITE);
     }
     A.d[(size_t)D_VX * n + i] = v[0];
     A.d[(size_t)D_VY * n + i] = v[1];
@@ -171,7 +172,7 @@ __device__ inline void first_group(const AtomSlots& as, bool valid, int M, const
 // Difference r_j - r_i of a listed pair under the reference's image rule
 // (pair_image); when the system's boxes make the minimum image unique
 // between rebuilds, the shift recorded at build time is applied instead.
-__device__ inline bool pair_delta(const SysDesc& s, const double4& pi, const double4& pj, int e, double& dx,
+__device__ inline bool pair_delta(const Frame& s, const double4& pi, const double4& pj, int e, double& dx,
                                   double& dy, double& dz, double& d2) {
     dx = pj.x - pi.x;
     dy = pj.y - pi.y;

@@ -141,101 +141,136 @@ __global__ void k_permute(const int* __restrict__ src, const double4* __restrict
     cell_out[p] = cell_in[s];
 }
 
-// One wavefront per atom: the lanes walk the concatenated candidate ranges
-// of the 27-cell stencil (with the reference's de-duplication for one or two
-// periodic cells, grid.cxx:150-166) 64 at a time, apply the FP64 test of
-// verlet-list.cxx:374-400, and compact the hits with a ballot so the slots
-// keep stencil order, then cell order.  Counts beyond the capacity are still
+// One workgroup per cell.  Wave 0 lays out the 27-cell stencil (with the
+// reference's de-duplication for one or two periodic cells,
+// grid.cxx:150-166) as concatenated candidate ranges; the workgroup stages
+// the candidates' positions and search radii in LDS, kTile at a time; each
+// wavefront then takes atoms of the cell and tests them against the staged
+// tile 64 candidates at a time with the FP64 rule of
+// verlet-list.cxx:374-400, compacting the hits with a ballot so the slots
+// keep stencil order, then cell order.  The candidates are gathered once
+// per cell instead of once per atom.  Counts beyond the capacity are still
 // counted so the host can grow the list.
+constexpr int kTile = 1024;
+
 __global__ void __launch_bounds__(256) k_build(const double4* __restrict__ pos, const double* __restrict__ rns,
-                                               const int* __restrict__ cell_of, const int* __restrict__ start,
-                                               const SysDesc* __restrict__ desc, int* __restrict__ nl,
-                                               int* __restrict__ nn, int* __restrict__ misc, int n, int tpa, int M,
-                                               int cap) {
-    __shared__ int sh_beg[4][32], sh_pre[4][32];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int i = blockIdx.x * 4 + wv;
-    if (i >= n) return;
-    const double4 pi = pos[i];
-    const SysDesc& s = desc[w_sys(pi.w)];
-    const double ri = rns[i];
-    const int lc = cell_of[i] - s.cell_off;
-    const int nx = s.nc[0], ny = s.nc[1], nz = s.nc[2];
-    const int cx = lc % nx, cy = (lc / nx) % ny, cz = lc / (nx * ny);
-    // stencil cell of this lane (lanes 0..26), in x, y, z offset order
-    int cnt_c = 0, beg_c = 0;
-    if (lane < 27) {
-        const int ox = lane / 9 - 1, oy = (lane / 3) % 3 - 1, oz = lane % 3 - 1;
-        bool ok = true;
-        int x = cx + ox, y = cy + oy, z = cz + oz;
-        if (s.per[0]) {
-            ok = ok && !((nx == 1 && ox != 0) || (nx == 2 && ox == 1));
-            x = (x + nx) % nx;
-        } else ok = ok && x >= 0 && x < nx;
-        if (s.per[1]) {
-            ok = ok && !((ny == 1 && oy != 0) || (ny == 2 && oy == 1));
-            y = (y + ny) % ny;
-        } else ok = ok && y >= 0 && y < ny;
-        if (s.per[2]) {
-            ok = ok && !((nz == 1 && oz != 0) || (nz == 2 && oz == 1));
-            z = (z + nz) % nz;
-        } else ok = ok && z >= 0 && z < nz;
-        if (ok) {
-            const int c = s.cell_off + x + nx * (y + ny * z);
-            beg_c = start[c];
-            cnt_c = start[c + 1] - beg_c;
-        }
-    }
-    int pre = cnt_c;
-    for (int o = 1; o < 32; o <<= 1) {
-        const int y = __shfl_up(pre, o, kWave);
-        if (lane >= o) pre += y;
-    }
-    if (lane < 27) {
-        sh_beg[wv][lane] = beg_c;
-        sh_pre[wv][lane] = pre;  // inclusive
-    }
-    const int total = __shfl(pre, 26, kWave);
-    __builtin_amdgcn_wave_barrier();
-    int cnt = 0;
-    int c = 0;  // stencil cell of this lane's candidate; only moves forward
-    for (int base = 0; base < total; base += kWave) {
-        const int idx = base + lane;
-        bool hit = false;
-        int j = -1;
-        if (idx < total) {
-            while (sh_pre[wv][c] <= idx) ++c;
-            const int first = c > 0 ? sh_pre[wv][c - 1] : 0;
-            j = sh_beg[wv][c] + (idx - first);
-            if (j != i) {  // j < 2^26 (checked on the host)
-                const double4 pj = pos[j];
-                const double rr = fmax(ri, rns[j]);
-                const double d2c = rr * rr;
-                double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                int sx = 0, sy = 0, sz = 0;
-                if (s.per[0] && (2 * dx < -s.L[0])) dx += s.L[0], sx = 1;
-                if (s.per[0] && (2 * dx > s.L[0])) dx -= s.L[0], sx = -1;
-                if (s.per[1] && (2 * dy < -s.L[1])) dy += s.L[1], sy = 1;
-                if (s.per[1] && (2 * dy > s.L[1])) dy -= s.L[1], sy = -1;
-                if (s.per[2] && (2 * dz < -s.L[2])) dz += s.L[2], sz = 1;
-                if (s.per[2] && (2 * dz > s.L[2])) dz -= s.L[2], sz = -1;
-                hit = sq3_exact(dx, dy, dz) <= d2c;
-                // the image of r_j - r_i is the negated shift of r_i - r_j
-                j = (int)((unsigned)j | ((unsigned)(1 - sx) << 26) | ((unsigned)(1 - sy) << 28) |
-                          ((unsigned)(1 - sz) << 30));
+                                               const int* __restrict__ start, const SysDesc* __restrict__ desc,
+                                               int* __restrict__ nl, int* __restrict__ nn, int* __restrict__ misc,
+                                               int tpa, int M, int cap) {
+    __shared__ int s_beg[32], s_pre[32];
+    __shared__ int c_j[kTile];
+    __shared__ double c_x[kTile], c_y[kTile], c_z[kTile], c_r[kTile];
+    __shared__ int s_cnt[64];
+    const int c = xcd_block(blockIdx.x, gridDim.x);
+    const int a0 = start[c], a1 = start[c + 1];
+    if (a0 == a1) return;  // uniform over the workgroup
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const SysDesc& s = desc[w_sys(pos[a0].w)];
+    if (wv == 0) {
+        const int lc = c - s.cell_off;
+        const int nx = s.nc[0], ny = s.nc[1], nz = s.nc[2];
+        const int cx = lc % nx, cy = (lc / nx) % ny, cz = lc / (nx * ny);
+        int cnt_c = 0, beg_c = 0;
+        if (lane < 27) {
+            const int ox = lane / 9 - 1, oy = (lane / 3) % 3 - 1, oz = lane % 3 - 1;
+            bool ok = true;
+            int x = cx + ox, y = cy + oy, z = cz + oz;
+            if (s.per[0]) {
+                ok = ok && !((nx == 1 && ox != 0) || (nx == 2 && ox == 1));
+                x = (x + nx) % nx;
+            } else ok = ok && x >= 0 && x < nx;
+            if (s.per[1]) {
+                ok = ok && !((ny == 1 && oy != 0) || (ny == 2 && oy == 1));
+                y = (y + ny) % ny;
+            } else ok = ok && y >= 0 && y < ny;
+            if (s.per[2]) {
+                ok = ok && !((nz == 1 && oz != 0) || (nz == 2 && oz == 1));
+                z = (z + nz) % nz;
+            } else ok = ok && z >= 0 && z < nz;
+            if (ok) {
+                const int cc = s.cell_off + x + nx * (y + ny * z);
+                beg_c = start[cc];
+                cnt_c = start[cc + 1] - beg_c;
             }
         }
-        const unsigned long long m = __ballot(hit);
-        if (hit) {
-            const int at = cnt + __popcll(m & ((1ull << lane) - 1ull));
-            if (at < cap) nl[list_index(i, at, tpa, M)] = j;
+        int pre = cnt_c;
+        for (int o = 1; o < 32; o <<= 1) {
+            const int y = __shfl_up(pre, o, kWave);
+            if (lane >= o) pre += y;
         }
-        cnt += __popcll(m);
+        if (lane < 32) {
+            s_beg[lane] = beg_c;
+            s_pre[lane] = pre;  // inclusive; lanes 27..31 repeat the total
+        }
     }
-    if (lane == 0) {
-        nn[i] = cnt;
-        // the host needs the maximum only when the list overflowed
-        if (cnt > cap) atomicMax(&misc[M_MAXNN], cnt);
+    __syncthreads();
+    const int total = s_pre[26];
+    const double L0 = s.L[0], L1 = s.L[1], L2 = s.L[2];
+    const int p0 = s.per[0], p1 = s.per[1], p2 = s.per[2];
+    for (int g0 = a0; g0 < a1; g0 += 64) {
+        const int ga = min(64, a1 - g0);
+        if (tid < 64) s_cnt[tid] = 0;
+        for (int t0 = 0; t0 < total; t0 += kTile) {
+            const int tn = min(kTile, total - t0);
+            __syncthreads();  // the previous tile is consumed and s_cnt is set
+            for (int q = tid; q < tn; q += 256) {
+                const int idx = t0 + q;
+                int cc = 0;
+                while (s_pre[cc] <= idx) ++cc;
+                const int j = s_beg[cc] + idx - (cc > 0 ? s_pre[cc - 1] : 0);
+                const double4 pj = pos[j];
+                c_j[q] = j;
+                c_x[q] = pj.x;
+                c_y[q] = pj.y;
+                c_z[q] = pj.z;
+                c_r[q] = rns[j];
+            }
+            __syncthreads();
+            for (int a = wv; a < ga; a += 4) {
+                const int i = g0 + a;
+                const double4 pi = pos[i];
+                const double ri = rns[i];
+                int cnt = s_cnt[a];
+                for (int q0 = 0; q0 < tn; q0 += kWave) {
+                    const int q = q0 + lane;
+                    bool hit = false;
+                    unsigned ent = 0;
+                    if (q < tn) {
+                        const int j = c_j[q];
+                        if (j != i) {  // j < 2^26 (checked on the host)
+                            const double rr = fmax(ri, c_r[q]);
+                            const double d2c = rr * rr;
+                            double dx = pi.x - c_x[q], dy = pi.y - c_y[q], dz = pi.z - c_z[q];
+                            int sx = 0, sy = 0, sz = 0;
+                            if (p0 && (2 * dx < -L0)) dx += L0, sx = 1;
+                            if (p0 && (2 * dx > L0)) dx -= L0, sx = -1;
+                            if (p1 && (2 * dy < -L1)) dy += L1, sy = 1;
+                            if (p1 && (2 * dy > L1)) dy -= L1, sy = -1;
+                            if (p2 && (2 * dz < -L2)) dz += L2, sz = 1;
+                            if (p2 && (2 * dz > L2)) dz -= L2, sz = -1;
+                            hit = sq3_exact(dx, dy, dz) <= d2c;
+                            // the image of r_j - r_i is the negated shift of r_i - r_j
+                            ent = (unsigned)j | ((unsigned)(1 - sx) << 26) | ((unsigned)(1 - sy) << 28) |
+                                  ((unsigned)(1 - sz) << 30);
+                        }
+                    }
+                    const unsigned long long m = __ballot(hit);
+                    if (hit) {
+                        const int at = cnt + __popcll(m & ((1ull << lane) - 1ull));
+                        if (at < cap) nl[list_index(i, at, tpa, M)] = (int)ent;
+                    }
+                    cnt += __popcll(m);
+                }
+                if (lane == 0) s_cnt[a] = cnt;
+            }
+        }
+        __syncthreads();
+        if (tid < ga) {
+            const int cnt = s_cnt[tid];
+            nn[g0 + tid] = cnt;
+            // the host needs the maximum only when the list overflowed
+            if (cnt > cap) atomicMax(&misc[M_MAXNN], cnt);
+        }
     }
 }
 
@@ -248,7 +283,7 @@ __global__ void k_decide(const double4* __restrict__ pos, const double4* __restr
     const SysDesc& s = desc[w_sys(p.w)];
     double dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     shortest3(s, dx, dy, dz);
-    if (sq3_exact(dx, dy, dz) > s.trig2) atomicOr(&misc[M_DECIDE], 1);
+    wave_atomic_or(sq3_exact(dx, dy, dz) > s.trig2, &misc[M_DECIDE], 1);
 }
 
 static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
@@ -276,8 +311,8 @@ void nlist_rebuild(Batch& b) {
     b.sc = s1;
     for (int attempt = 0; attempt < 3; ++attempt) {
         HIPCHECK(hipMemsetAsync(b.misc + M_MAXNN, 0, sizeof(int), st));
-        hipLaunchKernelGGL(k_build, dim3(nblk(n, 4)), dim3(256), 0, st, b.pos[b.cur], b.d(b.sc, D_RNS), b.cell_fill,
-                           b.cell_start, b.d_desc, b.nl, b.nn, b.misc, n, b.tpa, b.M, b.cap);
+        hipLaunchKernelGGL(k_build, dim3(b.ncells), dim3(256), 0, st, b.pos[b.cur], b.d(b.sc, D_RNS), b.cell_start,
+                           b.d_desc, b.nl, b.nn, b.misc, b.tpa, b.M, b.cap);
         HIPCHECK(hipMemcpyAsync(b.ctx->h_pin, b.misc, sizeof(int) * M_COUNT, hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
         int maxnn = b.ctx->h_pin[M_MAXNN];

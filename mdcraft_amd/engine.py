@@ -31,7 +31,7 @@ EXPORTS = [
     "mdc_pot_lj_create", "mdc_pot_eam_create", "mdc_pot_eam_create_file", "mdc_pot_eam_alloy_create",
     "mdc_pot_bpnn_create", "mdc_pot_rcut", "mdc_pot_destroy",
     "mdc_sys_create", "mdc_sys_destroy", "mdc_sys_upload", "mdc_sys_download", "mdc_sys_natoms",
-    "mdc_sys_set_lanes", "mdc_nlist_build", "mdc_nlist_decide", "mdc_nlist_export", "mdc_nlist_stats",
+    "mdc_sys_set_lanes", "mdc_sys_autotune", "mdc_sys_time_kernels", "mdc_nlist_build", "mdc_nlist_decide", "mdc_nlist_export", "mdc_nlist_stats",
     "mdc_compute", "mdc_step", "mdc_thermo", "mdc_sync", "mdc_timing", "mdc_bpnn_inspect",
 ]
 
@@ -64,6 +64,8 @@ def load_library(path=LIB_PATH):
         "mdc_sys_download": (I, [P, P, U32]),
         "mdc_sys_natoms": (I, [P, C.POINTER(U64)]),
         "mdc_sys_set_lanes": (I, [P, I]),
+        "mdc_sys_autotune": (I, [P, P, I, I, C.POINTER(I), DP]),
+        "mdc_sys_time_kernels": (I, [P, P, I, D, I, DP]),
         "mdc_nlist_build": (I, [P]),
         "mdc_nlist_decide": (I, [P, D, C.POINTER(I)]),
         "mdc_nlist_export": (I, [P, C.POINTER(U64), C.POINTER(U64), C.POINTER(U64)]),
@@ -214,6 +216,21 @@ class System:
 
     def set_lanes(self, lanes):
         _check(lib().mdc_sys_set_lanes(self.h, lanes))
+
+    def autotune(self, pot, precision, reps=5):
+        """Time one force evaluation for 1, 2, 4, 8, 16 lanes per atom and
+        keep the fastest; returns (lanes, {lanes: microseconds})."""
+        best = C.c_int(0)
+        us = np.zeros(5)
+        _check(lib().mdc_sys_autotune(self.h, pot.h, precision, reps, C.byref(best), _dp(us)))
+        return best.value, dict(zip((1, 2, 4, 8, 16), us.tolist()))
+
+    def time_kernels(self, pot, precision, dt, reps=100):
+        """Mean step-mode (density, force) kernel durations in microseconds,
+        each from reps back-to-back launches between two HIP events."""
+        us = np.zeros(2)
+        _check(lib().mdc_sys_time_kernels(self.h, pot.h, precision, dt, reps, _dp(us)))
+        return float(us[0]), float(us[1])
 
     def upload(self, atoms, fields=F_ALL):
         atoms = np.ascontiguousarray(atoms, dtype=ATOM_DTYPE)

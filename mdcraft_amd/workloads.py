@@ -117,7 +117,13 @@ def taper(r, rc=EAM_RC, h=0.5):
     return np.where(r < rc, s, 0.0)
 
 
-def morse(r, D0=0.2, alpha=1.6, r0=2.55):
+# Morse pair term and sqrt embedding fitted so that fcc Cu with the 5.0 A
+# tapered cutoff has its energy minimum at a = 3.615 A (E = -3.04 eV/atom,
+# B = 197 GPa); the crystal then stays crystalline at 300 K.
+EAM_F_A = 0.6  # eV
+
+
+def morse(r, D0=0.1353, alpha=1.6473, r0=2.906):
     e = np.exp(-alpha * (r - r0))
     return D0 * (e * e - 2.0 * e)
 
@@ -137,7 +143,7 @@ def eam_cu_tables(nr=5000, nrho=5000, rho_max=50.0, scale_rho=1.0, scale_F=1.0):
     rho = density(r) * t * scale_rho
     drho = rho_max / (nrho - 1)
     rh = np.arange(nrho) * drho
-    F_ev = -1.2 * np.sqrt(rh) * scale_F
+    F_ev = -EAM_F_A * np.sqrt(rh) * scale_F
     return {
         "F": F_ev * EV,
         "rho": rho,
@@ -151,7 +157,7 @@ def eam_cu_tables(nr=5000, nrho=5000, rho_max=50.0, scale_rho=1.0, scale_F=1.0):
 def eam_analytic(r_nm, rhobar):
     """Analytic phi (kJ/mol), rho, F (kJ/mol) of the tabulated functions."""
     r = np.asarray(r_nm) / ANG
-    return morse(r) * taper(r) * EV, density(r) * taper(r), -1.2 * np.sqrt(rhobar) * EV
+    return morse(r) * taper(r) * EV, density(r) * taper(r), -EAM_F_A * np.sqrt(rhobar) * EV
 
 
 # --------------------------------------------------------------------------

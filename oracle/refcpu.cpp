@@ -153,7 +153,10 @@ public:
         if (n < 1) return;
         int n_tasks = std::max(1, std::min(10 * count_, int(n / 500)));
         size_t bin = n / n_tasks;
-        std::atomic<int> left(n_tasks);
+        // The count is changed under dm: the caller can only see zero after
+        // the last task has released dm, so dm and dcv (locals of this
+        // frame) are never touched after the caller returns.
+        int left = n_tasks;
         std::mutex dm;
         std::condition_variable dcv;
         size_t from = 0;
@@ -161,15 +164,13 @@ public:
             size_t to = (t == n_tasks - 1) ? n : from + bin;
             push([&, from, to] {
                 for (size_t i = from; i < to; ++i) func(i);
-                if (left.fetch_sub(1) == 1) {
-                    std::lock_guard<std::mutex> lk(dm);
-                    dcv.notify_all();
-                }
+                std::lock_guard<std::mutex> lk(dm);
+                if (--left == 0) dcv.notify_all();
             });
             from = to;
         }
         std::unique_lock<std::mutex> lk(dm);
-        dcv.wait(lk, [&] { return left.load() == 0; });
+        dcv.wait(lk, [&] { return left == 0; });
     }
 
 private:

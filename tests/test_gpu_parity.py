@@ -237,3 +237,20 @@ def test_run_md_steps_harness(ctx):
         assert np.allclose(got_hist[:, k], hist[:, k], rtol=1e-12, atol=1e-9), (k, got_hist[:, k], hist[:, k])
     got = s.download()
     assert np.max(np.abs(vec(got, "r") - vec(a, "r"))) < 1e-12
+
+
+def test_time_kernels_leaves_state(ctx):
+    """The kernel-timing hook rewrites forces with the values they hold and
+    restores the velocities it advanced."""
+    w = W.eam_cu(reps=6)
+    eng = E()
+    s = eng.System(ctx, [len(w["atoms"])], [w["box"]])
+    s.upload(w["atoms"])
+    p = engine_potential(eng, ctx, w["pot"])
+    s.build()
+    s.compute(p, 1, 1)
+    before = s.download()
+    s.time_kernels(p, 1, w["dt"], reps=5)
+    after = s.download()
+    for k in ("r", "v", "f"):
+        assert np.array_equal(vec(after, k), vec(before, k)), k
