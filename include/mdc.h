@@ -69,12 +69,16 @@ mdc_status mdc_pot_eam_create(mdc_ctx* ctx, double rcut, const double* F, const 
 /* EAM(filename), single-element setfl with eV/Angstrom conversion:
  * solver/potential/EAM.cxx:55-87 */
 mdc_status mdc_pot_eam_create_file(mdc_ctx* ctx, const char* setfl_path, mdc_pot** out);
-/* setfl-alloy EAM for nkinds elements (the per-kind potential routing of
- * solver/multi.cxx:26-151 generalised to standard alloy semantics):
- * F[k*nrho + i], rho[k*nr + i], rphi[p*nr + i] with p = a*(a+1)/2 + b for
- * kinds a >= b. Atom kind selects the element. */
-mdc_status mdc_pot_eam_alloy_create(mdc_ctx* ctx, int nkinds, double rcut, const double* F, const double* rho,
-                                    const double* rphi, int nr, double dr, int nrho, double drho, mdc_pot** out);
+/* solver::Multi(nkinds) + add_potential(i, j, pot) for each pair of kinds
+ * (solver/multi.cxx:11-151), over single-element EAM objects: npots entries
+ * (kind_i[q], kind_j[q], pots[q]) that together cover every pair; a later
+ * entry for the same pair replaces an earlier one, as add_potential does.
+ * Atom i's list is split by neighbour kind and the object of (kind_i,
+ * kind_j) handles that sub-list: its rho table, its pair term, and F' of
+ * its own embedding table at both atoms' densities.  The member objects
+ * are copied and may be destroyed afterwards.  At most 4 kinds. */
+mdc_status mdc_pot_multi_create(mdc_ctx* ctx, int nkinds, int npots, const int* kind_i, const int* kind_j,
+                                const mdc_pot* const* pots, mdc_pot** out);
 /* MaiseBPNNPotential(filename, domain): solver/potential/maise_bpnn.cxx:21-30;
  * G2 radial + G4 angular descriptors, one species, 4-layer tanh network */
 mdc_status mdc_pot_bpnn_create(mdc_ctx* ctx, const char* maise_path, mdc_pot** out);

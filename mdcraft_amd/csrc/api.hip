@@ -128,7 +128,7 @@ void records(const double* y, const double* z, size_t N, std::vector<double>& ou
 
 Pot* make_eam(Ctx* ctx, int nk, double rcut, const double* F, const double* rho, const double* rphi, int nr, double dr,
               int nrho, double drho) {
-    if (nk < 1 || nk > kMaxKinds) throw Error(E_INVAL, "EAM kinds must be 1..8");
+    if (nk != 1) throw Error(E_INVAL, "an EAM object holds one element; combine objects with mdc_pot_multi_create");
     if (nr < 3 || nrho < 3) throw Error(E_INVAL, "EAM tables need at least 3 knots");
     if (!(dr > 0) || !(drho > 0) || !(rcut > 0)) throw Error(E_INVAL, "EAM steps and cutoff must be positive");
     auto p = std::make_unique<Pot>();
@@ -171,6 +171,8 @@ Pot* make_eam(Ctx* ctx, int nk, double rcut, const double* F, const double* rho,
         rec32[q + 3] = (float)(dr * aa);
     }
     const size_t nrec = rec64.size() / 4;
+    T.nrec = nrec;
+    T.nFrec = F64.size() / 4;
     T.r64 = dalloc<double4>(nrec);
     T.r32 = dalloc<float4>(nrec);
     T.F64 = dalloc<double4>(F64.size() / 4);
@@ -220,7 +222,74 @@ void free_pot(Pot* p) {
     dfree(p->eam.r32);
     dfree(p->eam.F64);
     dfree(p->eam.knots32);
+    dfree(p->eam.objs);
     delete p;
+}
+
+// solver::Multi over EAM objects (multi.cxx:11-151): the records of every
+// object are copied into one array so one kernel serves all pairs of kinds.
+Pot* make_multi(Ctx* ctx, int nk, int npots, const int* ki, const int* kj, const Pot* const* pots) {
+    if (nk < 1 || nk > kMaxMultiKinds) throw Error(E_INVAL, "Multi kinds must be 1..4");
+    const int npair = nk * (nk + 1) / 2;
+    std::vector<const Pot*> obj(npair, nullptr);
+    for (int q = 0; q < npots; ++q) {
+        if (ki[q] < 0 || kj[q] < 0 || ki[q] >= nk || kj[q] >= nk) throw Error(E_INVAL, "Multi: kind out of range");
+        const Pot* s = pots[q];
+        if (!s || s->type != PT_EAM || s->eam.nk != 1)
+            throw Error(E_INVAL, "Multi: every member must be a single-element EAM potential");
+        obj[pair_index(ki[q], kj[q])] = s;  // a later add_potential replaces an earlier one
+    }
+    for (auto* o : obj)
+        if (!o) throw Error(E_INVAL, "Multi: a pair of kinds has no potential");
+    auto p = std::make_unique<Pot>();
+    p->ctx = ctx;
+    p->type = PT_EAM;
+    EamTables& T = p->eam;
+    T.nk = nk;
+    T.npair = npair;
+    std::vector<EamObj> h(npair);
+    size_t nrec = 0, nF = 0;
+    double rc = 0;
+    for (int q = 0; q < npair; ++q) {
+        const EamTables& S = obj[q]->eam;
+        EamObj& o = h[q];
+        o.rho_off = (int)nrec;
+        o.phi_off = (int)(nrec + S.nr);
+        o.F_off = (int)nF;
+        o.nr = S.nr;
+        o.nrho = S.nrho;
+        o.pad = 0;
+        o.dr = S.dr;
+        o.dr1 = 1.0 / S.dr;
+        o.drho = S.drho;
+        o.drho1 = 1.0 / S.drho;
+        o.rc2 = S.rc * S.rc;
+        nrec += 2 * (size_t)S.nr;
+        nF += (size_t)S.nrho;
+        rc = std::max(rc, S.rc);
+    }
+    if (nrec >= (1ull << 27)) throw Error(E_INVAL, "Multi: tables too large");
+    T.r64 = dalloc<double4>(nrec);
+    T.r32 = dalloc<float4>(nrec);
+    T.F64 = dalloc<double4>(nF);
+    T.objs = dalloc<EamObj>(npair);
+    for (int q = 0; q < npair; ++q) {
+        const EamTables& S = obj[q]->eam;
+        HIPCHECK(hipMemcpy(T.r64 + h[q].rho_off, S.r64, sizeof(double4) * 2 * S.nr, hipMemcpyDeviceToDevice));
+        HIPCHECK(hipMemcpy(T.r32 + h[q].rho_off, S.r32, sizeof(float4) * 2 * S.nr, hipMemcpyDeviceToDevice));
+        HIPCHECK(hipMemcpy(T.F64 + h[q].F_off, S.F64, sizeof(double4) * S.nrho, hipMemcpyDeviceToDevice));
+    }
+    HIPCHECK(hipMemcpy(T.objs, h.data(), sizeof(EamObj) * npair, hipMemcpyHostToDevice));
+    T.nrec = nrec;
+    T.nFrec = nF;
+    // the single-object fields mirror object 0 (unused by the Multi kernels)
+    T.nr = h[0].nr;
+    T.nrho = h[0].nrho;
+    T.dr = h[0].dr;
+    T.drho = h[0].drho;
+    T.rc = rc;
+    p->rcut = rc;
+    return p.release();
 }
 
 // ---- systems -------------------------------------------------------------
@@ -277,6 +346,7 @@ void free_batch(Batch& b) {
     dfree(b.nl);
     dfree(b.nn);
     dfree(b.misc);
+    dfree(b.fpm);
     dfree(b.d_desc);
     dfree(b.thermo);
     bpnn_release_work(b);
@@ -510,7 +580,7 @@ void download(Batch& b, AtomRec* a, uint32_t fields) {
 void check_kinds(const Batch& b, const Pot& p) {
     if (p.type == PT_EAM && p.eam.nk > 1)
         for (int o = 0; o < b.n; ++o)
-            if (b.kind[o] >= p.eam.nk) throw Error(E_INVAL, "atom kind outside the alloy EAM's elements");
+            if (b.kind[o] >= p.eam.nk) throw Error(E_INVAL, "atom kind outside the Multi potential's kinds");
 }
 
 }  // namespace
@@ -582,12 +652,14 @@ mdc_status mdc_pot_eam_create(mdc_ctx* ctx, double rcut, const double* F, const 
     });
 }
 
-mdc_status mdc_pot_eam_alloy_create(mdc_ctx* ctx, int nk, double rcut, const double* F, const double* rho,
-                                    const double* rphi, int nr, double dr, int nrho, double drho, mdc_pot** out) {
+mdc_status mdc_pot_multi_create(mdc_ctx* ctx, int nkinds, int npots, const int* kind_i, const int* kind_j,
+                                const mdc_pot* const* pots, mdc_pot** out) {
     return guard(ctx ? &ctx->c : nullptr, [&] {
-        if (!ctx || !out || !F || !rho || !rphi) throw Error(E_INVAL, "null argument");
+        if (!ctx || !out || !kind_i || !kind_j || !pots || npots < 1) throw Error(E_INVAL, "null argument");
         HIPCHECK(hipSetDevice(ctx->c.device));
-        *out = reinterpret_cast<mdc_pot*>(make_eam(&ctx->c, nk, rcut, F, rho, rphi, nr, dr, nrho, drho));
+        std::vector<const Pot*> pp(npots);
+        for (int q = 0; q < npots; ++q) pp[q] = pots[q] ? &reinterpret_cast<const mdc_pot*>(pots[q])->p : nullptr;
+        *out = reinterpret_cast<mdc_pot*>(make_multi(&ctx->c, nkinds, npots, kind_i, kind_j, pp.data()));
     });
 }
 

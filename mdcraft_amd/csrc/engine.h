@@ -53,19 +53,39 @@ struct LJParams {
     double a, s, s6, lj1, lj2, factor, rc, rc2;
 };
 
-// Spline tables of an (alloy) EAM potential.  Interval record k of a table
-// holds (y[k], z[k], y[k+1], z[k+1]); the record past the end repeats the
-// last knot, as the reference's [N1+1] read at N1 = N-1 assumes a value.
+// One EAM object of a Multi potential, used for one pair of kinds
+// (solver::Multi, multi.cxx:26-49): offsets of its rho and r*phi interval
+// records in the r-record array and of its F records, and its grids.
+struct EamObj {
+    int rho_off, phi_off, F_off, nr, nrho, pad;
+    double dr, dr1, drho, drho1, rc2;
+};
+constexpr int kMaxMultiKinds = 4;
+constexpr int kMaxPairs = kMaxMultiKinds * (kMaxMultiKinds + 1) / 2;
+__host__ __device__ inline int pair_index(int a, int b) {
+    const int hi = a > b ? a : b, lo = a > b ? b : a;
+    return hi * (hi + 1) / 2 + lo;
+}
+
+// Spline tables of an EAM potential: one element, or a Multi potential of
+// one object per pair of kinds.  Interval record k of a table holds
+// (y[k], z[k], y[k+1], z[k+1]); the record past the end repeats the last
+// knot, as the reference's [N1+1] read at N1 = N-1 assumes a value.
 struct EamTables {
     int nk = 1;
     int nr = 0, nrho = 0;
     double dr = 0, drho = 0, rc = 0;
-    // device: rho tables [nk][nr], then pair tables [npair][nr] of r*phi;
-    // embedding tables [nk][nrho]
+    // device: per object the rho records [nr] then the r*phi records [nr];
+    // embedding records [nrho] per object
     double4* r64 = nullptr;
     float4* r32 = nullptr;
     double4* F64 = nullptr;
     float2* knots32 = nullptr;  // (y, z) knots of every r table for LDS staging: [ntab][nr+1]
+    // Multi potential (nk > 1): objects per pair of kinds, records of all
+    // objects concatenated in r64 / r32 / F64
+    int npair = 0;
+    size_t nrec = 0, nFrec = 0;
+    EamObj* objs = nullptr;
 };
 
 struct BpnnParams;
@@ -123,6 +143,9 @@ struct Batch {
     std::vector<uint32_t> uid;
     std::vector<double> rcut_host;
     std::vector<uint8_t> periodic_bc;  // per system: solver built with the Periodic boundary
+    // F'(nc_i) of every atom under each object of a Multi potential: [nk][n]
+    double* fpm = nullptr;
+    size_t fpm_elems = 0;
     // scratch for BPNN
     void* bp_scratch = nullptr;
     size_t bp_scratch_bytes = 0;

@@ -108,15 +108,18 @@ __global__ void __launch_bounds__(256) k_lj(const double4* __restrict__ pos, dou
 }
 
 // --------------------------------------------------------------------------
-// EAM (single element, or setfl-alloy with per-kind density/embedding and a
-// symmetric pair table)
+// EAM: one element (EAM.cxx), or solver::Multi over one EAM object per pair
+// of kinds (multi.cxx:11-151)
 // --------------------------------------------------------------------------
 struct EamK {
-    int nk, nr, nrho, ntab;
+    int nk, nr, nrho, npair;
+    unsigned nrec;  // interval records in r64 / r32
     double dr, dr1, drho, drho1, rc2;
     const double4* r64;
     const float4* r32;
     const double4* F64;
+    const EamObj* objs;  // Multi: objects by pair_index(kind_i, kind_j)
+    double* fpm;         // Multi: F' of every atom under the object of its kind and kind k, [nk][n]
 };
 
 template <typename R>
@@ -133,26 +136,23 @@ __device__ inline const double4* rtab<double>(const EamK& P) { return P.r64; }
 template <>
 __device__ inline const float4* rtab<float>(const EamK& P) { return P.r32; }
 
-__device__ inline int pair_tab(int nk, int a, int b) {
-    int hi = a > b ? a : b, lo = a > b ? b : a;
-    return nk + hi * (hi + 1) / 2 + lo;
-}
-
-// EAM.cxx:562-589 embedding function and derivative (FP64); flags a density
-// outside the table, where the reference's branch is undefined.
-__device__ inline void embed(const EamK& P, int kind, double rho, double& F, double& dF, int* misc) {
-    double RN = rho * P.drho1;
+// EAM.cxx:562-589 embedding function and derivative (FP64) on the records
+// F64[off, off + nrho); flags a density outside the table, where the
+// reference's branch is undefined.
+__device__ inline void embed_at(const double4* __restrict__ F64, int off, int nrho, double drho, double drho1,
+                                double rho, double& F, double& dF, int* misc) {
+    double RN = rho * drho1;
     int N1 = (int)trunc(RN);
-    if (!(RN >= 0.0 && N1 < P.nrho)) {
+    if (!(RN >= 0.0 && N1 < nrho)) {
         atomicOr(&misc[M_ERR], EB_RHO_RANGE);
         F = dF = nan("");
         return;
     }
-    const double4 c = P.F64[(size_t)kind * P.nrho + N1];
+    const double4 c = F64[off + N1];
     double VR = N1;
     double DV = RN - VR;
-    RN = rho - VR * P.drho;
-    VR = (c.z - c.x) * P.drho1;
+    RN = rho - VR * drho;
+    VR = (c.z - c.x) * drho1;
     double aa = (c.w - VR) - (VR - c.y);
     double bb = aa - (VR - c.y);
     aa = aa * DV;
@@ -171,6 +171,7 @@ struct PairGeo {
     R R1, RN, R2;
     int N1;
     bool ok;
+    bool ghost;  // the pair takes the Periodic boundary's ghost route (single-point evaluations)
 };
 
 template <typename R>
@@ -178,7 +179,11 @@ __device__ inline PairGeo<R> pair_geo(const Frame& s, const double4& pi, const d
                                       double rc2, R dr1, int nr) {
     PairGeo<R> g;
     double d2;
-    g.ok = valid && pair_delta(s, pi, pj, e, g.dx, g.dy, g.dz, d2) && !(d2 > rc2 || d2 < 1e-15);
+    const bool img = valid && pair_delta(s, pi, pj, e, g.dx, g.dy, g.dz, d2);
+    // periodic.cxx:36-60: a listed pair goes through a ghost when its raw
+    // distance reaches max(rns) and its shortest image does not exceed it
+    g.ghost = img && s.any_per && sq3_exact(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z) >= s.rns2;
+    g.ok = img && !(d2 > rc2 || d2 < 1e-15);
     g.R2 = g.ok ? (R)d2 : (R)1;
     g.R1 = sqrt(g.R2);
     g.RN = g.R1 * dr1;
@@ -190,11 +195,34 @@ __device__ inline PairGeo<R> pair_geo(const Frame& s, const double4& pi, const d
     return g;
 }
 
-template <typename R, int TPA>
+__device__ inline int wave_max_group(int v, int tpa) {
+    for (int o = tpa >> 1; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+
+// Multi: the objects are staged in LDS once per workgroup (before any
+// early return, so every thread reaches the barrier).
+template <bool MULTI>
+__device__ inline void stage_objs(EamObj* s_obj, const EamK& P) {
+    if (!MULTI) return;
+    if ((int)threadIdx.x < P.npair) s_obj[threadIdx.x] = P.objs[threadIdx.x];
+    __syncthreads();
+}
+
+// Density pass (EAM.cxx:532-560), then F(rho) and F'(rho) per atom
+// (EAM.cxx:562-601).  Multi: atom i sums the rho table of the object of
+// (kind_i, kind_j) over each neighbour (multi.cxx:64-99) and keeps F' under
+// every object of its kind; Em is F of the object of the last non-empty
+// kind sub-list of the last pass over the atom's list (direct list, self
+// included, then the ghost list when it is not empty), as the per-object
+// virial() calls leave it (multi.cxx:128-151, EAM.cxx:306-354).
+template <typename R, int TPA, bool MULTI>
 __global__ void __launch_bounds__(256) k_eam_rho(const double4* __restrict__ pos, const int* __restrict__ nl,
                                                  const int* __restrict__ nn, int M, int cap,
                                                  const SysDesc* __restrict__ desc, EamK P, AtomArrays A, StepArgs S,
                                                  int* __restrict__ misc) {
+    __shared__ EamObj s_obj[MULTI ? kMaxPairs : 1];
+    stage_objs<MULTI>(s_obj, P);
     const AtomSlots as = atom_slots<TPA>(M);
     const int i = as.i;
     const bool valid = i < A.n;
@@ -210,9 +238,11 @@ __global__ void __launch_bounds__(256) k_eam_rho(const double4* __restrict__ pos
     const int cnt = active ? min(nraw, cap) : 0;
     const auto* tab = rtab<R>(P);
     const R dr = (R)P.dr, dr1 = (R)P.dr1;
+    const int ki = MULTI ? w_kind(pi.w) : 0;
+    int kd = ki, kg = -1;  // Multi: largest kind in the direct list (self included) and among ghost pairs
     double rho = 0.0;
     const __amdgpu_buffer_rsrc_t posb = make_buf(pos, (unsigned)A.n * 32u);
-    const __amdgpu_buffer_rsrc_t tabb = make_buf(tab, (unsigned)(P.ntab * P.nr) * (unsigned)sizeof(*tab));
+    const __amdgpu_buffer_rsrc_t tabb = make_buf(tab, P.nrec * (unsigned)sizeof(*tab));
     for (int m0 = 0; m0 * TPA + as.sub < cnt; m0 += kU) {
         int e[kU];
         bool okk[kU];
@@ -226,54 +256,88 @@ __global__ void __launch_bounds__(256) k_eam_rho(const double4* __restrict__ pos
 #pragma unroll
         for (int u = 0; u < kU; ++u) pj[u] = buf_d4(posb, (unsigned)ent_j(e[u]) * 32u);
         PairGeo<R> g[kU];
-#pragma unroll
-        for (int u = 0; u < kU; ++u) g[u] = pair_geo<R>(fr, pi, pj[u], e[u], okk[u], P.rc2, dr1, P.nr);
-        typename Vec4T<R>::T rec[kU];
+        int ob[kU];
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
-            const int kj = P.nk > 1 ? w_kind(pj[u].w) : 0;
-            rec[u] = load_rec<R>(tabb, kj * P.nr + g[u].N1);
+            if (!MULTI) {
+                ob[u] = 0;
+                g[u] = pair_geo<R>(fr, pi, pj[u], e[u], okk[u], P.rc2, dr1, P.nr);
+            } else {
+                const int kj = w_kind(pj[u].w);
+                ob[u] = pair_index(ki, kj);
+                const EamObj& o = s_obj[ob[u]];
+                g[u] = pair_geo<R>(fr, pi, pj[u], e[u], okk[u], o.rc2, (R)o.dr1, o.nr);
+                if (okk[u]) kd = max(kd, kj);
+                if (g[u].ghost) kg = max(kg, kj);
+            }
         }
+        typename Vec4T<R>::T rec[kU];
 #pragma unroll
-        for (int u = 0; u < kU; ++u)
-            if (g[u].ok) rho += (double)spline_v<R>(rec[u], g[u].R1, g[u].RN, g[u].N1, dr, dr1);
+        for (int u = 0; u < kU; ++u) rec[u] = load_rec<R>(tabb, (MULTI ? s_obj[ob[u]].rho_off : 0) + g[u].N1);
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            if (!g[u].ok) continue;
+            if (!MULTI) rho += (double)spline_v<R>(rec[u], g[u].R1, g[u].RN, g[u].N1, dr, dr1);
+            else {
+                const EamObj& o = s_obj[ob[u]];
+                rho += (double)spline_v<R>(rec[u], g[u].R1, g[u].RN, g[u].N1, (R)o.dr, (R)o.dr1);
+            }
+        }
     }
     rho = wave_sum_group(rho, TPA);
+    if (MULTI) {
+        kd = wave_max_group(kd, TPA);
+        kg = wave_max_group(kg, TPA);
+    }
     if (!valid || as.sub != 0) return;
     const int n = A.n;
-    const int ki = P.nk > 1 ? w_kind(pi.w) : 0;
-    double F, dF;
-    if (!active) {
-        // tag == 0: the reference skips the atom's prepare and charge, yet
-        // neighbours still evaluate F'(nc) of its stored density
-        // (single.cxx:44-50, EAM.cxx:467-474)
-        embed(P, ki, A.d[(size_t)D_NC * n + i], F, dF, misc);
+    // tag == 0: the reference skips the atom's prepare and charge, yet
+    // neighbours still evaluate F'(nc) of its stored density
+    // (single.cxx:44-50, EAM.cxx:467-474)
+    const double rr = active ? rho : A.d[(size_t)D_NC * n + i];
+    if (!MULTI) {
+        double F, dF;
+        embed_at(P.F64, 0, P.nrho, P.drho, P.drho1, rr, F, dF, misc);
         A.d[(size_t)D_FP * n + i] = dF;
-        return;
+        if (active && S.store_aux && (S.flags & W_ENERGY)) A.d[(size_t)D_EM * n + i] = F;
+    } else {
+        const int klast = kg >= 0 ? kg : kd;
+        for (int k = 0; k < P.nk; ++k) {
+            const EamObj& o = s_obj[pair_index(ki, k)];
+            double F, dF;
+            embed_at(P.F64, o.F_off, o.nrho, o.drho, o.drho1, rr, F, dF, misc);
+            P.fpm[(size_t)k * n + i] = dF;
+            if (active && k == klast && S.store_aux && (S.flags & W_ENERGY)) A.d[(size_t)D_EM * n + i] = F;
+        }
     }
-    embed(P, ki, rho, F, dF, misc);
-    A.d[(size_t)D_FP * n + i] = dF;
-    if (S.store_aux) {
-        A.d[(size_t)D_NC * n + i] = rho;
-        if (S.flags & W_ENERGY) A.d[(size_t)D_EM * n + i] = F;
-    }
+    if (active && S.store_aux) A.d[(size_t)D_NC * n + i] = rho;
 }
 
-template <typename R, int TPA, bool ALLOY>
+// Force pass (EAM.cxx:356-497) and, when asked, pair energy and virial
+// (EAM.cxx:306-354); the velocity-Verlet finish is fused.  Multi: each pair
+// uses the object of (kind_i, kind_j): its rho'(r), its pair term and
+// F'(nc_i) + F'(nc_j) under its own embedding table.
+template <typename R, int TPA, bool MULTI>
 __global__ void __launch_bounds__(256) k_eam_force(const double4* __restrict__ pos, double4* __restrict__ pos_next,
                                                    const int* __restrict__ nl, const int* __restrict__ nn, int M,
                                                    int cap, const SysDesc* __restrict__ desc, EamK P, AtomArrays A,
                                                    StepArgs S, const double4* __restrict__ rsave,
                                                    int* __restrict__ misc) {
+    __shared__ EamObj s_obj[MULTI ? kMaxPairs : 1];
+    stage_objs<MULTI>(s_obj, P);
     const AtomSlots as = atom_slots<TPA>(M);
     const int i = as.i;
     const bool valid = i < A.n;
     const int n = A.n;
-    const double* __restrict__ Fp = A.d + (size_t)D_FP * n;
+    const double* __restrict__ Fp = MULTI ? P.fpm : A.d + (size_t)D_FP * n;
     const int ab = abort_flag(S, misc);
     const double4 pi = valid ? pos[i] : make_double4(0, 0, 0, 0);
     const int nraw = valid ? nn[i] : 0;
-    const double Fpi0 = valid ? Fp[i] : 0.0;
+    const int ki = MULTI ? w_kind(pi.w) : 0;
+    double Fpk[MULTI ? kMaxMultiKinds : 1];  // F'(nc_i) under the object of (kind_i, k)
+#pragma unroll
+    for (int k = 0; k < (MULTI ? kMaxMultiKinds : 1); ++k)
+        Fpk[k] = (valid && k < (MULTI ? P.nk : 1)) ? Fp[(size_t)k * n + i] : 0.0;
     int j0[kU];
     first_group<TPA, kU>(as, valid, M, nl, j0);
     if (ab <= S.step) return;
@@ -281,15 +345,14 @@ __global__ void __launch_bounds__(256) k_eam_force(const double4* __restrict__ p
     const SysDesc& s = desc[valid ? w_sys(pi.w) : 0];
     const Frame fr = frame_of(s);
     const int cnt = active ? min(nraw, cap) : 0;
-    const double Fpi = active ? Fpi0 : 0.0;
     const auto* tab = rtab<R>(P);
     const R dr = (R)P.dr, dr1 = (R)P.dr1;
-    const int ki = P.nk > 1 ? w_kind(pi.w) : 0;
     R fx = 0, fy = 0, fz = 0;
     double en = 0, vxx = 0, vyy = 0, vzz = 0, vxy = 0, vxz = 0, vyz = 0;
     const __amdgpu_buffer_rsrc_t posb = make_buf(pos, (unsigned)n * 32u);
-    const __amdgpu_buffer_rsrc_t fpb = make_buf(Fp, (unsigned)n * 8u);
-    const __amdgpu_buffer_rsrc_t tabb = make_buf(tab, (unsigned)(P.ntab * P.nr) * (unsigned)sizeof(*tab));
+    const __amdgpu_buffer_rsrc_t fpb = make_buf(Fp, (unsigned)(MULTI ? P.nk : 1) * (unsigned)n * 8u);
+    const __amdgpu_buffer_rsrc_t tabb = make_buf(tab, P.nrec * (unsigned)sizeof(*tab));
+    const unsigned fp_base = (unsigned)ki * (unsigned)n;  // F' of neighbours under the objects of (k_j, kind_i)
     for (int m0 = 0; m0 * TPA + as.sub < cnt; m0 += kU) {
         int e[kU];
         bool okk[kU];
@@ -304,22 +367,32 @@ __global__ void __launch_bounds__(256) k_eam_force(const double4* __restrict__ p
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             pj[u] = buf_d4(posb, (unsigned)ent_j(e[u]) * 32u);
-            Fpj[u] = (R)buf_d1(fpb, (unsigned)ent_j(e[u]) * 8u);
+            Fpj[u] = (R)buf_d1(fpb, (fp_base + (unsigned)ent_j(e[u])) * 8u);
         }
         PairGeo<R> g[kU];
-#pragma unroll
-        for (int u = 0; u < kU; ++u) g[u] = pair_geo<R>(fr, pi, pj[u], e[u], okk[u], P.rc2, dr1, P.nr);
-        typename Vec4T<R>::T rr[kU], rp[kU], rq[ALLOY ? kU : 1];
+        int ob[kU], kj[kU];
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
-            if (!ALLOY) {
+            if (!MULTI) {
+                ob[u] = kj[u] = 0;
+                g[u] = pair_geo<R>(fr, pi, pj[u], e[u], okk[u], P.rc2, dr1, P.nr);
+            } else {
+                kj[u] = w_kind(pj[u].w);
+                ob[u] = pair_index(ki, kj[u]);
+                const EamObj& o = s_obj[ob[u]];
+                g[u] = pair_geo<R>(fr, pi, pj[u], e[u], okk[u], o.rc2, (R)o.dr1, o.nr);
+            }
+        }
+        typename Vec4T<R>::T rr[kU], rp[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            if (!MULTI) {
                 rr[u] = load_rec<R>(tabb, g[u].N1);
                 rp[u] = load_rec<R>(tabb, P.nr + g[u].N1);
             } else {
-                const int kj = w_kind(pj[u].w);
-                rr[u] = load_rec<R>(tabb, kj * P.nr + g[u].N1);
-                rq[ALLOY ? u : 0] = load_rec<R>(tabb, ki * P.nr + g[u].N1);
-                rp[u] = load_rec<R>(tabb, pair_tab(P.nk, ki, kj) * P.nr + g[u].N1);
+                const EamObj& o = s_obj[ob[u]];
+                rr[u] = load_rec<R>(tabb, o.rho_off + g[u].N1);
+                rp[u] = load_rec<R>(tabb, o.phi_off + g[u].N1);
             }
         }
 #pragma unroll
@@ -327,19 +400,23 @@ __global__ void __launch_bounds__(256) k_eam_force(const double4* __restrict__ p
             if (!g[u].ok) continue;
             const R R1 = g[u].R1, RN = g[u].RN, R2 = g[u].R2;
             const int N1 = g[u].N1;
-            R nd, rv, phv, phd, FF;
-            spline_vd<R>(rp[u], R1, RN, N1, dr, dr1, phv, phd);
-            spline_vd<R>(rr[u], R1, RN, N1, dr, dr1, rv, nd);
+            R drp = dr, dr1p = dr1;
+            double Fpi = Fpk[0];
+            if (MULTI) {
+                const EamObj& o = s_obj[ob[u]];
+                drp = (R)o.dr;
+                dr1p = (R)o.dr1;
+#pragma unroll
+                for (int k = 1; k < kMaxMultiKinds; ++k)
+                    if (kj[u] == k) Fpi = Fpk[k];
+            }
+            R nd, rv, phv, phd;
+            spline_vd<R>(rp[u], R1, RN, N1, drp, dr1p, phv, phd);
+            spline_vd<R>(rr[u], R1, RN, N1, drp, dr1p, rv, nd);
             // FF = (AdF rho'(r) + phi'(r)) / r with phi' = ((r phi)' r - r phi) / r^2
             const R i1 = rcp_(R1), i2 = rcp_(R2);
-            if (!ALLOY) {
-                const R AdF = (R)Fpi + Fpj[u];
-                FF = (AdF * nd + (phd * R1 - phv) * i2) * i1;
-            } else {
-                R ndi;
-                spline_vd<R>(rq[ALLOY ? u : 0], R1, RN, N1, dr, dr1, rv, ndi);
-                FF = ((R)Fpi * nd + Fpj[u] * ndi + (phd * R1 - phv) * i2) * i1;
-            }
+            const R AdF = (R)(active ? Fpi : 0.0) + Fpj[u];
+            const R FF = (AdF * nd + (phd * R1 - phv) * i2) * i1;
             if (S.flags & W_ENERGY) en += 0.5 * (double)(phv * i1);
             const R rx = (R)g[u].dx, ry = (R)g[u].dy, rz = (R)g[u].dz;
             fx += FF * rx;
@@ -550,12 +627,13 @@ void set_trigger(Batch& b, double rcut, bool allow_shift) {
                                 b.ctx->stream));
 }
 
-static EamK eam_args(const Pot& p) {
+static EamK eam_args(const Pot& p, Batch& b) {
     EamK k;
     k.nk = p.eam.nk;
     k.nr = p.eam.nr;
     k.nrho = p.eam.nrho;
-    k.ntab = p.eam.nk + p.eam.nk * (p.eam.nk + 1) / 2;
+    k.npair = p.eam.npair;
+    k.nrec = (unsigned)p.eam.nrec;
     k.dr = p.eam.dr;
     k.dr1 = 1.0 / p.eam.dr;
     k.drho = p.eam.drho;
@@ -564,6 +642,20 @@ static EamK eam_args(const Pot& p) {
     k.r64 = p.eam.r64;
     k.r32 = p.eam.r32;
     k.F64 = p.eam.F64;
+    k.objs = p.eam.objs;
+    k.fpm = nullptr;
+    if (p.eam.nk > 1) {
+        const size_t need = (size_t)p.eam.nk * (size_t)b.n;
+        if (b.fpm_elems < need) {
+            if (b.fpm) HIPCHECK(hipFree(b.fpm));
+            b.fpm = nullptr;
+            b.fpm_elems = 0;
+            if (hipMalloc(&b.fpm, need * sizeof(double)) != hipSuccess)
+                throw Error(E_NOMEM, "out of device memory for the Multi embedding derivatives");
+            b.fpm_elems = need;
+        }
+        k.fpm = b.fpm;
+    }
     return k;
 }
 
@@ -585,10 +677,15 @@ static void launch_pair_t(Batch& b, const Pot& p, const StepArgs& S, const doubl
         launch_k(b, K_FORCE, k_lj<R, TPA>, dim3(grid), dim3(256), 0, pos, pos_next, b.nl, b.nn, b.M, b.cap, b.d_desc,
                  lj_args(p), AA, S, b.rsave, b.misc);
     } else {
-        EamK E = eam_args(p);
-        if (which & 1)
-            launch_k(b, K_DENSITY, k_eam_rho<R, TPA>, dim3(grid), dim3(256), 0, pos, b.nl, b.nn, b.M, b.cap, b.d_desc,
-                     E, AA, S, b.misc);
+        EamK E = eam_args(p, b);
+        if (which & 1) {
+            if (E.nk > 1)
+                launch_k(b, K_DENSITY, k_eam_rho<R, TPA, true>, dim3(grid), dim3(256), 0, pos, b.nl, b.nn, b.M, b.cap,
+                         b.d_desc, E, AA, S, b.misc);
+            else
+                launch_k(b, K_DENSITY, k_eam_rho<R, TPA, false>, dim3(grid), dim3(256), 0, pos, b.nl, b.nn, b.M,
+                         b.cap, b.d_desc, E, AA, S, b.misc);
+        }
         if (!(which & 2)) return;
         if (E.nk > 1)
             launch_k(b, K_FORCE, k_eam_force<R, TPA, true>, dim3(grid), dim3(256), 0, pos, pos_next, b.nl, b.nn, b.M,

@@ -28,7 +28,7 @@ F_ALL = (1 << 14) - 1
 
 EXPORTS = [
     "mdc_ctx_create", "mdc_ctx_destroy", "mdc_ctx_last_error", "mdc_last_error", "mdc_atom_record_size",
-    "mdc_pot_lj_create", "mdc_pot_eam_create", "mdc_pot_eam_create_file", "mdc_pot_eam_alloy_create",
+    "mdc_pot_lj_create", "mdc_pot_eam_create", "mdc_pot_eam_create_file", "mdc_pot_multi_create",
     "mdc_pot_bpnn_create", "mdc_pot_rcut", "mdc_pot_destroy",
     "mdc_sys_create", "mdc_sys_destroy", "mdc_sys_upload", "mdc_sys_download", "mdc_sys_natoms",
     "mdc_sys_set_lanes", "mdc_sys_autotune", "mdc_sys_time_kernels", "mdc_nlist_build", "mdc_nlist_decide", "mdc_nlist_export", "mdc_nlist_stats",
@@ -54,7 +54,7 @@ def load_library(path=LIB_PATH):
         "mdc_pot_lj_create": (I, [P, D, D, D, PP]),
         "mdc_pot_eam_create": (I, [P, D, DP, DP, DP, I, D, I, D, PP]),
         "mdc_pot_eam_create_file": (I, [P, C.c_char_p, PP]),
-        "mdc_pot_eam_alloy_create": (I, [P, I, D, DP, DP, DP, I, D, I, D, PP]),
+        "mdc_pot_multi_create": (I, [P, I, I, C.POINTER(I), C.POINTER(I), PP, PP]),
         "mdc_pot_bpnn_create": (I, [P, C.c_char_p, PP]),
         "mdc_pot_rcut": (I, [P, DP]),
         "mdc_pot_destroy": (I, [P]),
@@ -160,14 +160,15 @@ def EAM(ctx, rcut=None, F=None, rho=None, rphi=None, dr=None, drho=None, path=No
     return Potential(ctx, h, "eam")
 
 
-def EAMAlloy(ctx, rcut, F, rho, rphi, dr, drho):
-    """Alloy EAM: F [nk, nrho], rho [nk, nr], rphi [nk(nk+1)/2, nr] (pair
-    index a(a+1)/2 + b for kinds a >= b)."""
-    F, rho, rphi = (np.ascontiguousarray(x, dtype=np.float64) for x in (F, rho, rphi))
-    nk = F.shape[0]
+def Multi(ctx, nkinds, pairs):
+    """solver::Multi(nkinds) with add_potential(i, j, pot) for every
+    (i, j, pot) in pairs (solver/multi.cxx:11-49); members are EAM objects."""
+    pairs = list(pairs)
+    ki = (C.c_int * len(pairs))(*[p[0] for p in pairs])
+    kj = (C.c_int * len(pairs))(*[p[1] for p in pairs])
+    hs = (C.c_void_p * len(pairs))(*[p[2].h for p in pairs])
     h = C.c_void_p()
-    _check(lib().mdc_pot_eam_alloy_create(ctx.h, nk, rcut, _dp(F), _dp(rho), _dp(rphi), rho.shape[1], dr,
-                                          F.shape[1], drho, C.byref(h)))
+    _check(lib().mdc_pot_multi_create(ctx.h, nkinds, len(pairs), ki, kj, hs, C.byref(h)))
     return Potential(ctx, h, "eam")
 
 

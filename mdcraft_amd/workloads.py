@@ -317,22 +317,18 @@ def nial_cell(index, target=None):
     return at, _box(L)
 
 
-def nial_tables():
-    """Alloy tables: Ni (kind 0) density/embedding x1.1, Al (kind 1) x0.9,
-    one Morse pair for Ni-Ni and Al-Al and their mean for Ni-Al."""
-    ni = eam_cu_tables(scale_rho=1.1, scale_F=1.1)
-    al = eam_cu_tables(scale_rho=0.9, scale_F=0.9)
-    rphi = ni["rphi"]
-    return {
-        "F": np.stack([ni["F"], al["F"]]),
-        "rho": np.stack([ni["rho"], al["rho"]]),
-        "rphi": np.stack([rphi, 0.5 * (rphi + al["rphi"]), al["rphi"]]),
-        "dr": ni["dr"],
-        "drho": ni["drho"],
-        "rc": ni["rc"],
-    }
+def nial_pots():
+    """Multi potential over three EAM objects, as the reference's NiAl
+    problem builds it (problems/NiAl/test-NiAl.py:120-122): Ni-Ni for kind
+    0, Al-Al for kind 1 and Ni-Al for the mixed pairs.  Each object keeps
+    the EAM Cu functional forms: Ni scales rho and F by 1.1, Al by 0.9,
+    Ni-Al by 1.0; all three share the pair term."""
+    ni = dict(kind="eam", **eam_cu_tables(scale_rho=1.1, scale_F=1.1))
+    al = dict(kind="eam", **eam_cu_tables(scale_rho=0.9, scale_F=0.9))
+    nial = dict(kind="eam", **eam_cu_tables())
+    return {"kind": "eam_multi", "nkinds": 2, "pairs": [(0, 0, ni), (1, 1, al), (0, 1, nial)], "rc": ni["rc"]}
 
 
 def nial_batch(count=512):
     cells = [nial_cell(i) for i in range(count)]
-    return {"cells": cells, "pot": dict(kind="eam_alloy", **nial_tables()), "dt": 1 * FS, "name": "nial_batch"}
+    return {"cells": cells, "pot": nial_pots(), "dt": 1 * FS, "name": "nial_batch"}

@@ -439,11 +439,14 @@ struct Potential {
     int type;
     double R1cut = 0, R2cut = 0;
     bool half = false;  // the list the per-atom calls receive is a half list
+    bool multi = false;  // solver::Multi: prepare keeps rcut, always zeroes Em and nc
     explicit Potential(int t) : type(t) {}
     virtual ~Potential() {}
     virtual void force(Atom& atom, Atom* neibs, const size_t* list, size_t nl) {}
     virtual void virial(Atom& atom, Atom* neibs, const size_t* list, size_t nl) {}
     virtual void density(Atom& atom, Atom* neibs, const size_t* list, size_t nl) {}
+    // called once before a stage runs on the thread pool (whose tasks must not throw)
+    virtual void validate(const Atom* atoms, size_t n) const {}
 };
 
 // LJsCut (mdcraft/solver/potential/LJs-cut.cxx:8-131)
@@ -791,6 +794,50 @@ struct EAM : Potential {
     }
     void virial(Atom& atom, Atom* neibs, const size_t* list, size_t nl) override {
         gather(atom, neibs, list, nl, true);
+    }
+};
+
+// solver::Multi (mdcraft/solver/multi.cxx:11-151): one potential object per
+// pair of kinds, symmetric.  For every atom each stage splits the list it
+// receives (the direct list, then the ghost list) by neighbour kind, keeping
+// list order, and hands sub-list k to potentials[kind_i][k], kinds in
+// ascending order.  An EAM object therefore sums its own rho table into nc,
+// uses its own F table for F'(nc_i) and F'(nc_j), and its virial() sets Em,
+// so the last non-empty sub-list of the last pass decides Em.  Stages: any
+// EAM object gives [charge, forces] (multi.cxx:26-49).
+struct Multi : Potential {
+    int nk;
+    std::vector<Potential*> pots;  // [nk * nk]
+    explicit Multi(int nkinds) : Potential(PT_PAIR), nk(nkinds), pots((size_t)nkinds * nkinds, nullptr) {
+        multi = true;
+    }
+    void add(int i, int j, Potential* p) {
+        pots[(size_t)i * nk + j] = p;
+        pots[(size_t)j * nk + i] = p;
+        if (p->type == PT_EAM) type = PT_EAM;
+        R1cut = std::max(R1cut, p->R1cut);
+        R2cut = R1cut * R1cut;
+    }
+    void validate(const Atom* atoms, size_t n) const override {
+        for (size_t i = 0; i < n; ++i)
+            if (atoms[i].kind >= nk) throw std::runtime_error("atom kind outside the Multi solver's kinds");
+        for (auto* p : pots)
+            if (!p) throw std::runtime_error("Multi: a pair of kinds has no potential");
+    }
+    template <class F>
+    void split(const Atom& atom, Atom* neibs, const size_t* list, size_t nl, F&& call) {
+        std::vector<std::vector<size_t>> sub(nk);
+        for (size_t i = 0; i < nl; ++i) sub[neibs[list[i]].kind].push_back(list[i]);
+        for (int k = 0; k < nk; ++k) call(pots[(size_t)atom.kind * nk + k], sub[k]);
+    }
+    void density(Atom& atom, Atom* neibs, const size_t* list, size_t nl) override {
+        split(atom, neibs, list, nl, [&](Potential* p, std::vector<size_t>& l) { p->density(atom, neibs, l.data(), l.size()); });
+    }
+    void force(Atom& atom, Atom* neibs, const size_t* list, size_t nl) override {
+        split(atom, neibs, list, nl, [&](Potential* p, std::vector<size_t>& l) { p->force(atom, neibs, l.data(), l.size()); });
+    }
+    void virial(Atom& atom, Atom* neibs, const size_t* list, size_t nl) override {
+        split(atom, neibs, list, nl, [&](Potential* p, std::vector<size_t>& l) { p->virial(atom, neibs, l.data(), l.size()); });
     }
 };
 
@@ -1233,6 +1280,7 @@ struct Engine {
     }
 
     void prepare() {
+        pot->validate(atoms, n);
         if (pot->type == PT_TERNARY) {
             auto* b = static_cast<BPNN*>(pot);
             for (size_t i = 0; i < n; ++i)
@@ -1258,11 +1306,11 @@ struct Engine {
                 std::memset(a.f, 0, sizeof(a.f));
                 std::memset(a.V, 0, sizeof(a.V));
                 a.Ep = 0.0;
-                if (pot->type == PT_EAM) {
+                if (pot->type == PT_EAM || pot->multi) {
                     a.Em = 0.0;
                     a.nc = 0.0;
                 }
-                a.rcut = pot->R1cut;
+                if (!pot->multi) a.rcut = pot->R1cut;  // multi.cxx:56-66 leaves rcut
             }
             a.bc = 0x0;
             if (periodic_bc && a.tag != 0) {
@@ -1449,6 +1497,25 @@ void* ref_bpnn_create(const char* path, const ref_domain* d) {
         g_err = e.what();
         return nullptr;
     }
+}
+
+void* ref_multi_create(int nkinds) {
+    if (nkinds < 1 || nkinds > 8) {
+        g_err = "Multi kinds must be 1..8";
+        return nullptr;
+    }
+    return new Multi(nkinds);
+}
+
+// members stay owned by the caller
+int ref_multi_add(void* m, int i, int j, void* pot) {
+    auto* mu = static_cast<Multi*>(m);
+    if (i < 0 || j < 0 || i >= mu->nk || j >= mu->nk || !pot) {
+        g_err = "Multi.add_potential: kind out of range";
+        return -1;
+    }
+    mu->add(i, j, static_cast<Potential*>(pot));
+    return 0;
 }
 
 void ref_pot_free(void* p) { delete static_cast<Potential*>(p); }

@@ -37,6 +37,8 @@ def _load():
         "ref_eam_create": (P, [D, Z, Z, Z, I, D, I, D]),
         "ref_eam_create_file": (P, [C.c_char_p]),
         "ref_bpnn_create": (P, [C.c_char_p, dom]),
+        "ref_multi_create": (P, [I]),
+        "ref_multi_add": (I, [P, I, I, P]),
         "ref_pot_free": (None, [P]),
         "ref_pot_rcut": (D, [P]),
         "ref_pot_type": (I, [P]),
@@ -147,6 +149,23 @@ def EAM(rc=None, F=None, rho=None, rphi=None, dr=None, drho=None, path=None):
         return Potential(lib.ref_eam_create_file(path.encode()))
     F, rho, rphi = (np.ascontiguousarray(x, dtype=np.float64) for x in (F, rho, rphi))
     return Potential(lib.ref_eam_create(rc, _dp(F), _dp(rho), _dp(rphi), len(rho), dr, len(F), drho))
+
+
+class Multi(Potential):
+    """solver::Multi(nkinds) with add_potential(i, j, pot) (multi.cxx:11-49);
+    keeps its member potentials alive."""
+
+    def __init__(self, nkinds, pairs=()):
+        super().__init__(lib.ref_multi_create(nkinds))
+        self.nkinds = nkinds
+        self.members = []
+        for i, j, pot in pairs:
+            self.add_potential(i, j, pot)
+
+    def add_potential(self, i, j, pot):
+        if lib.ref_multi_add(self.h, i, j, pot.h):
+            raise RuntimeError(_err())
+        self.members.append(pot)
 
 
 def MaiseBPNN(path, domain):

@@ -22,6 +22,8 @@ def oracle_potential(R, pot, box=None):
         return R.EAM(rc=pot["rc"], F=pot["F"], rho=pot["rho"], rphi=pot["rphi"], dr=pot["dr"], drho=pot["drho"])
     if k == "bpnn":
         return R.MaiseBPNN(pot["path"], R.Domain(*box))
+    if k == "eam_multi":
+        return R.Multi(pot["nkinds"], [(i, j, oracle_potential(R, q)) for i, j, q in pot["pairs"]])
     raise ValueError(k)
 
 
@@ -34,8 +36,8 @@ def engine_potential(E, ctx, pot):
             return E.EAM(ctx, path=pot["path"])
         return E.EAM(ctx, rcut=pot["rc"], F=pot["F"], rho=pot["rho"], rphi=pot["rphi"], dr=pot["dr"],
                      drho=pot["drho"])
-    if k == "eam_alloy":
-        return E.EAMAlloy(ctx, pot["rc"], pot["F"], pot["rho"], pot["rphi"], pot["dr"], pot["drho"])
+    if k == "eam_multi":
+        return E.Multi(ctx, pot["nkinds"], [(i, j, engine_potential(E, ctx, q)) for i, j, q in pot["pairs"]])
     if k == "bpnn":
         return E.MaiseBPNNPotential(ctx, pot["path"])
     raise ValueError(k)
