@@ -65,7 +65,92 @@ def make_pot(E, ctx, pot):
     if pot["kind"] == "eam":
         return E.EAM(ctx, rcut=pot["rc"], F=pot["F"], rho=pot["rho"], rphi=pot["rphi"], dr=pot["dr"],
                      drho=pot["drho"])
+    if pot["kind"] == "eam_multi":
+        return E.Multi(ctx, pot["nkinds"], [(i, j, make_pot(E, ctx, q)) for i, j, q in pot["pairs"]])
+    if pot["kind"] == "bpnn":
+        return E.MaiseBPNNPotential(ctx, pot["path"])
     raise ValueError(pot["kind"])
+
+
+KINDS = ["force", "density", "predrift", "bp_desc", "bp_mlp", "bp_force", "bp_gather", "rebuild"]
+MLP_FLOP_PER_ATOM = 2 * (16 * 30 + 30 * 30 + 30) + 2 * (30 + 30 * 30 + 30 * 16)  # forward + input gradient
+MFMA_FP16_PEAK_TFLOPS = 2500.0
+
+
+def run_batch(E, ctx, cells, pot, dt, precision, steps, warmup, lanes=None):
+    s = E.System(ctx, [len(a) for a, _ in cells], [b for _, b in cells], True, lanes=lanes)
+    s.upload(np.concatenate([a for a, _ in cells]))
+    p = make_pot(E, ctx, pot)
+    s.build()
+    tune = None
+    if lanes is None:
+        lanes, tune = s.autotune(p, precision, reps=10)
+    s.compute(p, E.FORCE, precision)
+    if warmup:
+        s.step(p, dt, warmup, precision)
+    s.sync()
+    t0 = time.perf_counter()
+    reb = s.step(p, dt, steps, precision)
+    s.sync()
+    t1 = time.perf_counter()
+    s.timing(1)
+    s.step(p, dt, min(steps, 100), precision)
+    s.sync()
+    ms, nl = s.timing(0)
+    n = s.n
+    return {"value": n * steps / (t1 - t0), "ms_per_step": 1e3 * (t1 - t0) / steps, "atoms": n,
+            "systems": len(cells), "rebuilds": reb, "nbar": s.stats()["mean"], "lanes": lanes,
+            "kernel_us": {KINDS[k]: round(ms[k] / nl[k] * 1e3, 2) for k in range(8) if nl[k] > 0}}
+
+
+def run_batch_eval(E, ctx, cells, pot, precision, reps, lanes=None):
+    """Energy, forces and virial of every system of a batch, one density and
+    one force launch per evaluation for all systems together (state
+    resident; K evaluations timed)."""
+    s = E.System(ctx, [len(a) for a, _ in cells], [b for _, b in cells], True, lanes=lanes)
+    s.upload(np.concatenate([a for a, _ in cells]))
+    p = make_pot(E, ctx, pot)
+    s.build()
+    tune = None
+    if lanes is None:
+        lanes, tune = s.autotune(p, precision, reps=10)
+    what = E.FORCE | E.ENERGY | E.VIRIAL
+    s.compute(p, what, precision)
+    s.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        s.compute(p, what, precision)
+    s.sync()
+    t1 = time.perf_counter()
+    s.timing(1)
+    for _ in range(min(reps, 20)):
+        s.compute(p, what, precision)
+    ms, nl = s.timing(0)
+    n = s.n
+    return {"value": n * reps / (t1 - t0), "unit": "atom-evaluations/s", "ms_per_eval": 1e3 * (t1 - t0) / reps,
+            "atoms": n, "systems": len(cells), "nbar": s.stats()["mean"], "lanes": lanes,
+            "kernel_us": {KINDS[k]: round(ms[k] / nl[k] * 1e3, 2) for k in range(8) if nl[k] > 0}}
+
+
+def one_at_a_time(E, ctx, cells, pot, reps):
+    """The plain FP64 GPU path: each system alone in its own engine System,
+    evaluated in turn (state resident; creation and upload not timed)."""
+    p = make_pot(E, ctx, pot)
+    what = E.FORCE | E.ENERGY | E.VIRIAL
+    systems = []
+    for a, b in cells:
+        s = E.System(ctx, [len(a)], [b], True)
+        s.upload(a)
+        s.build()
+        s.compute(p, what, E.FP64)
+        systems.append(s)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for s in systems:
+            s.compute(p, what, E.FP64)
+    t1 = time.perf_counter()
+    n = sum(len(a) for a, _ in cells)
+    return {"value": n * reps / (t1 - t0), "unit": "atom-evaluations/s", "systems": len(cells), "atoms": n}
 
 
 def run_gpu(E, ctx, w, precision, steps, warmup, lanes=None):
@@ -82,6 +167,12 @@ def run_gpu(E, ctx, w, precision, steps, warmup, lanes=None):
     e0 = s.thermo()[0]
     if warmup:
         s.step(p, w["dt"], warmup, precision)
+    # mean durations of the step-mode pair kernels: back-to-back launches on
+    # a fresh list of the state the timed steps start from (the hook leaves
+    # positions, velocities and forces as they are)
+    s.build()
+    nbar_b2b = s.stats()["mean"]
+    b2b = s.time_kernels(p, precision, w["dt"], reps=200) if pot["kind"] in ("lj", "eam") else (0.0, 0.0)
     s.sync()
     t0 = time.perf_counter()
     reb = s.step(p, w["dt"], steps, precision)
@@ -94,20 +185,18 @@ def run_gpu(E, ctx, w, precision, steps, warmup, lanes=None):
     s.step(p, w["dt"], min(steps, 200), precision)
     s.sync()
     ms, nl = s.timing(0)
-    s.build()  # a fresh list, as after any rebuild: no atom past the trigger
-    b2b = s.time_kernels(p, precision, w["dt"], reps=200)
     s.compute(p, E.FORCE | E.ENERGY | E.VIRIAL, precision)
     e1 = s.thermo()[0]
     n = len(at)
     t_ps = (steps + warmup) * w["dt"]
     drift = (e1[3] - e0[3]) / n / EV_KJMOL / (t_ps * 1e-3)  # eV/atom/ns
-    kinds = ["force", "density", "predrift", "bp_desc", "bp_mlp", "bp_force", "bp_gather", "rebuild"]
-    per_kernel = {kinds[k]: round(ms[k] / nl[k] * 1e3, 2) for k in range(8) if nl[k] > 0}
+    per_kernel = {KINDS[k]: round(ms[k] / nl[k] * 1e3, 2) for k in range(8) if nl[k] > 0}
     return {
         "n": n, "steps": steps, "seconds": t1 - t0, "value": n * steps / (t1 - t0), "kernel_us": per_kernel,
         "ms_per_step": 1e3 * (t1 - t0) / steps, "kernel_ms": ms.tolist(), "kernel_launches": nl.tolist(),
         "nbar": st["mean"], "rebuilds": reb, "drift_eV_atom_ns": drift, "lanes": lanes,
         "autotune_us": tune, "b2b_us": {"density": round(b2b[0], 2), "force": round(b2b[1], 2)},
+        "nbar_b2b": nbar_b2b,
     }
 
 
@@ -260,7 +349,7 @@ def main():
     # mean duration of the step-mode force kernel: 200 back-to-back launches
     # between two HIP events on the engine's stream
     avg_ms = head["b2b_us"]["force"] * 1e-3
-    per_launch = algorithmic_bytes("eam_mixed", nbar) * head["n"]
+    per_launch = algorithmic_bytes("eam_mixed", head["nbar_b2b"]) * head["n"]
     achieved = per_launch / (avg_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic("eam_force_mixed"),
@@ -278,6 +367,24 @@ def main():
         wl = W.lj_argon()
         r = run_gpu(E, ctx, wl, E.FP64, 1000, args.warmup, args.lanes)
         extra["lj_argon_fp64"] = {kk: r[kk] for kk in keys}
+        # 512 NiAl cells (solver::Multi over three EAM objects), one batch
+        nb = W.nial_batch(512)
+        cells = nb["cells"]
+        for name, prec in (("nial_batch512_mixed", E.MIXED), ("nial_batch512_fp64", E.FP64)):
+            extra[name] = run_batch_eval(E, ctx, cells, nb["pot"], prec, 50)
+        seq = one_at_a_time(E, ctx, cells[:64], nb["pot"], 5)
+        seq["note"] = "first 64 of the 512 cells, each in its own System, FP64, evaluated in turn"
+        extra["nial_one_at_a_time_fp64"] = seq
+        # BPNN Cu 13500 atoms
+        wb = W.bpnn_cu()
+        for name, prec in (("bpnn_cu_mixed", E.MIXED), ("bpnn_cu_fp64", E.FP64)):
+            r = run_batch(E, ctx, [(wb["atoms"], wb["box"])], wb["pot"], wb["dt"], prec, 200, 10)
+            mlp_us = r["kernel_us"].get("bp_mlp")
+            if mlp_us and prec == E.MIXED:
+                tf = MLP_FLOP_PER_ATOM * r["atoms"] / (mlp_us * 1e-6) / 1e12
+                r["mlp_mfma"] = {"algorithmic_TFLOPs": round(tf, 2), "peak_fp16_dense_TFLOPs": MFMA_FP16_PEAK_TFLOPS,
+                                 "frac": round(tf / MFMA_FP16_PEAK_TFLOPS, 5), "flop_per_atom": MLP_FLOP_PER_ATOM}
+            extra[name] = r
     cpu = None if args.no_cpu else cpu_baseline(w, max_steps=500)
     line = {
         "metric": METRIC, "value": round(head["value"], 1), "unit": "atom-steps/s", "n_gpus": 1,

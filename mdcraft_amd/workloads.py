@@ -117,13 +117,15 @@ def taper(r, rc=EAM_RC, h=0.5):
     return np.where(r < rc, s, 0.0)
 
 
-# Morse pair term and sqrt embedding fitted so that fcc Cu with the 5.0 A
-# tapered cutoff has its energy minimum at a = 3.615 A (E = -3.04 eV/atom,
-# B = 197 GPa); the crystal then stays crystalline at 300 K.
-EAM_F_A = 0.6  # eV
+# The EAM Cu workload's functions: Morse D0 = 0.2 eV, alpha = 1.6 / A,
+# r0 = 2.55 A; rho(r) = exp(-3 (r / 2.556 - 1)); F(rho) = -1.2 sqrt(rho) eV.
+# With the tapered 5.0 A cutoff their fcc energy minimum lies below
+# a = 3.0 A, so the a = 3.615 A crystal is under tension and cavitates after
+# a few hundred femtoseconds at 300 K.
+EAM_F_A = 1.2  # eV
 
 
-def morse(r, D0=0.1353, alpha=1.6473, r0=2.906):
+def morse(r, D0=0.2, alpha=1.6, r0=2.55):
     e = np.exp(-alpha * (r - r0))
     return D0 * (e * e - 2.0 * e)
 
