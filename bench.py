@@ -153,6 +153,43 @@ def one_at_a_time(E, ctx, cells, pot, reps):
     return {"value": n * reps / (t1 - t0), "unit": "atom-evaluations/s", "systems": len(cells), "atoms": n}
 
 
+def served_stream(E, ctx, W, nreq=120, factors=(0.5, 1.0, 2.0)):
+    """Served mixed stream: a saturation run (every request at t = 0) gives
+    the capacity C; then Poisson arrivals at f x C for each factor, served
+    (LJ FP64, EAM and BPNN mixed) and by the plain FP64 one-at-a-time path
+    on the same traces."""
+    from mdcraft_amd import serve
+    model = W.write_maise_model(os.path.join(os.environ.get("TMPDIR", "/tmp"), "mdc_served_cu.maise"))
+    specs = W.served_requests(nreq, model_path=model)
+    pots = {}
+    for sp in specs:
+        if sp["kind"] not in pots:
+            pots[sp["kind"]] = make_pot(E, ctx, sp["pot"])
+    prec = {"lj": E.FP64, "eam": E.MIXED, "bpnn": E.MIXED}
+
+    def mk():
+        return [serve.Request(q, sp["kind"], sp["atoms"], sp["box"], sp["dt"], sp["steps"], 0.0)
+                for q, sp in enumerate(specs)]
+
+    # warm both paths (kernels, allocator) on a few requests
+    serve.Server(E, ctx, pots, prec).run_trace(serve.poisson_trace(mk()[:6], None))
+    serve.one_at_a_time(E, ctx, pots, serve.poisson_trace(mk()[:6], None))
+    sat = serve.latency_stats(serve.Server(E, ctx, pots, prec).run_trace(serve.poisson_trace(mk(), None)))
+    base_sat = serve.latency_stats(serve.one_at_a_time(E, ctx, pots, serve.poisson_trace(mk(), None)))
+    cap = sat["structures_per_s"]
+    out = {"requests": nreq, "mean_atoms": float(np.mean([len(sp["atoms"]) for sp in specs])),
+           "single_point_fraction": float(np.mean([sp["steps"] == 0 for sp in specs])),
+           "hbm_budget_GB": 200, "saturation": sat, "one_at_a_time_fp64_saturation": base_sat, "rates": []}
+    for f in factors:
+        rate = f * cap
+        srv = serve.Server(E, ctx, pots, prec)
+        st = serve.latency_stats(srv.run_trace(serve.poisson_trace(mk(), rate, seed=int(100 * f))))
+        st["peak_admitted_GB"] = srv.peak_bytes / 1e9
+        bt = serve.latency_stats(serve.one_at_a_time(E, ctx, pots, serve.poisson_trace(mk(), rate, seed=int(100 * f))))
+        out["rates"].append({"factor": f, "rate_per_s": rate, "served": st, "one_at_a_time_fp64": bt})
+    return out
+
+
 def run_gpu(E, ctx, w, precision, steps, warmup, lanes=None):
     pot = w["pot"]
     at = w["atoms"]
@@ -385,6 +422,7 @@ def main():
                 r["mlp_mfma"] = {"algorithmic_TFLOPs": round(tf, 2), "peak_fp16_dense_TFLOPs": MFMA_FP16_PEAK_TFLOPS,
                                  "frac": round(tf / MFMA_FP16_PEAK_TFLOPS, 5), "flop_per_atom": MLP_FLOP_PER_ATOM}
             extra[name] = r
+        extra["served_stream"] = served_stream(E, ctx, W)
     cpu = None if args.no_cpu else cpu_baseline(w, max_steps=500)
     line = {
         "metric": METRIC, "value": round(head["value"], 1), "unit": "atom-steps/s", "n_gpus": 1,

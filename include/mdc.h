@@ -96,6 +96,14 @@ mdc_status mdc_sys_create(mdc_ctx* ctx, int nsys, const uint64_t* natoms, const 
 mdc_status mdc_sys_destroy(mdc_sys* sys);
 mdc_status mdc_sys_upload(mdc_sys* sys, const void* atoms, uint32_t fields);
 mdc_status mdc_sys_download(mdc_sys* sys, void* atoms, uint32_t fields);
+/* Records [first, first + count) of the caller's order into atoms[0, count). */
+mdc_status mdc_sys_download_range(mdc_sys* sys, uint64_t first, uint64_t count, void* atoms, uint32_t fields);
+/* Switch box `box` off (active = 0) or back on: its atoms are then skipped
+ * by every stage exactly as atoms with tag == 0 are (single.cxx:44,
+ * verlet.cxx:73), so a finished box of a batch costs no work while the
+ * others step.  No reference counterpart: batches of boxes are this
+ * engine's. */
+mdc_status mdc_sys_set_active(mdc_sys* sys, int box, int active);
 mdc_status mdc_sys_natoms(const mdc_sys* sys, uint64_t* n);
 /* lanes per atom (1, 2, 4, 8, 16) of the force kernels; changes the list layout */
 mdc_status mdc_sys_set_lanes(mdc_sys* sys, int lanes_per_atom);

@@ -315,16 +315,29 @@ __global__ void k_scatter_in(const double* __restrict__ stage, const double4* __
         for (int c = 0; c < 3; ++c) fl[(size_t)c * n + p] = (float)stage[(size_t)(D_FX + c) * n + o];
 }
 
+// records [first, first + count) of the caller's order into a staging area
 __global__ void k_gather_out(double* __restrict__ stage, double4* __restrict__ spos, const int* __restrict__ where,
                              const double4* __restrict__ pos, const double* __restrict__ d, const float* __restrict__ fl,
-                             int f_mixed, int n) {
-    int o = blockIdx.x * blockDim.x + threadIdx.x;
-    if (o >= n) return;
-    int p = where[o];
-    spos[o] = pos[p];
-    for (int f = 0; f < D_NFIELDS; ++f) stage[(size_t)f * n + o] = d[(size_t)f * n + p];
+                             int f_mixed, int n, int first, int count) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= count) return;
+    int p = where[first + q];
+    spos[q] = pos[p];
+    for (int f = 0; f < D_NFIELDS; ++f) stage[(size_t)f * count + q] = d[(size_t)f * n + p];
     if (f_mixed)
-        for (int c = 0; c < 3; ++c) stage[(size_t)(D_FX + c) * n + o] = (double)fl[(size_t)c * n + p];
+        for (int c = 0; c < 3; ++c) stage[(size_t)(D_FX + c) * count + q] = (double)fl[(size_t)c * n + p];
+}
+
+// set or clear the skip flag of the atoms [first, first + count) of the
+// caller's order in the current positions
+__global__ void k_set_inactive(double4* __restrict__ pos, const int* __restrict__ where, int first, int count,
+                               int inactive) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= count) return;
+    const int p = where[first + q];
+    double4 v = pos[p];
+    v.w = pack_w(w_kind(v.w), inactive, w_sys(v.w));
+    pos[p] = v;
 }
 
 void free_batch(Batch& b) {
@@ -510,17 +523,18 @@ void upload(Batch& b, const AtomRec* a, uint32_t fields) {
     }
 }
 
-void download(Batch& b, AtomRec* a, uint32_t fields) {
+// records [first, first + count) of the caller's order into a[0, count)
+void download(Batch& b, AtomRec* a, uint32_t fields, int first, int count) {
     const int n = b.n;
-    if (n == 0) return;
+    if (count <= 0) return;
     hipStream_t st = b.ctx->stream;
-    double* d_stage = dalloc<double>((size_t)D_NFIELDS * n);
-    double4* d_spos = dalloc<double4>(n);
-    std::vector<double> stage((size_t)D_NFIELDS * n);
-    std::vector<double4> spos(n);
+    double* d_stage = dalloc<double>((size_t)D_NFIELDS * count);
+    double4* d_spos = dalloc<double4>(count);
+    std::vector<double> stage((size_t)D_NFIELDS * count);
+    std::vector<double4> spos(count);
     try {
-        hipLaunchKernelGGL(k_gather_out, dim3((n + 255) / 256), dim3(256), 0, st, d_stage, d_spos, b.where,
-                           b.pos[b.cur], b.dst[b.sc], b.fst[b.sc], b.f_prec == P_MIXED ? 1 : 0, n);
+        hipLaunchKernelGGL(k_gather_out, dim3((count + 255) / 256), dim3(256), 0, st, d_stage, d_spos, b.where,
+                           b.pos[b.cur], b.dst[b.sc], b.fst[b.sc], b.f_prec == P_MIXED ? 1 : 0, n, first, count);
         HIPCHECK(hipMemcpyAsync(stage.data(), d_stage, stage.size() * sizeof(double), hipMemcpyDeviceToHost, st));
         HIPCHECK(hipMemcpyAsync(spos.data(), d_spos, spos.size() * sizeof(double4), hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
@@ -531,15 +545,16 @@ void download(Batch& b, AtomRec* a, uint32_t fields) {
     }
     dfree(d_stage);
     dfree(d_spos);
-    auto get = [&](int f, int o) { return stage[(size_t)f * n + o]; };
+    auto get = [&](int f, int o) { return stage[(size_t)f * count + (o - first)]; };
     for (int s = 0; s < b.nsys; ++s) {
         const SysDesc& d = b.desc[s];
-        for (int o = b.atom_off[s]; o < b.atom_off[s + 1]; ++o) {
-            AtomRec& r = a[o];
+        const int o0 = std::max(first, b.atom_off[s]), o1 = std::min(first + count, b.atom_off[s + 1]);
+        for (int o = o0; o < o1; ++o) {
+            AtomRec& r = a[o - first];
             if (fields & MDC_F_R) {
-                r.r[0] = spos[o].x;
-                r.r[1] = spos[o].y;
-                r.r[2] = spos[o].z;
+                r.r[0] = spos[o - first].x;
+                r.r[1] = spos[o - first].y;
+                r.r[2] = spos[o - first].z;
             }
             if (fields & MDC_F_V)
                 for (int c = 0; c < 3; ++c) r.v[c] = get(D_VX + c, o);
@@ -564,7 +579,7 @@ void download(Batch& b, AtomRec* a, uint32_t fields) {
                 // boundary::Periodic::need_apply (periodic.cxx:9-21)
                 bool need = false;
                 if (b.periodic_bc[s] && b.tag[o] != 0) {
-                    const double rr[3] = {spos[o].x, spos[o].y, spos[o].z};
+                    const double rr[3] = {spos[o - first].x, spos[o - first].y, spos[o - first].z};
                     for (int ax = 0; ax < 3; ++ax)
                         if (d.per[ax]) {
                             double dist = std::min(std::fabs(rr[ax] - d.lo[ax]), std::fabs(rr[ax] - d.hi[ax]));
@@ -794,7 +809,36 @@ mdc_status mdc_sys_download(mdc_sys* sys, void* atoms, uint32_t fields) {
     return guard(sys ? sys->b.ctx : nullptr, [&] {
         if (!sys || !atoms) throw Error(E_INVAL, "null argument");
         HIPCHECK(hipSetDevice(sys->b.ctx->device));
-        download(sys->b, static_cast<AtomRec*>(atoms), fields);
+        download(sys->b, static_cast<AtomRec*>(atoms), fields, 0, sys->b.n);
+    });
+}
+
+mdc_status mdc_sys_download_range(mdc_sys* sys, uint64_t first, uint64_t count, void* atoms, uint32_t fields) {
+    return guard(sys ? sys->b.ctx : nullptr, [&] {
+        if (!sys || (!atoms && count)) throw Error(E_INVAL, "null argument");
+        if (first > (uint64_t)sys->b.n || count > (uint64_t)sys->b.n - first)
+            throw Error(E_INVAL, "record range outside the system");
+        HIPCHECK(hipSetDevice(sys->b.ctx->device));
+        download(sys->b, static_cast<AtomRec*>(atoms), fields, (int)first, (int)count);
+    });
+}
+
+mdc_status mdc_sys_set_active(mdc_sys* sys, int box, int active) {
+    return guard(sys ? sys->b.ctx : nullptr, [&] {
+        if (!sys) throw Error(E_INVAL, "null argument");
+        Batch& b = sys->b;
+        if (box < 0 || box >= b.nsys) throw Error(E_INVAL, "box index out of range");
+        HIPCHECK(hipSetDevice(b.ctx->device));
+        const int first = b.atom_off[box], count = b.atom_off[box + 1] - first;
+        if (count == 0) return;
+        // the skip flag travels with the positions; atoms whose own tag is 0
+        // stay skipped
+        bool any_tag0 = false;
+        for (int o = first; o < first + count; ++o) any_tag0 = any_tag0 || b.tag[o] == 0;
+        if (active && any_tag0) throw Error(E_INVAL, "box holds atoms with tag 0; re-upload it to reactivate");
+        hipLaunchKernelGGL(k_set_inactive, dim3((count + 255) / 256), dim3(256), 0, b.ctx->stream, b.pos[b.cur],
+                           b.where, first, count, active ? 0 : 1);
+        HIPCHECK(hipGetLastError());
     });
 }
 

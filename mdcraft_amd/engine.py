@@ -30,7 +30,7 @@ EXPORTS = [
     "mdc_ctx_create", "mdc_ctx_destroy", "mdc_ctx_last_error", "mdc_last_error", "mdc_atom_record_size",
     "mdc_pot_lj_create", "mdc_pot_eam_create", "mdc_pot_eam_create_file", "mdc_pot_multi_create",
     "mdc_pot_bpnn_create", "mdc_pot_rcut", "mdc_pot_destroy",
-    "mdc_sys_create", "mdc_sys_destroy", "mdc_sys_upload", "mdc_sys_download", "mdc_sys_natoms",
+    "mdc_sys_create", "mdc_sys_destroy", "mdc_sys_upload", "mdc_sys_download", "mdc_sys_download_range", "mdc_sys_set_active", "mdc_sys_natoms",
     "mdc_sys_set_lanes", "mdc_sys_autotune", "mdc_sys_time_kernels", "mdc_nlist_build", "mdc_nlist_decide", "mdc_nlist_export", "mdc_nlist_stats",
     "mdc_compute", "mdc_step", "mdc_thermo", "mdc_sync", "mdc_timing", "mdc_bpnn_inspect",
 ]
@@ -62,6 +62,8 @@ def load_library(path=LIB_PATH):
         "mdc_sys_destroy": (I, [P]),
         "mdc_sys_upload": (I, [P, P, U32]),
         "mdc_sys_download": (I, [P, P, U32]),
+        "mdc_sys_download_range": (I, [P, U64, U64, P, U32]),
+        "mdc_sys_set_active": (I, [P, I, I]),
         "mdc_sys_natoms": (I, [P, C.POINTER(U64)]),
         "mdc_sys_set_lanes": (I, [P, I]),
         "mdc_sys_autotune": (I, [P, P, I, I, C.POINTER(I), DP]),
@@ -244,6 +246,19 @@ class System:
         assert atoms.dtype == ATOM_DTYPE and atoms.flags.c_contiguous and len(atoms) == self.n
         _check(lib().mdc_sys_download(self.h, atoms.ctypes.data, fields))
         return atoms
+
+    def download_range(self, first, count, fields=F_ALL):
+        """Records [first, first + count) of the upload order."""
+        atoms = np.zeros(count, dtype=ATOM_DTYPE)
+        _check(lib().mdc_sys_download_range(self.h, first, count, atoms.ctypes.data, fields))
+        return atoms
+
+    def download_box(self, box, fields=F_ALL):
+        off = np.concatenate([[0], np.cumsum(self.natoms)]).astype(np.int64)
+        return self.download_range(int(off[box]), int(off[box + 1] - off[box]), fields)
+
+    def set_active(self, box, active):
+        _check(lib().mdc_sys_set_active(self.h, box, int(bool(active))))
 
     def build(self):
         _check(lib().mdc_nlist_build(self.h))

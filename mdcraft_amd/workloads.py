@@ -283,23 +283,32 @@ def bpnn_cu(reps=15, seed_index=0, T=300.0, rattle=0.05, model_dir=None):
     return {"atoms": at, "box": _box([L] * 3), "pot": pot, "dt": 1 * FS, "steps": 200, "name": "bpnn_cu"}
 
 
+def cubic_reps(cells, minimum=1):
+    """Replicas nx <= ny <= nz (each >= minimum): the most cubic product
+    within 10 % of cells, else the closest product."""
+    cells = max(cells, minimum ** 3)
+    best = None
+    for nx in range(minimum, cells + 1):
+        if nx ** 3 > 1.1 * cells:
+            break
+        for ny in range(nx, cells + 1):
+            if nx * ny * ny > 1.1 * cells:
+                break
+            for nz in {max(ny, int(cells // (nx * ny))), max(ny, int(cells // (nx * ny)) + 1)}:
+                c = nx * ny * nz
+                err = abs(c - cells) / cells
+                score = (err > 0.1, nz / nx if err <= 0.1 else err, err)
+                if best is None or score < best[0]:
+                    best = (score, (nx, ny, nz))
+    return best[1]
+
+
 def nial_cell(index, target=None):
     """One strained, rattled B2 NiAl cell; kind 0 = Ni, 1 = Al."""
     rng = Rng(structure_seed(4, index))
     if target is None:
         target = math.exp(math.log(32) + rng.uniform(1)[0] * (math.log(4096) - math.log(32)))
-    cells = max(1, int(round(target / 2)))
-    best = None
-    for nx in range(1, cells + 1):
-        for ny in range(nx, cells + 1):
-            if nx * ny > cells:
-                break
-            nz = max(ny, int(round(cells / (nx * ny))))
-            c = nx * ny * nz
-            score = (abs(c - cells), nz - nx)
-            if best is None or score < best[0]:
-                best = (score, (nx, ny, nz))
-    reps = best[1]
+    reps = cubic_reps(max(1, int(round(target / 2))))
     a = 2.88 * ANG
     pos, site = lattice(B2, reps, a)
     n = len(pos)
@@ -334,3 +343,70 @@ def nial_pots():
 def nial_batch(count=512):
     cells = [nial_cell(i) for i in range(count)]
     return {"cells": cells, "pot": nial_pots(), "dt": 1 * FS, "name": "nial_batch"}
+
+
+# --------------------------------------------------------------------------
+# served mixed stream
+# --------------------------------------------------------------------------
+SERVED_KINDS = ("lj", "eam", "bpnn")
+
+
+def served_structure(kind, natoms, seed, vacuum=False, model_path=None):
+    """An fcc structure of about natoms atoms built as the LJ argon, EAM Cu
+    or BPNN Cu workloads build theirs; vacuum adds a 50 A gap along z.
+    BPNN boxes keep at least 4 cells per axis (the descriptor path needs
+    box lengths of at least twice its 6.3 A search radius)."""
+    rng = Rng(seed)
+    spec = {"lj": (5.26, 39.948, 100.0, 0.0, 1), "eam": (3.615, 63.546, 300.0, 0.03, 2),
+            "bpnn": (3.615, 63.546, 300.0, 0.05, 3)}[kind]
+    a_ang, mass, T, rattle, wid = spec
+    a = a_ang * ANG
+    reps = cubic_reps(max(1, int(round(natoms / 4))), minimum=4 if kind == "bpnn" else 1)
+    pos, _ = lattice(FCC, reps, a)
+    n = len(pos)
+    L = np.array(reps, dtype=np.float64) * a
+    if rattle:
+        pos = np.mod(pos + rng.normal(3 * n).reshape(n, 3) * rattle * ANG, L)
+    if vacuum:
+        L[2] += 50.0 * ANG
+    at = make_atoms(n)
+    m = np.full(n, mass)
+    set_vec(at, "r", pos)
+    set_vec(at, "v", maxwell_boltzmann(rng, m, T))
+    at["m"] = m
+    at["uid"] = np.arange(1, n + 1)
+    if kind == "lj":
+        rc = 2.5 * 3.405 * ANG
+        pot = {"kind": "lj", "a": 0.0104 * EV, "s": 3.405 * ANG, "rc": rc}
+        dt = 2 * FS
+    elif kind == "eam":
+        tab = eam_cu_tables()
+        rc = tab["rc"]
+        pot = dict(kind="eam", **tab)
+        dt = 1 * FS
+    else:
+        rc = 0.6
+        pot = {"kind": "bpnn", "path": model_path, "rc": rc}
+        dt = 1 * FS
+    at["rcut"] = rc
+    at["rns"] = rc + 0.3 * ANG
+    return at, _box(L), pot, dt
+
+
+def served_requests(count, seed_index=0, model_path=None):
+    """count requests of the served mixed stream: potential uniform over LJ,
+    EAM Cu and BPNN Cu; atoms log-uniform over 64-20000; a 50 A vacuum slab
+    with probability 0.3; half single-point E/F/virial, half NVE runs of
+    10-200 steps (uniform).  Each as (kind, atoms, box, dt, steps)."""
+    rng = Rng(structure_seed(5, seed_index))
+    out = []
+    for q in range(count):
+        u = rng.uniform(5)
+        kind = SERVED_KINDS[min(2, int(u[0] * 3))]
+        natoms = int(round(math.exp(math.log(64) + u[1] * (math.log(20000) - math.log(64)))))
+        vacuum = u[2] < 0.3
+        steps = 0 if u[3] < 0.5 else 10 + min(190, int(u[4] * 191))
+        at, box, pot, dt = served_structure(kind, natoms, structure_seed(5, 100000 + seed_index * 10000 + q), vacuum,
+                                            model_path)
+        out.append({"kind": kind, "atoms": at, "box": box, "dt": dt, "steps": steps, "pot": pot})
+    return out
