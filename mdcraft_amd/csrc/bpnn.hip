@@ -40,15 +40,23 @@
 namespace mdc {
 
 constexpr int kBpMaxR = 16, kBpMaxA = 16, kBpMaxEta = 4, kBpMaxHidden = 64, kBpMaxIn = 32;
+constexpr int kBpMaxG = 8;  // distinct (lambda, zeta) pairs of the G4 terms
 
 struct BpDesc {
-    int nr, na, nin, h1, h2, ne;
+    int nr, na, nin, h1, h2, ne, ng;
     double Rc, Rc2, Rs, pi_rc, escale;
     double eta_r[kBpMaxR];
     double eta_e[kBpMaxEta];
     int a_e[kBpMaxA];
     double a_coef[kBpMaxA], a_lam[kBpMaxA], a_zeta[kBpMaxA];
     int a_iz[kBpMaxA];
+    // G4 terms on a (group, eta) grid: group g has lambda g_lam, zeta g_zeta
+    // (g_iz = integer zeta or -1); grid term t = g * kBpMaxEta + e is angular
+    // input t_q[t] (-1: none) with coefficient t_coef[t]
+    double g_lam[kBpMaxG], g_zeta[kBpMaxG], g_zl[kBpMaxG];
+    int g_iz[kBpMaxG];
+    int t_q[kBpMaxG * kBpMaxEta];
+    double t_coef[kBpMaxG * kBpMaxEta];
 };
 
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
@@ -89,6 +97,19 @@ __device__ inline float dexp_(float x) { return __expf(x); }
 __device__ inline double dexp_(double x) { return exp(x); }
 __device__ inline float dsqrt_(float x) { return sqrtf(x); }
 __device__ inline double dsqrt_(double x) { return sqrt(x); }
+// 1/x and 1/sqrt(x): the hardware approximations (1 ulp) in FP32, exact in FP64
+__device__ inline float drcp_(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline double drcp_(double x) { return 1.0 / x; }
+__device__ inline float drsq_(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ inline double drsq_(double x) { return 1.0 / sqrt(x); }
+
+// b^k for a whole k >= 0 (uniform across the wavefront)
+template <typename R>
+__device__ inline R powu_(R b, int k) {
+    R p = (R)1;
+    for (int q = 0; q < k; ++q) p *= b;
+    return p;
+}
 
 template <typename R>
 __device__ inline R powi_(R b, int iz, R z) {
@@ -108,7 +129,8 @@ __device__ inline R wave_sum(R v) {
 
 template <typename R>
 struct Stage {
-    R *x, *y, *z, *d, *fc, *dfc, *E;  // E: [ne][cap]
+    R *x, *y, *z, *d, *id, *fc, *dfc, *E;  // E: [ne][cap]; id = 1/d
+    R* g;                                  // force-share accumulators [3][cap] (pass 3)
     int* slot;
     int cap;
 };
@@ -123,15 +145,36 @@ __device__ inline Stage<R> carve(char* base, int cap, int ne) {
     S.d = p + 3 * cap;
     S.fc = p + 4 * cap;
     S.dfc = p + 5 * cap;
-    S.E = p + 6 * cap;
-    S.slot = reinterpret_cast<int*>(p + (6 + ne) * cap);
+    S.id = p + 6 * cap;
+    S.E = p + 7 * cap;
+    S.g = p + (7 + ne) * cap;
+    S.slot = reinterpret_cast<int*>(p + (10 + ne) * cap);
     S.cap = cap;
     return S;
 }
 
 __host__ __device__ inline size_t stage_bytes(int cap, int ne, int rsize) {
-    return (size_t)cap * ((6 + ne) * rsize + 4);
+    return (size_t)cap * ((10 + ne) * rsize + 4);
 }
+
+// First unordered pair (a, b = a + 1 + t) of the flattened pair index t
+// over nc staged neighbours; advance() moves to index t + kWave.
+struct PairWalk {
+    int a, t, nc;
+    __device__ PairWalk(int nc_, int t0) : a(0), t(t0), nc(nc_) { settle(); }
+    __device__ void settle() {
+        while (a < nc - 1 && t >= nc - 1 - a) {
+            t -= nc - 1 - a;
+            ++a;
+        }
+    }
+    __device__ bool valid() const { return a < nc - 1; }
+    __device__ int b() const { return a + 1 + t; }
+    __device__ void advance() {
+        t += kWave;
+        settle();
+    }
+};
 
 // Stage the in-range neighbours of centre i (1e-15 <= r^2 < Rc^2 under
 // Domain::shortest) into LDS, compacted in list order; out-of-range slots
@@ -168,9 +211,13 @@ __device__ int bp_stage(int i, const double4& pi, const SysDesc& s, const int* _
             S.y[at] = (R)dy;
             S.z[at] = (R)dz;
             S.d[at] = (R)d;
-            S.fc[at] = (R)(0.5 * (cos(P.pi_rc * d) + 1.0));
-            S.dfc[at] = (R)(-0.5 * sin(P.pi_rc * d) * P.pi_rc);
-            for (int e = 0; e < P.ne; ++e) S.E[e * S.cap + at] = (R)exp(-P.eta_e[e] * d2);
+            S.id[at] = (R)(1.0 / d);
+            // cutoff function and its derivative, exp(-eta d^2), in the
+            // precision of the pass
+            const R rd = (R)d, prc = (R)P.pi_rc;
+            S.fc[at] = (R)0.5 * (dcos_(prc * rd) + (R)1);
+            S.dfc[at] = (R)-0.5 * dsin_(prc * rd) * prc;
+            for (int e = 0; e < P.ne; ++e) S.E[e * S.cap + at] = dexp_(-(R)P.eta_e[e] * (R)d2);
             S.slot[at] = k;
         }
         base += __popcll(m);
@@ -181,8 +228,10 @@ __device__ int bp_stage(int i, const double4& pi, const SysDesc& s, const int* _
     return base;
 }
 
-// pass 1: descriptors
-template <typename R>
+// pass 1: descriptors.  Per pair of staged neighbours the G4 factor
+// (1 + lambda cos)^zeta is formed once per (lambda, zeta) group and shared by
+// the group's eta terms; NR, NG, NE bound the radial, group and eta counts.
+template <typename R, int NR, int NG, int NE>
 __global__ void __launch_bounds__(256) k_bp_desc(const double4* __restrict__ pos, const int* __restrict__ nl,
                                                  const int* __restrict__ nn, int tpa, int M, int cap,
                                                  const SysDesc* __restrict__ desc, const BpDesc* __restrict__ Pd,
@@ -199,56 +248,67 @@ __global__ void __launch_bounds__(256) k_bp_desc(const double4* __restrict__ pos
     const SysDesc& s = desc[w_sys(pi.w)];
     const int cnt = min(nn[i], cap);
     const int nc = bp_stage<R>(i, pi, s, nl, cnt, tpa, M, pos, P, S, (R*)nullptr, cap);
-    R acc[kBpMaxR + kBpMaxA];
+    R acc[NR + NG * NE];
 #pragma unroll
-    for (int q = 0; q < kBpMaxR + kBpMaxA; ++q) acc[q] = (R)0;
+    for (int q = 0; q < NR + NG * NE; ++q) acc[q] = (R)0;
+    // descriptor parameters in registers, in the precision of the pass
+    const int nr = P.nr, ng = P.ng, ne = P.ne;
+    R eta_r[NR], g_lam[NG], eta_e[NE];
+    int g_iz[NG];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) eta_r[r] = r < nr ? (R)P.eta_r[r] : (R)0;
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+        g_lam[g] = g < ng ? (R)P.g_lam[g] : (R)0;
+        g_iz[g] = g < ng ? P.g_iz[g] : 0;
+    }
+#pragma unroll
+    for (int e = 0; e < NE; ++e) eta_e[e] = e < ne ? (R)P.eta_e[e] : (R)0;
     const R Rs = (R)P.Rs;
     for (int a = lane; a < nc; a += kWave) {
         const R d = S.d[a], fc = S.fc[a];
 #pragma unroll
-        for (int r = 0; r < kBpMaxR; ++r)
-            if (r < P.nr) acc[r] += dexp_(-(R)P.eta_r[r] * (d - Rs) * (d - Rs)) * fc;
+        for (int r = 0; r < NR; ++r)
+            if (r < nr) acc[r] += dexp_(-eta_r[r] * (d - Rs) * (d - Rs)) * fc;
     }
     // pairs a < b over the staged list, flattened and strided over lanes
     const R Rc2 = (R)P.Rc2, pirc = (R)P.pi_rc;
-    int a = 0, t = lane;
-    while (a < nc - 1 && t >= nc - 1 - a) {
-        t -= nc - 1 - a;
-        ++a;
-    }
-    while (a < nc - 1) {
-        const int b = a + 1 + t;
+    for (PairWalk pw(nc, lane); pw.valid(); pw.advance()) {
+        const int a = pw.a, b = pw.b();
         const R cx = S.x[b] - S.x[a], cy = S.y[b] - S.y[a], cz = S.z[b] - S.z[a];
         const R d2c = cx * cx + cy * cy + cz * cz;
-        if (d2c >= (R)1e-15 && d2c < Rc2) {
-            const R dc = dsqrt_(d2c);
-            const R fcc = (R)0.5 * (dcos_(pirc * dc) + (R)1);
-            const R cs = (S.x[a] * S.x[b] + S.y[a] * S.y[b] + S.z[a] * S.z[b]) / (S.d[a] * S.d[b]);
-            const R F = S.fc[a] * S.fc[b] * fcc;
-            R Ex[kBpMaxEta];
+        if (!(d2c >= (R)1e-15 && d2c < Rc2)) continue;
+        const R dc = d2c * drsq_(d2c);
+        const R fcc = (R)0.5 * (dcos_(pirc * dc) + (R)1);
+        const R cs = (S.x[a] * S.x[b] + S.y[a] * S.y[b] + S.z[a] * S.z[b]) * S.id[a] * S.id[b];
+        const R F = S.fc[a] * S.fc[b] * fcc;
+        R Ex[NE];
 #pragma unroll
-            for (int e = 0; e < kBpMaxEta; ++e)
-                if (e < P.ne) Ex[e] = S.E[e * cap + a] * S.E[e * cap + b] * dexp_(-(R)P.eta_e[e] * d2c);
+        for (int e = 0; e < NE; ++e)
+            Ex[e] = e < ne ? S.E[e * cap + a] * S.E[e * cap + b] * dexp_(-eta_e[e] * d2c) * F : (R)0;
 #pragma unroll
-            for (int q = 0; q < kBpMaxA; ++q)
-                if (q < P.na) {
-                    const R base = (R)1 + (R)P.a_lam[q] * cs;
-                    acc[kBpMaxR + q] += (R)P.a_coef[q] * powi_(base, P.a_iz[q], (R)P.a_zeta[q]) * Ex[P.a_e[q]] * F;
-                }
-        }
-        t += kWave;
-        while (a < nc - 1 && t >= nc - 1 - a) {
-            t -= nc - 1 - a;
-            ++a;
-        }
+        for (int g = 0; g < NG; ++g)
+            if (g < ng) {
+                const R pz = powu_((R)1 + g_lam[g] * cs, g_iz[g]);
+#pragma unroll
+                for (int e = 0; e < NE; ++e) acc[NR + g * NE + e] += pz * Ex[e];
+            }
     }
 #pragma unroll
-    for (int q = 0; q < kBpMaxR + kBpMaxA; ++q) acc[q] = wave_sum(acc[q]);
+    for (int q = 0; q < NR + NG * NE; ++q) acc[q] = wave_sum(acc[q]);
     if (lane == 0) {
         R* g = G + (size_t)i * ldg;
-        for (int r = 0; r < P.nr; ++r) g[r] = acc[r];
-        for (int q = 0; q < P.na; ++q) g[P.nr + q] = acc[kBpMaxR + q];
-        for (int q = P.nr + P.na; q < ldg; ++q) g[q] = (R)0;
+        for (int q = 0; q < ldg; ++q) g[q] = (R)0;
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+            if (r < P.nr) g[r] = acc[r];
+#pragma unroll
+        for (int gg = 0; gg < NG; ++gg)
+#pragma unroll
+            for (int e = 0; e < NE; ++e) {
+                const int t = gg * kBpMaxEta + e;
+                if (gg < P.ng && e < P.ne && P.t_q[t] >= 0) g[P.nr + P.t_q[t]] = (R)P.t_coef[t] * acc[NR + gg * NE + e];
+            }
     }
 }
 
@@ -408,7 +468,7 @@ __global__ void __launch_bounds__(64) k_bp_mlp_mfma(const float* __restrict__ G,
 }
 
 // pass 3: force shares of centre i on its neighbour slots and on itself
-template <typename R>
+template <typename R, int NR, int NG, int NE>
 __global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos, const int* __restrict__ nl,
                                                  const int* __restrict__ nn, int tpa, int M, int cap,
                                                  const SysDesc* __restrict__ desc, const BpDesc* __restrict__ Pd,
@@ -425,66 +485,200 @@ __global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos
     const SysDesc& s = desc[w_sys(pi.w)];
     const int cnt = min(nn[i], cap);
     const int nc = bp_stage<R>(i, pi, s, nl, cnt, tpa, M, pos, P, S, Pout, cap);
-    R w[kBpMaxR + kBpMaxA];
+    const int nr = P.nr, ng = P.ng, ne = P.ne;
+    R eta_r[NR], g_lam[NG], g_zl[NG], eta_e[NE];
+    int g_iz[NG];
 #pragma unroll
-    for (int q = 0; q < kBpMaxR; ++q) w[q] = q < P.nr ? wgt[(size_t)i * ldg + q] : (R)0;
+    for (int r = 0; r < NR; ++r) eta_r[r] = r < nr ? (R)P.eta_r[r] : (R)0;
 #pragma unroll
-    for (int q = 0; q < kBpMaxA; ++q) w[kBpMaxR + q] = q < P.na ? wgt[(size_t)i * ldg + P.nr + q] : (R)0;
+    for (int g = 0; g < NG; ++g) {
+        g_lam[g] = g < ng ? (R)P.g_lam[g] : (R)0;
+        g_zl[g] = g < ng ? (R)P.g_zl[g] : (R)0;
+        g_iz[g] = g < ng ? P.g_iz[g] : 1;
+    }
+#pragma unroll
+    for (int e = 0; e < NE; ++e) eta_e[e] = e < ne ? (R)P.eta_e[e] : (R)0;
+    R w[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) w[q] = q < nr ? wgt[(size_t)i * ldg + q] : (R)0;
+    // dE/dG of the G4 terms times their coefficients on the (group, eta) grid
+    R C[NG * NE], Ce[NG * NE];
+#pragma unroll
+    for (int g = 0; g < NG; ++g)
+#pragma unroll
+        for (int e = 0; e < NE; ++e) {
+            const int t = g * kBpMaxEta + e;
+            const int q = (g < ng && e < ne) ? P.t_q[t] : -1;
+            const R c = q >= 0 ? wgt[(size_t)i * ldg + nr + q] * (R)P.t_coef[t] : (R)0;
+            C[g * NE + e] = c;
+            Ce[g * NE + e] = c * eta_e[e];
+        }
     const R Rs = (R)P.Rs, Rc2 = (R)P.Rc2, pirc = (R)P.pi_rc;
-    R ox = 0, oy = 0, oz = 0;
+    // radial part of every slot, owned by one lane
     for (int a = lane; a < nc; a += kWave) {
         const R ax = S.x[a], ay = S.y[a], az = S.z[a], da = S.d[a], fa = S.fc[a], dfa = S.dfc[a];
-        const R ux = ax / da, uy = ay / da, uz = az / da;
-        // radial: sum_r w_r g_r'(d) along u_a
         R sr = 0;
 #pragma unroll
-        for (int r = 0; r < kBpMaxR; ++r)
-            if (r < P.nr) {
-                const R eta = (R)P.eta_r[r];
+        for (int r = 0; r < NR; ++r)
+            if (r < nr) {
+                const R eta = eta_r[r];
                 sr += w[r] * dexp_(-eta * (da - Rs) * (da - Rs)) * ((R)-2 * eta * (da - Rs) * fa + dfa);
             }
-        R gx = sr * ux, gy = sr * uy, gz = sr * uz;
-        R Ea[kBpMaxEta];
+        const R ia = S.id[a];
+        S.g[a] = sr * ax * ia;
+        S.g[cap + a] = sr * ay * ia;
+        S.g[2 * cap + a] = sr * az * ia;
+    }
+    __builtin_amdgcn_wave_barrier();
+    if constexpr (sizeof(R) == 4) {
+    // angular part, FP32: ordered pairs (a, b), a != b: with A_q = c_q pz_q,
+    // Z_q = c_q zeta lam pz1_q (c_q = w_q coef_q Ex_e(q)) summed over q,
+    // d/dr_a = (-Z F cs / da - 2 F da SAe + fb dfa fcc SA) u_a
+    //          + Z F / da u_b + (2 F dc SAe - fa fb dfcc SA) u_c, c = r_b - r_a.
+    // Lane l takes the l-th contiguous 64th of the flattened pairs, so it
+    // stays on one row a for long runs: the row's sum collects in registers
+    // and goes to the LDS accumulator only when the row changes.  Twice the
+    // pair arithmetic of the unordered walk, but almost no LDS atomics.
+    const int row = nc - 1;
+    const int T = nc * row;
+    const int u0 = (int)(((long long)lane * T) / kWave), u1 = (int)(((long long)(lane + 1) * T) / kWave);
+    const int steps = (T + kWave - 1) / kWave;
+    int a = row > 0 ? u0 / row : 0, r = row > 0 ? u0 - a * row : 0;
+    R ax = 0, ay = 0, az = 0, da = 1, fa = 0, dfa = 0, ia = 1;
+    R Ea[NE];
+    int cur = -1;
+    R hx = 0, hy = 0, hz = 0;
+    for (int k = 0; k < steps; ++k) {
+        const int u = u0 + k;
+        if (u < u1) {
+            if (a != cur) {
+                if (cur >= 0) {
+                    atomicAdd(&S.g[cur], hx);
+                    atomicAdd(&S.g[cap + cur], hy);
+                    atomicAdd(&S.g[2 * cap + cur], hz);
+                }
+                cur = a;
+                hx = hy = hz = 0;
+                ax = S.x[a];
+                ay = S.y[a];
+                az = S.z[a];
+                da = S.d[a];
+                ia = S.id[a];
+                fa = S.fc[a];
+                dfa = S.dfc[a];
 #pragma unroll
-        for (int e = 0; e < kBpMaxEta; ++e)
-            if (e < P.ne) Ea[e] = S.E[e * cap + a];
-        for (int b = 0; b < nc; ++b) {
-            if (b == a) continue;
-            const R bx = S.x[b], by = S.y[b], bz = S.z[b], db = S.d[b];
+                for (int e = 0; e < NE; ++e) Ea[e] = e < ne ? S.E[e * cap + a] : (R)0;
+            }
+            const int b = r + (r >= a ? 1 : 0);
+            const R bx = S.x[b], by = S.y[b], bz = S.z[b];
             const R cx = bx - ax, cy = by - ay, cz = bz - az;
             const R d2c = cx * cx + cy * cy + cz * cz;
-            if (!(d2c >= (R)1e-15 && d2c < Rc2)) continue;
-            const R dc = dsqrt_(d2c);
-            const R fcc = (R)0.5 * (dcos_(pirc * dc) + (R)1);
-            const R dfcc = (R)-0.5 * dsin_(pirc * dc) * pirc;
-            const R fb = S.fc[b];
-            const R cs = (ax * bx + ay * by + az * bz) / (da * db);
-            const R F = fa * fb * fcc;
-            R Ex[kBpMaxEta];
+            if (d2c >= (R)1e-15 && d2c < Rc2) {
+                const R ib = S.id[b], fb = S.fc[b];
+                const R ic = drsq_(d2c);
+                const R dc = d2c * ic;
+                const R fcc = (R)0.5 * (dcos_(pirc * dc) + (R)1);
+                const R dfcc = (R)-0.5 * dsin_(pirc * dc) * pirc;
+                const R cs = (ax * bx + ay * by + az * bz) * ia * ib;
+                const R F = fa * fb * fcc;
+                R Ex[NE];
 #pragma unroll
-            for (int e = 0; e < kBpMaxEta; ++e)
-                if (e < P.ne) Ex[e] = Ea[e] * S.E[e * cap + b] * dexp_(-(R)P.eta_e[e] * d2c);
-            R ca = 0, cb = 0, cc = 0;
+                for (int e = 0; e < NE; ++e)
+                    Ex[e] = e < ne ? Ea[e] * S.E[e * cap + b] * dexp_(-eta_e[e] * d2c) : (R)0;
+                R SA = 0, SAe = 0, SZ = 0;
 #pragma unroll
-            for (int q = 0; q < kBpMaxA; ++q)
-                if (q < P.na) {
-                    const R lam = (R)P.a_lam[q], zeta = (R)P.a_zeta[q], eta = (R)P.eta_e[P.a_e[q]];
-                    const R p1 = (R)1 + lam * cs;
-                    const R pz1 = P.a_iz[q] >= 1 ? powi_(p1, P.a_iz[q] - 1, zeta - (R)1)
-                                                 : (R)pow((double)p1, (double)(zeta - (R)1));
-                    const R pz = pz1 * p1;
-                    const R cm = w[kBpMaxR + q] * (R)P.a_coef[q] * Ex[P.a_e[q]];
-                    const R zl = zeta * lam * pz1 * F / da;
-                    cb += cm * zl;
-                    ca += cm * (-zl * cs + pz * ((R)-2 * eta * F * da + fb * dfa * fcc));
-                    cc += cm * pz * ((R)2 * eta * F * dc - fb * fa * dfcc);
-                }
-            const R ubx = bx / db, uby = by / db, ubz = bz / db;
-            const R ucx = cx / dc, ucy = cy / dc, ucz = cz / dc;
-            gx += ca * ux + cb * ubx + cc * ucx;
-            gy += ca * uy + cb * uby + cc * ucy;
-            gz += ca * uz + cb * ubz + cc * ucz;
+                for (int g = 0; g < NG; ++g)
+                    if (g < ng) {
+                        const R p1 = (R)1 + g_lam[g] * cs;
+                        const R pz1 = powu_(p1, g_iz[g] - 1);
+                        R K = 0, Ke = 0;
+#pragma unroll
+                        for (int e = 0; e < NE; ++e) {
+                            K += C[g * NE + e] * Ex[e];
+                            Ke += Ce[g * NE + e] * Ex[e];
+                        }
+                        const R pz = pz1 * p1;
+                        SA += K * pz;
+                        SAe += Ke * pz;
+                        SZ += g_zl[g] * K * pz1;
+                    }
+                SZ *= F;
+                const R cc = ((R)2 * F * dc * SAe - fa * fb * dfcc * SA) * ic;
+                const R ka = (-SZ * cs * ia - (R)2 * F * da * SAe + fb * dfa * fcc * SA) * ia;
+                const R kb = SZ * ia * ib;
+                hx += ka * ax + kb * bx + cc * cx;
+                hy += ka * ay + kb * by + cc * cy;
+                hz += ka * az + kb * bz + cc * cz;
+            }
+            if (++r == row) {
+                r = 0;
+                ++a;
+            }
         }
+    }
+    if (cur >= 0) {
+        atomicAdd(&S.g[cur], hx);
+        atomicAdd(&S.g[cap + cur], hy);
+        atomicAdd(&S.g[2 * cap + cur], hz);
+    }
+    } else {
+    // angular part, FP64: every unordered pair {a, b} once; d/dr_b is d/dr_a
+    // with a and b exchanged (u_c changes sign).  Pairs are walked by
+    // diagonal, d = b - a, then by a, so the lanes of one step hold
+    // consecutive a and consecutive b and the LDS accumulators take both
+    // sides without two lanes hitting one address.
+    for (PairWalk pw(nc, lane); pw.valid(); pw.advance()) {
+        const int a = pw.t, b = pw.t + pw.a + 1;
+        const R ax = S.x[a], ay = S.y[a], az = S.z[a], da = S.d[a];
+        const R bx = S.x[b], by = S.y[b], bz = S.z[b], db = S.d[b];
+        const R cx = bx - ax, cy = by - ay, cz = bz - az;
+        const R d2c = cx * cx + cy * cy + cz * cz;
+        if (!(d2c >= (R)1e-15 && d2c < Rc2)) continue;
+        const R fa = S.fc[a], dfa = S.dfc[a], fb = S.fc[b], dfb = S.dfc[b];
+        const R ia = S.id[a], ib = S.id[b], ic = drsq_(d2c);
+        const R dc = d2c * ic;
+        const R fcc = (R)0.5 * (dcos_(pirc * dc) + (R)1);
+        const R dfcc = (R)-0.5 * dsin_(pirc * dc) * pirc;
+        const R cs = (ax * bx + ay * by + az * bz) * ia * ib;
+        const R F = fa * fb * fcc;
+        R Ex[NE];
+#pragma unroll
+        for (int e = 0; e < NE; ++e)
+            Ex[e] = e < ne ? S.E[e * cap + a] * S.E[e * cap + b] * dexp_(-eta_e[e] * d2c) : (R)0;
+        R SA = 0, SAe = 0, SZ = 0;
+#pragma unroll
+        for (int g = 0; g < NG; ++g)
+            if (g < ng) {
+                const R p1 = (R)1 + g_lam[g] * cs;
+                const R pz1 = powu_(p1, g_iz[g] - 1);
+                R K = 0, Ke = 0;
+#pragma unroll
+                for (int e = 0; e < NE; ++e) {
+                    K += C[g * NE + e] * Ex[e];
+                    Ke += Ce[g * NE + e] * Ex[e];
+                }
+                const R pz = pz1 * p1;
+                SA += K * pz;
+                SAe += Ke * pz;
+                SZ += g_zl[g] * K * pz1;
+            }
+        SZ *= F;
+        const R cc = ((R)2 * F * dc * SAe - fa * fb * dfcc * SA) * ic;
+        const R ka = (-SZ * cs * ia - (R)2 * F * da * SAe + fb * dfa * fcc * SA) * ia;
+        const R kb = SZ * ia * ib;
+        const R la = (-SZ * cs * ib - (R)2 * F * db * SAe + fa * dfb * fcc * SA) * ib;
+        atomicAdd(&S.g[a], ka * ax + kb * bx + cc * cx);
+        atomicAdd(&S.g[cap + a], ka * ay + kb * by + cc * cy);
+        atomicAdd(&S.g[2 * cap + a], ka * az + kb * bz + cc * cz);
+        atomicAdd(&S.g[b], la * bx + kb * ax - cc * cx);
+        atomicAdd(&S.g[cap + b], la * by + kb * ay - cc * cy);
+        atomicAdd(&S.g[2 * cap + b], la * bz + kb * az - cc * cz);
+    }
+    }
+    __builtin_amdgcn_wave_barrier();
+    R ox = 0, oy = 0, oz = 0;
+    for (int a = lane; a < nc; a += kWave) {
+        const R gx = S.g[a], gy = S.g[cap + a], gz = S.g[2 * cap + a];
         R* q = Pout + ((size_t)i * cap + S.slot[a]) * 3;
         q[0] = -gx;
         q[1] = -gy;
@@ -717,7 +911,9 @@ Pot* bpnn_create(Ctx* ctx, const char* path) {
             D.a_zeta[a] = z;
             D.a_lam[a] = l;
             D.a_coef[a] = std::pow(2.0, 1.0 - z) * 2.0;  // base.cxx:120 factor 2
-            D.a_iz[a] = (z == std::floor(z) && z >= 0 && z <= 32) ? (int)z : -1;
+            if (!(z == std::floor(z) && z >= 1 && z <= 32))
+                throw Error(E_INVAL, "G4 zeta must be a whole number from 1 to 32");
+            D.a_iz[a] = (int)z;
         } else {
             throw Error(E_INVAL, "only G2 radial and G4 angular descriptors are supported");
         }
@@ -726,6 +922,29 @@ Pot* bpnn_create(Ctx* ctx, const char* path) {
     if (D.nr + D.na != nin) throw Error(E_INVAL, "descriptor count must equal the network input width");
     D.ne = (int)etas.size();
     for (int e = 0; e < D.ne; ++e) D.eta_e[e] = etas[e];
+    // G4 terms by (lambda, zeta) group and eta
+    D.ng = 0;
+    for (int t = 0; t < kBpMaxG * kBpMaxEta; ++t) {
+        D.t_q[t] = -1;
+        D.t_coef[t] = 0.0;
+    }
+    for (int q = 0; q < D.na; ++q) {
+        int g = -1;
+        for (int h = 0; h < D.ng; ++h)
+            if (D.g_lam[h] == D.a_lam[q] && D.g_zeta[h] == D.a_zeta[q]) g = h;
+        if (g < 0) {
+            if (D.ng >= kBpMaxG) throw Error(E_INVAL, "at most 8 distinct (lambda, zeta) pairs among the G4 terms");
+            g = D.ng++;
+            D.g_lam[g] = D.a_lam[q];
+            D.g_zeta[g] = D.a_zeta[q];
+            D.g_zl[g] = D.a_zeta[q] * D.a_lam[q];
+            D.g_iz[g] = D.a_iz[q];
+        }
+        const int t = g * kBpMaxEta + D.a_e[q];
+        if (D.t_q[t] >= 0) throw Error(E_INVAL, "two G4 terms with the same eta, lambda and zeta");
+        D.t_q[t] = q;
+        D.t_coef[t] = D.a_coef[q];
+    }
     D.Rc2 = D.Rc * D.Rc;
     D.pi_rc = M_PI / D.Rc;
     D.nin = nin;
@@ -860,7 +1079,10 @@ void bpnn_compute(Batch& b, const Pot& p, int what, int prec, int mode, double d
         using R = decltype(rtag);
         const size_t lds = waves * stage_bytes(b.cap, bp.h.ne, sizeof(R));
         if (lds > 160 * 1024) throw Error(E_INVAL, "BPNN neighbour staging exceeds the LDS");
-        launch_k(b, K_BP_DESC, k_bp_desc<R>, gw, bw, (unsigned)lds, pos, b.nl, b.nn, b.tpa, b.M, b.cap, b.d_desc,
+        const bool small = bp.h.nr <= 8 && bp.h.ng <= 4 && bp.h.ne <= 2;
+        auto kdesc = small ? k_bp_desc<R, 8, 4, 2> : k_bp_desc<R, kBpMaxR, kBpMaxG, kBpMaxEta>;
+        auto kgrad = small ? k_bp_grad<R, 8, 4, 2> : k_bp_grad<R, kBpMaxR, kBpMaxG, kBpMaxEta>;
+        launch_k(b, K_BP_DESC, kdesc, gw, bw, (unsigned)lds, pos, b.nl, b.nn, b.tpa, b.M, b.cap, b.d_desc,
                  bp.d, (R*)w.G, ldg, n, S, b.misc);
         if (sizeof(R) == 8)
             launch_k(b, K_BP_MLP, k_bp_mlp64, dim3((n + 127) / 128), dim3(128), 0, (const double*)w.G, ldg, bp.w64,
@@ -868,7 +1090,7 @@ void bpnn_compute(Batch& b, const Pot& p, int what, int prec, int mode, double d
         else
             launch_k(b, K_BP_MLP, k_bp_mlp_mfma, dim3((n + 15) / 16), dim3(64), 0, (const float*)w.G, ldg, bp.bsplit,
                      boff, nts, kts, bp.w32, bp.d, pos, (float*)w.w, Ep, n, S, b.misc);
-        launch_k(b, K_BP_FORCE, k_bp_grad<R>, gw, bw, (unsigned)lds, pos, b.nl, b.nn, b.tpa, b.M, b.cap, b.d_desc,
+        launch_k(b, K_BP_FORCE, kgrad, gw, bw, (unsigned)lds, pos, b.nl, b.nn, b.tpa, b.M, b.cap, b.d_desc,
                  bp.d, (const R*)w.w, ldg, (R*)w.P, (R*)w.own, n, S, b.misc);
         launch_k(b, K_BP_GATHER, k_bp_gather<R>, dim3((n + 255) / 256), dim3(256), 0, pos, pos_next, b.nl, b.nn,
                  w.rev, b.tpa, b.M, b.cap, b.d_desc, (const R*)w.P, (const R*)w.own, AA, S, b.rsave, b.misc);
