@@ -364,6 +364,8 @@ def main():
     ap.add_argument("--summarise-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
     ap.add_argument("--summarise-stats", metavar="STATS_DIR")
     ap.add_argument("--kernel", default="k_eam_force<float")
+    ap.add_argument("--only", choices=("bpnn_cu", "nial_batch", "served"),
+                    help="run one of the extra workloads alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.summarise_pmc:
         print(summarise_pmc(*args.summarise_pmc, args.kernel))
@@ -380,6 +382,19 @@ def main():
     from mdcraft_amd import workloads as W
 
     ctx = E.Context(0)
+    if args.only:
+        if args.only == "bpnn_cu":
+            wb = W.bpnn_cu()
+            out = {p: run_batch(E, ctx, [(wb["atoms"], wb["box"])], wb["pot"], wb["dt"], pr, 200, 10, args.lanes)
+                   for p, pr in (("mixed", E.MIXED), ("fp64", E.FP64))}
+        elif args.only == "nial_batch":
+            nb = W.nial_batch(512)
+            out = {p: run_batch_eval(E, ctx, nb["cells"], nb["pot"], pr, 50, args.lanes)
+                   for p, pr in (("mixed", E.MIXED), ("fp64", E.FP64))}
+        else:
+            out = served_stream(E, ctx, W)
+        print(json.dumps({"workload": args.only, "result": out}), flush=True)
+        return
     w = W.eam_cu()
     head = run_gpu(E, ctx, w, E.MIXED, args.steps, args.warmup, args.lanes)
     nbar = head["nbar"]
