@@ -151,7 +151,7 @@ __global__ void k_permute(const int* __restrict__ src, const double4* __restrict
 // keep stencil order, then cell order.  The candidates are gathered once
 // per cell instead of once per atom.  Counts beyond the capacity are still
 // counted so the host can grow the list.
-constexpr int kTile = 1024;
+constexpr int kTile = 448;
 
 __global__ void __launch_bounds__(256) k_build(const double4* __restrict__ pos, const double* __restrict__ rns,
                                                const int* __restrict__ start, const SysDesc* __restrict__ desc,
