@@ -120,6 +120,24 @@ mdc_status mdc_sys_autotune(mdc_sys* sys, const mdc_pot* pot, int precision, int
 mdc_status mdc_nlist_build(mdc_sys* sys);
 /* decide(): 1 when any atom moved more than half the skin (rns - rcut) */
 mdc_status mdc_nlist_decide(mdc_sys* sys, double rcut, int* need);
+/* update(kbuf > 0): VerletList::optimize_search_radius
+ * (neibs/verlet-list.cxx:71-92) over the current list: rns_i = kbuf * max
+ * over atom i's list (self included) of rcut_i + rcut_j, with rcut as last
+ * uploaded; 0 for atoms of a system with no list yet (the reference's
+ * lists start empty).  rns_out (n doubles, caller's order, may be NULL)
+ * receives the new radii; the list is out of date until mdc_nlist_build. */
+mdc_status mdc_nlist_optimize_rns(mdc_sys* sys, double kbuf, double* rns_out);
+/* automatic = 1 (default): step sequences rebuild the list on the half-skin
+ * trigger.  automatic = 0: the list is rebuilt only by mdc_nlist_build, as
+ * the reference's callers drive VerletList::update themselves
+ * (test/python/solver/test-EAM.py: update every NCbuff steps); each pair's
+ * image is then chosen from the current positions at every step
+ * (solver/boundary/periodic.cxx:36-60).  No reference counterpart: the
+ * reference has no automatic rebuild. */
+mdc_status mdc_sys_set_rebuild(mdc_sys* sys, int automatic);
+/* Attach (1) or detach (0) boundary::Periodic after creation: the boundary
+ * argument of solver::Single (solver/single.h, python/mdcraft/solver/_solver.cxx:71-82). */
+mdc_status mdc_sys_set_boundary(mdc_sys* sys, int periodic_boundary);
 /* full list including self in the reference's form: CSR over original atom
  * order; offsets has n+1 entries; ids NULL queries the size via *total */
 mdc_status mdc_nlist_export(mdc_sys* sys, uint64_t* offsets, uint64_t* ids, uint64_t* total);
@@ -128,6 +146,20 @@ mdc_status mdc_nlist_stats(mdc_sys* sys, double* mean_neighbours, int* max_neigh
 /* ---- force evaluation and integration ----------------------------------- */
 /* prepare + stages (+ virials when MDC_ENERGY | MDC_VIRIAL) */
 mdc_status mdc_compute(mdc_sys* sys, const mdc_pot* pot, uint32_t what, int precision);
+/* One solver call of the reference's Python API on the resident state
+ * (solver::Pair::{prepare,charge,forces,virials}, solver/pair.cxx:16-127;
+ * solver::Single, single.cxx:38-83; solver::Ternary, ternary.cxx:19-67):
+ *   MDC_STAGE_PREPARE  zero f, V, Ep (and Em, nc for EAM) of atoms with
+ *                      tag != 0, set rcut; BPNN: descriptors, network,
+ *                      dE/dG and Ep (maise_bpnn.cxx:27-59, :275-325)
+ *   MDC_STAGE_CHARGE   EAM density into nc (EAM.cxx:499-560)
+ *   MDC_STAGE_FORCES   add the force to f; EAM takes F'(nc) of the nc the
+ *                      atoms hold, whether or not a charge call preceded
+ *   MDC_STAGE_VIRIALS  add the pair energy and virial to Ep and V, set Em =
+ *                      F(nc) for EAM; f is left as it is (BPNN: V stays 0)
+ * The list must be built. */
+enum { MDC_STAGE_PREPARE = 0, MDC_STAGE_CHARGE = 1, MDC_STAGE_FORCES = 2, MDC_STAGE_VIRIALS = 3 };
+mdc_status mdc_stage(mdc_sys* sys, const mdc_pot* pot, int stage, int precision);
 /* Mean duration in microseconds of the step-mode density (us_out[0], 0 for
  * LJ) and force (us_out[1]) kernels, each launched reps times back to back
  * between two events on the context stream.  Leaves positions, velocities,

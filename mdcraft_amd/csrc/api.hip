@@ -516,6 +516,7 @@ void upload(Batch& b, const AtomRec* a, uint32_t fields) {
     dfree(d_stage);
     dfree(d_spos);
     if (fields & MDC_F_F) b.f_prec = P_FP64;
+    if (fields & MDC_F_RCUT) b.last_rcut = -1;
     if (rns_changed) {
         layout_cells(b);
         size_list(b);
@@ -567,7 +568,7 @@ void download(Batch& b, AtomRec* a, uint32_t fields, int first, int count) {
                 std::memcpy(r.V, V, sizeof(V));
             }
             if (fields & MDC_F_M) r.m = get(D_M, o);
-            if (fields & MDC_F_RCUT) r.rcut = b.last_rcut >= 0 ? b.last_rcut : b.rcut_host[o];
+            if (fields & MDC_F_RCUT) r.rcut = b.last_rcut >= 0 && b.tag[o] != 0 ? b.last_rcut : b.rcut_host[o];
             if (fields & MDC_F_RNS) r.rns = get(D_RNS, o);
             if (fields & MDC_F_EP) r.Ep = get(D_EP, o);
             if (fields & MDC_F_EM) r.Em = get(D_EM, o);
@@ -590,6 +591,66 @@ void download(Batch& b, AtomRec* a, uint32_t fields, int first, int count) {
             }
         }
     }
+}
+
+// VerletList::optimize_search_radius (verlet-list.cxx:71-92): rns_i =
+// kbuf * the largest rcut_i + rcut_j over atom i's current list, self
+// included.  rc holds rcut in list (slot) order.
+__global__ void k_opt_rns(const int* __restrict__ nl, const int* __restrict__ nn, const double* __restrict__ rc,
+                          double* __restrict__ rns, int tpa, int M, int cap, int n, double kbuf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double ri = rc[i];
+    double h = ri + ri;
+    const int c = min(nn[i], cap);
+    for (int k = 0; k < c; ++k) h = fmax(h, ri + rc[ent_j(nl[list_index(i, k, tpa, M)])]);
+    rns[i] = kbuf * h;
+}
+
+// New rns of every atom from the current list (zero for an atom with no
+// list yet, as the reference's empty lists give), in the caller's order;
+// the list is then out of date and the cell layout follows the new radii.
+void optimize_rns(Batch& b, double kbuf, double* rns_out) {
+    const int n = b.n;
+    if (n == 0) return;
+    hipStream_t st = b.ctx->stream;
+    std::vector<double> rns(n, 0.0);
+    if (b.list_valid) {
+        std::vector<int> where;
+        sync_where(b, where);
+        std::vector<double> rc(n);
+        for (int o = 0; o < n; ++o) rc[where[o]] = b.rcut_host[o];
+        double* d_rc = dalloc<double>(n);
+        try {
+            HIPCHECK(hipMemcpyAsync(d_rc, rc.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_opt_rns, dim3((n + 255) / 256), dim3(256), 0, st, b.nl, b.nn, d_rc, b.d(b.sc, D_RNS),
+                               b.tpa, b.M, b.cap, n, kbuf);
+            HIPCHECK(hipMemcpyAsync(rc.data(), b.d(b.sc, D_RNS), sizeof(double) * n, hipMemcpyDeviceToHost, st));
+            HIPCHECK(hipStreamSynchronize(st));
+        } catch (...) {
+            dfree(d_rc);
+            throw;
+        }
+        dfree(d_rc);
+        for (int o = 0; o < n; ++o) rns[o] = rc[where[o]];
+    } else {
+        HIPCHECK(hipMemsetAsync(b.d(b.sc, D_RNS), 0, sizeof(double) * n, st));
+        HIPCHECK(hipStreamSynchronize(st));
+    }
+    for (int s = 0; s < b.nsys; ++s) {
+        double mn = 1e300, mx = 0;
+        for (int o = b.atom_off[s]; o < b.atom_off[s + 1]; ++o) {
+            mn = std::min(mn, rns[o]);
+            mx = std::max(mx, rns[o]);
+        }
+        if (b.atom_off[s + 1] == b.atom_off[s]) mn = mx = 0;
+        b.rns_min[s] = mn;
+        b.rns_max[s] = mx;
+    }
+    layout_cells(b);
+    size_list(b);
+    b.list_valid = false;
+    if (rns_out) std::memcpy(rns_out, rns.data(), sizeof(double) * n);
 }
 
 void check_kinds(const Batch& b, const Pot& p) {
@@ -894,6 +955,49 @@ mdc_status mdc_nlist_decide(mdc_sys* sys, double rcut, int* need) {
         if (!sys || !need) throw Error(E_INVAL, "null argument");
         HIPCHECK(hipSetDevice(sys->b.ctx->device));
         *need = nlist_decide(sys->b, rcut);
+    });
+}
+
+mdc_status mdc_nlist_optimize_rns(mdc_sys* sys, double kbuf, double* rns_out) {
+    return guard(sys ? sys->b.ctx : nullptr, [&] {
+        if (!sys) throw Error(E_INVAL, "null argument");
+        if (!(kbuf > 0) || !std::isfinite(kbuf)) throw Error(E_INVAL, "kbuf must be positive");
+        HIPCHECK(hipSetDevice(sys->b.ctx->device));
+        optimize_rns(sys->b, kbuf, rns_out);
+    });
+}
+
+mdc_status mdc_sys_set_rebuild(mdc_sys* sys, int automatic) {
+    return guard(sys ? sys->b.ctx : nullptr, [&] {
+        if (!sys) throw Error(E_INVAL, "null argument");
+        sys->b.manual_rebuild = automatic == 0;
+    });
+}
+
+mdc_status mdc_sys_set_boundary(mdc_sys* sys, int periodic_boundary) {
+    return guard(sys ? sys->b.ctx : nullptr, [&] {
+        if (!sys) throw Error(E_INVAL, "null argument");
+        Batch& b = sys->b;
+        HIPCHECK(hipSetDevice(b.ctx->device));
+        for (int s = 0; s < b.nsys; ++s) {
+            b.periodic_bc[s] = periodic_boundary ? 1 : 0;
+            SysDesc& d = b.desc[s];
+            d.any_per = 0;
+            for (int a = 0; a < 3; ++a) d.any_per |= (d.per[a] && periodic_boundary) ? 1 : 0;
+        }
+        HIPCHECK(hipMemcpyAsync(b.d_desc, b.desc.data(), sizeof(SysDesc) * b.nsys, hipMemcpyHostToDevice,
+                                b.ctx->stream));
+        HIPCHECK(hipStreamSynchronize(b.ctx->stream));
+    });
+}
+
+mdc_status mdc_stage(mdc_sys* sys, const mdc_pot* pot, int stage, int precision) {
+    return guard(sys ? sys->b.ctx : nullptr, [&] {
+        if (!sys || !pot) throw Error(E_INVAL, "null argument");
+        if (precision != MDC_FP64 && precision != MDC_MIXED) throw Error(E_INVAL, "precision must be FP64 or MIXED");
+        HIPCHECK(hipSetDevice(sys->b.ctx->device));
+        check_kinds(sys->b, pot->p);
+        stage_compute(sys->b, pot->p, stage, precision);
     });
 }
 

@@ -1050,7 +1050,7 @@ static void ensure_work(Batch& b, const BpnnParams& bp, int prec) {
 }
 
 void bpnn_compute(Batch& b, const Pot& p, int what, int prec, int mode, double dt, int step, const double4* pos,
-                  double4* pos_next) {
+                  double4* pos_next, int parts) {
     const BpnnParams& bp = *p.bp;
     for (int s = 0; s < b.nsys; ++s)
         for (int a = 0; a < 3; ++a)
@@ -1082,14 +1082,17 @@ void bpnn_compute(Batch& b, const Pot& p, int what, int prec, int mode, double d
         const bool small = bp.h.nr <= 8 && bp.h.ng <= 4 && bp.h.ne <= 2;
         auto kdesc = small ? k_bp_desc<R, 8, 4, 2> : k_bp_desc<R, kBpMaxR, kBpMaxG, kBpMaxEta>;
         auto kgrad = small ? k_bp_grad<R, 8, 4, 2> : k_bp_grad<R, kBpMaxR, kBpMaxG, kBpMaxEta>;
-        launch_k(b, K_BP_DESC, kdesc, gw, bw, (unsigned)lds, pos, b.nl, b.nn, b.tpa, b.M, b.cap, b.d_desc,
-                 bp.d, (R*)w.G, ldg, n, S, b.misc);
-        if (sizeof(R) == 8)
-            launch_k(b, K_BP_MLP, k_bp_mlp64, dim3((n + 127) / 128), dim3(128), 0, (const double*)w.G, ldg, bp.w64,
-                     bp.d, pos, (double*)w.w, Ep, n, S, b.misc);
-        else
-            launch_k(b, K_BP_MLP, k_bp_mlp_mfma, dim3((n + 15) / 16), dim3(64), 0, (const float*)w.G, ldg, bp.bsplit,
-                     boff, nts, kts, bp.w32, bp.d, pos, (float*)w.w, Ep, n, S, b.misc);
+        if (parts & 1) {
+            launch_k(b, K_BP_DESC, kdesc, gw, bw, (unsigned)lds, pos, b.nl, b.nn, b.tpa, b.M, b.cap, b.d_desc,
+                     bp.d, (R*)w.G, ldg, n, S, b.misc);
+            if (sizeof(R) == 8)
+                launch_k(b, K_BP_MLP, k_bp_mlp64, dim3((n + 127) / 128), dim3(128), 0, (const double*)w.G, ldg,
+                         bp.w64, bp.d, pos, (double*)w.w, Ep, n, S, b.misc);
+            else
+                launch_k(b, K_BP_MLP, k_bp_mlp_mfma, dim3((n + 15) / 16), dim3(64), 0, (const float*)w.G, ldg,
+                         bp.bsplit, boff, nts, kts, bp.w32, bp.d, pos, (float*)w.w, Ep, n, S, b.misc);
+        }
+        if (!(parts & 2)) return;
         launch_k(b, K_BP_FORCE, kgrad, gw, bw, (unsigned)lds, pos, b.nl, b.nn, b.tpa, b.M, b.cap, b.d_desc,
                  bp.d, (const R*)w.w, ldg, (R*)w.P, (R*)w.own, n, S, b.misc);
         launch_k(b, K_BP_GATHER, k_bp_gather<R>, dim3((n + 255) / 256), dim3(256), 0, pos, pos_next, b.nl, b.nn,

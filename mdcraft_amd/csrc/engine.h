@@ -35,7 +35,11 @@ enum DField {
 enum FField { F_FX, F_FY, F_FZ, F_NFIELDS };
 
 enum Prec { P_FP64 = 0, P_MIXED = 1 };
-enum What { W_FORCE = 1, W_ENERGY = 2, W_VIRIAL = 4 };
+// W_NC_STORED: the EAM density pass takes F(nc), F'(nc) of the stored
+// density instead of the one it sums, and leaves nc as it is
+enum What { W_FORCE = 1, W_ENERGY = 2, W_VIRIAL = 4, W_NC_STORED = 8 };
+// solver calls of the reference's Python API, one at a time
+enum SolverStage { ST_PREPARE = 0, ST_CHARGE = 1, ST_FORCES = 2, ST_VIRIALS = 3 };
 enum PotType { PT_LJ = 1, PT_EAM = 2, PT_BPNN = 4 };
 // misc device counters
 enum Misc { M_MAXNN = 0, M_ABORT = 1, M_ERR = 2, M_DECIDE = 3, M_COUNT = 8 };
@@ -134,6 +138,12 @@ struct Batch {
     int f_prec = -1;      // which force array holds the current force
     double last_rcut = -1;
     int rebuilds = 0;
+    // list rebuilt only on request (the reference's VerletList::update);
+    // step sequences then never trigger one and choose images per step
+    bool manual_rebuild = false;
+    // precision of the BPNN descriptors and dE/dG of the last prepare stage
+    int bp_prepared = -1;
+    int bp_prepared_builds = -1;
     // steps between rebuilds (running estimate) and steps since the last one,
     // used to size the launch chunks of a step sequence
     double pred_interval = 0.0;
@@ -205,10 +215,13 @@ void run_steps(Batch& b, const Pot& p, double dt, int nsteps, int prec, int chun
 void time_step_kernels(Batch& b, const Pot& p, int prec, double dt, int reps, double* us);
 void thermo(Batch& b, double* out);
 void set_trigger(Batch& b, double rcut, bool allow_shift);
+void stage_compute(Batch& b, const Pot& p, int stage, int prec);
 void check_errors(Batch& b);
 // launchers (bpnn.hip)
+// parts: 1 = descriptors and network (G, dE/dG, Ep), 2 = forces from the
+// stored dE/dG, 3 = both
 void bpnn_compute(Batch& b, const Pot& p, int what, int prec, int mode, double dt, int step, const double4* pos,
-                  double4* pos_next);
+                  double4* pos_next, int parts = 3);
 void bpnn_release_work(Batch& b);
 void bpnn_inspect(Batch& b, const Pot& p, double* G, double* dEdG);
 int bpnn_inputs(const Pot& p);

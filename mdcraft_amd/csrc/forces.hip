@@ -293,8 +293,10 @@ __global__ void __launch_bounds__(256) k_eam_rho(const double4* __restrict__ pos
     const int n = A.n;
     // tag == 0: the reference skips the atom's prepare and charge, yet
     // neighbours still evaluate F'(nc) of its stored density
-    // (single.cxx:44-50, EAM.cxx:467-474)
-    const double rr = active ? rho : A.d[(size_t)D_NC * n + i];
+    // (single.cxx:44-50, EAM.cxx:467-474).  W_NC_STORED: a forces() or
+    // virials() call on its own reads the density the atoms hold.
+    const bool from_stored = !active || (S.flags & W_NC_STORED);
+    const double rr = from_stored ? A.d[(size_t)D_NC * n + i] : rho;
     if (!MULTI) {
         double F, dF;
         embed_at(P.F64, 0, P.nrho, P.drho, P.drho1, rr, F, dF, misc);
@@ -310,7 +312,7 @@ __global__ void __launch_bounds__(256) k_eam_rho(const double4* __restrict__ pos
             if (active && k == klast && S.store_aux && (S.flags & W_ENERGY)) A.d[(size_t)D_EM * n + i] = F;
         }
     }
-    if (active && S.store_aux) A.d[(size_t)D_NC * n + i] = rho;
+    if (active && S.store_aux && !(S.flags & W_NC_STORED)) A.d[(size_t)D_NC * n + i] = rho;
 }
 
 // Force pass (EAM.cxx:356-497) and, when asked, pair energy and virial
@@ -616,6 +618,12 @@ void set_trigger(Batch& b, double rcut, bool allow_shift) {
         int sh = allow_shift && skin > 0.0 ? 1 : 0;
         for (int a = 0; a < 3; ++a)
             if (d.per[a] && !(0.5 * d.L[a] > b.rns_max[s] + skin)) sh = 0;
+        if (allow_shift && b.manual_rebuild) {
+            // the caller rebuilds: no trigger, and each pair's image is
+            // chosen from the current positions at every step
+            t = 1e300;
+            sh = 0;
+        }
         if (d.trig2 != t || d.shift_ok != sh) {
             d.trig2 = t;
             d.shift_ok = sh;
@@ -750,6 +758,134 @@ void force_compute(Batch& b, const Pot& p, int what, int prec) {
     launch_step_kernels(b, p, S, prec, b.pos[b.cur], nullptr);
     b.f_prec = prec;
     b.last_rcut = p.rcut;
+    read_misc(b);
+    check_errors(b);
+}
+
+// Per-atom prepare of solver::Single and solver::Ternary
+// (single.cxx:38-53, ternary.cxx:26-40): atoms with tag != 0 get zero
+// force, virial and pair energy, and for EAM zero embedding energy and
+// density.
+__global__ void k_prepare(const double4* __restrict__ pos, AtomArrays A, int eam) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = A.n;
+    if (i >= n || w_inactive(pos[i].w)) return;
+    for (int f = D_FX; f <= D_FZ; ++f) A.d[(size_t)f * n + i] = 0.0;
+    for (int f = F_FX; f <= F_FZ; ++f) A.f[(size_t)f * n + i] = 0.0f;
+    for (int f = D_VXX; f <= D_VYZ; ++f) A.d[(size_t)f * n + i] = 0.0;
+    A.d[(size_t)D_EP * n + i] = 0.0;
+    if (eam) {
+        A.d[(size_t)D_EM * n + i] = 0.0;
+        A.d[(size_t)D_NC * n + i] = 0.0;
+    }
+}
+
+// virials() adds the pair energy and virial to what the atoms hold and
+// leaves their force (pair.cxx:100-127, LJs-cut.cxx:86-131,
+// EAM.cxx:306-354).  The kernels wrote fresh values into the current
+// state; `was` holds the state from before the call.
+__global__ void k_virial_merge(const double4* __restrict__ pos, AtomArrays A, const double* __restrict__ was_d,
+                               const float* __restrict__ was_f) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = A.n;
+    if (i >= n || w_inactive(pos[i].w)) return;
+    for (int f = D_FX; f <= D_FZ; ++f) A.d[(size_t)f * n + i] = was_d[(size_t)f * n + i];
+    for (int f = F_FX; f <= F_FZ; ++f) A.f[(size_t)f * n + i] = was_f[(size_t)f * n + i];
+    A.d[(size_t)D_EP * n + i] += was_d[(size_t)D_EP * n + i];
+    for (int f = D_VXX; f <= D_VYZ; ++f) A.d[(size_t)f * n + i] += was_d[(size_t)f * n + i];
+}
+
+// forces() adds to the force the atoms hold (single.cxx:65-73,
+// pair.cxx:72-87); the kernels wrote the fresh force of precision `prec`,
+// `was` holds the state from before the call with its force in `was_prec`.
+__global__ void k_force_merge(const double4* __restrict__ pos, AtomArrays A, const double* __restrict__ was_d,
+                              const float* __restrict__ was_f, int was_prec, int prec) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = A.n;
+    if (i >= n || w_inactive(pos[i].w)) return;
+    for (int c = 0; c < 3; ++c) {
+        const double w = was_prec == P_MIXED ? (double)was_f[(size_t)(F_FX + c) * n + i]
+                                             : was_d[(size_t)(D_FX + c) * n + i];
+        if (prec == P_MIXED) A.f[(size_t)(F_FX + c) * n + i] = (float)(w + (double)A.f[(size_t)(F_FX + c) * n + i]);
+        else A.d[(size_t)(D_FX + c) * n + i] += w;
+    }
+}
+
+// One solver call of the reference's Python API on the resident state:
+// prepare, charge, forces or virials (pair.cxx:16-127, single.cxx:23-83,
+// ternary.cxx:19-67).  Each reads what the atoms hold, as the reference's
+// do: forces() without a charge() before it uses the stored density.
+void stage_compute(Batch& b, const Pot& p, int stage, int prec) {
+    if (b.n == 0) return;
+    if (!b.list_valid) throw Error(E_STATE, "no neighbour list: update the list first");
+    hipStream_t st = b.ctx->stream;
+    set_trigger(b, p.rcut, false);
+    const bool eam = p.type == PT_EAM;
+    const bool bpnn = p.type == PT_BPNN;
+    const int grid = nblk(b.n, 256);
+    AtomArrays AA{b.dst[b.sc], b.fst[b.sc], b.n};
+    const double4* pos = b.pos[b.cur];
+    auto pass = [&](int which, const StepArgs& S) {
+        if (prec == P_MIXED) launch_pair_r<float>(b, p, S, pos, nullptr, which);
+        else launch_pair_r<double>(b, p, S, pos, nullptr, which);
+    };
+    switch (stage) {
+        case ST_PREPARE:
+            hipLaunchKernelGGL(k_prepare, dim3(grid), dim3(256), 0, st, pos, AA, eam ? 1 : 0);
+            // solver::Multi's prepare leaves rcut (multi.cxx:56-66)
+            if (!(eam && p.eam.nk > 1)) b.last_rcut = p.rcut;
+            b.f_prec = P_FP64;
+            b.bp_prepared = -1;
+            if (bpnn) {
+                // the network runs in prepare (maise_bpnn.cxx:27-59, :275-325)
+                bpnn_compute(b, p, W_ENERGY, prec, MODE_COMPUTE, 0.0, -1, pos, nullptr, 1);
+                b.bp_prepared = prec;
+                b.bp_prepared_builds = b.rebuilds;
+            }
+            break;
+        case ST_CHARGE:
+            if (eam) pass(1, StepArgs{0.0, -1, MODE_COMPUTE, 0, 1});
+            break;
+        case ST_FORCES: {
+            const int idle = b.sc ^ 1;
+            const int was_prec = b.f_prec < 0 ? P_FP64 : b.f_prec;
+            if (b.f_prec < 0) HIPCHECK(hipMemsetAsync(b.d(idle, D_FX), 0, sizeof(double) * 3 * b.n, st));
+            else if (was_prec == P_MIXED)
+                HIPCHECK(hipMemcpyAsync(b.fl(idle, F_FX), b.fl(b.sc, F_FX), sizeof(float) * 3 * b.n,
+                                        hipMemcpyDeviceToDevice, st));
+            else
+                HIPCHECK(hipMemcpyAsync(b.d(idle, D_FX), b.d(b.sc, D_FX), sizeof(double) * 3 * b.n,
+                                        hipMemcpyDeviceToDevice, st));
+            if (bpnn) {
+                if (b.bp_prepared != prec || b.bp_prepared_builds != b.rebuilds)
+                    throw Error(E_STATE, "BPNN forces need a prepare at the same precision on the current list");
+                bpnn_compute(b, p, 0, prec, MODE_COMPUTE, 0.0, -1, pos, nullptr, 2);
+            } else {
+                if (eam) pass(1, StepArgs{0.0, -1, MODE_COMPUTE, W_NC_STORED, 1});
+                pass(2, StepArgs{0.0, -1, MODE_COMPUTE, 0, 1});
+            }
+            hipLaunchKernelGGL(k_force_merge, dim3(grid), dim3(256), 0, st, pos, AA, b.dst[idle], b.fst[idle],
+                               was_prec, prec);
+            b.f_prec = prec;
+            break;
+        }
+        case ST_VIRIALS: {
+            // the BPNN virial is left at zero (the reference's is a partial
+            // diagonal, maise_bpnn.cxx:98-120)
+            if (bpnn) break;
+            const int idle = b.sc ^ 1;
+            HIPCHECK(hipMemcpyAsync(b.dst[idle], b.dst[b.sc], sizeof(double) * D_NFIELDS * b.n,
+                                    hipMemcpyDeviceToDevice, st));
+            HIPCHECK(hipMemcpyAsync(b.fst[idle], b.fst[b.sc], sizeof(float) * F_NFIELDS * b.n,
+                                    hipMemcpyDeviceToDevice, st));
+            if (eam) pass(1, StepArgs{0.0, -1, MODE_COMPUTE, W_NC_STORED | W_ENERGY, 1});
+            pass(2, StepArgs{0.0, -1, MODE_COMPUTE, W_ENERGY | W_VIRIAL, 1});
+            hipLaunchKernelGGL(k_virial_merge, dim3(grid), dim3(256), 0, st, pos, AA, b.dst[idle], b.fst[idle]);
+            break;
+        }
+        default:
+            throw Error(E_INVAL, "stage must be prepare (0), charge (1), forces (2) or virials (3)");
+    }
     read_misc(b);
     check_errors(b);
 }

@@ -23,6 +23,7 @@ LIB_PATH = os.path.join(_HERE, "libmdc.so")
 
 FP64, MIXED = 0, 1
 FORCE, ENERGY, VIRIAL = 1, 2, 4
+PREPARE, CHARGE, FORCES, VIRIALS = 0, 1, 2, 3
 F_R, F_V, F_F, F_VIR, F_M, F_RCUT, F_RNS, F_EP, F_EM, F_NC, F_BC, F_KIND, F_UID, F_TAG = (1 << k for k in range(14))
 F_ALL = (1 << 14) - 1
 
@@ -31,8 +32,9 @@ EXPORTS = [
     "mdc_pot_lj_create", "mdc_pot_eam_create", "mdc_pot_eam_create_file", "mdc_pot_multi_create",
     "mdc_pot_bpnn_create", "mdc_pot_rcut", "mdc_pot_destroy",
     "mdc_sys_create", "mdc_sys_destroy", "mdc_sys_upload", "mdc_sys_download", "mdc_sys_download_range", "mdc_sys_set_active", "mdc_sys_natoms",
-    "mdc_sys_set_lanes", "mdc_sys_autotune", "mdc_sys_time_kernels", "mdc_nlist_build", "mdc_nlist_decide", "mdc_nlist_export", "mdc_nlist_stats",
-    "mdc_compute", "mdc_step", "mdc_thermo", "mdc_sync", "mdc_timing", "mdc_bpnn_inspect",
+    "mdc_sys_set_lanes", "mdc_sys_autotune", "mdc_sys_time_kernels", "mdc_sys_set_rebuild", "mdc_sys_set_boundary",
+    "mdc_nlist_build", "mdc_nlist_decide", "mdc_nlist_optimize_rns", "mdc_nlist_export", "mdc_nlist_stats",
+    "mdc_compute", "mdc_stage", "mdc_step", "mdc_thermo", "mdc_sync", "mdc_timing", "mdc_bpnn_inspect",
 ]
 
 
@@ -68,7 +70,11 @@ def load_library(path=LIB_PATH):
         "mdc_sys_set_lanes": (I, [P, I]),
         "mdc_sys_autotune": (I, [P, P, I, I, C.POINTER(I), DP]),
         "mdc_sys_time_kernels": (I, [P, P, I, D, I, DP]),
+        "mdc_sys_set_rebuild": (I, [P, I]),
+        "mdc_sys_set_boundary": (I, [P, I]),
         "mdc_nlist_build": (I, [P]),
+        "mdc_nlist_optimize_rns": (I, [P, D, DP]),
+        "mdc_stage": (I, [P, P, I, I]),
         "mdc_nlist_decide": (I, [P, D, C.POINTER(I)]),
         "mdc_nlist_export": (I, [P, C.POINTER(U64), C.POINTER(U64), C.POINTER(U64)]),
         "mdc_nlist_stats": (I, [P, DP, C.POINTER(I), C.POINTER(I), C.POINTER(I)]),
@@ -262,6 +268,25 @@ class System:
 
     def build(self):
         _check(lib().mdc_nlist_build(self.h))
+
+    def optimize_rns(self, kbuf):
+        """rns_i = kbuf * max over the current list of rcut_i + rcut_j
+        (update(kbuf > 0)); returns the new radii in upload order."""
+        out = np.zeros(self.n)
+        _check(lib().mdc_nlist_optimize_rns(self.h, kbuf, _dp(out)))
+        return out
+
+    def set_rebuild(self, automatic):
+        """False: the list is rebuilt only by build(), as the reference's
+        callers do; True (default): step() rebuilds on the half-skin trigger."""
+        _check(lib().mdc_sys_set_rebuild(self.h, int(bool(automatic))))
+
+    def set_boundary(self, periodic_boundary):
+        _check(lib().mdc_sys_set_boundary(self.h, int(bool(periodic_boundary))))
+
+    def stage(self, pot, stage, precision=FP64):
+        """One solver call: PREPARE, CHARGE, FORCES or VIRIALS."""
+        _check(lib().mdc_stage(self.h, pot.h, stage, precision))
 
     def decide(self, rcut):
         need = C.c_int()

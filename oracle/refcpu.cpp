@@ -1692,6 +1692,25 @@ int ref_stages(void* pot, void* pool, void* atoms, size_t n, const ref_domain* d
     return 0;
 }
 
+// pair.cxx:44-58 charge(), :72-87 forces() called on their own
+int ref_charge(void* pot, void* pool, void* atoms, size_t n, const ref_domain* d, void* nl, int boundary) {
+    if (check_list(nl, n)) return -1;
+    REF_TRY({
+        Domain dom(d->lo, d->hi, d->periodic);
+        make_engine(pot, pool, atoms, n, dom, nl, boundary).charge();
+    });
+    return 0;
+}
+
+int ref_forces(void* pot, void* pool, void* atoms, size_t n, const ref_domain* d, void* nl, int boundary) {
+    if (check_list(nl, n)) return -1;
+    REF_TRY({
+        Domain dom(d->lo, d->hi, d->periodic);
+        make_engine(pot, pool, atoms, n, dom, nl, boundary).forces();
+    });
+    return 0;
+}
+
 int ref_virials(void* pot, void* pool, void* atoms, size_t n, const ref_domain* d, void* nl, int boundary) {
     if (check_list(nl, n)) return -1;
     REF_TRY({

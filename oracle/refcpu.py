@@ -55,6 +55,8 @@ def _load():
         "ref_domain_op": (I, [dom, I, Z, I, Z]),
         "ref_prepare": (I, [P, P, P, S, dom, P, I]),
         "ref_stages": (I, [P, P, P, S, dom, P, I]),
+        "ref_charge": (I, [P, P, P, S, dom, P, I]),
+        "ref_forces": (I, [P, P, P, S, dom, P, I]),
         "ref_virials": (I, [P, P, P, S, dom, P, I]),
         "ref_make_step": (I, [P, P, P, S, dom, P, I, D]),
         "ref_measure_state": (None, [P, S, Z]),
@@ -240,6 +242,12 @@ class Solver:
 
     def stages(self, atoms, domain, nlist):
         self._call(lib.ref_stages, atoms, domain, nlist)
+
+    def charge(self, atoms, domain, nlist):
+        self._call(lib.ref_charge, atoms, domain, nlist)
+
+    def forces(self, atoms, domain, nlist):
+        self._call(lib.ref_forces, atoms, domain, nlist)
 
     def virials(self, atoms, domain, nlist):
         self._call(lib.ref_virials, atoms, domain, nlist)
