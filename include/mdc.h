@@ -146,6 +146,23 @@ mdc_status mdc_nlist_stats(mdc_sys* sys, double* mean_neighbours, int* max_neigh
 /* ---- force evaluation and integration ----------------------------------- */
 /* prepare + stages (+ virials when MDC_ENERGY | MDC_VIRIAL) */
 mdc_status mdc_compute(mdc_sys* sys, const mdc_pot* pot, uint32_t what, int precision);
+/* Langevin thermostats (solver/thermostat/langevin.cxx:7-42), applied one
+ * after the other as thermostat::Compound does, to every force evaluation
+ * of the system (mdc_compute, MDC_STAGE_FORCES, each step of mdc_step),
+ * after the potential's force (solver/pair.cxx:72-87): per atom whose x
+ * lies in [xmin, xmax],
+ *   f += ((sigma g) sqrt(m) - (v - U) beta m) * heat,
+ *   sigma = sqrt(2 beta kB T / time_step), kB = 0.008314462175 kJ/mol/K,
+ * with v the half-step velocity inside a step.  params holds 11 doubles
+ * per thermostat in the reference constructor's order: beta, temperature,
+ * time_step, heat_x, heat_y, heat_z, Ux, Uy, Uz, xmin, xmax.  count = 0
+ * detaches them.  g are normal deviates of a counter-based stream
+ * (Philox4x32-10 + Box-Muller) at (seed, atom index, application count,
+ * thermostat index): runs are reproducible for a seed, unlike the
+ * reference's std::random_device seeding.  A new seed restarts the count.
+ * Atoms with tag == 0 get no thermostat force (the reference adds one to
+ * their f, though they never move). */
+mdc_status mdc_sys_set_thermostat(mdc_sys* sys, int count, const double* params, uint64_t seed);
 /* One solver call of the reference's Python API on the resident state
  * (solver::Pair::{prepare,charge,forces,virials}, solver/pair.cxx:16-127;
  * solver::Single, single.cxx:38-83; solver::Ternary, ternary.cxx:19-67):

@@ -991,6 +991,36 @@ mdc_status mdc_sys_set_boundary(mdc_sys* sys, int periodic_boundary) {
     });
 }
 
+mdc_status mdc_sys_set_thermostat(mdc_sys* sys, int count, const double* params, uint64_t seed) {
+    return guard(sys ? sys->b.ctx : nullptr, [&] {
+        if (!sys || (count > 0 && !params)) throw Error(E_INVAL, "null argument");
+        if (count < 0 || count > kMaxTherm) throw Error(E_INVAL, "between 0 and 4 thermostats");
+        ThermArgs T{};
+        T.count = count;
+        T.seed = seed;
+        // thermostat/base.h:10
+        const double kb = 0.008314462175;
+        for (int k = 0; k < count; ++k) {
+            const double* q = params + 11 * k;
+            const double beta = q[0], temp = q[1], dt = q[2];
+            if (!(dt > 0) || !(temp >= 0) || !std::isfinite(beta)) throw Error(E_INVAL, "bad Langevin parameters");
+            LangevinP& L = T.t[k];
+            L.beta = beta;
+            // langevin.cxx:25
+            L.sigma = std::sqrt(2.0 * beta * kb * temp / dt);
+            for (int c = 0; c < 3; ++c) {
+                L.heat[c] = q[3 + c];
+                L.U[c] = q[6 + c];
+            }
+            L.xmin = q[9];
+            L.xmax = q[10];
+        }
+        Batch& b = sys->b;
+        if (seed != b.therm.seed) b.therm_calls = 0;
+        b.therm = T;
+    });
+}
+
 mdc_status mdc_stage(mdc_sys* sys, const mdc_pot* pot, int stage, int precision) {
     return guard(sys ? sys->b.ctx : nullptr, [&] {
         if (!sys || !pot) throw Error(E_INVAL, "null argument");

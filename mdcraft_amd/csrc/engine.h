@@ -53,6 +53,20 @@ struct Ctx {
     int cu_count = 0;
 };
 
+// Langevin thermostats (thermostat/langevin.cxx:7-42) applied one after
+// the other, as thermostat::Compound does
+constexpr int kMaxTherm = 4;
+struct LangevinP {
+    double beta, sigma, xmin, xmax;
+    double U[3], heat[3];
+};
+struct ThermArgs {
+    int count;
+    int pad;
+    uint64_t seed;
+    LangevinP t[kMaxTherm];
+};
+
 struct LJParams {
     double a, s, s6, lj1, lj2, factor, rc, rc2;
 };
@@ -142,6 +156,10 @@ struct Batch {
     // step sequences then never trigger one and choose images per step
     bool manual_rebuild = false;
     // precision of the BPNN descriptors and dE/dG of the last prepare stage
+    // attached thermostats and the number of applications so far (the
+    // counter of the random stream)
+    ThermArgs therm{};
+    uint64_t therm_calls = 0;
     int bp_prepared = -1;
     int bp_prepared_builds = -1;
     // steps between rebuilds (running estimate) and steps since the last one,

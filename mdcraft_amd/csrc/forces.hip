@@ -19,6 +19,7 @@
 #include <climits>
 #include <cmath>
 #include "kern.h"
+#include "rng.h"
 
 namespace mdc {
 
@@ -743,6 +744,86 @@ static void read_misc(Batch& b) {
     timer_drain(b);
 }
 
+// --------------------------------------------------------------------------
+// Langevin thermostat (thermostat/langevin.cxx:29-42, applied after the
+// atom's force in pair.cxx:72-87 and ternary.cxx:42-55)
+// --------------------------------------------------------------------------
+// f += ((sigma * g) sqrt(m) - (v - U) beta m) * heat for every thermostat
+// whose [xmin, xmax] holds the atom's x, with g three normal deviates of
+// the counter-based stream at (atom, call, thermostat).
+__device__ inline void langevin_add(const ThermArgs& T, uint64_t call, int atom, double x, const double v[3],
+                                    double m, double f[3]) {
+    const double sm = sqrt(m);
+    for (int k = 0; k < T.count; ++k) {
+        const LangevinP& L = T.t[k];
+        if (x < L.xmin || x > L.xmax) continue;
+        double g[3];
+        normal3(T.seed, call, (uint32_t)atom, k, g);
+        for (int c = 0; c < 3; ++c) {
+            double t = (L.sigma * g[c]) * sm;
+            t -= (v[c] - L.U[c]) * L.beta * m;
+            t *= L.heat[c];
+            f[c] += t;
+        }
+    }
+}
+
+// A force evaluation outside a step sequence: add the thermostat force to
+// the stored force of precision `prec`, with the stored velocity.
+__global__ void k_langevin_apply(const double4* __restrict__ pos, AtomArrays A, const int* __restrict__ orig,
+                                 ThermArgs T, uint64_t call, int prec) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = A.n;
+    if (i >= n) return;
+    const double4 pi = pos[i];
+    if (w_inactive(pi.w)) return;
+    double f[3], v[3];
+    for (int c = 0; c < 3; ++c) {
+        f[c] = prec == P_MIXED ? (double)A.f[(size_t)(F_FX + c) * n + i] : A.d[(size_t)(D_FX + c) * n + i];
+        v[c] = A.d[(size_t)(D_VX + c) * n + i];
+    }
+    langevin_add(T, call, orig[i], pi.x, v, A.d[(size_t)D_M * n + i], f);
+    for (int c = 0; c < 3; ++c) {
+        if (prec == P_MIXED) A.f[(size_t)(F_FX + c) * n + i] = (float)f[c];
+        else A.d[(size_t)(D_FX + c) * n + i] = f[c];
+    }
+}
+
+// Step t of a sequence with thermostats: the force pass stored the
+// potential force (MODE_FORCE_STEP); add the thermostat force at the
+// half-step velocity and the drifted position (verlet.cxx:71-85: the
+// solver's forces run between the drift and the closing kick), then the
+// closing kick and, unless last, the next opening kick and drift.
+template <typename R>
+__global__ void k_langevin_finish(const double4* __restrict__ pos, double4* __restrict__ pos_next, AtomArrays A,
+                                  StepArgs S, const int* __restrict__ orig, ThermArgs T, uint64_t call,
+                                  const SysDesc* __restrict__ desc, const double4* __restrict__ rsave,
+                                  int* __restrict__ misc) {
+    if (skip_step(S, misc)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = A.n;
+    if (i >= n) return;
+    const double4 pi = pos[i];
+    if (w_inactive(pi.w)) {
+        if (S.mode == MODE_FINAL_DRIFT) pos_next[i] = pi;
+        return;
+    }
+    const FinIn fin = load_fin(i, A, S, rsave, true);
+    double f[3];
+    for (int c = 0; c < 3; ++c)
+        f[c] = sizeof(R) == 4 ? (double)A.f[(size_t)(F_FX + c) * n + i] : A.d[(size_t)(D_FX + c) * n + i];
+    langevin_add(T, call, orig[i], pi.x, fin.v, fin.m, f);
+    finalize_atom<R>(i, pi, f[0], f[1], f[2], A, S, pos_next, fin, desc[w_sys(pi.w)], misc);
+}
+
+static void langevin_apply(Batch& b, int prec) {
+    if (b.therm.count == 0 || b.n == 0) return;
+    AtomArrays AA{b.dst[b.sc], b.fst[b.sc], b.n};
+    hipLaunchKernelGGL(k_langevin_apply, dim3(nblk(b.n, 256)), dim3(256), 0, b.ctx->stream, b.pos[b.cur], AA,
+                       b.orig[b.sc], b.therm, b.therm_calls, prec);
+    b.therm_calls++;
+}
+
 void force_compute(Batch& b, const Pot& p, int what, int prec) {
     if (b.n == 0) return;
     set_trigger(b, p.rcut, false);
@@ -756,6 +837,7 @@ void force_compute(Batch& b, const Pot& p, int what, int prec) {
     }
     if (!(what & W_VIRIAL)) HIPCHECK(hipMemsetAsync(b.d(b.sc, D_VXX), 0, sizeof(double) * 6 * b.n, b.ctx->stream));
     launch_step_kernels(b, p, S, prec, b.pos[b.cur], nullptr);
+    langevin_apply(b, prec);
     b.f_prec = prec;
     b.last_rcut = p.rcut;
     read_misc(b);
@@ -866,6 +948,7 @@ void stage_compute(Batch& b, const Pot& p, int stage, int prec) {
             }
             hipLaunchKernelGGL(k_force_merge, dim3(grid), dim3(256), 0, st, pos, AA, b.dst[idle], b.fst[idle],
                                was_prec, prec);
+            langevin_apply(b, prec);
             b.f_prec = prec;
             break;
         }
@@ -921,7 +1004,17 @@ void run_steps(Batch& b, const Pot& p, double dt, int nsteps, int prec, int chun
         for (int t = s; t < e; ++t) {
             const bool last = t == nsteps - 1;
             StepArgs S{dt, t, last ? MODE_FINAL : MODE_FINAL_DRIFT, 0, last ? 1 : 0};
-            launch_step_kernels(b, p, S, prec, b.pos[buf(t)], last ? nullptr : b.pos[buf(t) ^ 1]);
+            if (b.therm.count == 0) {
+                launch_step_kernels(b, p, S, prec, b.pos[buf(t)], last ? nullptr : b.pos[buf(t) ^ 1]);
+                continue;
+            }
+            StepArgs SF = S;
+            SF.mode = MODE_FORCE_STEP;
+            launch_step_kernels(b, p, SF, prec, b.pos[buf(t)], nullptr);
+            AtomArrays AA{b.dst[b.sc], b.fst[b.sc], b.n};
+            auto fin = prec == P_MIXED ? k_langevin_finish<float> : k_langevin_finish<double>;
+            hipLaunchKernelGGL(fin, dim3(grid), dim3(256), 0, st, b.pos[buf(t)], last ? nullptr : b.pos[buf(t) ^ 1],
+                               AA, S, b.orig[b.sc], b.therm, b.therm_calls + (uint64_t)t, b.d_desc, b.rsave, b.misc);
         }
         read_misc(b);
         check_errors(b);
@@ -943,6 +1036,7 @@ void run_steps(Batch& b, const Pot& p, double dt, int nsteps, int prec, int chun
         }
     }
     b.cur = buf(nsteps - 1);
+    if (b.therm.count) b.therm_calls += (uint64_t)nsteps;
     // after make_step the reference holds Ep = Em = 0 and V = 0 (prepare
     // zeroes them; only virials() fills them); BPNN sets Ep in prepare
     if (p.type != PT_BPNN) {

@@ -5,7 +5,9 @@
 
 namespace mdc {
 
-enum Mode { MODE_COMPUTE = 0, MODE_FINAL = 1, MODE_FINAL_DRIFT = 2 };
+// MODE_FORCE_STEP: a force pass inside a step sequence that only stores
+// the force; a separate kernel adds the thermostat force and integrates
+enum Mode { MODE_COMPUTE = 0, MODE_FINAL = 1, MODE_FINAL_DRIFT = 2, MODE_FORCE_STEP = 3 };
 
 __device__ inline float rcp_(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ inline double rcp_(double x) { return 1.0 / x; }
@@ -90,7 +92,7 @@ struct FinIn {
 __device__ inline FinIn load_fin(int i, const AtomArrays& A, const StepArgs& S, const double4* __restrict__ rsave,
                                  bool need) {
     FinIn f;
-    if (need && S.mode != MODE_COMPUTE) {
+    if (need && (S.mode == MODE_FINAL || S.mode == MODE_FINAL_DRIFT)) {
         const int n = A.n;
         f.v[0] = A.d[(size_t)D_VX * n + i];
         f.v[1] = A.d[(size_t)D_VY * n + i];
@@ -124,7 +126,7 @@ __device__ inline void finalize_atom(int i, const double4& pi, double fx, double
         A.d[(size_t)D_FY * n + i] = fy;
         A.d[(size_t)D_FZ * n + i] = fz;
     }
-    if (S.mode == MODE_COMPUTE) return;
+    if (S.mode != MODE_FINAL && S.mode != MODE_FINAL_DRIFT) return;
     const double m = in.m;
     double v[3] = {in.v[0], in.v[1], in.v[2]};
     const double F[3] = {fx, fy, fz};

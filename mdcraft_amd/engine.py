@@ -32,7 +32,7 @@ EXPORTS = [
     "mdc_pot_lj_create", "mdc_pot_eam_create", "mdc_pot_eam_create_file", "mdc_pot_multi_create",
     "mdc_pot_bpnn_create", "mdc_pot_rcut", "mdc_pot_destroy",
     "mdc_sys_create", "mdc_sys_destroy", "mdc_sys_upload", "mdc_sys_download", "mdc_sys_download_range", "mdc_sys_set_active", "mdc_sys_natoms",
-    "mdc_sys_set_lanes", "mdc_sys_autotune", "mdc_sys_time_kernels", "mdc_sys_set_rebuild", "mdc_sys_set_boundary",
+    "mdc_sys_set_lanes", "mdc_sys_autotune", "mdc_sys_time_kernels", "mdc_sys_set_rebuild", "mdc_sys_set_boundary", "mdc_sys_set_thermostat",
     "mdc_nlist_build", "mdc_nlist_decide", "mdc_nlist_optimize_rns", "mdc_nlist_export", "mdc_nlist_stats",
     "mdc_compute", "mdc_stage", "mdc_step", "mdc_thermo", "mdc_sync", "mdc_timing", "mdc_bpnn_inspect",
 ]
@@ -72,6 +72,7 @@ def load_library(path=LIB_PATH):
         "mdc_sys_time_kernels": (I, [P, P, I, D, I, DP]),
         "mdc_sys_set_rebuild": (I, [P, I]),
         "mdc_sys_set_boundary": (I, [P, I]),
+        "mdc_sys_set_thermostat": (I, [P, I, DP, U64]),
         "mdc_nlist_build": (I, [P]),
         "mdc_nlist_optimize_rns": (I, [P, D, DP]),
         "mdc_stage": (I, [P, P, I, I]),
@@ -283,6 +284,13 @@ class System:
 
     def set_boundary(self, periodic_boundary):
         _check(lib().mdc_sys_set_boundary(self.h, int(bool(periodic_boundary))))
+
+    def set_thermostat(self, langevins, seed=0):
+        """langevins: list of (beta, temperature, time_step, heat_x, heat_y,
+        heat_z, Ux, Uy, Uz, xmin, xmax); [] detaches."""
+        q = np.ascontiguousarray(np.asarray(langevins, dtype=np.float64).reshape(-1))
+        _check(lib().mdc_sys_set_thermostat(self.h, len(langevins), _dp(q) if len(langevins) else None,
+                                            int(seed) & (2 ** 64 - 1)))
 
     def stage(self, pot, stage, precision=FP64):
         """One solver call: PREPARE, CHARGE, FORCES or VIRIALS."""

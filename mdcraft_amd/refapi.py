@@ -28,7 +28,11 @@ Differences from the reference, each with the reason:
 * the half list of ``VerletList(half=True)`` is exported as the full
   list's upper half (j >= i); forces are identical either way;
 * potentials outside the engine (``LJs``, tabulated ``Pair``,
-  ``DeepModelingPotential``, ``MLIP4``) raise NotImplementedError.
+  ``DeepModelingPotential``, ``MLIP4``) raise NotImplementedError;
+* ``Langevin`` draws from a counter-based stream keyed by ``seed`` (an
+  added keyword; random when omitted) instead of std::random_device, one
+  stream per list's system; the members of a ``Compound`` share the first
+  member's seed; atoms with tag == 0 get no thermostat force.
 """
 import os
 
@@ -519,7 +523,10 @@ class Langevin(Thermostat):
     Uy, Uz, xmin, xmax) (thermostat/langevin.cxx:7-42)."""
 
     def __init__(self, beta, temperature, time_step, heat_x=1, heat_y=1, heat_z=1, Ux=0.0, Uy=0.0, Uz=0.0,
-                 xmin=-1e15, xmax=1e15):
+                 xmin=-1e15, xmax=1e15, seed=None):
+        # the reference seeds from std::random_device; ``seed`` (an
+        # addition) makes a run reproducible
+        self.seed = int.from_bytes(os.urandom(8), "little") if seed is None else int(seed)
         self._beta, self._T, self._dt = float(beta), float(temperature), float(time_step)
         self._heat = [int(heat_x), int(heat_y), int(heat_z)]
         self._U = [float(Ux), float(Uy), float(Uz)]
@@ -547,6 +554,10 @@ class Langevin(Thermostat):
 
     def set_dimensions(self, xmin=-1e15, xmax=1e15):
         self._xmin, self._xmax = float(xmin), float(xmax)
+
+    def row(self):
+        """The engine's parameter row (langevin.h:16-28 order)."""
+        return (self._beta, self._T, self._dt, *self._heat, *self._U, self._xmin, self._xmax)
 
 
 class Compound(Thermostat):
@@ -599,8 +610,12 @@ class _EngineSolver(ISolver):
             raise RuntimeError("the neighbour list is not built for these atoms: call update()")
         s = vl.system
         s.set_boundary(self.periodic)
-        if _langevins(self.thermostat):
-            raise NotImplementedError("thermostats are not implemented by the GPU engine")
+        th = _langevins(self.thermostat)
+        key = (tuple(t.row() for t in th), th[0].seed if th else getattr(s, "_therm_seed", 0))
+        if getattr(s, "_therm_key", None) != key:
+            # one stream per system; a Compound's members take its first seed
+            s.set_thermostat([t.row() for t in th], key[1])
+            s._therm_key, s._therm_seed = key, key[1]
         return s
 
     def _stage(self, atoms, neibs, nlist, stage, fields):

@@ -1237,6 +1237,86 @@ struct BPNN : Potential {
 };
 
 // ---------------------------------------------------------------------------
+// Langevin thermostat (solver/thermostat/langevin.cxx:7-42) and Compound
+// (compound.cxx).  The reference draws from std::default_random_engine
+// seeded by std::random_device, which no test can reproduce; the oracle
+// draws from Philox4x32-10 (Salmon et al., SC'11) at (seed, atom index,
+// application count, thermostat index) with Box-Muller normals, the same
+// stream the engine uses, so trajectories can be compared.
+// ---------------------------------------------------------------------------
+struct Philox {
+    static void block(const uint32_t ctr[4], uint32_t k0, uint32_t k1, uint32_t out[4]) {
+        uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+        for (int round = 0; round < 10; ++round) {
+            if (round) {
+                k0 += 0x9E3779B9u;
+                k1 += 0xBB67AE85u;
+            }
+            const uint64_t a = (uint64_t)0xD2511F53u * c0, b = (uint64_t)0xCD9E8D57u * c2;
+            const uint32_t n0 = (uint32_t)(b >> 32) ^ c1 ^ k0;
+            const uint32_t n2 = (uint32_t)(a >> 32) ^ c3 ^ k1;
+            c1 = (uint32_t)b;
+            c3 = (uint32_t)a;
+            c0 = n0;
+            c2 = n2;
+        }
+        out[0] = c0;
+        out[1] = c1;
+        out[2] = c2;
+        out[3] = c3;
+    }
+    static double uniform(uint32_t hi, uint32_t lo) {
+        uint64_t w = ((uint64_t)hi << 32) | lo;
+        return (double)((w >> 11) + 1) / 9007199254740992.0;
+    }
+    // three normal deviates: two Box-Muller pairs from counters (atom,
+    // call lo, call hi, 2k) and (..., 2k + 1), the fourth deviate unused
+    static void normals(uint64_t seed, uint64_t call, uint32_t atom, int k, double g[3]) {
+        double d[4];
+        for (int q = 0; q < 2; ++q) {
+            const uint32_t ctr[4] = {atom, (uint32_t)call, (uint32_t)(call >> 32), (uint32_t)(2 * k + q)};
+            uint32_t x[4];
+            block(ctr, (uint32_t)seed, (uint32_t)(seed >> 32), x);
+            const double u1 = uniform(x[0], x[1]), u2 = uniform(x[2], x[3]);
+            const double rad = std::sqrt(-2.0 * std::log(u1)), ang = 6.283185307179586 * u2;
+            d[2 * q] = rad * std::cos(ang);
+            d[2 * q + 1] = rad * std::sin(ang);
+        }
+        g[0] = d[0];
+        g[1] = d[1];
+        g[2] = d[2];
+    }
+};
+
+struct Langevin {
+    double beta, T, dt, sigma, xmin, xmax;
+    double heat[3], U[3];
+};
+
+struct Thermostats {
+    std::vector<Langevin> list;
+    uint64_t seed = 0, calls = 0;
+    // langevin.cxx:29-42 for each member in order (compound.cxx)
+    void apply_one(Atom& a, size_t index) const {
+        for (size_t k = 0; k < list.size(); ++k) {
+            const Langevin& L = list[k];
+            if (a.r[0] < L.xmin || a.r[0] > L.xmax) continue;
+            double g[3];
+            Philox::normals(seed, calls, (uint32_t)index, (int)k, g);
+            const double sm = std::sqrt(a.m);
+            for (int c = 0; c < 3; ++c) {
+                double t = (L.sigma * g[c]) * sm;
+                t -= (a.v[c] - L.U[c]) * L.beta * a.m;
+                t *= L.heat[c];
+                a.f[c] += t;
+            }
+        }
+    }
+};
+
+static Thermostats* g_thermo = nullptr;  // thermostats of the next solver calls
+
+// ---------------------------------------------------------------------------
 // Solvers: Single/Pair with the Periodic ghost pass (solver/pair.cxx:16-127,
 // solver/single.cxx:11-83, solver/boundary/periodic.cxx:9-102) and Ternary
 // (solver/ternary.cxx:19-67).
@@ -1249,6 +1329,7 @@ struct Engine {
     const Domain* dom;
     VerletList* vl;
     bool periodic_bc;  // solver built with boundary::Periodic(domain)
+    Thermostats* thermo = nullptr;
 
     // periodic.cxx:36-60
     void ghosts_of(size_t i, std::vector<Atom>& g) {
@@ -1335,22 +1416,27 @@ struct Engine {
         });
     }
 
+    // pair.cxx:72-87 and ternary.cxx:42-55: the thermostat follows each
+    // atom's own force, for every atom
     void forces() {
         if (pot->type == PT_TERNARY) {
             auto* b = static_cast<BPNN*>(pot);
             pool->for_n(n, [&](size_t i) {
                 if (atoms[i].tag != 0) b->force_one(i, atoms[i]);
+                if (thermo) thermo->apply_one(atoms[i], i);
             });
-            return;
+        } else {
+            pool->for_n(n, [&](size_t i) {
+                Atom& a = atoms[i];
+                if (a.tag != 0) pot->force(a, atoms, lp(i), ls(i));
+                if (thermo) thermo->apply_one(a, i);
+                if (a.bc)
+                    with_ghosts(i, [&](Atom* g, const size_t* idx, size_t ng) {
+                        if (a.tag != 0) pot->force(a, g, idx, ng);
+                    });
+            });
         }
-        pool->for_n(n, [&](size_t i) {
-            Atom& a = atoms[i];
-            if (a.tag != 0) pot->force(a, atoms, lp(i), ls(i));
-            if (a.bc)
-                with_ghosts(i, [&](Atom* g, const size_t* idx, size_t ng) {
-                    if (a.tag != 0) pot->force(a, g, idx, ng);
-                });
-        });
+        if (thermo) thermo->calls++;
     }
 
     void virials() {
@@ -1663,8 +1749,44 @@ static Engine make_engine(void* pot, void* pool, void* atoms, size_t n, const Do
     e.vl = &static_cast<ref_nlist*>(nl)->vl;
     e.periodic_bc = boundary != 0;
     e.pot->half = e.vl->half;
+    e.thermo = g_thermo && !g_thermo->list.empty() ? g_thermo : nullptr;
     return e;
 }
+
+// Thermostats for the following solver calls: count Langevin members with
+// 11 parameters each (beta, temperature, time_step, heat_x, heat_y,
+// heat_z, Ux, Uy, Uz, xmin, xmax; langevin.h:16-28); the handle keeps the
+// application count.  ref_thermo_use(NULL) detaches.
+void* ref_thermo_create(int count, const double* params, uint64_t seed) {
+    auto* t = new Thermostats();
+    t->seed = seed;
+    const double kb = 0.008314462175;  // thermostat/base.h:10
+    for (int k = 0; k < count; ++k) {
+        const double* q = params + 11 * k;
+        Langevin L;
+        L.beta = q[0];
+        L.T = q[1];
+        L.dt = q[2];
+        L.sigma = std::sqrt(2.0 * L.beta * kb * L.T / L.dt);  // langevin.cxx:25
+        for (int c = 0; c < 3; ++c) {
+            L.heat[c] = q[3 + c];
+            L.U[c] = q[6 + c];
+        }
+        L.xmin = q[9];
+        L.xmax = q[10];
+        t->list.push_back(L);
+    }
+    return t;
+}
+void ref_thermo_free(void* t) {
+    if (g_thermo == t) g_thermo = nullptr;
+    delete static_cast<Thermostats*>(t);
+}
+void ref_thermo_use(void* t) { g_thermo = static_cast<Thermostats*>(t); }
+uint64_t ref_thermo_calls(void* t) { return static_cast<Thermostats*>(t)->calls; }
+// Philox4x32-10 block and the normals of one atom, for known-answer tests
+void ref_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) { Philox::block(ctr, key[0], key[1], out); }
+void ref_normals(uint64_t seed, uint64_t call, uint32_t atom, int k, double* g) { Philox::normals(seed, call, atom, k, g); }
 
 static int check_list(void* nl, size_t n) {
     if (static_cast<ref_nlist*>(nl)->vl.nlist.size() != n) {

@@ -55,6 +55,12 @@ def _load():
         "ref_domain_op": (I, [dom, I, Z, I, Z]),
         "ref_prepare": (I, [P, P, P, S, dom, P, I]),
         "ref_stages": (I, [P, P, P, S, dom, P, I]),
+        "ref_thermo_create": (P, [I, Z, C.c_uint64]),
+        "ref_thermo_free": (None, [P]),
+        "ref_thermo_use": (None, [P]),
+        "ref_thermo_calls": (C.c_uint64, [P]),
+        "ref_philox": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+        "ref_normals": (None, [C.c_uint64, C.c_uint64, C.c_uint32, I, Z]),
         "ref_charge": (I, [P, P, P, S, dom, P, I]),
         "ref_forces": (I, [P, P, P, S, dom, P, I]),
         "ref_virials": (I, [P, P, P, S, dom, P, I]),
@@ -226,16 +232,56 @@ class VerletList:
         return [ids[off[i]:off[i + 1]] for i in range(n)]
 
 
+class Thermostats:
+    """Langevin thermostats applied in order (thermostat::Langevin,
+    thermostat::Compound); rows of (beta, temperature, time_step, heat_x,
+    heat_y, heat_z, Ux, Uy, Uz, xmin, xmax) and the stream seed."""
+
+    def __init__(self, rows, seed):
+        q = np.ascontiguousarray(np.asarray(rows, dtype=np.float64).reshape(-1))
+        self._q = q
+        self.h = lib.ref_thermo_create(len(rows), _dp(q), int(seed) & (2 ** 64 - 1))
+
+    @property
+    def calls(self):
+        return lib.ref_thermo_calls(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.ref_thermo_free(self.h)
+            self.h = None
+
+
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    out = (C.c_uint32 * 4)()
+    lib.ref_philox(c, k, out)
+    return list(out)
+
+
+def normals(seed, call, atom, k):
+    g = np.zeros(3)
+    lib.ref_normals(seed, call, atom, k, _dp(g))
+    return g
+
+
 class Solver:
     """Single / Ternary solver bound to one potential (solver::Single,
-    solver::Ternary); ``periodic_boundary`` attaches boundary::Periodic."""
+    solver::Ternary); ``periodic_boundary`` attaches boundary::Periodic,
+    ``thermostats`` a Thermostats object."""
 
-    def __init__(self, pot, pool, periodic_boundary=True):
+    def __init__(self, pot, pool, periodic_boundary=True, thermostats=None):
         self.pot, self.pool, self.bc = pot, pool, int(periodic_boundary)
+        self.thermostats = thermostats
 
     def _call(self, fn, atoms, domain, nlist, *extra):
-        if fn(self.pot.h, self.pool.h, atoms.ctypes.data, len(atoms), domain.ref(), nlist.h, self.bc, *extra):
-            raise RuntimeError(_err())
+        lib.ref_thermo_use(self.thermostats.h if self.thermostats is not None else None)
+        try:
+            if fn(self.pot.h, self.pool.h, atoms.ctypes.data, len(atoms), domain.ref(), nlist.h, self.bc, *extra):
+                raise RuntimeError(_err())
+        finally:
+            lib.ref_thermo_use(None)
 
     def prepare(self, atoms, domain, nlist):
         self._call(lib.ref_prepare, atoms, domain, nlist)
@@ -267,8 +313,12 @@ class Solver:
         return hist
 
     def run_nve(self, atoms, domain, nlist, dt, steps, initial_forces=True):
-        r = lib.ref_run_nve(self.pot.h, self.pool.h, atoms.ctypes.data, len(atoms), domain.ref(), nlist.h, self.bc,
-                            dt, steps, int(initial_forces))
+        lib.ref_thermo_use(self.thermostats.h if self.thermostats is not None else None)
+        try:
+            r = lib.ref_run_nve(self.pot.h, self.pool.h, atoms.ctypes.data, len(atoms), domain.ref(), nlist.h,
+                                self.bc, dt, steps, int(initial_forces))
+        finally:
+            lib.ref_thermo_use(None)
         if r < 0:
             raise RuntimeError(_err())
         return r
