@@ -398,6 +398,8 @@ void layout_cells(Batch& b) {
         dfree(b.cell_fill);
         b.ncells = off;
         b.cell_count = dalloc<int>(off + 1);
+        // zero once: each rebuild's scan clears the counts it consumed
+        HIPCHECK(hipMemsetAsync(b.cell_count, 0, sizeof(int) * (off + 1), b.ctx->stream));
         b.cell_start = dalloc<int>(off + 1);
         b.cell_fill = dalloc<int>(std::max(off + 1, b.n));
     }
@@ -678,9 +680,10 @@ mdc_status mdc_ctx_create(int device, mdc_ctx** out) {
         c->c.device = device;
         HIPCHECK(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));
         void* pin = nullptr;
-        HIPCHECK(hipHostMalloc(&pin, sizeof(int) * M_COUNT, hipHostMallocDefault));
+        HIPCHECK(hipHostMalloc(&pin, sizeof(int) * 3 * M_COUNT, hipHostMallocDefault));
         c->c.h_pin = static_cast<int*>(pin);
-        std::memset(c->c.h_pin, 0, sizeof(int) * M_COUNT);
+        std::memset(c->c.h_pin, 0, sizeof(int) * 3 * M_COUNT);
+        for (auto& ev : c->c.chunk_ev) HIPCHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
         hipDeviceProp_t prop;
         HIPCHECK(hipGetDeviceProperties(&prop, device));
         c->c.cu_count = prop.multiProcessorCount;
@@ -693,6 +696,8 @@ mdc_status mdc_ctx_destroy(mdc_ctx* ctx) {
     (void)hipSetDevice(ctx->c.device);
     if (ctx->c.stream) (void)hipStreamDestroy(ctx->c.stream);
     if (ctx->c.h_pin) (void)hipHostFree(ctx->c.h_pin);
+    for (auto ev : ctx->c.chunk_ev)
+        if (ev) (void)hipEventDestroy(ev);
     delete ctx;
     return 0;
 }

@@ -49,7 +49,11 @@ struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
-    int* h_pin = nullptr;  // pinned host mirror of the misc counters
+    // pinned host mirrors of the misc counters: slot 0 for synchronous
+    // reads, slots 1 and 2 for the two step chunks kept in flight, each
+    // with the event that marks its copy done
+    int* h_pin = nullptr;
+    hipEvent_t chunk_ev[2] = {nullptr, nullptr};
     int cu_count = 0;
 };
 
@@ -234,7 +238,7 @@ void time_step_kernels(Batch& b, const Pot& p, int prec, double dt, int reps, do
 void thermo(Batch& b, double* out);
 void set_trigger(Batch& b, double rcut, bool allow_shift);
 void stage_compute(Batch& b, const Pot& p, int stage, int prec);
-void check_errors(Batch& b);
+void check_errors(Batch& b, const int* counters = nullptr);
 // launchers (bpnn.hip)
 // parts: 1 = descriptors and network (G, dE/dG, Ep), 2 = forces from the
 // stored dE/dG, 3 = both

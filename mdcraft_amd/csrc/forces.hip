@@ -30,7 +30,7 @@ struct LJK {
     double a, s6, lj1, lj2, factor, rc2;
 };
 
-template <typename R, int TPA>
+template <typename R, int TPA, bool EV>
 __global__ void __launch_bounds__(256) k_lj(const double4* __restrict__ pos, double4* __restrict__ pos_next,
                                             const int* __restrict__ nl, const int* __restrict__ nn, int M, int cap,
                                             const SysDesc* __restrict__ desc, LJK P, AtomArrays A, StepArgs S,
@@ -63,11 +63,11 @@ __global__ void __launch_bounds__(256) k_lj(const double4* __restrict__ pos, dou
         fx += rx * pref;
         fy += ry * pref;
         fz += rz * pref;
-        if (S.flags & W_ENERGY) {
+        if (EV && (S.flags & W_ENERGY)) {
             const R t = c_s6 * r6inv;
             e += (double)(c_e * t * (t - (R)1));
         }
-        if (S.flags & W_VIRIAL) {
+        if (EV && (S.flags & W_VIRIAL)) {
             const double px = (double)(pref * rx), py = (double)(pref * ry), pz = (double)(pref * rz);
             vxx += px * (double)rx;
             vyy += py * (double)ry;
@@ -80,8 +80,8 @@ __global__ void __launch_bounds__(256) k_lj(const double4* __restrict__ pos, dou
     fx = wave_sum_group(fx, TPA);
     fy = wave_sum_group(fy, TPA);
     fz = wave_sum_group(fz, TPA);
-    if (S.flags & W_ENERGY) e = wave_sum_group(e, TPA);
-    if (S.flags & W_VIRIAL) {
+    if (EV && (S.flags & W_ENERGY)) e = wave_sum_group(e, TPA);
+    if (EV && (S.flags & W_VIRIAL)) {
         vxx = wave_sum_group(vxx, TPA);
         vyy = wave_sum_group(vyy, TPA);
         vzz = wave_sum_group(vzz, TPA);
@@ -95,8 +95,8 @@ __global__ void __launch_bounds__(256) k_lj(const double4* __restrict__ pos, dou
         return;
     }
     const int n = A.n;
-    if (S.flags & W_ENERGY) A.d[(size_t)D_EP * n + i] = e;
-    if (S.flags & W_VIRIAL) {
+    if (EV && (S.flags & W_ENERGY)) A.d[(size_t)D_EP * n + i] = e;
+    if (EV && (S.flags & W_VIRIAL)) {
         A.d[(size_t)D_VXX * n + i] = 0.5 * vxx;
         A.d[(size_t)D_VYY * n + i] = 0.5 * vyy;
         A.d[(size_t)D_VZZ * n + i] = 0.5 * vzz;
@@ -161,14 +161,20 @@ __device__ inline void embed_at(const double4* __restrict__ F64, int off, int nr
     dF = c.y + 2.0 * DV * (1.5 * aa - bb);
 }
 
-// Pairs of one lane are processed U at a time in predicated phases
-// (distances, then every table and F' read, then arithmetic), so the U
-// dependent table reads are in flight together.
-constexpr int kU = 4;
+// Pairs of one lane are processed kU at a time in predicated phases
+// (distances, then every table and F' read, then arithmetic), so the kU
+// dependent table reads are in flight together.  Two per group measured
+// fastest at both precisions: four keep too many registers live (the FP64
+// force kernel spills) and leave fewer waves to hide the gathers.
+#ifndef MDC_KU
+#define MDC_KU 2
+#endif
+constexpr int kU = MDC_KU;
 
 template <typename R>
 struct PairGeo {
-    double dx, dy, dz;
+    double dx, dy, dz;  // FP64 difference (virial); rx, ry, rz: the same in R
+    R rx, ry, rz;
     R R1, RN, R2;
     int N1;
     bool ok;
@@ -185,6 +191,9 @@ __device__ inline PairGeo<R> pair_geo(const Frame& s, const double4& pi, const d
     // distance reaches max(rns) and its shortest image does not exceed it
     g.ghost = img && s.any_per && sq3_exact(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z) >= s.rns2;
     g.ok = img && !(d2 > rc2 || d2 < 1e-15);
+    g.rx = (R)g.dx;
+    g.ry = (R)g.dy;
+    g.rz = (R)g.dz;
     g.R2 = g.ok ? (R)d2 : (R)1;
     g.R1 = sqrt(g.R2);
     g.RN = g.R1 * dr1;
@@ -244,6 +253,9 @@ __global__ void __launch_bounds__(256) k_eam_rho(const double4* __restrict__ pos
     double rho = 0.0;
     const __amdgpu_buffer_rsrc_t posb = make_buf(pos, (unsigned)A.n * 32u);
     const __amdgpu_buffer_rsrc_t tabb = make_buf(tab, P.nrec * (unsigned)sizeof(*tab));
+    int jn[kU];  // list entries of the next group, read one group ahead
+#pragma unroll
+    for (int u = 0; u < kU; ++u) jn[u] = j0[u];
     for (int m0 = 0; m0 * TPA + as.sub < cnt; m0 += kU) {
         int e[kU];
         bool okk[kU];
@@ -251,11 +263,12 @@ __global__ void __launch_bounds__(256) k_eam_rho(const double4* __restrict__ pos
         for (int u = 0; u < kU; ++u) {
             const int k = (m0 + u) * TPA + as.sub;
             okk[u] = k < cnt;
-            e[u] = okk[u] ? (m0 == 0 ? j0[u] : nl[as.base + (long long)(m0 + u) * kWave]) : 0;
+            e[u] = okk[u] ? (MDC_PREFETCH || m0 == 0 ? jn[u] : nl[as.base + (long long)(m0 + u) * kWave]) : 0;
         }
         double4 pj[kU];
 #pragma unroll
         for (int u = 0; u < kU; ++u) pj[u] = buf_d4(posb, (unsigned)ent_j(e[u]) * 32u);
+        next_group<TPA, kU>(as, m0, M, nl, jn);
         PairGeo<R> g[kU];
         int ob[kU];
 #pragma unroll
@@ -320,7 +333,7 @@ __global__ void __launch_bounds__(256) k_eam_rho(const double4* __restrict__ pos
 // (EAM.cxx:306-354); the velocity-Verlet finish is fused.  Multi: each pair
 // uses the object of (kind_i, kind_j): its rho'(r), its pair term and
 // F'(nc_i) + F'(nc_j) under its own embedding table.
-template <typename R, int TPA, bool MULTI>
+template <typename R, int TPA, bool MULTI, bool EV>
 __global__ void __launch_bounds__(256) k_eam_force(const double4* __restrict__ pos, double4* __restrict__ pos_next,
                                                    const int* __restrict__ nl, const int* __restrict__ nn, int M,
                                                    int cap, const SysDesc* __restrict__ desc, EamK P, AtomArrays A,
@@ -356,6 +369,9 @@ __global__ void __launch_bounds__(256) k_eam_force(const double4* __restrict__ p
     const __amdgpu_buffer_rsrc_t fpb = make_buf(Fp, (unsigned)(MULTI ? P.nk : 1) * (unsigned)n * 8u);
     const __amdgpu_buffer_rsrc_t tabb = make_buf(tab, P.nrec * (unsigned)sizeof(*tab));
     const unsigned fp_base = (unsigned)ki * (unsigned)n;  // F' of neighbours under the objects of (k_j, kind_i)
+    int jn[kU];  // list entries of the next group, read one group ahead
+#pragma unroll
+    for (int u = 0; u < kU; ++u) jn[u] = j0[u];
     for (int m0 = 0; m0 * TPA + as.sub < cnt; m0 += kU) {
         int e[kU];
         bool okk[kU];
@@ -363,7 +379,7 @@ __global__ void __launch_bounds__(256) k_eam_force(const double4* __restrict__ p
         for (int u = 0; u < kU; ++u) {
             const int k = (m0 + u) * TPA + as.sub;
             okk[u] = k < cnt;
-            e[u] = okk[u] ? (m0 == 0 ? j0[u] : nl[as.base + (long long)(m0 + u) * kWave]) : 0;
+            e[u] = okk[u] ? (MDC_PREFETCH || m0 == 0 ? jn[u] : nl[as.base + (long long)(m0 + u) * kWave]) : 0;
         }
         double4 pj[kU];
         R Fpj[kU];
@@ -372,6 +388,7 @@ __global__ void __launch_bounds__(256) k_eam_force(const double4* __restrict__ p
             pj[u] = buf_d4(posb, (unsigned)ent_j(e[u]) * 32u);
             Fpj[u] = (R)buf_d1(fpb, (fp_base + (unsigned)ent_j(e[u])) * 8u);
         }
+        next_group<TPA, kU>(as, m0, M, nl, jn);
         PairGeo<R> g[kU];
         int ob[kU], kj[kU];
 #pragma unroll
@@ -420,12 +437,12 @@ __global__ void __launch_bounds__(256) k_eam_force(const double4* __restrict__ p
             const R i1 = rcp_(R1), i2 = rcp_(R2);
             const R AdF = (R)(active ? Fpi : 0.0) + Fpj[u];
             const R FF = (AdF * nd + (phd * R1 - phv) * i2) * i1;
-            if (S.flags & W_ENERGY) en += 0.5 * (double)(phv * i1);
-            const R rx = (R)g[u].dx, ry = (R)g[u].dy, rz = (R)g[u].dz;
+            if (EV && (S.flags & W_ENERGY)) en += 0.5 * (double)(phv * i1);
+            const R rx = g[u].rx, ry = g[u].ry, rz = g[u].rz;
             fx += FF * rx;
             fy += FF * ry;
             fz += FF * rz;
-            if (S.flags & W_VIRIAL) {
+            if (EV && (S.flags & W_VIRIAL)) {
                 const double h = 0.5 * (double)FF;
                 const double ax = h * g[u].dx, ay = h * g[u].dy, az = h * g[u].dz;
                 vxx += ax * g[u].dx;
@@ -440,8 +457,8 @@ __global__ void __launch_bounds__(256) k_eam_force(const double4* __restrict__ p
     fx = wave_sum_group(fx, TPA);
     fy = wave_sum_group(fy, TPA);
     fz = wave_sum_group(fz, TPA);
-    if (S.flags & W_ENERGY) en = wave_sum_group(en, TPA);
-    if (S.flags & W_VIRIAL) {
+    if (EV && (S.flags & W_ENERGY)) en = wave_sum_group(en, TPA);
+    if (EV && (S.flags & W_VIRIAL)) {
         vxx = wave_sum_group(vxx, TPA);
         vyy = wave_sum_group(vyy, TPA);
         vzz = wave_sum_group(vzz, TPA);
@@ -454,8 +471,8 @@ __global__ void __launch_bounds__(256) k_eam_force(const double4* __restrict__ p
         if (S.mode == MODE_FINAL_DRIFT) pos_next[i] = pi;
         return;
     }
-    if (S.flags & W_ENERGY) A.d[(size_t)D_EP * n + i] = en;
-    if (S.flags & W_VIRIAL) {
+    if (EV && (S.flags & W_ENERGY)) A.d[(size_t)D_EP * n + i] = en;
+    if (EV && (S.flags & W_VIRIAL)) {
         A.d[(size_t)D_VXX * n + i] = vxx;
         A.d[(size_t)D_VYY * n + i] = vyy;
         A.d[(size_t)D_VZZ * n + i] = vzz;
@@ -683,7 +700,10 @@ static void launch_pair_t(Batch& b, const Pot& p, const StepArgs& S, const doubl
     AtomArrays AA{b.dst[b.sc], b.fst[b.sc], b.n};
     if (p.type == PT_LJ) {
         if (!(which & 2)) return;
-        launch_k(b, K_FORCE, k_lj<R, TPA>, dim3(grid), dim3(256), 0, pos, pos_next, b.nl, b.nn, b.M, b.cap, b.d_desc,
+        // energy and virial code only in the variant that computes them: the
+        // step kernels keep fewer registers live
+        auto k = S.flags & (W_ENERGY | W_VIRIAL) ? k_lj<R, TPA, true> : k_lj<R, TPA, false>;
+        launch_k(b, K_FORCE, k, dim3(grid), dim3(256), 0, pos, pos_next, b.nl, b.nn, b.M, b.cap, b.d_desc,
                  lj_args(p), AA, S, b.rsave, b.misc);
     } else {
         EamK E = eam_args(p, b);
@@ -696,12 +716,11 @@ static void launch_pair_t(Batch& b, const Pot& p, const StepArgs& S, const doubl
                          b.cap, b.d_desc, E, AA, S, b.misc);
         }
         if (!(which & 2)) return;
-        if (E.nk > 1)
-            launch_k(b, K_FORCE, k_eam_force<R, TPA, true>, dim3(grid), dim3(256), 0, pos, pos_next, b.nl, b.nn, b.M,
-                     b.cap, b.d_desc, E, AA, S, b.rsave, b.misc);
-        else
-            launch_k(b, K_FORCE, k_eam_force<R, TPA, false>, dim3(grid), dim3(256), 0, pos, pos_next, b.nl, b.nn, b.M,
-                     b.cap, b.d_desc, E, AA, S, b.rsave, b.misc);
+        const bool ev = S.flags & (W_ENERGY | W_VIRIAL);
+        auto k = E.nk > 1 ? (ev ? k_eam_force<R, TPA, true, true> : k_eam_force<R, TPA, true, false>)
+                          : (ev ? k_eam_force<R, TPA, false, true> : k_eam_force<R, TPA, false, false>);
+        launch_k(b, K_FORCE, k, dim3(grid), dim3(256), 0, pos, pos_next, b.nl, b.nn, b.M, b.cap, b.d_desc, E, AA, S,
+                 b.rsave, b.misc);
     }
 }
 
@@ -728,8 +747,8 @@ static void launch_step_kernels(Batch& b, const Pot& p, const StepArgs& S, int p
     else launch_pair_r<double>(b, p, S, pos, pos_next);
 }
 
-void check_errors(Batch& b) {
-    int e = b.ctx->h_pin[M_ERR];
+void check_errors(Batch& b, const int* counters) {
+    int e = (counters ? counters : b.ctx->h_pin)[M_ERR];
     if (!e) return;
     HIPCHECK(hipMemsetAsync(b.misc + M_ERR, 0, sizeof(int), b.ctx->stream));
     if (e & EB_RHO_RANGE)
@@ -989,19 +1008,18 @@ void run_steps(Batch& b, const Pot& p, double dt, int nsteps, int prec, int chun
                  b.d_desc, b.rsave, b.misc);
     }
     auto buf = [&](int t) { return c0 ^ 1 ^ (t & 1); };
-    int s = 0;
-    while (s < nsteps) {
-        // Launch up to the predicted next rebuild in one chunk, then in
-        // short chunks: kernels after a raised trigger exit at once, and each
-        // chunk ends with one host read of the trigger.
-        int c = chunk;
-        if (b.pred_interval > 0) {
-            const int left = (int)(0.8 * b.pred_interval) - b.since_rebuild;
-            c = left > 2 ? left : 2;
-        }
-        c = std::max(1, std::min(c, 64));
-        const int e = std::min(nsteps, s + c);
-        for (int t = s; t < e; ++t) {
+    // Steps go out in chunks, two chunks in flight: while one runs, the
+    // host reads the trigger and error counters the other left (copied to
+    // a pinned slot behind it), so the device never waits for the host
+    // between chunks.  Kernels after a raised trigger exit at once.
+    struct Chunk {
+        int s, e, slot;
+    };
+    Chunk q[2];
+    int nq = 0, head = 0, next_slot = 0;
+    int s = 0, launch_s = 0;
+    auto launch_chunk = [&](int a, int e) {
+        for (int t = a; t < e; ++t) {
             const bool last = t == nsteps - 1;
             StepArgs S{dt, t, last ? MODE_FINAL : MODE_FINAL_DRIFT, 0, last ? 1 : 0};
             if (b.therm.count == 0) {
@@ -1016,25 +1034,57 @@ void run_steps(Batch& b, const Pot& p, double dt, int nsteps, int prec, int chun
             hipLaunchKernelGGL(fin, dim3(grid), dim3(256), 0, st, b.pos[buf(t)], last ? nullptr : b.pos[buf(t) ^ 1],
                                AA, S, b.orig[b.sc], b.therm, b.therm_calls + (uint64_t)t, b.d_desc, b.rsave, b.misc);
         }
-        read_misc(b);
-        check_errors(b);
-        const int ab = b.ctx->h_pin[M_ABORT];
-        if (ab < e) {
-            // steps [s, ab) ran; positions of step ab wait in buf(ab)
+        int* slot = b.ctx->h_pin + M_COUNT * (1 + next_slot);
+        HIPCHECK(hipMemcpyAsync(slot, b.misc, sizeof(int) * M_COUNT, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipEventRecord(b.ctx->chunk_ev[next_slot], st));
+        q[(head + nq) % 2] = Chunk{a, e, next_slot};
+        ++nq;
+        next_slot ^= 1;
+    };
+    while (s < nsteps) {
+        while (nq < 2 && launch_s < nsteps) {
+            // up to the predicted next rebuild in one chunk, then short ones
+            int c = chunk;
+            if (b.pred_interval > 0) {
+                const int left = (int)(0.8 * b.pred_interval) - b.since_rebuild - (launch_s - s);
+                c = left > 2 ? left : 2;
+            }
+            c = std::max(1, std::min(c, 64));
+            const int e = std::min(nsteps, launch_s + c);
+            launch_chunk(launch_s, e);
+            launch_s = e;
+        }
+        const Chunk P = q[head];
+        head = (head + 1) % 2;
+        --nq;
+        HIPCHECK(hipEventSynchronize(b.ctx->chunk_ev[P.slot]));
+        const int* cnt = b.ctx->h_pin + M_COUNT * (1 + P.slot);
+        check_errors(b, cnt);
+        const int ab = cnt[M_ABORT];
+        if (ab <= P.e) {
+            // steps [s, ab) ran; positions of step ab wait in buf(ab) (the
+            // last step of a sequence never raises the trigger, so ab <
+            // nsteps).  A chunk queued behind this one exits at once.
+            if (nq) {
+                HIPCHECK(hipStreamSynchronize(st));
+                nq = 0;
+                head = 0;
+            }
             b.since_rebuild += ab - s;
             b.pred_interval = b.pred_interval > 0 ? 0.5 * b.pred_interval + 0.5 * b.since_rebuild
                                                   : (double)b.since_rebuild;
             b.since_rebuild = 0;
             b.cur = buf(ab);
             nlist_rebuild(b);
+            timer_drain(b);
             c0 = b.cur ^ 1 ^ (ab & 1);
-            HIPCHECK(hipMemcpyAsync(b.misc + M_ABORT, &abort_init, sizeof(int), hipMemcpyHostToDevice, st));
-            s = ab;
+            s = launch_s = ab;  // the rebuild cleared the trigger
         } else {
-            b.since_rebuild += e - s;
-            s = e;
+            b.since_rebuild += P.e - P.s;
+            s = P.e;
         }
     }
+    timer_drain(b);
     b.cur = buf(nsteps - 1);
     if (b.therm.count) b.therm_calls += (uint64_t)nsteps;
     // after make_step the reference holds Ep = Em = 0 and V = 0 (prepare

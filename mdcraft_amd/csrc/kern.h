@@ -171,6 +171,22 @@ __device__ inline void first_group(const AtomSlots& as, bool valid, int M, const
     for (int u = 0; u < U; ++u) j0[u] = (valid && u < M) ? nl[as.base + (long long)u * kWave] : 0;
 }
 
+// The entries of the group after the one starting at chunk m0, read while
+// the current group's gathers are in flight (chunks past M are not read).
+// MDC_PREFETCH = 0 reads each group at the top of its own iteration.
+#ifndef MDC_PREFETCH
+#define MDC_PREFETCH 1
+#endif
+template <int TPA, int U>
+__device__ inline void next_group(const AtomSlots& as, int m0, int M, const int* __restrict__ nl, int* en) {
+    if (!MDC_PREFETCH) return;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int c = m0 + U + u;
+        en[u] = c < M ? nl[as.base + (long long)c * kWave] : 0;
+    }
+}
+
 // Difference r_j - r_i of a listed pair under the reference's image rule
 // (pair_image); when the system's boxes make the minimum image unique
 // between rebuilds, the shift recorded at build time is applied instead.

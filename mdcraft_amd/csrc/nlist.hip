@@ -20,8 +20,13 @@ namespace mdc {
 // lattice::Domain::fit_in_period (domain.cxx:41-47) and the cell of an atom
 // in its system's grid (cells of edge >= max rns spanning the box).
 __global__ void k_wrap_count(double4* __restrict__ pos, const SysDesc* __restrict__ desc, int* __restrict__ atom_cell,
-                             int* __restrict__ cell_count, int n) {
+                             int* __restrict__ cell_count, int* __restrict__ misc, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        // a rebuild starts: clear the overflow maximum and the step trigger
+        misc[M_MAXNN] = 0;
+        misc[M_ABORT] = 0x7fffffff;
+    }
     if (i >= n) return;
     double4 p = pos[i];
     const SysDesc& s = desc[w_sys(p.w)];
@@ -44,7 +49,7 @@ __global__ void k_wrap_count(double4* __restrict__ pos, const SysDesc* __restric
 
 // Exclusive scan of the per-cell counts with one workgroup; also clears the
 // fill counters.  start has ncells + 1 entries.
-__global__ void __launch_bounds__(1024) k_scan_cells(const int* __restrict__ count, int* __restrict__ start,
+__global__ void __launch_bounds__(1024) k_scan_cells(int* __restrict__ count, int* __restrict__ start,
                                                      int* __restrict__ fill, int ncells) {
     __shared__ int warp_tot[16];
     __shared__ int carry;
@@ -54,7 +59,10 @@ __global__ void __launch_bounds__(1024) k_scan_cells(const int* __restrict__ cou
     for (int base = 0; base < ncells; base += 1024) {
         int idx = base + threadIdx.x;
         int v = idx < ncells ? count[idx] : 0;
-        if (idx < ncells) fill[idx] = 0;
+        if (idx < ncells) {
+            fill[idx] = 0;
+            count[idx] = 0;  // ready for the next rebuild's counting
+        }
         int x = v;
         for (int o = 1; o < 64; o <<= 1) {
             int y = __shfl_up(x, o, 64);
@@ -295,9 +303,8 @@ void nlist_rebuild(Batch& b) {
         b.list_valid = true;
         return;
     }
-    HIPCHECK(hipMemsetAsync(b.cell_count, 0, sizeof(int) * (b.ncells + 1), st));
     hipLaunchKernelGGL(k_wrap_count, dim3(nblk(n, 256)), dim3(256), 0, st, b.pos[b.cur], b.d_desc, b.atom_cell,
-                       b.cell_count, n);
+                       b.cell_count, b.misc, n);
     hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, st, b.cell_count, b.cell_start, b.cell_fill, b.ncells);
     hipLaunchKernelGGL(k_scatter, dim3(nblk(n, 256)), dim3(256), 0, st, b.atom_cell, b.cell_start, b.cell_fill,
                        b.sorted_src, n);
@@ -310,7 +317,7 @@ void nlist_rebuild(Batch& b) {
     b.cur ^= 1;
     b.sc = s1;
     for (int attempt = 0; attempt < 3; ++attempt) {
-        HIPCHECK(hipMemsetAsync(b.misc + M_MAXNN, 0, sizeof(int), st));
+        if (attempt > 0) HIPCHECK(hipMemsetAsync(b.misc + M_MAXNN, 0, sizeof(int), st));
         hipLaunchKernelGGL(k_build, dim3(b.ncells), dim3(256), 0, st, b.pos[b.cur], b.d(b.sc, D_RNS), b.cell_start,
                            b.d_desc, b.nl, b.nn, b.misc, b.tpa, b.M, b.cap);
         HIPCHECK(hipMemcpyAsync(b.ctx->h_pin, b.misc, sizeof(int) * M_COUNT, hipMemcpyDeviceToHost, st));
