@@ -166,6 +166,7 @@ __global__ void __launch_bounds__(256) k_build(const double4* __restrict__ pos, 
                                                int* __restrict__ nl, int* __restrict__ nn, int* __restrict__ misc,
                                                int tpa, int M, int cap) {
     __shared__ int s_beg[32], s_pre[32];
+    __shared__ unsigned s_img[32];  // image bits of each stencil cell (fast path)
     __shared__ int c_j[kTile];
     __shared__ double c_x[kTile], c_y[kTile], c_z[kTile], c_r[kTile];
     __shared__ int s_cnt[64];
@@ -183,6 +184,13 @@ __global__ void __launch_bounds__(256) k_build(const double4* __restrict__ pos, 
             const int ox = lane / 9 - 1, oy = (lane / 3) % 3 - 1, oz = lane % 3 - 1;
             bool ok = true;
             int x = cx + ox, y = cy + oy, z = cz + oz;
+            // r_i - r_j is near -L across the low face (the reference then
+            // adds L: shift +1) and near +L across the high face (shift -1);
+            // stored as 1 - shift per axis, the list entry's image bits
+            const unsigned ix = s.per[0] ? (x < 0 ? 0u : x >= nx ? 2u : 1u) : 1u;
+            const unsigned iy = s.per[1] ? (y < 0 ? 0u : y >= ny ? 2u : 1u) : 1u;
+            const unsigned iz = s.per[2] ? (z < 0 ? 0u : z >= nz ? 2u : 1u) : 1u;
+            s_img[lane] = (ix << 26) | (iy << 28) | (iz << 30);
             if (s.per[0]) {
                 ok = ok && !((nx == 1 && ox != 0) || (nx == 2 && ox == 1));
                 x = (x + nx) % nx;
@@ -215,6 +223,13 @@ __global__ void __launch_bounds__(256) k_build(const double4* __restrict__ pos, 
     const int total = s_pre[26];
     const double L0 = s.L[0], L1 = s.L[1], L2 = s.L[2];
     const int p0 = s.per[0], p1 = s.per[1], p2 = s.per[2];
+    // With at least 5 cells on every periodic axis, the reference's shift
+    // test (2 dx < -L, 2 dx > L) holds for every candidate of a wrapped
+    // stencil cell and for none of an unwrapped one, with a margin of a
+    // whole cell: |2 dx| exceeds L + cell across a face and stays below
+    // L - cell inside.  The shift then comes with the candidate, and dx is
+    // adjusted by the same exact add of +-L.
+    const bool fast = (!p0 || s.nc[0] >= 5) && (!p1 || s.nc[1] >= 5) && (!p2 || s.nc[2] >= 5);
     for (int g0 = a0; g0 < a1; g0 += 64) {
         const int ga = min(64, a1 - g0);
         if (tid < 64) s_cnt[tid] = 0;
@@ -227,7 +242,7 @@ __global__ void __launch_bounds__(256) k_build(const double4* __restrict__ pos, 
                 while (s_pre[cc] <= idx) ++cc;
                 const int j = s_beg[cc] + idx - (cc > 0 ? s_pre[cc - 1] : 0);
                 const double4 pj = pos[j];
-                c_j[q] = j;
+                c_j[q] = fast ? (int)((unsigned)j | s_img[cc]) : j;
                 c_x[q] = pj.x;
                 c_y[q] = pj.y;
                 c_z[q] = pj.z;
@@ -236,14 +251,32 @@ __global__ void __launch_bounds__(256) k_build(const double4* __restrict__ pos, 
             __syncthreads();
             for (int a = wv; a < ga; a += 4) {
                 const int i = g0 + a;
-                const double4 pi = pos[i];
-                const double ri = rns[i];
+                // the atom's own cell is stencil cell 13, staged from
+                // s_pre[12] on: read it from the tile when it is there
+                const int qi = s_pre[12] + (i - a0) - t0;
+                const bool own = qi >= 0 && qi < tn;
+                const double4 pi = own ? make_double4(c_x[qi], c_y[qi], c_z[qi], 0.0) : pos[i];
+                const double ri = own ? c_r[qi] : rns[i];
                 int cnt = s_cnt[a];
                 for (int q0 = 0; q0 < tn; q0 += kWave) {
                     const int q = q0 + lane;
                     bool hit = false;
                     unsigned ent = 0;
-                    if (q < tn) {
+                    if (fast) {
+                        if (q < tn) {
+                            ent = (unsigned)c_j[q];
+                            if ((int)(ent & kIdxMask) != i) {
+                                const double rr = fmax(ri, c_r[q]);
+                                const double d2c = rr * rr;
+                                double dx = pi.x - c_x[q], dy = pi.y - c_y[q], dz = pi.z - c_z[q];
+                                // 1 - stored bits = the shift of r_i - r_j; (+-1) * L is exact
+                                dx = fma((double)(1 - (int)((ent >> 26) & 3u)), L0, dx);
+                                dy = fma((double)(1 - (int)((ent >> 28) & 3u)), L1, dy);
+                                dz = fma((double)(1 - (int)((ent >> 30) & 3u)), L2, dz);
+                                hit = sq3_exact(dx, dy, dz) <= d2c;
+                            }
+                        }
+                    } else if (q < tn) {
                         const int j = c_j[q];
                         if (j != i) {  // j < 2^26 (checked on the host)
                             const double rr = fmax(ri, c_r[q]);
