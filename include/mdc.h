@@ -62,6 +62,15 @@ int mdc_atom_record_size(void); /* 208 */
 /* ---- potentials (immutable after create; may be shared by systems) ------ */
 /* LJsCut(a, s, rcut): solver/potential/LJs-cut.cxx:8-20 */
 mdc_status mdc_pot_lj_create(mdc_ctx* ctx, double epsilon, double sigma, double rcut, mdc_pot** out);
+/* LJs(aVr, rVr, Rcutoff): Lennard-Jones with a polynomial tail that takes
+ * V and V' to zero at the cutoff (solver/potential/LJs.cxx:7-256) */
+mdc_status mdc_pot_ljs_create(mdc_ctx* ctx, double aVr, double rVr, double rcut, mdc_pot** out);
+/* Pair(Rcutoff, Vpar, Nr1, dr): tabulated pair potential, r*phi on nr
+ * knots of step dr in reference units (solver/potential/pair.cxx:15-34) */
+mdc_status mdc_pot_pair_create(mdc_ctx* ctx, double rcut, const double* rphi, int nr, double dr, mdc_pot** out);
+/* Pair(filename): four lines skipped, then "Nr dr rcut" (Angstrom) and Nr
+ * values of r*phi (eV*Angstrom), converted as pair.cxx:36-56 does */
+mdc_status mdc_pot_pair_create_file(mdc_ctx* ctx, const char* path, mdc_pot** out);
 /* EAM(Rcutoff, Femb, Rho, Vpar, Nr1, dr, Nrh1, drh): solver/potential/EAM.cxx:15-53
  * (tables in reference units; rphi is r*phi) */
 mdc_status mdc_pot_eam_create(mdc_ctx* ctx, double rcut, const double* F, const double* rho, const double* rphi,

@@ -1,2 +1,3 @@
-from mdcraft_amd.refapi import (EAM, LJs, MLIP4, DeepModelingPotential, LJsCut, MaiseBPNNPotential, Pair, Potential,
+from mdcraft_amd.refapi import (EAM, LJs, MLIP4, DeepModelingPotential, LJsCut, MaiseBPNNPotential, Potential,
                                 PotentialTernary)
+from mdcraft_amd.refapi import TabulatedPair as Pair

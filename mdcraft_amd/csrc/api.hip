@@ -724,6 +724,111 @@ mdc_status mdc_pot_lj_create(mdc_ctx* ctx, double a, double s, double rcut, mdc_
     });
 }
 
+mdc_status mdc_pot_ljs_create(mdc_ctx* ctx, double aVr, double rVr, double rcut, mdc_pot** out) {
+    return guard(ctx ? &ctx->c : nullptr, [&] {
+        if (!ctx || !out) throw Error(E_INVAL, "null argument");
+        if (!(rcut > 0) || !(rVr > 0)) throw Error(E_INVAL, "LJs rVr and cutoff must be positive");
+        auto p = std::make_unique<Pot>();
+        p->ctx = &ctx->c;
+        p->type = PT_LJ;
+        p->rcut = rcut;
+        LJParams& L = p->lj;
+        L.fn = 1;
+        L.rc = rcut;
+        L.rc2 = rcut * rcut;
+        // LJs.cxx:7-26, in the constructor's order of operations
+        const double X2min = 1.2599210498948732;  // 2^(1/3): X = 0 at the LJ minimum
+        const double R2cut = L.rc2;
+        double a = aVr;
+        const double r2v = rVr * rVr;
+        L.rVr2 = r2v;
+        L.rVr1 = 1.0 / r2v;
+        double aVr2 = R2cut / r2v - X2min;
+        double aLJ3 = (r2v / R2cut) * (r2v / R2cut) * (r2v / R2cut);
+        double bLJ2 = (aLJ3 - 0.5) * aVr2 * r2v / R2cut;
+        L.aLJ = (aLJ3 - 1.0 + 2.0 * bLJ2) * aLJ3 / (aVr2 * aVr2);
+        L.bLJ = -(aLJ3 - 1.0 + 3.0 * bLJ2) * aLJ3 / (aVr2 * aVr2 * aVr2);
+        L.aLJ3 = 3.0 * L.aLJ;
+        L.bLJ2 = 2.0 * L.bLJ;
+        L.aVr2 = -48.0 * a / r2v;
+        L.aVr = 2.0 * a;  // V(r)/2 per pair
+        *out = reinterpret_cast<mdc_pot*>(p.release());
+    });
+}
+
+// Tabulated Pair (pair.cxx:15-34): r*phi on nr knots of step dr, spline
+// slopes as EAM's with the ib = 1 end condition.
+static Pot* make_pair(Ctx* ctx, double rcut, const double* rphi, int nr, double dr) {
+    if (nr < 3) throw Error(E_INVAL, "Pair tables need at least 3 knots");
+    if (!(dr > 0) || !(rcut > 0)) throw Error(E_INVAL, "Pair step and cutoff must be positive");
+    auto p = std::make_unique<Pot>();
+    p->ctx = ctx;
+    p->type = PT_LJ;
+    p->rcut = rcut;
+    p->lj.fn = 2;
+    p->lj.rc = rcut;
+    p->lj.rc2 = rcut * rcut;
+    EamTables& T = p->eam;
+    T.nr = nr;
+    T.dr = dr;
+    T.rc = rcut;
+    std::vector<double> rec64, z(nr);
+    spline_slopes(rphi, z.data(), 1.0 / dr, nr, 1);
+    records(rphi, z.data(), nr, rec64);
+    for (double x : rec64)
+        if (!std::isfinite(x)) throw Error(E_RANGE, "non-finite value in the Pair table");
+    std::vector<float> rec32(rec64.size());
+    for (size_t q = 0; q < rec64.size(); q += 4) {
+        const double y0 = rec64[q], z0 = rec64[q + 1], y1 = rec64[q + 2], z1 = rec64[q + 3];
+        const double sl = (y1 - y0) / dr;
+        const double aa = (z1 - sl) - (sl - z0);
+        const double bb = aa - (sl - z0);
+        rec32[q] = (float)y0;
+        rec32[q + 1] = (float)(dr * z0);
+        rec32[q + 2] = (float)(-dr * bb);
+        rec32[q + 3] = (float)(dr * aa);
+    }
+    T.nrec = rec64.size() / 4;
+    T.r64 = dalloc<double4>(T.nrec);
+    T.r32 = dalloc<float4>(T.nrec);
+    HIPCHECK(hipMemcpy(T.r64, rec64.data(), rec64.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(T.r32, rec32.data(), rec32.size() * sizeof(float), hipMemcpyHostToDevice));
+    return p.release();
+}
+
+mdc_status mdc_pot_pair_create(mdc_ctx* ctx, double rcut, const double* rphi, int nr, double dr, mdc_pot** out) {
+    return guard(ctx ? &ctx->c : nullptr, [&] {
+        if (!ctx || !out || !rphi) throw Error(E_INVAL, "null argument");
+        *out = reinterpret_cast<mdc_pot*>(make_pair(&ctx->c, rcut, rphi, nr, dr));
+    });
+}
+
+// Pair(filename) (pair.cxx:36-56): four lines skipped, then "Nr dr rcut"
+// in Angstrom and Nr values of r*phi in eV*Angstrom
+mdc_status mdc_pot_pair_create_file(mdc_ctx* ctx, const char* path, mdc_pot** out) {
+    return guard(ctx ? &ctx->c : nullptr, [&] {
+        if (!ctx || !out || !path) throw Error(E_INVAL, "null argument");
+        std::ifstream f(path);
+        if (!f.is_open()) throw Error(E_FILE, std::string("Failed to open file: ") + path);
+        std::string line;
+        for (int i = 0; i < 4; ++i) std::getline(f, line);
+        int nr = 0;
+        double dr = 0, rc = 0;
+        f >> nr >> dr >> rc;
+        if (!f || nr < 3) throw Error(E_FILE, std::string("malformed Pair header: ") + path);
+        const double Na = 6.02214076e+23, Qe = 1.602176634e-19;
+        const double UnMass = 1e-3 / Na, UnLeng = 1e-9, UnTime = 1e-12;
+        const double UnEnrg = UnMass * (UnLeng / UnTime) * (UnLeng / UnTime);
+        std::vector<double> rphi(nr);
+        for (auto& x : rphi) {
+            f >> x;
+            x *= Qe / (10 * UnEnrg);
+        }
+        if (!f) throw Error(E_FILE, std::string("truncated Pair table: ") + path);
+        *out = reinterpret_cast<mdc_pot*>(make_pair(&ctx->c, rc / 10, rphi.data(), nr, dr / 10));
+    });
+}
+
 mdc_status mdc_pot_eam_create(mdc_ctx* ctx, double rcut, const double* F, const double* rho, const double* rphi,
                               int nr, double dr, int nrho, double drho, mdc_pot** out) {
     return guard(ctx ? &ctx->c : nullptr, [&] {

@@ -71,8 +71,14 @@ struct ThermArgs {
     LangevinP t[kMaxTherm];
 };
 
+// Pair potentials: fn 0 = LJsCut (LJs-cut.cxx:8-20), 1 = LJs, the LJ with
+// a polynomial tail that takes V and V' to zero at rc (LJs.cxx:7-26,
+// coefficients as its constructor leaves them), 2 = tabulated Pair
+// (potential/pair.cxx; spline records in the Pot's EamTables r64/r32)
 struct LJParams {
+    int fn = 0;
     double a, s, s6, lj1, lj2, factor, rc, rc2;
+    double aVr, aVr2, rVr1, rVr2, aLJ, aLJ3, bLJ, bLJ2;
 };
 
 // One EAM object of a Multi potential, used for one pair of kinds

@@ -24,13 +24,30 @@
 namespace mdc {
 
 // --------------------------------------------------------------------------
-// LJsCut
+// Pair potentials: LJsCut, LJs, tabulated Pair
 // --------------------------------------------------------------------------
 struct LJK {
-    double a, s6, lj1, lj2, factor, rc2;
+    double rc2;
+    double a, s6, lj1, lj2, factor;                      // LJsCut
+    double aVr, aVr2, rVr1, rVr2, aLJ, aLJ3, bLJ, bLJ2;  // LJs
+    const double4* t64;                                  // Pair: r*phi records
+    const float4* t32;
+    unsigned nrec;
+    int nr;
+    double dr, dr1;
 };
 
-template <typename R, int TPA, bool EV>
+template <typename R>
+__device__ inline typename Vec4T<R>::T load_rec(__amdgpu_buffer_rsrc_t b, int rec);
+template <typename R>
+__device__ inline const void* rtab_p(const double4* t64, const float4* t32) {
+    return sizeof(R) == 8 ? (const void*)t64 : (const void*)t32;
+}
+
+// FN: 0 LJsCut (LJs-cut.cxx:22-131), 1 LJs (LJs.cxx:28-256), 2 tabulated
+// Pair (pair.cxx:59-493).  Each gives V'(r)/r (the force on i along
+// r_j - r_i) and V(r)/2 of one listed pair.
+template <typename R, int TPA, bool EV, int FN>
 __global__ void __launch_bounds__(256) k_lj(const double4* __restrict__ pos, double4* __restrict__ pos_next,
                                             const int* __restrict__ nl, const int* __restrict__ nn, int M, int cap,
                                             const SysDesc* __restrict__ desc, LJK P, AtomArrays A, StepArgs S,
@@ -49,24 +66,49 @@ __global__ void __launch_bounds__(256) k_lj(const double4* __restrict__ pos, dou
     const Frame fr = frame_of(s);
     const int cnt = active ? min(nraw, cap) : 0;
     const R c_pref = (R)(-0.5 * P.factor), c_lj1 = (R)P.lj1, c_lj2 = (R)P.lj2, c_s6 = (R)P.s6, c_e = (R)(0.5 * 4 * P.a);
+    const R c_x2min = (R)1.2599210498948732, c_rVr1 = (R)P.rVr1, c_rVr2 = (R)P.rVr2, c_aVr = (R)P.aVr,
+            c_aVr2 = (R)P.aVr2, c_aLJ = (R)P.aLJ, c_bLJ = (R)P.bLJ, c_aLJ3 = (R)P.aLJ3, c_bLJ2 = (R)P.bLJ2;
+    const R c_dr = (R)P.dr, c_dr1 = (R)P.dr1;
     R fx = 0, fy = 0, fz = 0;
     double e = 0, vxx = 0, vyy = 0, vzz = 0, vxy = 0, vxz = 0, vyz = 0;
     const __amdgpu_buffer_rsrc_t posb = make_buf(pos, (unsigned)A.n * 32u);
+    const __amdgpu_buffer_rsrc_t tabb =
+        make_buf(FN == 2 ? rtab_p<R>(P.t64, P.t32) : (const void*)pos,
+                 FN == 2 ? P.nrec * (unsigned)sizeof(typename Vec4T<R>::T) : 0u);
     for_neighbours<TPA, 4>(as, cnt, nl, posb, j0, [&](int ent, const double4& pj) {
         double dx, dy, dz, d2;
         if (!pair_delta(fr, pi, pj, ent, dx, dy, dz, d2)) return;
         if (d2 > P.rc2 || d2 < 1e-15) return;
         const R r2 = (R)d2, rx = (R)dx, ry = (R)dy, rz = (R)dz;
-        const R r2inv = rcp_(r2);
-        const R r6inv = r2inv * r2inv * r2inv;
-        const R pref = c_pref * (c_lj1 * r6inv - c_lj2) * r6inv * r2inv;
+        R pref, eh = 0;
+        if constexpr (FN == 0) {
+            const R r2inv = rcp_(r2);
+            const R r6inv = r2inv * r2inv * r2inv;
+            pref = c_pref * (c_lj1 * r6inv - c_lj2) * r6inv * r2inv;
+            if (EV) {
+                const R t = c_s6 * r6inv;
+                eh = c_e * t * (t - (R)1);
+            }
+        } else if constexpr (FN == 1) {
+            const R XX = c_rVr1 * r2 - c_x2min;
+            const R FV = c_rVr2 * rcp_(r2);
+            const R Ur = FV * FV * FV;
+            pref = c_aVr2 * (Ur * (Ur - (R)0.5) * FV + (c_aLJ + c_bLJ * XX) * XX);
+            if (EV) eh = c_aVr * (Ur * (Ur - (R)1) - (c_aLJ3 + c_bLJ2 * XX) * XX * XX);
+        } else {
+            const R R1 = sqrt(r2), RN = R1 * c_dr1;
+            const int N1 = (int)RN;
+            if (N1 >= P.nr) return;  // beyond the table: V = V' = 0 (pair.cxx:370-376)
+            R v, d;
+            spline_vd<R>(load_rec<R>(tabb, N1), R1, RN, N1, c_dr, c_dr1, v, d);
+            const R i1 = rcp_(R1);
+            pref = (d * R1 - v) * rcp_(r2) * i1;
+            if (EV) eh = (R)0.5 * v * i1;
+        }
         fx += rx * pref;
         fy += ry * pref;
         fz += rz * pref;
-        if (EV && (S.flags & W_ENERGY)) {
-            const R t = c_s6 * r6inv;
-            e += (double)(c_e * t * (t - (R)1));
-        }
+        if (EV && (S.flags & W_ENERGY)) e += (double)eh;
         if (EV && (S.flags & W_VIRIAL)) {
             const double px = (double)(pref * rx), py = (double)(pref * ry), pz = (double)(pref * rz);
             vxx += px * (double)rx;
@@ -686,7 +728,29 @@ static EamK eam_args(const Pot& p, Batch& b) {
 }
 
 static LJK lj_args(const Pot& p) {
-    return LJK{p.lj.a, p.lj.s6, p.lj.lj1, p.lj.lj2, p.lj.factor, p.lj.rc2};
+    const LJParams& L = p.lj;
+    LJK k{};
+    k.rc2 = L.rc2;
+    k.a = L.a;
+    k.s6 = L.s6;
+    k.lj1 = L.lj1;
+    k.lj2 = L.lj2;
+    k.factor = L.factor;
+    k.aVr = L.aVr;
+    k.aVr2 = L.aVr2;
+    k.rVr1 = L.rVr1;
+    k.rVr2 = L.rVr2;
+    k.aLJ = L.aLJ;
+    k.aLJ3 = L.aLJ3;
+    k.bLJ = L.bLJ;
+    k.bLJ2 = L.bLJ2;
+    k.t64 = p.eam.r64;
+    k.t32 = p.eam.r32;
+    k.nrec = (unsigned)p.eam.nrec;
+    k.nr = p.eam.nr;
+    k.dr = p.eam.dr;
+    k.dr1 = p.eam.dr > 0 ? 1.0 / p.eam.dr : 0.0;
+    return k;
 }
 
 // which: 1 = density pass (EAM), 2 = force pass, 3 = both
@@ -702,7 +766,14 @@ static void launch_pair_t(Batch& b, const Pot& p, const StepArgs& S, const doubl
         if (!(which & 2)) return;
         // energy and virial code only in the variant that computes them: the
         // step kernels keep fewer registers live
-        auto k = S.flags & (W_ENERGY | W_VIRIAL) ? k_lj<R, TPA, true> : k_lj<R, TPA, false>;
+        const bool ev = S.flags & (W_ENERGY | W_VIRIAL);
+        auto pick = [&](auto fn) {
+            constexpr int F = decltype(fn)::value;
+            return ev ? k_lj<R, TPA, true, F> : k_lj<R, TPA, false, F>;
+        };
+        auto k = p.lj.fn == 1 ? pick(std::integral_constant<int, 1>{})
+                              : p.lj.fn == 2 ? pick(std::integral_constant<int, 2>{})
+                                             : pick(std::integral_constant<int, 0>{});
         launch_k(b, K_FORCE, k, dim3(grid), dim3(256), 0, pos, pos_next, b.nl, b.nn, b.M, b.cap, b.d_desc,
                  lj_args(p), AA, S, b.rsave, b.misc);
     } else {

@@ -29,7 +29,7 @@ F_ALL = (1 << 14) - 1
 
 EXPORTS = [
     "mdc_ctx_create", "mdc_ctx_destroy", "mdc_ctx_last_error", "mdc_last_error", "mdc_atom_record_size",
-    "mdc_pot_lj_create", "mdc_pot_eam_create", "mdc_pot_eam_create_file", "mdc_pot_multi_create",
+    "mdc_pot_lj_create", "mdc_pot_ljs_create", "mdc_pot_pair_create", "mdc_pot_pair_create_file", "mdc_pot_eam_create", "mdc_pot_eam_create_file", "mdc_pot_multi_create",
     "mdc_pot_bpnn_create", "mdc_pot_rcut", "mdc_pot_destroy",
     "mdc_sys_create", "mdc_sys_destroy", "mdc_sys_upload", "mdc_sys_download", "mdc_sys_download_range", "mdc_sys_set_active", "mdc_sys_natoms",
     "mdc_sys_set_lanes", "mdc_sys_autotune", "mdc_sys_time_kernels", "mdc_sys_set_rebuild", "mdc_sys_set_boundary", "mdc_sys_set_thermostat",
@@ -54,6 +54,9 @@ def load_library(path=LIB_PATH):
         "mdc_last_error": (C.c_char_p, []),
         "mdc_atom_record_size": (I, []),
         "mdc_pot_lj_create": (I, [P, D, D, D, PP]),
+        "mdc_pot_ljs_create": (I, [P, D, D, D, PP]),
+        "mdc_pot_pair_create": (I, [P, D, DP, I, D, PP]),
+        "mdc_pot_pair_create_file": (I, [P, C.c_char_p, PP]),
         "mdc_pot_eam_create": (I, [P, D, DP, DP, DP, I, D, I, D, PP]),
         "mdc_pot_eam_create_file": (I, [P, C.c_char_p, PP]),
         "mdc_pot_multi_create": (I, [P, I, I, C.POINTER(I), C.POINTER(I), PP, PP]),
@@ -153,6 +156,25 @@ def LJsCut(ctx, a, s, rcut):
     """Truncated LJ without energy shift, LJsCut(a, s, rcut)."""
     h = C.c_void_p()
     _check(lib().mdc_pot_lj_create(ctx.h, a, s, rcut, C.byref(h)))
+    return Potential(ctx, h, "pair")
+
+
+def LJs(ctx, aVr, rVr, rcut):
+    """LJ with a smooth polynomial tail, LJs(aVr, rVr, Rcutoff)."""
+    h = C.c_void_p()
+    _check(lib().mdc_pot_ljs_create(ctx.h, aVr, rVr, rcut, C.byref(h)))
+    return Potential(ctx, h, "pair")
+
+
+def Pair(ctx, rcut=None, rphi=None, dr=None, path=None):
+    """Tabulated pair potential: Pair(Rcutoff, Vpar, Nr1, dr) or
+    Pair(filename)."""
+    h = C.c_void_p()
+    if path is not None:
+        _check(lib().mdc_pot_pair_create_file(ctx.h, path.encode(), C.byref(h)))
+    else:
+        rphi = np.ascontiguousarray(rphi, dtype=np.float64)
+        _check(lib().mdc_pot_pair_create(ctx.h, rcut, _dp(rphi), len(rphi), dr, C.byref(h)))
     return Potential(ctx, h, "pair")
 
 

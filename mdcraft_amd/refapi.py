@@ -27,8 +27,8 @@ Differences from the reference, each with the reason:
   and rcut every solver sets, the two agree;
 * the half list of ``VerletList(half=True)`` is exported as the full
   list's upper half (j >= i); forces are identical either way;
-* potentials outside the engine (``LJs``, tabulated ``Pair``,
-  ``DeepModelingPotential``, ``MLIP4``) raise NotImplementedError;
+* potentials outside the engine (``DeepModelingPotential``, ``MLIP4``)
+  raise NotImplementedError;
 * ``Langevin`` draws from a counter-based stream keyed by ``seed`` (an
   added keyword; random when omitted) instead of std::random_device, one
   stream per list's system; the members of a ``Compound`` share the first
@@ -494,8 +494,32 @@ def _not_in_engine(name):
     return _Missing
 
 
-LJs = _not_in_engine("LJs")
-Pair = _not_in_engine("Pair")
+class LJs(Potential):
+    """LJs(aVr, rVr, Rcutoff): LJ with a polynomial tail taking V and V' to
+    zero at the cutoff (LJs.cxx:7-256)."""
+
+    def __init__(self, aVr, rVr, Rcutoff):
+        self._engine = E.LJs(context(), aVr, rVr, Rcutoff)
+
+
+class TabulatedPair(Potential):
+    """mdcraft.solver.potential.Pair: Pair(Rcutoff, pairPotential, NrPot,
+    drPot) with r*phi in reference units, or Pair(filename)
+    (pair.cxx:15-56)."""
+
+    def __init__(self, Rcutoff=None, pairPotential=None, NrPot=None, drPot=None, filename=None):
+        if isinstance(Rcutoff, (str, os.PathLike)) and filename is None:
+            filename, Rcutoff = Rcutoff, None
+        if filename is not None:
+            self._engine = E.Pair(context(), path=os.fspath(filename))
+            return
+        nr = int(NrPot)
+        rphi = np.asarray(pairPotential, dtype=np.float64)[:nr]
+        if len(rphi) < nr:
+            raise ValueError("table shorter than NrPot")
+        self._engine = E.Pair(context(), rcut=Rcutoff, rphi=rphi, dr=drPot)
+
+
 DeepModelingPotential = _not_in_engine("DeepModelingPotential")
 MLIP4 = _not_in_engine("MLIP4")
 

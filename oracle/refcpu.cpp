@@ -532,6 +532,64 @@ static const double kUnTime = 1e-12;
 static const double kUnEnrg = kUnMass * (kUnLeng / kUnTime) * (kUnLeng / kUnTime);
 
 // EAM (mdcraft/solver/potential/EAM.cxx)
+// The per-atom loops shared by LJs and the tabulated Pair (LJs.cxx:89-130,
+// :193-255; pair.cxx:235-324, :411-493): r_ji = r_j - r_i, pairs with
+// r^2 > rc^2 or r^2 < 1e-15 skipped, f_i += FF r_ji (a half list also
+// f_j -= FF r_ji), V_i += 0.5 F r_ji^T and Ep_i += V(r)/2.
+template <class P>
+static void pair_force(const P& pot, Atom& atom, Atom* neibs, const size_t* list, size_t nl) {
+    double fx = 0, fy = 0, fz = 0;
+    for (size_t k = 0; k < nl; ++k) {
+        Atom& nb = neibs[list[k]];
+        double rx = nb.r[0] - atom.r[0], ry = nb.r[1] - atom.r[1], rz = nb.r[2] - atom.r[2];
+        double r2 = rx * rx + ry * ry + rz * rz;
+        if (r2 > pot.R2cut || r2 < 1e-15) continue;
+        double FF, E;
+        pot.pair(r2, FF, E);
+        fx += FF * rx;
+        fy += FF * ry;
+        fz += FF * rz;
+        if (pot.half) {
+            nb.f[0] -= FF * rx;
+            nb.f[1] -= FF * ry;
+            nb.f[2] -= FF * rz;
+        }
+    }
+    atom.f[0] += fx;
+    atom.f[1] += fy;
+    atom.f[2] += fz;
+}
+
+template <class P>
+static void pair_virial(const P& pot, Atom& atom, Atom* neibs, const size_t* list, size_t nl) {
+    double V[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, Ep = 0;
+    for (size_t k = 0; k < nl; ++k) {
+        Atom& nb = neibs[list[k]];
+        double r[3] = {nb.r[0] - atom.r[0], nb.r[1] - atom.r[1], nb.r[2] - atom.r[2]};
+        double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+        if (r2 > pot.R2cut || r2 < 1e-15) continue;
+        double FF, E;
+        pot.pair(r2, FF, E);
+        // column-major V[col * 3 + row] = 0.5 F[row] r[col]
+        double Vk[9];
+        for (int c = 0; c < 3; ++c)
+            for (int q = 0; q < 3; ++q) Vk[c * 3 + q] = 0.5 * (FF * r[q]) * r[c];
+        if (pot.half) {
+            for (int t = 0; t < 9; ++t) {
+                atom.V[t] += Vk[t];
+                nb.V[t] += Vk[t];
+            }
+            atom.Ep += E;
+            nb.Ep += E;
+        } else {
+            for (int t = 0; t < 9; ++t) V[t] += Vk[t];
+            Ep += E;
+        }
+    }
+    for (int t = 0; t < 9; ++t) atom.V[t] += V[t];
+    atom.Ep += Ep;
+}
+
 struct EAM : Potential {
     int Nr1 = 0, Nrh1 = 0;
     double dr = 0, drh = 0, dr1 = 0, drh1 = 0;
@@ -795,6 +853,85 @@ struct EAM : Potential {
     void virial(Atom& atom, Atom* neibs, const size_t* list, size_t nl) override {
         gather(atom, neibs, list, nl, true);
     }
+};
+
+// LJs (mdcraft/solver/potential/LJs.cxx:7-256): LJ with a polynomial tail
+// in X = (r / rVr)^2 - 2^(1/3) that takes V and V' to zero at rc; the
+// coefficients as the constructor leaves them.
+struct LJs : Potential {
+    const double X2min = 1.2599210498948732;
+    double aVr, aVr2, rVr, rVr1, rVr2, aLJ, aLJ3, bLJ, bLJ2;
+    LJs(double a, double rv, double rcut) : Potential(PT_PAIR) {
+        R1cut = rcut;
+        R2cut = rcut * rcut;
+        aVr = a;
+        rVr = rv;
+        rVr2 = rVr * rVr;
+        rVr1 = 1.0 / rVr2;
+        aVr2 = R2cut / rVr2 - X2min;
+        aLJ3 = (rVr2 / R2cut) * (rVr2 / R2cut) * (rVr2 / R2cut);
+        bLJ2 = (aLJ3 - 0.5) * aVr2 * rVr2 / R2cut;
+        aLJ = (aLJ3 - 1.0 + 2.0 * bLJ2) * aLJ3 / (aVr2 * aVr2);
+        bLJ = -(aLJ3 - 1.0 + 3.0 * bLJ2) * aLJ3 / (aVr2 * aVr2 * aVr2);
+        aLJ3 = 3.0 * aLJ;
+        bLJ2 = 2.0 * bLJ;
+        aVr2 = -48.0 * aVr / rVr2;
+        aVr = 2.0 * aVr;
+    }
+    // V'(r)/r and V(r)/2 of one pair (LJs.cxx:49-56, :80-84)
+    void pair(double r2, double& FF, double& E) const {
+        double XX = rVr1 * r2 - X2min;
+        double FV = rVr2 / r2;
+        double Ur = FV * FV * FV;
+        FF = aVr2 * (Ur * (Ur - 0.5) * FV + (aLJ + bLJ * XX) * XX);
+        E = aVr * (Ur * (Ur - 1.0) - (aLJ3 + bLJ2 * XX) * XX * XX);
+    }
+    void force(Atom& atom, Atom* neibs, const size_t* list, size_t nl) override { pair_force(*this, atom, neibs, list, nl); }
+    void virial(Atom& atom, Atom* neibs, const size_t* list, size_t nl) override { pair_virial(*this, atom, neibs, list, nl); }
+};
+
+// Tabulated Pair (mdcraft/solver/potential/pair.cxx): r*phi on Nr1 knots,
+// EAM's spline (init_spline with ib = 1); the knot past the end repeats the
+// last one, as the reference's read of Vpar[N1 + 1] at N1 = Nr1 - 1 assumes.
+struct TabPair : Potential {
+    int Nr1 = 0;
+    double dr = 0, dr1 = 0;
+    std::vector<double> Vpar, dVpar;
+    TabPair(double rcut, const double* v, int nr, double step) : Potential(PT_PAIR) {
+        R1cut = rcut;
+        R2cut = rcut * rcut;
+        Nr1 = nr;
+        dr = step;
+        dr1 = 1.0 / dr;
+        Vpar.assign(v, v + nr);
+        dVpar.assign(nr, 0.0);
+        EAM::init_spline(Vpar.data(), dVpar.data(), dr1, nr, 1);
+        Vpar.push_back(Vpar.back());
+        dVpar.push_back(dVpar.back());
+    }
+    // pair.cxx:59-149, :326-366: V'(r)/r and V(r)/2
+    void pair(double R2, double& FF, double& E) const {
+        double R1 = std::sqrt(R2), RN = R1 * dr1, VR, DV;
+        int N1 = (int)std::trunc(RN);
+        if (N1 < Nr1) {
+            VR = N1;
+            DV = RN - VR;
+            RN = R1 - VR * dr;
+            VR = (Vpar[N1 + 1] - Vpar[N1]) * dr1;
+            double aa = (dVpar[N1 + 1] - VR) - (VR - dVpar[N1]);
+            double bb = aa - (VR - dVpar[N1]);
+            aa = aa * DV;
+            VR = Vpar[N1] + RN * (dVpar[N1] + DV * (aa - bb));
+            DV = dVpar[N1] + 2.0 * DV * (1.5 * aa - bb);
+        } else {
+            VR = 0.0;
+            DV = 0.0;
+        }
+        FF = ((DV * R1 - VR) / R2) / R1;
+        E = 0.5 * VR / R1;
+    }
+    void force(Atom& atom, Atom* neibs, const size_t* list, size_t nl) override { pair_force(*this, atom, neibs, list, nl); }
+    void virial(Atom& atom, Atom* neibs, const size_t* list, size_t nl) override { pair_virial(*this, atom, neibs, list, nl); }
 };
 
 // solver::Multi (mdcraft/solver/multi.cxx:11-151): one potential object per
@@ -1585,6 +1722,14 @@ void* ref_bpnn_create(const char* path, const ref_domain* d) {
     }
 }
 
+void* ref_ljs_create(double aVr, double rVr, double rcut) { return new LJs(aVr, rVr, rcut); }
+void* ref_pair_create(double rcut, const double* rphi, int nr, double dr) {
+    if (nr < 3) {
+        g_err = "Pair tables need at least 3 knots";
+        return nullptr;
+    }
+    return new TabPair(rcut, rphi, nr, dr);
+}
 void* ref_multi_create(int nkinds) {
     if (nkinds < 1 || nkinds > 8) {
         g_err = "Multi kinds must be 1..8";

@@ -37,6 +37,8 @@ def _load():
         "ref_eam_create": (P, [D, Z, Z, Z, I, D, I, D]),
         "ref_eam_create_file": (P, [C.c_char_p]),
         "ref_bpnn_create": (P, [C.c_char_p, dom]),
+        "ref_ljs_create": (P, [D, D, D]),
+        "ref_pair_create": (P, [D, Z, I, D]),
         "ref_multi_create": (P, [I]),
         "ref_multi_add": (I, [P, I, I, P]),
         "ref_pot_free": (None, [P]),
@@ -150,6 +152,15 @@ class Potential:
 
 def LJsCut(a, s, rcut):
     return Potential(lib.ref_lj_create(a, s, rcut))
+
+
+def LJs(aVr, rVr, rcut):
+    return Potential(lib.ref_ljs_create(aVr, rVr, rcut))
+
+
+def Pair(rc, rphi, dr):
+    rphi = np.ascontiguousarray(rphi, dtype=np.float64)
+    return Potential(lib.ref_pair_create(rc, _dp(rphi), len(rphi), dr))
 
 
 def EAM(rc=None, F=None, rho=None, rphi=None, dr=None, drho=None, path=None):

@@ -20,6 +20,10 @@ def oracle_potential(R, pot, box=None):
         if "path" in pot:
             return R.EAM(path=pot["path"])
         return R.EAM(rc=pot["rc"], F=pot["F"], rho=pot["rho"], rphi=pot["rphi"], dr=pot["dr"], drho=pot["drho"])
+    if k == "ljs":
+        return R.LJs(pot["aVr"], pot["rVr"], pot["rc"])
+    if k == "pair":
+        return R.Pair(pot["rc"], pot["rphi"], pot["dr"])
     if k == "bpnn":
         return R.MaiseBPNN(pot["path"], R.Domain(*box))
     if k == "eam_multi":
@@ -38,6 +42,10 @@ def engine_potential(E, ctx, pot):
                      drho=pot["drho"])
     if k == "eam_multi":
         return E.Multi(ctx, pot["nkinds"], [(i, j, engine_potential(E, ctx, q)) for i, j, q in pot["pairs"]])
+    if k == "ljs":
+        return E.LJs(ctx, pot["aVr"], pot["rVr"], pot["rc"])
+    if k == "pair":
+        return E.Pair(ctx, rcut=pot["rc"], rphi=pot["rphi"], dr=pot["dr"])
     if k == "bpnn":
         return E.MaiseBPNNPotential(ctx, pot["path"])
     raise ValueError(k)

@@ -1,0 +1,85 @@
+"""LJs and the tabulated Pair on the device against the oracle.
+
+Bars: FP64 within 1e-12 relative (max |difference| over the max
+magnitude); mixed precision force RMS within 1e-5 and energy sums within
+1e-6; a 30-step FP64 trajectory within 1e-9 nm."""
+import numpy as np
+import pytest
+
+from mdcraft_amd import workloads as W
+from mdcraft_amd.atoms import mat, vec
+from tests.helpers import engine_potential, force_rms_rel, max_rel, oracle_compute, oracle_mod, oracle_potential
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+MIXED_FORCE_RMS = 1e-5
+MIXED_ENERGY = 1e-6
+
+
+def E():
+    from mdcraft_amd import engine
+    return engine
+
+
+def workload(kind):
+    if kind == "ljs":
+        w = W.lj_argon(reps=5, rattle=0.1)
+        sigma = w["pot"]["s"]
+        w["pot"] = {"kind": "ljs", "aVr": w["pot"]["a"], "rVr": sigma, "rc": 2.5 * sigma}
+        return w
+    w = W.eam_cu(reps=5, rattle=0.05)
+    t = w["pot"]
+    w["pot"] = {"kind": "pair", "rc": t["rc"], "rphi": t["rphi"], "dr": t["dr"]}
+    return w
+
+
+def run(ctx, w, precision, lanes):
+    eng = E()
+    s = eng.System(ctx, [len(w["atoms"])], [w["box"]], lanes=lanes)
+    s.upload(w["atoms"])
+    p = engine_potential(eng, ctx, w["pot"])
+    s.build()
+    s.compute(p, 7, precision)
+    return s.download(), s, p
+
+
+@pytest.mark.parametrize("kind", ["ljs", "pair"])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_single_point(ctx, kind, precision):
+    w = workload(kind)
+    ref, _ = oracle_compute(w["atoms"], w["box"], w["pot"])
+    for lanes in (1, 8, 16):
+        got, _, _ = run(ctx, w, precision, lanes)
+        if precision == 0:
+            assert max_rel(vec(got, "f"), vec(ref, "f")) <= FP64_TOL, (kind, lanes)
+            assert max_rel(got["Ep"], ref["Ep"]) <= FP64_TOL, (kind, lanes)
+            assert max_rel(mat(got), mat(ref)) <= FP64_TOL, (kind, lanes)
+        else:
+            assert force_rms_rel(vec(got, "f"), vec(ref, "f")) <= MIXED_FORCE_RMS, (kind, lanes)
+            assert abs(got["Ep"].sum() - ref["Ep"].sum()) <= MIXED_ENERGY * abs(ref["Ep"].sum()), (kind, lanes)
+
+
+@pytest.mark.parametrize("kind", ["ljs", "pair"])
+def test_nve_fp64(ctx, kind):
+    w = workload(kind)
+    dt, steps = w["dt"], 30
+    got_at, s, p = run(ctx, w, 0, None)
+    s.step(p, dt, steps, 0)
+    got = s.download()
+    R = oracle_mod()
+    a = w["atoms"].copy()
+    dom = R.Domain(*w["box"])
+    pool = R.Pool(4)
+    op = oracle_potential(R, w["pot"])
+    nl = R.VerletList(pool)
+    nl.update(a, dom)
+    sol = R.Solver(op, pool, True)
+    sol.prepare(a, dom, nl)
+    sol.stages(a, dom, nl)
+    sol.run_nve(a, dom, nl, dt, steps, initial_forces=False)
+    L = w["box"][1] - w["box"][0]
+    d = vec(got, "r") - vec(a, "r")
+    d -= np.round(d / L) * L
+    assert np.max(np.abs(d)) < 1e-9
+    assert max_rel(vec(got, "v"), vec(a, "v")) < 1e-8

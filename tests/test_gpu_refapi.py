@@ -310,3 +310,34 @@ def test_multi_nial_stages():
         getattr(solver, name)(atoms, atoms, nlist.get())
         getattr(o, name)()
         compare(atoms.data, o.a, CALL_TOL, name, fields=("f", "Ep", "Em", "nc", "V"))
+
+
+@pytest.mark.parametrize("kind", ["ljs", "pair"])
+def test_ljs_and_tabulated_pair(kind):
+    """LJs(aVr, rVr, Rcutoff) and Pair(Rcutoff, pairPotential, NrPot,
+    drPot) through mdcraft.solver.potential, one prepare/forces/virials."""
+    m = api()
+    if kind == "ljs":
+        w = W.lj_argon(reps=4, rattle=0.1)
+        s = w["pot"]["s"]
+        spec = {"kind": "ljs", "aVr": w["pot"]["a"], "rVr": s, "rc": 2.5 * s}
+        potential = m.solver.potential.LJs(aVr=spec["aVr"], rVr=s, Rcutoff=spec["rc"])
+    else:
+        w = W.eam_cu(reps=4, rattle=0.05)
+        t = w["pot"]
+        spec = {"kind": "pair", "rc": t["rc"], "rphi": t["rphi"], "dr": t["dr"]}
+        potential = m.solver.potential.Pair(Rcutoff=t["rc"], pairPotential=t["rphi"], NrPot=len(t["rphi"]),
+                                            drPot=t["dr"])
+    at, box = w["atoms"], w["box"]
+    atoms = m.data.Atoms(len(at))
+    atoms.data[:] = at
+    domain = make_domain(m, box)
+    solver = m.solver.Single(potential, m.solver.boundary.Periodic(domain))
+    nlist = m.neibs.VerletList(atoms, atoms, domain)
+    nlist.update()
+    o = OracleScript(at, box, spec)
+    o.update()
+    for name in ("prepare", "forces", "virials"):
+        getattr(solver, name)(atoms, atoms, nlist.get())
+        getattr(o, name)()
+        compare(atoms.data, o.a, CALL_TOL, name, fields=("f", "Ep", "V"))
