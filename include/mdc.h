@@ -79,7 +79,9 @@ mdc_status mdc_pot_eam_create(mdc_ctx* ctx, double rcut, const double* F, const 
  * solver/potential/EAM.cxx:55-87 */
 mdc_status mdc_pot_eam_create_file(mdc_ctx* ctx, const char* setfl_path, mdc_pot** out);
 /* solver::Multi(nkinds) + add_potential(i, j, pot) for each pair of kinds
- * (solver/multi.cxx:11-151), over single-element EAM objects: npots entries
+ * (solver/multi.cxx:11-151), over single-element EAM objects and tabulated
+ * Pair potentials (a Pair member adds no density and no embedding term, and
+ * leaves Em, as pair.cxx's virial does): npots entries
  * (kind_i[q], kind_j[q], pots[q]) that together cover every pair; a later
  * entry for the same pair replaces an earlier one, as add_potential does.
  * Atom i's list is split by neighbour kind and the object of (kind_i,

@@ -235,8 +235,10 @@ Pot* make_multi(Ctx* ctx, int nk, int npots, const int* ki, const int* kj, const
     for (int q = 0; q < npots; ++q) {
         if (ki[q] < 0 || kj[q] < 0 || ki[q] >= nk || kj[q] >= nk) throw Error(E_INVAL, "Multi: kind out of range");
         const Pot* s = pots[q];
-        if (!s || s->type != PT_EAM || s->eam.nk != 1)
-            throw Error(E_INVAL, "Multi: every member must be a single-element EAM potential");
+        const bool eam1 = s && s->type == PT_EAM && s->eam.nk == 1;
+        const bool tab = s && s->type == PT_LJ && s->lj.fn == 2;
+        if (!eam1 && !tab)
+            throw Error(E_INVAL, "Multi: every member must be a single-element EAM or a tabulated Pair potential");
         obj[pair_index(ki[q], kj[q])] = s;  // a later add_potential replaces an earlier one
     }
     for (auto* o : obj)
@@ -250,22 +252,28 @@ Pot* make_multi(Ctx* ctx, int nk, int npots, const int* ki, const int* kj, const
     std::vector<EamObj> h(npair);
     size_t nrec = 0, nF = 0;
     double rc = 0;
+    // A tabulated Pair member becomes an object whose density records are
+    // zero (rho' = 0: the pair term alone acts) and whose embedding table
+    // is three zero knots on a step so wide that every density falls in
+    // its first interval (F = F' = 0).
+    const int kZeroF = 3;
     for (int q = 0; q < npair; ++q) {
         const EamTables& S = obj[q]->eam;
+        const bool tab = obj[q]->type == PT_LJ;
         EamObj& o = h[q];
         o.rho_off = (int)nrec;
         o.phi_off = (int)(nrec + S.nr);
         o.F_off = (int)nF;
         o.nr = S.nr;
-        o.nrho = S.nrho;
-        o.pad = 0;
+        o.nrho = tab ? kZeroF : S.nrho;
+        o.pair_only = tab ? 1 : 0;
         o.dr = S.dr;
         o.dr1 = 1.0 / S.dr;
-        o.drho = S.drho;
-        o.drho1 = 1.0 / S.drho;
+        o.drho = tab ? 1e300 : S.drho;
+        o.drho1 = 1.0 / o.drho;
         o.rc2 = S.rc * S.rc;
         nrec += 2 * (size_t)S.nr;
-        nF += (size_t)S.nrho;
+        nF += (size_t)o.nrho;
         rc = std::max(rc, S.rc);
     }
     if (nrec >= (1ull << 27)) throw Error(E_INVAL, "Multi: tables too large");
@@ -275,6 +283,14 @@ Pot* make_multi(Ctx* ctx, int nk, int npots, const int* ki, const int* kj, const
     T.objs = dalloc<EamObj>(npair);
     for (int q = 0; q < npair; ++q) {
         const EamTables& S = obj[q]->eam;
+        if (h[q].pair_only) {
+            HIPCHECK(hipMemset(T.r64 + h[q].rho_off, 0, sizeof(double4) * S.nr));
+            HIPCHECK(hipMemset(T.r32 + h[q].rho_off, 0, sizeof(float4) * S.nr));
+            HIPCHECK(hipMemcpy(T.r64 + h[q].phi_off, S.r64, sizeof(double4) * S.nr, hipMemcpyDeviceToDevice));
+            HIPCHECK(hipMemcpy(T.r32 + h[q].phi_off, S.r32, sizeof(float4) * S.nr, hipMemcpyDeviceToDevice));
+            HIPCHECK(hipMemset(T.F64 + h[q].F_off, 0, sizeof(double4) * kZeroF));
+            continue;
+        }
         HIPCHECK(hipMemcpy(T.r64 + h[q].rho_off, S.r64, sizeof(double4) * 2 * S.nr, hipMemcpyDeviceToDevice));
         HIPCHECK(hipMemcpy(T.r32 + h[q].rho_off, S.r32, sizeof(float4) * 2 * S.nr, hipMemcpyDeviceToDevice));
         HIPCHECK(hipMemcpy(T.F64 + h[q].F_off, S.F64, sizeof(double4) * S.nrho, hipMemcpyDeviceToDevice));

@@ -85,7 +85,9 @@ struct LJParams {
 // (solver::Multi, multi.cxx:26-49): offsets of its rho and r*phi interval
 // records in the r-record array and of its F records, and its grids.
 struct EamObj {
-    int rho_off, phi_off, F_off, nr, nrho, pad;
+    // pair_only: a tabulated Pair member (no density, no embedding; its
+    // virial leaves Em, pair.cxx:235-324)
+    int rho_off, phi_off, F_off, nr, nrho, pair_only;
     double dr, dr1, drho, drho1, rc2;
 };
 constexpr int kMaxMultiKinds = 4;

@@ -323,8 +323,9 @@ __global__ void __launch_bounds__(256) k_eam_rho(const double4* __restrict__ pos
                 ob[u] = pair_index(ki, kj);
                 const EamObj& o = s_obj[ob[u]];
                 g[u] = pair_geo<R>(fr, pi, pj[u], e[u], okk[u], o.rc2, (R)o.dr1, o.nr);
-                if (okk[u]) kd = max(kd, kj);
-                if (g[u].ghost) kg = max(kg, kj);
+                // only an EAM object's virial sets Em (a Pair member's leaves it)
+                if (okk[u] && !o.pair_only) kd = max(kd, kj);
+                if (g[u].ghost && !o.pair_only) kg = max(kg, kj);
             }
         }
         typename Vec4T<R>::T rec[kU];
@@ -360,6 +361,8 @@ __global__ void __launch_bounds__(256) k_eam_rho(const double4* __restrict__ pos
         if (active && S.store_aux && (S.flags & W_ENERGY)) A.d[(size_t)D_EM * n + i] = F;
     } else {
         const int klast = kg >= 0 ? kg : kd;
+        // no EAM sub-list at all: Em stays as prepare left it (zero)
+        if (klast < 0 && active && S.store_aux && (S.flags & W_ENERGY)) A.d[(size_t)D_EM * n + i] = 0.0;
         for (int k = 0; k < P.nk; ++k) {
             const EamObj& o = s_obj[pair_index(ki, k)];
             double F, dF;

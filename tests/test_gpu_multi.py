@@ -161,3 +161,48 @@ def test_batch_small_boxes_rebuild_every_step_fp64(ctx):
         for k in (0, 1, 2, 4):
             assert np.allclose(hist[:, q, k], ref[:, k], rtol=1e-11, atol=1e-9), (q, k)
         off += len(a)
+
+
+def _pair_member_spec(cross):
+    ni_al = W.nial_pots()
+    like = {(i, j): q for i, j, q in ni_al["pairs"] if i == j}
+    return {"kind": "eam_multi", "nkinds": 2, "pairs": [(0, 0, like[(0, 0)]), (1, 1, like[(1, 1)]), (0, 1, cross)]}
+
+
+def _write_pair_file(path, rphi, dr, rc):
+    """The reference's Pair file layout (pair.cxx:36-56): four lines, then
+    "Nr dr rcut" in Angstrom and Nr values of r*phi in eV*Angstrom."""
+    Na, Qe = 6.02214076e+23, 1.602176634e-19
+    UnEnrg = (1e-3 / Na) * (1e-9 / 1e-12) * (1e-9 / 1e-12)
+    conv = Qe / (10 * UnEnrg)
+    vals = np.asarray(rphi) / conv
+    with open(path, "w") as f:
+        f.write(" synthetic cross pair\n line 2\n line 3\n 2 Ni Al\n")
+        f.write("%d %r %r\n" % (len(vals), dr * 10, rc * 10))
+        for v in vals:
+            f.write("%r\n" % float(v))
+    back = np.array([float("%r" % float(v)) for v in vals]) * conv
+    return back
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_multi_with_tabulated_pair_member(ctx, precision, tmp_path):
+    """problems/NiAl/test-NiAl.py builds Multi from EAM objects for like
+    kinds and a tabulated Pair read from a file for the cross kinds."""
+    t = W.eam_cu_tables()
+    path = str(tmp_path / "NiAlpair.txt")
+    rphi = _write_pair_file(path, 0.9 * t["rphi"], t["dr"], t["rc"])
+    oracle_spec = _pair_member_spec({"kind": "pair", "rc": t["rc"], "rphi": rphi, "dr": t["dr"]})
+    eng = E()
+    cross = eng.Pair(ctx, path=path)
+    assert cross.rcut == pytest.approx(t["rc"], rel=1e-15)
+    for idx, target in ((3, 500), (8, 32)):
+        at, box = W.nial_cell(idx, target)
+        ref, _ = oracle_compute(at, box, oracle_spec)
+        like = {(i, j): engine_potential(eng, ctx, q) for i, j, q in oracle_spec["pairs"] if i == j}
+        p = eng.Multi(ctx, 2, [(0, 0, like[(0, 0)]), (1, 1, like[(1, 1)]), (0, 1, cross)])
+        s = eng.System(ctx, [len(at)], [box])
+        s.upload(at)
+        s.build()
+        s.compute(p, 7, precision)
+        compare(s.download(), ref, precision, f"pair-member cell{idx}")

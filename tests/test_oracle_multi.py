@@ -111,3 +111,43 @@ def test_multi_rejects_unknown_kind():
     at, box = cluster([[0, 0, 0], [0.25, 0, 0]], [0, 2])
     with pytest.raises(RuntimeError, match="kind"):
         oracle_compute(at, box, W.nial_pots(), periodic_boundary=False)
+
+
+def pair_member_pots(scale=0.8):
+    """The reference's NiAl set-up (problems/NiAl/test-NiAl.py:37-39,
+    :120-122): EAM objects for like kinds and a tabulated Pair for the
+    cross kinds; here the Pair samples scale * r*phi of the analytic EAM."""
+    t = W.eam_cu_tables()
+    eam = dict(kind="eam", **t)
+    pair = {"kind": "pair", "rc": t["rc"], "rphi": scale * t["rphi"], "dr": t["dr"]}
+    return {"kind": "eam_multi", "nkinds": 2, "pairs": [(0, 0, eam), (1, 1, eam), (0, 1, pair)]}
+
+
+def test_multi_pair_member_trimer_closed_form():
+    """Cross-kind pairs under a Pair member give no density and only their
+    pair term; like-kind pairs keep the EAM object's terms; Em comes from
+    the like-kind EAM object (a Pair's virial leaves Em)."""
+    pos = np.array([[0.0, 0.0, 0.0], [0.25, 0.0, 0.0], [0.0, 0.26, 0.0]])
+    kinds = [0, 0, 1]
+    at, box = cluster(pos, kinds)
+    got, _ = oracle_compute(at, box, pair_member_pots(), periodic_boundary=False)
+    d = lambda a, b: float(np.linalg.norm(pos[a] - pos[b]))
+    nc = np.array([rho_a(d(0, 1), 1.0), rho_a(d(0, 1), 1.0), 0.0])
+    assert got["nc"] == pytest.approx(nc, rel=CLOSED_FORM_RTOL, abs=1e-300)
+    f = np.zeros((3, 3))
+    for a in range(3):
+        for b in range(3):
+            if a == b:
+                continue
+            r = d(a, b)
+            if kinds[a] == kinds[b]:
+                FF = (dF_a(nc[a], 1.0) + dF_a(nc[b], 1.0)) * deriv(lambda x: rho_a(x, 1.0), r) + deriv(phi_a, r)
+            else:
+                FF = 0.8 * deriv(phi_a, r)
+            f[a] += FF * (pos[b] - pos[a]) / r
+    assert np.allclose(vec(got, "f"), f, rtol=CLOSED_FORM_RTOL, atol=CLOSED_FORM_RTOL * np.abs(f).max())
+    assert got["Em"][:2] == pytest.approx([F_a(nc[0], 1.0), F_a(nc[1], 1.0)], rel=CLOSED_FORM_RTOL)
+    assert got["Em"][2] == 0.0  # F(0) of the like-kind object of its self sub-list
+    Ep = [0.5 * (phi_a(d(0, 1)) + 0.8 * phi_a(d(0, 2))), 0.5 * (phi_a(d(0, 1)) + 0.8 * phi_a(d(1, 2))),
+          0.5 * 0.8 * (phi_a(d(0, 2)) + phi_a(d(1, 2)))]
+    assert got["Ep"] == pytest.approx(Ep, rel=CLOSED_FORM_RTOL)
