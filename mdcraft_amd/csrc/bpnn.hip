@@ -312,14 +312,20 @@ __global__ void __launch_bounds__(256) k_bp_desc(const double4* __restrict__ pos
     }
 }
 
-// pass 2 (FP64): one thread per atom, vector units
-__global__ void __launch_bounds__(128) k_bp_mlp64(const double* __restrict__ G, int ldg, const double* __restrict__ W,
+// pass 2 (FP64): L lanes per atom (32 when the input and hidden widths fit,
+// else 64), lane q holding unit q of each layer; activations move between
+// lanes by shuffles, the weights are read row- or column-wise so a group's
+// reads are contiguous (W01T and W12T are transposed copies).
+template <int L>
+__global__ void __launch_bounds__(256) k_bp_mlp64(const double* __restrict__ G, int ldg, const double* __restrict__ W,
                                                   const BpDesc* __restrict__ Pd, const double4* __restrict__ pos,
                                                   double* __restrict__ wout, double* __restrict__ Ep, int n,
                                                   StepArgs SA, int* __restrict__ misc) {
     if (skip_step(SA, misc)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const int lane = threadIdx.x & 63;
+    const int sub = lane & (L - 1), base = lane - sub;
+    const int i = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) / L);
+    const bool valid = i < n;
     const BpDesc& P = *Pd;
     const int nin = P.nin, h1 = P.h1, h2 = P.h2;
     const double* W01 = W;
@@ -328,38 +334,44 @@ __global__ void __launch_bounds__(128) k_bp_mlp64(const double* __restrict__ G, 
     const double* b12 = W12 + h2 * h1;
     const double* W23 = b12 + h2;
     const double b23 = W23[h2];
-    double* wo = wout + (size_t)i * ldg;
-    if (w_inactive(pos[i].w)) {
-        for (int q = 0; q < ldg; ++q) wo[q] = 0.0;
-        return;
+    const double* W01T = W23 + h2 + 1;  // [nin][h1]
+    const double* W12T = W01T + nin * h1;  // [h1][h2]
+    const bool act = valid && !w_inactive(pos[i].w);
+    const double* g = G + (size_t)(valid ? i : 0) * ldg;
+    // every lane of the wave takes part in every shuffle below
+    double y1 = 0.0;
+    if (act && sub < h1) {
+        double s = 0.0;
+        for (int k = 0; k < nin; ++k) s += W01T[k * h1 + sub] * g[k];
+        y1 = tanh(s + b01[sub]);
     }
-    const double* g = G + (size_t)i * ldg;
-    double y1[kBpMaxHidden], y2[kBpMaxHidden], d1[kBpMaxHidden];
-    for (int q = 0; q < h1; ++q) {
-        double sacc = 0.0;
-        for (int k = 0; k < nin; ++k) sacc += W01[q * nin + k] * g[k];
-        y1[q] = tanh(sacc + b01[q]);
-    }
-    double E = b23;
-    for (int q = 0; q < h2; ++q) {
-        double sacc = 0.0;
-        for (int p = 0; p < h1; ++p) sacc += W12[q * h1 + p] * y1[p];
-        y2[q] = tanh(sacc + b12[q]);
-        E += W23[q] * y2[q];
-    }
-    for (int q = 0; q < h2; ++q) y2[q] = W23[q] * (1.0 - y2[q] * y2[q]);
+    double s2 = 0.0;
     for (int p = 0; p < h1; ++p) {
-        double sacc = 0.0;
-        for (int q = 0; q < h2; ++q) sacc += W12[q * h1 + p] * y2[q];
-        d1[p] = sacc * (1.0 - y1[p] * y1[p]);
+        const double yp = __shfl(y1, base + p, kWave);
+        if (sub < h2) s2 += W12T[p * h2 + sub] * yp;
     }
-    for (int k = 0; k < nin; ++k) {
-        double sacc = 0.0;
-        for (int q = 0; q < h1; ++q) sacc += W01[q * nin + k] * d1[q];
-        wo[k] = sacc * P.escale;
+    double t = 0.0, e = 0.0;
+    if (act && sub < h2) {
+        const double y2 = tanh(s2 + b12[sub]);
+        e = W23[sub] * y2;
+        t = W23[sub] * (1.0 - y2 * y2);
     }
-    for (int k = nin; k < ldg; ++k) wo[k] = 0.0;
-    Ep[i] = E * P.escale;
+    for (int o = L >> 1; o > 0; o >>= 1) e += __shfl_xor(e, o, kWave);
+    double s1 = 0.0;
+    for (int q = 0; q < h2; ++q) {
+        const double tq = __shfl(t, base + q, kWave);
+        if (sub < h1) s1 += W12[q * h1 + sub] * tq;
+    }
+    const double d1 = sub < h1 ? s1 * (1.0 - y1 * y1) : 0.0;
+    double so = 0.0;
+    for (int q = 0; q < h1; ++q) {
+        const double dq = __shfl(d1, base + q, kWave);
+        if (sub < nin) so += W01[q * nin + sub] * dq;
+    }
+    if (!valid) return;
+    double* wo = wout + (size_t)i * ldg;
+    for (int k = sub; k < ldg; k += L) wo[k] = (act && k < nin) ? so * P.escale : 0.0;
+    if (act && sub == 0) Ep[i] = (b23 + e) * P.escale;
 }
 
 // pass 2 (mixed): one wavefront per 16 atoms on v_mfma_f32_16x16x16_f16.
@@ -697,23 +709,29 @@ __global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos
     }
 }
 
-// reverse slot index: rev(m, k) = slot of m in the list of its k-th neighbour
+// reverse slot index: rev(m, k) = slot of m in the list of its k-th
+// neighbour i (-1 when a capacity cut left m out of i's list).  One thread
+// per (m, k) searches i's list, four slots per step so the reads overlap.
 __global__ void k_bp_rev(const int* __restrict__ nl, const int* __restrict__ nn, int* __restrict__ rev, int tpa, int M,
                          int cap, int n) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = (int)(t / cap), k = (int)(t % cap);
     if (m >= n) return;
     const int cm = min(nn[m], cap);
-    for (int k = 0; k < cm; ++k) {
-        const int i = ent_j(nl[list_index(m, k, tpa, M)]);
-        const int ci = min(nn[i], cap);
-        int r = -1;
-        for (int q = 0; q < ci; ++q)
-            if (ent_j(nl[list_index(i, q, tpa, M)]) == m) {
-                r = q;
-                break;
-            }
-        rev[list_index(m, k, tpa, M)] = r;
+    if (k >= cm) return;
+    const long long lk = list_index(m, k, tpa, M);
+    const int i = ent_j(nl[lk]);
+    const int ci = min(nn[i], cap);
+    int r = -1;
+    for (int q0 = 0; q0 < ci && r < 0; q0 += 4) {
+        int v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = q0 + u < ci ? ent_j(nl[list_index(i, q0 + u, tpa, M)]) : -1;
+#pragma unroll
+        for (int u = 3; u >= 0; --u)
+            if (v[u] == m) r = q0 + u;
     }
+    rev[lk] = r;
 }
 
 // pass 4: gather the force shares, then the fused velocity-Verlet update
@@ -967,6 +985,11 @@ Pot* bpnn_create(Ctx* ctx, const char* path) {
     std::vector<double> W64;
     for (auto* v : {&W01, &b01, &W12, &b12, &W23}) W64.insert(W64.end(), v->begin(), v->end());
     W64.push_back(b23[0]);
+    // transposed copies for the FP64 network's column reads
+    for (int k = 0; k < nin; ++k)
+        for (int q = 0; q < h1; ++q) W64.push_back(W01[(size_t)q * nin + k]);
+    for (int p = 0; p < h1; ++p)
+        for (int q = 0; q < h2; ++q) W64.push_back(W12[(size_t)q * h1 + p]);
     std::vector<float> W32;
     for (auto* v : {&b01, &b12, &W23}) W32.insert(W32.end(), v->begin(), v->end());
     W32.push_back((float)b23[0]);
@@ -1063,7 +1086,9 @@ void bpnn_compute(Batch& b, const Pot& p, int what, int prec, int mode, double d
     const int n = b.n;
     StepArgs S{dt, step, mode, what & (W_ENERGY | W_VIRIAL), mode == 0 ? 1 : 0};
     if (!w.rev_valid || w.rev_builds != b.rebuilds) {
-        hipLaunchKernelGGL(k_bp_rev, dim3((n + 255) / 256), dim3(256), 0, st, b.nl, b.nn, w.rev, b.tpa, b.M, b.cap, n);
+        const long long pairs = (long long)n * b.cap;
+        hipLaunchKernelGGL(k_bp_rev, dim3((unsigned)((pairs + 255) / 256)), dim3(256), 0, st, b.nl, b.nn, w.rev, b.tpa,
+                           b.M, b.cap, n);
         w.rev_valid = true;
         w.rev_builds = b.rebuilds;
     }
@@ -1085,10 +1110,13 @@ void bpnn_compute(Batch& b, const Pot& p, int what, int prec, int mode, double d
         if (parts & 1) {
             launch_k(b, K_BP_DESC, kdesc, gw, bw, (unsigned)lds, pos, b.nl, b.nn, b.tpa, b.M, b.cap, b.d_desc,
                      bp.d, (R*)w.G, ldg, n, S, b.misc);
-            if (sizeof(R) == 8)
-                launch_k(b, K_BP_MLP, k_bp_mlp64, dim3((n + 127) / 128), dim3(128), 0, (const double*)w.G, ldg,
-                         bp.w64, bp.d, pos, (double*)w.w, Ep, n, S, b.misc);
-            else
+            if (sizeof(R) == 8) {
+                const bool narrow = std::max(bp.h.nin, std::max(bp.h.h1, bp.h.h2)) <= 32;
+                const int L = narrow ? 32 : 64;
+                const dim3 grid((unsigned)(((long long)n * L + 255) / 256));
+                launch_k(b, K_BP_MLP, narrow ? k_bp_mlp64<32> : k_bp_mlp64<64>, grid, dim3(256), 0,
+                         (const double*)w.G, ldg, bp.w64, bp.d, pos, (double*)w.w, Ep, n, S, b.misc);
+            } else
                 launch_k(b, K_BP_MLP, k_bp_mlp_mfma, dim3((n + 15) / 16), dim3(64), 0, (const float*)w.G, ldg,
                          bp.bsplit, boff, nts, kts, bp.w32, bp.d, pos, (float*)w.w, Ep, n, S, b.misc);
         }
