@@ -83,3 +83,33 @@ def test_nve_fp64(ctx, kind):
     d -= np.round(d / L) * L
     assert np.max(np.abs(d)) < 1e-9
     assert max_rel(vec(got, "v"), vec(a, "v")) < 1e-8
+
+
+def test_checkpoint_and_resume(ctx, tmp_path):
+    """A resident run saved to a snapshot after 15 steps and resumed in a
+    new System reaches the uninterrupted run's state after 30 (FP64; the
+    resumed run rebuilds its list at once, which moves rebuild steps but
+    not the physics of boxes above 2 (rns + skin))."""
+    from mdcraft_amd import snapshot
+    eng = E()
+    w = W.eam_cu(reps=5, rattle=0.05)
+    p = engine_potential(eng, ctx, w["pot"])
+    ref = eng.System(ctx, [len(w["atoms"])], [w["box"]])
+    ref.upload(w["atoms"])
+    ref.build()
+    ref.compute(p, 1, 0)
+    ref.step(p, w["dt"], 15, 0)
+    path = str(tmp_path / "run000015.h5")
+    snapshot.save_system(path, ref, [w["box"]], cycles=np.array([15]))
+    ref.step(p, w["dt"], 15, 0)
+    want = ref.download()
+    s, atoms, boxes, extra = snapshot.load_system(path, eng, ctx)
+    assert int(extra["cycles"][0]) == 15 and len(boxes) == 1
+    s.build()
+    s.step(p, w["dt"], 15, 0)
+    got = s.download()
+    L = w["box"][1] - w["box"][0]
+    d = vec(got, "r") - vec(want, "r")
+    d -= np.round(d / L) * L
+    assert np.max(np.abs(d)) < 1e-9
+    assert max_rel(vec(got, "v"), vec(want, "v")) < 1e-8
