@@ -366,6 +366,7 @@ void free_batch(Batch& b) {
     dfree(b.orig[0]);
     dfree(b.orig[1]);
     dfree(b.rsave);
+    dfree(b.posF);
     dfree(b.where);
     dfree(b.cell_count);
     dfree(b.cell_start);
@@ -950,6 +951,7 @@ mdc_status mdc_sys_create(mdc_ctx* ctx, int nsys, const uint64_t* natoms, const 
                 HIPCHECK(hipMemset(b.fst[k], 0, sizeof(float) * F_NFIELDS * std::max(n, 1)));
             }
             b.rsave = dalloc<double4>(n);
+            b.posF = dalloc<double4>(n);
             b.where = dalloc<int>(n);
             b.atom_cell = dalloc<int>(n);
             b.sorted_src = dalloc<int>(n);

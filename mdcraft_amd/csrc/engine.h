@@ -144,6 +144,9 @@ struct Batch {
     int* orig[2] = {nullptr, nullptr};
     int sc = 0;  // which dst/fst/orig copy is current
     double4* rsave = nullptr;
+    // single-element EAM: each atom's position with F'(nc) in w, written by
+    // the density pass so the force pass gathers both in one record
+    double4* posF = nullptr;
     int* where = nullptr;  // original index -> sorted index
     // cells
     int ncells = 0;

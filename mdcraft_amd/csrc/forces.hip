@@ -163,6 +163,7 @@ struct EamK {
     const double4* F64;
     const EamObj* objs;  // Multi: objects by pair_index(kind_i, kind_j)
     double* fpm;         // Multi: F' of every atom under the object of its kind and kind k, [nk][n]
+    double4* posF;       // one element: (x, y, z, F'(nc)) of every atom, [n]
 };
 
 template <typename R>
@@ -358,6 +359,7 @@ __global__ void __launch_bounds__(256) k_eam_rho(const double4* __restrict__ pos
         double F, dF;
         embed_at(P.F64, 0, P.nrho, P.drho, P.drho1, rr, F, dF, misc);
         A.d[(size_t)D_FP * n + i] = dF;
+        P.posF[i] = make_double4(pi.x, pi.y, pi.z, dF);
         if (active && S.store_aux && (S.flags & W_ENERGY)) A.d[(size_t)D_EM * n + i] = F;
     } else {
         const int klast = kg >= 0 ? kg : kd;
@@ -410,7 +412,8 @@ __global__ void __launch_bounds__(256) k_eam_force(const double4* __restrict__ p
     const R dr = (R)P.dr, dr1 = (R)P.dr1;
     R fx = 0, fy = 0, fz = 0;
     double en = 0, vxx = 0, vyy = 0, vzz = 0, vxy = 0, vxz = 0, vyz = 0;
-    const __amdgpu_buffer_rsrc_t posb = make_buf(pos, (unsigned)n * 32u);
+    // one element: neighbour position and F' come in one record (posF)
+    const __amdgpu_buffer_rsrc_t posb = make_buf(MULTI ? (const void*)pos : (const void*)P.posF, (unsigned)n * 32u);
     const __amdgpu_buffer_rsrc_t fpb = make_buf(Fp, (unsigned)(MULTI ? P.nk : 1) * (unsigned)n * 8u);
     const __amdgpu_buffer_rsrc_t tabb = make_buf(tab, P.nrec * (unsigned)sizeof(*tab));
     const unsigned fp_base = (unsigned)ki * (unsigned)n;  // F' of neighbours under the objects of (k_j, kind_i)
@@ -431,7 +434,7 @@ __global__ void __launch_bounds__(256) k_eam_force(const double4* __restrict__ p
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             pj[u] = buf_d4(posb, (unsigned)ent_j(e[u]) * 32u);
-            Fpj[u] = (R)buf_d1(fpb, (fp_base + (unsigned)ent_j(e[u])) * 8u);
+            Fpj[u] = MULTI ? (R)buf_d1(fpb, (fp_base + (unsigned)ent_j(e[u])) * 8u) : (R)pj[u].w;
         }
         next_group<TPA, kU>(as, m0, M, nl, jn);
         PairGeo<R> g[kU];
@@ -727,6 +730,7 @@ static EamK eam_args(const Pot& p, Batch& b) {
         }
         k.fpm = b.fpm;
     }
+    k.posF = b.posF;
     return k;
 }
 
