@@ -49,6 +49,9 @@ __host__ __device__ inline int ent_j(int e) { return (int)((unsigned)e & kIdxMas
 __host__ __device__ inline int ent_s(int e, int axis) { return (int)(((unsigned)e >> (26 + 2 * axis)) & 3u) - 1; }
 
 // raw buffer loads: 32-bit offsets, out-of-range reads return zero
+// without touching memory; kSkip is beyond every buffer here, for lanes
+// whose value is not needed
+constexpr unsigned kSkip = 0x80000000u;
 typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
 typedef double d2_t __attribute__((ext_vector_type(2)));

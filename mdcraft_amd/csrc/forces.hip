@@ -172,6 +172,11 @@ template <>
 __device__ inline double4 load_rec<double>(__amdgpu_buffer_rsrc_t b, int rec) { return buf_d4(b, (unsigned)rec * 32u); }
 template <>
 __device__ inline float4 load_rec<float>(__amdgpu_buffer_rsrc_t b, int rec) { return buf_f4(b, (unsigned)rec * 16u); }
+// the record, or zeros without a memory access for a pair that is not used
+template <typename R>
+__device__ inline typename Vec4T<R>::T load_rec_if(__amdgpu_buffer_rsrc_t b, int rec, bool ok) {
+    return load_rec<R>(b, ok ? rec : (int)(kSkip / (unsigned)sizeof(typename Vec4T<R>::T)));
+}
 
 template <typename R>
 __device__ inline const typename Vec4T<R>::T* rtab(const EamK& P);
@@ -310,7 +315,7 @@ __global__ void __launch_bounds__(256) k_eam_rho(const double4* __restrict__ pos
         }
         double4 pj[kU];
 #pragma unroll
-        for (int u = 0; u < kU; ++u) pj[u] = buf_d4(posb, (unsigned)ent_j(e[u]) * 32u);
+        for (int u = 0; u < kU; ++u) pj[u] = buf_d4(posb, okk[u] ? (unsigned)ent_j(e[u]) * 32u : kSkip);
         next_group<TPA, kU>(as, m0, M, nl, jn);
         PairGeo<R> g[kU];
         int ob[kU];
@@ -331,7 +336,8 @@ __global__ void __launch_bounds__(256) k_eam_rho(const double4* __restrict__ pos
         }
         typename Vec4T<R>::T rec[kU];
 #pragma unroll
-        for (int u = 0; u < kU; ++u) rec[u] = load_rec<R>(tabb, (MULTI ? s_obj[ob[u]].rho_off : 0) + g[u].N1);
+        for (int u = 0; u < kU; ++u)
+            rec[u] = load_rec_if<R>(tabb, (MULTI ? s_obj[ob[u]].rho_off : 0) + g[u].N1, g[u].ok);
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             if (!g[u].ok) continue;
@@ -433,8 +439,8 @@ __global__ void __launch_bounds__(256) k_eam_force(const double4* __restrict__ p
         R Fpj[kU];
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
-            pj[u] = buf_d4(posb, (unsigned)ent_j(e[u]) * 32u);
-            Fpj[u] = MULTI ? (R)buf_d1(fpb, (fp_base + (unsigned)ent_j(e[u])) * 8u) : (R)pj[u].w;
+            pj[u] = buf_d4(posb, okk[u] ? (unsigned)ent_j(e[u]) * 32u : kSkip);
+            Fpj[u] = MULTI ? (R)buf_d1(fpb, okk[u] ? (fp_base + (unsigned)ent_j(e[u])) * 8u : kSkip) : (R)pj[u].w;
         }
         next_group<TPA, kU>(as, m0, M, nl, jn);
         PairGeo<R> g[kU];
@@ -455,12 +461,12 @@ __global__ void __launch_bounds__(256) k_eam_force(const double4* __restrict__ p
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             if (!MULTI) {
-                rr[u] = load_rec<R>(tabb, g[u].N1);
-                rp[u] = load_rec<R>(tabb, P.nr + g[u].N1);
+                rr[u] = load_rec_if<R>(tabb, g[u].N1, g[u].ok);
+                rp[u] = load_rec_if<R>(tabb, P.nr + g[u].N1, g[u].ok);
             } else {
                 const EamObj& o = s_obj[ob[u]];
-                rr[u] = load_rec<R>(tabb, o.rho_off + g[u].N1);
-                rp[u] = load_rec<R>(tabb, o.phi_off + g[u].N1);
+                rr[u] = load_rec_if<R>(tabb, o.rho_off + g[u].N1, g[u].ok);
+                rp[u] = load_rec_if<R>(tabb, o.phi_off + g[u].N1, g[u].ok);
             }
         }
 #pragma unroll

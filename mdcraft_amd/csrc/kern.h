@@ -219,7 +219,7 @@ __device__ inline void for_neighbours(const AtomSlots& as, int cnt, const int* _
         }
         double4 pj[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) pj[u] = buf_d4(posb, (unsigned)ent_j(e[u]) * 32u);
+        for (int u = 0; u < U; ++u) pj[u] = buf_d4(posb, ok[u] ? (unsigned)ent_j(e[u]) * 32u : kSkip);
 #pragma unroll
         for (int u = 0; u < U; ++u)
             if (ok[u]) body(e[u], pj[u]);
