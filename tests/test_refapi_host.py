@@ -191,3 +191,28 @@ def test_list_for_matches_oracle_grid():
                 if dr @ dr <= rr * rr:
                     want.append(j)
         assert g.list_for(q).tolist() == want
+
+
+def _atoms3(xs, rns=0.40):
+    a = atoms_at([[x, 0.10, 0.10] for x in xs], rns)
+    a.data["rcut"] = 0.40
+    a.data["m"] = 1.0
+    return a
+
+
+def test_list_for_known_answer():
+    # test/cxx/neibs/test_verlet_list.cxx:85-105
+    a = _atoms3([0.15, 0.55, 1.60])
+    d = A.Domain(0.0, 2.0, 0.0, 1.0, 0.0, 1.0)
+    g = A._Grid(d, a.data)  # the grid update() leaves for list_for
+    found = sorted(g.list_for(np.array([0.40, 0.10, 0.10])).tolist())
+    assert found == [0, 1]
+
+
+def test_sort_known_answer():
+    # test_verlet_list.cxx:107-123
+    a = _atoms3([1.60, 0.15, 0.95])
+    d = A.Domain(0.0, 2.0, 0.0, 1.0, 0.0, 1.0)
+    out = A.VerletList(a, a, d).sort(a)
+    x = out.data["r"]["x"].tolist()
+    assert len(x) == 3 and x[0] < x[1] < x[2]
