@@ -78,6 +78,21 @@ mdc_status mdc_pot_eam_create(mdc_ctx* ctx, double rcut, const double* F, const 
 /* EAM(filename), single-element setfl with eV/Angstrom conversion:
  * solver/potential/EAM.cxx:55-87 */
 mdc_status mdc_pot_eam_create_file(mdc_ctx* ctx, const char* setfl_path, mdc_pot** out);
+/* Alloy EAM of 1..4 elements (element e = atom kind e): atom i of element a
+ * sums rho_b(r) over neighbours of element b, embeds with F_a, and each
+ * pair of elements (a, b) adds r*phi_ab(r)/r.  Tables in reference units,
+ * rows concatenated: F [nelements][nrho], rho [nelements][nr], rphi
+ * [nelements*(nelements+1)/2][nr] in the setfl pair order (1,1), (2,1),
+ * (2,2), (3,1), ...  The reference has no such potential: its EAM reads one
+ * element (solver/potential/EAM.cxx:15-87) and its Multi shares one density
+ * and one embedding table per pair of kinds (solver/multi.cxx:64-99); with
+ * one element this is mdc_pot_eam_create. */
+mdc_status mdc_pot_alloy_create(mdc_ctx* ctx, int nelements, double rcut, const double* F, const double* rho,
+                                const double* rphi, int nr, double dr, int nrho, double drho, mdc_pot** out);
+/* The same from an N-element setfl ("eam/alloy") file, the format of
+ * problems/NiAl/Ni.eam.alloy, with the eV/Angstrom conversion of
+ * solver/potential/EAM.cxx:55-87 */
+mdc_status mdc_pot_alloy_create_file(mdc_ctx* ctx, const char* setfl_path, mdc_pot** out);
 /* solver::Multi(nkinds) + add_potential(i, j, pot) for each pair of kinds
  * (solver/multi.cxx:11-151), over single-element EAM objects and tabulated
  * Pair potentials (a Pair member adds no density and no embedding term, and

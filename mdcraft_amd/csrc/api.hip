@@ -126,9 +126,13 @@ void records(const double* y, const double* z, size_t N, std::vector<double>& ou
     }
 }
 
+// One EAM element (nk = 1, EAM.cxx:15-53) or the nk elements of an alloy
+// EAM: per element the rho records [nr], per pair of elements (setfl order
+// (1,1), (2,1), (2,2), ...) the r*phi records [nr], per element the
+// embedding records [nrho].
 Pot* make_eam(Ctx* ctx, int nk, double rcut, const double* F, const double* rho, const double* rphi, int nr, double dr,
               int nrho, double drho) {
-    if (nk != 1) throw Error(E_INVAL, "an EAM object holds one element; combine objects with mdc_pot_multi_create");
+    if (nk < 1 || nk > kMaxMultiKinds) throw Error(E_INVAL, "alloy elements must be 1..4");
     if (nr < 3 || nrho < 3) throw Error(E_INVAL, "EAM tables need at least 3 knots");
     if (!(dr > 0) || !(drho > 0) || !(rcut > 0)) throw Error(E_INVAL, "EAM steps and cutoff must be positive");
     auto p = std::make_unique<Pot>();
@@ -171,6 +175,7 @@ Pot* make_eam(Ctx* ctx, int nk, double rcut, const double* F, const double* rho,
         rec32[q + 3] = (float)(dr * aa);
     }
     const size_t nrec = rec64.size() / 4;
+    if (nrec >= (1ull << 27)) throw Error(E_INVAL, "EAM tables too large");
     T.nrec = nrec;
     T.nFrec = F64.size() / 4;
     T.r64 = dalloc<double4>(nrec);
@@ -179,6 +184,34 @@ Pot* make_eam(Ctx* ctx, int nk, double rcut, const double* F, const double* rho,
     HIPCHECK(hipMemcpy(T.r64, rec64.data(), rec64.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(T.r32, rec32.data(), rec32.size() * sizeof(float), hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(T.F64, F64.data(), F64.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (nk > 1) {
+        // one object per pair of elements lo <= hi, as the Multi kernels
+        // take them: the lo atom sums rho_hi and embeds with F_lo, the hi
+        // atom sums rho_lo and embeds with F_hi
+        std::vector<EamObj> h(npair);
+        for (int hi = 0; hi < nk; ++hi)
+            for (int lo = 0; lo <= hi; ++lo) {
+                const int q = pair_index(hi, lo);
+                EamObj& o = h[q];
+                o.rho_off = hi * nr;
+                o.rho_hi_off = lo * nr;
+                o.phi_off = (nk + q) * nr;
+                o.F_off = lo * nrho;
+                o.F_hi_off = hi * nrho;
+                o.nr = nr;
+                o.nrho = nrho;
+                o.pair_only = 0;
+                o.dr = dr;
+                o.dr1 = 1.0 / dr;
+                o.drho = drho;
+                o.drho1 = 1.0 / drho;
+                o.rc2 = rcut * rcut;
+            }
+        T.npair = npair;
+        T.alloy = true;
+        T.objs = dalloc<EamObj>(npair);
+        HIPCHECK(hipMemcpy(T.objs, h.data(), sizeof(EamObj) * npair, hipMemcpyHostToDevice));
+    }
     return p.release();
 }
 
@@ -212,6 +245,48 @@ Pot* make_eam_file(Ctx* ctx, const char* path) {
     return make_eam(ctx, 1, rc / 10, F.data(), rho.data(), rphi.data(), nr, dr / 10, nrho, drho);
 }
 
+// N-element setfl ("eam/alloy", the format of problems/NiAl/*.eam.alloy):
+// 3 comment lines; "Nelements El1 El2 ..."; Nrho drho Nr dr rc; per element
+// "Z mass a lattice", F (eV) on the rho grid, rho on the r grid; then r*phi
+// (eV*Angstrom) per pair of elements in the order (1,1), (2,1), (2,2), ...
+// Units become the reference's as in EAM.cxx:55-87.  One element gives the
+// reference's EAM object.
+Pot* make_alloy_file(Ctx* ctx, const char* path) {
+    std::ifstream f(path);
+    if (!f.is_open()) throw Error(E_FILE, std::string("Failed to open file: ") + path);
+    std::string line;
+    for (int i = 0; i < 3; ++i) std::getline(f, line);
+    std::getline(f, line);
+    std::istringstream head(line);
+    int ne = 0, nrho = 0, nr = 0;
+    double drho = 0, dr = 0, rc = 0;
+    head >> ne;
+    f >> nrho >> drho >> nr >> dr >> rc;
+    if (!head || !f || ne < 1 || nrho < 3 || nr < 3) throw Error(E_FILE, std::string("malformed setfl header: ") + path);
+    if (ne > kMaxMultiKinds) throw Error(E_INVAL, "alloy elements must be 1..4");
+    const double Na = 6.02214076e+23, Qe = 1.602176634e-19;
+    const double UnMass = 1e-3 / Na, UnLeng = 1e-9, UnTime = 1e-12;
+    const double UnEnrg = UnMass * (UnLeng / UnTime) * (UnLeng / UnTime);
+    std::vector<double> F((size_t)ne * nrho), rho((size_t)ne * nr), rphi((size_t)ne * (ne + 1) / 2 * nr);
+    for (int e = 0; e < ne; ++e) {
+        int Z = 0;
+        double M = 0, a0 = 0;
+        std::string lattice;
+        f >> Z >> M >> a0 >> lattice;
+        for (int k = 0; k < nrho; ++k) {
+            f >> F[(size_t)e * nrho + k];
+            F[(size_t)e * nrho + k] *= Qe / UnEnrg;
+        }
+        for (int k = 0; k < nr; ++k) f >> rho[(size_t)e * nr + k];
+    }
+    for (auto& x : rphi) {
+        f >> x;
+        x *= Qe / (10 * UnEnrg);
+    }
+    if (!f) throw Error(E_FILE, std::string("truncated setfl tables: ") + path);
+    return make_eam(ctx, ne, rc / 10, F.data(), rho.data(), rphi.data(), nr, dr / 10, nrho, drho);
+}
+
 void free_pot(Pot* p) {
     if (!p) return;
     if (p->type == PT_BPNN) {
@@ -235,7 +310,7 @@ Pot* make_multi(Ctx* ctx, int nk, int npots, const int* ki, const int* kj, const
     for (int q = 0; q < npots; ++q) {
         if (ki[q] < 0 || kj[q] < 0 || ki[q] >= nk || kj[q] >= nk) throw Error(E_INVAL, "Multi: kind out of range");
         const Pot* s = pots[q];
-        const bool eam1 = s && s->type == PT_EAM && s->eam.nk == 1;
+        const bool eam1 = s && s->type == PT_EAM && s->eam.nk == 1 && !s->eam.alloy;
         const bool tab = s && s->type == PT_LJ && s->lj.fn == 2;
         if (!eam1 && !tab)
             throw Error(E_INVAL, "Multi: every member must be a single-element EAM or a tabulated Pair potential");
@@ -264,6 +339,8 @@ Pot* make_multi(Ctx* ctx, int nk, int npots, const int* ki, const int* kj, const
         o.rho_off = (int)nrec;
         o.phi_off = (int)(nrec + S.nr);
         o.F_off = (int)nF;
+        o.rho_hi_off = o.rho_off;
+        o.F_hi_off = o.F_off;
         o.nr = S.nr;
         o.nrho = tab ? kZeroF : S.nrho;
         o.pair_only = tab ? 1 : 0;
@@ -675,7 +752,9 @@ void optimize_rns(Batch& b, double kbuf, double* rns_out) {
 void check_kinds(const Batch& b, const Pot& p) {
     if (p.type == PT_EAM && p.eam.nk > 1)
         for (int o = 0; o < b.n; ++o)
-            if (b.kind[o] >= p.eam.nk) throw Error(E_INVAL, "atom kind outside the Multi potential's kinds");
+            if (b.kind[o] >= p.eam.nk)
+                throw Error(E_INVAL, p.eam.alloy ? "atom kind outside the alloy's elements"
+                                                 : "atom kind outside the Multi potential's kinds");
 }
 
 }  // namespace
@@ -871,6 +950,23 @@ mdc_status mdc_pot_eam_create_file(mdc_ctx* ctx, const char* path, mdc_pot** out
         if (!ctx || !out || !path) throw Error(E_INVAL, "null argument");
         HIPCHECK(hipSetDevice(ctx->c.device));
         *out = reinterpret_cast<mdc_pot*>(make_eam_file(&ctx->c, path));
+    });
+}
+
+mdc_status mdc_pot_alloy_create(mdc_ctx* ctx, int nelements, double rcut, const double* F, const double* rho,
+                                const double* rphi, int nr, double dr, int nrho, double drho, mdc_pot** out) {
+    return guard(ctx ? &ctx->c : nullptr, [&] {
+        if (!ctx || !out || !F || !rho || !rphi) throw Error(E_INVAL, "null argument");
+        HIPCHECK(hipSetDevice(ctx->c.device));
+        *out = reinterpret_cast<mdc_pot*>(make_eam(&ctx->c, nelements, rcut, F, rho, rphi, nr, dr, nrho, drho));
+    });
+}
+
+mdc_status mdc_pot_alloy_create_file(mdc_ctx* ctx, const char* path, mdc_pot** out) {
+    return guard(ctx ? &ctx->c : nullptr, [&] {
+        if (!ctx || !out || !path) throw Error(E_INVAL, "null argument");
+        HIPCHECK(hipSetDevice(ctx->c.device));
+        *out = reinterpret_cast<mdc_pot*>(make_alloy_file(&ctx->c, path));
     });
 }
 

@@ -86,8 +86,14 @@ struct LJParams {
 // records in the r-record array and of its F records, and its grids.
 struct EamObj {
     // pair_only: a tabulated Pair member (no density, no embedding; its
-    // virial leaves Em, pair.cxx:235-324)
-    int rho_off, phi_off, F_off, nr, nrho, pair_only;
+    // virial leaves Em, pair.cxx:235-324).
+    // rho_off / F_off: the density table that an atom of the lower kind of
+    // the pair sums over its neighbour and that atom's embedding table;
+    // rho_hi_off / F_hi_off: the same for the atom of the higher kind.  A
+    // Multi object uses one density and one embedding table for both atoms
+    // (the offsets agree); an alloy object of elements a < b holds rho_b and
+    // F_a, then rho_a and F_b.
+    int rho_off, phi_off, F_off, nr, nrho, pair_only, rho_hi_off, F_hi_off;
     double dr, dr1, drho, drho1, rc2;
 };
 constexpr int kMaxMultiKinds = 4;
@@ -112,8 +118,11 @@ struct EamTables {
     double4* F64 = nullptr;
     float2* knots32 = nullptr;  // (y, z) knots of every r table for LDS staging: [ntab][nr+1]
     // Multi potential (nk > 1): objects per pair of kinds, records of all
-    // objects concatenated in r64 / r32 / F64
+    // objects concatenated in r64 / r32 / F64.  alloy: the nk elements of an
+    // alloy EAM (per-element density and embedding, a pair term per pair of
+    // elements) in the same objects, whose two sides then differ.
     int npair = 0;
+    bool alloy = false;
     size_t nrec = 0, nFrec = 0;
     EamObj* objs = nullptr;
 };

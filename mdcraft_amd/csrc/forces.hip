@@ -318,16 +318,17 @@ __global__ void __launch_bounds__(256) k_eam_rho(const double4* __restrict__ pos
         for (int u = 0; u < kU; ++u) pj[u] = buf_d4(posb, okk[u] ? (unsigned)ent_j(e[u]) * 32u : kSkip);
         next_group<TPA, kU>(as, m0, M, nl, jn);
         PairGeo<R> g[kU];
-        int ob[kU];
+        int ob[kU], toff[kU];
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             if (!MULTI) {
-                ob[u] = 0;
+                ob[u] = toff[u] = 0;
                 g[u] = pair_geo<R>(fr, pi, pj[u], e[u], okk[u], P.rc2, dr1, P.nr);
             } else {
                 const int kj = w_kind(pj[u].w);
                 ob[u] = pair_index(ki, kj);
                 const EamObj& o = s_obj[ob[u]];
+                toff[u] = ki > kj ? o.rho_hi_off : o.rho_off;  // the table of atom i's side
                 g[u] = pair_geo<R>(fr, pi, pj[u], e[u], okk[u], o.rc2, (R)o.dr1, o.nr);
                 // only an EAM object's virial sets Em (a Pair member's leaves it)
                 if (okk[u] && !o.pair_only) kd = max(kd, kj);
@@ -336,8 +337,7 @@ __global__ void __launch_bounds__(256) k_eam_rho(const double4* __restrict__ pos
         }
         typename Vec4T<R>::T rec[kU];
 #pragma unroll
-        for (int u = 0; u < kU; ++u)
-            rec[u] = load_rec_if<R>(tabb, (MULTI ? s_obj[ob[u]].rho_off : 0) + g[u].N1, g[u].ok);
+        for (int u = 0; u < kU; ++u) rec[u] = load_rec_if<R>(tabb, toff[u] + g[u].N1, g[u].ok);
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             if (!g[u].ok) continue;
@@ -374,7 +374,7 @@ __global__ void __launch_bounds__(256) k_eam_rho(const double4* __restrict__ pos
         for (int k = 0; k < P.nk; ++k) {
             const EamObj& o = s_obj[pair_index(ki, k)];
             double F, dF;
-            embed_at(P.F64, o.F_off, o.nrho, o.drho, o.drho1, rr, F, dF, misc);
+            embed_at(P.F64, ki > k ? o.F_hi_off : o.F_off, o.nrho, o.drho, o.drho1, rr, F, dF, misc);
             P.fpm[(size_t)k * n + i] = dF;
             if (active && k == klast && S.store_aux && (S.flags & W_ENERGY)) A.d[(size_t)D_EM * n + i] = F;
         }
@@ -385,8 +385,10 @@ __global__ void __launch_bounds__(256) k_eam_rho(const double4* __restrict__ pos
 // Force pass (EAM.cxx:356-497) and, when asked, pair energy and virial
 // (EAM.cxx:306-354); the velocity-Verlet finish is fused.  Multi: each pair
 // uses the object of (kind_i, kind_j): its rho'(r), its pair term and
-// F'(nc_i) + F'(nc_j) under its own embedding table.
-template <typename R, int TPA, bool MULTI, bool EV>
+// F'(nc_i) + F'(nc_j) under its own embedding table.  DIR (alloy EAM): the
+// two sides of an object differ, so a pair of elements a != b takes
+// F_a'(nc_i) rho_b'(r) + F_b'(nc_j) rho_a'(r), one more table record.
+template <typename R, int TPA, bool MULTI, bool EV, bool DIR = false>
 __global__ void __launch_bounds__(256) k_eam_force(const double4* __restrict__ pos, double4* __restrict__ pos_next,
                                                    const int* __restrict__ nl, const int* __restrict__ nn, int M,
                                                    int cap, const SysDesc* __restrict__ desc, EamK P, AtomArrays A,
@@ -457,7 +459,7 @@ __global__ void __launch_bounds__(256) k_eam_force(const double4* __restrict__ p
                 g[u] = pair_geo<R>(fr, pi, pj[u], e[u], okk[u], o.rc2, (R)o.dr1, o.nr);
             }
         }
-        typename Vec4T<R>::T rr[kU], rp[kU];
+        typename Vec4T<R>::T rr[kU], rp[kU], rj[DIR ? kU : 1];
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             if (!MULTI) {
@@ -465,8 +467,10 @@ __global__ void __launch_bounds__(256) k_eam_force(const double4* __restrict__ p
                 rp[u] = load_rec_if<R>(tabb, P.nr + g[u].N1, g[u].ok);
             } else {
                 const EamObj& o = s_obj[ob[u]];
-                rr[u] = load_rec_if<R>(tabb, o.rho_off + g[u].N1, g[u].ok);
+                const bool hi = ki > kj[u];  // atom i is the higher kind of the pair
+                rr[u] = load_rec_if<R>(tabb, (hi ? o.rho_hi_off : o.rho_off) + g[u].N1, g[u].ok);
                 rp[u] = load_rec_if<R>(tabb, o.phi_off + g[u].N1, g[u].ok);
+                if (DIR) rj[u] = load_rec_if<R>(tabb, (hi ? o.rho_off : o.rho_hi_off) + g[u].N1, g[u].ok && kj[u] != ki);
             }
         }
 #pragma unroll
@@ -489,8 +493,15 @@ __global__ void __launch_bounds__(256) k_eam_force(const double4* __restrict__ p
             spline_vd<R>(rr[u], R1, RN, N1, drp, dr1p, rv, nd);
             // FF = (AdF rho'(r) + phi'(r)) / r with phi' = ((r phi)' r - r phi) / r^2
             const R i1 = rcp_(R1), i2 = rcp_(R2);
-            const R AdF = (R)(active ? Fpi : 0.0) + Fpj[u];
-            const R FF = (AdF * nd + (phd * R1 - phv) * i2) * i1;
+            R FF;
+            if (DIR && kj[u] != ki) {
+                R rvj, ndj;
+                spline_vd<R>(rj[u], R1, RN, N1, drp, dr1p, rvj, ndj);
+                FF = ((R)(active ? Fpi : 0.0) * nd + Fpj[u] * ndj + (phd * R1 - phv) * i2) * i1;
+            } else {
+                const R AdF = (R)(active ? Fpi : 0.0) + Fpj[u];
+                FF = (AdF * nd + (phd * R1 - phv) * i2) * i1;
+            }
             if (EV && (S.flags & W_ENERGY)) en += 0.5 * (double)(phv * i1);
             const R rx = g[u].rx, ry = g[u].ry, rz = g[u].rz;
             fx += FF * rx;
@@ -801,8 +812,9 @@ static void launch_pair_t(Batch& b, const Pot& p, const StepArgs& S, const doubl
         }
         if (!(which & 2)) return;
         const bool ev = S.flags & (W_ENERGY | W_VIRIAL);
-        auto k = E.nk > 1 ? (ev ? k_eam_force<R, TPA, true, true> : k_eam_force<R, TPA, true, false>)
-                          : (ev ? k_eam_force<R, TPA, false, true> : k_eam_force<R, TPA, false, false>);
+        auto k = p.eam.alloy ? (ev ? k_eam_force<R, TPA, true, true, true> : k_eam_force<R, TPA, true, false, true>)
+                 : E.nk > 1  ? (ev ? k_eam_force<R, TPA, true, true> : k_eam_force<R, TPA, true, false>)
+                             : (ev ? k_eam_force<R, TPA, false, true> : k_eam_force<R, TPA, false, false>);
         launch_k(b, K_FORCE, k, dim3(grid), dim3(256), 0, pos, pos_next, b.nl, b.nn, b.M, b.cap, b.d_desc, E, AA, S,
                  b.rsave, b.misc);
     }
@@ -1018,7 +1030,7 @@ void stage_compute(Batch& b, const Pot& p, int stage, int prec) {
         case ST_PREPARE:
             hipLaunchKernelGGL(k_prepare, dim3(grid), dim3(256), 0, st, pos, AA, eam ? 1 : 0);
             // solver::Multi's prepare leaves rcut (multi.cxx:56-66)
-            if (!(eam && p.eam.nk > 1)) b.last_rcut = p.rcut;
+            if (!(eam && p.eam.nk > 1 && !p.eam.alloy)) b.last_rcut = p.rcut;
             b.f_prec = P_FP64;
             b.bp_prepared = -1;
             if (bpnn) {

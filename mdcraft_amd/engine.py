@@ -30,6 +30,7 @@ F_ALL = (1 << 14) - 1
 EXPORTS = [
     "mdc_ctx_create", "mdc_ctx_destroy", "mdc_ctx_last_error", "mdc_last_error", "mdc_atom_record_size",
     "mdc_pot_lj_create", "mdc_pot_ljs_create", "mdc_pot_pair_create", "mdc_pot_pair_create_file", "mdc_pot_eam_create", "mdc_pot_eam_create_file", "mdc_pot_multi_create",
+    "mdc_pot_alloy_create", "mdc_pot_alloy_create_file",
     "mdc_pot_bpnn_create", "mdc_pot_rcut", "mdc_pot_destroy",
     "mdc_sys_create", "mdc_sys_destroy", "mdc_sys_upload", "mdc_sys_download", "mdc_sys_download_range", "mdc_sys_set_active", "mdc_sys_natoms",
     "mdc_sys_set_lanes", "mdc_sys_autotune", "mdc_sys_time_kernels", "mdc_sys_set_rebuild", "mdc_sys_set_boundary", "mdc_sys_set_thermostat",
@@ -60,6 +61,8 @@ def load_library(path=LIB_PATH):
         "mdc_pot_eam_create": (I, [P, D, DP, DP, DP, I, D, I, D, PP]),
         "mdc_pot_eam_create_file": (I, [P, C.c_char_p, PP]),
         "mdc_pot_multi_create": (I, [P, I, I, C.POINTER(I), C.POINTER(I), PP, PP]),
+        "mdc_pot_alloy_create": (I, [P, I, D, DP, DP, DP, I, D, I, D, PP]),
+        "mdc_pot_alloy_create_file": (I, [P, C.c_char_p, PP]),
         "mdc_pot_bpnn_create": (I, [P, C.c_char_p, PP]),
         "mdc_pot_rcut": (I, [P, DP]),
         "mdc_pot_destroy": (I, [P]),
@@ -200,6 +203,25 @@ def Multi(ctx, nkinds, pairs):
     hs = (C.c_void_p * len(pairs))(*[p[2].h for p in pairs])
     h = C.c_void_p()
     _check(lib().mdc_pot_multi_create(ctx.h, nkinds, len(pairs), ki, kj, hs, C.byref(h)))
+    return Potential(ctx, h, "eam")
+
+
+def Alloy(ctx, rcut=None, F=None, rho=None, rphi=None, dr=None, drho=None, path=None):
+    """Alloy EAM of 1..4 elements (element e is atom kind e) from an
+    N-element setfl file or from tables: F [ne][nrho], rho [ne][nr], rphi
+    [ne(ne+1)/2][nr] in the setfl pair order (1,1), (2,1), (2,2), ...
+    Atom i of element a sums rho_b over neighbours of element b, embeds
+    with F_a and adds the pair term of (a, b)."""
+    h = C.c_void_p()
+    if path is not None:
+        _check(lib().mdc_pot_alloy_create_file(ctx.h, path.encode(), C.byref(h)))
+    else:
+        F, rho, rphi = (np.ascontiguousarray(x, dtype=np.float64) for x in (F, rho, rphi))
+        ne = F.shape[0]
+        if F.ndim != 2 or rho.shape[0] != ne or rphi.shape != (ne * (ne + 1) // 2, rho.shape[1]):
+            raise ValueError("alloy tables: F [ne][nrho], rho [ne][nr], rphi [ne(ne+1)/2][nr]")
+        _check(lib().mdc_pot_alloy_create(ctx.h, ne, rcut, _dp(F), _dp(rho), _dp(rphi), rho.shape[1], dr, F.shape[1],
+                                          drho, C.byref(h)))
     return Potential(ctx, h, "eam")
 
 

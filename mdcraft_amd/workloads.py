@@ -340,6 +340,83 @@ def nial_pots():
     return {"kind": "eam_multi", "nkinds": 2, "pairs": [(0, 0, ni), (1, 1, al), (0, 1, nial)], "rc": ni["rc"]}
 
 
+# Alloy EAM elements on the EAM Cu forms with constants of their own:
+# rho_e(r) = A exp(-beta (r / re - 1)), F_e(rho) = -E sqrt(rho) eV, and a
+# Morse pair term (D eV, alpha / A, r0 A) per pair of elements, all tapered
+# to the 5.0 A cutoff.  Synthetic functions: they give the elements
+# different densities, embeddings and pair terms, not a fitted alloy.
+ALLOY_ELEMENTS = {
+    "Ni": dict(Z=28, mass=58.6934, a=3.52, lattice="fcc", A=1.1, beta=3.2, re=2.49, E=1.35),
+    "Al": dict(Z=13, mass=26.9815385, a=4.05, lattice="fcc", A=0.8, beta=2.6, re=2.86, E=0.95),
+    "Cu": dict(Z=29, mass=63.546, a=3.615, lattice="fcc", A=1.0, beta=3.0, re=2.556, E=1.2),
+}
+ALLOY_PAIRS = {
+    ("Ni", "Ni"): (0.22, 1.65, 2.49),
+    ("Al", "Ni"): (0.26, 1.55, 2.55),
+    ("Al", "Al"): (0.15, 1.45, 2.86),
+    ("Cu", "Ni"): (0.21, 1.60, 2.52),
+    ("Al", "Cu"): (0.19, 1.50, 2.70),
+    ("Cu", "Cu"): (0.20, 1.60, 2.55),
+}
+
+
+def alloy_functions(a, b=None):
+    """Analytic rho_a(r), F_a(rho) (eV) or, with b, phi_ab(r) (eV); r in A."""
+    if b is None:
+        e = ALLOY_ELEMENTS[a]
+        rho = lambda r: e["A"] * np.exp(-e["beta"] * (np.asarray(r) / e["re"] - 1.0)) * taper(np.asarray(r))
+        F = lambda x: -e["E"] * np.sqrt(x)
+        return rho, F
+    D, al, r0 = ALLOY_PAIRS[tuple(sorted((a, b)))]
+    return lambda r: morse(np.asarray(r), D, al, r0) * taper(np.asarray(r))
+
+
+def alloy_tables(elements, nr=5000, nrho=5000, rho_max=50.0):
+    """Tables of an alloy of the named elements in setfl order and
+    reference units: F [ne][nrho] (kJ/mol), rho [ne][nr], rphi
+    [ne(ne+1)/2][nr] (kJ nm/mol; pairs (1,1), (2,1), (2,2), ...)."""
+    dr_a = 5.01 / (nr - 1)
+    r = np.arange(nr) * dr_a
+    drho = rho_max / (nrho - 1)
+    rh = np.arange(nrho) * drho
+    F, rho, rphi = [], [], []
+    for e in elements:
+        fr, fF = alloy_functions(e)
+        rho.append(fr(r))
+        F.append(fF(rh) * EV)
+    for i in range(len(elements)):
+        for j in range(i + 1):
+            rphi.append((r * ANG) * alloy_functions(elements[i], elements[j])(r) * EV)
+    return {"elements": list(elements), "F": np.array(F), "rho": np.array(rho), "rphi": np.array(rphi),
+            "dr": dr_a * ANG, "drho": drho, "rc": EAM_RC * ANG}
+
+
+def write_setfl(path, elements, nr=5000, nrho=5000, rho_max=50.0):
+    """The alloy tables as an N-element setfl file (eV, Angstrom), the
+    format of problems/NiAl/*.eam.alloy."""
+    t = alloy_tables(elements, nr, nrho, rho_max)
+    fmt = lambda v: "\n".join(" ".join("%.16e" % x for x in v[k:k + 5]) for k in range(0, len(v), 5))
+    with open(path, "w") as f:
+        f.write("synthetic alloy EAM\ntapered Morse pair terms\nexponential densities, square-root embedding\n")
+        f.write("%d %s\n" % (len(elements), " ".join(elements)))
+        f.write("%d %.16e %d %.16e %.16e\n" % (nrho, t["drho"], nr, t["dr"] / ANG, t["rc"] / ANG))
+        for k, e in enumerate(elements):
+            p = ALLOY_ELEMENTS[e]
+            f.write("%d %.6f %.4f %s\n" % (p["Z"], p["mass"], p["a"], p["lattice"]))
+            f.write(fmt(t["F"][k] / EV) + "\n")
+            f.write(fmt(t["rho"][k]) + "\n")
+        for q in range(len(t["rphi"])):
+            f.write(fmt(t["rphi"][q] / (EV * ANG)) + "\n")
+    return t
+
+
+def nial_alloy_batch(count=64):
+    """B2 NiAl cells (kind 0 = Ni, 1 = Al) for the alloy EAM."""
+    cells = [nial_cell(i) for i in range(count)]
+    pot = dict(kind="alloy", **alloy_tables(["Ni", "Al"]))
+    return {"cells": cells, "pot": pot, "dt": 1 * FS, "name": "nial_alloy_batch"}
+
+
 def nial_batch(count=512):
     cells = [nial_cell(i) for i in range(count)]
     return {"cells": cells, "pot": nial_pots(), "dt": 1 * FS, "name": "nial_batch"}

@@ -855,6 +855,207 @@ struct EAM : Potential {
     }
 };
 
+// N-element setfl ("eam/alloy") potential: the standard alloy EAM on the
+// reference's splines, evaluation forms and units.  Atom i of element a sums
+// rho_b(r) over neighbours of element b, its embedding energy is F_a(nc_i),
+// and a pair of elements (a, b) adds r*phi_ab(r)/r; the force on i from j is
+// [F_a'(nc_i) rho_b'(r) + F_b'(nc_j) rho_a'(r) + phi_ab'(r)] along r_ij.
+// The reference's EAM reads one element (EAM.cxx:55-87) and its Multi
+// shares one density and one embedding table between both atoms of a pair
+// (multi.cxx:64-99 with EAM.cxx:367-394), so this form goes beyond it.  The
+// tables use EAM's init_spline (ib = 1 for r tables, 2 for F) and its
+// evaluation forms (EAM.cxx:499-601); elements are atom kinds 0..ne-1; the
+// pair tables follow the setfl order (1,1), (2,1), (2,2), (3,1), ...
+struct Alloy : Potential {
+    int ne = 0, Nr1 = 0, Nrh1 = 0;
+    double dr = 0, dr1 = 0, drh = 0, drh1 = 0;
+    // per element F and rho, per pair of elements r*phi; each with its
+    // slopes and one guard knot (the reference's [N1+1] read at N1 = N-1)
+    std::vector<std::vector<double>> F, dF, rho, drho, rphi, drphi;
+
+    Alloy() : Potential(PT_EAM) {}
+
+    static int pair(int a, int b) {
+        const int hi = std::max(a, b), lo = std::min(a, b);
+        return hi * (hi + 1) / 2 + lo;
+    }
+
+    static void table(const double* y, int N, double h1, int ib, std::vector<double>& v, std::vector<double>& z) {
+        v.assign(y, y + N);
+        z.assign(N, 0.0);
+        EAM::init_spline(v.data(), z.data(), h1, N, ib);
+        v.push_back(v.back());
+        z.push_back(z.back());
+    }
+
+    static Alloy* from_tables(int ne, double rc, const double* Fe, const double* rhoe, const double* rphie, int nr,
+                              double dr_, int nrho, double drho_) {
+        if (ne < 1 || ne > 8) throw std::runtime_error("alloy elements must be 1..8");
+        if (nr < 3 || nrho < 3) throw std::runtime_error("EAM tables need at least 3 knots");
+        auto* p = new Alloy();
+        p->ne = ne;
+        p->R1cut = rc;
+        p->R2cut = rc * rc;
+        p->Nr1 = nr;
+        p->Nrh1 = nrho;
+        p->dr = dr_;
+        p->drh = drho_;
+        p->dr1 = 1.0 / dr_;
+        p->drh1 = 1.0 / drho_;
+        const int np = ne * (ne + 1) / 2;
+        p->F.resize(ne);
+        p->dF.resize(ne);
+        p->rho.resize(ne);
+        p->drho.resize(ne);
+        p->rphi.resize(np);
+        p->drphi.resize(np);
+        for (int e = 0; e < ne; ++e) {
+            table(Fe + (size_t)e * nrho, nrho, p->drh1, 2, p->F[e], p->dF[e]);
+            table(rhoe + (size_t)e * nr, nr, p->dr1, 1, p->rho[e], p->drho[e]);
+        }
+        for (int q = 0; q < np; ++q) table(rphie + (size_t)q * nr, nr, p->dr1, 1, p->rphi[q], p->drphi[q]);
+        return p;
+    }
+
+    // N-element setfl: 3 comment lines; "Nelements El1 El2 ..."; Nrho drho
+    // Nr dr rc; per element "Z mass a lattice", F (eV) on the rho grid, rho
+    // on the r grid; then r*phi (eV*Angstrom) per pair of elements.  The
+    // units become the reference's as in EAM.cxx:55-87.
+    static Alloy* from_file(const char* path) {
+        std::ifstream file(path);
+        if (!file.is_open()) throw std::runtime_error(std::string("Failed to open file: ") + path);
+        std::string line;
+        for (int i = 0; i < 3; ++i) std::getline(file, line);
+        std::getline(file, line);
+        std::istringstream head(line);
+        int ne = 0, nrho = 0, nr = 0;
+        double drho_ = 0, dr_ = 0, rc = 0;
+        head >> ne;
+        file >> nrho >> drho_ >> nr >> dr_ >> rc;
+        if (!head || !file || ne < 1 || ne > 8 || nrho < 3 || nr < 3)
+            throw std::runtime_error(std::string("malformed setfl header: ") + path);
+        std::vector<double> Fe((size_t)ne * nrho), rhoe((size_t)ne * nr), rphie((size_t)ne * (ne + 1) / 2 * nr);
+        for (int e = 0; e < ne; ++e) {
+            int Z;
+            double M, a0;
+            std::string lattice;
+            file >> Z >> M >> a0 >> lattice;
+            for (int k = 0; k < nrho; ++k) {
+                file >> Fe[(size_t)e * nrho + k];
+                Fe[(size_t)e * nrho + k] *= kQe / kUnEnrg;
+            }
+            for (int k = 0; k < nr; ++k) file >> rhoe[(size_t)e * nr + k];
+        }
+        for (auto& x : rphie) {
+            file >> x;
+            x *= kQe / (10 * kUnEnrg);
+        }
+        if (!file) throw std::runtime_error(std::string("truncated setfl tables: ") + path);
+        return from_tables(ne, rc / 10, Fe.data(), rhoe.data(), rphie.data(), nr, dr_ / 10, nrho, drho_);
+    }
+
+    void validate(const Atom* atoms, size_t n) const override {
+        for (size_t i = 0; i < n; ++i)
+            if (atoms[i].kind >= ne) throw std::runtime_error("atom kind outside the alloy's elements");
+    }
+
+    // value and slope of an r table at R1 (EAM.cxx:367-394 forms); zero
+    // beyond the last knot
+    void rtab(const std::vector<double>& y, const std::vector<double>& z, double R1, double& v, double& d) const {
+        double RN = R1 * dr1;
+        int N1 = (int)std::trunc(RN);
+        if (N1 >= Nr1) {
+            v = d = 0.0;
+            return;
+        }
+        double VR = N1, DV = RN - VR;
+        RN = R1 - VR * dr;
+        VR = (y[N1 + 1] - y[N1]) * dr1;
+        double aa = (z[N1 + 1] - VR) - (VR - z[N1]);
+        double bb = aa - (VR - z[N1]);
+        aa = aa * DV;
+        v = y[N1] + RN * (z[N1] + DV * (aa - bb));
+        d = z[N1] + 2.0 * DV * (1.5 * aa - bb);
+    }
+
+    // F_e(nc) and F_e'(nc) (EAM.cxx:562-601; NaN outside the table)
+    void embed(int e, double nc, double& Fv, double& Fd) const {
+        double RN = nc * drh1;
+        int N1 = (int)std::trunc(RN);
+        if (!(RN >= 0.0 && N1 < Nrh1)) {
+            Fv = Fd = std::numeric_limits<double>::quiet_NaN();
+            return;
+        }
+        const std::vector<double>&y = F[e], &z = dF[e];
+        double VR = N1, DV = RN - VR;
+        RN = nc - VR * drh;
+        VR = (y[N1 + 1] - y[N1]) * drh1;
+        double aa = (z[N1 + 1] - VR) - (VR - z[N1]);
+        double bb = aa - (VR - z[N1]);
+        aa = aa * DV;
+        Fv = y[N1] + RN * (z[N1] + DV * (aa - bb));
+        Fd = z[N1] + 2.0 * DV * (1.5 * aa - bb);
+    }
+
+    void density(Atom& atom, Atom* neibs, const size_t* list, size_t nl) override {
+        double sum = 0.0;
+        for (size_t k = 0; k < nl; ++k) {
+            const Atom& nb = neibs[list[k]];
+            double d[3] = {nb.r[0] - atom.r[0], nb.r[1] - atom.r[1], nb.r[2] - atom.r[2]};
+            double r2 = sqnorm(d);
+            if (r2 > R2cut || r2 < 1e-15) continue;
+            double v, s;
+            rtab(rho[nb.kind], drho[nb.kind], std::sqrt(r2), v, s);
+            sum += v;
+        }
+        atom.nc += sum;
+    }
+
+    void gather(Atom& atom, Atom* neibs, const size_t* list, size_t nl, bool is_virial) {
+        if (nl == 0) return;
+        const int a = atom.kind;
+        double Fi, dFi;
+        embed(a, atom.nc, Fi, dFi);
+        if (is_virial) atom.Em = Fi;
+        double Fs[3] = {0, 0, 0}, Vt[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, Ut = 0.0;
+        for (size_t k = 0; k < nl; ++k) {
+            const Atom& nb = neibs[list[k]];
+            double rk[3] = {nb.r[0] - atom.r[0], nb.r[1] - atom.r[1], nb.r[2] - atom.r[2]};
+            double r2 = sqnorm(rk);
+            if (r2 > R2cut || r2 < 1e-15) continue;
+            const int b = nb.kind;
+            double Fj, dFj;
+            embed(b, nb.nc, Fj, dFj);
+            const double R1 = std::sqrt(r2);
+            double rb, rbd, ra, rad, pv, pd;
+            rtab(rho[b], drho[b], R1, rb, rbd);
+            rtab(rho[a], drho[a], R1, ra, rad);
+            rtab(rphi[pair(a, b)], drphi[pair(a, b)], R1, pv, pd);
+            const double FF = (dFi * rbd + dFj * rad + (pd * R1 - pv) / r2) / R1;
+            if (!is_virial) {
+                for (int c = 0; c < 3; ++c) Fs[c] += FF * rk[c];
+            } else {
+                double s = 0.5 * FF;
+                for (int col = 0; col < 3; ++col)
+                    for (int row = 0; row < 3; ++row) Vt[col * 3 + row] += (s * rk[row]) * rk[col];
+                Ut += 0.5 * (pv / R1);
+            }
+        }
+        if (!is_virial) {
+            for (int c = 0; c < 3; ++c) atom.f[c] += Fs[c];
+        } else {
+            for (int t = 0; t < 9; ++t) atom.V[t] += Vt[t];
+            atom.Ep += Ut;
+        }
+    }
+    void force(Atom& atom, Atom* neibs, const size_t* list, size_t nl) override {
+        gather(atom, neibs, list, nl, false);
+    }
+    void virial(Atom& atom, Atom* neibs, const size_t* list, size_t nl) override {
+        gather(atom, neibs, list, nl, true);
+    }
+};
+
 // LJs (mdcraft/solver/potential/LJs.cxx:7-256): LJ with a polynomial tail
 // in X = (r / rVr)^2 - 2^(1/3) that takes V and V' to zero at rc; the
 // coefficients as the constructor leaves them.
@@ -1706,6 +1907,25 @@ void* ref_eam_create(double rc, const double* F, const double* rho, const double
 void* ref_eam_create_file(const char* path) {
     try {
         return EAM::from_file(path);
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
+void* ref_alloy_create(int ne, double rc, const double* F, const double* rho, const double* rphi, int nr, double dr,
+                       int nrho, double drho) {
+    try {
+        return Alloy::from_tables(ne, rc, F, rho, rphi, nr, dr, nrho, drho);
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
+void* ref_alloy_create_file(const char* path) {
+    try {
+        return Alloy::from_file(path);
     } catch (const std::exception& e) {
         g_err = e.what();
         return nullptr;

@@ -36,6 +36,8 @@ def _load():
         "ref_lj_create": (P, [D, D, D]),
         "ref_eam_create": (P, [D, Z, Z, Z, I, D, I, D]),
         "ref_eam_create_file": (P, [C.c_char_p]),
+        "ref_alloy_create": (P, [I, D, Z, Z, Z, I, D, I, D]),
+        "ref_alloy_create_file": (P, [C.c_char_p]),
         "ref_bpnn_create": (P, [C.c_char_p, dom]),
         "ref_ljs_create": (P, [D, D, D]),
         "ref_pair_create": (P, [D, Z, I, D]),
@@ -168,6 +170,18 @@ def EAM(rc=None, F=None, rho=None, rphi=None, dr=None, drho=None, path=None):
         return Potential(lib.ref_eam_create_file(path.encode()))
     F, rho, rphi = (np.ascontiguousarray(x, dtype=np.float64) for x in (F, rho, rphi))
     return Potential(lib.ref_eam_create(rc, _dp(F), _dp(rho), _dp(rphi), len(rho), dr, len(F), drho))
+
+
+def Alloy(rc=None, F=None, rho=None, rphi=None, dr=None, drho=None, path=None):
+    """N-element alloy EAM: F [ne][nrho], rho [ne][nr], rphi [ne(ne+1)/2][nr]
+    in the setfl pair order, or an N-element setfl file."""
+    if path is not None:
+        return Potential(lib.ref_alloy_create_file(path.encode()))
+    F, rho, rphi = (np.ascontiguousarray(x, dtype=np.float64) for x in (F, rho, rphi))
+    ne = F.shape[0]
+    if rho.shape[0] != ne or rphi.shape[0] != ne * (ne + 1) // 2 or rphi.shape[1] != rho.shape[1]:
+        raise ValueError("alloy tables: F [ne][nrho], rho [ne][nr], rphi [ne(ne+1)/2][nr]")
+    return Potential(lib.ref_alloy_create(ne, rc, _dp(F), _dp(rho), _dp(rphi), rho.shape[1], dr, F.shape[1], drho))
 
 
 class Multi(Potential):

@@ -28,6 +28,10 @@ def oracle_potential(R, pot, box=None):
         return R.MaiseBPNN(pot["path"], R.Domain(*box))
     if k == "eam_multi":
         return R.Multi(pot["nkinds"], [(i, j, oracle_potential(R, q)) for i, j, q in pot["pairs"]])
+    if k == "alloy":
+        if "path" in pot:
+            return R.Alloy(path=pot["path"])
+        return R.Alloy(rc=pot["rc"], F=pot["F"], rho=pot["rho"], rphi=pot["rphi"], dr=pot["dr"], drho=pot["drho"])
     raise ValueError(k)
 
 
@@ -42,6 +46,11 @@ def engine_potential(E, ctx, pot):
                      drho=pot["drho"])
     if k == "eam_multi":
         return E.Multi(ctx, pot["nkinds"], [(i, j, engine_potential(E, ctx, q)) for i, j, q in pot["pairs"]])
+    if k == "alloy":
+        if "path" in pot:
+            return E.Alloy(ctx, path=pot["path"])
+        return E.Alloy(ctx, rcut=pot["rc"], F=pot["F"], rho=pot["rho"], rphi=pot["rphi"], dr=pot["dr"],
+                       drho=pot["drho"])
     if k == "ljs":
         return E.LJs(ctx, pot["aVr"], pot["rVr"], pot["rc"])
     if k == "pair":
