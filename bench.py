@@ -67,6 +67,9 @@ def make_pot(E, ctx, pot):
                      drho=pot["drho"])
     if pot["kind"] == "eam_multi":
         return E.Multi(ctx, pot["nkinds"], [(i, j, make_pot(E, ctx, q)) for i, j, q in pot["pairs"]])
+    if pot["kind"] == "alloy":
+        return E.Alloy(ctx, rcut=pot["rc"], F=pot["F"], rho=pot["rho"], rphi=pot["rphi"], dr=pot["dr"],
+                       drho=pot["drho"])
     if pot["kind"] == "bpnn":
         return E.MaiseBPNNPotential(ctx, pot["path"])
     raise ValueError(pot["kind"])
@@ -364,7 +367,7 @@ def main():
     ap.add_argument("--summarise-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
     ap.add_argument("--summarise-stats", metavar="STATS_DIR")
     ap.add_argument("--kernel", default="k_eam_force<float")
-    ap.add_argument("--only", choices=("bpnn_cu", "nial_batch", "served"),
+    ap.add_argument("--only", choices=("bpnn_cu", "nial_batch", "nial_alloy", "served"),
                     help="run one of the extra workloads alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.summarise_pmc:
@@ -387,8 +390,8 @@ def main():
             wb = W.bpnn_cu()
             out = {p: run_batch(E, ctx, [(wb["atoms"], wb["box"])], wb["pot"], wb["dt"], pr, 200, 10, args.lanes)
                    for p, pr in (("mixed", E.MIXED), ("fp64", E.FP64))}
-        elif args.only == "nial_batch":
-            nb = W.nial_batch(512)
+        elif args.only in ("nial_batch", "nial_alloy"):
+            nb = W.nial_batch(512) if args.only == "nial_batch" else W.nial_alloy_batch(512)
             out = {p: run_batch_eval(E, ctx, nb["cells"], nb["pot"], pr, 50, args.lanes)
                    for p, pr in (("mixed", E.MIXED), ("fp64", E.FP64))}
         else:
@@ -424,6 +427,11 @@ def main():
         cells = nb["cells"]
         for name, prec in (("nial_batch512_mixed", E.MIXED), ("nial_batch512_fp64", E.FP64)):
             extra[name] = run_batch_eval(E, ctx, cells, nb["pot"], prec, 50)
+        # the same cells under the two-element alloy EAM (per-element
+        # densities and embeddings, a pair term per pair of elements)
+        na = W.nial_alloy_batch(512)
+        for name, prec in (("nial_alloy_batch512_mixed", E.MIXED), ("nial_alloy_batch512_fp64", E.FP64)):
+            extra[name] = run_batch_eval(E, ctx, na["cells"], na["pot"], prec, 50)
         seq = one_at_a_time(E, ctx, cells[:64], nb["pot"], 5)
         seq["note"] = "first 64 of the 512 cells, each in its own System, FP64, evaluated in turn"
         extra["nial_one_at_a_time_fp64"] = seq

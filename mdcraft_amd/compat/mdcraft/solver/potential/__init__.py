@@ -1,3 +1,3 @@
-from mdcraft_amd.refapi import (EAM, LJs, MLIP4, DeepModelingPotential, LJsCut, MaiseBPNNPotential, Potential,
+from mdcraft_amd.refapi import (EAM, EAMAlloy, LJs, MLIP4, DeepModelingPotential, LJsCut, MaiseBPNNPotential, Potential,
                                 PotentialTernary)
 from mdcraft_amd.refapi import TabulatedPair as Pair

@@ -472,6 +472,18 @@ class EAM(Potential):
         self._engine = E.EAM(context(), rcut=Rcutoff, F=F, rho=rho, rphi=rphi, dr=drPot, drho=drEmb)
 
 
+class EAMAlloy(Potential):
+    """Alloy EAM of 1..4 elements from an N-element setfl ("eam/alloy")
+    file, element e acting on atoms of kind e: each atom sums the density
+    function of every neighbour's element, embeds with its own element's
+    function and adds the pair term of the two elements.  Not in the
+    reference, whose EAM(filename) reads one element (EAM.cxx:55-87); it
+    plugs into Single like any potential."""
+
+    def __init__(self, filename):
+        self._engine = E.Alloy(context(), path=os.fspath(filename))
+
+
 class PotentialTernary(Potential):
     pass
 

@@ -341,3 +341,43 @@ def test_ljs_and_tabulated_pair(kind):
         getattr(solver, name)(atoms, atoms, nlist.get())
         getattr(o, name)()
         compare(atoms.data, o.a, CALL_TOL, name, fields=("f", "Ep", "V"))
+
+
+def test_alloy_script_matches_oracle(tmp_path):
+    """test-EAM.py's loop with the potential read from a three-element setfl
+    file (EAMAlloy) on a NiAl cell with copper on some aluminium sites."""
+    m = api()
+    from tests.test_gpu_alloy import with_copper
+    path = str(tmp_path / "NiAlCu.eam.alloy")
+    W.write_setfl(path, ["Ni", "Al", "Cu"])
+    at, box = W.nial_cell(4, 500)
+    at = with_copper(at, 4)
+    dt = 1 * W.FS
+    atoms = m.data.Atoms(len(at))
+    atoms.data[:] = at
+    domain = make_domain(m, box)
+    potential = m.solver.potential.EAMAlloy(path)
+    solver = m.solver.Single(potential=potential, boundary=m.solver.boundary.Periodic(domain))
+    stepper = m.solver.stepper.Verlet(solver=solver)
+    nlist = m.neibs.VerletList(atoms=atoms, neighbors=atoms, domain=domain)
+    nlist.update()
+    o = OracleScript(at, box, {"kind": "alloy", "path": path})
+    o.update()
+    solver.prepare(atoms, atoms, nlist.get())
+    o.prepare()
+    solver.charge(atoms, atoms, nlist.get())
+    o.charge()
+    solver.forces(atoms, atoms, nlist.get())
+    o.forces()
+    solver.virials(atoms, atoms, nlist.get())
+    o.virials()
+    compare(atoms.data, o.a, CALL_TOL, "virials")
+    for step in range(1, 21):
+        stepper.make_step(atoms, nlist.get(), dt)
+        o.make_step(dt)
+        if step % 10 == 0:
+            nlist.update()
+            o.update()
+    d = vec(atoms.data, "r") - vec(o.a, "r")
+    assert np.max(np.abs(d)) < RUN_R_ABS
+    assert max_rel(vec(atoms.data, "v"), vec(o.a, "v")) < RUN_REL
