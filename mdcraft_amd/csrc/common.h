@@ -44,6 +44,10 @@ struct SysDesc {
 // A list entry holds the neighbour index in bits 0-25 and, per axis, the
 // image shift (-1, 0, +1) + 1 of r_j - r_i in bits 26-27, 28-29, 30-31.
 constexpr unsigned kIdxMask = (1u << 26) - 1u;
+// image bits of an unshifted pair (1 - shift = 1 on every axis)
+constexpr unsigned kNoShift = (1u << 26) | (1u << 28) | (1u << 30);
+// step abort flag value that stops every step (a list that overflowed)
+constexpr int kAbortAll = -0x7fffffff - 1;
 constexpr int kMaxAtomsShift = 1 << 26;
 __host__ __device__ inline int ent_j(int e) { return (int)((unsigned)e & kIdxMask); }
 __host__ __device__ inline int ent_s(int e, int axis) { return (int)(((unsigned)e >> (26 + 2 * axis)) & 3u) - 1; }

@@ -249,7 +249,8 @@ void launch_k(Batch& b, int kind, void (*k)(P...), dim3 grid, dim3 block, unsign
 }
 
 // launchers (nlist.hip)
-void nlist_rebuild(Batch& b);
+void nlist_rebuild(Batch& b, bool sync = true);
+void nlist_regrow(Batch& b, int maxnn);
 int nlist_decide(Batch& b, double rcut);
 // launchers (forces.hip)
 void force_compute(Batch& b, const Pot& p, int what, int prec);

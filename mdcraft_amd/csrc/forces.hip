@@ -1157,21 +1157,33 @@ void run_steps(Batch& b, const Pot& p, double dt, int nsteps, int prec, int chun
         const int* cnt = b.ctx->h_pin + M_COUNT * (1 + P.slot);
         check_errors(b, cnt);
         const int ab = cnt[M_ABORT];
-        if (ab <= P.e) {
-            // steps [s, ab) ran; positions of step ab wait in buf(ab) (the
-            // last step of a sequence never raises the trigger, so ab <
-            // nsteps).  A chunk queued behind this one exits at once.
+        if (cnt[M_MAXNN] > b.cap) {
+            // the rebuild before this chunk overflowed the list, and the
+            // abort flag it raised stopped every step since (s = P.s): grow
+            // the list, build it again and go on from there
             if (nq) {
                 HIPCHECK(hipStreamSynchronize(st));
                 nq = 0;
                 head = 0;
             }
+            nlist_regrow(b, cnt[M_MAXNN]);
+            launch_s = s;
+            continue;
+        }
+        if (ab <= P.e) {
+            // steps [s, ab) ran; positions of step ab wait in buf(ab) (the
+            // last step of a sequence never raises the trigger, so ab <
+            // nsteps).  A chunk queued behind this one exits at once; the
+            // rebuild queues behind it without waiting, and its counters
+            // and event are never read.
+            nq = 0;
+            head = 0;
             b.since_rebuild += ab - s;
             b.pred_interval = b.pred_interval > 0 ? 0.5 * b.pred_interval + 0.5 * b.since_rebuild
                                                   : (double)b.since_rebuild;
             b.since_rebuild = 0;
             b.cur = buf(ab);
-            nlist_rebuild(b);
+            nlist_rebuild(b, false);
             timer_drain(b);
             c0 = b.cur ^ 1 ^ (ab & 1);
             s = launch_s = ab;  // the rebuild cleared the trigger

@@ -17,11 +17,38 @@
 
 namespace mdc {
 
+// The per-atom kernels of a rebuild take atoms by XCD the way the step
+// kernels do (xcd_block): each XCD binds, reorders and lists the slab of
+// cells whose positions its L2 already holds from the last step.
+
 // lattice::Domain::fit_in_period (domain.cxx:41-47) and the cell of an atom
 // in its system's grid (cells of edge >= max rns spanning the box).
+// Runs of equal keys among the active lanes of a wavefront (inactive lanes
+// above the last active one, as at the end of an array): whether this lane
+// starts a run, the run's length, and the lane of its head.
+struct RunOf {
+    bool head;
+    int len, lead;
+};
+__device__ inline RunOf run_of(int key, bool valid) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int prev = __shfl_up(key, 1, kWave);
+    const bool head = valid && (lane == 0 || prev != key);
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long live = __ballot(valid);
+    const unsigned long long upto = (2ull << lane) - 1ull;  // lanes 0..lane (all lanes at 63)
+    RunOf r;
+    r.head = head;
+    r.lead = 63 - __clzll(heads & upto);
+    const unsigned long long later = heads & ~upto;
+    const int end = later ? __ffsll((long long)later) - 1 : 64 - __clzll(live);
+    r.len = end - lane;
+    return r;
+}
+
 __global__ void k_wrap_count(double4* __restrict__ pos, const SysDesc* __restrict__ desc, int* __restrict__ atom_cell,
                              int* __restrict__ cell_count, int* __restrict__ misc, int n) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i == 0) {
         // a rebuild starts: clear the overflow maximum and the step trigger
         misc[M_MAXNN] = 0;
@@ -44,7 +71,9 @@ __global__ void k_wrap_count(double4* __restrict__ pos, const SysDesc* __restric
     pos[i] = p;
     int cid = s.cell_off + c[0] + s.nc[0] * (c[1] + s.nc[1] * c[2]);
     atom_cell[i] = cid;
-    atomicAdd(&cell_count[cid], 1);
+    // atoms come in the previous cell order: one add per run of equal cells
+    const RunOf run = run_of(cid, true);
+    if (run.head) atomicAdd(&cell_count[cid], run.len);
 }
 
 // Exclusive scan of the per-cell counts with one workgroup; also clears the
@@ -88,13 +117,18 @@ __global__ void __launch_bounds__(1024) k_scan_cells(int* __restrict__ count, in
     if (threadIdx.x == 0) start[ncells] = carry;
 }
 
+// Each run of equal cells in a wavefront reserves its slots with one
+// atomic, in lane order; the order of runs inside a cell is settled by
+// k_cell_sort.
 __global__ void k_scatter(const int* __restrict__ atom_cell, const int* __restrict__ start, int* __restrict__ fill,
                           int* __restrict__ src, int n) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int c = atom_cell[i];
-    int slot = start[c] + atomicAdd(&fill[c], 1);
-    src[slot] = i;
+    const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    const int c = valid ? atom_cell[i] : -1;
+    const RunOf r = run_of(c, valid);
+    int base = r.head ? atomicAdd(&fill[c], r.len) : 0;
+    base = __shfl(base, r.lead, kWave);
+    if (valid) src[start[c] + base + ((int)(threadIdx.x & (kWave - 1)) - r.lead)] = i;
 }
 
 // Deterministic order inside each cell: ascending previous index.  One
@@ -102,12 +136,15 @@ __global__ void k_scatter(const int* __restrict__ atom_cell, const int* __restri
 // fall back to an insertion sort by the first lane.
 __global__ void __launch_bounds__(256) k_cell_sort(const int* __restrict__ start, int* __restrict__ src, int ncells) {
     const int lane = threadIdx.x & 63;
-    const int c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int c = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (c >= ncells) return;
     const int b = start[c], e = start[c + 1], m = e - b;
     if (m <= 1) return;
     if (m <= kWave) {
         const int v = lane < m ? src[b + lane] : 0x7fffffff;
+        // most cells come out of the scatter in order already
+        const int nxt = __shfl_down(v, 1, kWave);
+        if (!__ballot(lane + 1 < m && nxt < v)) return;
         int rank = 0;
         for (int q = 0; q < kWave; ++q) {
             const int o = __shfl(v, q, kWave);
@@ -135,7 +172,7 @@ __global__ void k_permute(const int* __restrict__ src, const double4* __restrict
                           float* __restrict__ fout, const int* __restrict__ oin, int* __restrict__ oout,
                           const int* __restrict__ cell_in, int* __restrict__ cell_out, double4* __restrict__ rsave,
                           int* __restrict__ where, int n) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    int p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (p >= n) return;
     int s = src[p];
     double4 q = pos_in[s];
@@ -158,15 +195,55 @@ __global__ void k_permute(const int* __restrict__ src, const double4* __restrict
 // verlet-list.cxx:374-400, compacting the hits with a ballot so the slots
 // keep stencil order, then cell order.  The candidates are gathered once
 // per cell instead of once per atom.  Counts beyond the capacity are still
-// counted so the host can grow the list.
+// counted, and an overflow raises the step abort flag for every later step,
+// so that no step runs on the short list before the host grows it.
 constexpr int kTile = 448;
+// Four wavefronts per cell (two measured slower: less latency hidden).
+constexpr int kBuildThreads = 256;
+constexpr int kBuildWaves = kBuildThreads / kWave;
 
-__global__ void __launch_bounds__(256) k_build(const double4* __restrict__ pos, const double* __restrict__ rns,
+// The fast-path test of atom i against one staged tile (stencil image bits
+// carried by the candidates): SHIFT adds the candidate's image, skipped for
+// a home cell with no stencil cell across a periodic face (the shift is
+// then zero and fma(0, L, dx) is dx).  Returns the atom's running count.
+// ibase is the atom's slot 0 in the wavefront-packed list; lt = log2(tpa).
+template <bool SHIFT>
+__device__ inline int build_tile(const int* c_j, const double* c_x, const double* c_y, const double* c_z,
+                                 const double* c_r, int tn, int i, double4 pi, double ri, double L0, double L1,
+                                 double L2, int lane, int cnt, int* __restrict__ nl, long long ibase, int lt, int cap) {
+    const int tmask = (1 << lt) - 1;
+    for (int q0 = 0; q0 < tn; q0 += kWave) {
+        const int q = q0 + lane;
+        const bool in = q < tn;
+        const int qq = in ? q : 0;
+        const unsigned ent = (unsigned)c_j[qq];
+        const double rr = fmax(ri, c_r[qq]);
+        const double d2c = rr * rr;
+        double dx = pi.x - c_x[qq], dy = pi.y - c_y[qq], dz = pi.z - c_z[qq];
+        if (SHIFT) {
+            // 1 - stored bits = the shift of r_i - r_j; (+-1) * L is exact
+            dx = fma((double)(1 - (int)((ent >> 26) & 3u)), L0, dx);
+            dy = fma((double)(1 - (int)((ent >> 28) & 3u)), L1, dy);
+            dz = fma((double)(1 - (int)((ent >> 30) & 3u)), L2, dz);
+        }
+        const bool hit = in && (int)(ent & kIdxMask) != i && sq3_exact(dx, dy, dz) <= d2c;
+        const unsigned long long m = __ballot(hit);
+        if (hit) {
+            const int at = cnt + __popcll(m & ((1ull << lane) - 1ull));
+            if (at < cap) nl[ibase + (long long)(at >> lt) * kWave + (at & tmask)] = (int)ent;
+        }
+        cnt += __popcll(m);
+    }
+    return cnt;
+}
+
+__global__ void __launch_bounds__(kBuildThreads) k_build(const double4* __restrict__ pos, const double* __restrict__ rns,
                                                const int* __restrict__ start, const SysDesc* __restrict__ desc,
                                                int* __restrict__ nl, int* __restrict__ nn, int* __restrict__ misc,
                                                int tpa, int M, int cap) {
     __shared__ int s_beg[32], s_pre[32];
     __shared__ unsigned s_img[32];  // image bits of each stencil cell (fast path)
+    __shared__ int s_anyimg;        // some stencil cell lies across a periodic face
     __shared__ int c_j[kTile];
     __shared__ double c_x[kTile], c_y[kTile], c_z[kTile], c_r[kTile];
     __shared__ int s_cnt[64];
@@ -209,6 +286,8 @@ __global__ void __launch_bounds__(256) k_build(const double4* __restrict__ pos, 
                 cnt_c = start[cc + 1] - beg_c;
             }
         }
+        const unsigned long long wrapped = __ballot(lane < 27 && cnt_c > 0 && s_img[lane] != kNoShift);
+        if (lane == 0) s_anyimg = wrapped != 0ull;
         int pre = cnt_c;
         for (int o = 1; o < 32; o <<= 1) {
             const int y = __shfl_up(pre, o, kWave);
@@ -230,16 +309,23 @@ __global__ void __launch_bounds__(256) k_build(const double4* __restrict__ pos, 
     // L - cell inside.  The shift then comes with the candidate, and dx is
     // adjusted by the same exact add of +-L.
     const bool fast = (!p0 || s.nc[0] >= 5) && (!p1 || s.nc[1] >= 5) && (!p2 || s.nc[2] >= 5);
+    const bool shifted = s_anyimg;
+    int lt = 0;
+    while ((1 << lt) < tpa) ++lt;
+    const int lA = 6 - lt;  // log2 of the atoms per wavefront
     for (int g0 = a0; g0 < a1; g0 += 64) {
         const int ga = min(64, a1 - g0);
         if (tid < 64) s_cnt[tid] = 0;
         for (int t0 = 0; t0 < total; t0 += kTile) {
             const int tn = min(kTile, total - t0);
             __syncthreads();  // the previous tile is consumed and s_cnt is set
-            for (int q = tid; q < tn; q += 256) {
+            for (int q = tid; q < tn; q += kBuildThreads) {
                 const int idx = t0 + q;
+                // stencil cell of candidate idx: the count of inclusive
+                // prefixes <= idx (s_pre[27..31] hold the total, > idx)
                 int cc = 0;
-                while (s_pre[cc] <= idx) ++cc;
+                for (int step = 16; step > 0; step >>= 1)
+                    if (s_pre[cc + step - 1] <= idx) cc += step;
                 const int j = s_beg[cc] + idx - (cc > 0 ? s_pre[cc - 1] : 0);
                 const double4 pj = pos[j];
                 c_j[q] = fast ? (int)((unsigned)j | s_img[cc]) : j;
@@ -249,7 +335,7 @@ __global__ void __launch_bounds__(256) k_build(const double4* __restrict__ pos, 
                 c_r[q] = rns[j];
             }
             __syncthreads();
-            for (int a = wv; a < ga; a += 4) {
+            for (int a = wv; a < ga; a += kBuildWaves) {
                 const int i = g0 + a;
                 // the atom's own cell is stencil cell 13, staged from
                 // s_pre[12] on: read it from the tile when it is there
@@ -258,25 +344,18 @@ __global__ void __launch_bounds__(256) k_build(const double4* __restrict__ pos, 
                 const double4 pi = own ? make_double4(c_x[qi], c_y[qi], c_z[qi], 0.0) : pos[i];
                 const double ri = own ? c_r[qi] : rns[i];
                 int cnt = s_cnt[a];
-                for (int q0 = 0; q0 < tn; q0 += kWave) {
+                if (fast) {
+                    const long long ibase = ((long long)(i >> lA) * M) * kWave + ((i & ((1 << lA) - 1)) << lt);
+                    cnt = shifted ? build_tile<true>(c_j, c_x, c_y, c_z, c_r, tn, i, pi, ri, L0, L1, L2, lane, cnt,
+                                                     nl, ibase, lt, cap)
+                                  : build_tile<false>(c_j, c_x, c_y, c_z, c_r, tn, i, pi, ri, L0, L1, L2, lane, cnt,
+                                                      nl, ibase, lt, cap);
+                }
+                for (int q0 = 0; !fast && q0 < tn; q0 += kWave) {
                     const int q = q0 + lane;
                     bool hit = false;
                     unsigned ent = 0;
-                    if (fast) {
-                        if (q < tn) {
-                            ent = (unsigned)c_j[q];
-                            if ((int)(ent & kIdxMask) != i) {
-                                const double rr = fmax(ri, c_r[q]);
-                                const double d2c = rr * rr;
-                                double dx = pi.x - c_x[q], dy = pi.y - c_y[q], dz = pi.z - c_z[q];
-                                // 1 - stored bits = the shift of r_i - r_j; (+-1) * L is exact
-                                dx = fma((double)(1 - (int)((ent >> 26) & 3u)), L0, dx);
-                                dy = fma((double)(1 - (int)((ent >> 28) & 3u)), L1, dy);
-                                dz = fma((double)(1 - (int)((ent >> 30) & 3u)), L2, dz);
-                                hit = sq3_exact(dx, dy, dz) <= d2c;
-                            }
-                        }
-                    } else if (q < tn) {
+                    if (q < tn) {
                         const int j = c_j[q];
                         if (j != i) {  // j < 2^26 (checked on the host)
                             const double rr = fmax(ri, c_r[q]);
@@ -310,7 +389,10 @@ __global__ void __launch_bounds__(256) k_build(const double4* __restrict__ pos, 
             const int cnt = s_cnt[tid];
             nn[g0 + tid] = cnt;
             // the host needs the maximum only when the list overflowed
-            if (cnt > cap) atomicMax(&misc[M_MAXNN], cnt);
+            if (cnt > cap) {
+                atomicMax(&misc[M_MAXNN], cnt);
+                atomicMin(&misc[M_ABORT], kAbortAll);
+            }
         }
     }
 }
@@ -329,7 +411,47 @@ __global__ void k_decide(const double4* __restrict__ pos, const double4* __restr
 
 static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
 
-void nlist_rebuild(Batch& b) {
+// A list of capacity for maxnn neighbours per atom (a quarter more, in
+// whole groups of lanes).
+static void grow_list(Batch& b, int maxnn) {
+    const int cap = ((maxnn + maxnn / 4 + 8 + b.tpa - 1) / b.tpa) * b.tpa;
+    const int A = kWave / b.tpa;
+    const size_t waves = (size_t)(b.n + A - 1) / A;
+    const size_t elems = waves * (size_t)(cap / b.tpa) * kWave;
+    if (b.nl) HIPCHECK(hipFree(b.nl));
+    b.nl = nullptr;
+    if (hipMalloc(&b.nl, elems * sizeof(int)) != hipSuccess)
+        throw Error(E_NOMEM, "out of device memory growing the neighbour list");
+    b.cap = cap;
+    b.M = cap / b.tpa;
+    b.nl_elems = elems;
+}
+
+static void launch_build(Batch& b) {
+    hipLaunchKernelGGL(k_build, dim3(b.ncells), dim3(kBuildThreads), 0, b.ctx->stream, b.pos[b.cur], b.d(b.sc, D_RNS),
+                       b.cell_start, b.d_desc, b.nl, b.nn, b.misc, b.tpa, b.M, b.cap);
+}
+
+// Build the list again after an overflow: grow it until every atom fits,
+// then clear the abort flag the overflow raised.
+void nlist_regrow(Batch& b, int maxnn) {
+    hipStream_t st = b.ctx->stream;
+    while (maxnn > b.cap) {
+        grow_list(b, maxnn);
+        HIPCHECK(hipMemsetAsync(b.misc + M_MAXNN, 0, sizeof(int), st));
+        launch_build(b);
+        HIPCHECK(hipMemcpyAsync(b.ctx->h_pin, b.misc, sizeof(int) * M_COUNT, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        maxnn = b.ctx->h_pin[M_MAXNN];
+    }
+    const int abort_init = 0x7fffffff;
+    HIPCHECK(hipMemcpyAsync(b.misc + M_ABORT, &abort_init, sizeof(int), hipMemcpyHostToDevice, st));
+}
+
+// sync: wait for the build and grow the list now if it overflowed.
+// Otherwise the overflow shows in the counters of the next steps, whose
+// kernels the raised abort flag stops (the stepping loop regrows).
+void nlist_rebuild(Batch& b, bool sync) {
     hipStream_t st = b.ctx->stream;
     const int n = b.n;
     if (n == 0) {
@@ -349,26 +471,12 @@ void nlist_rebuild(Batch& b) {
                        b.rsave, b.where, n);
     b.cur ^= 1;
     b.sc = s1;
-    for (int attempt = 0; attempt < 3; ++attempt) {
-        if (attempt > 0) HIPCHECK(hipMemsetAsync(b.misc + M_MAXNN, 0, sizeof(int), st));
-        hipLaunchKernelGGL(k_build, dim3(b.ncells), dim3(256), 0, st, b.pos[b.cur], b.d(b.sc, D_RNS), b.cell_start,
-                           b.d_desc, b.nl, b.nn, b.misc, b.tpa, b.M, b.cap);
+    launch_build(b);
+    if (sync) {
         HIPCHECK(hipMemcpyAsync(b.ctx->h_pin, b.misc, sizeof(int) * M_COUNT, hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
-        int maxnn = b.ctx->h_pin[M_MAXNN];
-        if (maxnn <= b.cap) break;  // zero unless some atom overflowed
-        // grow the list and rebuild it
-        int cap = ((maxnn + maxnn / 4 + 8 + b.tpa - 1) / b.tpa) * b.tpa;
-        int A = kWave / b.tpa;
-        size_t waves = (size_t)(n + A - 1) / A;
-        size_t elems = waves * (size_t)(cap / b.tpa) * kWave;
-        if (b.nl) HIPCHECK(hipFree(b.nl));
-        b.nl = nullptr;
-        if (hipMalloc(&b.nl, elems * sizeof(int)) != hipSuccess)
-            throw Error(E_NOMEM, "out of device memory growing the neighbour list");
-        b.cap = cap;
-        b.M = cap / b.tpa;
-        b.nl_elems = elems;
+        const int maxnn = b.ctx->h_pin[M_MAXNN];  // zero unless some atom overflowed
+        if (maxnn > b.cap) nlist_regrow(b, maxnn);
     }
     b.list_valid = true;
     b.rebuilds++;
