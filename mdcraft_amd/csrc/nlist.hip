@@ -21,8 +21,6 @@ namespace mdc {
 // kernels do (xcd_block): each XCD binds, reorders and lists the slab of
 // cells whose positions its L2 already holds from the last step.
 
-// lattice::Domain::fit_in_period (domain.cxx:41-47) and the cell of an atom
-// in its system's grid (cells of edge >= max rns spanning the box).
 // Runs of equal keys among the active lanes of a wavefront (inactive lanes
 // above the last active one, as at the end of an array): whether this lane
 // starts a run, the run's length, and the lane of its head.
@@ -46,80 +44,90 @@ __device__ inline RunOf run_of(int key, bool valid) {
     return r;
 }
 
-__global__ void k_wrap_count(double4* __restrict__ pos, const SysDesc* __restrict__ desc, int* __restrict__ atom_cell,
-                             int* __restrict__ cell_count, int* __restrict__ misc, int n) {
-    int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (i == 0) {
-        // a rebuild starts: clear the overflow maximum and the step trigger
-        misc[M_MAXNN] = 0;
-        misc[M_ABORT] = 0x7fffffff;
-    }
-    if (i >= n) return;
-    double4 p = pos[i];
-    const SysDesc& s = desc[w_sys(p.w)];
-    double r[3] = {p.x, p.y, p.z};
-    int c[3];
-    for (int a = 0; a < 3; ++a) {
-        if (s.per[a]) r[a] = r[a] < s.lo[a] ? r[a] + s.L[a] : r[a] > s.hi[a] ? r[a] - s.L[a] : r[a];
-        double q = floor((r[a] - s.lo[a]) / s.csize[a]);
-        int ci = q < 0.0 ? 0 : (q >= (double)s.nc[a] ? s.nc[a] - 1 : (int)q);
-        c[a] = ci;
-    }
-    p.x = r[0];
-    p.y = r[1];
-    p.z = r[2];
-    pos[i] = p;
-    int cid = s.cell_off + c[0] + s.nc[0] * (c[1] + s.nc[1] * c[2]);
-    atom_cell[i] = cid;
-    // atoms come in the previous cell order: one add per run of equal cells
-    const RunOf run = run_of(cid, true);
-    if (run.head) atomicAdd(&cell_count[cid], run.len);
-}
-
-// Exclusive scan of the per-cell counts with one workgroup; also clears the
-// fill counters.  start has ncells + 1 entries.
+// Exclusive scan of the per-cell counts by one workgroup of 1024 threads;
+// also clears the counts and the fill counters for the next rebuild.  start
+// has ncells + 1 entries.  (A scan by the binning kernel's last workgroup
+// measured slower than this launch.)
 __global__ void __launch_bounds__(1024) k_scan_cells(int* __restrict__ count, int* __restrict__ start,
                                                      int* __restrict__ fill, int ncells) {
     __shared__ int warp_tot[16];
     __shared__ int carry;
+    const int nt = blockDim.x, nw = nt >> 6;
     if (threadIdx.x == 0) carry = 0;
     __syncthreads();
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int base = 0; base < ncells; base += 1024) {
-        int idx = base + threadIdx.x;
-        int v = idx < ncells ? count[idx] : 0;
+    for (int base = 0; base < ncells; base += nt) {
+        const int idx = base + threadIdx.x;
+        const int v = idx < ncells ? count[idx] : 0;
         if (idx < ncells) {
             fill[idx] = 0;
-            count[idx] = 0;  // ready for the next rebuild's counting
+            count[idx] = 0;
         }
         int x = v;
         for (int o = 1; o < 64; o <<= 1) {
-            int y = __shfl_up(x, o, 64);
+            const int y = __shfl_up(x, o, 64);
             if (lane >= o) x += y;
         }
         if (lane == 63) warp_tot[wv] = x;
         __syncthreads();
         if (wv == 0) {
-            int t = lane < 16 ? warp_tot[lane] : 0;
+            int t = lane < nw ? warp_tot[lane] : 0;
             for (int o = 1; o < 16; o <<= 1) {
-                int y = __shfl_up(t, o, 64);
+                const int y = __shfl_up(t, o, 64);
                 if (lane >= o) t += y;
             }
-            if (lane < 16) warp_tot[lane] = t;
+            if (lane < nw) warp_tot[lane] = t;
         }
         __syncthreads();
-        int before = carry + (wv > 0 ? warp_tot[wv - 1] : 0) + x - v;
+        const int before = carry + (wv > 0 ? warp_tot[wv - 1] : 0) + x - v;
         if (idx < ncells) start[idx] = before;
         __syncthreads();
-        if (threadIdx.x == 1023) carry = before + v;
+        if (threadIdx.x == nt - 1) carry = before + v;
         __syncthreads();
     }
     if (threadIdx.x == 0) start[ncells] = carry;
 }
 
+// lattice::Domain::fit_in_period (domain.cxx:41-47) and the cell of an atom
+// in its system's grid (cells of edge >= max rns spanning the box), counted
+// per cell.
+__global__ void __launch_bounds__(256) k_wrap_count(double4* __restrict__ pos, const SysDesc* __restrict__ desc,
+                                                    int* __restrict__ atom_cell, int* __restrict__ cell_count,
+                                                    int* __restrict__ misc, int n) {
+    const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        // a rebuild starts: clear the overflow maximum and the step trigger
+        misc[M_MAXNN] = 0;
+        misc[M_ABORT] = 0x7fffffff;
+    }
+    const bool valid = i < n;
+    int cid = -1;
+    if (valid) {
+        double4 p = pos[i];
+        const SysDesc& s = desc[w_sys(p.w)];
+        double r[3] = {p.x, p.y, p.z};
+        int c[3];
+        for (int a = 0; a < 3; ++a) {
+            if (s.per[a]) r[a] = r[a] < s.lo[a] ? r[a] + s.L[a] : r[a] > s.hi[a] ? r[a] - s.L[a] : r[a];
+            double q = floor((r[a] - s.lo[a]) / s.csize[a]);
+            int ci = q < 0.0 ? 0 : (q >= (double)s.nc[a] ? s.nc[a] - 1 : (int)q);
+            c[a] = ci;
+        }
+        p.x = r[0];
+        p.y = r[1];
+        p.z = r[2];
+        pos[i] = p;
+        cid = s.cell_off + c[0] + s.nc[0] * (c[1] + s.nc[1] * c[2]);
+        atom_cell[i] = cid;
+    }
+    // atoms come in the previous cell order: one add per run of equal cells
+    const RunOf run = run_of(cid, valid);
+    if (run.head) atomicAdd(&cell_count[cid], run.len);
+}
+
 // Each run of equal cells in a wavefront reserves its slots with one
 // atomic, in lane order; the order of runs inside a cell is settled by
-// k_cell_sort.
+// k_permute.
 __global__ void k_scatter(const int* __restrict__ atom_cell, const int* __restrict__ start, int* __restrict__ fill,
                           int* __restrict__ src, int n) {
     const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -131,59 +139,35 @@ __global__ void k_scatter(const int* __restrict__ atom_cell, const int* __restri
     if (valid) src[start[c] + base + ((int)(threadIdx.x & (kWave - 1)) - r.lead)] = i;
 }
 
-// Deterministic order inside each cell: ascending previous index.  One
-// wavefront per cell ranks up to 64 entries with shuffles; larger cells
-// fall back to an insertion sort by the first lane.
-__global__ void __launch_bounds__(256) k_cell_sort(const int* __restrict__ start, int* __restrict__ src, int ncells) {
-    const int lane = threadIdx.x & 63;
-    const int c = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (c >= ncells) return;
-    const int b = start[c], e = start[c + 1], m = e - b;
-    if (m <= 1) return;
-    if (m <= kWave) {
-        const int v = lane < m ? src[b + lane] : 0x7fffffff;
-        // most cells come out of the scatter in order already
-        const int nxt = __shfl_down(v, 1, kWave);
-        if (!__ballot(lane + 1 < m && nxt < v)) return;
-        int rank = 0;
-        for (int q = 0; q < kWave; ++q) {
-            const int o = __shfl(v, q, kWave);
-            rank += (q < m && o < v) ? 1 : 0;
-        }
-        if (lane < m) src[b + rank] = v;
-        return;
-    }
-    if (lane != 0) return;
-    for (int k = b + 1; k < e; ++k) {
-        int v = src[k];
-        int q = k - 1;
-        while (q >= b && src[q] > v) {
-            src[q + 1] = src[q];
-            --q;
-        }
-        src[q + 1] = v;
-    }
-}
-
-// Move every per-atom array into the new (cell) order; record the cell of
+// Move every per-atom array into the new (cell) order, ascending previous
+// index inside each cell: the thread of scatter slot sp takes atom s =
+// src[sp] to slot start[c] + (the number of the cell's atoms with a smaller
+// index), so the scatter's run order never shows.  Also records the cell of
 // each sorted atom and the displacement reference positions.
-__global__ void k_permute(const int* __restrict__ src, const double4* __restrict__ pos_in, double4* __restrict__ pos_out,
-                          const double* __restrict__ din, double* __restrict__ dout, const float* __restrict__ fin,
-                          float* __restrict__ fout, const int* __restrict__ oin, int* __restrict__ oout,
-                          const int* __restrict__ cell_in, int* __restrict__ cell_out, double4* __restrict__ rsave,
-                          int* __restrict__ where, int n) {
-    int p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    int s = src[p];
-    double4 q = pos_in[s];
+__global__ void __launch_bounds__(256) k_permute(const int* __restrict__ src, const int* __restrict__ start,
+                                                 const double4* __restrict__ pos_in, double4* __restrict__ pos_out,
+                                                 const double* __restrict__ din, double* __restrict__ dout,
+                                                 const float* __restrict__ fin, float* __restrict__ fout,
+                                                 const int* __restrict__ oin, int* __restrict__ oout,
+                                                 const int* __restrict__ cell_in, int* __restrict__ cell_out,
+                                                 double4* __restrict__ rsave, int* __restrict__ where, int n) {
+    const int sp = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (sp >= n) return;
+    const int s = src[sp];
+    const int c = cell_in[s];
+    const int b = start[c], e = start[c + 1];
+    int rank = 0;
+    for (int q = b; q < e; ++q) rank += src[q] < s ? 1 : 0;
+    const int p = b + rank;
+    const double4 q = pos_in[s];
     pos_out[p] = q;
     rsave[p] = q;
     for (int f = 0; f < D_NFIELDS; ++f) dout[(size_t)f * n + p] = din[(size_t)f * n + s];
     for (int f = 0; f < F_NFIELDS; ++f) fout[(size_t)f * n + p] = fin[(size_t)f * n + s];
-    int o = oin[s];
+    const int o = oin[s];
     oout[p] = o;
     where[o] = p;
-    cell_out[p] = cell_in[s];
+    cell_out[p] = c;
 }
 
 // One workgroup per cell.  Wave 0 lays out the 27-cell stencil (with the
@@ -201,40 +185,60 @@ constexpr int kTile = 448;
 // Four wavefronts per cell (two measured slower: less latency hidden).
 constexpr int kBuildThreads = 256;
 constexpr int kBuildWaves = kBuildThreads / kWave;
+// atoms a wave tests together on the fast path: two chains hide the FP64
+// latency; three or four need more than 64 VGPRs and halve the occupancy
+constexpr int kGroup = 2;
 
-// The fast-path test of atom i against one staged tile (stencil image bits
-// carried by the candidates): SHIFT adds the candidate's image, skipped for
-// a home cell with no stencil cell across a periodic face (the shift is
-// then zero and fma(0, L, dx) is dx).  Returns the atom's running count.
-// ibase is the atom's slot 0 in the wavefront-packed list; lt = log2(tpa).
-template <bool SHIFT>
-__device__ inline int build_tile(const int* c_j, const double* c_x, const double* c_y, const double* c_z,
-                                 const double* c_r, int tn, int i, double4 pi, double ri, double L0, double L1,
-                                 double L2, int lane, int cnt, int* __restrict__ nl, long long ibase, int lt, int cap) {
+// The fast-path test of NA atoms of the cell against one staged tile
+// (stencil image bits carried by the candidates): the atoms share each
+// candidate's LDS reads and image decode, and their chains are independent.
+// SHIFT adds the candidate's image, skipped for a home cell with no stencil
+// cell across a periodic face (the shift is then zero and fma(0, L, dx) is
+// dx).  ibase[t] is atom t's slot 0 in the wavefront-packed list; lt =
+// log2(tpa); cnt[t] is atom t's running count.
+template <bool SHIFT, int NA>
+__device__ inline void build_tile(const int* c_j, const double* c_x, const double* c_y, const double* c_z,
+                                  const double* c_r, int tn, const int* ia, const double4* pa, const double* ra,
+                                  double L0, double L1, double L2, int lane, int* cnt, int* __restrict__ nl,
+                                  const long long* ibase, int lt, int cap) {
     const int tmask = (1 << lt) - 1;
     for (int q0 = 0; q0 < tn; q0 += kWave) {
         const int q = q0 + lane;
         const bool in = q < tn;
         const int qq = in ? q : 0;
         const unsigned ent = (unsigned)c_j[qq];
-        const double rr = fmax(ri, c_r[qq]);
-        const double d2c = rr * rr;
-        double dx = pi.x - c_x[qq], dy = pi.y - c_y[qq], dz = pi.z - c_z[qq];
+        const double cr = c_r[qq], cx = c_x[qq], cy = c_y[qq], cz = c_z[qq];
+        const int jj = (int)(ent & kIdxMask);
+        double sx = 0.0, sy = 0.0, sz = 0.0;
         if (SHIFT) {
             // 1 - stored bits = the shift of r_i - r_j; (+-1) * L is exact
-            dx = fma((double)(1 - (int)((ent >> 26) & 3u)), L0, dx);
-            dy = fma((double)(1 - (int)((ent >> 28) & 3u)), L1, dy);
-            dz = fma((double)(1 - (int)((ent >> 30) & 3u)), L2, dz);
+            sx = (double)(1 - (int)((ent >> 26) & 3u));
+            sy = (double)(1 - (int)((ent >> 28) & 3u));
+            sz = (double)(1 - (int)((ent >> 30) & 3u));
         }
-        const bool hit = in && (int)(ent & kIdxMask) != i && sq3_exact(dx, dy, dz) <= d2c;
-        const unsigned long long m = __ballot(hit);
-        if (hit) {
-            const int at = cnt + __popcll(m & ((1ull << lane) - 1ull));
-            if (at < cap) nl[ibase + (long long)(at >> lt) * kWave + (at & tmask)] = (int)ent;
+        bool hit[NA];
+#pragma unroll
+        for (int t = 0; t < NA; ++t) {
+            const double rr = fmax(ra[t], cr);
+            const double d2c = rr * rr;
+            double dx = pa[t].x - cx, dy = pa[t].y - cy, dz = pa[t].z - cz;
+            if (SHIFT) {
+                dx = fma(sx, L0, dx);
+                dy = fma(sy, L1, dy);
+                dz = fma(sz, L2, dz);
+            }
+            hit[t] = in && jj != ia[t] && sq3_exact(dx, dy, dz) <= d2c;
         }
-        cnt += __popcll(m);
+#pragma unroll
+        for (int t = 0; t < NA; ++t) {
+            const unsigned long long m = __ballot(hit[t]);
+            if (hit[t]) {
+                const int at = cnt[t] + __popcll(m & ((1ull << lane) - 1ull));
+                if (at < cap) nl[ibase[t] + (long long)(at >> lt) * kWave + (at & tmask)] = (int)ent;
+            }
+            cnt[t] += __popcll(m);
+        }
     }
-    return cnt;
 }
 
 __global__ void __launch_bounds__(kBuildThreads) k_build(const double4* __restrict__ pos, const double* __restrict__ rns,
@@ -335,23 +339,57 @@ __global__ void __launch_bounds__(kBuildThreads) k_build(const double4* __restri
                 c_r[q] = rns[j];
             }
             __syncthreads();
-            for (int a = wv; a < ga; a += kBuildWaves) {
-                const int i = g0 + a;
-                // the atom's own cell is stencil cell 13, staged from
-                // s_pre[12] on: read it from the tile when it is there
+            // the atom's own cell is stencil cell 13, staged from s_pre[12]
+            // on: read it from the tile when it is there
+            auto atom_in = [&](int a, int& i, double4& pi, double& ri) {
+                i = g0 + a;
                 const int qi = s_pre[12] + (i - a0) - t0;
                 const bool own = qi >= 0 && qi < tn;
-                const double4 pi = own ? make_double4(c_x[qi], c_y[qi], c_z[qi], 0.0) : pos[i];
-                const double ri = own ? c_r[qi] : rns[i];
-                int cnt = s_cnt[a];
-                if (fast) {
-                    const long long ibase = ((long long)(i >> lA) * M) * kWave + ((i & ((1 << lA) - 1)) << lt);
-                    cnt = shifted ? build_tile<true>(c_j, c_x, c_y, c_z, c_r, tn, i, pi, ri, L0, L1, L2, lane, cnt,
-                                                     nl, ibase, lt, cap)
-                                  : build_tile<false>(c_j, c_x, c_y, c_z, c_r, tn, i, pi, ri, L0, L1, L2, lane, cnt,
-                                                      nl, ibase, lt, cap);
+                pi = own ? make_double4(c_x[qi], c_y[qi], c_z[qi], 0.0) : pos[i];
+                ri = own ? c_r[qi] : rns[i];
+            };
+            // fast path: kGroup atoms a wave at a time
+            for (int a = wv; fast && a < ga; a += kGroup * kBuildWaves) {
+                int ia[kGroup], cnt[kGroup];
+                double4 pa[kGroup];
+                double ra[kGroup];
+                long long ibase[kGroup];
+                int na = 0;
+                for (int t = 0; t < kGroup; ++t) {
+                    const int at = a + t * kBuildWaves;
+                    ia[t] = 0;
+                    pa[t] = make_double4(0, 0, 0, 0);
+                    ra[t] = 0.0;
+                    cnt[t] = 0;
+                    if (at < ga) {
+                        atom_in(at, ia[t], pa[t], ra[t]);
+                        cnt[t] = s_cnt[at];
+                        na = t + 1;
+                    }
+                    ibase[t] = ((long long)(ia[t] >> lA) * M) * kWave + ((ia[t] & ((1 << lA) - 1)) << lt);
                 }
-                for (int q0 = 0; !fast && q0 < tn; q0 += kWave) {
+                auto run = [&](auto shift_tag) {
+                    constexpr bool SH = decltype(shift_tag)::value;
+                    static_assert(kGroup == 2, "one instantiation per group size");
+                    if (na == 2)
+                        build_tile<SH, 2>(c_j, c_x, c_y, c_z, c_r, tn, ia, pa, ra, L0, L1, L2, lane, cnt, nl, ibase, lt,
+                                          cap);
+                    else
+                        build_tile<SH, 1>(c_j, c_x, c_y, c_z, c_r, tn, ia, pa, ra, L0, L1, L2, lane, cnt, nl, ibase, lt,
+                                          cap);
+                };
+                if (shifted) run(std::true_type{});
+                else run(std::false_type{});
+                if (lane == 0)
+                    for (int t = 0; t < na; ++t) s_cnt[a + t * kBuildWaves] = cnt[t];
+            }
+            for (int a = wv; !fast && a < ga; a += kBuildWaves) {
+                int i;
+                double4 pi;
+                double ri;
+                atom_in(a, i, pi, ri);
+                int cnt = s_cnt[a];
+                for (int q0 = 0; q0 < tn; q0 += kWave) {
                     const int q = q0 + lane;
                     bool hit = false;
                     unsigned ent = 0;
@@ -463,12 +501,11 @@ void nlist_rebuild(Batch& b, bool sync) {
     hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, st, b.cell_count, b.cell_start, b.cell_fill, b.ncells);
     hipLaunchKernelGGL(k_scatter, dim3(nblk(n, 256)), dim3(256), 0, st, b.atom_cell, b.cell_start, b.cell_fill,
                        b.sorted_src, n);
-    hipLaunchKernelGGL(k_cell_sort, dim3(nblk(b.ncells, 4)), dim3(256), 0, st, b.cell_start, b.sorted_src, b.ncells);
     const int s0 = b.sc, s1 = b.sc ^ 1;
     // cell_fill is free after the scatter: reuse it for the sorted cell ids
-    hipLaunchKernelGGL(k_permute, dim3(nblk(n, 256)), dim3(256), 0, st, b.sorted_src, b.pos[b.cur], b.pos[b.cur ^ 1],
-                       b.dst[s0], b.dst[s1], b.fst[s0], b.fst[s1], b.orig[s0], b.orig[s1], b.atom_cell, b.cell_fill,
-                       b.rsave, b.where, n);
+    hipLaunchKernelGGL(k_permute, dim3(nblk(n, 256)), dim3(256), 0, st, b.sorted_src, b.cell_start, b.pos[b.cur],
+                       b.pos[b.cur ^ 1], b.dst[s0], b.dst[s1], b.fst[s0], b.fst[s1], b.orig[s0], b.orig[s1],
+                       b.atom_cell, b.cell_fill, b.rsave, b.where, n);
     b.cur ^= 1;
     b.sc = s1;
     launch_build(b);
