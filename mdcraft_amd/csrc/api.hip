@@ -296,7 +296,6 @@ void free_pot(Pot* p) {
     dfree(p->eam.r64);
     dfree(p->eam.r32);
     dfree(p->eam.F64);
-    dfree(p->eam.knots32);
     dfree(p->eam.objs);
     delete p;
 }

@@ -116,7 +116,6 @@ struct EamTables {
     double4* r64 = nullptr;
     float4* r32 = nullptr;
     double4* F64 = nullptr;
-    float2* knots32 = nullptr;  // (y, z) knots of every r table for LDS staging: [ntab][nr+1]
     // Multi potential (nk > 1): objects per pair of kinds, records of all
     // objects concatenated in r64 / r32 / F64.  alloy: the nk elements of an
     // alloy EAM (per-element density and embedding, a pair term per pair of

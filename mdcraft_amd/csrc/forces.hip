@@ -214,10 +214,7 @@ __device__ inline void embed_at(const double4* __restrict__ F64, int off, int nr
 // dependent table reads are in flight together.  Two per group measured
 // fastest at both precisions: four keep too many registers live (the FP64
 // force kernel spills) and leave fewer waves to hide the gathers.
-#ifndef MDC_KU
-#define MDC_KU 2
-#endif
-constexpr int kU = MDC_KU;
+constexpr int kU = 2;
 
 template <typename R>
 struct PairGeo {
@@ -311,7 +308,7 @@ __global__ void __launch_bounds__(256) k_eam_rho(const double4* __restrict__ pos
         for (int u = 0; u < kU; ++u) {
             const int k = (m0 + u) * TPA + as.sub;
             okk[u] = k < cnt;
-            e[u] = okk[u] ? (MDC_PREFETCH || m0 == 0 ? jn[u] : nl[as.base + (long long)(m0 + u) * kWave]) : 0;
+            e[u] = okk[u] ? jn[u] : 0;
         }
         double4 pj[kU];
 #pragma unroll
@@ -435,7 +432,7 @@ __global__ void __launch_bounds__(256) k_eam_force(const double4* __restrict__ p
         for (int u = 0; u < kU; ++u) {
             const int k = (m0 + u) * TPA + as.sub;
             okk[u] = k < cnt;
-            e[u] = okk[u] ? (MDC_PREFETCH || m0 == 0 ? jn[u] : nl[as.base + (long long)(m0 + u) * kWave]) : 0;
+            e[u] = okk[u] ? jn[u] : 0;
         }
         double4 pj[kU];
         R Fpj[kU];

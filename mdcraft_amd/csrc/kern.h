@@ -173,13 +173,8 @@ __device__ inline void first_group(const AtomSlots& as, bool valid, int M, const
 
 // The entries of the group after the one starting at chunk m0, read while
 // the current group's gathers are in flight (chunks past M are not read).
-// MDC_PREFETCH = 0 reads each group at the top of its own iteration.
-#ifndef MDC_PREFETCH
-#define MDC_PREFETCH 1
-#endif
 template <int TPA, int U>
 __device__ inline void next_group(const AtomSlots& as, int m0, int M, const int* __restrict__ nl, int* en) {
-    if (!MDC_PREFETCH) return;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const int c = m0 + U + u;
