@@ -322,36 +322,104 @@ def summarise_stats(d, kernel):
     return "\n".join(lines)
 
 
-def cpu_baseline(w, max_seconds=20.0, max_steps=None):
+def engine_check(E, ctx, w):
+    """The engine's FP64 and mixed evaluation (forces, energies, virial) of
+    the workload's first configuration and its neighbour pairs, for the
+    oracle's comparison in cpu_baseline."""
+    s = E.System(ctx, [len(w["atoms"])], [w["box"]], True)
+    s.upload(w["atoms"])
+    p = make_pot(E, ctx, w["pot"])
+    s.build()
+    out = {}
+    for name, prec in (("fp64", E.FP64), ("mixed", E.MIXED)):
+        s.compute(p, E.FORCE | E.ENERGY | E.VIRIAL, prec)
+        out[name] = s.download()
+    out["pairs"] = s.export()
+    s.close()
+    return out
+
+
+def _vec(a, f):
+    return np.stack([a[f]["x"], a[f]["y"], a[f]["z"]], axis=1)
+
+
+def _pair_keys(off, ids):
+    n = len(off) - 1
+    return np.sort(np.repeat(np.arange(n, dtype=np.int64), np.diff(off)) * n + ids)
+
+
+def accuracy(check, ref, ref_pairs):
+    """Errors of the engine's results against the oracle's on the same
+    configuration: FP64 max |df| over max |f|, mixed force RMS over the
+    force RMS, relative total energy and virial trace; the neighbour pair
+    sets compared exactly."""
+    fr = _vec(ref, "f")
+    e_ref = ref["Ep"].sum() + ref["Em"].sum()
+    w_ref = (ref["V"]["xx"] + ref["V"]["yy"] + ref["V"]["zz"]).sum()
+    out = {}
+    for name in ("fp64", "mixed"):
+        a = check[name]
+        f = _vec(a, "f")
+        e = a["Ep"].sum() + a["Em"].sum()
+        wv = (a["V"]["xx"] + a["V"]["yy"] + a["V"]["zz"]).sum()
+        out[name] = {"force_max_rel": float(np.abs(f - fr).max() / np.abs(fr).max()),
+                     "force_rms_rel": float(np.sqrt(np.mean(np.sum((f - fr) ** 2, 1)) / np.mean(np.sum(fr ** 2, 1)))),
+                     "energy_rel": float(abs(e - e_ref) / abs(e_ref)), "virial_trace_rel": float(abs(wv - w_ref) / abs(w_ref))}
+    got, want = _pair_keys(*check["pairs"]), _pair_keys(*ref_pairs)
+    out["pair_sets_equal"] = bool(len(got) == len(want) and np.array_equal(got, want))
+    out["pairs"] = int(len(want))
+    out["bars"] = "FP64 <= 1e-12 relative; mixed force RMS <= 1e-5; pair sets bit-exact"
+    return out
+
+
+def cpu_baseline(w, max_seconds=20.0, max_steps=None, check=None):
     """The FP64 oracle (the reference algorithm: AoS records, full lists
     with self, Periodic ghost pass, decide()/update() rebuilds, std::thread
-    pool with the reference chunking rule) on the host cores."""
+    pool with the reference chunking rule) on the host cores, then on one
+    core; with check (engine_check's results) the oracle also evaluates the
+    same first configuration as the checker of the engine's."""
     from oracle import refcpu as R
     threads = min(16, os.cpu_count() or 1)
     pot = w["pot"]
-    a = w["atoms"].copy()
     dom = R.Domain(*w["box"])
-    pool = R.Pool(threads)
     p = R.LJsCut(pot["a"], pot["s"], pot["rc"]) if pot["kind"] == "lj" else R.EAM(
         rc=pot["rc"], F=pot["F"], rho=pot["rho"], rphi=pot["rphi"], dr=pot["dr"], drho=pot["drho"])
-    a["rcut"] = p.rcut
-    nl = R.VerletList(pool)
-    nl.update(a, dom)
-    s = R.Solver(p, pool, True)
-    s.prepare(a, dom, nl)
-    s.stages(a, dom, nl)
-    t0 = time.perf_counter()
-    s.run_nve(a, dom, nl, w["dt"], 2, False)
-    per = (time.perf_counter() - t0) / 2
-    steps = max(2, int(max_seconds / max(per, 1e-6)))
-    if max_steps:
-        steps = min(steps, max_steps)
-    t0 = time.perf_counter()
-    s.run_nve(a, dom, nl, w["dt"], steps, False)
-    dt = time.perf_counter() - t0
-    return {"value": len(a) * steps / dt, "unit": "atom-steps/s", "cores": threads, "kind": "port",
-            "sample": "%s, %d atoms, %d FP64 NVE steps with decide()/update() rebuilds, %d threads, %.1f s" % (
-                w["name"], len(a), steps, threads, dt)}
+
+    def timed(nthreads, seconds, cap):
+        pool = R.Pool(nthreads)
+        a = w["atoms"].copy()
+        a["rcut"] = p.rcut
+        nl = R.VerletList(pool)
+        nl.update(a, dom)
+        s = R.Solver(p, pool, True)
+        s.prepare(a, dom, nl)
+        s.stages(a, dom, nl)
+        t0 = time.perf_counter()
+        s.run_nve(a, dom, nl, w["dt"], 2, False)
+        per = (time.perf_counter() - t0) / 2
+        steps = max(2, int(seconds / max(per, 1e-6)))
+        if cap:
+            steps = min(steps, cap)
+        t0 = time.perf_counter()
+        s.run_nve(a, dom, nl, w["dt"], steps, False)
+        dt = time.perf_counter() - t0
+        return len(a) * steps / dt, steps, dt
+
+    v, steps, dt = timed(threads, max_seconds, max_steps)
+    v1, steps1, dt1 = timed(1, 8.0, max_steps)
+    out = {"value": v, "unit": "atom-steps/s", "cores": threads, "kind": "port",
+           "sample": "%s, %d atoms, %d FP64 NVE steps with decide()/update() rebuilds, %d threads, %.1f s" % (
+               w["name"], len(w["atoms"]), steps, threads, dt),
+           "single_core": {"value": v1, "sample": "the same, %d steps on 1 thread, %.1f s" % (steps1, dt1)}}
+    if check is not None:
+        pool = R.Pool(threads)
+        ref = w["atoms"].copy()
+        ref["rcut"] = p.rcut
+        nl = R.VerletList(pool)
+        nl.update(ref, dom)
+        R.Solver(p, pool, True).compute(ref, dom, nl)
+        out["accuracy"] = accuracy(check, ref, nl.export(len(ref)))
+    return out
 
 
 def main():
@@ -446,7 +514,7 @@ def main():
                                  "frac": round(tf / MFMA_FP16_PEAK_TFLOPS, 5), "flop_per_atom": MLP_FLOP_PER_ATOM}
             extra[name] = r
         extra["served_stream"] = served_stream(E, ctx, W)
-    cpu = None if args.no_cpu else cpu_baseline(w, max_steps=500)
+    cpu = None if args.no_cpu else cpu_baseline(w, max_steps=500, check=engine_check(E, ctx, w))
     line = {
         "metric": METRIC, "value": round(head["value"], 1), "unit": "atom-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(head["ms_per_step"], 5),
