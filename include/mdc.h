@@ -133,9 +133,11 @@ mdc_status mdc_sys_set_active(mdc_sys* sys, int box, int active);
 mdc_status mdc_sys_natoms(const mdc_sys* sys, uint64_t* n);
 /* lanes per atom (1, 2, 4, 8, 16) of the force kernels; changes the list layout */
 mdc_status mdc_sys_set_lanes(mdc_sys* sys, int lanes_per_atom);
-/* Times one force evaluation (MDC_FORCE at the given precision, reps
- * repetitions after a warm-up) with 1, 2, 4, 8 and 16 lanes per atom,
- * keeps the fastest and leaves a fresh list and forces for it.
+/* Times 1, 2, 4, 8 and 16 lanes per atom and keeps the fastest, leaving a
+ * fresh list and forces for it: for LJ and EAM the kernels of a step
+ * (density and force in step mode, max(reps, 20) launches back to back),
+ * for BPNN one force evaluation (MDC_FORCE, reps repetitions after a
+ * warm-up), at the given precision.
  * us_per_eval (may be NULL) receives the 5 mean times in microseconds.
  * No reference counterpart: the reference has no device launch shape. */
 mdc_status mdc_sys_autotune(mdc_sys* sys, const mdc_pot* pot, int precision, int reps, int* best_lanes,

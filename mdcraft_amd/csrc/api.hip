@@ -1337,14 +1337,23 @@ mdc_status mdc_sys_autotune(mdc_sys* sys, const mdc_pot* pot, int precision, int
                 const int l = cand[c];
                 set_lanes(b, l);
                 nlist_rebuild(b);
-                force_compute(b, pot->p, MDC_FORCE, precision);  // warm-up
-                HIPCHECK(hipEventRecord(e0, st));
-                for (int r = 0; r < reps; ++r) force_compute(b, pot->p, MDC_FORCE, precision);
-                HIPCHECK(hipEventRecord(e1, st));
-                HIPCHECK(hipEventSynchronize(e1));
-                float ms = 0.f;
-                HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
-                const double us = 1e3 * ms / reps;
+                force_compute(b, pot->p, MDC_FORCE, precision);  // warm-up, and the force to start from
+                double us = 0.0;
+                if (pot->p.type == PT_BPNN) {
+                    HIPCHECK(hipEventRecord(e0, st));
+                    for (int r = 0; r < reps; ++r) force_compute(b, pot->p, MDC_FORCE, precision);
+                    HIPCHECK(hipEventRecord(e1, st));
+                    HIPCHECK(hipEventSynchronize(e1));
+                    float ms = 0.f;
+                    HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+                    us = 1e3 * ms / reps;
+                } else {
+                    // pair potentials: the kernels a step runs (density and
+                    // force in step mode), back to back, without host syncs
+                    double t[2];
+                    time_step_kernels(b, pot->p, precision, 1e-6, std::max(reps, 20), t);
+                    us = t[0] + t[1];
+                }
                 if (us_per_eval) us_per_eval[c] = us;
                 if (us < best) {
                     best = us;
