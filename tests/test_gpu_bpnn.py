@@ -73,3 +73,25 @@ def test_bpnn_fp64_short_nve(ctx, small):
     d -= np.round(d / L) * L
     assert np.max(np.abs(d)) < 1e-11
     assert max_rel(vec(got, "f"), vec(ref, "f")) < 1e-10
+
+
+def test_bpnn_fp64_nve_through_rebuilds(ctx, small):
+    """24 FP64 steps, the list rebuilt on the half-skin trigger inside
+    step() (with the reverse-slot index built again), against the oracle's
+    run with decide()/update()."""
+    from mdcraft_amd import engine as E
+    ref, _, _, (s, nl, dom, p) = oracle(small)
+    s.run_nve(ref, dom, nl, small["dt"], 24, initial_forces=False)
+    es = E.System(ctx, [len(small["atoms"])], [small["box"]], lanes=16)
+    es.upload(small["atoms"])
+    pot = E.MaiseBPNNPotential(ctx, small["pot"]["path"])
+    es.build()
+    es.compute(pot, 7, 0)
+    rebuilds = es.step(pot, small["dt"], 24, 0)
+    got = es.download()
+    assert rebuilds >= 1
+    L = small["box"][1] - small["box"][0]
+    d = vec(got, "r") - vec(ref, "r")
+    d -= np.round(d / L) * L
+    assert np.max(np.abs(d)) < 1e-9
+    assert max_rel(vec(got, "v"), vec(ref, "v")) < 1e-8
