@@ -735,26 +735,25 @@ __global__ void k_bp_rev(const int* __restrict__ nl, const int* __restrict__ nn,
 }
 
 // pass 4: gather the force shares, then the fused velocity-Verlet update
-template <typename R>
+template <typename R, int TPA>
 __global__ void __launch_bounds__(256) k_bp_gather(const double4* __restrict__ pos, double4* __restrict__ pos_next,
                                                    const int* __restrict__ nl, const int* __restrict__ nn,
-                                                   const int* __restrict__ rev, int tpa, int M, int cap,
+                                                   const int* __restrict__ rev, int M, int cap,
                                                    const SysDesc* __restrict__ desc, const R* __restrict__ Pin,
                                                    const R* __restrict__ own, AtomArrays A, StepArgs SA,
                                                    const double4* __restrict__ rsave, int* __restrict__ misc) {
     if (skip_step(SA, misc)) return;
-    const int m = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (m >= A.n) return;
-    const FinIn fin = load_fin(m, A, SA, rsave, true);
-    const double4 pm = pos[m];
-    if (w_inactive(pm.w)) {
-        if (SA.mode == MODE_FINAL_DRIFT) pos_next[m] = pm;
-        return;
-    }
-    R fx = own[(size_t)m * 3 + 0], fy = own[(size_t)m * 3 + 1], fz = own[(size_t)m * 3 + 2];
-    const int cm = min(nn[m], cap);
-    for (int k = 0; k < cm; ++k) {
-        const long long li = list_index(m, k, tpa, M);
+    // TPA lanes per atom walk its list slots in the wavefront-packed layout
+    // (coalesced list and reverse-index reads), then sum their shares
+    const AtomSlots as = atom_slots<TPA>(M);
+    const int m = as.i;
+    const bool valid = m < A.n;
+    const double4 pm = valid ? pos[m] : make_double4(0, 0, 0, 0);
+    const bool active = valid && !w_inactive(pm.w);
+    const int cm = active ? min(nn[m], cap) : 0;
+    R fx = 0, fy = 0, fz = 0;
+    for (int c = 0; c * TPA + as.sub < cm; ++c) {
+        const long long li = as.base + (long long)c * kWave;
         const int i = ent_j(nl[li]);
         const int r = rev[li];
         if (r < 0) continue;
@@ -763,6 +762,18 @@ __global__ void __launch_bounds__(256) k_bp_gather(const double4* __restrict__ p
         fy += q[1];
         fz += q[2];
     }
+    fx = wave_sum_group(fx, TPA);
+    fy = wave_sum_group(fy, TPA);
+    fz = wave_sum_group(fz, TPA);
+    if (!valid || as.sub != 0) return;
+    if (!active) {
+        if (SA.mode == MODE_FINAL_DRIFT) pos_next[m] = pm;
+        return;
+    }
+    fx += own[(size_t)m * 3 + 0];
+    fy += own[(size_t)m * 3 + 1];
+    fz += own[(size_t)m * 3 + 2];
+    const FinIn fin = load_fin(m, A, SA, rsave, true);
     const SysDesc& s = desc[w_sys(pm.w)];
     finalize_atom<R>(m, pm, (double)fx, (double)fy, (double)fz, A, SA, pos_next, fin, s, misc);
 }
@@ -1123,8 +1134,20 @@ void bpnn_compute(Batch& b, const Pot& p, int what, int prec, int mode, double d
         if (!(parts & 2)) return;
         launch_k(b, K_BP_FORCE, kgrad, gw, bw, (unsigned)lds, pos, b.nl, b.nn, b.tpa, b.M, b.cap, b.d_desc,
                  bp.d, (const R*)w.w, ldg, (R*)w.P, (R*)w.own, n, S, b.misc);
-        launch_k(b, K_BP_GATHER, k_bp_gather<R>, dim3((n + 255) / 256), dim3(256), 0, pos, pos_next, b.nl, b.nn,
-                 w.rev, b.tpa, b.M, b.cap, b.d_desc, (const R*)w.P, (const R*)w.own, AA, S, b.rsave, b.misc);
+        auto gather = [&](auto tpa_tag) {
+            constexpr int T = decltype(tpa_tag)::value;
+            const long long waves = (n + kWave / T - 1) / (kWave / T);
+            launch_k(b, K_BP_GATHER, k_bp_gather<R, T>, dim3((unsigned)((waves * kWave + 255) / 256)), dim3(256), 0,
+                     pos, pos_next, b.nl, b.nn, w.rev, b.M, b.cap, b.d_desc, (const R*)w.P, (const R*)w.own, AA, S,
+                     b.rsave, b.misc);
+        };
+        switch (b.tpa) {
+            case 1: gather(std::integral_constant<int, 1>{}); break;
+            case 2: gather(std::integral_constant<int, 2>{}); break;
+            case 4: gather(std::integral_constant<int, 4>{}); break;
+            case 8: gather(std::integral_constant<int, 8>{}); break;
+            default: gather(std::integral_constant<int, 16>{}); break;
+        }
     };
     if (prec == P_FP64) run(double(0));
     else run(float(0));
