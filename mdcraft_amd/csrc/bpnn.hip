@@ -135,8 +135,11 @@ struct Stage {
     int cap;
 };
 
+// grad: the force pass also keeps the force-share accumulators and each
+// staged neighbour's list slot; the descriptor pass stages without them
+// (less LDS per wavefront, more wavefronts per CU).
 template <typename R>
-__device__ inline Stage<R> carve(char* base, int cap, int ne) {
+__device__ inline Stage<R> carve(char* base, int cap, int ne, bool grad) {
     Stage<R> S;
     R* p = reinterpret_cast<R*>(base);
     S.x = p;
@@ -147,14 +150,14 @@ __device__ inline Stage<R> carve(char* base, int cap, int ne) {
     S.dfc = p + 5 * cap;
     S.id = p + 6 * cap;
     S.E = p + 7 * cap;
-    S.g = p + (7 + ne) * cap;
-    S.slot = reinterpret_cast<int*>(p + (10 + ne) * cap);
+    S.g = grad ? p + (7 + ne) * cap : nullptr;
+    S.slot = grad ? reinterpret_cast<int*>(p + (10 + ne) * cap) : nullptr;
     S.cap = cap;
     return S;
 }
 
-__host__ __device__ inline size_t stage_bytes(int cap, int ne, int rsize) {
-    return (size_t)cap * ((10 + ne) * rsize + 4);
+__host__ __device__ inline size_t stage_bytes(int cap, int ne, int rsize, bool grad) {
+    return grad ? (size_t)cap * ((10 + ne) * rsize + 4) : (size_t)cap * (7 + ne) * rsize;
 }
 
 // First unordered pair (a, b = a + 1 + t) of the flattened pair index t
@@ -218,7 +221,7 @@ __device__ int bp_stage(int i, const double4& pi, const SysDesc& s, const int* _
             S.fc[at] = (R)0.5 * (dcos_(prc * rd) + (R)1);
             S.dfc[at] = (R)-0.5 * dsin_(prc * rd) * prc;
             for (int e = 0; e < P.ne; ++e) S.E[e * S.cap + at] = dexp_(-(R)P.eta_e[e] * (R)d2);
-            S.slot[at] = k;
+            if (S.slot) S.slot[at] = k;
         }
         base += __popcll(m);
     }
@@ -243,7 +246,7 @@ __global__ void __launch_bounds__(256) k_bp_desc(const double4* __restrict__ pos
     const int i = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + wv;
     if (i >= n) return;
     const BpDesc& P = *Pd;
-    const Stage<R> S = carve<R>(smem + wv * stage_bytes(cap, P.ne, sizeof(R)), cap, P.ne);
+    const Stage<R> S = carve<R>(smem + wv * stage_bytes(cap, P.ne, sizeof(R), false), cap, P.ne, false);
     const double4 pi = pos[i];
     const SysDesc& s = desc[w_sys(pi.w)];
     const int cnt = min(nn[i], cap);
@@ -492,7 +495,7 @@ __global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos
     const int i = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + wv;
     if (i >= n) return;
     const BpDesc& P = *Pd;
-    const Stage<R> S = carve<R>(smem + wv * stage_bytes(cap, P.ne, sizeof(R)), cap, P.ne);
+    const Stage<R> S = carve<R>(smem + wv * stage_bytes(cap, P.ne, sizeof(R), true), cap, P.ne, true);
     const double4 pi = pos[i];
     const SysDesc& s = desc[w_sys(pi.w)];
     const int cnt = min(nn[i], cap);
@@ -1113,13 +1116,14 @@ void bpnn_compute(Batch& b, const Pot& p, int what, int prec, int mode, double d
     const int4 kts = make_int4(bp.kt[0], bp.kt[1], bp.kt[2], bp.kt[3]);
     auto run = [&](auto rtag) {
         using R = decltype(rtag);
-        const size_t lds = waves * stage_bytes(b.cap, bp.h.ne, sizeof(R));
+        const size_t lds_desc = waves * stage_bytes(b.cap, bp.h.ne, sizeof(R), false);
+        const size_t lds = waves * stage_bytes(b.cap, bp.h.ne, sizeof(R), true);
         if (lds > 160 * 1024) throw Error(E_INVAL, "BPNN neighbour staging exceeds the LDS");
         const bool small = bp.h.nr <= 8 && bp.h.ng <= 4 && bp.h.ne <= 2;
         auto kdesc = small ? k_bp_desc<R, 8, 4, 2> : k_bp_desc<R, kBpMaxR, kBpMaxG, kBpMaxEta>;
         auto kgrad = small ? k_bp_grad<R, 8, 4, 2> : k_bp_grad<R, kBpMaxR, kBpMaxG, kBpMaxEta>;
         if (parts & 1) {
-            launch_k(b, K_BP_DESC, kdesc, gw, bw, (unsigned)lds, pos, b.nl, b.nn, b.tpa, b.M, b.cap, b.d_desc,
+            launch_k(b, K_BP_DESC, kdesc, gw, bw, (unsigned)lds_desc, pos, b.nl, b.nn, b.tpa, b.M, b.cap, b.d_desc,
                      bp.d, (R*)w.G, ldg, n, S, b.misc);
             if (sizeof(R) == 8) {
                 const bool narrow = std::max(bp.h.nin, std::max(bp.h.h1, bp.h.h2)) <= 32;
