@@ -142,22 +142,31 @@ template <typename R>
 __device__ inline Stage<R> carve(char* base, int cap, int ne, bool grad) {
     Stage<R> S;
     R* p = reinterpret_cast<R*>(base);
-    S.x = p;
-    S.y = p + cap;
-    S.z = p + 2 * cap;
-    S.d = p + 3 * cap;
-    S.fc = p + 4 * cap;
-    S.dfc = p + 5 * cap;
-    S.id = p + 6 * cap;
-    S.E = p + 7 * cap;
-    S.g = grad ? p + (7 + ne) * cap : nullptr;
-    S.slot = grad ? reinterpret_cast<int*>(p + (10 + ne) * cap) : nullptr;
+    int k = 0;
+    auto take = [&](int rows) {
+        R* q = p + (size_t)k * cap;
+        k += rows;
+        return q;
+    };
+    S.x = take(1);
+    S.y = take(1);
+    S.z = take(1);
+    S.d = take(1);
+    S.fc = take(1);
+    S.dfc = take(1);
+    S.id = take(1);
+    S.E = take(ne);
+    S.g = grad ? take(3) : nullptr;
+    S.slot = grad ? reinterpret_cast<int*>(p + (size_t)k * cap) : nullptr;
     S.cap = cap;
     return S;
 }
 
 __host__ __device__ inline size_t stage_bytes(int cap, int ne, int rsize, bool grad) {
-    return grad ? (size_t)cap * ((10 + ne) * rsize + 4) : (size_t)cap * (7 + ne) * rsize;
+    const int rows = 7 + ne + (grad ? 3 : 0);
+    // a multiple of 8 bytes, so that every wavefront's stage starts aligned
+    // for FP64 rows whatever the capacity
+    return ((size_t)cap * (rows * rsize + (grad ? 4 : 0)) + 7) & ~(size_t)7;
 }
 
 // First unordered pair (a, b = a + 1 + t) of the flattened pair index t
