@@ -212,8 +212,10 @@ mdc_status mdc_stage(mdc_sys* sys, const mdc_pot* pot, int stage, int precision)
  * No reference counterpart: a measurement hook. */
 mdc_status mdc_sys_time_kernels(mdc_sys* sys, const mdc_pot* pot, int precision, double dt, int reps,
                                 double* us_out);
-/* nsteps of velocity Verlet (make_step), list rebuilt on the half-skin
- * trigger; *rebuilds receives the number of rebuilds (may be NULL) */
+/* nsteps of velocity Verlet (make_step), list rebuilt on the device on the
+ * half-skin trigger (a list that overflows there is grown and built again
+ * before any step uses it); *rebuilds receives the number of rebuilds (may
+ * be NULL) */
 mdc_status mdc_step(mdc_sys* sys, const mdc_pot* pot, double dt, int nsteps, int precision, int* rebuilds);
 /* per system: Ep, Em, KE, Etot, W (sum of virial traces), com velocity[3] */
 mdc_status mdc_thermo(mdc_sys* sys, double* out);
