@@ -422,6 +422,32 @@ def cpu_baseline(w, max_seconds=20.0, max_steps=None, check=None):
     return out
 
 
+def eam_sizes(E, ctx, W, lanes=None, reps=(20, 40, 64, 101)):
+    """The EAM Cu step (mixed precision) at growing sizes of the same fcc
+    crystal: 32 000 atoms up to 4.1 million.  Per size: throughput over
+    timed steps with rebuilds, and the force kernel's back-to-back launch
+    time and HBM roofline fraction on the same algorithmic-bytes formula
+    as the headline line."""
+    out = {}
+    for r in reps:
+        w = W.eam_cu(reps=r)
+        steps = 200 if r <= 40 else 100
+        g = run_gpu(E, ctx, w, E.MIXED, steps, 10, lanes)
+        force_us = g["b2b_us"]["force"]
+        per_launch = algorithmic_bytes("eam_mixed", g["nbar_b2b"]) * g["n"]
+        gbs = per_launch / (force_us * 1e-6) / 1e9
+        out[str(g["n"])] = {
+            "value": round(g["value"], 1), "ms_per_step": round(g["ms_per_step"], 5), "steps": steps,
+            "rebuilds": g["rebuilds"], "lanes": g["lanes"], "nbar": round(g["nbar"], 2),
+            "b2b_us": g["b2b_us"], "kernel_us": g["kernel_us"],
+            "force_roofline": {"achieved_GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+                               "algorithmic_bytes_per_launch": int(per_launch)},
+        }
+        print("eam_sizes: %d atoms done" % g["n"], file=sys.stderr, flush=True)
+        del w, g
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -435,7 +461,7 @@ def main():
     ap.add_argument("--summarise-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
     ap.add_argument("--summarise-stats", metavar="STATS_DIR")
     ap.add_argument("--kernel", default="k_eam_force<float")
-    ap.add_argument("--only", choices=("bpnn_cu", "nial_batch", "nial_alloy", "served"),
+    ap.add_argument("--only", choices=("bpnn_cu", "nial_batch", "nial_alloy", "served", "eam_sizes"),
                     help="run one of the extra workloads alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.summarise_pmc:
@@ -458,6 +484,8 @@ def main():
             wb = W.bpnn_cu()
             out = {p: run_batch(E, ctx, [(wb["atoms"], wb["box"])], wb["pot"], wb["dt"], pr, 200, 10, args.lanes)
                    for p, pr in (("mixed", E.MIXED), ("fp64", E.FP64))}
+        elif args.only == "eam_sizes":
+            out = eam_sizes(E, ctx, W, args.lanes)
         elif args.only in ("nial_batch", "nial_alloy"):
             nb = W.nial_batch(512) if args.only == "nial_batch" else W.nial_alloy_batch(512)
             out = {p: run_batch_eval(E, ctx, nb["cells"], nb["pot"], pr, 50, args.lanes)
