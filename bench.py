@@ -542,6 +542,8 @@ def main():
                                  "frac": round(tf / MFMA_FP16_PEAK_TFLOPS, 5), "flop_per_atom": MLP_FLOP_PER_ATOM}
             extra[name] = r
         extra["served_stream"] = served_stream(E, ctx, W)
+        # the headline step at 32 000 to 4.1 million atoms
+        extra["eam_cu_sizes_mixed"] = eam_sizes(E, ctx, W)
     cpu = None if args.no_cpu else cpu_baseline(w, max_steps=500, check=engine_check(E, ctx, w))
     line = {
         "metric": METRIC, "value": round(head["value"], 1), "unit": "atom-steps/s", "n_gpus": 1,
