@@ -1039,8 +1039,9 @@ mdc_status mdc_sys_create(mdc_ctx* ctx, int nsys, const uint64_t* natoms, const 
             b.pos[0] = dalloc<double4>(n);
             b.pos[1] = dalloc<double4>(n);
             for (int k = 0; k < 2; ++k) {
-                b.dst[k] = dalloc<double>((size_t)D_NFIELDS * n);
-                b.fst[k] = dalloc<float>((size_t)F_NFIELDS * n);
+                // at least one record each, as the clears below assume
+                b.dst[k] = dalloc<double>((size_t)D_NFIELDS * std::max(n, 1));
+                b.fst[k] = dalloc<float>((size_t)F_NFIELDS * std::max(n, 1));
                 b.orig[k] = dalloc<int>(n);
                 HIPCHECK(hipMemset(b.dst[k], 0, sizeof(double) * D_NFIELDS * std::max(n, 1)));
                 HIPCHECK(hipMemset(b.fst[k], 0, sizeof(float) * F_NFIELDS * std::max(n, 1)));
