@@ -12,7 +12,7 @@ import pytest
 
 from mdcraft_amd import workloads as W
 from mdcraft_amd.atoms import vec
-from tests.helpers import engine_potential, max_rel, oracle_compute, oracle_mod, oracle_potential
+from tests.helpers import engine_potential, force_rms_rel, max_rel, oracle_compute, oracle_mod, oracle_potential
 from tests.test_gpu_parity import compare
 
 pytestmark = pytest.mark.gpu
@@ -120,3 +120,41 @@ def test_batch_without_atoms(ctx, nsys):
     assert len(s.download()) == 0
     assert np.all(s.thermo() == 0.0)
     assert s.stats()["rebuilds"] >= 0
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_bpnn_ragged_batch(ctx, tmp_path, precision):
+    """BPNN over the same ragged shapes: FP64 forces and energies within
+    1e-12, mixed force RMS within 1e-5 and energy sums within 1e-5 of the
+    sum of |E_i| (the cut crystals' per-atom energies take both signs, so
+    their sum cancels: 21 of 481 for the 37-atom cut)."""
+    from tests.test_gpu_bpnn import oracle as bpnn_oracle
+    eng = E()
+    w = W.bpnn_cu(reps=4, model_dir=str(tmp_path))
+    cs = [(w["atoms"][:k].copy(), w["box"]) for k in SIZES]
+    print(f"BPNN ragged batch: precision {precision}, sizes {SIZES}", flush=True)
+    s = eng.System(ctx, SIZES, [w["box"]] * len(SIZES))
+    s.upload(np.concatenate([a for a, _ in cs]))
+    p = eng.MaiseBPNNPotential(ctx, w["pot"]["path"])
+    s.build()
+    s.compute(p, 7, precision)
+    got = s.download()
+    off = 0
+    for q, (at, box) in enumerate(cs):
+        n = len(at)
+        if n == 0:
+            continue
+        ref = bpnn_oracle({"atoms": at, "box": box, "pot": w["pot"]})[0]
+        f, fr = vec(got[off:off + n], "f"), vec(ref, "f")
+        if n == 1:
+            assert np.array_equal(f, np.zeros((1, 3))) and np.array_equal(fr, np.zeros((1, 3)))
+        elif precision == 0:
+            assert max_rel(f, fr) < 1e-12, (q, n, max_rel(f, fr))
+        else:
+            assert force_rms_rel(f, fr) < 1e-5, (q, n, force_rms_rel(f, fr))
+        e, er = got["Ep"][off:off + n], ref["Ep"]
+        if precision == 0:
+            assert max_rel(e, er) < 1e-12, (q, n)
+        else:
+            assert abs(e.sum() - er.sum()) <= 1e-5 * np.abs(er).sum(), (q, n, e.sum(), er.sum())
+        off += n
