@@ -740,6 +740,19 @@ __global__ void k_bp_rev(const int* __restrict__ nl, const int* __restrict__ nn,
     const int i = ent_j(nl[lk]);
     const int ci = min(nn[i], cap);
     int r = -1;
+    if (tpa >= 8) {
+        // eight adjacent slots (q0 a multiple of 8), two 16-byte loads
+        for (int q0 = 0; q0 < ci && r < 0; q0 += 8) {
+            const int4* p = reinterpret_cast<const int4*>(nl + list_index(i, q0, tpa, M));
+            const int4 v0 = p[0], v1 = p[1];
+            const int e[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+#pragma unroll
+            for (int u = 7; u >= 0; --u)
+                if (q0 + u < ci && ent_j(e[u]) == m) r = q0 + u;
+        }
+        rev[lk] = r;
+        return;
+    }
     if (tpa >= 4) {
         for (int q0 = 0; q0 < ci && r < 0; q0 += 4) {
             const int4 v = *reinterpret_cast<const int4*>(nl + list_index(i, q0, tpa, M));
