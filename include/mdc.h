@@ -116,7 +116,9 @@ mdc_status mdc_pot_destroy(mdc_pot* pot);
  * (lattice::Domain, lattice/domain.h:14-126).  periodic_boundary: 1 attaches
  * the reference's boundary::Periodic to the solver (solver/boundary/periodic.h),
  * 0 the null boundary.  Atoms of system s are records
- * [off_s, off_s + natoms[s]) of every upload/download buffer. */
+ * [off_s, off_s + natoms[s]) of every upload/download buffer.  natoms[s] may
+ * be 0 (an empty system: zero energies in mdc_thermo), also for every s (a
+ * batch with no atoms, on which build, compute and step do nothing). */
 mdc_status mdc_sys_create(mdc_ctx* ctx, int nsys, const uint64_t* natoms, const double* box_lo, const double* box_hi,
                           const uint8_t* periodic, int periodic_boundary, mdc_sys** out);
 mdc_status mdc_sys_destroy(mdc_sys* sys);
