@@ -122,6 +122,9 @@ mdc_status mdc_pot_destroy(mdc_pot* pot);
 mdc_status mdc_sys_create(mdc_ctx* ctx, int nsys, const uint64_t* natoms, const double* box_lo, const double* box_hi,
                           const uint8_t* periodic, int periodic_boundary, mdc_sys** out);
 mdc_status mdc_sys_destroy(mdc_sys* sys);
+/* Uploading positions (MDC_F_R) keeps the list; until its next rebuild,
+ * each pair's image is chosen from the positions at every evaluation (the
+ * image shifts recorded at the build may no longer hold). */
 mdc_status mdc_sys_upload(mdc_sys* sys, const void* atoms, uint32_t fields);
 mdc_status mdc_sys_download(mdc_sys* sys, void* atoms, uint32_t fields);
 /* Records [first, first + count) of the caller's order into atoms[0, count). */
@@ -174,7 +177,9 @@ mdc_status mdc_nlist_export(mdc_sys* sys, uint64_t* offsets, uint64_t* ids, uint
 mdc_status mdc_nlist_stats(mdc_sys* sys, double* mean_neighbours, int* max_neighbours, int* capacity, int* rebuilds);
 
 /* ---- force evaluation and integration ----------------------------------- */
-/* prepare + stages (+ virials when MDC_ENERGY | MDC_VIRIAL) */
+/* prepare + stages (+ virials when MDC_ENERGY | MDC_VIRIAL; for BPNN the
+ * virial is the reference's diagonal one, maise_bpnn.cxx:98-120, with the
+ * off-diagonal entries zero) */
 mdc_status mdc_compute(mdc_sys* sys, const mdc_pot* pot, uint32_t what, int precision);
 /* Langevin thermostats (solver/thermostat/langevin.cxx:7-42), applied one
  * after the other as thermostat::Compound does, to every force evaluation
@@ -203,7 +208,9 @@ mdc_status mdc_sys_set_thermostat(mdc_sys* sys, int count, const double* params,
  *   MDC_STAGE_FORCES   add the force to f; EAM takes F'(nc) of the nc the
  *                      atoms hold, whether or not a charge call preceded
  *   MDC_STAGE_VIRIALS  add the pair energy and virial to Ep and V, set Em =
- *                      F(nc) for EAM; f is left as it is (BPNN: V stays 0)
+ *                      F(nc) for EAM; f is left as it is.  BPNN: adds the
+ *                      reference's diagonal virial (maise_bpnn.cxx:98-120)
+ *                      from the dE/dG of the last prepare
  * The list must be built. */
 enum { MDC_STAGE_PREPARE = 0, MDC_STAGE_CHARGE = 1, MDC_STAGE_FORCES = 2, MDC_STAGE_VIRIALS = 3 };
 mdc_status mdc_stage(mdc_sys* sys, const mdc_pot* pot, int stage, int precision);

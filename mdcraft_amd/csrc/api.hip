@@ -612,6 +612,7 @@ void upload(Batch& b, const AtomRec* a, uint32_t fields) {
     dfree(d_spos);
     if (fields & MDC_F_F) b.f_prec = P_FP64;
     if (fields & MDC_F_RCUT) b.last_rcut = -1;
+    if (fields & MDC_F_R) b.shifts_stale = true;
     if (rns_changed) {
         layout_cells(b);
         size_list(b);
@@ -1001,9 +1002,13 @@ mdc_status mdc_sys_create(mdc_ctx* ctx, int nsys, const uint64_t* natoms, const 
         b.ctx = &ctx->c;
         b.nsys = nsys;
         b.atom_off.assign(nsys + 1, 0);
+        uint64_t total = 0;
         for (int i = 0; i < nsys; ++i) {
-            b.atom_off[i + 1] = b.atom_off[i] + (int)natoms[i];
-            if ((uint64_t)b.atom_off[i + 1] > (1ull << 30)) throw Error(E_INVAL, "too many atoms");
+            // checked as 64-bit counts before any narrowing
+            if (natoms[i] >= (uint64_t)kMaxAtomsShift) throw Error(E_INVAL, "at most 2^26 - 1 atoms per batch");
+            total += natoms[i];
+            if (total >= (uint64_t)kMaxAtomsShift) throw Error(E_INVAL, "at most 2^26 - 1 atoms per batch");
+            b.atom_off[i + 1] = (int)total;
         }
         b.n = b.atom_off[nsys];
         if (b.n >= kMaxAtomsShift) throw Error(E_INVAL, "at most 2^26 - 1 atoms per batch");

@@ -73,15 +73,29 @@ struct BpnnParams {
     int nin_pad = 0, h1_pad = 0, h2_pad = 0;
 };
 
-// per-batch BPNN work arrays (in Batch::bp_scratch)
-struct BpWork {
-    size_t bytes = 0;
-    int cap = 0, n = 0, prec = -1;
+// Work arrays of one BPNN evaluation: descriptors G and dE/dG w [n][ldg],
+// force shares P [n][cap][3] and the centres' own shares own [n][3], the
+// virial shares PV [n][cap][3] and OV [n][3] (only when the virial is
+// asked for), and the reverse-slot index rev (one int per list element).
+struct BpBufs {
     void* G = nullptr;
     void* w = nullptr;
     void* P = nullptr;
     void* own = nullptr;
+    void* PV = nullptr;
+    void* OV = nullptr;
     int* rev = nullptr;
+};
+
+// per-batch BPNN work arrays (in Batch::bp_scratch), sized for the list
+// they were made for: rebuilt when the capacity, the atom count, the
+// precision or the list's element count (which grows when the lanes per
+// atom drop) no longer match
+struct BpWork {
+    int cap = 0, n = 0, prec = -1;
+    size_t rev_elems = 0;
+    bool has_vir = false;
+    BpBufs a;
     bool rev_valid = false;
     int rev_builds = -1;
 };
@@ -190,11 +204,12 @@ struct PairWalk {
 
 // Stage the in-range neighbours of centre i (1e-15 <= r^2 < Rc^2 under
 // Domain::shortest) into LDS, compacted in list order; out-of-range slots
-// get zero force shares in P when P is given.
+// get zero force shares in Pzero and zero virial shares in PVzero when
+// those are given.
 template <typename R>
 __device__ int bp_stage(int i, const double4& pi, const SysDesc& s, const int* __restrict__ nl, int cnt, int tpa,
                         int M, const double4* __restrict__ pos, const BpDesc& P, const Stage<R>& S,
-                        R* __restrict__ Pzero, int cap) {
+                        R* __restrict__ Pzero, R* __restrict__ PVzero, int cap) {
     const int lane = threadIdx.x & 63;
     int base = 0;
     for (int k0 = 0; k0 < cnt; k0 += kWave) {
@@ -212,6 +227,10 @@ __device__ int bp_stage(int i, const double4& pi, const SysDesc& s, const int* _
             ok = d2 >= 1e-15 && d2 < P.Rc2;
             if (!ok && Pzero) {
                 R* q = Pzero + ((size_t)i * cap + k) * 3;
+                q[0] = q[1] = q[2] = (R)0;
+            }
+            if (!ok && PVzero) {
+                R* q = PVzero + ((size_t)i * cap + k) * 3;
                 q[0] = q[1] = q[2] = (R)0;
             }
         }
@@ -259,7 +278,7 @@ __global__ void __launch_bounds__(256) k_bp_desc(const double4* __restrict__ pos
     const double4 pi = pos[i];
     const SysDesc& s = desc[w_sys(pi.w)];
     const int cnt = min(nn[i], cap);
-    const int nc = bp_stage<R>(i, pi, s, nl, cnt, tpa, M, pos, P, S, (R*)nullptr, cap);
+    const int nc = bp_stage<R>(i, pi, s, nl, cnt, tpa, M, pos, P, S, (R*)nullptr, (R*)nullptr, cap);
     R acc[NR + NG * NE];
 #pragma unroll
     for (int q = 0; q < NR + NG * NE; ++q) acc[q] = (R)0;
@@ -491,13 +510,20 @@ __global__ void __launch_bounds__(64) k_bp_mlp_mfma(const float* __restrict__ G,
     }
 }
 
-// pass 3: force shares of centre i on its neighbour slots and on itself
-template <typename R, int NR, int NG, int NE>
+// pass 3: force shares of centre i on its neighbour slots and on itself.
+// VIR: also the diagonal virial of maise_bpnn.cxx:98-120.  Its radial term
+// on atom m sums, over the centres i listing m, d(w_i . G2_i)/dr r_im(c)^2
+// / r_im with the centre's dE/dG: centre i writes that share for each slot
+// (PV, gathered like the force shares).  Its angular term on atom i is the
+// centre's own dE/dG contracted with the pattern_dG4_s_ stress of
+// tools/bpnn/base.h:157-181 over the pairs of i's neighbours (OV).
+template <typename R, int NR, int NG, int NE, bool VIR>
 __global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos, const int* __restrict__ nl,
                                                  const int* __restrict__ nn, int tpa, int M, int cap,
                                                  const SysDesc* __restrict__ desc, const BpDesc* __restrict__ Pd,
                                                  const R* __restrict__ wgt, int ldg, R* __restrict__ Pout,
-                                                 R* __restrict__ own, int n, StepArgs SA, int* __restrict__ misc) {
+                                                 R* __restrict__ own, R* __restrict__ PV, R* __restrict__ OV, int n,
+                                                 StepArgs SA, int* __restrict__ misc) {
     if (skip_step(SA, misc)) return;
     extern __shared__ char smem[];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -508,7 +534,7 @@ __global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos
     const double4 pi = pos[i];
     const SysDesc& s = desc[w_sys(pi.w)];
     const int cnt = min(nn[i], cap);
-    const int nc = bp_stage<R>(i, pi, s, nl, cnt, tpa, M, pos, P, S, Pout, cap);
+    const int nc = bp_stage<R>(i, pi, s, nl, cnt, tpa, M, pos, P, S, Pout, VIR ? PV : (R*)nullptr, cap);
     const int nr = P.nr, ng = P.ng, ne = P.ne;
     R eta_r[NR], g_lam[NG], g_zl[NG], eta_e[NE];
     int g_iz[NG];
@@ -552,8 +578,29 @@ __global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos
         S.g[a] = sr * ax * ia;
         S.g[cap + a] = sr * ay * ia;
         S.g[2 * cap + a] = sr * az * ia;
+        if constexpr (VIR) {
+            R* q = PV + ((size_t)i * cap + S.slot[a]) * 3;
+            q[0] = sr * ax * ax * ia;
+            q[1] = sr * ay * ay * ia;
+            q[2] = sr * az * az * ia;
+        }
     }
     __builtin_amdgcn_wave_barrier();
+    // angular virial of the centre, per lane: for a pair (a, b) with
+    // c = r_b - r_a, per axis (alpha + beta_a) x_a^2 + (alpha + beta_b) x_b^2
+    // + (alpha + gam) x_c^2 + del x_a x_b (pattern_dG4_s_ contracted)
+    R vx = 0, vy = 0, vz = 0;
+    auto vir_pair = [&](R h, R ax_, R ay_, R az_, R bx_, R by_, R bz_, R cx_, R cy_, R cz_, R ia_, R ib_, R ic_,
+                        R fa_, R fb_, R dfa_, R dfb_, R fcc_, R dfcc_, R cs_, R F_, R SA_, R SAe_, R SZ_) {
+        const R al = (R)-2 * F_ * SAe_;
+        const R ba = al - SZ_ * cs_ * ia_ * ia_ + SA_ * fb_ * fcc_ * dfa_ * ia_;
+        const R bb = al - SZ_ * cs_ * ib_ * ib_ + SA_ * fa_ * fcc_ * dfb_ * ib_;
+        const R gc = al + SA_ * fa_ * fb_ * dfcc_ * ic_;
+        const R de = (R)2 * SZ_ * ia_ * ib_;
+        vx += h * (ba * ax_ * ax_ + bb * bx_ * bx_ + gc * cx_ * cx_ + de * ax_ * bx_);
+        vy += h * (ba * ay_ * ay_ + bb * by_ * by_ + gc * cy_ * cy_ + de * ay_ * by_);
+        vz += h * (ba * az_ * az_ + bb * bz_ * bz_ + gc * cz_ * cz_ + de * az_ * bz_);
+    };
     if constexpr (sizeof(R) == 4) {
     // angular part, FP32: ordered pairs (a, b), a != b: with A_q = c_q pz_q,
     // Z_q = c_q zeta lam pz1_q (c_q = w_q coef_q Ex_e(q)) summed over q,
@@ -633,6 +680,10 @@ __global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos
                 hx += ka * ax + kb * bx + cc * cx;
                 hy += ka * ay + kb * by + cc * cy;
                 hz += ka * az + kb * bz + cc * cz;
+                // each unordered pair is visited twice here
+                if constexpr (VIR)
+                    vir_pair((R)0.5, ax, ay, az, bx, by, bz, cx, cy, cz, ia, ib, ic, fa, fb, dfa, S.dfc[b], fcc, dfcc,
+                             cs, F, SA, SAe, SZ);
             }
             if (++r == row) {
                 r = 0;
@@ -691,6 +742,9 @@ __global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos
         const R ka = (-SZ * cs * ia - (R)2 * F * da * SAe + fb * dfa * fcc * SA) * ia;
         const R kb = SZ * ia * ib;
         const R la = (-SZ * cs * ib - (R)2 * F * db * SAe + fa * dfb * fcc * SA) * ib;
+        if constexpr (VIR)
+            vir_pair((R)1, ax, ay, az, bx, by, bz, cx, cy, cz, ia, ib, ic, fa, fb, dfa, dfb, fcc, dfcc, cs, F, SA, SAe,
+                     SZ);
         atomicAdd(&S.g[a], ka * ax + kb * bx + cc * cx);
         atomicAdd(&S.g[cap + a], ka * ay + kb * by + cc * cy);
         atomicAdd(&S.g[2 * cap + a], ka * az + kb * bz + cc * cz);
@@ -718,6 +772,16 @@ __global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos
         own[(size_t)i * 3 + 0] = ox;
         own[(size_t)i * 3 + 1] = oy;
         own[(size_t)i * 3 + 2] = oz;
+    }
+    if constexpr (VIR) {
+        vx = wave_sum(vx);
+        vy = wave_sum(vy);
+        vz = wave_sum(vz);
+        if (lane == 0) {
+            OV[(size_t)i * 3 + 0] = vx;
+            OV[(size_t)i * 3 + 1] = vy;
+            OV[(size_t)i * 3 + 2] = vz;
+        }
     }
 }
 
@@ -775,13 +839,18 @@ __global__ void k_bp_rev(const int* __restrict__ nl, const int* __restrict__ nn,
     rev[lk] = r;
 }
 
-// pass 4: gather the force shares, then the fused velocity-Verlet update
-template <typename R, int TPA>
+// pass 4: gather the force shares, then the fused velocity-Verlet update.
+// VIR: also the virial shares; vmode 1 writes V (its diagonal from the
+// shares, the rest zero, as the reference's prepare leaves it) with the
+// force, vmode 2 adds the diagonal to V and leaves the force (the virials
+// call of solver::Ternary, ternary.cxx:57-67).
+template <typename R, int TPA, bool VIR>
 __global__ void __launch_bounds__(256) k_bp_gather(const double4* __restrict__ pos, double4* __restrict__ pos_next,
                                                    const int* __restrict__ nl, const int* __restrict__ nn,
                                                    const int* __restrict__ rev, int M, int cap,
                                                    const SysDesc* __restrict__ desc, const R* __restrict__ Pin,
-                                                   const R* __restrict__ own, AtomArrays A, StepArgs SA,
+                                                   const R* __restrict__ own, const R* __restrict__ PV,
+                                                   const R* __restrict__ OV, int vmode, AtomArrays A, StepArgs SA,
                                                    const double4* __restrict__ rsave, int* __restrict__ misc) {
     if (skip_step(SA, misc)) return;
     // TPA lanes per atom walk its list slots in the wavefront-packed layout
@@ -792,24 +861,45 @@ __global__ void __launch_bounds__(256) k_bp_gather(const double4* __restrict__ p
     const double4 pm = valid ? pos[m] : make_double4(0, 0, 0, 0);
     const bool active = valid && !w_inactive(pm.w);
     const int cm = active ? min(nn[m], cap) : 0;
-    R fx = 0, fy = 0, fz = 0;
+    R fx = 0, fy = 0, fz = 0, vx = 0, vy = 0, vz = 0;
     for (int c = 0; c * TPA + as.sub < cm; ++c) {
         const long long li = as.base + (long long)c * kWave;
         const int i = ent_j(nl[li]);
         const int r = rev[li];
         if (r < 0) continue;
-        const R* q = Pin + ((size_t)i * cap + r) * 3;
-        fx += q[0];
-        fy += q[1];
-        fz += q[2];
+        const size_t o = ((size_t)i * cap + r) * 3;
+        fx += Pin[o];
+        fy += Pin[o + 1];
+        fz += Pin[o + 2];
+        if constexpr (VIR) {
+            vx += PV[o];
+            vy += PV[o + 1];
+            vz += PV[o + 2];
+        }
     }
     fx = wave_sum_group(fx, TPA);
     fy = wave_sum_group(fy, TPA);
     fz = wave_sum_group(fz, TPA);
+    if constexpr (VIR) {
+        vx = wave_sum_group(vx, TPA);
+        vy = wave_sum_group(vy, TPA);
+        vz = wave_sum_group(vz, TPA);
+    }
     if (!valid || as.sub != 0) return;
     if (!active) {
         if (SA.mode == MODE_FINAL_DRIFT) pos_next[m] = pm;
         return;
+    }
+    if constexpr (VIR) {
+        const int n = A.n;
+        const double V[3] = {(double)vx + (double)OV[(size_t)m * 3 + 0], (double)vy + (double)OV[(size_t)m * 3 + 1],
+                             (double)vz + (double)OV[(size_t)m * 3 + 2]};
+        if (vmode == 2) {
+            for (int c = 0; c < 3; ++c) A.d[(size_t)(D_VXX + c) * n + m] += V[c];
+            return;
+        }
+        for (int c = 0; c < 3; ++c) A.d[(size_t)(D_VXX + c) * n + m] = V[c];
+        for (int f = D_VXY; f <= D_VYZ; ++f) A.d[(size_t)f * n + m] = 0.0;
     }
     fx += own[(size_t)m * 3 + 0];
     fy += own[(size_t)m * 3 + 1];
@@ -923,11 +1013,14 @@ BpWork& work(Batch& b) {
 }
 
 void free_work(BpWork& w) {
-    for (void* p : {w.G, w.w, w.P, w.own, (void*)w.rev})
+    for (void* p : {w.a.G, w.a.w, w.a.P, w.a.own, w.a.PV, w.a.OV, (void*)w.a.rev})
         if (p) (void)hipFree(p);
-    w.G = w.w = w.P = w.own = nullptr;
-    w.rev = nullptr;
-    w.bytes = 0;
+    w.a = BpBufs{};
+    w.cap = w.n = 0;
+    w.prec = -1;
+    w.rev_elems = 0;
+    w.has_vir = false;
+    w.rev_valid = false;
 }
 
 }  // namespace
@@ -1107,42 +1200,55 @@ void bpnn_destroy(Pot* p) {
     delete p;
 }
 
-static void ensure_work(Batch& b, const BpnnParams& bp, int prec) {
+static void ensure_work(Batch& b, const BpnnParams& bp, int prec, bool vir) {
     BpWork& w = work(b);
     const size_t rs = prec == P_FP64 ? 8 : 4;
     const int ldg = bp.nin_pad;
-    if (w.cap == b.cap && w.n == b.n && w.prec == prec && w.G) return;
-    free_work(w);
-    w.G = dnew<char>((size_t)b.n * ldg * rs);
-    w.w = dnew<char>((size_t)b.n * ldg * rs);
-    w.P = dnew<char>((size_t)b.n * b.cap * 3 * rs);
-    w.own = dnew<char>((size_t)b.n * 3 * rs);
-    w.rev = dnew<int>(b.nl_elems);
-    w.cap = b.cap;
-    w.n = b.n;
-    w.prec = prec;
-    w.rev_valid = false;
+    const bool fits = w.a.G && w.cap == b.cap && w.n == b.n && w.prec == prec && w.rev_elems >= b.nl_elems;
+    if (!fits) {
+        free_work(w);
+        w.a.G = dnew<char>((size_t)b.n * ldg * rs);
+        w.a.w = dnew<char>((size_t)b.n * ldg * rs);
+        w.a.P = dnew<char>((size_t)b.n * b.cap * 3 * rs);
+        w.a.own = dnew<char>((size_t)b.n * 3 * rs);
+        w.a.rev = dnew<int>(b.nl_elems);
+        w.cap = b.cap;
+        w.n = b.n;
+        w.prec = prec;
+        w.rev_elems = b.nl_elems;
+        w.rev_valid = false;
+    }
+    if (vir && !w.has_vir) {
+        w.a.PV = dnew<char>((size_t)b.n * b.cap * 3 * rs);
+        w.a.OV = dnew<char>((size_t)b.n * 3 * rs);
+        w.has_vir = true;
+    }
 }
 
-void bpnn_compute(Batch& b, const Pot& p, int what, int prec, int mode, double dt, int step, const double4* pos,
-                  double4* pos_next, int parts) {
-    const BpnnParams& bp = *p.bp;
+static void check_bpnn_box(const Batch& b) {
     for (int s = 0; s < b.nsys; ++s)
         for (int a = 0; a < 3; ++a)
             if (b.desc[s].per[a] && b.desc[s].L[a] < 2.0 * std::sqrt(b.desc[s].rns2))
                 throw Error(E_INVAL, "BPNN needs periodic box lengths of at least twice the search radius");
     if (b.cap > 1024) throw Error(E_INVAL, "BPNN neighbour capacity above 1024");
-    ensure_work(b, bp, prec);
-    BpWork& w = work(b);
+}
+
+// parts: 1 = descriptors and network (G, dE/dG, Ep), 2 = forces from the
+// stored dE/dG, 4 (with 2) = the virial alone, added to V, force left as
+// it is.  rev_fresh: the reverse-slot index in bufs is already built for
+// the current list.
+static void bpnn_run(Batch& b, const Pot& p, const BpBufs& W, bool rev_fresh, int what, int prec, int mode,
+                     double dt, int step, const double4* pos, double4* pos_next, int parts) {
+    const BpnnParams& bp = *p.bp;
     hipStream_t st = b.ctx->stream;
     const int n = b.n;
+    const bool vir = (parts & 4) || (what & W_VIRIAL);
+    const int vmode = (parts & 4) ? 2 : (vir ? 1 : 0);
     StepArgs S{dt, step, mode, what & (W_ENERGY | W_VIRIAL), mode == 0 ? 1 : 0};
-    if (!w.rev_valid || w.rev_builds != b.rebuilds) {
+    if (!rev_fresh) {
         const long long pairs = (long long)n * b.cap;
-        hipLaunchKernelGGL(k_bp_rev, dim3((unsigned)((pairs + 255) / 256)), dim3(256), 0, st, b.nl, b.nn, w.rev, b.tpa,
+        hipLaunchKernelGGL(k_bp_rev, dim3((unsigned)((pairs + 255) / 256)), dim3(256), 0, st, b.nl, b.nn, W.rev, b.tpa,
                            b.M, b.cap, n);
-        w.rev_valid = true;
-        w.rev_builds = b.rebuilds;
     }
     const int waves = 4;
     const dim3 gw((n + waves - 1) / waves), bw(64 * waves);
@@ -1159,40 +1265,62 @@ void bpnn_compute(Batch& b, const Pot& p, int what, int prec, int mode, double d
         if (lds > 160 * 1024) throw Error(E_INVAL, "BPNN neighbour staging exceeds the LDS");
         const bool small = bp.h.nr <= 8 && bp.h.ng <= 4 && bp.h.ne <= 2;
         auto kdesc = small ? k_bp_desc<R, 8, 4, 2> : k_bp_desc<R, kBpMaxR, kBpMaxG, kBpMaxEta>;
-        auto kgrad = small ? k_bp_grad<R, 8, 4, 2> : k_bp_grad<R, kBpMaxR, kBpMaxG, kBpMaxEta>;
         if (parts & 1) {
             launch_k(b, K_BP_DESC, kdesc, gw, bw, (unsigned)lds_desc, pos, b.nl, b.nn, b.tpa, b.M, b.cap, b.d_desc,
-                     bp.d, (R*)w.G, ldg, n, S, b.misc);
+                     bp.d, (R*)W.G, ldg, n, S, b.misc);
             if (sizeof(R) == 8) {
                 const bool narrow = std::max(bp.h.nin, std::max(bp.h.h1, bp.h.h2)) <= 32;
                 const int L = narrow ? 32 : 64;
                 const dim3 grid((unsigned)(((long long)n * L + 255) / 256));
                 launch_k(b, K_BP_MLP, narrow ? k_bp_mlp64<32> : k_bp_mlp64<64>, grid, dim3(256), 0,
-                         (const double*)w.G, ldg, bp.w64, bp.d, pos, (double*)w.w, Ep, n, S, b.misc);
+                         (const double*)W.G, ldg, bp.w64, bp.d, pos, (double*)W.w, Ep, n, S, b.misc);
             } else
-                launch_k(b, K_BP_MLP, k_bp_mlp_mfma, dim3((n + 15) / 16), dim3(64), 0, (const float*)w.G, ldg,
-                         bp.bsplit, boff, nts, kts, bp.w32, bp.d, pos, (float*)w.w, Ep, n, S, b.misc);
+                launch_k(b, K_BP_MLP, k_bp_mlp_mfma, dim3((n + 15) / 16), dim3(64), 0, (const float*)W.G, ldg,
+                         bp.bsplit, boff, nts, kts, bp.w32, bp.d, pos, (float*)W.w, Ep, n, S, b.misc);
         }
         if (!(parts & 2)) return;
-        launch_k(b, K_BP_FORCE, kgrad, gw, bw, (unsigned)lds, pos, b.nl, b.nn, b.tpa, b.M, b.cap, b.d_desc,
-                 bp.d, (const R*)w.w, ldg, (R*)w.P, (R*)w.own, n, S, b.misc);
-        auto gather = [&](auto tpa_tag) {
-            constexpr int T = decltype(tpa_tag)::value;
-            const long long waves = (n + kWave / T - 1) / (kWave / T);
-            launch_k(b, K_BP_GATHER, k_bp_gather<R, T>, dim3((unsigned)((waves * kWave + 255) / 256)), dim3(256), 0,
-                     pos, pos_next, b.nl, b.nn, w.rev, b.M, b.cap, b.d_desc, (const R*)w.P, (const R*)w.own, AA, S,
-                     b.rsave, b.misc);
+        auto grad = [&](auto vtag) {
+            constexpr bool V = decltype(vtag)::value;
+            auto kgrad = small ? k_bp_grad<R, 8, 4, 2, V> : k_bp_grad<R, kBpMaxR, kBpMaxG, kBpMaxEta, V>;
+            launch_k(b, K_BP_FORCE, kgrad, gw, bw, (unsigned)lds, pos, b.nl, b.nn, b.tpa, b.M, b.cap, b.d_desc, bp.d,
+                     (const R*)W.w, ldg, (R*)W.P, (R*)W.own, (R*)W.PV, (R*)W.OV, n, S, b.misc);
         };
-        switch (b.tpa) {
-            case 1: gather(std::integral_constant<int, 1>{}); break;
-            case 2: gather(std::integral_constant<int, 2>{}); break;
-            case 4: gather(std::integral_constant<int, 4>{}); break;
-            case 8: gather(std::integral_constant<int, 8>{}); break;
-            default: gather(std::integral_constant<int, 16>{}); break;
-        }
+        if (vir) grad(std::true_type{});
+        else grad(std::false_type{});
+        auto gather = [&](auto tpa_tag, auto vtag) {
+            constexpr int T = decltype(tpa_tag)::value;
+            constexpr bool V = decltype(vtag)::value;
+            const long long waves = (n + kWave / T - 1) / (kWave / T);
+            launch_k(b, K_BP_GATHER, k_bp_gather<R, T, V>, dim3((unsigned)((waves * kWave + 255) / 256)), dim3(256), 0,
+                     pos, pos_next, b.nl, b.nn, W.rev, b.M, b.cap, b.d_desc, (const R*)W.P, (const R*)W.own,
+                     (const R*)W.PV, (const R*)W.OV, vmode, AA, S, b.rsave, b.misc);
+        };
+        auto by_lanes = [&](auto vtag) {
+            switch (b.tpa) {
+                case 1: gather(std::integral_constant<int, 1>{}, vtag); break;
+                case 2: gather(std::integral_constant<int, 2>{}, vtag); break;
+                case 4: gather(std::integral_constant<int, 4>{}, vtag); break;
+                case 8: gather(std::integral_constant<int, 8>{}, vtag); break;
+                default: gather(std::integral_constant<int, 16>{}, vtag); break;
+            }
+        };
+        if (vir) by_lanes(std::true_type{});
+        else by_lanes(std::false_type{});
     };
     if (prec == P_FP64) run(double(0));
     else run(float(0));
+}
+
+void bpnn_compute(Batch& b, const Pot& p, int what, int prec, int mode, double dt, int step, const double4* pos,
+                  double4* pos_next, int parts) {
+    check_bpnn_box(b);
+    const bool vir = (parts & 4) || (what & W_VIRIAL);
+    ensure_work(b, *p.bp, prec, vir);
+    BpWork& w = work(b);
+    const bool fresh = w.rev_valid && w.rev_builds == b.rebuilds;
+    bpnn_run(b, p, w.a, fresh, what, prec, mode, dt, step, pos, pos_next, parts);
+    w.rev_valid = true;
+    w.rev_builds = b.rebuilds;
 }
 
 // G and dE/dG of the last evaluation, original atom order, as doubles
@@ -1204,8 +1332,8 @@ void bpnn_inspect(Batch& b, const Pot& p, double* G, double* dEdG) {
     std::vector<char> g((size_t)n * ldg * rs), e((size_t)n * ldg * rs);
     std::vector<int> where(n);
     hipStream_t st = b.ctx->stream;
-    HIPCHECK(hipMemcpyAsync(g.data(), w.G, g.size(), hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipMemcpyAsync(e.data(), w.w, e.size(), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(g.data(), w.a.G, g.size(), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(e.data(), w.a.w, e.size(), hipMemcpyDeviceToHost, st));
     HIPCHECK(hipMemcpyAsync(where.data(), b.where, sizeof(int) * n, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
     for (int o = 0; o < n; ++o) {

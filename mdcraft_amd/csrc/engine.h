@@ -172,6 +172,9 @@ struct Batch {
     int* nn = nullptr;
     int* misc = nullptr;
     bool list_valid = false;
+    // positions were uploaded after the last build: the image shifts stored
+    // in the list entries may not match them until the next build
+    bool shifts_stale = false;
     int f_prec = -1;      // which force array holds the current force
     double last_rcut = -1;
     int rebuilds = 0;

@@ -695,7 +695,7 @@ void set_trigger(Batch& b, double rcut, bool allow_shift) {
         // position at the last build, so listed pairs stay within
         // rns_max + skin; their minimum image is unique while that is below
         // half of every periodic edge
-        int sh = allow_shift && skin > 0.0 ? 1 : 0;
+        int sh = allow_shift && skin > 0.0 && !b.shifts_stale ? 1 : 0;
         for (int a = 0; a < 3; ++a)
             if (d.per[a] && !(0.5 * d.L[a] > b.rns_max[s] + skin)) sh = 0;
         if (allow_shift && b.manual_rebuild) {
@@ -1065,9 +1065,14 @@ void stage_compute(Batch& b, const Pot& p, int stage, int prec) {
             break;
         }
         case ST_VIRIALS: {
-            // the BPNN virial is left at zero (the reference's is a partial
-            // diagonal, maise_bpnn.cxx:98-120)
-            if (bpnn) break;
+            if (bpnn) {
+                // the diagonal virial of maise_bpnn.cxx:98-120 from the
+                // dE/dG of the last prepare, added to V
+                if (b.bp_prepared != prec || b.bp_prepared_builds != b.rebuilds)
+                    throw Error(E_STATE, "BPNN virials need a prepare at the same precision on the current list");
+                bpnn_compute(b, p, 0, prec, MODE_COMPUTE, 0.0, -1, pos, nullptr, 2 | 4);
+                break;
+            }
             const int idle = b.sc ^ 1;
             HIPCHECK(hipMemcpyAsync(b.dst[idle], b.dst[b.sc], sizeof(double) * D_NFIELDS * b.n,
                                     hipMemcpyDeviceToDevice, st));
@@ -1088,8 +1093,8 @@ void stage_compute(Batch& b, const Pot& p, int stage, int prec) {
 void run_steps(Batch& b, const Pot& p, double dt, int nsteps, int prec, int chunk) {
     if (b.n == 0 || nsteps <= 0) return;
     hipStream_t st = b.ctx->stream;
-    set_trigger(b, p.rcut, true);
     if (!b.list_valid) nlist_rebuild(b);
+    set_trigger(b, p.rcut, true);
     if (b.f_prec < 0) throw Error(E_STATE, "no force to start from: compute forces once before stepping");
     const int grid = nblk(b.n, 256);
     int abort_init = INT_MAX;
@@ -1214,8 +1219,8 @@ void time_step_kernels(Batch& b, const Pot& p, int prec, double dt, int reps, do
     if (p.type == PT_BPNN) throw Error(E_INVAL, "kernel timing covers the LJ and EAM kernels");
     if (b.f_prec < 0) throw Error(E_STATE, "no force to start from: compute forces once before timing");
     hipStream_t st = b.ctx->stream;
-    set_trigger(b, p.rcut, true);
     if (!b.list_valid) nlist_rebuild(b);
+    set_trigger(b, p.rcut, true);
     const int idle = b.sc ^ 1;
     const size_t vbytes = sizeof(double) * 3 * (size_t)b.n;
     HIPCHECK(hipMemcpyAsync(b.d(idle, D_VX), b.d(b.sc, D_VX), vbytes, hipMemcpyDeviceToDevice, st));

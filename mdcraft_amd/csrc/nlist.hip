@@ -516,6 +516,7 @@ void nlist_rebuild(Batch& b, bool sync) {
         if (maxnn > b.cap) nlist_regrow(b, maxnn);
     }
     b.list_valid = true;
+    b.shifts_stale = false;
     b.rebuilds++;
     b.since_rebuild = 0;
 }

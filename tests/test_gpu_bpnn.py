@@ -1,10 +1,12 @@
 """BPNN on the engine against the oracle (FP64: 1e-12; mixed: FP32
-descriptors and an FP16-split MFMA network, force RMS within 1e-5)."""
+descriptors and an FP16-split MFMA network, force RMS within 1e-5 and the
+virial trace within 1e-5).  The virial is the reference's diagonal one
+(maise_bpnn.cxx:98-120)."""
 import numpy as np
 import pytest
 
 from mdcraft_amd import workloads as W
-from mdcraft_amd.atoms import vec
+from mdcraft_amd.atoms import mat, trace_V, vec
 from tests.helpers import force_rms_rel, max_rel, oracle_mod
 
 pytestmark = pytest.mark.gpu
@@ -25,9 +27,9 @@ def oracle(w, threads=8):
     return a, G, dE, (s, nl, dom, p)
 
 
-def engine(ctx, w, precision, steps=0):
+def engine(ctx, w, precision, steps=0, lanes=16):
     from mdcraft_amd import engine as E
-    s = E.System(ctx, [len(w["atoms"])], [w["box"]], lanes=16)
+    s = E.System(ctx, [len(w["atoms"])], [w["box"]], lanes=lanes)
     s.upload(w["atoms"])
     p = E.MaiseBPNNPotential(ctx, w["pot"]["path"])
     s.build()
@@ -44,23 +46,81 @@ def small(tmp_path_factory):
     return W.bpnn_cu(reps=4, model_dir=str(tmp_path_factory.mktemp("bp")))
 
 
-def test_bpnn_fp64_parity(ctx, small):
+LANES = [1, 2, 4, 8, 16]
+
+
+@pytest.mark.parametrize("lanes", LANES)
+def test_bpnn_fp64_parity(ctx, small, lanes):
     ref, G, dE, _ = oracle(small)
-    got, g, de = engine(ctx, small, 0)
+    got, g, de = engine(ctx, small, 0, lanes=lanes)
     assert max_rel(g, G) < 1e-12
     assert max_rel(de, dE) < 1e-11
     assert max_rel(got["Ep"], ref["Ep"]) < 1e-12
     assert max_rel(vec(got, "f"), vec(ref, "f")) < 1e-12
+    V, Vr = mat(got), mat(ref)
+    assert np.max(np.abs(Vr)) > 0
+    assert max_rel(V, Vr) < 1e-12
+    # only the diagonal is written, as the reference's virial() does
+    off = V.copy()
+    for c in range(3):
+        off[:, c, c] = 0.0
+    assert np.all(off == 0.0)
 
 
-def test_bpnn_mixed_accuracy(ctx, small):
+@pytest.mark.parametrize("lanes", LANES)
+def test_bpnn_mixed_accuracy(ctx, small, lanes):
     ref, G, dE, _ = oracle(small)
-    got, g, de = engine(ctx, small, 1)
+    got, g, de = engine(ctx, small, 1, lanes=lanes)
     assert max_rel(g, G) < 1e-5
     assert max_rel(de, dE) < 1e-4
     e, er = got["Ep"].sum(), ref["Ep"].sum()
     assert abs(e - er) < 1e-5 * abs(er)
     assert force_rms_rel(vec(got, "f"), vec(ref, "f")) < 1e-5
+    w, wr = trace_V(got).sum(), trace_V(ref).sum()
+    assert abs(w - wr) < 1e-5 * abs(wr)
+    assert max_rel(mat(got), mat(ref)) < 1e-4
+
+
+def test_bpnn_stage_virials(ctx, small):
+    """prepare, forces, then virials() adding the diagonal virial to V, as
+    solver::Ternary's calls do one at a time (ternary.cxx:19-67)."""
+    from mdcraft_amd import engine as E
+    ref, _, _, _ = oracle(small)
+    s = E.System(ctx, [len(small["atoms"])], [small["box"]], lanes=4)
+    s.upload(small["atoms"])
+    p = E.MaiseBPNNPotential(ctx, small["pot"]["path"])
+    s.build()
+    for st in (E.PREPARE, E.FORCES, E.VIRIALS):
+        s.stage(p, st, 0)
+    got = s.download()
+    assert max_rel(vec(got, "f"), vec(ref, "f")) < 1e-12
+    assert max_rel(mat(got), mat(ref)) < 1e-12
+    # a second virials() adds the same diagonal again
+    s.stage(p, E.VIRIALS, 0)
+    got2 = s.download()
+    assert max_rel(mat(got2), 2 * mat(ref)) < 1e-12
+
+
+def test_bpnn_lanes_drop_after_evaluation(ctx, tmp_path):
+    """500 atoms (not a multiple of 64) evaluated at 8 lanes per atom, then
+    at 1 lane: the list holds more elements at 1 lane, and the work arrays
+    made for the first list must be made again for the second."""
+    from mdcraft_amd import engine as E
+    w = W.bpnn_cu(reps=5, model_dir=str(tmp_path))
+    assert len(w["atoms"]) % 64 != 0
+    ref, _, _, _ = oracle(w)
+    s = E.System(ctx, [len(w["atoms"])], [w["box"]], lanes=8)
+    s.upload(w["atoms"])
+    p = E.MaiseBPNNPotential(ctx, w["pot"]["path"])
+    s.build()
+    s.compute(p, 7, 0)
+    for lanes in (1, 2, 16):
+        s.set_lanes(lanes)
+        s.build()
+        s.compute(p, 7, 0)
+        got = s.download()
+        assert max_rel(vec(got, "f"), vec(ref, "f")) < 1e-12, lanes
+        assert max_rel(mat(got), mat(ref)) < 1e-12, lanes
 
 
 def test_bpnn_fp64_short_nve(ctx, small):

@@ -54,8 +54,6 @@ def test_engine_matches_fixture(ctx, name, precision, tmp_path):
             checks["v"] = vec(got, "v")
         for k, v in checks.items():
             ref = outs[k]
-            if k == "V" and pot["kind"] == "bpnn":
-                continue  # the engine leaves the BPNN virial at zero
             tol = 1e-8 if meta["nve_steps"] else FP64_TOL
             assert np.max(np.abs(v - ref)) <= tol * max(np.max(np.abs(ref)), 1e-300), (name, k)
     else:

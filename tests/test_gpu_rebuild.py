@@ -71,3 +71,45 @@ def test_list_overflow_inside_steps_regrows(ctx, kind):
     assert np.max(np.abs(d)) < 1e-9
     assert max_rel(vec(got, "v"), vec(want, "v")) < 1e-8
     assert max_rel(vec(got, "f"), vec(want, "f")) < 1e-8
+
+
+def test_upload_of_rewrapped_positions_between_steps(ctx):
+    """Positions uploaded between step sequences may put an atom at another
+    periodic image than the list was built with (here the atom nearest the
+    upper x face moves to its image below the lower face).  The list stays,
+    and the next steps must use each pair's image from the new positions,
+    not the shift recorded at the build; compared with the oracle's
+    make_step loop from the same state."""
+    from oracle import refcpu as R
+    from tests.helpers import oracle_potential
+    eng = E()
+    w = W.eam_cu(reps=7, rattle=0.05)
+    at, box, pot, dt = w["atoms"], w["box"], w["pot"], w["dt"]
+    p = engine_potential(eng, ctx, pot)
+    s = eng.System(ctx, [len(at)], [box])
+    s.upload(at)
+    s.build()
+    s.compute(p, eng.FORCE, eng.FP64)
+    s.step(p, dt, 3, eng.FP64)
+    mid = s.download()
+    L = np.asarray(box[1]) - np.asarray(box[0])
+    r = vec(mid, "r")
+    k = int(np.argmax(r[:, 0]))
+    r[k, 0] -= L[0]
+    set_vec(mid, "r", r)
+    s.upload(mid, eng.F_R)
+    s.step(p, dt, 5, eng.FP64)
+    got = s.download()
+    a = mid.copy()
+    dom = R.Domain(*box)
+    pool = R.Pool(8)
+    rp = oracle_potential(R, pot)
+    a["rcut"] = rp.rcut
+    nl = R.VerletList(pool)
+    nl.update(a, dom)
+    R.Solver(rp, pool, True).run_nve(a, dom, nl, dt, 5, initial_forces=True)
+    d = vec(got, "r") - vec(a, "r")
+    d -= np.round(d / L) * L
+    assert np.max(np.abs(d)) < 1e-9
+    assert max_rel(vec(got, "v"), vec(a, "v")) < 1e-8
+    assert max_rel(vec(got, "f"), vec(a, "f")) < 1e-8
