@@ -181,6 +181,21 @@ mdc_status mdc_nlist_stats(mdc_sys* sys, double* mean_neighbours, int* max_neigh
  * virial is the reference's diagonal one, maise_bpnn.cxx:98-120, with the
  * off-diagonal entries zero) */
 mdc_status mdc_compute(mdc_sys* sys, const mdc_pot* pot, uint32_t what, int precision);
+/* No counterpart in the reference.  For callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): mdc_compute for a BPNN potential whose
+ * work arrays (descriptors, dE/dG, force and virial shares, reverse-slot
+ * index; carried by the system and allocated on its first evaluation in
+ * mdc_compute) come instead from a fixed 64 MiB work area that the library
+ * holds on each device for the life of the process.  Allocates nothing.
+ * Needs a built list (MDC_ESTATE otherwise: it builds none) and a BPNN
+ * potential (MDC_EINVAL otherwise); an evaluation whose arrays exceed the
+ * work area fails with MDC_EINVAL and a message naming the size (13 500
+ * atoms with 136 list slots need about 53 MiB in MDC_MIXED with
+ * MDC_VIRIAL).  Every call on one device shares that device's work area:
+ * calls on one context are ordered by its stream, but calls that overlap on
+ * one GPU from different contexts race. */
+mdc_status mdc_compute_noalloc(mdc_sys* sys, const mdc_pot* pot, uint32_t what, int precision);
 /* Langevin thermostats (solver/thermostat/langevin.cxx:7-42), applied one
  * after the other as thermostat::Compound does, to every force evaluation
  * of the system (mdc_compute, MDC_STAGE_FORCES, each step of mdc_step),

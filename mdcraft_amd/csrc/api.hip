@@ -7,6 +7,7 @@
 #include <cstring>
 #include <fstream>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <sstream>
 #include "engine.h"
@@ -24,9 +25,36 @@ struct mdc_sys {
     Batch b;
 };
 
+constexpr size_t kNoAllocBytes = (size_t)64 << 20;
+static __device__ __align__(256) char scratch[kNoAllocBytes];
+
 namespace {
 
 thread_local std::string g_last;
+
+// Work area of mdc_compute_noalloc: one copy per GPU, held until the process
+// ends.  In host code `scratch` is only a stand-in for the device symbol;
+// its address on each device is looked up once, when the first context on
+// that device is created, and only that address is handed to kernels.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_ptr[kMaxDevices] = {};
+
+// the device must be current
+void scratch_lookup(int device) {
+    if (device < 0 || device >= kMaxDevices) throw Error(E_INVAL, "device index above the supported range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void* p = nullptr;
+    HIPCHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw Error(E_HIP, "no device address for the allocation-free work area");
+    g_scratch_ptr[device] = static_cast<char*>(p);
+}
+
+char* scratch_on(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return device >= 0 && device < kMaxDevices ? g_scratch_ptr[device] : nullptr;
+}
 
 // 208-byte record (mdcraft/data/atom.h:21-38)
 struct AtomRec {
@@ -783,6 +811,7 @@ mdc_status mdc_ctx_create(int device, mdc_ctx** out) {
         hipDeviceProp_t prop;
         HIPCHECK(hipGetDeviceProperties(&prop, device));
         c->c.cu_count = prop.multiProcessorCount;
+        scratch_lookup(device);
         *out = c.release();
     });
 }
@@ -1319,6 +1348,39 @@ mdc_status mdc_compute(mdc_sys* sys, const mdc_pot* pot, uint32_t what, int prec
         HIPCHECK(hipSetDevice(sys->b.ctx->device));
         check_kinds(sys->b, pot->p);
         force_compute(sys->b, pot->p, (int)what, precision);
+    });
+}
+
+mdc_status mdc_compute_noalloc(mdc_sys* sys, const mdc_pot* pot, uint32_t what, int precision) {
+    return guard(sys ? sys->b.ctx : nullptr, [&] {
+        if (!sys || !pot) throw Error(E_INVAL, "null argument");
+        if (precision != MDC_FP64 && precision != MDC_MIXED) throw Error(E_INVAL, "precision must be FP64 or MIXED");
+        Batch& b = sys->b;
+        if (pot->p.type != PT_BPNN)
+            throw Error(E_INVAL, "mdc_compute_noalloc evaluates BPNN potentials (the only ones with work arrays)");
+        if (!b.list_valid) throw Error(E_STATE, "no neighbour list: build it first (this call builds none)");
+        char* base = scratch_on(b.ctx->device);
+        if (!base) throw Error(E_STATE, "no work area on this device");
+        const size_t need = bpnn_fixed_bytes(b, pot->p, (int)what, precision);
+        if (need > kNoAllocBytes)
+            throw Error(E_INVAL, "evaluation needs " + std::to_string((need + (1 << 20) - 1) >> 20) +
+                                     " MiB of work arrays; the allocation-free work area holds " +
+                                     std::to_string(kNoAllocBytes >> 20) + " MiB");
+        HIPCHECK(hipSetDevice(b.ctx->device));
+        check_kinds(b, pot->p);
+        // timing events are created on demand: off for this call
+        const bool timing = b.timing;
+        b.timing = false;
+        b.bp_fixed = base;
+        try {
+            force_compute(b, pot->p, (int)what, precision);
+        } catch (...) {
+            b.bp_fixed = nullptr;
+            b.timing = timing;
+            throw;
+        }
+        b.bp_fixed = nullptr;
+        b.timing = timing;
     });
 }
 

@@ -1311,9 +1311,45 @@ static void bpnn_run(Batch& b, const Pot& p, const BpBufs& W, bool rev_fresh, in
     else run(float(0));
 }
 
+// The work arrays carved one after the other from base (256-byte aligned
+// each); returns the bytes used.  base may be null to count only.
+static size_t carve_fixed(const Batch& b, const BpnnParams& bp, int prec, bool vir, char* base, BpBufs* out) {
+    const size_t rs = prec == P_FP64 ? 8 : 4;
+    size_t off = 0;
+    auto take = [&](size_t bytes) -> void* {
+        void* q = base ? base + off : nullptr;
+        off += (bytes + 255) & ~(size_t)255;
+        return q;
+    };
+    BpBufs W;
+    W.G = take((size_t)b.n * bp.nin_pad * rs);
+    W.w = take((size_t)b.n * bp.nin_pad * rs);
+    W.P = take((size_t)b.n * b.cap * 3 * rs);
+    W.own = take((size_t)b.n * 3 * rs);
+    if (vir) {
+        W.PV = take((size_t)b.n * b.cap * 3 * rs);
+        W.OV = take((size_t)b.n * 3 * rs);
+    }
+    W.rev = static_cast<int*>(take(b.nl_elems * sizeof(int)));
+    if (out) *out = W;
+    return off;
+}
+
+size_t bpnn_fixed_bytes(const Batch& b, const Pot& p, int what, int prec) {
+    return carve_fixed(b, *p.bp, prec, (what & W_VIRIAL) != 0, nullptr, nullptr);
+}
+
 void bpnn_compute(Batch& b, const Pot& p, int what, int prec, int mode, double dt, int step, const double4* pos,
                   double4* pos_next, int parts) {
     check_bpnn_box(b);
+    if (b.bp_fixed) {
+        // allocation-free: the work area is shared by every batch on the
+        // device, so the reverse-slot index is built again for this list
+        BpBufs W;
+        carve_fixed(b, *p.bp, prec, (parts & 4) || (what & W_VIRIAL), b.bp_fixed, &W);
+        bpnn_run(b, p, W, false, what, prec, mode, dt, step, pos, pos_next, parts);
+        return;
+    }
     const bool vir = (parts & 4) || (what & W_VIRIAL);
     ensure_work(b, *p.bp, prec, vir);
     BpWork& w = work(b);

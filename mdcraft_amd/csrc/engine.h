@@ -203,6 +203,9 @@ struct Batch {
     // scratch for BPNN
     void* bp_scratch = nullptr;
     size_t bp_scratch_bytes = 0;
+    // set for the duration of an allocation-free evaluation: the BPNN work
+    // arrays are carved from this device work area instead
+    char* bp_fixed = nullptr;
     // reduction output
     double* thermo = nullptr;
     // per-system extent of rns (host), for the displacement trigger
@@ -270,6 +273,9 @@ void bpnn_compute(Batch& b, const Pot& p, int what, int prec, int mode, double d
 void bpnn_release_work(Batch& b);
 void bpnn_inspect(Batch& b, const Pot& p, double* G, double* dEdG);
 int bpnn_inputs(const Pot& p);
+// bytes of BPNN work arrays one evaluation of this batch needs when they are
+// carved from one work area (each array 256-byte aligned)
+size_t bpnn_fixed_bytes(const Batch& b, const Pot& p, int what, int prec);
 Pot* bpnn_create(Ctx* ctx, const char* path);
 void bpnn_destroy(Pot* p);
 

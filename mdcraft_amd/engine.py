@@ -35,7 +35,7 @@ EXPORTS = [
     "mdc_sys_create", "mdc_sys_destroy", "mdc_sys_upload", "mdc_sys_download", "mdc_sys_download_range", "mdc_sys_set_active", "mdc_sys_natoms",
     "mdc_sys_set_lanes", "mdc_sys_autotune", "mdc_sys_time_kernels", "mdc_sys_set_rebuild", "mdc_sys_set_boundary", "mdc_sys_set_thermostat",
     "mdc_nlist_build", "mdc_nlist_decide", "mdc_nlist_optimize_rns", "mdc_nlist_export", "mdc_nlist_stats",
-    "mdc_compute", "mdc_stage", "mdc_step", "mdc_thermo", "mdc_sync", "mdc_timing", "mdc_bpnn_inspect",
+    "mdc_compute", "mdc_compute_noalloc", "mdc_stage", "mdc_step", "mdc_thermo", "mdc_sync", "mdc_timing", "mdc_bpnn_inspect",
 ]
 
 
@@ -86,6 +86,7 @@ def load_library(path=LIB_PATH):
         "mdc_nlist_export": (I, [P, C.POINTER(U64), C.POINTER(U64), C.POINTER(U64)]),
         "mdc_nlist_stats": (I, [P, DP, C.POINTER(I), C.POINTER(I), C.POINTER(I)]),
         "mdc_compute": (I, [P, P, U32, I]),
+        "mdc_compute_noalloc": (I, [P, P, U32, I]),
         "mdc_step": (I, [P, P, D, I, I, C.POINTER(I)]),
         "mdc_thermo": (I, [P, DP]),
         "mdc_sync": (I, [P]),
@@ -361,6 +362,12 @@ class System:
 
     def compute(self, pot, what=FORCE | ENERGY | VIRIAL, precision=FP64):
         _check(lib().mdc_compute(self.h, pot.h, what, precision))
+
+    def compute_noalloc(self, pot, what=FORCE | ENERGY | VIRIAL, precision=FP64):
+        """mdc_compute for a BPNN potential with its work arrays in the
+        library's fixed per-device work area (allocates nothing; the list
+        must be built)."""
+        _check(lib().mdc_compute_noalloc(self.h, pot.h, what, precision))
 
     def step(self, pot, dt, nsteps=1, precision=FP64):
         reb = C.c_int()
