@@ -811,6 +811,7 @@ mdc_status mdc_ctx_create(int device, mdc_ctx** out) {
         hipDeviceProp_t prop;
         HIPCHECK(hipGetDeviceProperties(&prop, device));
         c->c.cu_count = prop.multiProcessorCount;
+        c->c.lds_max = prop.sharedMemPerBlock;
         scratch_lookup(device);
         *out = c.release();
     });
@@ -1004,6 +1005,13 @@ mdc_status mdc_pot_bpnn_create(mdc_ctx* ctx, const char* path, mdc_pot** out) {
         if (!ctx || !out || !path) throw Error(E_INVAL, "null argument");
         HIPCHECK(hipSetDevice(ctx->c.device));
         *out = reinterpret_cast<mdc_pot*>(bpnn_create(&ctx->c, path));
+    });
+}
+
+mdc_status mdc_pot_bpnn_describe(const char* maise_path, int* arch, int* counts, double* rcut) {
+    return guard(nullptr, [&] {
+        if (!maise_path) throw Error(E_INVAL, "null path");
+        bpnn_describe(maise_path, arch, counts, rcut);
     });
 }
 

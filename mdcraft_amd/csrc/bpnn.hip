@@ -32,6 +32,7 @@
 #include <cstring>
 #include <fstream>
 #include <functional>
+#include <memory>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -39,7 +40,7 @@
 
 namespace mdc {
 
-constexpr int kBpMaxR = 16, kBpMaxA = 16, kBpMaxEta = 4, kBpMaxHidden = 64, kBpMaxIn = 32;
+constexpr int kBpMaxR = 16, kBpMaxA = 64, kBpMaxEta = 8, kBpMaxHidden = 64, kBpMaxIn = 64;
 constexpr int kBpMaxG = 8;  // distinct (lambda, zeta) pairs of the G4 terms
 
 struct BpDesc {
@@ -117,11 +118,14 @@ __device__ inline double drcp_(double x) { return 1.0 / x; }
 __device__ inline float drsq_(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ inline double drsq_(double x) { return 1.0 / sqrt(x); }
 
-// b^k for a whole k >= 0 (uniform across the wavefront)
+// b^k for a whole k >= 0 (uniform across the wavefront), by squaring
 template <typename R>
 __device__ inline R powu_(R b, int k) {
     R p = (R)1;
-    for (int q = 0; q < k; ++q) p *= b;
+    for (; k > 0; k >>= 1) {
+        if (k & 1) p *= b;
+        b *= b;
+    }
     return p;
 }
 
@@ -464,10 +468,11 @@ __global__ void __launch_bounds__(64) k_bp_mlp_mfma(const float* __restrict__ G,
     const float* b12 = b01 + h1;
     const float* W23 = b12 + h2;
     const float b23 = W23[h2];
+    const float* xs = W23 + h2 + 1;  // power-of-two input scales [nin_pad]
     for (int e = lane; e < 16 * inp; e += 64) {
         const int r = e / inp, c = e % inp;
         const int i = a0 + r;
-        X[r * kBpMaxHidden + c] = (i < n && c < nin) ? G[(size_t)i * ldg + c] : 0.f;
+        X[r * kBpMaxHidden + c] = (i < n && c < nin) ? G[(size_t)i * ldg + c] * xs[c] : 0.f;
     }
     __syncthreads();
     // forward
@@ -505,7 +510,8 @@ __global__ void __launch_bounds__(64) k_bp_mlp_mfma(const float* __restrict__ G,
         const int i = a0 + r;
         if (i < n) {
             const bool act = !w_inactive(pos[i].w);
-            wout[(size_t)i * ldg + c] = (act && c < nin) ? (float)(X[r * kBpMaxHidden + c] * P.escale) : 0.f;
+            wout[(size_t)i * ldg + c] =
+                (act && c < nin) ? (float)(X[r * kBpMaxHidden + c] * xs[c] * P.escale) : 0.f;
         }
     }
 }
@@ -552,16 +558,14 @@ __global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos
 #pragma unroll
     for (int q = 0; q < NR; ++q) w[q] = q < nr ? wgt[(size_t)i * ldg + q] : (R)0;
     // dE/dG of the G4 terms times their coefficients on the (group, eta) grid
-    R C[NG * NE], Ce[NG * NE];
+    R C[NG * NE];
 #pragma unroll
     for (int g = 0; g < NG; ++g)
 #pragma unroll
         for (int e = 0; e < NE; ++e) {
             const int t = g * kBpMaxEta + e;
             const int q = (g < ng && e < ne) ? P.t_q[t] : -1;
-            const R c = q >= 0 ? wgt[(size_t)i * ldg + nr + q] * (R)P.t_coef[t] : (R)0;
-            C[g * NE + e] = c;
-            Ce[g * NE + e] = c * eta_e[e];
+            C[g * NE + e] = q >= 0 ? wgt[(size_t)i * ldg + nr + q] * (R)P.t_coef[t] : (R)0;
         }
     const R Rs = (R)P.Rs, Rc2 = (R)P.Rc2, pirc = (R)P.pi_rc;
     // radial part of every slot, owned by one lane
@@ -652,10 +656,12 @@ __global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos
                 const R dfcc = (R)-0.5 * dsin_(pirc * dc) * pirc;
                 const R cs = (ax * bx + ay * by + az * bz) * ia * ib;
                 const R F = fa * fb * fcc;
-                R Ex[NE];
+                R Ex[NE], Exe[NE];
 #pragma unroll
-                for (int e = 0; e < NE; ++e)
+                for (int e = 0; e < NE; ++e) {
                     Ex[e] = e < ne ? Ea[e] * S.E[e * cap + b] * dexp_(-eta_e[e] * d2c) : (R)0;
+                    Exe[e] = Ex[e] * eta_e[e];
+                }
                 R SA = 0, SAe = 0, SZ = 0;
 #pragma unroll
                 for (int g = 0; g < NG; ++g)
@@ -666,7 +672,7 @@ __global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos
 #pragma unroll
                         for (int e = 0; e < NE; ++e) {
                             K += C[g * NE + e] * Ex[e];
-                            Ke += Ce[g * NE + e] * Ex[e];
+                            Ke += C[g * NE + e] * Exe[e];
                         }
                         const R pz = pz1 * p1;
                         SA += K * pz;
@@ -716,10 +722,12 @@ __global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos
         const R dfcc = (R)-0.5 * dsin_(pirc * dc) * pirc;
         const R cs = (ax * bx + ay * by + az * bz) * ia * ib;
         const R F = fa * fb * fcc;
-        R Ex[NE];
+        R Ex[NE], Exe[NE];
 #pragma unroll
-        for (int e = 0; e < NE; ++e)
+        for (int e = 0; e < NE; ++e) {
             Ex[e] = e < ne ? S.E[e * cap + a] * S.E[e * cap + b] * dexp_(-eta_e[e] * d2c) : (R)0;
+            Exe[e] = Ex[e] * eta_e[e];
+        }
         R SA = 0, SAe = 0, SZ = 0;
 #pragma unroll
         for (int g = 0; g < NG; ++g)
@@ -730,7 +738,7 @@ __global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos
 #pragma unroll
                 for (int e = 0; e < NE; ++e) {
                     K += C[g * NE + e] * Ex[e];
-                    Ke += Ce[g * NE + e] * Ex[e];
+                    Ke += C[g * NE + e] * Exe[e];
                 }
                 const R pz = pz1 * p1;
                 SA += K * pz;
@@ -1033,17 +1041,21 @@ void bpnn_release_work(Batch& b) {
     b.bp_scratch = nullptr;
 }
 
-Pot* bpnn_create(Ctx* ctx, const char* path) {
+// The model of a MAISE file checked against the engine's limits, with the
+// network weights in the layouts the kernels read (host side only).
+static std::unique_ptr<BpnnParams> parse_model(const char* path, std::vector<double>& W64, std::vector<float>& W32,
+                                               std::vector<half4_t>& B) {
     Maise m = read_maise(path);
     if (m.NN.size() != 4) throw Error(E_FILE, "MaiseBPNN: only 4-layer networks are supported");
     if (m.nspc != 1) throw Error(E_INVAL, "MaiseBPNN: one species is supported");
     if (m.NN[3] != 1) throw Error(E_FILE, "MaiseBPNN: the output layer must have one neuron");
-    auto* bp = new BpnnParams();
+    auto owner = std::make_unique<BpnnParams>();
+    BpnnParams* bp = owner.get();
     BpDesc& D = bp->h;
     std::memset(&D, 0, sizeof(D));
     const int nin = m.NN[0], h1 = m.NN[1], h2 = m.NN[2];
     if (h1 > kBpMaxHidden || h2 > kBpMaxHidden || nin > kBpMaxIn)
-        throw Error(E_INVAL, "MaiseBPNN: at most 32 inputs and 64 neurons per hidden layer are supported");
+        throw Error(E_INVAL, "MaiseBPNN: at most 64 inputs and 64 neurons per hidden layer are supported");
     // maise_model.cxx:34-50 unit scaling
     const double lin = m.b2a * m.scaling * 0.1, isq = 1.0 / lin / lin;
     if (m.vals.size() < 7) throw Error(E_FILE, "MAISE descriptor table incomplete");
@@ -1059,13 +1071,13 @@ Pot* bpnn_create(Ctx* ctx, const char* path) {
             if (D.nr >= kBpMaxR) throw Error(E_INVAL, "at most 16 radial descriptors are supported");
             D.eta_r[D.nr++] = eta1.at(row[4]);
         } else if (row[1] == 4) {
-            if (D.na >= kBpMaxA) throw Error(E_INVAL, "at most 16 angular descriptors are supported");
+            if (D.na >= kBpMaxA) throw Error(E_INVAL, "at most 64 angular descriptors are supported");
             double eta = eta2.at(row[5]), z = m.vals[5].at(row[7]), l = m.vals[6].at(row[8]);
             int e = -1;
             for (size_t q = 0; q < etas.size(); ++q)
                 if (etas[q] == eta) e = (int)q;
             if (e < 0) {
-                if ((int)etas.size() >= kBpMaxEta) throw Error(E_INVAL, "at most 4 distinct G4 eta values are supported");
+                if ((int)etas.size() >= kBpMaxEta) throw Error(E_INVAL, "at most 8 distinct G4 eta values are supported");
                 etas.push_back(eta);
                 e = (int)etas.size() - 1;
             }
@@ -1127,7 +1139,7 @@ Pot* bpnn_create(Ctx* ctx, const char* path) {
     if ((int)w != m.nw) throw Error(E_FILE, "MAISE weight count does not match the architecture");
     for (int k = 0; k < nin; ++k)
         for (int q = 0; q < h1; ++q) W01[(size_t)q * nin + k] *= m.range.at(k);
-    std::vector<double> W64;
+    W64.clear();
     for (auto* v : {&W01, &b01, &W12, &b12, &W23}) W64.insert(W64.end(), v->begin(), v->end());
     W64.push_back(b23[0]);
     // transposed copies for the FP64 network's column reads
@@ -1135,25 +1147,35 @@ Pot* bpnn_create(Ctx* ctx, const char* path) {
         for (int q = 0; q < h1; ++q) W64.push_back(W01[(size_t)q * nin + k]);
     for (int p = 0; p < h1; ++p)
         for (int q = 0; q < h2; ++q) W64.push_back(W12[(size_t)q * h1 + p]);
-    std::vector<float> W32;
-    for (auto* v : {&b01, &b12, &W23}) W32.insert(W32.end(), v->begin(), v->end());
-    W32.push_back((float)b23[0]);
     // B operands of the four GEMMs in the MFMA lane layout, split hi/lo
     auto pad16 = [](int x) { return (x + 15) / 16 * 16; };
     bp->nin_pad = pad16(nin);
     bp->h1_pad = pad16(h1);
     bp->h2_pad = pad16(h2);
+    // The MFMA network takes input k as G_k xs_k with xs_k the power of two
+    // nearest the file's range_k, and W01 column k divided by it (both
+    // exact): its inputs are then of order one, as the range normalises
+    // them, and the FP16 low halves of small descriptors stay normal.
+    std::vector<double> xs(bp->nin_pad, 1.0);
+    for (int k = 0; k < nin; ++k) {
+        const double r = m.range.at(k);
+        xs[k] = (r > 0 && std::isfinite(r)) ? std::ldexp(1.0, (int)std::lround(std::log2(r))) : 1.0;
+    }
+    W32.clear();
+    for (auto* v : {&b01, &b12, &W23}) W32.insert(W32.end(), v->begin(), v->end());
+    W32.push_back((float)b23[0]);
+    for (double x : xs) W32.push_back((float)x);
     struct Gm {
         int K, N;
         std::function<double(int, int)> at;
     };
     std::vector<Gm> gm = {
-        {nin, h1, [&](int k, int n) { return W01[(size_t)n * nin + k]; }},  // X W01^T
-        {h1, h2, [&](int k, int n) { return W12[(size_t)n * h1 + k]; }},    // H1 W12^T
-        {h2, h1, [&](int k, int n) { return W12[(size_t)k * h1 + n]; }},    // D2 W12
-        {h1, nin, [&](int k, int n) { return W01[(size_t)k * nin + n]; }},  // D1 W01
+        {nin, h1, [&](int k, int n) { return W01[(size_t)n * nin + k] / xs[k]; }},  // X W01^T
+        {h1, h2, [&](int k, int n) { return W12[(size_t)n * h1 + k]; }},            // H1 W12^T
+        {h2, h1, [&](int k, int n) { return W12[(size_t)k * h1 + n]; }},            // D2 W12
+        {h1, nin, [&](int k, int n) { return W01[(size_t)k * nin + n] / xs[n]; }},  // D1 W01
     };
-    std::vector<half4_t> B;
+    B.clear();
     for (int g = 0; g < 4; ++g) {
         const int kt = pad16(gm[g].K) / 16, nt = pad16(gm[g].N) / 16;
         bp->boff[g] = (int)B.size();
@@ -1174,20 +1196,56 @@ Pot* bpnn_create(Ctx* ctx, const char* path) {
                         B.push_back(v);
                     }
     }
-    bp->d = dnew<BpDesc>(1);
-    bp->w64 = dnew<double>(W64.size());
-    bp->w32 = dnew<float>(W32.size());
-    bp->bsplit = dnew<half4_t>(B.size());
-    HIPCHECK(hipMemcpy(bp->d, &D, sizeof(D), hipMemcpyHostToDevice));
-    HIPCHECK(hipMemcpy(bp->w64, W64.data(), W64.size() * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHECK(hipMemcpy(bp->w32, W32.data(), W32.size() * sizeof(float), hipMemcpyHostToDevice));
-    HIPCHECK(hipMemcpy(bp->bsplit, B.data(), B.size() * sizeof(half4_t), hipMemcpyHostToDevice));
+    return owner;
+}
+
+Pot* bpnn_create(Ctx* ctx, const char* path) {
+    std::vector<double> W64;
+    std::vector<float> W32;
+    std::vector<half4_t> B;
+    std::unique_ptr<BpnnParams> owner = parse_model(path, W64, W32, B);
+    BpnnParams* bp = owner.get();
+    const BpDesc& D = bp->h;
     auto* p = new Pot();
     p->ctx = ctx;
     p->type = PT_BPNN;
     p->rcut = D.Rc;
-    p->bp = bp;
+    p->bp = owner.release();
+    try {
+        bp->d = dnew<BpDesc>(1);
+        bp->w64 = dnew<double>(W64.size());
+        bp->w32 = dnew<float>(W32.size());
+        bp->bsplit = dnew<half4_t>(B.size());
+        HIPCHECK(hipMemcpy(bp->d, &D, sizeof(D), hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(bp->w64, W64.data(), W64.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(bp->w32, W32.data(), W32.size() * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(bp->bsplit, B.data(), B.size() * sizeof(half4_t), hipMemcpyHostToDevice));
+    } catch (...) {
+        bpnn_destroy(p);
+        throw;
+    }
     return p;
+}
+
+void bpnn_describe(const char* path, int* arch, int* counts, double* rcut) {
+    std::vector<double> W64;
+    std::vector<float> W32;
+    std::vector<half4_t> B;
+    std::unique_ptr<BpnnParams> bp = parse_model(path, W64, W32, B);
+    const BpDesc& D = bp->h;
+    if (arch) {
+        arch[0] = D.nin;
+        arch[1] = D.h1;
+        arch[2] = D.h2;
+        arch[3] = 1;
+    }
+    if (counts) {
+        counts[0] = D.nr;
+        counts[1] = D.na;
+        counts[2] = D.ne;
+        counts[3] = D.ng;
+    }
+    if (rcut) *rcut = D.Rc;
 }
 
 void bpnn_destroy(Pot* p) {
@@ -1250,8 +1308,6 @@ static void bpnn_run(Batch& b, const Pot& p, const BpBufs& W, bool rev_fresh, in
         hipLaunchKernelGGL(k_bp_rev, dim3((unsigned)((pairs + 255) / 256)), dim3(256), 0, st, b.nl, b.nn, W.rev, b.tpa,
                            b.M, b.cap, n);
     }
-    const int waves = 4;
-    const dim3 gw((n + waves - 1) / waves), bw(64 * waves);
     const int ldg = bp.nin_pad;
     AtomArrays AA{b.dst[b.sc], b.fst[b.sc], n};
     double* Ep = b.d(b.sc, D_EP);
@@ -1260,11 +1316,21 @@ static void bpnn_run(Batch& b, const Pot& p, const BpBufs& W, bool rev_fresh, in
     const int4 kts = make_int4(bp.kt[0], bp.kt[1], bp.kt[2], bp.kt[3]);
     auto run = [&](auto rtag) {
         using R = decltype(rtag);
+        // one wavefront per centre, four per workgroup unless the staged
+        // neighbours of four take more LDS than a workgroup may hold
+        const size_t lmax = std::min<size_t>(b.ctx->lds_max, 160 * 1024);
+        int waves = 4;
+        while (waves > 1 && waves * stage_bytes(b.cap, bp.h.ne, sizeof(R), true) > lmax) waves >>= 1;
         const size_t lds_desc = waves * stage_bytes(b.cap, bp.h.ne, sizeof(R), false);
         const size_t lds = waves * stage_bytes(b.cap, bp.h.ne, sizeof(R), true);
-        if (lds > 160 * 1024) throw Error(E_INVAL, "BPNN neighbour staging exceeds the LDS");
-        const bool small = bp.h.nr <= 8 && bp.h.ng <= 4 && bp.h.ne <= 2;
-        auto kdesc = small ? k_bp_desc<R, 8, 4, 2> : k_bp_desc<R, kBpMaxR, kBpMaxG, kBpMaxEta>;
+        if (lds > lmax) throw Error(E_INVAL, "BPNN neighbour staging exceeds the LDS");
+        const dim3 gw((n + waves - 1) / waves), bw(64 * waves);
+        // register arrays sized for the model: radial terms, (lambda, zeta)
+        // groups and G4 eta values
+        const int shape = (bp.h.nr <= 8 && bp.h.ng <= 4 && bp.h.ne <= 2) ? 0 : (bp.h.nr <= 8 ? 1 : 2);
+        auto kdesc = shape == 0 ? k_bp_desc<R, 8, 4, 2>
+                                : shape == 1 ? k_bp_desc<R, 8, kBpMaxG, kBpMaxEta>
+                                             : k_bp_desc<R, kBpMaxR, kBpMaxG, kBpMaxEta>;
         if (parts & 1) {
             launch_k(b, K_BP_DESC, kdesc, gw, bw, (unsigned)lds_desc, pos, b.nl, b.nn, b.tpa, b.M, b.cap, b.d_desc,
                      bp.d, (R*)W.G, ldg, n, S, b.misc);
@@ -1281,7 +1347,9 @@ static void bpnn_run(Batch& b, const Pot& p, const BpBufs& W, bool rev_fresh, in
         if (!(parts & 2)) return;
         auto grad = [&](auto vtag) {
             constexpr bool V = decltype(vtag)::value;
-            auto kgrad = small ? k_bp_grad<R, 8, 4, 2, V> : k_bp_grad<R, kBpMaxR, kBpMaxG, kBpMaxEta, V>;
+            auto kgrad = shape == 0 ? k_bp_grad<R, 8, 4, 2, V>
+                                    : shape == 1 ? k_bp_grad<R, 8, kBpMaxG, kBpMaxEta, V>
+                                                 : k_bp_grad<R, kBpMaxR, kBpMaxG, kBpMaxEta, V>;
             launch_k(b, K_BP_FORCE, kgrad, gw, bw, (unsigned)lds, pos, b.nl, b.nn, b.tpa, b.M, b.cap, b.d_desc, bp.d,
                      (const R*)W.w, ldg, (R*)W.P, (R*)W.own, (R*)W.PV, (R*)W.OV, n, S, b.misc);
         };

@@ -55,6 +55,7 @@ struct Ctx {
     int* h_pin = nullptr;
     hipEvent_t chunk_ev[2] = {nullptr, nullptr};
     int cu_count = 0;
+    size_t lds_max = 65536;  // LDS bytes one workgroup may use
 };
 
 // Langevin thermostats (thermostat/langevin.cxx:7-42) applied one after
@@ -277,6 +278,9 @@ int bpnn_inputs(const Pot& p);
 // carved from one work area (each array 256-byte aligned)
 size_t bpnn_fixed_bytes(const Batch& b, const Pot& p, int what, int prec);
 Pot* bpnn_create(Ctx* ctx, const char* path);
+// architecture (nin, h1, h2, 1), descriptor counts (radial, angular, G4 eta
+// values, (lambda, zeta) groups) and cutoff of a MAISE file, host only
+void bpnn_describe(const char* path, int* arch, int* counts, double* rcut);
 void bpnn_destroy(Pot* p);
 
 }  // namespace mdc

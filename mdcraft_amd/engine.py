@@ -31,7 +31,7 @@ EXPORTS = [
     "mdc_ctx_create", "mdc_ctx_destroy", "mdc_ctx_last_error", "mdc_last_error", "mdc_atom_record_size",
     "mdc_pot_lj_create", "mdc_pot_ljs_create", "mdc_pot_pair_create", "mdc_pot_pair_create_file", "mdc_pot_eam_create", "mdc_pot_eam_create_file", "mdc_pot_multi_create",
     "mdc_pot_alloy_create", "mdc_pot_alloy_create_file",
-    "mdc_pot_bpnn_create", "mdc_pot_rcut", "mdc_pot_destroy",
+    "mdc_pot_bpnn_create", "mdc_pot_bpnn_describe", "mdc_pot_rcut", "mdc_pot_destroy",
     "mdc_sys_create", "mdc_sys_destroy", "mdc_sys_upload", "mdc_sys_download", "mdc_sys_download_range", "mdc_sys_set_active", "mdc_sys_natoms",
     "mdc_sys_set_lanes", "mdc_sys_autotune", "mdc_sys_time_kernels", "mdc_sys_set_rebuild", "mdc_sys_set_boundary", "mdc_sys_set_thermostat",
     "mdc_nlist_build", "mdc_nlist_decide", "mdc_nlist_optimize_rns", "mdc_nlist_export", "mdc_nlist_stats",
@@ -64,6 +64,7 @@ def load_library(path=LIB_PATH):
         "mdc_pot_alloy_create": (I, [P, I, D, DP, DP, DP, I, D, I, D, PP]),
         "mdc_pot_alloy_create_file": (I, [P, C.c_char_p, PP]),
         "mdc_pot_bpnn_create": (I, [P, C.c_char_p, PP]),
+        "mdc_pot_bpnn_describe": (I, [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]),
         "mdc_pot_rcut": (I, [P, DP]),
         "mdc_pot_destroy": (I, [P]),
         "mdc_sys_create": (I, [P, I, C.POINTER(U64), DP, DP, C.POINTER(C.c_uint8), I, PP]),
@@ -230,6 +231,15 @@ def MaiseBPNNPotential(ctx, path):
     h = C.c_void_p()
     _check(lib().mdc_pot_bpnn_create(ctx.h, path.encode(), C.byref(h)))
     return Potential(ctx, h, "ternary")
+
+
+def describe_bpnn(path):
+    """Architecture, descriptor counts and cutoff (nm) of a MAISE model as
+    the engine reads it, without a device."""
+    arch, counts, rc = (C.c_int * 4)(), (C.c_int * 4)(), C.c_double()
+    _check(lib().mdc_pot_bpnn_describe(path.encode(), arch, counts, C.byref(rc)))
+    return {"arch": list(arch), "radial": counts[0], "angular": counts[1], "g4_eta": counts[2],
+            "g4_groups": counts[3], "rcut": rc.value}
 
 
 class System:

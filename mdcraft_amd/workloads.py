@@ -208,12 +208,48 @@ def eam_cu(reps=20, seed_index=0, T=300.0, rattle=0.03):
     return {"atoms": at, "box": _box([L] * 3), "pot": pot, "dt": 1 * FS, "steps": 500, "name": "eam_cu"}
 
 
+def _write_maise(path, arch, flat, ranges, b2a, scaling, g2_rows, g4_rows, values, model_id="0000BEEF",
+                 vmax=100.0):
+    """A MAISE-format model file in the layout the reference parser reads
+    (mdcraft/tools/parser/maise_model.cxx:29-129): head fields, the input
+    component range block (min, max, range per input), one weight per line,
+    B2A and Scaling, the descriptor map (N GN Rc Rs n1 n2 k z l) and the 7
+    value rows."""
+    bar = "-" * 77
+
+    def head(label, value):
+        return "|  " + label.ljust(21) + "|  " + str(value).ljust(49) + "|"
+
+    lines = [bar, "|" + "neural network general information".center(75) + "|", bar,
+             head("model unique ID", model_id), head("number of species", 1), head("species types", 29),
+             head("species names", "Cu"), head("number of layers", 4), head("architecture", " ".join(map(str, arch))),
+             head("number of weights", len(flat)), head("reference", "synthetic"), bar,
+             "|                          input component range                            |", bar,
+             "        minimum                    maximum                     range"]
+    for r in ranges:
+        lines.append("   0.0000000000000000      %.16f         %.16f" % (vmax, r))
+    lines += [bar, "|                        neural network weights                             |", bar]
+    lines += ["  %.16f" % w for w in flat]
+    lines += [bar, "  B2A          Scaling", "  %s  %s" % (b2a, scaling), "", "  N  GN  Rc  Rs   n1  n2  k   z   l", ""]
+    row = 1
+    for i in g2_rows:
+        lines.append("%3d   2   0   0   %d   0   0   0   0" % (row, i))
+        row += 1
+    for ie, iz, il in g4_rows:
+        lines.append("%3d   4   0   0   0   %d   0   %d   %d" % (row, ie, iz, il))
+        row += 1
+    lines.append("")
+    lines += values
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    return path
+
+
 def write_maise_model(path, seed_index=0, hidden=(30, 30)):
     """MAISE-format model file for the BPNN workload: 8 G2 (eta log-spaced
     0.003-0.5 A^-2, Rs = 0) and 8 G4 (eta in {0.003, 0.03}, zeta in {1, 4},
     lambda = +-1), Rc = 6 A, input range 1, Xavier-uniform weights and zero
-    biases.  Layout follows the reference parser
-    (mdcraft/tools/parser/maise_model.cxx:29-129)."""
+    biases."""
     rng = Rng(structure_seed(3, 1000 + seed_index))
     nin = 16
     arch = [nin, hidden[0], hidden[1], 1]
@@ -225,39 +261,93 @@ def write_maise_model(path, seed_index=0, hidden=(30, 30)):
         lim = math.sqrt(6.0 / (arch[l] + arch[l + 1]))
         mats.append((rng.uniform(arch[l] * arch[l + 1]) * 2 - 1) * lim)
     flat = np.concatenate(weights + mats)
-    bar = "-" * 77
-
-    def head(label, value):
-        return "|  " + label.ljust(21) + "|  " + str(value).ljust(49) + "|"
-
-    lines = [bar, "|" + "neural network general information".center(75) + "|", bar,
-             head("model unique ID", "0000BEEF"), head("number of species", 1), head("species types", 29),
-             head("species names", "Cu"), head("number of layers", 4), head("architecture", " ".join(map(str, arch))),
-             head("number of weights", len(flat)), head("reference", "synthetic"), bar,
-             "|                          input component range                            |", bar,
-             "        minimum                    maximum                     range"]
-    for _ in range(nin):
-        lines.append("   0.0000000000000000      100.0000000000000000         1.0000000000000000")
-    lines += [bar, "|                        neural network weights                             |", bar]
-    lines += ["  %.16f" % w for w in flat]
-    lines += [bar, "  B2A          Scaling", "  1.0  1.0", "", "  N  GN  Rc  Rs   n1  n2  k   z   l", ""]
-    row = 1
-    for i in range(8):
-        lines.append("%3d   2   0   0   %d   0   0   0   0" % (row, i))
-        row += 1
-    for ie in range(2):
-        for iz in range(2):
-            for il in range(2):
-                lines.append("%3d   4   0   0   0   %d   0   %d   %d" % (row, ie, iz, il))
-                row += 1
-    lines.append("")
+    g4 = [(ie, iz, il) for ie in range(2) for iz in range(2) for il in range(2)]
     eta2 = np.exp(np.linspace(np.log(0.003), np.log(0.5), 8))
-    lines += [" Rc   1   6.0", " Rs   1   0.0", " n1   8   " + "   ".join("%.10f" % e for e in eta2),
+    values = [" Rc   1   6.0", " Rs   1   0.0", " n1   8   " + "   ".join("%.10f" % e for e in eta2),
               " n2   2   0.0030000000   0.0300000000", " k    1   1.0000", " z    2   1.0000   4.0000",
               " l    2  -1.0000   1.0000"]
-    with open(path, "w") as f:
-        f.write("\n".join(lines) + "\n")
-    return path
+    return _write_maise(path, arch, flat, [1.0] * nin, "1.0", "1.0", range(8), g4, values)
+
+
+# Descriptor set of the wide BPNN model: one term per (eta, zeta, lambda) on
+# the G4 grid, the two lowest zeta values for the three smallest eta and all
+# four for the others, the last term left out: 43 G4 terms over 7 eta.
+# eta in (B2A Scaling)^-2 units, log-spaced, as the 6-decimal values the file holds
+WIDE_G2_ETA = np.array([float("%.6f" % e) for e in np.exp(np.linspace(np.log(0.001), np.log(0.4), 8))])
+WIDE_G4_ETA = np.array([float("%.6f" % e) for e in np.exp(np.linspace(np.log(0.0001), np.log(0.08), 7))])
+WIDE_ZETA = (1.0, 2.0, 4.0, 16.0)
+WIDE_B2A, WIDE_SCALING, WIDE_RC = 0.529177249, 1.25, 11.3
+
+
+def _wide_g4_rows():
+    rows = [(ie, iz, il) for ie in range(7) for iz in range(2 if ie < 3 else 4) for il in range(2)]
+    return rows[:-1]
+
+
+def _fcc_descriptor_scale(a_nm, rc_nm, eta2_nm, g4_terms):
+    """Magnitudes of the G2 and G4 descriptors of an ideal fcc site (the
+    definitions of tools/bpnn/base.h:95-155, angular terms with the factor 2
+    of base.cxx:120), to give the wide model input ranges that keep its
+    first layer out of saturation."""
+    reps = int(math.ceil(rc_nm / a_nm)) + 1
+    pts = []
+    for i in range(-reps, reps + 1):
+        for j in range(-reps, reps + 1):
+            for k in range(-reps, reps + 1):
+                for b in FCC:
+                    pts.append((np.array([i, j, k]) + b) * a_nm)
+    pts = np.array(pts)
+    d = np.linalg.norm(pts, axis=1)
+    pts = pts[(d > 1e-9) & (d < rc_nm)]
+    d = np.linalg.norm(pts, axis=1)
+    fc = 0.5 * (np.cos(np.pi * d / rc_nm) + 1)
+    g2 = [np.sum(np.exp(-e * d * d) * fc) for e in eta2_nm]
+    u = pts / d[:, None]
+    cs = u @ u.T
+    dj = np.linalg.norm(pts[:, None, :] - pts[None, :, :], axis=2)
+    ok = (dj > 1e-9) & (dj < rc_nm)
+    fcj = np.where(ok, 0.5 * (np.cos(np.pi * np.minimum(dj, rc_nm) / rc_nm) + 1), 0.0)
+    F = fc[:, None] * fc[None, :] * fcj
+    D2 = d[:, None] ** 2 + d[None, :] ** 2 + dj ** 2
+    iu = np.triu_indices(len(d), 1)
+    g4 = []
+    for eta, z, lam in g4_terms:
+        t = 2.0 * 2.0 ** (1 - z) * (1 + lam * cs) ** z * np.exp(-eta * D2) * F
+        g4.append(np.sum(t[iu]))
+    return np.array(g2), np.array(g4)
+
+
+def write_maise_model_wide(path, seed_index=0):
+    """MAISE-format model of the shape of the reference's Cu model
+    (problems/Cu-bpnn/nn_Cu_d3_v0: architecture 51-10-10-1, 8 G2 and 43 G4
+    over 7 G4 eta, zeta in {1, 2, 4, 16}, lambda = +-1, B2A Scaling = 0.529177249
+    1.25, so a length unit of 0.066 nm) with its own descriptor parameters,
+    seeded weights and non-zero biases, and input ranges that differ from 1
+    and from each other (each the inverse of the descriptor's ideal-fcc value
+    times a factor in [0.6, 1.6])."""
+    rng = Rng(structure_seed(3, 2000 + seed_index))
+    g4_rows = _wide_g4_rows()
+    nin = 8 + len(g4_rows)
+    arch = [nin, 10, 10, 1]
+    lin = WIDE_B2A * WIDE_SCALING * 0.1
+    eta2_nm = WIDE_G2_ETA / lin ** 2
+    terms = [(WIDE_G4_ETA[ie] / lin ** 2, WIDE_ZETA[iz], (-1.0, 1.0)[il]) for ie, iz, il in g4_rows]
+    g2s, g4s = _fcc_descriptor_scale(0.3615, WIDE_RC * lin, eta2_nm, terms)
+    scale = np.concatenate([g2s, g4s])
+    ranges = (0.6 + rng.uniform(nin)) / scale
+    biases = [(rng.uniform(arch[l + 1]) * 2 - 1) * 0.2 for l in range(3)]
+    mats = []
+    for l in range(3):
+        lim = math.sqrt(6.0 / (arch[l] + arch[l + 1]))
+        mats.append((rng.uniform(arch[l] * arch[l + 1]) * 2 - 1) * lim)
+    flat = np.concatenate(biases + mats)
+    values = [" Rc   1   %.4f" % WIDE_RC, " Rs   1   0.0000",
+              " n1   8   " + "   ".join("%.6f" % e for e in WIDE_G2_ETA),
+              " n2   7   " + "   ".join("%.6f" % e for e in WIDE_G4_ETA),
+              " k    1   1.0000", " z    4   " + "   ".join("%.4f" % z for z in WIDE_ZETA),
+              " l    2  -1.0000   1.0000"]
+    return _write_maise(path, arch, flat, ranges, str(WIDE_B2A), str(WIDE_SCALING), range(8), g4_rows, values,
+                        model_id="0000CAFE", vmax=0.0)
 
 
 def bpnn_cu(reps=15, seed_index=0, T=300.0, rattle=0.05, model_dir=None):
@@ -281,6 +371,44 @@ def bpnn_cu(reps=15, seed_index=0, T=300.0, rattle=0.05, model_dir=None):
     path = write_maise_model(os.path.join(model_dir, "bpnn_cu_%d.maise" % seed_index))
     pot = {"kind": "bpnn", "path": path, "rc": 0.6}
     return {"atoms": at, "box": _box([L] * 3), "pot": pot, "dt": 1 * FS, "steps": 200, "name": "bpnn_cu"}
+
+
+def bpnn_cu_wide(reps=5, seed_index=0, T=300.0, rattle=0.05, model_dir=None, periodic=(True, True, True)):
+    """fcc Cu (a = 3.615 A) under the wide BPNN model (51-10-10-1, Rc =
+    7.47 A, 43 G4 terms over 7 eta), skin 0.3 A.  The reference's own Cu
+    example uses 4^3 cells of a = 3.61 A, a box shorter than twice the
+    cutoff; 5^3 cells (500 atoms, 18.1 A) is the smallest cubic cell whose
+    minimum images are unique.  reps may be a triple; a non-periodic axis
+    gets 5 A of vacuum on both sides."""
+    a = 3.615 * ANG
+    reps3 = (reps,) * 3 if np.isscalar(reps) else tuple(reps)
+    pos, _ = lattice(FCC, reps3, a)
+    n = len(pos)
+    rng = Rng(structure_seed(6, seed_index))
+    L = np.array(reps3, dtype=np.float64) * a
+    pos = pos + rng.normal(3 * n).reshape(n, 3) * rattle * ANG
+    lo, hi = np.zeros(3), L.copy()
+    for c in range(3):
+        if periodic[c]:
+            pos[:, c] = np.mod(pos[:, c], L[c])
+        else:
+            lo[c], hi[c] = -0.5, L[c] + 0.5
+    at = make_atoms(n)
+    m = np.full(n, 63.546)
+    set_vec(at, "r", pos)
+    set_vec(at, "v", maxwell_boltzmann(rng, m, T))
+    rc = WIDE_RC * WIDE_B2A * WIDE_SCALING * 0.1
+    at["m"] = m
+    at["rcut"] = rc
+    at["rns"] = rc + 0.3 * ANG
+    at["kind"] = 0
+    at["uid"] = np.arange(1, n + 1)
+    model_dir = model_dir or os.path.join(os.environ.get("TMPDIR", "/tmp"), "mdc_models")
+    os.makedirs(model_dir, exist_ok=True)
+    path = write_maise_model_wide(os.path.join(model_dir, "bpnn_cu_wide_%d.maise" % seed_index))
+    pot = {"kind": "bpnn", "path": path, "rc": rc}
+    box = (lo, hi, tuple(bool(p) for p in periodic))
+    return {"atoms": at, "box": box, "pot": pot, "dt": 1 * FS, "steps": 50, "name": "bpnn_cu_wide"}
 
 
 def cubic_reps(cells, minimum=1):

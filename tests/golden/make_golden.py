@@ -60,7 +60,7 @@ def potential_spec(pot):
 def evaluate(R, atoms, box, pot, nve_steps=0, dt=None):
     a = atoms.copy()
     dom = R.Domain(*box)
-    pool = R.Pool(4)
+    pool = R.Pool(min(8, os.cpu_count() or 4))
     p = oracle_potential(R, pot, box)
     a["rcut"] = p.rcut
     nl = R.VerletList(pool)
@@ -78,11 +78,11 @@ def evaluate(R, atoms, box, pot, nve_steps=0, dt=None):
     return a, pair_hash(off, ids), out
 
 
-def write(name, atoms, box, pot, nve_steps=0, dt=None):
+def write(name, atoms, box, pot, nve_steps=0, dt=None, slow=False):
     R = oracle_mod()
     res, ph, extra = evaluate(R, atoms, box, pot, nve_steps, dt)
     meta = {"name": name, "potential": potential_spec(pot), "pair_sha256": ph, "nve_steps": nve_steps,
-            "dt": dt, "periodic": [bool(x) for x in box[2]]}
+            "dt": dt, "periodic": [bool(x) for x in box[2]], "slow": slow}
     from mdcraft_amd.atoms import mat, vec
     outs = {"f": vec(res, "f"), "Ep": res["Ep"].copy(), "Em": res["Em"].copy(), "nc": res["nc"].copy(),
             "V": mat(res).reshape(len(res), 9)}
@@ -108,7 +108,22 @@ def main():
         for reps in (4, 5):
             w = W.bpnn_cu(reps=reps, model_dir=d)
             write("bpnn_cu_%d" % len(w["atoms"]), w["atoms"], w["box"], w["pot"])
+    wide()
+
+
+def wide():
+    """The wide BPNN model: a 200-atom slab (periodic in x and y, open in
+    z) and the 500-atom periodic crystal (its oracle evaluation takes about
+    two minutes on 8 cores, so tests re-check it only on request)."""
+    with tempfile.TemporaryDirectory() as d:
+        w = W.bpnn_cu_wide(reps=(5, 5, 2), model_dir=d, periodic=(True, True, False))
+        write("bpnn_cu_wide_slab_200", w["atoms"], w["box"], w["pot"])
+        w = W.bpnn_cu_wide(reps=5, model_dir=d)
+        write("bpnn_cu_wide_500", w["atoms"], w["box"], w["pot"], slow=True)
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["wide"]:
+        wide()
+    else:
+        main()

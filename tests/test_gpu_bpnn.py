@@ -155,3 +155,26 @@ def test_bpnn_fp64_nve_through_rebuilds(ctx, small):
     d -= np.round(d / L) * L
     assert np.max(np.abs(d)) < 1e-9
     assert max_rel(vec(got, "v"), vec(ref, "v")) < 1e-8
+
+
+def test_bpnn_13500_atoms_against_oracle(ctx, tmp_path):
+    """The benchmark's BPNN workload at its full size (13 500 atoms, 15^3
+    fcc cells) against the oracle: FP64 forces, energies and virials within
+    1e-12, mixed precision force RMS within 1e-5 and energy and virial trace
+    within 1e-5."""
+    w = W.bpnn_cu(reps=15, model_dir=str(tmp_path))
+    print("bpnn 13500: oracle", flush=True)
+    ref, _, _, _ = oracle(w, threads=16)
+    for prec in (0, 1):
+        print("bpnn 13500: engine precision=%d" % prec, flush=True)
+        got, _, _ = engine(ctx, w, prec, lanes=8)
+        f, fr = vec(got, "f"), vec(ref, "f")
+        if prec == 0:
+            assert max_rel(f, fr) < 1e-12
+            assert max_rel(got["Ep"], ref["Ep"]) < 1e-12
+            assert max_rel(mat(got), mat(ref)) < 1e-12
+        else:
+            assert force_rms_rel(f, fr) < 1e-5
+            assert abs(got["Ep"].sum() - ref["Ep"].sum()) < 1e-5 * abs(ref["Ep"].sum())
+            w_, wr = trace_V(got).sum(), trace_V(ref).sum()
+            assert abs(w_ - wr) < 1e-5 * np.abs(trace_V(ref)).sum()
