@@ -35,6 +35,15 @@ def algorithmic_bytes(kind, nbar):
             "lj_mixed": 108 + 4 * nbar}[kind]
 
 
+def eval_force_bytes(nbar, nk, precision_bytes):
+    """Compulsory HBM bytes per atom of one evaluation-mode launch of the EAM
+    force kernel with energy and virial (the batched E/F/virial workload):
+    position record read (32), list count (4) and list (4 per neighbour),
+    F'(rho) under each of the nk objects of the atom's kind (8 each), force
+    write (12 FP32 / 24 FP64), pair energy (8) and virial (48) writes."""
+    return 32 + 4 + 4 * nbar + 8 * nk + 3 * precision_bytes + 8 + 48
+
+
 def step_bytes(kind, nbar, k):
     """Per atom-step bytes of the whole step (shared formula): LJ
     120 + 4 n + (64 + 4 n)/k; EAM 160 + 8 n + (64 + 4 n)/k."""
@@ -130,9 +139,21 @@ def run_batch_eval(E, ctx, cells, pot, precision, reps, lanes=None):
         s.compute(p, what, precision)
     ms, nl = s.timing(0)
     n = s.n
-    return {"value": n * reps / (t1 - t0), "unit": "atom-evaluations/s", "ms_per_eval": 1e3 * (t1 - t0) / reps,
-            "atoms": n, "systems": len(cells), "nbar": s.stats()["mean"], "lanes": lanes,
-            "kernel_us": {KINDS[k]: round(ms[k] / nl[k] * 1e3, 2) for k in range(8) if nl[k] > 0}}
+    nbar = s.stats()["mean"]
+    out = {"value": n * reps / (t1 - t0), "unit": "atom-evaluations/s", "ms_per_eval": 1e3 * (t1 - t0) / reps,
+           "atoms": n, "systems": len(cells), "nbar": nbar, "lanes": lanes,
+           "kernel_us": {KINDS[k]: round(ms[k] / nl[k] * 1e3, 2) for k in range(8) if nl[k] > 0}}
+    if nl[0] > 0 and pot["kind"] in ("eam_multi", "alloy", "eam"):
+        nk = pot.get("nkinds", 1) if pot["kind"] == "eam_multi" else (
+            len(pot["F"]) if pot["kind"] == "alloy" and np.ndim(pot["F"]) == 2 else 1)
+        per_launch = eval_force_bytes(nbar, nk, 4 if precision == E.MIXED else 8) * n
+        us = ms[0] / nl[0] * 1e3
+        gbs = per_launch / (us * 1e-6) / 1e9
+        out["roofline"] = {"bound": "hbm", "kernel": "k_eam_force (evaluation mode, energy and virial)",
+                           "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(gbs / HBM_PEAK_GBS, 4), "avg_launch_us": round(us, 2),
+                           "algorithmic_bytes_per_launch": int(per_launch), "traffic": None}
+    return out
 
 
 def one_at_a_time(E, ctx, cells, pot, reps):
@@ -204,7 +225,6 @@ def run_gpu(E, ctx, w, precision, steps, warmup, lanes=None):
     if lanes is None:
         lanes, tune = s.autotune(p, precision, reps=20)
     s.compute(p, E.FORCE | E.ENERGY | E.VIRIAL, precision)
-    e0 = s.thermo()[0]
     if warmup:
         s.step(p, w["dt"], warmup, precision)
     # mean durations of the step-mode pair kernels: back-to-back launches on
@@ -225,19 +245,49 @@ def run_gpu(E, ctx, w, precision, steps, warmup, lanes=None):
     s.step(p, w["dt"], min(steps, 200), precision)
     s.sync()
     ms, nl = s.timing(0)
-    s.compute(p, E.FORCE | E.ENERGY | E.VIRIAL, precision)
-    e1 = s.thermo()[0]
     n = len(at)
-    t_ps = (steps + warmup) * w["dt"]
-    drift = (e1[3] - e0[3]) / n / EV_KJMOL / (t_ps * 1e-3)  # eV/atom/ns
     per_kernel = {KINDS[k]: round(ms[k] / nl[k] * 1e3, 2) for k in range(8) if nl[k] > 0}
     return {
         "n": n, "steps": steps, "seconds": t1 - t0, "value": n * steps / (t1 - t0), "kernel_us": per_kernel,
         "ms_per_step": 1e3 * (t1 - t0) / steps, "kernel_ms": ms.tolist(), "kernel_launches": nl.tolist(),
-        "nbar": st["mean"], "rebuilds": reb, "drift_eV_atom_ns": drift, "lanes": lanes,
+        "nbar": st["mean"], "rebuilds": reb, "lanes": lanes,
         "autotune_us": tune, "b2b_us": {"density": round(b2b[0], 2), "force": round(b2b[1], 2)},
         "nbar_b2b": nbar_b2b,
     }
+
+
+def nve_drift(E, ctx, w, precision, ps=1.0, samples=21, lanes=None):
+    """NVE energy drift: the workload from its initial state for `ps`
+    picoseconds, the total energy (potential + kinetic, per atom) read at
+    `samples` evenly spaced times, and the slope of its least-squares line
+    in eV/atom/ns.  Every step between the first and last reading counts."""
+    at = w["atoms"]
+    s = E.System(ctx, [len(at)], [w["box"]], True, lanes=lanes)
+    s.upload(at)
+    p = make_pot(E, ctx, w["pot"])
+    s.build()
+    what = E.FORCE | E.ENERGY | E.VIRIAL
+    nsteps = int(round(ps / w["dt"]))
+    chunk = max(1, nsteps // (samples - 1))
+    t, e = [], []
+    s.compute(p, what, precision)
+    done = 0
+    while True:
+        th = s.thermo()[0]
+        t.append(done * w["dt"])
+        e.append(th[3] / len(at) / EV_KJMOL)
+        if done >= nsteps:
+            break
+        k = min(chunk, nsteps - done)
+        s.step(p, w["dt"], k, precision)
+        done += k
+        s.compute(p, what, precision)  # energies at the new positions (the step sequence leaves them)
+    t, e = np.array(t), np.array(e)
+    slope = np.polyfit(t, e, 1)[0] * 1e3  # eV/atom/ps -> eV/atom/ns
+    return {"drift_eV_atom_ns": float(slope), "fit": "least-squares line of E_total/atom over %d readings" % len(t),
+            "span_ps": float(t[-1]), "steps": int(done), "dt_fs": w["dt"] * 1e3,
+            "E_total_eV_atom": [round(float(e[0]), 6), round(float(e[-1]), 6)],
+            "rms_about_line_eV_atom": float(np.std(e - np.polyval(np.polyfit(t, e, 1), t)))}
 
 
 COPY_MIB = 256
@@ -253,15 +303,19 @@ def copy_calibration_buffers():
     return a, torch.empty_like(a)
 
 
-def copy_calibration(bufs):
-    """One plain elementwise kernel over 256 MiB (reads 256 MiB, writes
-    256 MiB), run three times: what FETCH_SIZE / WRITE_SIZE read for it
-    under rocprofv3 --pmc scales the counters of the engine's kernels."""
+def copy_calibration(bufs, E=None):
+    """Two kernels of known traffic, each reading 256 MiB and writing 256 MiB:
+    the engine's hand-written copy (k_copy_calibration, 16-byte loads and
+    stores), whose FETCH_SIZE / WRITE_SIZE readings under rocprofv3 --pmc
+    scale the counters of the engine's kernels, and torch's elementwise
+    multiply as a second, independent calibration."""
     import torch
     a, b = bufs
     for _ in range(3):
         torch.mul(a, 2.0, out=b)
     torch.cuda.synchronize()
+    if E is not None:
+        E.copy_calibration(0, COPY_MIB << 20, 3)
 
 
 def _csv_rows(d, suffix):
@@ -284,18 +338,22 @@ def summarise_pmc(fetch_dir, write_dir, kernel):
     for tag, d, ctr in (("fetch", fetch_dir, "FETCH_SIZE"), ("write", write_dir, "WRITE_SIZE")):
         rows = [r for r in _csv_rows(d, "counter_collection.csv") if r["Counter_Name"] == ctr]
         mine = sorted(float(r["Counter_Value"]) for r in rows if kernel in r["Kernel_Name"])
-        cal = [float(r["Counter_Value"]) for r in rows if "MulFunctor" in r["Kernel_Name"]]
+        cal = [float(r["Counter_Value"]) for r in rows if "k_copy_calibration" in r["Kernel_Name"]]
+        cal_torch = [float(r["Counter_Value"]) for r in rows if "MulFunctor" in r["Kernel_Name"]]
         if not mine or not cal:
             raise SystemExit("missing %s rows for %s or the calibration kernel" % (ctr, kernel))
         # full-work dispatches only (launches queued ahead of a rebuild exit at once)
         med = mine[len(mine) // 2]
         mine = [v for v in mine if v >= 0.5 * med]
-        kib, cal_kib = sum(mine) / len(mine), max(cal)
+        kib, cal_kib = sum(mine) / len(mine), float(np.median(cal))
         factor = COPY_MIB * 1024.0 / cal_kib
         res[tag] = kib * factor * 1024.0
         out += ["%s_launches: %d" % (tag, len(mine)), "%s_kib_per_launch: %.2f" % (tag, kib),
-                "%s_calibration_kib: %.1f (expected %d)" % (tag, cal_kib, COPY_MIB * 1024),
+                "%s_calibration_kib: %.1f (expected %d; k_copy_calibration, %d dispatches)" % (
+                    tag, cal_kib, COPY_MIB * 1024, len(cal)),
                 "%s_factor: %.5f" % (tag, factor)]
+        if cal_torch:
+            out.append("%s_factor_torch_elementwise: %.5f (cross-check)" % (tag, COPY_MIB * 1024.0 / max(cal_torch)))
     out.append("hbm_bytes_per_launch: %.0f" % (res["fetch"] + res["write"]))
     return "\n".join(out)
 
@@ -372,14 +430,31 @@ def accuracy(check, ref, ref_pairs):
     return out
 
 
-def cpu_baseline(w, max_seconds=20.0, max_steps=None, check=None):
+def cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(w, max_seconds=3.0, max_steps=None, check=None, runs=3):
     """The FP64 oracle (the reference algorithm: AoS records, full lists
     with self, Periodic ghost pass, decide()/update() rebuilds, std::thread
-    pool with the reference chunking rule) on the host cores, then on one
-    core; with check (engine_check's results) the oracle also evaluates the
-    same first configuration as the checker of the engine's."""
+    pool with the reference chunking rule) on the host's cores and on one
+    core, each the median of `runs` timed runs of about max_seconds; with
+    check (engine_check's results) the oracle also evaluates the same first
+    configuration as the checker of the engine's.  The thread count is the
+    cores this process may run on, at most 16: the CPU share of a one-GPU
+    host here (nproc is recorded beside it)."""
     from oracle import refcpu as R
-    threads = min(16, os.cpu_count() or 1)
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    threads = max(1, min(16, avail))
     pot = w["pot"]
     dom = R.Domain(*w["box"])
     p = R.LJsCut(pot["a"], pot["s"], pot["rc"]) if pot["kind"] == "lj" else R.EAM(
@@ -405,12 +480,18 @@ def cpu_baseline(w, max_seconds=20.0, max_steps=None, check=None):
         dt = time.perf_counter() - t0
         return len(a) * steps / dt, steps, dt
 
-    v, steps, dt = timed(threads, max_seconds, max_steps)
-    v1, steps1, dt1 = timed(1, 8.0, max_steps)
+    def median_of(nthreads, seconds):
+        rs = sorted((timed(nthreads, seconds, max_steps) for _ in range(runs)), key=lambda r: r[0])
+        return rs[len(rs) // 2], [round(r[0], 1) for r in rs]
+
+    (v, steps, dt), all_v = median_of(threads, max_seconds)
+    (v1, steps1, dt1), all_v1 = median_of(1, max_seconds)
     out = {"value": v, "unit": "atom-steps/s", "cores": threads, "kind": "port",
-           "sample": "%s, %d atoms, %d FP64 NVE steps with decide()/update() rebuilds, %d threads, %.1f s" % (
-               w["name"], len(w["atoms"]), steps, threads, dt),
-           "single_core": {"value": v1, "sample": "the same, %d steps on 1 thread, %.1f s" % (steps1, dt1)}}
+           "sample": "%s, %d atoms, %d FP64 NVE steps with decide()/update() rebuilds, %d threads, %.1f s; "
+                     "median of %d runs" % (w["name"], len(w["atoms"]), steps, threads, dt, runs),
+           "runs": all_v, "cpu_model": cpu_model(), "nproc": os.cpu_count(), "affinity_cpus": avail,
+           "single_core": {"value": v1, "runs": all_v1,
+                           "sample": "the same, %d steps on 1 thread, %.1f s; median of %d runs" % (steps1, dt1, runs)}}
     if check is not None:
         pool = R.Pool(threads)
         ref = w["atoms"].copy()
@@ -461,6 +542,10 @@ def main():
     ap.add_argument("--summarise-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
     ap.add_argument("--summarise-stats", metavar="STATS_DIR")
     ap.add_argument("--kernel", default="k_eam_force<float")
+    ap.add_argument("--eam-reps", type=int, nargs="+", default=None,
+                    help="--only eam_sizes: fcc cells per edge of the sizes to run (default 20 40 64 101)")
+    ap.add_argument("--precision", choices=("mixed", "fp64", "both"), default="both",
+                    help="--only bpnn_cu / nial_batch: precisions to run")
     ap.add_argument("--only", choices=("bpnn_cu", "nial_batch", "nial_alloy", "served", "eam_sizes"),
                     help="run one of the extra workloads alone and print its JSON (for profiling)")
     args = ap.parse_args()
@@ -480,19 +565,21 @@ def main():
 
     ctx = E.Context(0)
     if args.only:
+        precs = [(p, pr) for p, pr in (("mixed", E.MIXED), ("fp64", E.FP64)) if args.precision in (p, "both")]
         if args.only == "bpnn_cu":
             wb = W.bpnn_cu()
-            out = {p: run_batch(E, ctx, [(wb["atoms"], wb["box"])], wb["pot"], wb["dt"], pr, 200, 10, args.lanes)
-                   for p, pr in (("mixed", E.MIXED), ("fp64", E.FP64))}
+            out = {p: run_batch(E, ctx, [(wb["atoms"], wb["box"])], wb["pot"], wb["dt"], pr, min(args.steps, 200),
+                                10, args.lanes) for p, pr in precs}
         elif args.only == "eam_sizes":
-            out = eam_sizes(E, ctx, W, args.lanes)
+            out = eam_sizes(E, ctx, W, args.lanes, **({"reps": tuple(args.eam_reps)} if args.eam_reps else {}))
         elif args.only in ("nial_batch", "nial_alloy"):
             nb = W.nial_batch(512) if args.only == "nial_batch" else W.nial_alloy_batch(512)
-            out = {p: run_batch_eval(E, ctx, nb["cells"], nb["pot"], pr, 50, args.lanes)
-                   for p, pr in (("mixed", E.MIXED), ("fp64", E.FP64))}
+            out = {p: run_batch_eval(E, ctx, nb["cells"], nb["pot"], pr, 50, args.lanes) for p, pr in precs}
         else:
             out = served_stream(E, ctx, W)
         print(json.dumps({"workload": args.only, "result": out}), flush=True)
+        if cal is not None:
+            copy_calibration(cal, E)
         return
     w = W.eam_cu()
     head = run_gpu(E, ctx, w, E.MIXED, args.steps, args.warmup, args.lanes)
@@ -509,10 +596,18 @@ def main():
             "algorithmic_bytes_per_launch": int(per_launch)}
     k = args.steps / max(head["rebuilds"], 1)
     step_gbs = step_bytes("eam", nbar, k) * head["value"] / 1e9
-    keys = ("value", "ms_per_step", "nbar", "rebuilds", "drift_eV_atom_ns", "kernel_us", "b2b_us", "lanes")
+    keys = ("value", "ms_per_step", "nbar", "rebuilds", "kernel_us", "b2b_us", "lanes")
     extra = {"eam_cu_mixed": {kk: head[kk] for kk in keys},
              "eam_cu_mixed_step_GBs": round(step_gbs, 1)}
+    # NVE energy drift: 1 ps of each workload from its initial state, slope
+    # of a line fitted to the total energy
+    drift = {"eam_cu_mixed": nve_drift(E, ctx, w, E.MIXED), "eam_cu_fp64": nve_drift(E, ctx, w, E.FP64)}
+    extra["nve_drift"] = drift
     if not args.no_extra:
+        drift["lj_argon_fp64"] = nve_drift(E, ctx, W.lj_argon(), E.FP64, ps=2.0)
+        wbd = W.bpnn_cu()
+        drift["bpnn_cu_mixed"] = nve_drift(E, ctx, wbd, E.MIXED)
+        drift["bpnn_cu_fp64"] = nve_drift(E, ctx, wbd, E.FP64)
         r = run_gpu(E, ctx, w, E.FP64, args.steps, args.warmup, args.lanes)
         extra["eam_cu_fp64"] = {kk: r[kk] for kk in keys}
         wl = W.lj_argon()
@@ -557,7 +652,7 @@ def main():
     }
     print(json.dumps(line), flush=True)
     if cal is not None:
-        copy_calibration(cal)
+        copy_calibration(cal, E)
 
 
 if __name__ == "__main__":

@@ -259,6 +259,12 @@ mdc_status mdc_sync(mdc_sys* sys);
  * evaluation, [n][nin] in original atom order (the per-atom g and dedg of
  * maise_bpnn.h:40-47); G or dEdG may be NULL; *nin receives the width */
 mdc_status mdc_bpnn_inspect(mdc_sys* sys, const mdc_pot* pot, double* G, double* dEdG, int* nin);
+/* Measurement hook, no reference counterpart: `reps` launches of a plain
+ * device copy of `bytes` (a multiple of 16) from one buffer to another on
+ * the device's default stream, after one untimed launch; *us_per_copy (may
+ * be NULL) receives the mean time per copy.  Its FETCH_SIZE / WRITE_SIZE
+ * counter readings calibrate the engine kernels' traffic figures. */
+mdc_status mdc_copy_calibration(int device, uint64_t bytes, int reps, double* us_per_copy);
 /* kernel timing with hipEvents on the launch stream: kind 0 pair force,
  * 1 EAM density, 2 pre-drift, 3 BPNN descriptors, 4 BPNN MLP, 5 BPNN force,
  * 6 BPNN gather, 7 rebuild */

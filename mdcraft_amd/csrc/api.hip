@@ -212,6 +212,16 @@ Pot* make_eam(Ctx* ctx, int nk, double rcut, const double* F, const double* rho,
     HIPCHECK(hipMemcpy(T.r64, rec64.data(), rec64.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(T.r32, rec32.data(), rec32.size() * sizeof(float), hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(T.F64, F64.data(), F64.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (nk == 1) {
+        std::vector<float> pr((size_t)8 * nr);
+        for (int k = 0; k < nr; ++k)
+            for (int c = 0; c < 4; ++c) {
+                pr[(size_t)8 * k + c] = rec32[(size_t)4 * k + c];
+                pr[(size_t)8 * k + 4 + c] = rec32[(size_t)4 * (nr + k) + c];
+            }
+        T.r32p = dalloc<float4>((size_t)2 * nr);
+        HIPCHECK(hipMemcpy(T.r32p, pr.data(), pr.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
     if (nk > 1) {
         // one object per pair of elements lo <= hi, as the Multi kernels
         // take them: the lo atom sums rho_hi and embeds with F_lo, the hi
@@ -323,6 +333,7 @@ void free_pot(Pot* p) {
     }
     dfree(p->eam.r64);
     dfree(p->eam.r32);
+    dfree(p->eam.r32p);
     dfree(p->eam.F64);
     dfree(p->eam.objs);
     delete p;
@@ -1496,6 +1507,10 @@ mdc_status mdc_bpnn_inspect(mdc_sys* sys, const mdc_pot* pot, double* G, double*
         if (nin) *nin = bpnn_inputs(pot->p);
         if (G || dEdG) bpnn_inspect(sys->b, pot->p, G, dEdG);
     });
+}
+
+mdc_status mdc_copy_calibration(int device, uint64_t bytes, int reps, double* us_per_copy) {
+    return guard(nullptr, [&] { copy_calibration(device, (size_t)bytes, reps, us_per_copy); });
 }
 
 mdc_status mdc_timing(mdc_sys* sys, int enable, double* total_ms, int64_t* launches) {

@@ -78,6 +78,51 @@ __device__ inline double buf_d1(__amdgpu_buffer_rsrc_t r, unsigned off) {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
 }
 
+// A dword of the adjacent lane (lane 2k <-> 2k + 1), by a DPP quad
+// permutation [1, 0, 3, 2]; every lane of the wave must be active.
+__device__ inline unsigned swap_adj(unsigned v) {
+    return (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0xB1, 0xF, 0xF, false);
+}
+
+// Cooperative 32-byte gathers.  Lanes 2k and 2k + 1 each need one 32-byte
+// record, at byte offsets off (kSkip: none, read as zeros).  In the first
+// load both lanes read the even lane's record, the even lane its low half
+// and the odd lane its high half; in the second both read the odd lane's
+// record; one exchange of 16 bytes then gives each lane its whole record.
+// Both halves of a record come from one instruction, so a record costs one
+// cache-line lookup in the vector L1 instead of two, which is what bounds
+// neighbour gathers here.  Every lane of the wave must execute it (uniform
+// control flow).
+__device__ inline void load32_pair(__amdgpu_buffer_rsrc_t r, unsigned off, u32x4_t& lo, u32x4_t& hi) {
+    const bool odd = (__lane_id() & 1u) != 0;
+    const unsigned other = swap_adj(off);
+    const unsigned half = odd ? 16u : 0u;
+    const u32x4_t a = __builtin_amdgcn_raw_buffer_load_b128(r, (odd ? other : off) + half, 0, 0);
+    const u32x4_t b = __builtin_amdgcn_raw_buffer_load_b128(r, (odd ? off : other) + half, 0, 0);
+    const u32x4_t send = odd ? a : b;
+    u32x4_t recv;
+    recv[0] = swap_adj(send[0]);
+    recv[1] = swap_adj(send[1]);
+    recv[2] = swap_adj(send[2]);
+    recv[3] = swap_adj(send[3]);
+    lo = odd ? recv : a;
+    hi = odd ? b : recv;
+}
+__device__ inline double4 buf_d4_pair(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    u32x4_t lo, hi;
+    load32_pair(r, off, lo, hi);
+    const d2_t a = __builtin_bit_cast(d2_t, lo), b = __builtin_bit_cast(d2_t, hi);
+    return make_double4(a[0], a[1], b[0], b[1]);
+}
+// two float4 records side by side (32 bytes), read the same way
+__device__ inline void buf_f4x2_pair(__amdgpu_buffer_rsrc_t r, unsigned off, float4& first, float4& second) {
+    u32x4_t lo, hi;
+    load32_pair(r, off, lo, hi);
+    const f32x4_t a = __builtin_bit_cast(f32x4_t, lo), b = __builtin_bit_cast(f32x4_t, hi);
+    first = make_float4(a[0], a[1], a[2], a[3]);
+    second = make_float4(b[0], b[1], b[2], b[3]);
+}
+
 // tag word packed into pos.w
 __host__ __device__ inline double pack_w(int kind, int inactive, int sys) {
     return (double)((long long)kind | ((long long)inactive << 8) | ((long long)sys << 9));

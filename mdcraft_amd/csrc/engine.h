@@ -117,6 +117,10 @@ struct EamTables {
     double4* r64 = nullptr;
     float4* r32 = nullptr;
     double4* F64 = nullptr;
+    // one element: the FP32 rho and r*phi records of each interval side by
+    // side ([2 nr]: rho k, r*phi k), so the force pass reads both in one
+    // 32-byte record
+    float4* r32p = nullptr;
     // Multi potential (nk > 1): objects per pair of kinds, records of all
     // objects concatenated in r64 / r32 / F64.  alloy: the nk elements of an
     // alloy EAM (per-element density and embedding, a pair term per pair of
@@ -266,6 +270,8 @@ void thermo(Batch& b, double* out);
 void set_trigger(Batch& b, double rcut, bool allow_shift);
 void stage_compute(Batch& b, const Pot& p, int stage, int prec);
 void check_errors(Batch& b, const int* counters = nullptr);
+// measurement hook (calib.hip)
+void copy_calibration(int device, size_t bytes, int reps, double* us_per_copy);
 // launchers (bpnn.hip)
 // parts: 1 = descriptors and network (G, dE/dG, Ep), 2 = forces from the
 // stored dE/dG, 3 = both

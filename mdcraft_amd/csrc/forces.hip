@@ -160,6 +160,7 @@ struct EamK {
     double dr, dr1, drho, drho1, rc2;
     const double4* r64;
     const float4* r32;
+    const float4* r32p;  // one element: rho and r*phi FP32 records of each interval side by side
     const double4* F64;
     const EamObj* objs;  // Multi: objects by pair_index(kind_i, kind_j)
     double* fpm;         // Multi: F' of every atom under the object of its kind and kind k, [nk][n]
@@ -176,6 +177,18 @@ __device__ inline float4 load_rec<float>(__amdgpu_buffer_rsrc_t b, int rec) { re
 template <typename R>
 __device__ inline typename Vec4T<R>::T load_rec_if(__amdgpu_buffer_rsrc_t b, int rec, bool ok) {
     return load_rec<R>(b, ok ? rec : (int)(kSkip / (unsigned)sizeof(typename Vec4T<R>::T)));
+}
+// The same inside neighbour loops that every lane runs alike: FP64 records
+// (32 bytes) by the cooperative lane-pair load
+template <typename R>
+__device__ inline typename Vec4T<R>::T load_rec_lanes(__amdgpu_buffer_rsrc_t b, int rec, bool ok);
+template <>
+__device__ inline double4 load_rec_lanes<double>(__amdgpu_buffer_rsrc_t b, int rec, bool ok) {
+    return buf_d4_pair(b, ok ? (unsigned)rec * 32u : kSkip);
+}
+template <>
+__device__ inline float4 load_rec_lanes<float>(__amdgpu_buffer_rsrc_t b, int rec, bool ok) {
+    return buf_f4(b, ok ? (unsigned)rec * 16u : kSkip);
 }
 
 template <typename R>
@@ -301,7 +314,9 @@ __global__ void __launch_bounds__(256) k_eam_rho(const double4* __restrict__ pos
     int jn[kU];  // list entries of the next group, read one group ahead
 #pragma unroll
     for (int u = 0; u < kU; ++u) jn[u] = j0[u];
-    for (int m0 = 0; m0 * TPA + as.sub < cnt; m0 += kU) {
+    // every lane runs the wave's longest list (cooperative loads)
+    const int nch = wave_chunks<TPA>(cnt);
+    for (int m0 = 0; m0 < nch; m0 += kU) {
         int e[kU];
         bool okk[kU];
 #pragma unroll
@@ -312,7 +327,7 @@ __global__ void __launch_bounds__(256) k_eam_rho(const double4* __restrict__ pos
         }
         double4 pj[kU];
 #pragma unroll
-        for (int u = 0; u < kU; ++u) pj[u] = buf_d4(posb, okk[u] ? (unsigned)ent_j(e[u]) * 32u : kSkip);
+        for (int u = 0; u < kU; ++u) pj[u] = buf_d4_pair(posb, okk[u] ? (unsigned)ent_j(e[u]) * 32u : kSkip);
         next_group<TPA, kU>(as, m0, M, nl, jn);
         PairGeo<R> g[kU];
         int ob[kU], toff[kU];
@@ -334,7 +349,7 @@ __global__ void __launch_bounds__(256) k_eam_rho(const double4* __restrict__ pos
         }
         typename Vec4T<R>::T rec[kU];
 #pragma unroll
-        for (int u = 0; u < kU; ++u) rec[u] = load_rec_if<R>(tabb, toff[u] + g[u].N1, g[u].ok);
+        for (int u = 0; u < kU; ++u) rec[u] = load_rec_lanes<R>(tabb, toff[u] + g[u].N1, g[u].ok);
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             if (!g[u].ok) continue;
@@ -421,11 +436,15 @@ __global__ void __launch_bounds__(256) k_eam_force(const double4* __restrict__ p
     const __amdgpu_buffer_rsrc_t posb = make_buf(MULTI ? (const void*)pos : (const void*)P.posF, (unsigned)n * 32u);
     const __amdgpu_buffer_rsrc_t fpb = make_buf(Fp, (unsigned)(MULTI ? P.nk : 1) * (unsigned)n * 8u);
     const __amdgpu_buffer_rsrc_t tabb = make_buf(tab, P.nrec * (unsigned)sizeof(*tab));
+    // one element, FP32: each interval's rho and r*phi records as one 32-byte record
+    const __amdgpu_buffer_rsrc_t tabpb = make_buf(P.r32p, (!MULTI && P.r32p) ? (unsigned)P.nr * 32u : 0u);
     const unsigned fp_base = (unsigned)ki * (unsigned)n;  // F' of neighbours under the objects of (k_j, kind_i)
     int jn[kU];  // list entries of the next group, read one group ahead
 #pragma unroll
     for (int u = 0; u < kU; ++u) jn[u] = j0[u];
-    for (int m0 = 0; m0 * TPA + as.sub < cnt; m0 += kU) {
+    // every lane runs the wave's longest list (cooperative loads)
+    const int nch = wave_chunks<TPA>(cnt);
+    for (int m0 = 0; m0 < nch; m0 += kU) {
         int e[kU];
         bool okk[kU];
 #pragma unroll
@@ -438,7 +457,7 @@ __global__ void __launch_bounds__(256) k_eam_force(const double4* __restrict__ p
         R Fpj[kU];
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
-            pj[u] = buf_d4(posb, okk[u] ? (unsigned)ent_j(e[u]) * 32u : kSkip);
+            pj[u] = buf_d4_pair(posb, okk[u] ? (unsigned)ent_j(e[u]) * 32u : kSkip);
             Fpj[u] = MULTI ? (R)buf_d1(fpb, okk[u] ? (fp_base + (unsigned)ent_j(e[u])) * 8u : kSkip) : (R)pj[u].w;
         }
         next_group<TPA, kU>(as, m0, M, nl, jn);
@@ -460,14 +479,19 @@ __global__ void __launch_bounds__(256) k_eam_force(const double4* __restrict__ p
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             if (!MULTI) {
-                rr[u] = load_rec_if<R>(tabb, g[u].N1, g[u].ok);
-                rp[u] = load_rec_if<R>(tabb, P.nr + g[u].N1, g[u].ok);
+                if constexpr (sizeof(R) == 4) {
+                    buf_f4x2_pair(tabpb, g[u].ok ? (unsigned)g[u].N1 * 32u : kSkip, rr[u], rp[u]);
+                } else {
+                    rr[u] = load_rec_lanes<R>(tabb, g[u].N1, g[u].ok);
+                    rp[u] = load_rec_lanes<R>(tabb, P.nr + g[u].N1, g[u].ok);
+                }
             } else {
                 const EamObj& o = s_obj[ob[u]];
                 const bool hi = ki > kj[u];  // atom i is the higher kind of the pair
-                rr[u] = load_rec_if<R>(tabb, (hi ? o.rho_hi_off : o.rho_off) + g[u].N1, g[u].ok);
-                rp[u] = load_rec_if<R>(tabb, o.phi_off + g[u].N1, g[u].ok);
-                if (DIR) rj[u] = load_rec_if<R>(tabb, (hi ? o.rho_off : o.rho_hi_off) + g[u].N1, g[u].ok && kj[u] != ki);
+                rr[u] = load_rec_lanes<R>(tabb, (hi ? o.rho_hi_off : o.rho_off) + g[u].N1, g[u].ok);
+                rp[u] = load_rec_lanes<R>(tabb, o.phi_off + g[u].N1, g[u].ok);
+                if (DIR)
+                    rj[u] = load_rec_lanes<R>(tabb, (hi ? o.rho_off : o.rho_hi_off) + g[u].N1, g[u].ok && kj[u] != ki);
             }
         }
 #pragma unroll
@@ -728,6 +752,7 @@ static EamK eam_args(const Pot& p, Batch& b) {
     k.drho1 = 1.0 / p.eam.drho;
     k.rc2 = p.eam.rc * p.eam.rc;
     k.r64 = p.eam.r64;
+    k.r32p = p.eam.r32p;
     k.r32 = p.eam.r32;
     k.F64 = p.eam.F64;
     k.objs = p.eam.objs;

@@ -200,10 +200,21 @@ __device__ inline bool pair_delta(const Frame& s, const double4& pi, const doubl
     return pair_image(s, dx, dy, dz, d2);
 }
 
+// Chunks of the longest list among the wave's lanes, the same value in
+// every lane: neighbour loops over it run all lanes the same number of
+// times, as the cooperative 32-byte loads (load32_pair) need.
+template <int TPA>
+__device__ inline int wave_chunks(int cnt) {
+    int c = (cnt + TPA - 1) / TPA;
+    for (int o = kWave / 2; o > 0; o >>= 1) c = max(c, __shfl_xor(c, o, kWave));
+    return __builtin_amdgcn_readfirstlane(c);
+}
+
 template <int TPA, int U, class Body>
 __device__ inline void for_neighbours(const AtomSlots& as, int cnt, const int* __restrict__ nl,
                                       __amdgpu_buffer_rsrc_t posb, const int* j0, Body&& body) {
-    for (int m0 = 0; m0 * TPA + as.sub < cnt; m0 += U) {
+    const int nch = wave_chunks<TPA>(cnt);
+    for (int m0 = 0; m0 < nch; m0 += U) {
         int e[U];
         bool ok[U];
 #pragma unroll
@@ -214,7 +225,7 @@ __device__ inline void for_neighbours(const AtomSlots& as, int cnt, const int* _
         }
         double4 pj[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) pj[u] = buf_d4(posb, ok[u] ? (unsigned)ent_j(e[u]) * 32u : kSkip);
+        for (int u = 0; u < U; ++u) pj[u] = buf_d4_pair(posb, ok[u] ? (unsigned)ent_j(e[u]) * 32u : kSkip);
 #pragma unroll
         for (int u = 0; u < U; ++u)
             if (ok[u]) body(e[u], pj[u]);

@@ -36,6 +36,7 @@ EXPORTS = [
     "mdc_sys_set_lanes", "mdc_sys_autotune", "mdc_sys_time_kernels", "mdc_sys_set_rebuild", "mdc_sys_set_boundary", "mdc_sys_set_thermostat",
     "mdc_nlist_build", "mdc_nlist_decide", "mdc_nlist_optimize_rns", "mdc_nlist_export", "mdc_nlist_stats",
     "mdc_compute", "mdc_compute_noalloc", "mdc_stage", "mdc_step", "mdc_thermo", "mdc_sync", "mdc_timing", "mdc_bpnn_inspect",
+    "mdc_copy_calibration",
 ]
 
 
@@ -88,6 +89,7 @@ def load_library(path=LIB_PATH):
         "mdc_nlist_stats": (I, [P, DP, C.POINTER(I), C.POINTER(I), C.POINTER(I)]),
         "mdc_compute": (I, [P, P, U32, I]),
         "mdc_compute_noalloc": (I, [P, P, U32, I]),
+        "mdc_copy_calibration": (I, [I, C.c_uint64, I, C.POINTER(C.c_double)]),
         "mdc_step": (I, [P, P, D, I, I, C.POINTER(I)]),
         "mdc_thermo": (I, [P, DP]),
         "mdc_sync": (I, [P]),
@@ -231,6 +233,14 @@ def MaiseBPNNPotential(ctx, path):
     h = C.c_void_p()
     _check(lib().mdc_pot_bpnn_create(ctx.h, path.encode(), C.byref(h)))
     return Potential(ctx, h, "ternary")
+
+
+def copy_calibration(device, nbytes, reps=3):
+    """Mean microseconds of a plain device copy of nbytes (the calibration
+    kernel of the traffic counters)."""
+    us = C.c_double()
+    _check(lib().mdc_copy_calibration(device, nbytes, reps, C.byref(us)))
+    return us.value
 
 
 def describe_bpnn(path):
