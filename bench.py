@@ -195,20 +195,43 @@ def served_stream(E, ctx, W, nreq=120, factors=(0.5, 1.0, 2.0)):
         return [serve.Request(q, sp["kind"], sp["atoms"], sp["box"], sp["dt"], sp["steps"], 0.0)
                 for q, sp in enumerate(specs)]
 
+    def served(trace):
+        srv = serve.Server(E, ctx, pots, prec)
+        t0 = time.perf_counter()
+        done = srv.run_trace(trace)
+        wall = time.perf_counter() - t0
+        st = serve.latency_stats(done)
+        info = srv.stats()
+        srv.close()
+        st["peak_open_batch_GB"] = info["peak_bytes"] / 1e9
+        st["batches_opened"] = int(info["batches_opened"])
+        return st, wall
+
     # warm both paths (kernels, allocator) on a few requests
-    serve.Server(E, ctx, pots, prec).run_trace(serve.poisson_trace(mk()[:6], None))
+    served(serve.poisson_trace(mk()[:6], None))
     serve.one_at_a_time(E, ctx, pots, serve.poisson_trace(mk()[:6], None))
-    sat = serve.latency_stats(serve.Server(E, ctx, pots, prec).run_trace(serve.poisson_trace(mk(), None)))
+    sat, sat_wall = served(serve.poisson_trace(mk(), None))
     base_sat = serve.latency_stats(serve.one_at_a_time(E, ctx, pots, serve.poisson_trace(mk(), None)))
     cap = sat["structures_per_s"]
+    # The whole stream against the HBM roof: every request's atom-steps
+    # (steps plus its final evaluation) at the per-atom-step algorithmic
+    # bytes of its potential with the workloads' mean list lengths (LJ
+    # 120 + 4 n, n = 79; EAM 160 + 8 n, n = 53; BPNN 120 + 8 n + 128, n = 89),
+    # over the saturated run's wall time.
+    per = {"lj": 120 + 4 * 79, "eam": 160 + 8 * 53, "bpnn": 120 + 8 * 89 + 128}
+    alg = sum(len(sp["atoms"]) * (sp["steps"] + 1) * per[sp["kind"]] for sp in specs)
+    gbs = alg / sat_wall / 1e9
     out = {"requests": nreq, "mean_atoms": float(np.mean([len(sp["atoms"]) for sp in specs])),
            "single_point_fraction": float(np.mean([sp["steps"] == 0 for sp in specs])),
-           "hbm_budget_GB": 200, "saturation": sat, "one_at_a_time_fp64_saturation": base_sat, "rates": []}
+           "hbm_budget_GB": 200, "saturation": sat, "one_at_a_time_fp64_saturation": base_sat,
+           "saturation_vs_one_at_a_time": round(sat["structures_per_s"] / base_sat["structures_per_s"], 3),
+           "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(gbs / HBM_PEAK_GBS, 4), "algorithmic_bytes": int(alg),
+                        "wall_s": round(sat_wall, 4), "note": "whole saturated stream, all kernels"},
+           "rates": []}
     for f in factors:
         rate = f * cap
-        srv = serve.Server(E, ctx, pots, prec)
-        st = serve.latency_stats(srv.run_trace(serve.poisson_trace(mk(), rate, seed=int(100 * f))))
-        st["peak_admitted_GB"] = srv.peak_bytes / 1e9
+        st, _ = served(serve.poisson_trace(mk(), rate, seed=int(100 * f)))
         bt = serve.latency_stats(serve.one_at_a_time(E, ctx, pots, serve.poisson_trace(mk(), rate, seed=int(100 * f))))
         out["rates"].append({"factor": f, "rate_per_s": rate, "served": st, "one_at_a_time_fp64": bt})
     return out

@@ -144,6 +144,17 @@ mdc_status mdc_sys_download_range(mdc_sys* sys, uint64_t first, uint64_t count, 
  * others step.  No reference counterpart: batches of boxes are this
  * engine's. */
 mdc_status mdc_sys_set_active(mdc_sys* sys, int box, int active);
+/* Slot refill for continuous batching (no reference counterpart): box `box`
+ * takes `count` atom records (at most the box's record count fixed at
+ * mdc_sys_create), in caller's order [off_box, off_box + count), with a new
+ * geometry; the box's remaining records become void: they hold no atom,
+ * take part in nothing and download as zero records with tag 0.  Every
+ * field comes from the records (forces included: compute before stepping
+ * when they are not current).  The other boxes keep their state; the list
+ * is rebuilt before the next evaluation.  mdc_sys_set_active covers the
+ * box's atoms only. */
+mdc_status mdc_sys_refill_box(mdc_sys* sys, int box, uint64_t count, const void* atoms, const double* box_lo,
+                              const double* box_hi, const uint8_t* periodic);
 mdc_status mdc_sys_natoms(const mdc_sys* sys, uint64_t* n);
 /* lanes per atom (1, 2, 4, 8, 16) of the force kernels; changes the list layout */
 mdc_status mdc_sys_set_lanes(mdc_sys* sys, int lanes_per_atom);
@@ -253,6 +264,46 @@ mdc_status mdc_step(mdc_sys* sys, const mdc_pot* pot, double dt, int nsteps, int
 /* per system: Ep, Em, KE, Etot, W (sum of virial traces), com velocity[3] */
 mdc_status mdc_thermo(mdc_sys* sys, double* out);
 mdc_status mdc_sync(mdc_sys* sys);
+
+/* ---- serving: continuous batching of requests (no reference counterpart;
+ * the reference runs one system per process) ------------------------------
+ * A request is one structure (atom records, box) for one registered
+ * potential, with a task: a single-point energy/force/virial evaluation
+ * (steps = 0) or `steps` velocity-Verlet steps followed by one.  Requests
+ * are admitted in arrival order into free slots of running batches (a
+ * batch is one system whose boxes are slots of one size class, 64 *
+ * 2^(k/2) records, for one potential and time step), so arrivals join
+ * batches that are already stepping; a request that finds no free slot
+ * opens a new batch of slots_per_batch slots while the estimated device
+ * bytes of all open batches stay within budget_bytes (idle batches are
+ * closed first; otherwise the queue waits).  Every batch runs on its own
+ * context (stream) and host thread, so batches overlap on the device.
+ * Each batch steps in chunks of at most chunk_steps, never past the
+ * smallest step count left among its slots; a request whose count is
+ * reached gets a final E/F/virial evaluation of the batch and its slot is
+ * freed.  submit, wait, take and stats may be called from any thread.  The
+ * potentials must outlive the server. */
+typedef struct mdc_server mdc_server;
+mdc_status mdc_serve_create(int device, double budget_bytes, int chunk_steps, int slots_per_batch, mdc_server** out);
+/* register a potential evaluated at `precision`; *kind is its index */
+mdc_status mdc_serve_add_potential(mdc_server* srv, const mdc_pot* pot, int precision, int* kind);
+/* copies the natoms records and the box (3 doubles each, 3 periodic
+ * flags); dt is unused for steps = 0; *id names the request */
+mdc_status mdc_serve_submit(mdc_server* srv, int kind, const void* atoms, uint64_t natoms, const double* box_lo,
+                            const double* box_hi, const uint8_t* periodic, double dt, int steps, uint64_t* id);
+/* waits up to timeout_s (0: no wait) for the request; *state = 1 done,
+ * -1 failed, 0 not yet */
+mdc_status mdc_serve_wait(mdc_server* srv, uint64_t id, double timeout_s, int* state);
+/* a finished request's records after its task (natoms records), its thermo
+ * row (Ep, Em, KE, Etot, W, com velocity[3]; may be NULL) and times in
+ * seconds since the server started (submitted, admitted, done; may be
+ * NULL); releases the request.  A failed request returns its error. */
+mdc_status mdc_serve_take(mdc_server* srv, uint64_t id, void* atoms_out, double* thermo, double* times);
+/* done, failed, queued, running, open batches, batches opened, open batch
+ * bytes, peak open batch bytes */
+mdc_status mdc_serve_stats(mdc_server* srv, double* out);
+/* finishes the running requests, then stops every batch */
+mdc_status mdc_serve_destroy(mdc_server* srv);
 
 /* ---- instrumentation ---------------------------------------------------- */
 /* BPNN descriptors G and dE/dG (kJ/mol per descriptor unit) of the last

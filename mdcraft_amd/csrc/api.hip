@@ -10,20 +10,9 @@
 #include <mutex>
 #include <numeric>
 #include <sstream>
-#include "engine.h"
-#include "../../include/mdc.h"
+#include "abi.h"
 
 using namespace mdc;
-
-struct mdc_ctx {
-    Ctx c;
-};
-struct mdc_pot {
-    Pot p;
-};
-struct mdc_sys {
-    Batch b;
-};
 
 constexpr size_t kNoAllocBytes = (size_t)64 << 20;
 static __device__ __align__(256) char scratch[kNoAllocBytes];
@@ -55,16 +44,6 @@ char* scratch_on(int device) {
     std::lock_guard<std::mutex> lk(g_scratch_mu);
     return device >= 0 && device < kMaxDevices ? g_scratch_ptr[device] : nullptr;
 }
-
-// 208-byte record (mdcraft/data/atom.h:21-38)
-struct AtomRec {
-    double r[3], v[3], f[3], V[9];
-    double m, rcut, rns, Ep, Em, nc;
-    uint16_t bc, kind;
-    uint32_t uid;
-    uint16_t tag, pad[3];
-};
-static_assert(sizeof(AtomRec) == 208, "atom record must be 208 bytes");
 
 template <class F>
 int guard(Ctx* c, F&& f) {
@@ -446,6 +425,24 @@ __global__ void k_scatter_in(const double* __restrict__ stage, const double4* __
         for (int c = 0; c < 3; ++c) fl[(size_t)c * n + p] = (float)stage[(size_t)(D_FX + c) * n + o];
 }
 
+// The records of one box, [first, first + count) of the caller's order and
+// all of the box's storage, from a staging area of stride count: every
+// field, the positions and their tag words, each record at storage slot
+// first + q as a fresh upload places it (boxes keep their storage range
+// between rebuilds, since cells are grouped by box).
+__global__ void k_scatter_range(const double* __restrict__ stage, const double4* __restrict__ spos,
+                                int* __restrict__ where, int* __restrict__ orig, double4* __restrict__ pos,
+                                double* __restrict__ d, float* __restrict__ fl, int first, int count, int n) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= count) return;
+    const int p = first + q;
+    where[p] = p;
+    orig[p] = p;
+    pos[p] = spos[q];
+    for (int f = 0; f < D_NFIELDS; ++f) d[(size_t)f * n + p] = stage[(size_t)f * count + q];
+    for (int c = 0; c < 3; ++c) fl[(size_t)c * n + p] = (float)stage[(size_t)(D_FX + c) * count + q];
+}
+
 // records [first, first + count) of the caller's order into a staging area
 __global__ void k_gather_out(double* __restrict__ stage, double4* __restrict__ spos, const int* __restrict__ where,
                              const double4* __restrict__ pos, const double* __restrict__ d, const float* __restrict__ fl,
@@ -467,7 +464,7 @@ __global__ void k_set_inactive(double4* __restrict__ pos, const int* __restrict_
     if (q >= count) return;
     const int p = where[first + q];
     double4 v = pos[p];
-    v.w = pack_w(w_kind(v.w), inactive, w_sys(v.w));
+    v.w = pack_w(w_kind(v.w), inactive, w_sys(v.w), w_void(v.w));
     pos[p] = v;
 }
 
@@ -522,7 +519,8 @@ void layout_cells(Batch& b) {
         }
         d.rns2 = rmax * rmax;
         d.cell_off = off;
-        off += d.nc[0] * d.nc[1] * d.nc[2];
+        d.void_cell = off + d.nc[0] * d.nc[1] * d.nc[2];
+        off += d.nc[0] * d.nc[1] * d.nc[2] + 1;
     }
     if (off != b.ncells) {
         dfree(b.cell_count);
@@ -652,11 +650,106 @@ void upload(Batch& b, const AtomRec* a, uint32_t fields) {
     if (fields & MDC_F_F) b.f_prec = P_FP64;
     if (fields & MDC_F_RCUT) b.last_rcut = -1;
     if (fields & MDC_F_R) b.shifts_stale = true;
+    // every record now holds an atom (the tag words were written afresh)
+    for (int s = 0; s < b.nsys; ++s) b.present[s] = b.atom_off[s + 1] - b.atom_off[s];
     if (rns_changed) {
         layout_cells(b);
         size_list(b);
         b.list_valid = false;
     }
+}
+
+// Box `box` takes `count` records of the caller (at most its record count)
+// in a new geometry; its remaining records become void (zero fields, tag
+// 0, binned into the box's void cell).  Positions, velocities, forces and
+// every other field come from the records.  The list is out of date until
+// the next build; the other boxes keep their state.
+void refill_box(Batch& b, int box, int count, const AtomRec* a, const double* lo, const double* hi,
+                const uint8_t* per) {
+    if (box < 0 || box >= b.nsys) throw Error(E_INVAL, "box index out of range");
+    const int first = b.atom_off[box], cap = b.atom_off[box + 1] - first;
+    if (count < 0 || count > cap) throw Error(E_INVAL, "more atoms than the box's records");
+    if (count > 0 && !a) throw Error(E_INVAL, "null atom records");
+    SysDesc d = b.desc[box];
+    d.any_per = 0;
+    for (int ax = 0; ax < 3; ++ax) {
+        d.lo[ax] = lo[ax];
+        d.hi[ax] = hi[ax];
+        d.L[ax] = hi[ax] - lo[ax];
+        d.per[ax] = per[ax] ? 1 : 0;
+        if (d.per[ax] && !(d.L[ax] > 0)) throw Error(E_INVAL, "periodic axis with non-positive length");
+        d.any_per |= (d.per[ax] && b.periodic_bc[box]) ? 1 : 0;
+    }
+    double mn = 1e300, mx = 0.0;
+    for (int q = 0; q < count; ++q) {
+        const AtomRec& r = a[q];
+        if (r.kind > 255) throw Error(E_INVAL, "atom kind above 255 is not supported");
+        if (!(r.rns >= 0) || !std::isfinite(r.rns)) throw Error(E_INVAL, "rns must be finite and >= 0");
+        mn = std::min(mn, r.rns);
+        mx = std::max(mx, r.rns);
+    }
+    if (count == 0) mn = mx = 0.0;
+    if (cap == 0) return;
+    std::vector<double> stage((size_t)D_NFIELDS * cap, 0.0);
+    std::vector<double4> spos(cap);
+    auto put = [&](int f, int q, double v) { stage[(size_t)f * cap + q] = v; };
+    for (int q = 0; q < cap; ++q) {
+        const int o = first + q;
+        if (q < count) {
+            const AtomRec& r = a[q];
+            b.kind[o] = r.kind;
+            b.tag[o] = r.tag;
+            b.uid[o] = r.uid;
+            b.rcut_host[o] = r.rcut;
+            spos[q] = make_double4(r.r[0], r.r[1], r.r[2], pack_w(r.kind, r.tag == 0 ? 1 : 0, box));
+            for (int c = 0; c < 3; ++c) {
+                put(D_VX + c, q, r.v[c]);
+                put(D_FX + c, q, r.f[c]);
+            }
+            put(D_VXX, q, r.V[0]);
+            put(D_VYY, q, r.V[4]);
+            put(D_VZZ, q, r.V[8]);
+            put(D_VXY, q, r.V[3]);
+            put(D_VXZ, q, r.V[6]);
+            put(D_VYZ, q, r.V[7]);
+            put(D_M, q, r.m);
+            put(D_RNS, q, r.rns);
+            put(D_EP, q, r.Ep);
+            put(D_EM, q, r.Em);
+            put(D_NC, q, r.nc);
+        } else {
+            b.kind[o] = 0;
+            b.tag[o] = 0;
+            b.uid[o] = 0;
+            b.rcut_host[o] = 0.0;
+            spos[q] = make_double4(d.lo[0], d.lo[1], d.lo[2], pack_w(0, 1, box, 1));
+        }
+    }
+    hipStream_t st = b.ctx->stream;
+    double* d_stage = dalloc<double>(stage.size());
+    double4* d_spos = dalloc<double4>(cap);
+    try {
+        HIPCHECK(hipMemcpyAsync(d_stage, stage.data(), stage.size() * sizeof(double), hipMemcpyHostToDevice, st));
+        HIPCHECK(hipMemcpyAsync(d_spos, spos.data(), spos.size() * sizeof(double4), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_scatter_range, dim3((cap + 255) / 256), dim3(256), 0, st, d_stage, d_spos, b.where,
+                           b.orig[b.sc], b.pos[b.cur], b.dst[b.sc], b.fst[b.sc], first, cap, b.n);
+        HIPCHECK(hipStreamSynchronize(st));
+    } catch (...) {
+        dfree(d_stage);
+        dfree(d_spos);
+        throw;
+    }
+    dfree(d_stage);
+    dfree(d_spos);
+    b.desc[box] = d;
+    b.rns_min[box] = mn;
+    b.rns_max[box] = mx;
+    b.present[box] = count;
+    layout_cells(b);
+    size_list(b);
+    b.list_valid = false;
+    b.shifts_stale = true;
+    b.last_rcut = -1;
 }
 
 // records [first, first + count) of the caller's order into a[0, count)
@@ -797,6 +890,14 @@ void check_kinds(const Batch& b, const Pot& p) {
 }
 
 }  // namespace
+
+namespace mdc {
+int report_error(Ctx* c, int code, const std::string& msg) {
+    g_last = msg;
+    if (c) c->err = msg;
+    return code;
+}
+}  // namespace mdc
 
 extern "C" {
 
@@ -1082,6 +1183,8 @@ mdc_status mdc_sys_create(mdc_ctx* ctx, int nsys, const uint64_t* natoms, const 
         }
         b.rns_min.assign(nsys, 0.0);
         b.rns_max.assign(nsys, 0.0);
+        b.present.resize(nsys);
+        for (int i = 0; i < nsys; ++i) b.present[i] = b.atom_off[i + 1] - b.atom_off[i];
         b.kind.assign(b.n, 0);
         b.tag.assign(b.n, 65535);
         b.uid.assign(b.n, 0);
@@ -1167,7 +1270,7 @@ mdc_status mdc_sys_set_active(mdc_sys* sys, int box, int active) {
         Batch& b = sys->b;
         if (box < 0 || box >= b.nsys) throw Error(E_INVAL, "box index out of range");
         HIPCHECK(hipSetDevice(b.ctx->device));
-        const int first = b.atom_off[box], count = b.atom_off[box + 1] - first;
+        const int first = b.atom_off[box], count = b.present[box];
         if (count == 0) return;
         // the skip flag travels with the positions; atoms whose own tag is 0
         // stay skipped
@@ -1177,6 +1280,16 @@ mdc_status mdc_sys_set_active(mdc_sys* sys, int box, int active) {
         hipLaunchKernelGGL(k_set_inactive, dim3((count + 255) / 256), dim3(256), 0, b.ctx->stream, b.pos[b.cur],
                            b.where, first, count, active ? 0 : 1);
         HIPCHECK(hipGetLastError());
+    });
+}
+
+mdc_status mdc_sys_refill_box(mdc_sys* sys, int box, uint64_t count, const void* atoms, const double* box_lo,
+                              const double* box_hi, const uint8_t* periodic) {
+    return guard(sys ? sys->b.ctx : nullptr, [&] {
+        if (!sys || !box_lo || !box_hi || !periodic) throw Error(E_INVAL, "null argument");
+        if (count > (uint64_t)kMaxAtomsShift) throw Error(E_INVAL, "more atoms than the box's records");
+        HIPCHECK(hipSetDevice(sys->b.ctx->device));
+        refill_box(sys->b, box, (int)count, static_cast<const AtomRec*>(atoms), box_lo, box_hi, periodic);
     });
 }
 

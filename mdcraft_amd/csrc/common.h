@@ -38,7 +38,9 @@ struct SysDesc {
     // of a listed pair is then unique and fixed between rebuilds, and the
     // image shift stored in the list entry gives the reference's image rule
     int shift_ok;
-    int pad_;
+    // the box's last cell (after its nc[0] nc[1] nc[2] grid): void records
+    // of the box are binned there and no list build visits it
+    int void_cell;
 };
 
 // A list entry holds the neighbour index in bits 0-25 and, per axis, the
@@ -123,13 +125,17 @@ __device__ inline void buf_f4x2_pair(__amdgpu_buffer_rsrc_t r, unsigned off, flo
     second = make_float4(b[0], b[1], b[2], b[3]);
 }
 
-// tag word packed into pos.w
-__host__ __device__ inline double pack_w(int kind, int inactive, int sys) {
-    return (double)((long long)kind | ((long long)inactive << 8) | ((long long)sys << 9));
+// tag word packed into pos.w: kind (bits 0-7), skipped (8), void (9: a
+// record of a box slot that holds no atom; it is binned into the box's void
+// cell, which no list build visits, and is skipped as a centre), system
+// (bits 10 and up)
+__host__ __device__ inline double pack_w(int kind, int inactive, int sys, int voided = 0) {
+    return (double)((long long)kind | ((long long)inactive << 8) | ((long long)voided << 9) | ((long long)sys << 10));
 }
 __device__ inline int w_kind(double w) { return (int)((long long)w & 0xff); }
 __device__ inline int w_inactive(double w) { return (int)(((long long)w >> 8) & 1); }
-__device__ inline int w_sys(double w) { return (int)((long long)w >> 9); }
+__device__ inline int w_void(double w) { return (int)(((long long)w >> 9) & 1); }
+__device__ inline int w_sys(double w) { return (int)((long long)w >> 10); }
 
 // The part of a system's descriptor that the pair loops read, copied into
 // registers once per atom so it is not reloaded per neighbour.

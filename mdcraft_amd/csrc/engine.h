@@ -149,6 +149,9 @@ struct Batch {
     std::vector<SysDesc> desc;
     SysDesc* d_desc = nullptr;
     std::vector<int> atom_off;  // [nsys+1]
+    // records of each box that hold atoms; the rest of the box's records
+    // are void (mdc_sys_refill_box)
+    std::vector<int> present;
     // state
     double4* pos[2] = {nullptr, nullptr};
     int cur = 0;

@@ -105,19 +105,23 @@ __global__ void __launch_bounds__(256) k_wrap_count(double4* __restrict__ pos, c
     if (valid) {
         double4 p = pos[i];
         const SysDesc& s = desc[w_sys(p.w)];
-        double r[3] = {p.x, p.y, p.z};
-        int c[3];
-        for (int a = 0; a < 3; ++a) {
-            if (s.per[a]) r[a] = r[a] < s.lo[a] ? r[a] + s.L[a] : r[a] > s.hi[a] ? r[a] - s.L[a] : r[a];
-            double q = floor((r[a] - s.lo[a]) / s.csize[a]);
-            int ci = q < 0.0 ? 0 : (q >= (double)s.nc[a] ? s.nc[a] - 1 : (int)q);
-            c[a] = ci;
+        if (w_void(p.w)) {
+            cid = s.void_cell;
+        } else {
+            double r[3] = {p.x, p.y, p.z};
+            int c[3];
+            for (int a = 0; a < 3; ++a) {
+                if (s.per[a]) r[a] = r[a] < s.lo[a] ? r[a] + s.L[a] : r[a] > s.hi[a] ? r[a] - s.L[a] : r[a];
+                double q = floor((r[a] - s.lo[a]) / s.csize[a]);
+                int ci = q < 0.0 ? 0 : (q >= (double)s.nc[a] ? s.nc[a] - 1 : (int)q);
+                c[a] = ci;
+            }
+            p.x = r[0];
+            p.y = r[1];
+            p.z = r[2];
+            pos[i] = p;
+            cid = s.cell_off + c[0] + s.nc[0] * (c[1] + s.nc[1] * c[2]);
         }
-        p.x = r[0];
-        p.y = r[1];
-        p.z = r[2];
-        pos[i] = p;
-        cid = s.cell_off + c[0] + s.nc[0] * (c[1] + s.nc[1] * c[2]);
         atom_cell[i] = cid;
     }
     // atoms come in the previous cell order: one add per run of equal cells
@@ -256,6 +260,11 @@ __global__ void __launch_bounds__(kBuildThreads) k_build(const double4* __restri
     if (a0 == a1) return;  // uniform over the workgroup
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const SysDesc& s = desc[w_sys(pos[a0].w)];
+    if (c == s.void_cell) {
+        // void records of empty slot space: no list (uniform over the workgroup)
+        for (int a = a0 + tid; a < a1; a += kBuildThreads) nn[a] = 0;
+        return;
+    }
     if (wv == 0) {
         const int lc = c - s.cell_off;
         const int nx = s.nc[0], ny = s.nc[1], nz = s.nc[2];

@@ -36,7 +36,9 @@ EXPORTS = [
     "mdc_sys_set_lanes", "mdc_sys_autotune", "mdc_sys_time_kernels", "mdc_sys_set_rebuild", "mdc_sys_set_boundary", "mdc_sys_set_thermostat",
     "mdc_nlist_build", "mdc_nlist_decide", "mdc_nlist_optimize_rns", "mdc_nlist_export", "mdc_nlist_stats",
     "mdc_compute", "mdc_compute_noalloc", "mdc_stage", "mdc_step", "mdc_thermo", "mdc_sync", "mdc_timing", "mdc_bpnn_inspect",
-    "mdc_copy_calibration",
+    "mdc_copy_calibration", "mdc_sys_refill_box",
+    "mdc_serve_create", "mdc_serve_add_potential", "mdc_serve_submit", "mdc_serve_wait", "mdc_serve_take",
+    "mdc_serve_stats", "mdc_serve_destroy",
 ]
 
 
@@ -95,6 +97,14 @@ def load_library(path=LIB_PATH):
         "mdc_sync": (I, [P]),
         "mdc_timing": (I, [P, I, DP, C.POINTER(C.c_int64)]),
         "mdc_bpnn_inspect": (I, [P, P, DP, DP, C.POINTER(I)]),
+        "mdc_sys_refill_box": (I, [P, I, U64, P, DP, DP, C.POINTER(C.c_uint8)]),
+        "mdc_serve_create": (I, [I, D, I, I, PP]),
+        "mdc_serve_add_potential": (I, [P, P, I, C.POINTER(I)]),
+        "mdc_serve_submit": (I, [P, I, P, U64, DP, DP, C.POINTER(C.c_uint8), D, I, C.POINTER(U64)]),
+        "mdc_serve_wait": (I, [P, U64, D, C.POINTER(I)]),
+        "mdc_serve_take": (I, [P, U64, P, DP, DP]),
+        "mdc_serve_stats": (I, [P, DP]),
+        "mdc_serve_destroy": (I, [P]),
     }
     for name in EXPORTS:
         fn = getattr(lib, name)
@@ -130,6 +140,7 @@ class Context:
         h = C.c_void_p()
         _check(lib().mdc_ctx_create(device, C.byref(h)))
         self.h = h
+        self.device = device
 
     def close(self):
         if getattr(self, "h", None):
@@ -331,6 +342,17 @@ class System:
 
     def set_active(self, box, active):
         _check(lib().mdc_sys_set_active(self.h, box, int(bool(active))))
+
+    def refill_box(self, box, atoms, box_geometry):
+        """Box `box` takes the records `atoms` (at most its record count) in
+        the geometry (lo, hi, periodic); its other records become void."""
+        lo, hi, per = box_geometry
+        lo = np.ascontiguousarray(lo, dtype=np.float64)
+        hi = np.ascontiguousarray(hi, dtype=np.float64)
+        pf = np.ascontiguousarray([1 if p else 0 for p in per], dtype=np.uint8)
+        a = np.ascontiguousarray(atoms, dtype=ATOM_DTYPE)
+        _check(lib().mdc_sys_refill_box(self.h, box, len(a), a.ctypes.data if len(a) else None, _dp(lo), _dp(hi),
+                                        pf.ctypes.data_as(C.POINTER(C.c_uint8))))
 
     def build(self):
         _check(lib().mdc_nlist_build(self.h))

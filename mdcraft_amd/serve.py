@@ -1,201 +1,130 @@
-"""Served mixed stream on one GPU: continuous batching of MD requests.
+"""Served mixed stream on one GPU: the engine's continuous-batching server
+(mdc_serve_* of include/mdc.h) from Python.
 
 A request is one structure (atoms, box) for one potential, with a task: a
 single-point energy/force/virial evaluation (steps = 0) or an NVE run of
 `steps` velocity-Verlet steps followed by an energy/force/virial
-evaluation.  The server
+evaluation.  The server (C++, in the engine) admits requests in arrival
+order into free slots of running batches (one system per batch, boxes as
+slots of one size class), opens batches within a device-memory budget,
+runs every batch on its own stream and host thread, and frees a request's
+slot when its count is reached; see the header for the full contract.
 
-* admits arrived requests in arrival order while the estimated device
-  memory of everything admitted stays within the HBM budget;
-* at each scheduling tick, puts the newly admitted requests of each
-  potential into one engine System (a batch of boxes: one launch per
-  kernel for all of them) and evaluates them;
-* advances every running batch by up to `chunk` steps, never past the
-  smallest step count left among its running boxes;
-* retires a request as soon as its count is reached: its box alone is
-  downloaded and then switched off on the device, so the rest of the batch
-  runs on without it; a batch with no running box is released.
-
-The scheduler is host code; all force and integration work is the
-engine's.  One thread drives the GPU: `submit` may be called from other
-threads, `tick` / `run_trace` from the driving thread only.
+This module submits traces of requests at their arrival times, collects the
+results, and runs the same traces through the plain FP64 one-at-a-time path
+for comparison.
 """
-import collections
-import threading
+import ctypes as C
 import time
 
 import numpy as np
 
-# device bytes per atom used for admission: state and list (about 1 KB for
-# LJ / EAM lists of up to ~100 neighbours), BPNN adds its per-neighbour
-# descriptor-gradient scratch
-BYTES_PER_ATOM = {"lj": 1536, "eam": 1280, "bpnn": 96 * 1024}
+from .atoms import ATOM_DTYPE
 
 
 class Request:
     __slots__ = ("rid", "kind", "atoms", "box", "dt", "steps", "arrival", "admitted", "done", "result", "thermo",
-                 "left", "batch", "box_index")
+                 "sid")
 
     def __init__(self, rid, kind, atoms, box, dt, steps, arrival):
         self.rid, self.kind, self.atoms, self.box, self.dt, self.steps = rid, kind, atoms, box, dt, steps
         self.arrival = arrival
         self.admitted = self.done = None
         self.result = self.thermo = None
-        self.left = steps
-        self.batch = None
-        self.box_index = -1
-
-    @property
-    def bytes(self):
-        return BYTES_PER_ATOM[self.kind] * len(self.atoms)
-
-
-class _Batch:
-    def __init__(self, system, pot, kind, members, precision):
-        self.system, self.pot, self.kind, self.members, self.precision = system, pot, kind, members, precision
-        self.running = [m for m in members]
+        self.sid = None
 
 
 class Server:
-    """pots: {kind: engine potential}; precision: {kind: MDC precision}."""
+    """pots: {kind: engine potential}; precision: {kind: MDC precision}.
+    budget_bytes bounds the estimated device bytes of the open batches;
+    chunk is the most steps a batch runs between scheduling points; slots
+    is the number of slots a new batch gets."""
 
-    def __init__(self, engine, ctx, pots, precision, budget_bytes=200e9, chunk=25, lanes=None,
-                 clock=time.perf_counter):
-        self.E, self.ctx, self.pots, self.precision = engine, ctx, pots, precision
-        self.budget = float(budget_bytes)
-        self.chunk = int(chunk)
-        self.lanes = lanes
-        self.clock = clock
-        self._lock = threading.Lock()
-        self._arrived = collections.deque()
-        self.waiting = collections.deque()  # arrived, not admitted
-        self.admitted_bytes = 0.0
-        self.batches = []
-        self.completed = []
-        self.peak_bytes = 0.0
-        self.ticks = 0
+    def __init__(self, engine, ctx, pots, precision, budget_bytes=200e9, chunk=25, slots=8):
+        self.E = engine
+        self._lib = engine.lib()
+        h = C.c_void_p()
+        engine._check(self._lib.mdc_serve_create(ctx.device, float(budget_bytes), int(chunk), int(slots), C.byref(h)))
+        self.h = h
+        self.pots = pots  # the potentials must outlive the server
+        self.kind_id = {}
+        for kind, pot in pots.items():
+            k = C.c_int()
+            engine._check(self._lib.mdc_serve_add_potential(self.h, pot.h, int(precision[kind]), C.byref(k)))
+            self.kind_id[kind] = k.value
 
-    # ---- producer side --------------------------------------------------
     def submit(self, req):
-        with self._lock:
-            self._arrived.append(req)
+        a = np.ascontiguousarray(req.atoms, dtype=ATOM_DTYPE)
+        lo = np.ascontiguousarray(req.box[0], dtype=np.float64)
+        hi = np.ascontiguousarray(req.box[1], dtype=np.float64)
+        per = np.ascontiguousarray([1 if p else 0 for p in req.box[2]], dtype=np.uint8)
+        sid = C.c_uint64()
+        self.E._check(self._lib.mdc_serve_submit(
+            self.h, self.kind_id[req.kind], a.ctypes.data, len(a), lo.ctypes.data_as(C.POINTER(C.c_double)),
+            hi.ctypes.data_as(C.POINTER(C.c_double)), per.ctypes.data_as(C.POINTER(C.c_uint8)), float(req.dt),
+            int(req.steps), C.byref(sid)))
+        req.sid = sid.value
+        return req.sid
 
-    # ---- scheduling -----------------------------------------------------
-    def _admit(self):
-        with self._lock:
-            while self._arrived:
-                self.waiting.append(self._arrived.popleft())
-        new = collections.defaultdict(list)
-        while self.waiting:
-            r = self.waiting[0]
-            if r.bytes > self.budget:
-                raise RuntimeError("request %d needs %.3g bytes, above the whole budget" % (r.rid, r.bytes))
-            if self.admitted_bytes + r.bytes > self.budget:
-                break  # arrival order: later requests wait behind it
-            self.waiting.popleft()
-            r.admitted = self.clock()
-            self.admitted_bytes += r.bytes
-            new[r.kind].append(r)
-        self.peak_bytes = max(self.peak_bytes, self.admitted_bytes)
-        return new
-
-    def _finish(self, b, reqs):
-        """Evaluate E/F/virial of the batch (the finishing boxes take theirs
-        from it), download each finishing box and switch it off."""
-        s = b.system
-        s.compute(b.pot, self.E.FORCE | self.E.ENERGY | self.E.VIRIAL, b.precision)
-        th = s.thermo()
-        for r in reqs:
-            r.result = s.download_box(r.box_index)
-            r.thermo = th[r.box_index].copy()
-            s.set_active(r.box_index, False)
-            r.done = self.clock()
-            r.batch = None
-            self.admitted_bytes -= r.bytes
-            self.completed.append(r)
-        b.running = [m for m in b.running if m.done is None]
-
-    def _start(self, kind, reqs):
-        E = self.E
-        s = E.System(self.ctx, [len(r.atoms) for r in reqs], [r.box for r in reqs], True, lanes=self.lanes)
-        s.upload(np.concatenate([r.atoms for r in reqs]))
-        s.build()
-        b = _Batch(s, self.pots[kind], kind, reqs, self.precision[kind])
-        for q, r in enumerate(reqs):
-            r.batch, r.box_index = b, q
-        singles = [r for r in reqs if r.left == 0]
-        if singles:
-            self._finish(b, singles)
-        else:
-            s.compute(b.pot, E.FORCE, b.precision)
-        if b.running:
-            self.batches.append(b)
-        else:
-            s.close()
-
-    def _advance(self, b):
-        k = min(self.chunk, min(r.left for r in b.running))
-        dts = {r.dt for r in b.running}
-        if len(dts) != 1:
-            raise RuntimeError("one batch steps with one time step")
-        b.system.step(b.pot, dts.pop(), k, b.precision)
-        for r in b.running:
-            r.left -= k
-        done = [r for r in b.running if r.left == 0]
-        if done:
-            self._finish(b, done)
-
-    def tick(self):
-        """One scheduling round; returns True while work remains."""
-        self.ticks += 1
-        for kind, reqs in self._admit().items():
-            # one batch per potential and time step
-            by_dt = collections.defaultdict(list)
-            for r in reqs:
-                by_dt[r.dt].append(r)
-            for group in by_dt.values():
-                self._start(kind, group)
-        for b in list(self.batches):
-            self._advance(b)
-            if not b.running:
-                b.system.close()
-                self.batches.remove(b)
-        with self._lock:
-            pending = bool(self._arrived)
-        return bool(self.batches or self.waiting or pending)
-
-    def run_trace(self, trace):
-        """trace: [(arrival offset s, Request)] sorted by offset.  Submits
-        each request when its arrival time has passed and ticks until all
-        are done.  Request.arrival is set to its nominal arrival time."""
-        t0 = self.clock()
-        q = 0
+    def wait(self, req, timeout=600.0):
+        """Blocks until the request is done (or the timeout passes), then
+        fills its result, thermo row and server-clock times."""
+        state = C.c_int()
+        t_end = time.perf_counter() + timeout
         while True:
-            now = self.clock()
-            while q < len(trace) and t0 + trace[q][0] <= now:
-                r = trace[q][1]
-                r.arrival = t0 + trace[q][0]
-                self.submit(r)
-                q += 1
-            busy = self.tick()
-            if q == len(trace) and not busy:
+            self.E._check(self._lib.mdc_serve_wait(self.h, req.sid, 1.0, C.byref(state)))
+            if state.value != 0:
                 break
-            if not busy and q < len(trace):
-                # idle until the next arrival
-                wait = t0 + trace[q][0] - self.clock()
-                if wait > 0:
-                    time.sleep(min(wait, 0.01))
-        self.ctx_sync()
-        return self.completed
+            if time.perf_counter() > t_end:
+                raise TimeoutError("request %d not done after %.0f s" % (req.rid, timeout))
+        out = np.zeros(len(req.atoms), dtype=ATOM_DTYPE)
+        th = np.zeros(8)
+        tm = np.zeros(3)
+        self.E._check(self._lib.mdc_serve_take(self.h, req.sid, out.ctypes.data,
+                                               th.ctypes.data_as(C.POINTER(C.c_double)),
+                                               tm.ctypes.data_as(C.POINTER(C.c_double))))
+        req.result, req.thermo = out, th
+        req.arrival, req.admitted, req.done = tm
+        return req
 
-    def ctx_sync(self):
-        for b in self.batches:
-            b.system.sync()
+    def stats(self):
+        """done, failed, queued, running, open batches, batches opened, open
+        batch bytes, peak open batch bytes."""
+        out = np.zeros(8)
+        self.E._check(self._lib.mdc_serve_stats(self.h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        keys = ("done", "failed", "queued", "running", "open_batches", "batches_opened", "open_bytes", "peak_bytes")
+        return dict(zip(keys, out.tolist()))
+
+    @property
+    def peak_bytes(self):
+        return self.stats()["peak_bytes"]
+
+    def run_trace(self, trace, timeout=600.0):
+        """trace: [(arrival offset s, Request)] sorted by offset.  Submits
+        each request at its arrival time, then collects every result.  The
+        requests' arrival, admitted and done times are on the server's
+        clock (seconds since it started)."""
+        t0 = time.perf_counter()
+        for off, r in trace:
+            wait = t0 + off - time.perf_counter()
+            if wait > 0:
+                time.sleep(wait)
+            self.submit(r)
+        return [self.wait(r, timeout) for _, r in trace]
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.mdc_serve_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
 
 
-def one_at_a_time(engine, ctx, pots, trace, clock=time.perf_counter):
-    """The plain FP64 path on the same trace: each request alone in its own
-    System, in arrival order, evaluated or stepped in FP64."""
+def one_at_a_time(engine, ctx, pots, trace, clock=time.perf_counter, precision=None):
+    """The plain one-at-a-time path on the same trace: each request alone in
+    its own System, in arrival order, evaluated or stepped in FP64 (or in
+    precision[kind] when given)."""
     E = engine
     t0 = clock()
     out = []
@@ -205,14 +134,15 @@ def one_at_a_time(engine, ctx, pots, trace, clock=time.perf_counter):
         if now < arrival:
             time.sleep(arrival - now)
         r.arrival = arrival
+        prec = E.FP64 if precision is None else precision[r.kind]
         s = E.System(ctx, [len(r.atoms)], [r.box], True)
         s.upload(r.atoms)
         s.build()
         p = pots[r.kind]
         if r.steps:
-            s.compute(p, E.FORCE, E.FP64)
-            s.step(p, r.dt, r.steps, E.FP64)
-        s.compute(p, E.FORCE | E.ENERGY | E.VIRIAL, E.FP64)
+            s.compute(p, E.FORCE, prec)
+            s.step(p, r.dt, r.steps, prec)
+        s.compute(p, E.FORCE | E.ENERGY | E.VIRIAL, prec)
         r.thermo = s.thermo()[0]
         r.result = s.download()
         r.done = clock()

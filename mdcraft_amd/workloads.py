@@ -558,11 +558,12 @@ SERVED_KINDS = ("lj", "eam", "bpnn")
 
 def served_structure(kind, natoms, seed, vacuum=False, model_path=None):
     """An fcc structure of about natoms atoms built as the LJ argon, EAM Cu
-    or BPNN Cu workloads build theirs; vacuum adds a 50 A gap along z.
+    or BPNN Cu workloads build theirs, rattled so that a single-point
+    evaluation has nonzero forces; vacuum adds a 50 A gap along z.
     BPNN boxes keep at least 4 cells per axis (the descriptor path needs
     box lengths of at least twice its 6.3 A search radius)."""
     rng = Rng(seed)
-    spec = {"lj": (5.26, 39.948, 100.0, 0.0, 1), "eam": (3.615, 63.546, 300.0, 0.03, 2),
+    spec = {"lj": (5.26, 39.948, 100.0, 0.05, 1), "eam": (3.615, 63.546, 300.0, 0.03, 2),
             "bpnn": (3.615, 63.546, 300.0, 0.05, 3)}[kind]
     a_ang, mass, T, rattle, wid = spec
     a = a_ang * ANG

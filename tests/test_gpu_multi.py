@@ -206,3 +206,23 @@ def test_multi_with_tabulated_pair_member(ctx, precision, tmp_path):
         s.build()
         s.compute(p, 7, precision)
         compare(s.download(), ref, precision, f"pair-member cell{idx}")
+
+
+def test_all_512_cells_of_the_batched_workload(ctx):
+    """The batched NiAl workload as benchmarked (512 cells, 32 to 4096
+    atoms, one Multi potential, one launch per pass for all cells), every
+    cell against its own oracle evaluation: FP64 within 1e-12 and mixed
+    precision force RMS within 1e-5 per cell."""
+    nb = W.nial_batch(512)
+    cs, pot = nb["cells"], nb["pot"]
+    assert len(cs) == 512
+    print("512 cells: engine", flush=True)
+    got = {prec: run_batch(ctx, cs, pot, prec)[0] for prec in (0, 1)}
+    print("512 cells: oracle", flush=True)
+    off = 0
+    for q, (at, box) in enumerate(cs):
+        ref, _ = oracle_compute(at, box, pot, threads=4)
+        for prec in (0, 1):
+            compare(got[prec][off:off + len(at)], ref, prec, f"cell{q}/{len(at)}")
+        off += len(at)
+    assert off == sum(len(a) for a, _ in cs)

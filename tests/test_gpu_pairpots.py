@@ -113,3 +113,54 @@ def test_checkpoint_and_resume(ctx, tmp_path):
     d -= np.round(d / L) * L
     assert np.max(np.abs(d)) < 1e-9
     assert max_rel(vec(got, "v"), vec(want, "v")) < 1e-8
+
+
+def test_snapshot_resume_matches_oracle_run(ctx, tmp_path):
+    """The checkpoint layout of the reference's scripts
+    (test/python/solver/test-EAM.py:193-197: the atom records and the domain
+    in one file) written after 15 engine steps holds the oracle's state
+    after 15 steps, and a new System resumed from it reaches the oracle's
+    uninterrupted 30-step state (FP64: positions within 1e-9 nm,
+    velocities within 1e-8 relative)."""
+    from mdcraft_amd import snapshot
+    from oracle import refcpu as R
+    from tests.helpers import oracle_potential
+    eng = E()
+    w = W.eam_cu(reps=5, rattle=0.05)
+    L = w["box"][1] - w["box"][0]
+    p = engine_potential(eng, ctx, w["pot"])
+    s0 = eng.System(ctx, [len(w["atoms"])], [w["box"]])
+    s0.upload(w["atoms"])
+    s0.build()
+    s0.compute(p, 1, 0)
+    s0.step(p, w["dt"], 15, 0)
+    path = str(tmp_path / "run000015.h5")
+    snapshot.save_system(path, s0, [w["box"]], cycles=np.array([15]))
+    s0.close()
+    # the oracle's make_step loop from the same start
+    a = w["atoms"].copy()
+    dom = R.Domain(*w["box"])
+    pool = R.Pool(8)
+    rp = oracle_potential(R, w["pot"])
+    a["rcut"] = rp.rcut
+    nl = R.VerletList(pool)
+    nl.update(a, dom)
+    sol = R.Solver(rp, pool, True)
+    sol.run_nve(a, dom, nl, w["dt"], 15, initial_forces=True)
+    with snapshot.File(path, "r") as f:
+        saved = f["atoms"]
+    d = vec(saved, "r") - vec(a, "r")
+    d -= np.round(d / L) * L
+    assert np.max(np.abs(d)) < 1e-9
+    assert max_rel(vec(saved, "v"), vec(a, "v")) < 1e-8
+    sol.run_nve(a, dom, nl, w["dt"], 15, initial_forces=False)
+    s, atoms, boxes, extra = snapshot.load_system(path, eng, ctx)
+    assert int(extra["cycles"][0]) == 15
+    s.build()
+    s.compute(p, 1, 0)
+    s.step(p, w["dt"], 15, 0)
+    got = s.download()
+    d = vec(got, "r") - vec(a, "r")
+    d -= np.round(d / L) * L
+    assert np.max(np.abs(d)) < 1e-9
+    assert max_rel(vec(got, "v"), vec(a, "v")) < 1e-8
