@@ -81,6 +81,17 @@ void dfree(T*& p) {
     p = nullptr;
 }
 
+// The batch's device staging area of at least `bytes` (kept between
+// transfers; every transfer waits for its copies before it returns).
+char* stage_area(Batch& b, size_t bytes) {
+    if (b.stage_bytes < bytes) {
+        dfree(b.stage);
+        b.stage = dalloc<char>(bytes);
+        b.stage_bytes = bytes;
+    }
+    return b.stage;
+}
+
 // Cubic spline slopes of the reference's tables (EAM.cxx:603-686): sweep
 // solve of the tridiagonal system with end condition ib (1: z[N-1] = 0,
 // 2: second-derivative end), including the reference's back-substitution
@@ -425,22 +436,25 @@ __global__ void k_scatter_in(const double* __restrict__ stage, const double4* __
         for (int c = 0; c < 3; ++c) fl[(size_t)c * n + p] = (float)stage[(size_t)(D_FX + c) * n + o];
 }
 
-// The records of one box, [first, first + count) of the caller's order and
-// all of the box's storage, from a staging area of stride count: every
-// field, the positions and their tag words, each record at storage slot
-// first + q as a fresh upload places it (boxes keep their storage range
-// between rebuilds, since cells are grouped by box).
+// The records of one box, [first, first + cap) of the caller's order, each
+// at its sorted place where[first + q]: the first `count` from a staging
+// area of stride count (every field, the position and its tag word), the
+// rest void (position void_pos with its tag word, every field zero).
 __global__ void k_scatter_range(const double* __restrict__ stage, const double4* __restrict__ spos,
-                                int* __restrict__ where, int* __restrict__ orig, double4* __restrict__ pos,
-                                double* __restrict__ d, float* __restrict__ fl, int first, int count, int n) {
+                                const int* __restrict__ where, double4* __restrict__ pos, double* __restrict__ d,
+                                float* __restrict__ fl, int first, int count, int cap, int n, double4 void_pos) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= count) return;
-    const int p = first + q;
-    where[p] = p;
-    orig[p] = p;
-    pos[p] = spos[q];
-    for (int f = 0; f < D_NFIELDS; ++f) d[(size_t)f * n + p] = stage[(size_t)f * count + q];
-    for (int c = 0; c < 3; ++c) fl[(size_t)c * n + p] = (float)stage[(size_t)(D_FX + c) * count + q];
+    if (q >= cap) return;
+    const int p = where[first + q];
+    if (q < count) {
+        pos[p] = spos[q];
+        for (int f = 0; f < D_NFIELDS; ++f) d[(size_t)f * n + p] = stage[(size_t)f * count + q];
+        for (int c = 0; c < 3; ++c) fl[(size_t)c * n + p] = (float)stage[(size_t)(D_FX + c) * count + q];
+    } else {
+        pos[p] = void_pos;
+        for (int f = 0; f < D_NFIELDS; ++f) d[(size_t)f * n + p] = 0.0;
+        for (int c = 0; c < F_NFIELDS; ++c) fl[(size_t)c * n + p] = 0.0f;
+    }
 }
 
 // records [first, first + count) of the caller's order into a staging area
@@ -491,6 +505,8 @@ void free_batch(Batch& b) {
     dfree(b.fpm);
     dfree(b.d_desc);
     dfree(b.thermo);
+    dfree(b.stage);
+    b.stage_bytes = 0;
     bpnn_release_work(b);
     for (auto& t : b.ev_pending) {
         b.ev_free.push_back(t.a);
@@ -502,7 +518,9 @@ void free_batch(Batch& b) {
 }
 
 // Cell grid of each system: cells of edge >= max rns over the box (capped
-// per axis), cell offsets into the global cell arrays.
+// per axis), cell offsets into the global cell arrays, and one void cell
+// after every box's grid for the void records of all boxes (which the sort
+// therefore puts after every live record).
 void layout_cells(Batch& b) {
     int off = 0;
     for (int s = 0; s < b.nsys; ++s) {
@@ -519,36 +537,48 @@ void layout_cells(Batch& b) {
         }
         d.rns2 = rmax * rmax;
         d.cell_off = off;
-        d.void_cell = off + d.nc[0] * d.nc[1] * d.nc[2];
-        off += d.nc[0] * d.nc[1] * d.nc[2] + 1;
+        off += d.nc[0] * d.nc[1] * d.nc[2];
     }
-    if (off != b.ncells) {
+    for (int s = 0; s < b.nsys; ++s) b.desc[s].void_cell = off;
+    off += 1;
+    if (off > b.cell_alloc) {
+        // grown only: boxes refilled with new geometries change the count
         dfree(b.cell_count);
         dfree(b.cell_start);
         dfree(b.cell_fill);
+        const int alloc = std::max(off, b.cell_alloc + b.cell_alloc / 2);
+        b.cell_count = dalloc<int>(alloc + 1);
+        b.cell_start = dalloc<int>(alloc + 1);
+        b.cell_fill = dalloc<int>(std::max(alloc + 1, b.n));
+        b.cell_alloc = alloc;
+        b.ncells = -1;
+    }
+    if (off != b.ncells) {
+        // each rebuild's scan clears the counts it consumed: zero them when
+        // the grid changes
+        HIPCHECK(hipMemsetAsync(b.cell_count, 0, sizeof(int) * (b.cell_alloc + 1), b.ctx->stream));
         b.ncells = off;
-        b.cell_count = dalloc<int>(off + 1);
-        // zero once: each rebuild's scan clears the counts it consumed
-        HIPCHECK(hipMemsetAsync(b.cell_count, 0, sizeof(int) * (off + 1), b.ctx->stream));
-        b.cell_start = dalloc<int>(off + 1);
-        b.cell_fill = dalloc<int>(std::max(off + 1, b.n));
     }
     HIPCHECK(hipMemcpyAsync(b.d_desc, b.desc.data(), sizeof(SysDesc) * b.nsys, hipMemcpyHostToDevice, b.ctx->stream));
 }
 
-void size_list(Batch& b) {
-    // initial capacity from the number density: 1.3 x (4/3) pi rns^3 n + 16
+// grow_only: keep at least the current capacity (refills of a running
+// batch).
+void size_list(Batch& b, bool grow_only = false) {
+    // initial capacity from the number density of the atoms each box
+    // holds: 1.3 x (4/3) pi rns^3 n + 16
     double est = 16;
     for (int s = 0; s < b.nsys; ++s) {
         const SysDesc& d = b.desc[s];
         double vol = 1.0;
         for (int a = 0; a < 3; ++a) vol *= d.L[a] > 0 ? d.L[a] : 1.0;
-        double nat = b.atom_off[s + 1] - b.atom_off[s];
+        double nat = b.present[s];
         double r = b.rns_max[s];
         est = std::max(est, 1.3 * nat / vol * 4.18879 * r * r * r + 16);
     }
     est = std::min(est, 4096.0);
     int cap = (((int)est + b.tpa - 1) / b.tpa) * b.tpa;
+    if (grow_only && b.nl && b.cap % b.tpa == 0) cap = std::max(cap, b.cap);
     const int A = kWave / b.tpa;
     size_t waves = (size_t)(b.n + A - 1) / A;
     size_t elems = waves * (size_t)(cap / b.tpa) * kWave;
@@ -631,26 +661,26 @@ void upload(Batch& b, const AtomRec* a, uint32_t fields) {
             b.rns_max[s] = mx;
         }
     }
-    double* d_stage = dalloc<double>((size_t)D_NFIELDS * n);
-    double4* d_spos = dalloc<double4>(n);
-    try {
-        HIPCHECK(hipMemcpyAsync(d_stage, stage.data(), stage.size() * sizeof(double), hipMemcpyHostToDevice, st));
-        HIPCHECK(hipMemcpyAsync(d_spos, spos.data(), spos.size() * sizeof(double4), hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_scatter_in, dim3((n + 255) / 256), dim3(256), 0, st, d_stage, d_spos, b.where, b.pos[b.cur],
-                           b.dst[b.sc], b.fst[b.sc], dmask, (fields & MDC_F_R) ? 1 : 0,
-                           (fields & MDC_F_F) ? 1 : 0, n);
-        HIPCHECK(hipStreamSynchronize(st));
-    } catch (...) {
-        dfree(d_stage);
-        dfree(d_spos);
-        throw;
-    }
-    dfree(d_stage);
-    dfree(d_spos);
+    const size_t sbytes = stage.size() * sizeof(double);
+    char* area = stage_area(b, sbytes + spos.size() * sizeof(double4));
+    double* d_stage = reinterpret_cast<double*>(area);
+    double4* d_spos = reinterpret_cast<double4*>(area + sbytes);
+    HIPCHECK(hipMemcpyAsync(d_stage, stage.data(), sbytes, hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemcpyAsync(d_spos, spos.data(), spos.size() * sizeof(double4), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_scatter_in, dim3((n + 255) / 256), dim3(256), 0, st, d_stage, d_spos, b.where, b.pos[b.cur],
+                       b.dst[b.sc], b.fst[b.sc], dmask, (fields & MDC_F_R) ? 1 : 0, (fields & MDC_F_F) ? 1 : 0, n);
+    HIPCHECK(hipStreamSynchronize(st));
     if (fields & MDC_F_F) b.f_prec = P_FP64;
     if (fields & MDC_F_RCUT) b.last_rcut = -1;
     if (fields & MDC_F_R) b.shifts_stale = true;
-    // every record now holds an atom (the tag words were written afresh)
+    // every record now holds an atom (the tag words were written afresh):
+    // records of the void tail became live, so the next build sorts all
+    long long held = 0;
+    for (int s = 0; s < b.nsys; ++s) held += b.present[s];
+    if (held < n) {
+        b.full_sort = true;
+        b.list_valid = false;
+    }
     for (int s = 0; s < b.nsys; ++s) b.present[s] = b.atom_off[s + 1] - b.atom_off[s];
     if (rns_changed) {
         layout_cells(b);
@@ -690,9 +720,11 @@ void refill_box(Batch& b, int box, int count, const AtomRec* a, const double* lo
     }
     if (count == 0) mn = mx = 0.0;
     if (cap == 0) return;
-    std::vector<double> stage((size_t)D_NFIELDS * cap, 0.0);
-    std::vector<double4> spos(cap);
-    auto put = [&](int f, int q, double v) { stage[(size_t)f * cap + q] = v; };
+    // the atoms staged with stride count; the void records are written on
+    // the device
+    std::vector<double> stage((size_t)D_NFIELDS * count, 0.0);
+    std::vector<double4> spos(count);
+    auto put = [&](int f, int q, double v) { stage[(size_t)f * count + q] = v; };
     for (int q = 0; q < cap; ++q) {
         const int o = first + q;
         if (q < count) {
@@ -722,32 +754,31 @@ void refill_box(Batch& b, int box, int count, const AtomRec* a, const double* lo
             b.tag[o] = 0;
             b.uid[o] = 0;
             b.rcut_host[o] = 0.0;
-            spos[q] = make_double4(d.lo[0], d.lo[1], d.lo[2], pack_w(0, 1, box, 1));
         }
     }
     hipStream_t st = b.ctx->stream;
-    double* d_stage = dalloc<double>(stage.size());
-    double4* d_spos = dalloc<double4>(cap);
-    try {
-        HIPCHECK(hipMemcpyAsync(d_stage, stage.data(), stage.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    const size_t sbytes = stage.size() * sizeof(double);
+    char* area = stage_area(b, sbytes + spos.size() * sizeof(double4));
+    double* d_stage = reinterpret_cast<double*>(area);
+    double4* d_spos = reinterpret_cast<double4*>(area + sbytes);
+    if (count > 0) {
+        HIPCHECK(hipMemcpyAsync(d_stage, stage.data(), sbytes, hipMemcpyHostToDevice, st));
         HIPCHECK(hipMemcpyAsync(d_spos, spos.data(), spos.size() * sizeof(double4), hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_scatter_range, dim3((cap + 255) / 256), dim3(256), 0, st, d_stage, d_spos, b.where,
-                           b.orig[b.sc], b.pos[b.cur], b.dst[b.sc], b.fst[b.sc], first, cap, b.n);
-        HIPCHECK(hipStreamSynchronize(st));
-    } catch (...) {
-        dfree(d_stage);
-        dfree(d_spos);
-        throw;
     }
-    dfree(d_stage);
-    dfree(d_spos);
+    const double4 void_pos = make_double4(d.lo[0], d.lo[1], d.lo[2], pack_w(0, 1, box, 1));
+    hipLaunchKernelGGL(k_scatter_range, dim3((cap + 255) / 256), dim3(256), 0, st, d_stage, d_spos, b.where,
+                       b.pos[b.cur], b.dst[b.sc], b.fst[b.sc], first, count, cap, b.n, void_pos);
+    HIPCHECK(hipStreamSynchronize(st));
     b.desc[box] = d;
     b.rns_min[box] = mn;
     b.rns_max[box] = mx;
     b.present[box] = count;
     layout_cells(b);
-    size_list(b);
+    // the list capacity only grows here: refills come and go in a running
+    // batch, and a smaller list would be regrown by the next denser box
+    size_list(b, true);
     b.list_valid = false;
+    b.full_sort = true;
     b.shifts_stale = true;
     b.last_rcut = -1;
 }
@@ -757,23 +788,17 @@ void download(Batch& b, AtomRec* a, uint32_t fields, int first, int count) {
     const int n = b.n;
     if (count <= 0) return;
     hipStream_t st = b.ctx->stream;
-    double* d_stage = dalloc<double>((size_t)D_NFIELDS * count);
-    double4* d_spos = dalloc<double4>(count);
     std::vector<double> stage((size_t)D_NFIELDS * count);
     std::vector<double4> spos(count);
-    try {
-        hipLaunchKernelGGL(k_gather_out, dim3((count + 255) / 256), dim3(256), 0, st, d_stage, d_spos, b.where,
-                           b.pos[b.cur], b.dst[b.sc], b.fst[b.sc], b.f_prec == P_MIXED ? 1 : 0, n, first, count);
-        HIPCHECK(hipMemcpyAsync(stage.data(), d_stage, stage.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipMemcpyAsync(spos.data(), d_spos, spos.size() * sizeof(double4), hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipStreamSynchronize(st));
-    } catch (...) {
-        dfree(d_stage);
-        dfree(d_spos);
-        throw;
-    }
-    dfree(d_stage);
-    dfree(d_spos);
+    const size_t sbytes = stage.size() * sizeof(double);
+    char* area = stage_area(b, sbytes + spos.size() * sizeof(double4));
+    double* d_stage = reinterpret_cast<double*>(area);
+    double4* d_spos = reinterpret_cast<double4*>(area + sbytes);
+    hipLaunchKernelGGL(k_gather_out, dim3((count + 255) / 256), dim3(256), 0, st, d_stage, d_spos, b.where,
+                       b.pos[b.cur], b.dst[b.sc], b.fst[b.sc], b.f_prec == P_MIXED ? 1 : 0, n, first, count);
+    HIPCHECK(hipMemcpyAsync(stage.data(), d_stage, sbytes, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(spos.data(), d_spos, spos.size() * sizeof(double4), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
     auto get = [&](int f, int o) { return stage[(size_t)f * count + (o - first)]; };
     for (int s = 0; s < b.nsys; ++s) {
         const SysDesc& d = b.desc[s];
@@ -851,8 +876,10 @@ void optimize_rns(Batch& b, double kbuf, double* rns_out) {
         double* d_rc = dalloc<double>(n);
         try {
             HIPCHECK(hipMemcpyAsync(d_rc, rc.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
-            hipLaunchKernelGGL(k_opt_rns, dim3((n + 255) / 256), dim3(256), 0, st, b.nl, b.nn, d_rc, b.d(b.sc, D_RNS),
-                               b.tpa, b.M, b.cap, n, kbuf);
+            // the live records (the void tail keeps its zero radii)
+            if (b.live > 0)
+                hipLaunchKernelGGL(k_opt_rns, dim3((b.live + 255) / 256), dim3(256), 0, st, b.nl, b.nn, d_rc,
+                                   b.d(b.sc, D_RNS), b.tpa, b.M, b.cap, b.live, kbuf);
             HIPCHECK(hipMemcpyAsync(rc.data(), b.d(b.sc, D_RNS), sizeof(double) * n, hipMemcpyDeviceToHost, st));
             HIPCHECK(hipStreamSynchronize(st));
         } catch (...) {
@@ -866,12 +893,13 @@ void optimize_rns(Batch& b, double kbuf, double* rns_out) {
         HIPCHECK(hipStreamSynchronize(st));
     }
     for (int s = 0; s < b.nsys; ++s) {
+        // over the atoms the box holds
         double mn = 1e300, mx = 0;
-        for (int o = b.atom_off[s]; o < b.atom_off[s + 1]; ++o) {
+        for (int o = b.atom_off[s]; o < b.atom_off[s] + b.present[s]; ++o) {
             mn = std::min(mn, rns[o]);
             mx = std::max(mx, rns[o]);
         }
-        if (b.atom_off[s + 1] == b.atom_off[s]) mn = mx = 0;
+        if (b.present[s] == 0) mn = mx = 0;
         b.rns_min[s] = mn;
         b.rns_max[s] = mx;
     }
@@ -1185,6 +1213,8 @@ mdc_status mdc_sys_create(mdc_ctx* ctx, int nsys, const uint64_t* natoms, const 
         b.rns_max.assign(nsys, 0.0);
         b.present.resize(nsys);
         for (int i = 0; i < nsys; ++i) b.present[i] = b.atom_off[i + 1] - b.atom_off[i];
+        b.live = b.n;
+        b.full_sort = true;
         b.kind.assign(b.n, 0);
         b.tag.assign(b.n, 65535);
         b.uid.assign(b.n, 0);
@@ -1466,7 +1496,8 @@ mdc_status mdc_nlist_stats(mdc_sys* sys, double* mean, int* maxn, int* capacity,
             s += x;
             mx = std::max(mx, x);
         }
-        if (mean) *mean = b.n ? s / b.n : 0.0;
+        // per live record (void records have no list)
+        if (mean) *mean = b.live ? s / b.live : 0.0;
         if (maxn) *maxn = mx;
         if (capacity) *capacity = b.cap;
         if (rebuilds) *rebuilds = b.rebuilds;

@@ -865,7 +865,7 @@ __global__ void __launch_bounds__(256) k_bp_gather(const double4* __restrict__ p
     // (coalesced list and reverse-index reads), then sum their shares
     const AtomSlots as = atom_slots<TPA>(M);
     const int m = as.i;
-    const bool valid = m < A.n;
+    const bool valid = m < A.live;
     const double4 pm = valid ? pos[m] : make_double4(0, 0, 0, 0);
     const bool active = valid && !w_inactive(pm.w);
     const int cm = active ? min(nn[m], cap) : 0;
@@ -1262,7 +1262,9 @@ static void ensure_work(Batch& b, const BpnnParams& bp, int prec, bool vir) {
     BpWork& w = work(b);
     const size_t rs = prec == P_FP64 ? 8 : 4;
     const int ldg = bp.nin_pad;
-    const bool fits = w.a.G && w.cap == b.cap && w.n == b.n && w.prec == prec && w.rev_elems >= b.nl_elems;
+    // arrays sized for a larger list capacity serve a smaller one (the
+    // kernels index them with the current capacity)
+    const bool fits = w.a.G && w.cap >= b.cap && w.n == b.n && w.prec == prec && w.rev_elems >= b.nl_elems;
     if (!fits) {
         free_work(w);
         w.a.G = dnew<char>((size_t)b.n * ldg * rs);
@@ -1277,7 +1279,7 @@ static void ensure_work(Batch& b, const BpnnParams& bp, int prec, bool vir) {
         w.rev_valid = false;
     }
     if (vir && !w.has_vir) {
-        w.a.PV = dnew<char>((size_t)b.n * b.cap * 3 * rs);
+        w.a.PV = dnew<char>((size_t)b.n * w.cap * 3 * rs);
         w.a.OV = dnew<char>((size_t)b.n * 3 * rs);
         w.has_vir = true;
     }
@@ -1299,7 +1301,9 @@ static void bpnn_run(Batch& b, const Pot& p, const BpBufs& W, bool rev_fresh, in
                      double dt, int step, const double4* pos, double4* pos_next, int parts) {
     const BpnnParams& bp = *p.bp;
     hipStream_t st = b.ctx->stream;
-    const int n = b.n;
+    // centres: the live records (the void tail has no list and no atoms)
+    const int n = b.live;
+    if (n == 0) return;
     const bool vir = (parts & 4) || (what & W_VIRIAL);
     const int vmode = (parts & 4) ? 2 : (vir ? 1 : 0);
     StepArgs S{dt, step, mode, what & (W_ENERGY | W_VIRIAL), mode == 0 ? 1 : 0};
@@ -1309,7 +1313,7 @@ static void bpnn_run(Batch& b, const Pot& p, const BpBufs& W, bool rev_fresh, in
                            b.M, b.cap, n);
     }
     const int ldg = bp.nin_pad;
-    AtomArrays AA{b.dst[b.sc], b.fst[b.sc], n};
+    AtomArrays AA = arrays(b);
     double* Ep = b.d(b.sc, D_EP);
     const int4 boff = make_int4(bp.boff[0], bp.boff[1], bp.boff[2], bp.boff[3]);
     const int4 nts = make_int4(bp.nt[0], bp.nt[1], bp.nt[2], bp.nt[3]);

@@ -152,6 +152,17 @@ struct Batch {
     // records of each box that hold atoms; the rest of the box's records
     // are void (mdc_sys_refill_box)
     std::vector<int> present;
+    // Sorted order: every box's live records in cell order, box after box,
+    // in [0, live), and the void records of all boxes after them (one void
+    // cell past every box's grid).  Per-atom kernels run over [0, live);
+    // n stays the stride of the per-field arrays.  Both position buffers
+    // and both copies of the field arrays hold the same void tail, so the
+    // stepping kernels never touch it.  full_sort: records became void or
+    // live since the last sort (refill, upload over voids), so the next
+    // rebuild sorts all n records and copies the new tail; otherwise it
+    // sorts [0, live) only.
+    int live = 0;
+    bool full_sort = true;
     // state
     double4* pos[2] = {nullptr, nullptr};
     int cur = 0;
@@ -166,6 +177,7 @@ struct Batch {
     int* where = nullptr;  // original index -> sorted index
     // cells
     int ncells = 0;
+    int cell_alloc = 0;  // cells the cell arrays hold (grown, never shrunk)
     int* cell_count = nullptr;
     int* cell_start = nullptr;
     int* cell_fill = nullptr;
@@ -216,6 +228,10 @@ struct Batch {
     char* bp_fixed = nullptr;
     // reduction output
     double* thermo = nullptr;
+    // device staging of record transfers (upload, refill, download), grown
+    // as needed and kept, so transfers allocate and free nothing
+    char* stage = nullptr;
+    size_t stage_bytes = 0;
     // per-system extent of rns (host), for the displacement trigger
     std::vector<double> rns_min, rns_max;
     // optional kernel timing: hipEvents recorded on the launch stream around

@@ -54,7 +54,7 @@ __global__ void __launch_bounds__(256) k_lj(const double4* __restrict__ pos, dou
                                             const double4* __restrict__ rsave, int* __restrict__ misc) {
     const AtomSlots as = atom_slots<TPA>(M);
     const int i = as.i;
-    const bool valid = i < A.n;
+    const bool valid = i < A.live;
     const int ab = abort_flag(S, misc);
     const double4 pi = valid ? pos[i] : make_double4(0, 0, 0, 0);
     const int nraw = valid ? nn[i] : 0;
@@ -293,7 +293,7 @@ __global__ void __launch_bounds__(256) k_eam_rho(const double4* __restrict__ pos
     stage_objs<MULTI>(s_obj, P);
     const AtomSlots as = atom_slots<TPA>(M);
     const int i = as.i;
-    const bool valid = i < A.n;
+    const bool valid = i < A.live;
     const int ab = abort_flag(S, misc);
     const double4 pi = valid ? pos[i] : make_double4(0, 0, 0, 0);
     const int nraw = valid ? nn[i] : 0;
@@ -410,7 +410,7 @@ __global__ void __launch_bounds__(256) k_eam_force(const double4* __restrict__ p
     stage_objs<MULTI>(s_obj, P);
     const AtomSlots as = atom_slots<TPA>(M);
     const int i = as.i;
-    const bool valid = i < A.n;
+    const bool valid = i < A.live;
     const int n = A.n;
     const double* __restrict__ Fp = MULTI ? P.fpm : A.d + (size_t)D_FP * n;
     const int ab = abort_flag(S, misc);
@@ -577,7 +577,7 @@ __global__ void k_predrift(const double4* __restrict__ pos, double4* __restrict_
                            int* __restrict__ misc) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = A.n;
-    if (i >= n) return;
+    if (i >= A.live) return;
     const double4 pi = pos[i];
     if (w_inactive(pi.w)) {
         pos_next[i] = pi;
@@ -611,13 +611,16 @@ __global__ void k_predrift(const double4* __restrict__ pos, double4* __restrict_
 
 // Per-system FP64 reduction in a fixed order (one workgroup per system):
 // out[s] = [Ep, Em, KE, Etot, W, com_x, com_y, com_z].
+// One workgroup per box over the box's atoms, [off[2 s], off[2 s] +
+// off[2 s + 1]) of the caller's order, each at its sorted place where[o].
 __global__ void __launch_bounds__(256) k_thermo(const double* __restrict__ d, int n, const int* __restrict__ off,
-                                                double* __restrict__ out) {
+                                                const int* __restrict__ where, double* __restrict__ out) {
     const int sys = blockIdx.x;
-    const int b = off[sys], e = off[sys + 1];
+    const int b = off[2 * sys], e = b + off[2 * sys + 1];
     __shared__ double red[8][256];
     double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // Ep, Em, W, px, py, pz, M
-    for (int i = b + threadIdx.x; i < e; i += 256) {
+    for (int o = b + threadIdx.x; o < e; o += 256) {
+        const int i = where[o];
         acc[0] += d[(size_t)D_EP * n + i];
         acc[1] += d[(size_t)D_EM * n + i];
         acc[2] += d[(size_t)D_VXX * n + i] + d[(size_t)D_VYY * n + i] + d[(size_t)D_VZZ * n + i];
@@ -640,7 +643,8 @@ __global__ void __launch_bounds__(256) k_thermo(const double* __restrict__ d, in
         for (int c = 0; c < 3; ++c) com[c] = red[3 + c][0] / M;
     __syncthreads();
     double ke = 0.0;
-    for (int i = b + threadIdx.x; i < e; i += 256) {
+    for (int o = b + threadIdx.x; o < e; o += 256) {
+        const int i = where[o];
         const double m = d[(size_t)D_M * n + i];
         double rx = d[(size_t)D_VX * n + i] - com[0], ry = d[(size_t)D_VY * n + i] - com[1],
                rz = d[(size_t)D_VZ * n + i] - com[2];
@@ -804,10 +808,11 @@ template <typename R, int TPA>
 static void launch_pair_t(Batch& b, const Pot& p, const StepArgs& S, const double4* pos, double4* pos_next,
                           int which) {
     hipStream_t st = b.ctx->stream;
+    if (b.live == 0) return;
     const int A = kWave / TPA;
-    const long long waves = (b.n + A - 1) / A;
+    const long long waves = (b.live + A - 1) / A;
     const int grid = nblk(waves * kWave, 256);
-    AtomArrays AA{b.dst[b.sc], b.fst[b.sc], b.n};
+    AtomArrays AA = arrays(b);
     if (p.type == PT_LJ) {
         if (!(which & 2)) return;
         // energy and virial code only in the variant that computes them: the
@@ -911,7 +916,7 @@ __global__ void k_langevin_apply(const double4* __restrict__ pos, AtomArrays A, 
                                  ThermArgs T, uint64_t call, int prec) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = A.n;
-    if (i >= n) return;
+    if (i >= A.live) return;
     const double4 pi = pos[i];
     if (w_inactive(pi.w)) return;
     double f[3], v[3];
@@ -939,7 +944,7 @@ __global__ void k_langevin_finish(const double4* __restrict__ pos, double4* __re
     if (skip_step(S, misc)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = A.n;
-    if (i >= n) return;
+    if (i >= A.live) return;
     const double4 pi = pos[i];
     if (w_inactive(pi.w)) {
         if (S.mode == MODE_FINAL_DRIFT) pos_next[i] = pi;
@@ -954,9 +959,9 @@ __global__ void k_langevin_finish(const double4* __restrict__ pos, double4* __re
 }
 
 static void langevin_apply(Batch& b, int prec) {
-    if (b.therm.count == 0 || b.n == 0) return;
-    AtomArrays AA{b.dst[b.sc], b.fst[b.sc], b.n};
-    hipLaunchKernelGGL(k_langevin_apply, dim3(nblk(b.n, 256)), dim3(256), 0, b.ctx->stream, b.pos[b.cur], AA,
+    if (b.therm.count == 0 || b.live == 0) return;
+    AtomArrays AA = arrays(b);
+    hipLaunchKernelGGL(k_langevin_apply, dim3(nblk(b.live, 256)), dim3(256), 0, b.ctx->stream, b.pos[b.cur], AA,
                        b.orig[b.sc], b.therm, b.therm_calls, prec);
     b.therm_calls++;
 }
@@ -988,7 +993,7 @@ void force_compute(Batch& b, const Pot& p, int what, int prec) {
 __global__ void k_prepare(const double4* __restrict__ pos, AtomArrays A, int eam) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = A.n;
-    if (i >= n || w_inactive(pos[i].w)) return;
+    if (i >= A.live || w_inactive(pos[i].w)) return;
     for (int f = D_FX; f <= D_FZ; ++f) A.d[(size_t)f * n + i] = 0.0;
     for (int f = F_FX; f <= F_FZ; ++f) A.f[(size_t)f * n + i] = 0.0f;
     for (int f = D_VXX; f <= D_VYZ; ++f) A.d[(size_t)f * n + i] = 0.0;
@@ -1007,7 +1012,7 @@ __global__ void k_virial_merge(const double4* __restrict__ pos, AtomArrays A, co
                                const float* __restrict__ was_f) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = A.n;
-    if (i >= n || w_inactive(pos[i].w)) return;
+    if (i >= A.live || w_inactive(pos[i].w)) return;
     for (int f = D_FX; f <= D_FZ; ++f) A.d[(size_t)f * n + i] = was_d[(size_t)f * n + i];
     for (int f = F_FX; f <= F_FZ; ++f) A.f[(size_t)f * n + i] = was_f[(size_t)f * n + i];
     A.d[(size_t)D_EP * n + i] += was_d[(size_t)D_EP * n + i];
@@ -1021,7 +1026,7 @@ __global__ void k_force_merge(const double4* __restrict__ pos, AtomArrays A, con
                               const float* __restrict__ was_f, int was_prec, int prec) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = A.n;
-    if (i >= n || w_inactive(pos[i].w)) return;
+    if (i >= A.live || w_inactive(pos[i].w)) return;
     for (int c = 0; c < 3; ++c) {
         const double w = was_prec == P_MIXED ? (double)was_f[(size_t)(F_FX + c) * n + i]
                                              : was_d[(size_t)(D_FX + c) * n + i];
@@ -1041,8 +1046,8 @@ void stage_compute(Batch& b, const Pot& p, int stage, int prec) {
     set_trigger(b, p.rcut, false);
     const bool eam = p.type == PT_EAM;
     const bool bpnn = p.type == PT_BPNN;
-    const int grid = nblk(b.n, 256);
-    AtomArrays AA{b.dst[b.sc], b.fst[b.sc], b.n};
+    const int grid = nblk(b.live, 256);
+    AtomArrays AA = arrays(b);
     const double4* pos = b.pos[b.cur];
     auto pass = [&](int which, const StepArgs& S) {
         if (prec == P_MIXED) launch_pair_r<float>(b, p, S, pos, nullptr, which);
@@ -1050,7 +1055,7 @@ void stage_compute(Batch& b, const Pot& p, int stage, int prec) {
     };
     switch (stage) {
         case ST_PREPARE:
-            hipLaunchKernelGGL(k_prepare, dim3(grid), dim3(256), 0, st, pos, AA, eam ? 1 : 0);
+            if (grid) hipLaunchKernelGGL(k_prepare, dim3(grid), dim3(256), 0, st, pos, AA, eam ? 1 : 0);
             // solver::Multi's prepare leaves rcut (multi.cxx:56-66)
             if (!(eam && p.eam.nk > 1 && !p.eam.alloy)) b.last_rcut = p.rcut;
             b.f_prec = P_FP64;
@@ -1083,8 +1088,9 @@ void stage_compute(Batch& b, const Pot& p, int stage, int prec) {
                 if (eam) pass(1, StepArgs{0.0, -1, MODE_COMPUTE, W_NC_STORED, 1});
                 pass(2, StepArgs{0.0, -1, MODE_COMPUTE, 0, 1});
             }
-            hipLaunchKernelGGL(k_force_merge, dim3(grid), dim3(256), 0, st, pos, AA, b.dst[idle], b.fst[idle],
-                               was_prec, prec);
+            if (grid)
+                hipLaunchKernelGGL(k_force_merge, dim3(grid), dim3(256), 0, st, pos, AA, b.dst[idle], b.fst[idle],
+                                   was_prec, prec);
             langevin_apply(b, prec);
             b.f_prec = prec;
             break;
@@ -1105,7 +1111,7 @@ void stage_compute(Batch& b, const Pot& p, int stage, int prec) {
                                     hipMemcpyDeviceToDevice, st));
             if (eam) pass(1, StepArgs{0.0, -1, MODE_COMPUTE, W_NC_STORED | W_ENERGY, 1});
             pass(2, StepArgs{0.0, -1, MODE_COMPUTE, W_ENERGY | W_VIRIAL, 1});
-            hipLaunchKernelGGL(k_virial_merge, dim3(grid), dim3(256), 0, st, pos, AA, b.dst[idle], b.fst[idle]);
+            if (grid) hipLaunchKernelGGL(k_virial_merge, dim3(grid), dim3(256), 0, st, pos, AA, b.dst[idle], b.fst[idle]);
             break;
         }
         default:
@@ -1121,12 +1127,13 @@ void run_steps(Batch& b, const Pot& p, double dt, int nsteps, int prec, int chun
     if (!b.list_valid) nlist_rebuild(b);
     set_trigger(b, p.rcut, true);
     if (b.f_prec < 0) throw Error(E_STATE, "no force to start from: compute forces once before stepping");
-    const int grid = nblk(b.n, 256);
+    if (b.live == 0) return;  // every record void: nothing moves
+    const int grid = nblk(b.live, 256);
     int abort_init = INT_MAX;
     HIPCHECK(hipMemcpyAsync(b.misc + M_ABORT, &abort_init, sizeof(int), hipMemcpyHostToDevice, st));
     int c0 = b.cur;
     {
-        AtomArrays AA{b.dst[b.sc], b.fst[b.sc], b.n};
+        AtomArrays AA = arrays(b);
         launch_k(b, K_PREDRIFT, k_predrift, dim3(grid), dim3(256), 0, b.pos[c0], b.pos[c0 ^ 1], AA, b.f_prec, dt,
                  b.d_desc, b.rsave, b.misc);
     }
@@ -1152,7 +1159,7 @@ void run_steps(Batch& b, const Pot& p, double dt, int nsteps, int prec, int chun
             StepArgs SF = S;
             SF.mode = MODE_FORCE_STEP;
             launch_step_kernels(b, p, SF, prec, b.pos[buf(t)], nullptr);
-            AtomArrays AA{b.dst[b.sc], b.fst[b.sc], b.n};
+            AtomArrays AA = arrays(b);
             auto fin = prec == P_MIXED ? k_langevin_finish<float> : k_langevin_finish<double>;
             hipLaunchKernelGGL(fin, dim3(grid), dim3(256), 0, st, b.pos[buf(t)], last ? nullptr : b.pos[buf(t) ^ 1],
                                AA, S, b.orig[b.sc], b.therm, b.therm_calls + (uint64_t)t, b.d_desc, b.rsave, b.misc);
@@ -1245,6 +1252,7 @@ void time_step_kernels(Batch& b, const Pot& p, int prec, double dt, int reps, do
     if (b.f_prec < 0) throw Error(E_STATE, "no force to start from: compute forces once before timing");
     hipStream_t st = b.ctx->stream;
     if (!b.list_valid) nlist_rebuild(b);
+    if (b.live == 0) return;
     set_trigger(b, p.rcut, true);
     const int idle = b.sc ^ 1;
     const size_t vbytes = sizeof(double) * 3 * (size_t)b.n;
@@ -1291,9 +1299,16 @@ void time_step_kernels(Batch& b, const Pot& p, int prec, double dt, int reps, do
 
 void thermo(Batch& b, double* out) {
     hipStream_t st = b.ctx->stream;
+    // each box's atoms: the first present[s] of its records in the
+    // caller's order (the rest are void)
+    std::vector<int> off(2 * (size_t)b.nsys);
+    for (int s = 0; s < b.nsys; ++s) {
+        off[2 * s] = b.atom_off[s];
+        off[2 * s + 1] = b.present[s];
+    }
     int* d_off = reinterpret_cast<int*>(b.thermo + 8 * b.nsys);
-    HIPCHECK(hipMemcpyAsync(d_off, b.atom_off.data(), sizeof(int) * (b.nsys + 1), hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_thermo, dim3(b.nsys), dim3(256), 0, st, b.dst[b.sc], b.n, d_off, b.thermo);
+    HIPCHECK(hipMemcpyAsync(d_off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_thermo, dim3(b.nsys), dim3(256), 0, st, b.dst[b.sc], b.n, d_off, b.where, b.thermo);
     HIPCHECK(hipMemcpyAsync(out, b.thermo, sizeof(double) * 8 * b.nsys, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
 }

@@ -79,7 +79,10 @@ struct AtomArrays {
     double* d;   // current double fields base (stride n)
     float* f;    // current float fields base (stride n)
     int n;
+    int live;    // records [0, live) hold atoms; the rest are void
 };
+
+inline AtomArrays arrays(const Batch& b) { return AtomArrays{b.dst[b.sc], b.fst[b.sc], b.n, b.live}; }
 
 // Per-atom inputs of the velocity-Verlet finish, loaded in the kernel
 // prologue so their latency overlaps the neighbour loop.

@@ -143,26 +143,33 @@ __global__ void k_scatter(const int* __restrict__ atom_cell, const int* __restri
     if (valid) src[start[c] + base + ((int)(threadIdx.x & (kWave - 1)) - r.lead)] = i;
 }
 
-// Move every per-atom array into the new (cell) order, ascending previous
-// index inside each cell: the thread of scatter slot sp takes atom s =
-// src[sp] to slot start[c] + (the number of the cell's atoms with a smaller
-// index), so the scatter's run order never shows.  Also records the cell of
-// each sorted atom and the displacement reference positions.
+// Move the records [0, count) (arrays of stride n) into the new (cell)
+// order, ascending previous index inside each cell: the thread of scatter
+// slot sp takes atom s = src[sp] to slot start[c] + (the number of the
+// cell's atoms with a smaller index), so the scatter's run order never
+// shows.  Void records (the void cell) keep their scatter slot: their order
+// carries nothing, and ranking a cell of thousands of them would cost the
+// square of its size.  Also records the cell of each sorted atom and the
+// displacement reference positions.
 __global__ void __launch_bounds__(256) k_permute(const int* __restrict__ src, const int* __restrict__ start,
                                                  const double4* __restrict__ pos_in, double4* __restrict__ pos_out,
                                                  const double* __restrict__ din, double* __restrict__ dout,
                                                  const float* __restrict__ fin, float* __restrict__ fout,
                                                  const int* __restrict__ oin, int* __restrict__ oout,
                                                  const int* __restrict__ cell_in, int* __restrict__ cell_out,
-                                                 double4* __restrict__ rsave, int* __restrict__ where, int n) {
+                                                 double4* __restrict__ rsave, int* __restrict__ where, int count, int n,
+                                                 int void_cell) {
     const int sp = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (sp >= n) return;
+    if (sp >= count) return;
     const int s = src[sp];
     const int c = cell_in[s];
-    const int b = start[c], e = start[c + 1];
-    int rank = 0;
-    for (int q = b; q < e; ++q) rank += src[q] < s ? 1 : 0;
-    const int p = b + rank;
+    int p = sp;
+    if (c != void_cell) {
+        const int b = start[c], e = start[c + 1];
+        int rank = 0;
+        for (int q = b; q < e; ++q) rank += src[q] < s ? 1 : 0;
+        p = b + rank;
+    }
     const double4 q = pos_in[s];
     pos_out[p] = q;
     rsave[p] = q;
@@ -172,6 +179,21 @@ __global__ void __launch_bounds__(256) k_permute(const int* __restrict__ src, co
     oout[p] = o;
     where[o] = p;
     cell_out[p] = c;
+}
+
+// After a sort of all records: the void tail [live, n) of the current
+// arrays into the idle ones, so that both position buffers and both field
+// copies hold it and sorts of [0, live) and steps leave it as it is.
+__global__ void __launch_bounds__(256) k_copy_tail(const double4* __restrict__ pos_a, double4* __restrict__ pos_b,
+                                                   const double* __restrict__ da, double* __restrict__ db,
+                                                   const float* __restrict__ fa, float* __restrict__ fb,
+                                                   const int* __restrict__ oa, int* __restrict__ ob, int live, int n) {
+    const long long i = live + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    pos_b[i] = pos_a[i];
+    for (int f = 0; f < D_NFIELDS; ++f) db[(size_t)f * n + i] = da[(size_t)f * n + i];
+    for (int f = 0; f < F_NFIELDS; ++f) fb[(size_t)f * n + i] = fa[(size_t)f * n + i];
+    ob[i] = oa[i];
 }
 
 // One workgroup per cell.  Wave 0 lays out the 27-cell stencil (with the
@@ -447,6 +469,7 @@ __global__ void __launch_bounds__(kBuildThreads) k_build(const double4* __restri
 // verlet-list.cxx:110-158 decide(): |shortest(r - r_saved)|^2 > (skin/2)^2
 __global__ void k_decide(const double4* __restrict__ pos, const double4* __restrict__ rsave,
                          const SysDesc* __restrict__ desc, int* __restrict__ misc, int n) {
+    // n: the live records
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double4 p = pos[i], q = rsave[i];
@@ -501,22 +524,36 @@ void nlist_regrow(Batch& b, int maxnn) {
 void nlist_rebuild(Batch& b, bool sync) {
     hipStream_t st = b.ctx->stream;
     const int n = b.n;
-    if (n == 0) {
+    // all records when some became void or live since the last sort, else
+    // the live prefix only (the void tail stays where it is)
+    const int count = b.full_sort ? n : b.live;
+    if (count == 0) {
         b.list_valid = true;
+        b.shifts_stale = false;
         return;
     }
-    hipLaunchKernelGGL(k_wrap_count, dim3(nblk(n, 256)), dim3(256), 0, st, b.pos[b.cur], b.d_desc, b.atom_cell,
-                       b.cell_count, b.misc, n);
+    hipLaunchKernelGGL(k_wrap_count, dim3(nblk(count, 256)), dim3(256), 0, st, b.pos[b.cur], b.d_desc, b.atom_cell,
+                       b.cell_count, b.misc, count);
     hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, st, b.cell_count, b.cell_start, b.cell_fill, b.ncells);
-    hipLaunchKernelGGL(k_scatter, dim3(nblk(n, 256)), dim3(256), 0, st, b.atom_cell, b.cell_start, b.cell_fill,
-                       b.sorted_src, n);
+    hipLaunchKernelGGL(k_scatter, dim3(nblk(count, 256)), dim3(256), 0, st, b.atom_cell, b.cell_start, b.cell_fill,
+                       b.sorted_src, count);
     const int s0 = b.sc, s1 = b.sc ^ 1;
     // cell_fill is free after the scatter: reuse it for the sorted cell ids
-    hipLaunchKernelGGL(k_permute, dim3(nblk(n, 256)), dim3(256), 0, st, b.sorted_src, b.cell_start, b.pos[b.cur],
+    hipLaunchKernelGGL(k_permute, dim3(nblk(count, 256)), dim3(256), 0, st, b.sorted_src, b.cell_start, b.pos[b.cur],
                        b.pos[b.cur ^ 1], b.dst[s0], b.dst[s1], b.fst[s0], b.fst[s1], b.orig[s0], b.orig[s1],
-                       b.atom_cell, b.cell_fill, b.rsave, b.where, n);
+                       b.atom_cell, b.cell_fill, b.rsave, b.where, count, n, b.ncells - 1);
     b.cur ^= 1;
     b.sc = s1;
+    if (b.full_sort) {
+        int live = 0;
+        for (int s = 0; s < b.nsys; ++s) live += b.present[s];
+        b.live = live;
+        if (live < n)
+            hipLaunchKernelGGL(k_copy_tail, dim3(nblk(n - live, 256)), dim3(256), 0, st, b.pos[b.cur],
+                               b.pos[b.cur ^ 1], b.dst[s1], b.dst[s0], b.fst[s1], b.fst[s0], b.orig[s1], b.orig[s0],
+                               live, n);
+        b.full_sort = false;
+    }
     launch_build(b);
     if (sync) {
         HIPCHECK(hipMemcpyAsync(b.ctx->h_pin, b.misc, sizeof(int) * M_COUNT, hipMemcpyDeviceToHost, st));
@@ -535,7 +572,9 @@ int nlist_decide(Batch& b, double rcut) {
     if (!b.list_valid) return 1;
     set_trigger(b, rcut, false);
     HIPCHECK(hipMemsetAsync(b.misc + M_DECIDE, 0, sizeof(int), st));
-    hipLaunchKernelGGL(k_decide, dim3(nblk(b.n, 256)), dim3(256), 0, st, b.pos[b.cur], b.rsave, b.d_desc, b.misc, b.n);
+    if (b.live > 0)
+        hipLaunchKernelGGL(k_decide, dim3(nblk(b.live, 256)), dim3(256), 0, st, b.pos[b.cur], b.rsave, b.d_desc,
+                           b.misc, b.live);
     HIPCHECK(hipMemcpyAsync(b.ctx->h_pin, b.misc, sizeof(int) * M_COUNT, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
     return b.ctx->h_pin[M_DECIDE] != 0;

@@ -151,17 +151,20 @@ std::string ctx_msg(mdc_ctx* c) {
 
 // The final E/F/virial evaluation of the batch and, for each finishing
 // request, its records and the thermo row of its slot; then the slot is
-// switched off and freed.
+// emptied (all its records void, so the next sort moves them out of the
+// kernels' range) and freed.
 void finish(mdc_server* S, SBatch* B, mdc_ctx* ctx, mdc_sys* sys, const SKind& K, const std::vector<SReq*>& reqs) {
     SRV_CALL(mdc_compute(sys, K.pot, MDC_FORCE | MDC_ENERGY | MDC_VIRIAL, K.precision));
     std::vector<double> th((size_t)8 * B->nslots);
     SRV_CALL(mdc_thermo(sys, th.data()));
+    const double lo0[3] = {0.0, 0.0, 0.0}, hi0[3] = {1.0, 1.0, 1.0};
+    const uint8_t per0[3] = {0, 0, 0};
     for (SReq* r : reqs) {
         r->out.resize(r->atoms.size());
         const uint64_t first = (uint64_t)r->slot * (uint64_t)B->cls;
         SRV_CALL(mdc_sys_download_range(sys, first, r->atoms.size(), r->out.data(), MDC_F_ALL));
         std::memcpy(r->thermo, &th[(size_t)8 * r->slot], sizeof(r->thermo));
-        SRV_CALL(mdc_sys_set_active(sys, r->slot, 0));
+        SRV_CALL(mdc_sys_refill_box(sys, r->slot, 0, nullptr, lo0, hi0, per0));
     }
     std::lock_guard<std::mutex> lk(S->mu);
     for (SReq* r : reqs) {
