@@ -123,17 +123,28 @@ def run_batch_eval(E, ctx, cells, pot, precision, reps, lanes=None):
     s.upload(np.concatenate([a for a, _ in cells]))
     p = make_pot(E, ctx, pot)
     s.build()
-    tune = None
+    threads = 256
     if lanes is None:
-        lanes, tune = s.autotune(p, precision, reps=10)
+        lanes, threads, _ = s.autotune_tiles(p, precision, reps=10)
     what = E.FORCE | E.ENERGY | E.VIRIAL
-    s.compute(p, what, precision)
-    s.sync()
-    t0 = time.perf_counter()
-    for _ in range(reps):
+
+    def timed():
         s.compute(p, what, precision)
-    s.sync()
-    t1 = time.perf_counter()
+        s.compute(p, what, precision)
+        s.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            s.compute(p, what, precision)
+        s.sync()
+        return time.perf_counter() - t0
+
+    eager = timed()
+    # graph mode: the evaluation captured once and replayed
+    s.set_graphs(True)
+    graph = timed()
+    gstats = s.graph_stats()
+    s.set_graphs(False)
+    t1, t0 = graph, 0.0
     s.timing(1)
     for _ in range(min(reps, 20)):
         s.compute(p, what, precision)
@@ -141,7 +152,9 @@ def run_batch_eval(E, ctx, cells, pot, precision, reps, lanes=None):
     n = s.n
     nbar = s.stats()["mean"]
     out = {"value": n * reps / (t1 - t0), "unit": "atom-evaluations/s", "ms_per_eval": 1e3 * (t1 - t0) / reps,
-           "atoms": n, "systems": len(cells), "nbar": nbar, "lanes": lanes,
+           "mode": "graph-captured evaluation, replayed", "graph": gstats,
+           "eager": {"value": n * reps / eager, "ms_per_eval": 1e3 * eager / reps},
+           "atoms": n, "systems": len(cells), "nbar": nbar, "lanes": lanes, "threads_per_workgroup": threads,
            "kernel_us": {KINDS[k]: round(ms[k] / nl[k] * 1e3, 2) for k in range(8) if nl[k] > 0}}
     if nl[0] > 0 and pot["kind"] in ("eam_multi", "alloy", "eam"):
         nk = pot.get("nkinds", 1) if pot["kind"] == "eam_multi" else (

@@ -55,6 +55,10 @@ enum {
 /* ---- context: one device, one ordered stream ---------------------------- */
 mdc_status mdc_ctx_create(int device, mdc_ctx** out);
 mdc_status mdc_ctx_destroy(mdc_ctx* ctx);
+/* How calls on this context wait for the device: 0 blocks (the default),
+ * 1 polls the stream, keeping a host core busy but waking sooner after
+ * short evaluations.  No reference counterpart. */
+mdc_status mdc_ctx_set_spin(mdc_ctx* ctx, int spin);
 const char* mdc_ctx_last_error(const mdc_ctx* ctx);
 const char* mdc_last_error(void);
 int mdc_atom_record_size(void); /* 208 */
@@ -167,6 +171,28 @@ mdc_status mdc_sys_set_lanes(mdc_sys* sys, int lanes_per_atom);
  * No reference counterpart: the reference has no device launch shape. */
 mdc_status mdc_sys_autotune(mdc_sys* sys, const mdc_pot* pot, int precision, int reps, int* best_lanes,
                             double* us_per_eval);
+/* Threads per workgroup (64, 128 or 256) of the LJ and EAM kernels.
+ * No reference counterpart: the reference has no device launch shape. */
+mdc_status mdc_sys_set_block(mdc_sys* sys, int threads);
+/* The launch tile of the LJ and EAM kernels: lanes per atom (1, 2, 4, 8,
+ * 16) times threads per workgroup (64, 128, 256), timed as
+ * mdc_sys_autotune times lanes, keeping the fastest pair.  us (may be
+ * NULL) receives the 15 mean times in microseconds, lanes-major.  BPNN
+ * potentials are refused (their workgroup shape follows the LDS staging).
+ * No reference counterpart. */
+mdc_status mdc_sys_autotune_tiles(mdc_sys* sys, const mdc_pot* pot, int precision, int reps, int* best_lanes,
+                                  int* best_threads, double* us);
+/* Graph mode (on != 0) of mdc_compute on this system: the device work of
+ * an evaluation (fills, density and force kernels, the counter readback) is
+ * captured in a HIP graph for its configuration (potential, flags,
+ * precision, list build, lanes, workgroup size and the arrays it reads)
+ * and replayed while the configuration holds; a configuration's first call
+ * runs as usual, its second captures it.  Results are those of the plain
+ * path.  Evaluations with thermostats attached or with kernel timing on
+ * run as usual.  captures and launches (may be NULL) report the counts.
+ * No reference counterpart: the reference launches nothing. */
+mdc_status mdc_sys_set_graphs(mdc_sys* sys, int on);
+mdc_status mdc_sys_graph_stats(const mdc_sys* sys, uint64_t* captures, uint64_t* launches);
 
 /* ---- neighbour list (neibs::VerletList) --------------------------------- */
 /* update(kbuf = -1): wraps positions into periodic boxes, bins, rebuilds */

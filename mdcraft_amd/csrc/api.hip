@@ -507,6 +507,7 @@ void free_batch(Batch& b) {
     dfree(b.thermo);
     dfree(b.stage);
     b.stage_bytes = 0;
+    graph_release(b);
     bpnn_release_work(b);
     for (auto& t : b.ev_pending) {
         b.ev_free.push_back(t.a);
@@ -552,6 +553,7 @@ void layout_cells(Batch& b) {
         b.cell_fill = dalloc<int>(std::max(alloc + 1, b.n));
         b.cell_alloc = alloc;
         b.ncells = -1;
+        b.alloc_gen++;
     }
     if (off != b.ncells) {
         // each rebuild's scan clears the counts it consumed: zero them when
@@ -586,6 +588,7 @@ void size_list(Batch& b, bool grow_only = false) {
         dfree(b.nl);
         b.nl = dalloc<int>(elems);
         b.nl_elems = elems;
+        b.alloc_gen++;
     }
     b.cap = cap;
     b.M = cap / b.tpa;
@@ -612,6 +615,7 @@ void upload(Batch& b, const AtomRec* a, uint32_t fields) {
             if (fields & MDC_F_KIND) {
                 if (r.kind > 255) throw Error(E_INVAL, "atom kind above 255 is not supported");
                 b.kind[o] = r.kind;
+                b.kind_max = -1;
             }
             if (fields & MDC_F_TAG) b.tag[o] = r.tag;
             if (fields & MDC_F_UID) b.uid[o] = r.uid;
@@ -773,6 +777,7 @@ void refill_box(Batch& b, int box, int count, const AtomRec* a, const double* lo
     b.rns_min[box] = mn;
     b.rns_max[box] = mx;
     b.present[box] = count;
+    b.kind_max = -1;
     layout_cells(b);
     // the list capacity only grows here: refills come and go in a running
     // batch, and a smaller list would be regrown by the next denser box
@@ -909,12 +914,17 @@ void optimize_rns(Batch& b, double kbuf, double* rns_out) {
     if (rns_out) std::memcpy(rns_out, rns.data(), sizeof(double) * n);
 }
 
-void check_kinds(const Batch& b, const Pot& p) {
-    if (p.type == PT_EAM && p.eam.nk > 1)
-        for (int o = 0; o < b.n; ++o)
-            if (b.kind[o] >= p.eam.nk)
-                throw Error(E_INVAL, p.eam.alloy ? "atom kind outside the alloy's elements"
-                                                 : "atom kind outside the Multi potential's kinds");
+void check_kinds(Batch& b, const Pot& p) {
+    if (p.type != PT_EAM || p.eam.nk <= 1) return;
+    if (b.kind_max < 0) {
+        // recounted once after kinds change, not per evaluation
+        int mx = 0;
+        for (int o = 0; o < b.n; ++o) mx = std::max(mx, (int)b.kind[o]);
+        b.kind_max = mx;
+    }
+    if (b.kind_max >= p.eam.nk)
+        throw Error(E_INVAL, p.eam.alloy ? "atom kind outside the alloy's elements"
+                                         : "atom kind outside the Multi potential's kinds");
 }
 
 }  // namespace
@@ -954,6 +964,13 @@ mdc_status mdc_ctx_create(int device, mdc_ctx** out) {
         c->c.lds_max = prop.sharedMemPerBlock;
         scratch_lookup(device);
         *out = c.release();
+    });
+}
+
+mdc_status mdc_ctx_set_spin(mdc_ctx* ctx, int spin) {
+    return guard(ctx ? &ctx->c : nullptr, [&] {
+        if (!ctx) throw Error(E_INVAL, "null argument");
+        ctx->c.spin = spin != 0;
     });
 }
 
@@ -1346,6 +1363,7 @@ static void set_lanes(Batch& b, int lanes) {
             dfree(b.nl);
             b.nl = dalloc<int>(elems);
             b.nl_elems = elems;
+            b.alloc_gen++;
         }
         b.cap = cap;
         b.M = cap / b.tpa;
@@ -1602,6 +1620,71 @@ mdc_status mdc_sys_autotune(mdc_sys* sys, const mdc_pot* pot, int precision, int
         nlist_rebuild(b);
         force_compute(b, pot->p, MDC_FORCE, precision);
         *best_lanes = best_l;
+    });
+}
+
+mdc_status mdc_sys_set_block(mdc_sys* sys, int threads) {
+    return guard(sys ? sys->b.ctx : nullptr, [&] {
+        if (!sys) throw Error(E_INVAL, "null argument");
+        if (threads != 64 && threads != 128 && threads != 256)
+            throw Error(E_INVAL, "threads per workgroup must be 64, 128 or 256");
+        sys->b.block = threads;
+    });
+}
+
+mdc_status mdc_sys_autotune_tiles(mdc_sys* sys, const mdc_pot* pot, int precision, int reps, int* best_lanes,
+                                  int* best_threads, double* us_out) {
+    return guard(sys ? sys->b.ctx : nullptr, [&] {
+        if (!sys || !pot || !best_lanes || !best_threads) throw Error(E_INVAL, "null argument");
+        if (precision != MDC_FP64 && precision != MDC_MIXED) throw Error(E_INVAL, "precision must be FP64 or MIXED");
+        if (reps < 1) throw Error(E_INVAL, "reps must be >= 1");
+        if (pot->p.type == PT_BPNN) throw Error(E_INVAL, "the tile autotuner covers the LJ and EAM kernels");
+        Batch& b = sys->b;
+        HIPCHECK(hipSetDevice(b.ctx->device));
+        check_kinds(b, pot->p);
+        const int lanes[5] = {1, 2, 4, 8, 16}, threads[3] = {64, 128, 256};
+        double best = 1e300;
+        int bl = b.tpa, bt = b.block;
+        for (int c = 0; c < 5; ++c) {
+            set_lanes(b, lanes[c]);
+            nlist_rebuild(b);
+            force_compute(b, pot->p, MDC_FORCE, precision);  // warm-up, and the force to start from
+            for (int q = 0; q < 3; ++q) {
+                b.block = threads[q];
+                double t[2];
+                time_step_kernels(b, pot->p, precision, 1e-6, std::max(reps, 20), t);
+                const double us = t[0] + t[1];
+                if (us_out) us_out[3 * c + q] = us;
+                if (us < best) {
+                    best = us;
+                    bl = lanes[c];
+                    bt = threads[q];
+                }
+            }
+        }
+        set_lanes(b, bl);
+        b.block = bt;
+        nlist_rebuild(b);
+        force_compute(b, pot->p, MDC_FORCE, precision);
+        *best_lanes = bl;
+        *best_threads = bt;
+    });
+}
+
+mdc_status mdc_sys_set_graphs(mdc_sys* sys, int on) {
+    return guard(sys ? sys->b.ctx : nullptr, [&] {
+        if (!sys) throw Error(E_INVAL, "null argument");
+        HIPCHECK(hipSetDevice(sys->b.ctx->device));
+        sys->b.graphs = on != 0;
+        if (!sys->b.graphs) graph_release(sys->b);
+    });
+}
+
+mdc_status mdc_sys_graph_stats(const mdc_sys* sys, uint64_t* captures, uint64_t* launches) {
+    return guard(sys ? sys->b.ctx : nullptr, [&] {
+        if (!sys) throw Error(E_INVAL, "null argument");
+        if (captures) *captures = (uint64_t)sys->b.graph_captures;
+        if (launches) *launches = (uint64_t)sys->b.graph_launches;
     });
 }
 

@@ -1277,11 +1277,13 @@ static void ensure_work(Batch& b, const BpnnParams& bp, int prec, bool vir) {
         w.prec = prec;
         w.rev_elems = b.nl_elems;
         w.rev_valid = false;
+        b.alloc_gen++;
     }
     if (vir && !w.has_vir) {
         w.a.PV = dnew<char>((size_t)b.n * w.cap * 3 * rs);
         w.a.OV = dnew<char>((size_t)b.n * 3 * rs);
         w.has_vir = true;
+        b.alloc_gen++;
     }
 }
 

@@ -56,7 +56,15 @@ struct Ctx {
     hipEvent_t chunk_ev[2] = {nullptr, nullptr};
     int cu_count = 0;
     size_t lds_max = 65536;  // LDS bytes one workgroup may use
+    // waits on the stream poll it (a host core kept busy) instead of
+    // blocking: a shorter wake-up for short evaluations
+    bool spin = false;
 };
+
+// Waits for the context's stream, or for an event on it, by blocking or,
+// with Ctx::spin, by polling (forces.hip).
+void stream_wait(Ctx& c);
+void event_wait(Ctx& c, hipEvent_t e);
 
 // Langevin thermostats (thermostat/langevin.cxx:7-42) applied one after
 // the other, as thermostat::Compound does
@@ -214,6 +222,8 @@ struct Batch {
     int since_rebuild = 0;
     // host copies of the integer fields the device does not need
     std::vector<uint16_t> kind, tag;
+    // largest atom kind held (-1: to be recounted after kinds were written)
+    int kind_max = -1;
     std::vector<uint32_t> uid;
     std::vector<double> rcut_host;
     std::vector<uint8_t> periodic_bc;  // per system: solver built with the Periodic boundary
@@ -232,6 +242,31 @@ struct Batch {
     // as needed and kept, so transfers allocate and free nothing
     char* stage = nullptr;
     size_t stage_bytes = 0;
+    // Reallocations of any device array the evaluation kernels take (list,
+    // cells, work arrays): a captured evaluation is valid only for the
+    // arrays it was captured with.
+    uint64_t alloc_gen = 0;
+    // Graph mode of mdc_compute: the device work of an evaluation captured
+    // once per configuration (key) and replayed.  A configuration's first
+    // call runs as usual (so that every allocation happens outside a
+    // capture), the second captures it, later ones replay the graph.
+    bool graphs = false;
+    struct GraphKey {
+        const void* pot = nullptr;
+        int what = 0, prec = 0, rebuilds = 0, cap = 0, live = 0, tpa = 0, block = 0, cur = 0, sc = 0;
+        uint64_t alloc_gen = 0;
+        bool operator==(const GraphKey& o) const {
+            return pot == o.pot && what == o.what && prec == o.prec && rebuilds == o.rebuilds && cap == o.cap &&
+                   live == o.live && tpa == o.tpa && block == o.block && cur == o.cur && sc == o.sc &&
+                   alloc_gen == o.alloc_gen;
+        }
+    };
+    GraphKey gkey;        // configuration of gexec, or of the last eager call
+    bool gkey_set = false;
+    hipGraphExec_t gexec = nullptr;
+    long graph_launches = 0, graph_captures = 0;
+    // threads per workgroup of the pair-potential kernels (64, 128 or 256)
+    int block = 256;
     // per-system extent of rns (host), for the displacement trigger
     std::vector<double> rns_min, rns_max;
     // optional kernel timing: hipEvents recorded on the launch stream around
@@ -252,6 +287,8 @@ enum KKind { K_FORCE = 0, K_DENSITY = 1, K_PREDRIFT = 2, K_BP_DESC = 3, K_BP_MLP
 hipEvent_t timer_begin(Batch& b);
 void timer_end(Batch& b, hipEvent_t a, int kind);
 void timer_drain(Batch& b);
+// drops the batch's captured evaluation (forces.hip)
+void graph_release(Batch& b);
 hipEvent_t timer_event(Batch& b);
 
 // Launches kernel k on the context's stream.  While timing is on, the
@@ -272,7 +309,8 @@ void launch_k(Batch& b, int kind, void (*k)(P...), dim3 grid, dim3 block, unsign
         e0 = timer_event(b);
         e1 = timer_event(b);
     }
-    const hipError_t err = hipExtLaunchKernel((const void*)k, grid, block, argv, lds, b.ctx->stream, e0, e1, 0);
+    const hipError_t err = e0 ? hipExtLaunchKernel((const void*)k, grid, block, argv, lds, b.ctx->stream, e0, e1, 0)
+                              : hipLaunchKernel((const void*)k, grid, block, argv, lds, b.ctx->stream);
     if (err != hipSuccess) throw Error(E_HIP, std::string("kernel launch: ") + hipGetErrorString(err));
     if (e0) b.ev_pending.push_back({e0, e1, kind});
 }

@@ -273,7 +273,7 @@ __device__ inline int wave_max_group(int v, int tpa) {
 template <bool MULTI>
 __device__ inline void stage_objs(EamObj* s_obj, const EamK& P) {
     if (!MULTI) return;
-    if ((int)threadIdx.x < P.npair) s_obj[threadIdx.x] = P.objs[threadIdx.x];
+    for (int q = threadIdx.x; q < P.npair; q += blockDim.x) s_obj[q] = P.objs[q];
     __syncthreads();
 }
 
@@ -770,6 +770,7 @@ static EamK eam_args(const Pot& p, Batch& b) {
             if (hipMalloc(&b.fpm, need * sizeof(double)) != hipSuccess)
                 throw Error(E_NOMEM, "out of device memory for the Multi embedding derivatives");
             b.fpm_elems = need;
+            b.alloc_gen++;
         }
         k.fpm = b.fpm;
     }
@@ -811,7 +812,7 @@ static void launch_pair_t(Batch& b, const Pot& p, const StepArgs& S, const doubl
     if (b.live == 0) return;
     const int A = kWave / TPA;
     const long long waves = (b.live + A - 1) / A;
-    const int grid = nblk(waves * kWave, 256);
+    const int grid = nblk(waves * kWave, b.block);
     AtomArrays AA = arrays(b);
     if (p.type == PT_LJ) {
         if (!(which & 2)) return;
@@ -825,16 +826,16 @@ static void launch_pair_t(Batch& b, const Pot& p, const StepArgs& S, const doubl
         auto k = p.lj.fn == 1 ? pick(std::integral_constant<int, 1>{})
                               : p.lj.fn == 2 ? pick(std::integral_constant<int, 2>{})
                                              : pick(std::integral_constant<int, 0>{});
-        launch_k(b, K_FORCE, k, dim3(grid), dim3(256), 0, pos, pos_next, b.nl, b.nn, b.M, b.cap, b.d_desc,
+        launch_k(b, K_FORCE, k, dim3(grid), dim3(b.block), 0, pos, pos_next, b.nl, b.nn, b.M, b.cap, b.d_desc,
                  lj_args(p), AA, S, b.rsave, b.misc);
     } else {
         EamK E = eam_args(p, b);
         if (which & 1) {
             if (E.nk > 1)
-                launch_k(b, K_DENSITY, k_eam_rho<R, TPA, true>, dim3(grid), dim3(256), 0, pos, b.nl, b.nn, b.M, b.cap,
+                launch_k(b, K_DENSITY, k_eam_rho<R, TPA, true>, dim3(grid), dim3(b.block), 0, pos, b.nl, b.nn, b.M, b.cap,
                          b.d_desc, E, AA, S, b.misc);
             else
-                launch_k(b, K_DENSITY, k_eam_rho<R, TPA, false>, dim3(grid), dim3(256), 0, pos, b.nl, b.nn, b.M,
+                launch_k(b, K_DENSITY, k_eam_rho<R, TPA, false>, dim3(grid), dim3(b.block), 0, pos, b.nl, b.nn, b.M,
                          b.cap, b.d_desc, E, AA, S, b.misc);
         }
         if (!(which & 2)) return;
@@ -842,7 +843,7 @@ static void launch_pair_t(Batch& b, const Pot& p, const StepArgs& S, const doubl
         auto k = p.eam.alloy ? (ev ? k_eam_force<R, TPA, true, true, true> : k_eam_force<R, TPA, true, false, true>)
                  : E.nk > 1  ? (ev ? k_eam_force<R, TPA, true, true> : k_eam_force<R, TPA, true, false>)
                              : (ev ? k_eam_force<R, TPA, false, true> : k_eam_force<R, TPA, false, false>);
-        launch_k(b, K_FORCE, k, dim3(grid), dim3(256), 0, pos, pos_next, b.nl, b.nn, b.M, b.cap, b.d_desc, E, AA, S,
+        launch_k(b, K_FORCE, k, dim3(grid), dim3(b.block), 0, pos, pos_next, b.nl, b.nn, b.M, b.cap, b.d_desc, E, AA, S,
                  b.rsave, b.misc);
     }
 }
@@ -882,7 +883,7 @@ void check_errors(Batch& b, const int* counters) {
 
 static void read_misc(Batch& b) {
     HIPCHECK(hipMemcpyAsync(b.ctx->h_pin, b.misc, sizeof(int) * M_COUNT, hipMemcpyDeviceToHost, b.ctx->stream));
-    HIPCHECK(hipStreamSynchronize(b.ctx->stream));
+    stream_wait(*b.ctx);
     timer_drain(b);
 }
 
@@ -966,23 +967,120 @@ static void langevin_apply(Batch& b, int prec) {
     b.therm_calls++;
 }
 
+void stream_wait(Ctx& c) {
+    if (!c.spin) {
+        HIPCHECK(hipStreamSynchronize(c.stream));
+        return;
+    }
+    hipError_t e;
+    while ((e = hipStreamQuery(c.stream)) == hipErrorNotReady) {
+    }
+    HIPCHECK(e);
+}
+
+void event_wait(Ctx& c, hipEvent_t ev) {
+    if (!c.spin) {
+        HIPCHECK(hipEventSynchronize(ev));
+        return;
+    }
+    hipError_t e;
+    while ((e = hipEventQuery(ev)) == hipErrorNotReady) {
+    }
+    HIPCHECK(e);
+}
+
+__global__ void k_zero(double* __restrict__ p, long long count) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = 0.0;
+}
+
+// The device work of one evaluation: zero what the evaluation does not
+// write, the force (and density) kernels, the counters to the host slot.
+// Fill kernels rather than memsets, so that the sequence can be captured.
+static void compute_sequence(Batch& b, const Pot& p, int what, int prec) {
+    hipStream_t st = b.ctx->stream;
+    StepArgs S{0.0, -1, MODE_COMPUTE, what & (W_ENERGY | W_VIRIAL), 1};
+    // the reference's prepare zeroes Ep, Em and V and only virials() fills
+    // them (solver/single.cxx:38-53)
+    if (p.type != PT_BPNN && !(what & W_ENERGY))
+        hipLaunchKernelGGL(k_zero, dim3(nblk(2LL * b.n, 256)), dim3(256), 0, st, b.d(b.sc, D_EP), 2LL * b.n);
+    if (!(what & W_VIRIAL))
+        hipLaunchKernelGGL(k_zero, dim3(nblk(6LL * b.n, 256)), dim3(256), 0, st, b.d(b.sc, D_VXX), 6LL * b.n);
+    launch_step_kernels(b, p, S, prec, b.pos[b.cur], nullptr);
+    langevin_apply(b, prec);
+    HIPCHECK(hipMemcpyAsync(b.ctx->h_pin, b.misc, sizeof(int) * M_COUNT, hipMemcpyDeviceToHost, st));
+}
+
+void graph_release(Batch& b) {
+    if (b.gexec) (void)hipGraphExecDestroy(b.gexec);
+    b.gexec = nullptr;
+    b.gkey_set = false;
+}
+
+// Graph mode: replay the captured evaluation of this configuration, or
+// capture it if the last call ran the same configuration eagerly, or run a
+// new configuration eagerly and remember it.
+static void compute_graph(Batch& b, const Pot& p, int what, int prec) {
+    Batch::GraphKey k;
+    k.pot = &p;
+    k.what = what;
+    k.prec = prec;
+    k.rebuilds = b.rebuilds;
+    k.cap = b.cap;
+    k.live = b.live;
+    k.tpa = b.tpa;
+    k.block = b.block;
+    k.cur = b.cur;
+    k.sc = b.sc;
+    k.alloc_gen = b.alloc_gen;
+    hipStream_t st = b.ctx->stream;
+    if (b.gkey_set && b.gkey == k) {
+        if (!b.gexec) {
+            hipGraph_t g = nullptr;
+            HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            try {
+                compute_sequence(b, p, what, prec);
+            } catch (...) {
+                (void)hipStreamEndCapture(st, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            HIPCHECK(hipStreamEndCapture(st, &g));
+            const hipError_t e = hipGraphInstantiate(&b.gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                b.gexec = nullptr;
+                throw Error(E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+            }
+            b.graph_captures++;
+        }
+        HIPCHECK(hipGraphLaunch(b.gexec, st));
+        b.graph_launches++;
+        return;
+    }
+    // a new configuration: drop the old graph; this call runs eagerly and
+    // its configuration (after any allocation it makes) is the next key
+    graph_release(b);
+    compute_sequence(b, p, what, prec);
+    k.alloc_gen = b.alloc_gen;
+    k.cap = b.cap;
+    b.gkey = k;
+    b.gkey_set = true;
+}
+
 void force_compute(Batch& b, const Pot& p, int what, int prec) {
     if (b.n == 0) return;
     set_trigger(b, p.rcut, false);
     if (!b.list_valid) nlist_rebuild(b);
-    StepArgs S{0.0, -1, MODE_COMPUTE, what & (W_ENERGY | W_VIRIAL), 1};
-    // the reference's prepare zeroes Ep, Em and V and only virials() fills
-    // them (solver/single.cxx:38-53)
-    if (p.type != PT_BPNN && !(what & W_ENERGY)) {
-        HIPCHECK(hipMemsetAsync(b.d(b.sc, D_EP), 0, sizeof(double) * b.n, b.ctx->stream));
-        HIPCHECK(hipMemsetAsync(b.d(b.sc, D_EM), 0, sizeof(double) * b.n, b.ctx->stream));
-    }
-    if (!(what & W_VIRIAL)) HIPCHECK(hipMemsetAsync(b.d(b.sc, D_VXX), 0, sizeof(double) * 6 * b.n, b.ctx->stream));
-    launch_step_kernels(b, p, S, prec, b.pos[b.cur], nullptr);
-    langevin_apply(b, prec);
+    // graphs hold the thermostat's random-stream counter and the arrays
+    // they were captured with: eager with thermostats, timing or a borrowed
+    // work area
+    if (b.graphs && b.therm.count == 0 && !b.timing && !b.bp_fixed) compute_graph(b, p, what, prec);
+    else compute_sequence(b, p, what, prec);
     b.f_prec = prec;
     b.last_rcut = p.rcut;
-    read_misc(b);
+    stream_wait(*b.ctx);
+    timer_drain(b);
     check_errors(b);
 }
 
@@ -1187,7 +1285,7 @@ void run_steps(Batch& b, const Pot& p, double dt, int nsteps, int prec, int chun
         const Chunk P = q[head];
         head = (head + 1) % 2;
         --nq;
-        HIPCHECK(hipEventSynchronize(b.ctx->chunk_ev[P.slot]));
+        event_wait(*b.ctx, b.ctx->chunk_ev[P.slot]);
         const int* cnt = b.ctx->h_pin + M_COUNT * (1 + P.slot);
         check_errors(b, cnt);
         const int ab = cnt[M_ABORT];

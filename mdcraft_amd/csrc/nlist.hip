@@ -495,6 +495,7 @@ static void grow_list(Batch& b, int maxnn) {
     b.cap = cap;
     b.M = cap / b.tpa;
     b.nl_elems = elems;
+    b.alloc_gen++;
 }
 
 static void launch_build(Batch& b) {
@@ -557,7 +558,7 @@ void nlist_rebuild(Batch& b, bool sync) {
     launch_build(b);
     if (sync) {
         HIPCHECK(hipMemcpyAsync(b.ctx->h_pin, b.misc, sizeof(int) * M_COUNT, hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipStreamSynchronize(st));
+        stream_wait(*b.ctx);
         const int maxnn = b.ctx->h_pin[M_MAXNN];  // zero unless some atom overflowed
         if (maxnn > b.cap) nlist_regrow(b, maxnn);
     }

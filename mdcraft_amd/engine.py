@@ -33,7 +33,8 @@ EXPORTS = [
     "mdc_pot_alloy_create", "mdc_pot_alloy_create_file",
     "mdc_pot_bpnn_create", "mdc_pot_bpnn_describe", "mdc_pot_rcut", "mdc_pot_destroy",
     "mdc_sys_create", "mdc_sys_destroy", "mdc_sys_upload", "mdc_sys_download", "mdc_sys_download_range", "mdc_sys_set_active", "mdc_sys_natoms",
-    "mdc_sys_set_lanes", "mdc_sys_autotune", "mdc_sys_time_kernels", "mdc_sys_set_rebuild", "mdc_sys_set_boundary", "mdc_sys_set_thermostat",
+    "mdc_sys_set_lanes", "mdc_sys_autotune", "mdc_sys_set_block", "mdc_sys_autotune_tiles", "mdc_sys_set_graphs",
+    "mdc_sys_graph_stats", "mdc_ctx_set_spin", "mdc_sys_time_kernels", "mdc_sys_set_rebuild", "mdc_sys_set_boundary", "mdc_sys_set_thermostat",
     "mdc_nlist_build", "mdc_nlist_decide", "mdc_nlist_optimize_rns", "mdc_nlist_export", "mdc_nlist_stats",
     "mdc_compute", "mdc_compute_noalloc", "mdc_stage", "mdc_step", "mdc_thermo", "mdc_sync", "mdc_timing", "mdc_bpnn_inspect",
     "mdc_copy_calibration", "mdc_sys_refill_box",
@@ -79,6 +80,11 @@ def load_library(path=LIB_PATH):
         "mdc_sys_natoms": (I, [P, C.POINTER(U64)]),
         "mdc_sys_set_lanes": (I, [P, I]),
         "mdc_sys_autotune": (I, [P, P, I, I, C.POINTER(I), DP]),
+        "mdc_sys_set_block": (I, [P, I]),
+        "mdc_sys_autotune_tiles": (I, [P, P, I, I, C.POINTER(I), C.POINTER(I), DP]),
+        "mdc_sys_set_graphs": (I, [P, I]),
+        "mdc_ctx_set_spin": (I, [P, I]),
+        "mdc_sys_graph_stats": (I, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
         "mdc_sys_time_kernels": (I, [P, P, I, D, I, DP]),
         "mdc_sys_set_rebuild": (I, [P, I]),
         "mdc_sys_set_boundary": (I, [P, I]),
@@ -141,6 +147,10 @@ class Context:
         _check(lib().mdc_ctx_create(device, C.byref(h)))
         self.h = h
         self.device = device
+
+    def set_spin(self, spin):
+        """Wait for the device by polling (True) or by blocking (False)."""
+        _check(lib().mdc_ctx_set_spin(self.h, int(bool(spin))))
 
     def close(self):
         if getattr(self, "h", None):
@@ -310,6 +320,30 @@ class System:
         us = np.zeros(5)
         _check(lib().mdc_sys_autotune(self.h, pot.h, precision, reps, C.byref(best), _dp(us)))
         return best.value, dict(zip((1, 2, 4, 8, 16), us.tolist()))
+
+    def set_block(self, threads):
+        """Threads per workgroup (64, 128, 256) of the LJ and EAM kernels."""
+        _check(lib().mdc_sys_set_block(self.h, threads))
+
+    def autotune_tiles(self, pot, precision, reps=5):
+        """Time the step kernels for every lanes-per-atom x threads-per-
+        workgroup tile and keep the fastest; returns (lanes, threads,
+        {(lanes, threads): microseconds})."""
+        bl, bt = C.c_int(0), C.c_int(0)
+        us = np.zeros(15)
+        _check(lib().mdc_sys_autotune_tiles(self.h, pot.h, precision, reps, C.byref(bl), C.byref(bt), _dp(us)))
+        tiles = [(l, t) for l in (1, 2, 4, 8, 16) for t in (64, 128, 256)]
+        return bl.value, bt.value, dict(zip(tiles, us.tolist()))
+
+    def set_graphs(self, on):
+        """Graph mode of compute(): each configuration's evaluation captured
+        once and replayed."""
+        _check(lib().mdc_sys_set_graphs(self.h, int(bool(on))))
+
+    def graph_stats(self):
+        cap, lau = C.c_uint64(), C.c_uint64()
+        _check(lib().mdc_sys_graph_stats(self.h, C.byref(cap), C.byref(lau)))
+        return {"captures": cap.value, "launches": lau.value}
 
     def time_kernels(self, pot, precision, dt, reps=100):
         """Mean step-mode (density, force) kernel durations in microseconds,
