@@ -42,6 +42,22 @@ namespace mdc {
 
 constexpr int kBpMaxR = 16, kBpMaxA = 64, kBpMaxEta = 8, kBpMaxHidden = 64, kBpMaxIn = 64;
 constexpr int kBpMaxG = 8;  // distinct (lambda, zeta) pairs of the G4 terms
+// Ladder layout: every G4 term has lambda = +-1 and zeta = 2^b, b < 5.  The
+// groups are then g = 2 b + (lambda < 0), all 2 lz of them (lz = 1 + the
+// largest b), and (1 + lambda cos)^zeta comes from repeated squaring of
+// 1 + cos and 1 - cos without a per-group power.
+constexpr int kBpMaxLadder = 5;
+constexpr int kBpGridG = 2 * kBpMaxLadder;  // group rows of the term grid
+
+// Rungs of the ladder kernels that serve nr radial terms, ne G4 etas and
+// zeta up to 2^(lz - 1) (rungs without terms are summed and dropped), or 0
+// when no ladder kernel does: 8 G2 with 2 etas to zeta 4 (the 16-input Cu
+// model) and 8 G2 with 7 etas to zeta 16 (the 51-input Cu model).
+inline int ladder_rungs(int nr, int ne, int lz) {
+    if (nr == 8 && ne == 2 && lz <= 3) return 3;
+    if (nr == 8 && ne == 7 && lz <= 5) return 5;
+    return 0;
+}
 
 struct BpDesc {
     int nr, na, nin, h1, h2, ne, ng;
@@ -54,10 +70,11 @@ struct BpDesc {
     // G4 terms on a (group, eta) grid: group g has lambda g_lam, zeta g_zeta
     // (g_iz = integer zeta or -1); grid term t = g * kBpMaxEta + e is angular
     // input t_q[t] (-1: none) with coefficient t_coef[t]
-    double g_lam[kBpMaxG], g_zeta[kBpMaxG], g_zl[kBpMaxG];
-    int g_iz[kBpMaxG];
-    int t_q[kBpMaxG * kBpMaxEta];
-    double t_coef[kBpMaxG * kBpMaxEta];
+    double g_lam[kBpGridG], g_zeta[kBpGridG], g_zl[kBpGridG];
+    int g_iz[kBpGridG];
+    int t_q[kBpGridG * kBpMaxEta];
+    double t_coef[kBpGridG * kBpMaxEta];
+    int lz;  // ladder layout: the number of zeta rungs; 0: groups in order of appearance
 };
 
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
@@ -266,7 +283,8 @@ __device__ int bp_stage(int i, const double4& pi, const SysDesc& s, const int* _
 // pass 1: descriptors.  Per pair of staged neighbours the G4 factor
 // (1 + lambda cos)^zeta is formed once per (lambda, zeta) group and shared by
 // the group's eta terms; NR, NG, NE bound the radial, group and eta counts.
-template <typename R, int NR, int NG, int NE>
+// LZ > 0: the ladder layout with LZ rungs (NG = 2 LZ), NR and NE exact.
+template <typename R, int NR, int NG, int NE, int LZ = 0>
 __global__ void __launch_bounds__(256) k_bp_desc(const double4* __restrict__ pos, const int* __restrict__ nl,
                                                  const int* __restrict__ nn, int tpa, int M, int cap,
                                                  const SysDesc* __restrict__ desc, const BpDesc* __restrict__ Pd,
@@ -304,7 +322,7 @@ __global__ void __launch_bounds__(256) k_bp_desc(const double4* __restrict__ pos
         const R d = S.d[a], fc = S.fc[a];
 #pragma unroll
         for (int r = 0; r < NR; ++r)
-            if (r < nr) acc[r] += dexp_(-eta_r[r] * (d - Rs) * (d - Rs)) * fc;
+            if (LZ > 0 || r < nr) acc[r] += dexp_(-eta_r[r] * (d - Rs) * (d - Rs)) * fc;
     }
     // pairs a < b over the staged list, flattened and strided over lanes
     const R Rc2 = (R)P.Rc2, pirc = (R)P.pi_rc;
@@ -320,14 +338,31 @@ __global__ void __launch_bounds__(256) k_bp_desc(const double4* __restrict__ pos
         R Ex[NE];
 #pragma unroll
         for (int e = 0; e < NE; ++e)
-            Ex[e] = e < ne ? S.E[e * cap + a] * S.E[e * cap + b] * dexp_(-eta_e[e] * d2c) * F : (R)0;
+            Ex[e] = (LZ > 0 || e < ne) ? S.E[e * cap + a] * S.E[e * cap + b] * dexp_(-eta_e[e] * d2c) * F : (R)0;
+        if constexpr (LZ > 0) {
+            // rung b: (1 + cos)^(2^b) and (1 - cos)^(2^b) by squaring
+            R pp = (R)1 + cs, pm = (R)1 - cs;
 #pragma unroll
-        for (int g = 0; g < NG; ++g)
-            if (g < ng) {
-                const R pz = powu_((R)1 + g_lam[g] * cs, g_iz[g]);
+            for (int b = 0; b < LZ; ++b) {
 #pragma unroll
-                for (int e = 0; e < NE; ++e) acc[NR + g * NE + e] += pz * Ex[e];
+                for (int e = 0; e < NE; ++e) {
+                    acc[NR + (2 * b) * NE + e] += pp * Ex[e];
+                    acc[NR + (2 * b + 1) * NE + e] += pm * Ex[e];
+                }
+                if (b + 1 < LZ) {
+                    pp *= pp;
+                    pm *= pm;
+                }
             }
+        } else {
+#pragma unroll
+            for (int g = 0; g < NG; ++g)
+                if (g < ng) {
+                    const R pz = powu_((R)1 + g_lam[g] * cs, g_iz[g]);
+#pragma unroll
+                    for (int e = 0; e < NE; ++e) acc[NR + g * NE + e] += pz * Ex[e];
+                }
+        }
     }
 #pragma unroll
     for (int q = 0; q < NR + NG * NE; ++q) acc[q] = wave_sum(acc[q]);
@@ -523,7 +558,58 @@ __global__ void __launch_bounds__(64) k_bp_mlp_mfma(const float* __restrict__ G,
 // (PV, gathered like the force shares).  Its angular term on atom i is the
 // centre's own dE/dG contracted with the pattern_dG4_s_ stress of
 // tools/bpnn/base.h:157-181 over the pairs of i's neighbours (OV).
-template <typename R, int NR, int NG, int NE, bool VIR>
+// The pair's sums over the G4 groups of the gradient pass: with c_q =
+// dE/dG_q coef_q and Ex_e = E_e(a) E_e(b) exp(-eta_e d_c^2),
+// SA = sum_q c_q Ex pz_q, SAe = sum_q c_q eta Ex pz_q and
+// SZ = sum_q c_q Ex zeta lam pz1_q, where pz = (1 + lam cos)^zeta and pz1 =
+// (1 + lam cos)^(zeta - 1).  LZ > 0 (ladder layout): rung b holds
+// (1 +- cos)^(2^b) and (1 +- cos)^(2^b - 1) by squaring and running products.
+template <typename R, int NG, int NE, int LZ>
+__device__ inline void group_sums(R cs, const R* C, const R* Ex, const R* Exe, const R* g_lam, const R* g_zl,
+                                  const int* g_iz, int ng, R& SA, R& SAe, R& SZ) {
+    if constexpr (LZ > 0) {
+        R Pp = (R)1 + cs, Pm = (R)1 - cs, Qp = (R)1, Qm = (R)1;
+#pragma unroll
+        for (int b = 0; b < LZ; ++b) {
+            R Kp = 0, Kpe = 0, Km = 0, Kme = 0;
+#pragma unroll
+            for (int e = 0; e < NE; ++e) {
+                Kp += C[(2 * b) * NE + e] * Ex[e];
+                Kpe += C[(2 * b) * NE + e] * Exe[e];
+                Km += C[(2 * b + 1) * NE + e] * Ex[e];
+                Kme += C[(2 * b + 1) * NE + e] * Exe[e];
+            }
+            SA += Kp * Pp + Km * Pm;
+            SAe += Kpe * Pp + Kme * Pm;
+            SZ += (R)(1 << b) * (Kp * Qp - Km * Qm);
+            if (b + 1 < LZ) {
+                Qp *= Pp;
+                Pp *= Pp;
+                Qm *= Pm;
+                Pm *= Pm;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int g = 0; g < NG; ++g)
+            if (g < ng) {
+                const R p1 = (R)1 + g_lam[g] * cs;
+                const R pz1 = powu_(p1, g_iz[g] - 1);
+                R K = 0, Ke = 0;
+#pragma unroll
+                for (int e = 0; e < NE; ++e) {
+                    K += C[g * NE + e] * Ex[e];
+                    Ke += C[g * NE + e] * Exe[e];
+                }
+                const R pz = pz1 * p1;
+                SA += K * pz;
+                SAe += Ke * pz;
+                SZ += g_zl[g] * K * pz1;
+            }
+    }
+}
+
+template <typename R, int NR, int NG, int NE, bool VIR, int LZ = 0>
 __global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos, const int* __restrict__ nl,
                                                  const int* __restrict__ nn, int tpa, int M, int cap,
                                                  const SysDesc* __restrict__ desc, const BpDesc* __restrict__ Pd,
@@ -574,7 +660,7 @@ __global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos
         R sr = 0;
 #pragma unroll
         for (int r = 0; r < NR; ++r)
-            if (r < nr) {
+            if (LZ > 0 || r < nr) {
                 const R eta = eta_r[r];
                 sr += w[r] * dexp_(-eta * (da - Rs) * (da - Rs)) * ((R)-2 * eta * (da - Rs) * fa + dfa);
             }
@@ -659,26 +745,11 @@ __global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos
                 R Ex[NE], Exe[NE];
 #pragma unroll
                 for (int e = 0; e < NE; ++e) {
-                    Ex[e] = e < ne ? Ea[e] * S.E[e * cap + b] * dexp_(-eta_e[e] * d2c) : (R)0;
+                    Ex[e] = (LZ > 0 || e < ne) ? Ea[e] * S.E[e * cap + b] * dexp_(-eta_e[e] * d2c) : (R)0;
                     Exe[e] = Ex[e] * eta_e[e];
                 }
                 R SA = 0, SAe = 0, SZ = 0;
-#pragma unroll
-                for (int g = 0; g < NG; ++g)
-                    if (g < ng) {
-                        const R p1 = (R)1 + g_lam[g] * cs;
-                        const R pz1 = powu_(p1, g_iz[g] - 1);
-                        R K = 0, Ke = 0;
-#pragma unroll
-                        for (int e = 0; e < NE; ++e) {
-                            K += C[g * NE + e] * Ex[e];
-                            Ke += C[g * NE + e] * Exe[e];
-                        }
-                        const R pz = pz1 * p1;
-                        SA += K * pz;
-                        SAe += Ke * pz;
-                        SZ += g_zl[g] * K * pz1;
-                    }
+                group_sums<R, NG, NE, LZ>(cs, C, Ex, Exe, g_lam, g_zl, g_iz, ng, SA, SAe, SZ);
                 SZ *= F;
                 const R cc = ((R)2 * F * dc * SAe - fa * fb * dfcc * SA) * ic;
                 const R ka = (-SZ * cs * ia - (R)2 * F * da * SAe + fb * dfa * fcc * SA) * ia;
@@ -725,26 +796,11 @@ __global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos
         R Ex[NE], Exe[NE];
 #pragma unroll
         for (int e = 0; e < NE; ++e) {
-            Ex[e] = e < ne ? S.E[e * cap + a] * S.E[e * cap + b] * dexp_(-eta_e[e] * d2c) : (R)0;
+            Ex[e] = (LZ > 0 || e < ne) ? S.E[e * cap + a] * S.E[e * cap + b] * dexp_(-eta_e[e] * d2c) : (R)0;
             Exe[e] = Ex[e] * eta_e[e];
         }
         R SA = 0, SAe = 0, SZ = 0;
-#pragma unroll
-        for (int g = 0; g < NG; ++g)
-            if (g < ng) {
-                const R p1 = (R)1 + g_lam[g] * cs;
-                const R pz1 = powu_(p1, g_iz[g] - 1);
-                R K = 0, Ke = 0;
-#pragma unroll
-                for (int e = 0; e < NE; ++e) {
-                    K += C[g * NE + e] * Ex[e];
-                    Ke += C[g * NE + e] * Exe[e];
-                }
-                const R pz = pz1 * p1;
-                SA += K * pz;
-                SAe += Ke * pz;
-                SZ += g_zl[g] * K * pz1;
-            }
+        group_sums<R, NG, NE, LZ>(cs, C, Ex, Exe, g_lam, g_zl, g_iz, ng, SA, SAe, SZ);
         SZ *= F;
         const R cc = ((R)2 * F * dc * SAe - fa * fb * dfcc * SA) * ic;
         const R ka = (-SZ * cs * ia - (R)2 * F * da * SAe + fb * dfa * fcc * SA) * ia;
@@ -1099,14 +1155,42 @@ static std::unique_ptr<BpnnParams> parse_model(const char* path, std::vector<dou
     for (int e = 0; e < D.ne; ++e) D.eta_e[e] = etas[e];
     // G4 terms by (lambda, zeta) group and eta
     D.ng = 0;
-    for (int t = 0; t < kBpMaxG * kBpMaxEta; ++t) {
+    for (int t = 0; t < kBpGridG * kBpMaxEta; ++t) {
         D.t_q[t] = -1;
         D.t_coef[t] = 0.0;
     }
+    // the ladder layout when every term allows it
+    int zb_max = -1;
+    bool ladder = D.na > 0;
+    auto zeta_bit = [](int iz) {
+        for (int b = 0; b < kBpMaxLadder; ++b)
+            if (iz == (1 << b)) return b;
+        return -1;
+    };
+    for (int q = 0; q < D.na; ++q) {
+        const int b = zeta_bit(D.a_iz[q]);
+        ladder = ladder && b >= 0 && (D.a_lam[q] == 1.0 || D.a_lam[q] == -1.0);
+        zb_max = std::max(zb_max, b);
+    }
+    D.lz = ladder ? ladder_rungs(D.nr, D.ne, zb_max + 1) : 0;
+    ladder = D.lz > 0;
+    if (ladder) {
+        D.ng = 2 * D.lz;
+        for (int g = 0; g < D.ng; ++g) {
+            D.g_lam[g] = (g & 1) ? -1.0 : 1.0;
+            D.g_iz[g] = 1 << (g >> 1);
+            D.g_zeta[g] = D.g_iz[g];
+            D.g_zl[g] = D.g_zeta[g] * D.g_lam[g];
+        }
+    }
     for (int q = 0; q < D.na; ++q) {
         int g = -1;
-        for (int h = 0; h < D.ng; ++h)
-            if (D.g_lam[h] == D.a_lam[q] && D.g_zeta[h] == D.a_zeta[q]) g = h;
+        if (ladder) {
+            g = 2 * zeta_bit(D.a_iz[q]) + (D.a_lam[q] < 0 ? 1 : 0);
+        } else {
+            for (int h = 0; h < D.ng; ++h)
+                if (D.g_lam[h] == D.a_lam[q] && D.g_zeta[h] == D.a_zeta[q]) g = h;
+        }
         if (g < 0) {
             if (D.ng >= kBpMaxG) throw Error(E_INVAL, "at most 8 distinct (lambda, zeta) pairs among the G4 terms");
             g = D.ng++;
@@ -1243,7 +1327,14 @@ void bpnn_describe(const char* path, int* arch, int* counts, double* rcut) {
         counts[0] = D.nr;
         counts[1] = D.na;
         counts[2] = D.ne;
-        counts[3] = D.ng;
+        // distinct (lambda, zeta) pairs, whatever the kernels' group layout
+        int distinct = 0;
+        for (int q = 0; q < D.na; ++q) {
+            bool seen = false;
+            for (int r = 0; r < q; ++r) seen = seen || (D.a_lam[r] == D.a_lam[q] && D.a_zeta[r] == D.a_zeta[q]);
+            distinct += seen ? 0 : 1;
+        }
+        counts[3] = distinct;
     }
     if (rcut) *rcut = D.Rc;
 }
@@ -1333,10 +1424,16 @@ static void bpnn_run(Batch& b, const Pot& p, const BpBufs& W, bool rev_fresh, in
         const dim3 gw((n + waves - 1) / waves), bw(64 * waves);
         // register arrays sized for the model: radial terms, (lambda, zeta)
         // groups and G4 eta values
-        const int shape = (bp.h.nr <= 8 && bp.h.ng <= 4 && bp.h.ne <= 2) ? 0 : (bp.h.nr <= 8 ? 1 : 2);
-        auto kdesc = shape == 0 ? k_bp_desc<R, 8, 4, 2>
-                                : shape == 1 ? k_bp_desc<R, 8, kBpMaxG, kBpMaxEta>
-                                             : k_bp_desc<R, kBpMaxR, kBpMaxG, kBpMaxEta>;
+        // 3, 4: the ladder kernels; 0-2: groups in order of appearance
+        const int shape = bp.h.lz == 3 ? 3
+                          : bp.h.lz == 5 ? 4
+                          : (bp.h.nr <= 8 && bp.h.ng <= 4 && bp.h.ne <= 2) ? 0
+                          : (bp.h.nr <= 8 ? 1 : 2);
+        auto kdesc = shape == 3 ? k_bp_desc<R, 8, 6, 2, 3>
+                     : shape == 4 ? k_bp_desc<R, 8, 10, 7, 5>
+                     : shape == 0 ? k_bp_desc<R, 8, 4, 2>
+                     : shape == 1 ? k_bp_desc<R, 8, kBpMaxG, kBpMaxEta>
+                                  : k_bp_desc<R, kBpMaxR, kBpMaxG, kBpMaxEta>;
         if (parts & 1) {
             launch_k(b, K_BP_DESC, kdesc, gw, bw, (unsigned)lds_desc, pos, b.nl, b.nn, b.tpa, b.M, b.cap, b.d_desc,
                      bp.d, (R*)W.G, ldg, n, S, b.misc);
@@ -1353,9 +1450,11 @@ static void bpnn_run(Batch& b, const Pot& p, const BpBufs& W, bool rev_fresh, in
         if (!(parts & 2)) return;
         auto grad = [&](auto vtag) {
             constexpr bool V = decltype(vtag)::value;
-            auto kgrad = shape == 0 ? k_bp_grad<R, 8, 4, 2, V>
-                                    : shape == 1 ? k_bp_grad<R, 8, kBpMaxG, kBpMaxEta, V>
-                                                 : k_bp_grad<R, kBpMaxR, kBpMaxG, kBpMaxEta, V>;
+            auto kgrad = shape == 3 ? k_bp_grad<R, 8, 6, 2, V, 3>
+                         : shape == 4 ? k_bp_grad<R, 8, 10, 7, V, 5>
+                         : shape == 0 ? k_bp_grad<R, 8, 4, 2, V>
+                         : shape == 1 ? k_bp_grad<R, 8, kBpMaxG, kBpMaxEta, V>
+                                      : k_bp_grad<R, kBpMaxR, kBpMaxG, kBpMaxEta, V>;
             launch_k(b, K_BP_FORCE, kgrad, gw, bw, (unsigned)lds, pos, b.nl, b.nn, b.tpa, b.M, b.cap, b.d_desc, bp.d,
                      (const R*)W.w, ldg, (R*)W.P, (R*)W.own, (R*)W.PV, (R*)W.OV, n, S, b.misc);
         };
