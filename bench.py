@@ -87,6 +87,63 @@ def make_pot(E, ctx, pot):
 KINDS = ["force", "density", "predrift", "bp_desc", "bp_mlp", "bp_force", "bp_gather", "rebuild"]
 MLP_FLOP_PER_ATOM = 2 * (16 * 30 + 30 * 30 + 30) + 2 * (30 + 30 * 30 + 30 * 16)  # forward + input gradient
 MFMA_FP16_PEAK_TFLOPS = 2500.0
+# FP32 vector peak (published), and the rate without packed FP32 math,
+# which these builds leave out
+FP32_VALU_PEAK_TFLOPS = 157.3
+FP32_VALU_NONPACKED_TFLOPS = 78.65
+
+
+def bpnn_flops(nr=8, ne=2, lz=3):
+    """FP32 FLOPs of the descriptor and force passes per unit, counted from
+    their formulas (a transcendental counts one, an FMA two): per neighbour
+    in the cutoff (radial terms) and per neighbour pair in the cutoff of the
+    centre and of each other (unordered in the descriptor pass, ordered in
+    the force pass).  The ladder kernels: nr radial terms, ne G4 etas, lz
+    zeta rungs."""
+    geo = 14  # r_b - r_a, |r_c|^2, 1/|r_c|, the cutoff factor
+    desc_pair = geo + 9 + 5 * ne + 2 + 2 * (lz - 1) + 4 * lz * ne
+    grad_pair = geo + 3 + 9 + 6 * ne + lz * (8 * ne + 16) - 4 + 1 + 6 + 9 + 2 + 15
+    return {"desc_neighbour": 6 * nr, "desc_pair": desc_pair, "grad_neighbour": 10 * nr + 6, "grad_pair": grad_pair}
+
+
+def bpnn_units(atoms, box, rc):
+    """Neighbours within rc of each atom, and unordered pairs of them within
+    rc of each other, summed over the atoms (host side, once per workload)."""
+    from scipy.spatial import cKDTree
+    lo, hi = np.asarray(box[0]), np.asarray(box[1])
+    L = hi - lo
+    r = np.mod(_vec(atoms, "r") - lo, L)
+    nb = cKDTree(r, boxsize=L).query_ball_point(r, rc)
+    nrad = npair = 0
+    for i, js in enumerate(nb):
+        js = np.asarray([j for j in js if j != i], dtype=np.int64)
+        d = r[js] - r[i]
+        d -= np.round(d / L) * L
+        d2 = (d * d).sum(1)
+        d = d[(d2 >= 1e-15) & (d2 < rc * rc)]
+        nrad += len(d)
+        dd = ((d[:, None, :] - d[None, :, :]) ** 2).sum(-1)
+        npair += int(np.count_nonzero(np.triu((dd >= 1e-15) & (dd < rc * rc), 1)))
+    return nrad, npair
+
+
+def bpnn_valu(kernel_us, units):
+    """The descriptor and force kernels against the FP32 vector peak."""
+    nrad, npair = units
+    f = bpnn_flops()
+    out = {}
+    for kind, flop in (("bp_desc", f["desc_neighbour"] * nrad + f["desc_pair"] * npair),
+                       ("bp_force", f["grad_neighbour"] * nrad + f["grad_pair"] * 2 * npair)):
+        us = kernel_us.get(kind)
+        if not us:
+            continue
+        tf = flop / (us * 1e-6) / 1e12
+        out[kind] = {"bound": "valu", "algorithmic_flop_per_launch": int(flop), "avg_launch_us": us,
+                     "achieved": round(tf, 2), "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(tf / FP32_VALU_PEAK_TFLOPS, 4),
+                     "frac_of_nonpacked": round(tf / FP32_VALU_NONPACKED_TFLOPS, 4)}
+    out["units"] = {"neighbours": nrad, "neighbour_pairs": npair, "flop_per_unit": f}
+    return out
 
 
 def run_batch(E, ctx, cells, pot, dt, precision, steps, warmup, lanes=None):
@@ -606,6 +663,9 @@ def main():
             wb = W.bpnn_cu()
             out = {p: run_batch(E, ctx, [(wb["atoms"], wb["box"])], wb["pot"], wb["dt"], pr, min(args.steps, 200),
                                 10, args.lanes) for p, pr in precs}
+            if "mixed" in out:
+                out["mixed"]["valu"] = bpnn_valu(out["mixed"]["kernel_us"], bpnn_units(wb["atoms"], wb["box"],
+                                                                                       wb["pot"]["rc"]))
         elif args.only == "eam_sizes":
             out = eam_sizes(E, ctx, W, args.lanes, **({"reps": tuple(args.eam_reps)} if args.eam_reps else {}))
         elif args.only in ("nial_batch", "nial_alloy"):
@@ -671,6 +731,7 @@ def main():
                 tf = MLP_FLOP_PER_ATOM * r["atoms"] / (mlp_us * 1e-6) / 1e12
                 r["mlp_mfma"] = {"algorithmic_TFLOPs": round(tf, 2), "peak_fp16_dense_TFLOPs": MFMA_FP16_PEAK_TFLOPS,
                                  "frac": round(tf / MFMA_FP16_PEAK_TFLOPS, 5), "flop_per_atom": MLP_FLOP_PER_ATOM}
+                r["valu"] = bpnn_valu(r["kernel_us"], bpnn_units(wb["atoms"], wb["box"], wb["pot"]["rc"]))
             extra[name] = r
         extra["served_stream"] = served_stream(E, ctx, W)
         # the headline step at 32 000 to 4.1 million atoms

@@ -162,11 +162,19 @@ __device__ inline R wave_sum(R v) {
     return v;
 }
 
+// The FP64 descriptor pass compacts the neighbour pairs in the cutoff
+// before their arithmetic (about half of all pairs of a centre's
+// neighbours are farther apart than the cutoff): it queues them per wave,
+// kPairQueue entries (up to 63 waiting and 64 new).  In FP32 the queue's
+// own instructions cost about what it saves.
+constexpr int kPairQueue = 2 * kWave;
+
 template <typename R>
 struct Stage {
     R *x, *y, *z, *d, *id, *fc, *dfc, *E;  // E: [ne][cap]; id = 1/d
     R* g;                                  // force-share accumulators [3][cap] (pass 3)
     int* slot;
+    int* queue;  // descriptor pass: in-range pairs a | b << 16
     int cap;
 };
 
@@ -192,16 +200,19 @@ __device__ inline Stage<R> carve(char* base, int cap, int ne, bool grad) {
     S.id = take(1);
     S.E = take(ne);
     S.g = grad ? take(3) : nullptr;
-    S.slot = grad ? reinterpret_cast<int*>(p + (size_t)k * cap) : nullptr;
+    int* tail = reinterpret_cast<int*>(p + (size_t)k * cap);
+    S.slot = grad ? tail : nullptr;
+    S.queue = grad ? nullptr : tail;
     S.cap = cap;
     return S;
 }
 
 __host__ __device__ inline size_t stage_bytes(int cap, int ne, int rsize, bool grad) {
     const int rows = 7 + ne + (grad ? 3 : 0);
+    const size_t queue = grad || rsize != 8 ? 0 : (size_t)kPairQueue * sizeof(int);
     // a multiple of 8 bytes, so that every wavefront's stage starts aligned
     // for FP64 rows whatever the capacity
-    return ((size_t)cap * (rows * rsize + (grad ? 4 : 0)) + 7) & ~(size_t)7;
+    return ((size_t)cap * (rows * rsize + (grad ? 4 : 0)) + queue + 7) & ~(size_t)7;
 }
 
 // First unordered pair (a, b = a + 1 + t) of the flattened pair index t
@@ -324,13 +335,14 @@ __global__ void __launch_bounds__(256) k_bp_desc(const double4* __restrict__ pos
         for (int r = 0; r < NR; ++r)
             if (LZ > 0 || r < nr) acc[r] += dexp_(-eta_r[r] * (d - Rs) * (d - Rs)) * fc;
     }
-    // pairs a < b over the staged list, flattened and strided over lanes
+    // pairs a < b over the staged list: the flattened pair index strided
+    // over the lanes is tested against the cutoff; in FP64 the pairs in
+    // range are queued (ballot and prefix count) and taken 64 at a time, so
+    // that every lane of the arithmetic has a pair
     const R Rc2 = (R)P.Rc2, pirc = (R)P.pi_rc;
-    for (PairWalk pw(nc, lane); pw.valid(); pw.advance()) {
-        const int a = pw.a, b = pw.b();
+    auto pair_terms = [&](int a, int b) {
         const R cx = S.x[b] - S.x[a], cy = S.y[b] - S.y[a], cz = S.z[b] - S.z[a];
         const R d2c = cx * cx + cy * cy + cz * cz;
-        if (!(d2c >= (R)1e-15 && d2c < Rc2)) continue;
         const R dc = d2c * drsq_(d2c);
         const R fcc = (R)0.5 * (dcos_(pirc * dc) + (R)1);
         const R cs = (S.x[a] * S.x[b] + S.y[a] * S.y[b] + S.z[a] * S.z[b]) * S.id[a] * S.id[b];
@@ -362,6 +374,45 @@ __global__ void __launch_bounds__(256) k_bp_desc(const double4* __restrict__ pos
 #pragma unroll
                     for (int e = 0; e < NE; ++e) acc[NR + g * NE + e] += pz * Ex[e];
                 }
+        }
+    };
+    if constexpr (sizeof(R) == 8) {
+        int* Q = S.queue;
+        int qn = 0;  // queued pairs (uniform over the wave)
+        PairWalk pw(nc, lane);
+        while (true) {
+            bool ok = false;
+            int ent = 0;
+            if (pw.valid()) {
+                const int a = pw.a, b = pw.b();
+                const R cx = S.x[b] - S.x[a], cy = S.y[b] - S.y[a], cz = S.z[b] - S.z[a];
+                const R d2c = cx * cx + cy * cy + cz * cz;
+                ok = d2c >= (R)1e-15 && d2c < Rc2;
+                ent = a | (b << 16);
+                pw.advance();
+            }
+            const unsigned long long m = __ballot(ok);
+            if (ok) Q[qn + __popcll(m & ((1ull << lane) - 1ull))] = ent;
+            qn += __popcll(m);
+            const bool more = __ballot(pw.valid()) != 0ull;
+            __builtin_amdgcn_wave_barrier();
+            while (qn >= kWave || (!more && qn > 0)) {
+                const int take = qn < kWave ? qn : kWave;
+                qn -= take;
+                if (lane < take) {
+                    const int e = Q[qn + lane];
+                    pair_terms(e & 0xffff, e >> 16);
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+            if (!more) break;
+        }
+    } else {
+        for (PairWalk pw(nc, lane); pw.valid(); pw.advance()) {
+            const int a = pw.a, b = pw.b();
+            const R cx = S.x[b] - S.x[a], cy = S.y[b] - S.y[a], cz = S.z[b] - S.z[a];
+            const R d2c = cx * cx + cy * cy + cz * cz;
+            if (d2c >= (R)1e-15 && d2c < Rc2) pair_terms(a, b);
         }
     }
 #pragma unroll
