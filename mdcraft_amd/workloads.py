@@ -209,20 +209,24 @@ def eam_cu(reps=20, seed_index=0, T=300.0, rattle=0.03):
 
 
 def _write_maise(path, arch, flat, ranges, b2a, scaling, g2_rows, g4_rows, values, model_id="0000BEEF",
-                 vmax=100.0):
+                 vmax=100.0, species=(("Cu", 29),)):
     """A MAISE-format model file in the layout the reference parser reads
     (mdcraft/tools/parser/maise_model.cxx:29-129): head fields, the input
-    component range block (min, max, range per input), one weight per line,
-    B2A and Scaling, the descriptor map (N GN Rc Rs n1 n2 k z l) and the 7
-    value rows."""
+    component range block (min, max, range per input, species after
+    species), one weight per line (networks species after species), B2A and
+    Scaling, the descriptor map (N GN Rc Rs n1 n2 k z l) and the 7 value
+    rows (written once: the parser gives every species the first species'
+    tables when the file ends, maise_model.cxx:93-98)."""
     bar = "-" * 77
 
     def head(label, value):
         return "|  " + label.ljust(21) + "|  " + str(value).ljust(49) + "|"
 
     lines = [bar, "|" + "neural network general information".center(75) + "|", bar,
-             head("model unique ID", model_id), head("number of species", 1), head("species types", 29),
-             head("species names", "Cu"), head("number of layers", 4), head("architecture", " ".join(map(str, arch))),
+             head("model unique ID", model_id), head("number of species", len(species)),
+             head("species types", " ".join(str(t) for _, t in species)),
+             head("species names", " ".join(n for n, _ in species)), head("number of layers", 4),
+             head("architecture", " ".join(map(str, arch))),
              head("number of weights", len(flat)), head("reference", "synthetic"), bar,
              "|                          input component range                            |", bar,
              "        minimum                    maximum                     range"]
@@ -267,6 +271,61 @@ def write_maise_model(path, seed_index=0, hidden=(30, 30)):
               " n2   2   0.0030000000   0.0300000000", " k    1   1.0000", " z    2   1.0000   4.0000",
               " l    2  -1.0000   1.0000"]
     return _write_maise(path, arch, flat, [1.0] * nin, "1.0", "1.0", range(8), g4, values)
+
+
+def _xavier_net(rng, arch):
+    """Zero biases and Xavier-uniform weights, in the file order of one
+    species (biases of every layer, then the weights of every layer)."""
+    biases = [np.zeros(arch[l + 1]) for l in range(3)]
+    mats = []
+    for l in range(3):
+        lim = math.sqrt(6.0 / (arch[l] + arch[l + 1]))
+        mats.append((rng.uniform(arch[l] * arch[l + 1]) * 2 - 1) * lim)
+    return biases, mats
+
+
+# The 16-input Cu descriptor set (8 G2, 8 G4) as blocks of a two-species
+# model: nspc radial blocks and nspc (nspc + 1) / 2 angular blocks in the
+# order of the reference's shift tables (tools/bpnn/maise_model.cxx:113-131):
+# R(d = 0), A(0), R(1), A(1), A(2) for nspc = 2, d the kind difference and
+# the angular block that of d_ij^2 + d_ik^2.
+BINARY_BLOCKS = (("r", 0), ("a", 0), ("r", 1), ("a", 1), ("a", 2))
+
+
+def write_maise_model_binary(path, seed_index=0, hidden=(30, 30), tied=False, ranges=False):
+    """A two-species MAISE model (Cu and Ag) over the descriptor set of the
+    BPNN workload: 2 x 8 G2 and 3 x 8 G4 inputs, one network per species.
+    tied: both species take the 16-input workload model's network
+    (write_maise_model with the same seed_index) with each W01 column
+    repeated in every block of its kind, which must give the single-species
+    model's results on any structure.  Otherwise each species has its own
+    Xavier network with biases in +-0.1, and ranges gives species 1 input
+    ranges of 0.5 and 2 alternately (species 0 keeps 1)."""
+    nin = 2 * 8 + 3 * 8
+    arch = [nin, hidden[0], hidden[1], 1]
+    flat = []
+    if tied:
+        rng = Rng(structure_seed(3, 1000 + seed_index))
+        b, m = _xavier_net(rng, [16, hidden[0], hidden[1], 1])
+        w01 = m[0].reshape(hidden[0], 16)
+        cols = np.concatenate([w01[:, :8] if kind == "r" else w01[:, 8:] for kind, _ in BINARY_BLOCKS], axis=1)
+        for _ in range(2):
+            flat += b + [cols.reshape(-1), m[1], m[2]]
+    else:
+        for s in range(2):
+            rng = Rng(structure_seed(3, 2000 + 10 * seed_index + s))
+            b, m = _xavier_net(rng, arch)
+            b = [(rng.uniform(len(x)) * 2 - 1) * 0.1 for x in b]
+            flat += b + m
+    flat = np.concatenate(flat)
+    rg = [1.0] * nin + ([0.5 if q % 2 else 2.0 for q in range(nin)] if ranges else [1.0] * nin)
+    g4 = [(ie, iz, il) for ie in range(2) for iz in range(2) for il in range(2)]
+    eta2 = np.exp(np.linspace(np.log(0.003), np.log(0.5), 8))
+    values = [" Rc   1   6.0", " Rs   1   0.0", " n1   8   " + "   ".join("%.10f" % e for e in eta2),
+              " n2   2   0.0030000000   0.0300000000", " k    1   1.0000", " z    2   1.0000   4.0000",
+              " l    2  -1.0000   1.0000"]
+    return _write_maise(path, arch, flat, rg, "1.0", "1.0", range(8), g4, values,
+                        species=(("Cu", 29), ("Ag", 47)))
 
 
 # Descriptor set of the wide BPNN model: one term per (eta, zeta, lambda) on

@@ -1305,6 +1305,11 @@ struct BPNN : Potential {
     };
     std::vector<Ann> anns;
     unsigned nradial = 0, nangular = 0;
+    // kinds and the block offsets of the descriptor vector
+    // (tools/bpnn/maise_model.cxx:113-131): radial block of the kind
+    // difference d, angular block of d_1^2 + d_2^2
+    int nkinds = 1;
+    unsigned shift_r[4] = {0, 0, 0, 0}, shift_a[19] = {};
 
     struct TInfo {
         std::vector<double> g, dedg;
@@ -1318,7 +1323,8 @@ struct BPNN : Potential {
         MaiseModel m;
         m.read(path);
         if (m.NN.size() != 4) throw std::runtime_error("MaiseBPNN: only 4-layer networks are supported");
-        if (m.nspc != 1) throw std::runtime_error("MaiseBPNN restatement supports one species");
+        if (m.nspc < 1 || m.nspc > 4) throw std::runtime_error("MaiseBPNN: 1 to 4 species");
+        nkinds = m.nspc;
         nin = m.NN[0];
         h1 = m.NN[1];
         h2 = m.NN[2];
@@ -1345,7 +1351,26 @@ struct BPNN : Potential {
                 throw std::runtime_error("only G2 radial and G4 angular descriptors are supported");
             }
         }
-        if ((int)(nradial + nangular) < nin) throw std::runtime_error("fewer descriptors than network inputs");
+        {
+            const unsigned R = nradial, A = nangular;
+            shift_r[0] = 0;
+            shift_r[1] = 1 * R + 1 * A;
+            shift_r[2] = 2 * R + 3 * A;
+            shift_r[3] = 3 * R + 6 * A;
+            shift_a[0] = 1 * R;
+            shift_a[1] = 2 * R + 1 * A;
+            shift_a[2] = 2 * R + 2 * A;
+            shift_a[4] = 3 * R + 3 * A;
+            shift_a[8] = 3 * R + 4 * A;
+            shift_a[5] = 3 * R + 5 * A;
+            shift_a[9] = 4 * R + 6 * A;
+            shift_a[18] = 4 * R + 7 * A;
+            shift_a[10] = 4 * R + 8 * A;
+            shift_a[13] = 4 * R + 9 * A;
+            const int K = nkinds;
+            if ((int)(K * R + K * (K + 1) / 2 * A) != nin)
+                throw std::runtime_error("network inputs must be nspc radial and nspc (nspc + 1) / 2 angular blocks");
+        }
         R1cut = Rc;
         R2cut = Rc * Rc;
         // maise_ann.cxx:23-51 weight order: per kind, biases of every layer,
@@ -1425,7 +1450,7 @@ struct BPNN : Potential {
         const auto& nli = L[ii];
         TInfo& T = tinfo[ii];
         const size_t NS = nli.size();
-        const unsigned IN = nradial + nangular;
+        const unsigned IN = (unsigned)nin;
         T.g.assign(IN, 0.0);
         T.dedg.assign(IN, 0.0);
         T.dgdr.assign((NS + 1) * IN * 3, 0.0);
@@ -1437,6 +1462,9 @@ struct BPNN : Potential {
         auto DX = [&](size_t slot, unsigned pos, int c) -> double& { return T.dgdr_X_r[(slot * IN + pos) * 3 + c]; };
         const size_t ss = (size_t)T.self_slot;
         const Atom& ai = atoms[ii];
+        // maise_bpnn.cxx:141, :181, :232: kind differences modulo the kinds
+        const int K = nkinds;
+        auto kdiff = [K](int a, int b) { return ((a - b) % K + K) % K; };
         for (size_t j = 0; j < NS; ++j) {
             size_t jj = nli[j];
             const Atom& aj = atoms[jj];
@@ -1448,9 +1476,10 @@ struct BPNN : Potential {
             if (!(r2ij < R2cut)) continue;
             double rij = std::sqrt(r2ij);
             double drij[3] = {rij_v[0] / rij, rij_v[1] / rij, rij_v[2] / rij};
+            const int kd_ij = kdiff(ai.kind, aj.kind);
             // radial part
             for (unsigned ir = 0; ir < nradial; ++ir) {
-                unsigned pos = ir;
+                unsigned pos = ir + shift_r[kd_ij];
                 T.g[pos] += G2(ir, rij);
                 for (int c = 0; c < 3; ++c) {
                     double add = dG2(ir, rij) * drij[c];
@@ -1476,8 +1505,10 @@ struct BPNN : Potential {
                 double rik = std::sqrt(r2ik);
                 double drik[3] = {rik_v[0] / rik, rik_v[1] / rik, rik_v[2] / rik};
                 double cos_j = -(drij[0] * drjk[0] + drij[1] * drjk[1] + drij[2] * drjk[2]);
+                const int kd_jk = kdiff(ak.kind, aj.kind);
+                const unsigned sa = shift_a[kd_ij * kd_ij + kd_jk * kd_jk];
                 for (unsigned ia = 0; ia < nangular; ++ia) {
-                    unsigned pos = ia + nradial;
+                    unsigned pos = ia + sa;
                     for (int c = 0; c < 3; ++c) {
                         double dcos = rij_v[c] * cos_j / rij / rij + rjk_v[c] / rij / rjk;
                         D(j, pos, c) -= 2.0 * dG4(ia, rij, rik, rjk, cos_j, dcos, -drij[c], -drik[c]);
@@ -1500,8 +1531,10 @@ struct BPNN : Potential {
                 double rik = std::sqrt(r2ik);
                 double drik[3] = {rik_v[0] / rik, rik_v[1] / rik, rik_v[2] / rik};
                 double cos_i = drij[0] * drik[0] + drij[1] * drik[1] + drij[2] * drik[2];
+                const int kd_ik = kdiff(ak.kind, ai.kind);
+                const unsigned sa = shift_a[kd_ij * kd_ij + kd_ik * kd_ik];
                 for (unsigned ia = 0; ia < nangular; ++ia) {
-                    unsigned pos = ia + nradial;
+                    unsigned pos = ia + sa;
                     T.g[pos] += G4v(ia, rij, rik, rjk, cos_i);
                     for (int c = 0; c < 3; ++c) {
                         double dcos = cos_i * (drij[c] / rij + drik[c] / rik) - drij[c] / rik - drik[c] / rij;
@@ -1554,7 +1587,7 @@ struct BPNN : Potential {
     void force_one(size_t ii, Atom& atom) {
         const auto& nli = (*nl)[ii];
         const TInfo& T = tinfo[ii];
-        const unsigned IN = nradial + nangular;
+        const unsigned IN = (unsigned)nin;
         for (size_t j = 0; j < nli.size(); ++j) {
             const TInfo& Tj = tinfo[nli[j]];
             for (unsigned l = 0; l < IN; ++l)
@@ -1565,7 +1598,7 @@ struct BPNN : Potential {
     void virial_one(size_t ii, Atom& atom) {
         const auto& nli = (*nl)[ii];
         const TInfo& T = tinfo[ii];
-        const unsigned IN = nradial + nangular;
+        const unsigned IN = (unsigned)nin;
         for (size_t j = 0; j < nli.size(); ++j) {
             const TInfo& Tj = tinfo[nli[j]];
             for (unsigned l = 0; l < IN; ++l)
