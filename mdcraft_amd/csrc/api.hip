@@ -915,16 +915,18 @@ void optimize_rns(Batch& b, double kbuf, double* rns_out) {
 }
 
 void check_kinds(Batch& b, const Pot& p) {
-    if (p.type != PT_EAM || p.eam.nk <= 1) return;
+    const int limit = p.type == PT_BPNN ? bpnn_species(p) : (p.type == PT_EAM && p.eam.nk > 1 ? p.eam.nk : 0);
+    if (limit == 0) return;
     if (b.kind_max < 0) {
         // recounted once after kinds change, not per evaluation
         int mx = 0;
         for (int o = 0; o < b.n; ++o) mx = std::max(mx, (int)b.kind[o]);
         b.kind_max = mx;
     }
-    if (b.kind_max >= p.eam.nk)
-        throw Error(E_INVAL, p.eam.alloy ? "atom kind outside the alloy's elements"
-                                         : "atom kind outside the Multi potential's kinds");
+    if (b.kind_max >= limit)
+        throw Error(E_INVAL, p.type == PT_BPNN ? "atom kind has no network in the MAISE model"
+                             : p.eam.alloy  ? "atom kind outside the alloy's elements"
+                                            : "atom kind outside the Multi potential's kinds");
 }
 
 }  // namespace

@@ -75,6 +75,11 @@ struct BpDesc {
     int t_q[kBpGridG * kBpMaxEta];
     double t_coef[kBpGridG * kBpMaxEta];
     int lz;  // ladder layout: the number of zeta rungs; 0: groups in order of appearance
+    // species (1 or 2) and the block offsets of the input vector
+    // (tools/bpnn/maise_model.cxx:113-131): radial block sr[d] of the kind
+    // difference d = [k_j != k_i], angular block sa[d_ij + d_ik]
+    int nspc;
+    int sr[2], sa[3];
 };
 
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
@@ -89,6 +94,8 @@ struct BpnnParams {
     int boff[4] = {0, 0, 0, 0};
     int nt[4] = {0, 0, 0, 0}, kt[4] = {0, 0, 0, 0};
     int nin_pad = 0, h1_pad = 0, h2_pad = 0;
+    // elements of one species' block in w64, w32 and bsplit
+    size_t w64_stride = 0, w32_stride = 0, b_stride = 0;
 };
 
 // Work arrays of one BPNN evaluation: descriptors G and dE/dG w [n][ldg],
@@ -237,20 +244,23 @@ struct PairWalk {
 // Stage the in-range neighbours of centre i (1e-15 <= r^2 < Rc^2 under
 // Domain::shortest) into LDS, compacted in list order; out-of-range slots
 // get zero force shares in Pzero and zero virial shares in PVzero when
-// those are given.
+// those are given.  ck >= 0 (two species, ck the centre's kind): the
+// neighbours of kind ck go to the front [0, returned count), the others to
+// the back [cap - nd, cap).
 template <typename R>
 __device__ int bp_stage(int i, const double4& pi, const SysDesc& s, const int* __restrict__ nl, int cnt, int tpa,
                         int M, const double4* __restrict__ pos, const BpDesc& P, const Stage<R>& S,
-                        R* __restrict__ Pzero, R* __restrict__ PVzero, int cap) {
+                        R* __restrict__ Pzero, R* __restrict__ PVzero, int cap, int ck, int& nd) {
     const int lane = threadIdx.x & 63;
-    int base = 0;
+    int base = 0, based = 0;
     for (int k0 = 0; k0 < cnt; k0 += kWave) {
         const int k = k0 + lane;
-        bool ok = false;
+        bool ok = false, other = false;
         double dx = 0, dy = 0, dz = 0, d2 = 0;
         if (k < cnt) {
             const int j = ent_j(nl[list_index(i, k, tpa, M)]);
             const double4 pj = pos[j];
+            other = ck >= 0 && w_kind(pj.w) != ck;
             dx = pj.x - pi.x;
             dy = pj.y - pi.y;
             dz = pj.z - pi.z;
@@ -266,9 +276,10 @@ __device__ int bp_stage(int i, const double4& pi, const SysDesc& s, const int* _
                 q[0] = q[1] = q[2] = (R)0;
             }
         }
-        const unsigned long long m = __ballot(ok);
+        const unsigned long long m = __ballot(ok && !other), md = __ballot(ok && other);
         if (ok) {
-            const int at = base + __popcll(m & ((1ull << lane) - 1ull));
+            const unsigned long long lt = (1ull << lane) - 1ull;
+            const int at = other ? cap - 1 - (based + __popcll(md & lt)) : base + __popcll(m & lt);
             const double d = sqrt(d2);
             S.x[at] = (R)dx;
             S.y[at] = (R)dy;
@@ -284,10 +295,12 @@ __device__ int bp_stage(int i, const double4& pi, const SysDesc& s, const int* _
             if (S.slot) S.slot[at] = k;
         }
         base += __popcll(m);
+        based += __popcll(md);
     }
     // LDS traffic of one wavefront is processed in order; the barrier only
     // stops the compiler from moving the readers above the staging stores
     __builtin_amdgcn_wave_barrier();
+    nd = based;
     return base;
 }
 
@@ -295,7 +308,12 @@ __device__ int bp_stage(int i, const double4& pi, const SysDesc& s, const int* _
 // (1 + lambda cos)^zeta is formed once per (lambda, zeta) group and shared by
 // the group's eta terms; NR, NG, NE bound the radial, group and eta counts.
 // LZ > 0: the ladder layout with LZ rungs (NG = 2 LZ), NR and NE exact.
-template <typename R, int NR, int NG, int NE, int LZ = 0>
+// K2: two species; the neighbours of the centre's kind are staged first
+// and those of the other kind last, the radial terms go to the block of the
+// kind difference and the angular terms of the same/same, same/other and
+// other/other pairs to blocks 0, 1 and 2 (maise_bpnn.cxx:141-257 for two
+// kinds, whose differences are 0 or 1).
+template <typename R, int NR, int NG, int NE, int LZ = 0, bool K2 = false>
 __global__ void __launch_bounds__(256) k_bp_desc(const double4* __restrict__ pos, const int* __restrict__ nl,
                                                  const int* __restrict__ nn, int tpa, int M, int cap,
                                                  const SysDesc* __restrict__ desc, const BpDesc* __restrict__ Pd,
@@ -311,10 +329,22 @@ __global__ void __launch_bounds__(256) k_bp_desc(const double4* __restrict__ pos
     const double4 pi = pos[i];
     const SysDesc& s = desc[w_sys(pi.w)];
     const int cnt = min(nn[i], cap);
-    const int nc = bp_stage<R>(i, pi, s, nl, cnt, tpa, M, pos, P, S, (R*)nullptr, (R*)nullptr, cap);
+    int nd = 0;
+    const int nc = bp_stage<R>(i, pi, s, nl, cnt, tpa, M, pos, P, S, (R*)nullptr, (R*)nullptr, cap,
+                               K2 ? (int)w_kind(pi.w) : -1, nd);
+    const int d0 = cap - nd;  // the other kind's neighbours: [d0, cap)
     R acc[NR + NG * NE];
+    R acc1[K2 ? NR : 1];  // radial block of the other kind
 #pragma unroll
     for (int q = 0; q < NR + NG * NE; ++q) acc[q] = (R)0;
+#pragma unroll
+    for (int q = 0; q < (K2 ? NR : 1); ++q) acc1[q] = (R)0;
+    R* grow = G + (size_t)i * ldg;
+    if constexpr (K2) {
+        // blocks are stored as they complete
+        for (int q = lane; q < ldg; q += kWave) grow[q] = (R)0;
+        __builtin_amdgcn_wave_barrier();
+    }
     // descriptor parameters in registers, in the precision of the pass
     const int nr = P.nr, ng = P.ng, ne = P.ne;
     R eta_r[NR], g_lam[NG], eta_e[NE];
@@ -335,6 +365,13 @@ __global__ void __launch_bounds__(256) k_bp_desc(const double4* __restrict__ pos
         for (int r = 0; r < NR; ++r)
             if (LZ > 0 || r < nr) acc[r] += dexp_(-eta_r[r] * (d - Rs) * (d - Rs)) * fc;
     }
+    if constexpr (K2)
+        for (int a = d0 + lane; a < cap; a += kWave) {
+            const R d = S.d[a], fc = S.fc[a];
+#pragma unroll
+            for (int r = 0; r < NR; ++r)
+                if (LZ > 0 || r < nr) acc1[r] += dexp_(-eta_r[r] * (d - Rs) * (d - Rs)) * fc;
+        }
     // pairs a < b over the staged list: the flattened pair index strided
     // over the lanes is tested against the cutoff; in FP64 the pairs in
     // range are queued (ballot and prefix count) and taken 64 at a time, so
@@ -415,10 +452,59 @@ __global__ void __launch_bounds__(256) k_bp_desc(const double4* __restrict__ pos
             if (d2c >= (R)1e-15 && d2c < Rc2) pair_terms(a, b);
         }
     }
+    if constexpr (K2) {
+        // the angular sums of the pairs walked so far into block `key`
+        auto store_ang = [&](int key) {
+#pragma unroll
+            for (int q = NR; q < NR + NG * NE; ++q) acc[q] = wave_sum(acc[q]);
+            if (lane == 0) {
+#pragma unroll
+                for (int gg = 0; gg < NG; ++gg)
+#pragma unroll
+                    for (int e = 0; e < NE; ++e) {
+                        const int t = gg * kBpMaxEta + e;
+                        if (gg < P.ng && e < P.ne && P.t_q[t] >= 0)
+                            grow[P.sa[key] + P.t_q[t]] = (R)P.t_coef[t] * acc[NR + gg * NE + e];
+                    }
+            }
+#pragma unroll
+            for (int q = NR; q < NR + NG * NE; ++q) acc[q] = (R)0;
+        };
+        auto in_range = [&](int a, int b) {
+            const R cx = S.x[b] - S.x[a], cy = S.y[b] - S.y[a], cz = S.z[b] - S.z[a];
+            const R d2c = cx * cx + cy * cy + cz * cz;
+            return d2c >= (R)1e-15 && d2c < Rc2;
+        };
+        store_ang(0);
+        // same/other pairs: a in [0, nc), b in [d0, cap)
+        for (int t = lane; t < nc * nd; t += kWave) {
+            const int a = t / nd, b = d0 + (t - a * nd);
+            if (in_range(a, b)) pair_terms(a, b);
+        }
+        store_ang(1);
+        for (PairWalk pw(nd, lane); pw.valid(); pw.advance()) {
+            const int a = d0 + pw.a, b = d0 + pw.b();
+            if (in_range(a, b)) pair_terms(a, b);
+        }
+        store_ang(2);
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            acc[r] = wave_sum(acc[r]);
+            acc1[r] = wave_sum(acc1[r]);
+        }
+        if (lane == 0)
+#pragma unroll
+            for (int r = 0; r < NR; ++r)
+                if (r < P.nr) {
+                    grow[P.sr[0] + r] = acc[r];
+                    grow[P.sr[1] + r] = acc1[r];
+                }
+        return;
+    }
 #pragma unroll
     for (int q = 0; q < NR + NG * NE; ++q) acc[q] = wave_sum(acc[q]);
     if (lane == 0) {
-        R* g = G + (size_t)i * ldg;
+        R* g = grow;
         for (int q = 0; q < ldg; ++q) g[q] = (R)0;
 #pragma unroll
         for (int r = 0; r < NR; ++r)
@@ -440,8 +526,9 @@ __global__ void __launch_bounds__(256) k_bp_desc(const double4* __restrict__ pos
 template <int L>
 __global__ void __launch_bounds__(256) k_bp_mlp64(const double* __restrict__ G, int ldg, const double* __restrict__ W,
                                                   const BpDesc* __restrict__ Pd, const double4* __restrict__ pos,
-                                                  double* __restrict__ wout, double* __restrict__ Ep, int n,
+                                                  double* __restrict__ wout, double* __restrict__ Ep, int n, int sp,
                                                   StepArgs SA, int* __restrict__ misc) {
+    // sp >= 0: the network of species sp, for the atoms of that kind only
     if (skip_step(SA, misc)) return;
     const int lane = threadIdx.x & 63;
     const int sub = lane & (L - 1), base = lane - sub;
@@ -457,7 +544,8 @@ __global__ void __launch_bounds__(256) k_bp_mlp64(const double* __restrict__ G, 
     const double b23 = W23[h2];
     const double* W01T = W23 + h2 + 1;  // [nin][h1]
     const double* W12T = W01T + nin * h1;  // [h1][h2]
-    const bool act = valid && !w_inactive(pos[i].w);
+    const bool mine = valid && (sp < 0 || w_kind(pos[i].w) == sp);
+    const bool act = mine && !w_inactive(pos[i].w);
     const double* g = G + (size_t)(valid ? i : 0) * ldg;
     // every lane of the wave takes part in every shuffle below
     double y1 = 0.0;
@@ -489,7 +577,7 @@ __global__ void __launch_bounds__(256) k_bp_mlp64(const double* __restrict__ G, 
         const double dq = __shfl(d1, base + q, kWave);
         if (sub < nin) so += W01[q * nin + sub] * dq;
     }
-    if (!valid) return;
+    if (!mine) return;
     double* wo = wout + (size_t)i * ldg;
     for (int k = sub; k < ldg; k += L) wo[k] = (act && k < nin) ? so * P.escale : 0.0;
     if (act && sub == 0) Ep[i] = (b23 + e) * P.escale;
@@ -541,10 +629,12 @@ __device__ inline void mfma_gemm(const float* As, int lda, const half4_t* __rest
 __global__ void __launch_bounds__(64) k_bp_mlp_mfma(const float* __restrict__ G, int ldg, const half4_t* __restrict__ B,
                                                     int4 boff, int4 nts, int4 kts, const float* __restrict__ w32,
                                                     const BpDesc* __restrict__ Pd, const double4* __restrict__ pos,
-                                                    float* __restrict__ wout, double* __restrict__ Ep, int n,
+                                                    float* __restrict__ wout, double* __restrict__ Ep, int n, int sp,
                                                     StepArgs SA, int* __restrict__ misc) {
+    // sp >= 0: the network of species sp, for the atoms of that kind only
     if (skip_step(SA, misc)) return;
     __shared__ float X[16 * kBpMaxHidden], H1[16 * kBpMaxHidden], H2[16 * kBpMaxHidden], T[16 * kBpMaxHidden];
+    auto mine = [&](int i) { return i < n && (sp < 0 || w_kind(pos[i].w) == sp); };
     const BpDesc& P = *Pd;
     const int lane = threadIdx.x;
     const int a0 = blockIdx.x * 16;
@@ -570,7 +660,7 @@ __global__ void __launch_bounds__(64) k_bp_mlp_mfma(const float* __restrict__ G,
         const int i = a0 + lane;
         float E = b23;
         for (int q = 0; q < h2; ++q) E += W23[q] * H2[lane * kBpMaxHidden + q];
-        if (i < n && !w_inactive(pos[i].w)) Ep[i] = (double)E * P.escale;
+        if (mine(i) && !w_inactive(pos[i].w)) Ep[i] = (double)E * P.escale;
     }
     // backward: D2 = W23 o (1 - h2^2)
     for (int e = lane; e < 16 * h2p; e += 64) {
@@ -594,7 +684,7 @@ __global__ void __launch_bounds__(64) k_bp_mlp_mfma(const float* __restrict__ G,
     for (int e = lane; e < 16 * ldg; e += 64) {
         const int r = e / ldg, c = e % ldg;
         const int i = a0 + r;
-        if (i < n) {
+        if (mine(i)) {
             const bool act = !w_inactive(pos[i].w);
             wout[(size_t)i * ldg + c] =
                 (act && c < nin) ? (float)(X[r * kBpMaxHidden + c] * xs[c] * P.escale) : 0.f;
@@ -660,7 +750,7 @@ __device__ inline void group_sums(R cs, const R* C, const R* Ex, const R* Exe, c
     }
 }
 
-template <typename R, int NR, int NG, int NE, bool VIR, int LZ = 0>
+template <typename R, int NR, int NG, int NE, bool VIR, int LZ = 0, bool K2 = false>
 __global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos, const int* __restrict__ nl,
                                                  const int* __restrict__ nn, int tpa, int M, int cap,
                                                  const SysDesc* __restrict__ desc, const BpDesc* __restrict__ Pd,
@@ -677,7 +767,10 @@ __global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos
     const double4 pi = pos[i];
     const SysDesc& s = desc[w_sys(pi.w)];
     const int cnt = min(nn[i], cap);
-    const int nc = bp_stage<R>(i, pi, s, nl, cnt, tpa, M, pos, P, S, Pout, VIR ? PV : (R*)nullptr, cap);
+    int nd = 0;
+    const int nc = bp_stage<R>(i, pi, s, nl, cnt, tpa, M, pos, P, S, Pout, VIR ? PV : (R*)nullptr, cap,
+                               K2 ? (int)w_kind(pi.w) : -1, nd);
+    const int d0 = cap - nd;  // K2: the other kind's neighbours, [d0, cap)
     const int nr = P.nr, ng = P.ng, ne = P.ne;
     R eta_r[NR], g_lam[NG], g_zl[NG], eta_e[NE];
     int g_iz[NG];
@@ -691,29 +784,37 @@ __global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos
     }
 #pragma unroll
     for (int e = 0; e < NE; ++e) eta_e[e] = e < ne ? (R)P.eta_e[e] : (R)0;
-    R w[NR];
+    R w[NR], w1[K2 ? NR : 1];
 #pragma unroll
-    for (int q = 0; q < NR; ++q) w[q] = q < nr ? wgt[(size_t)i * ldg + q] : (R)0;
-    // dE/dG of the G4 terms times their coefficients on the (group, eta) grid
+    for (int q = 0; q < NR; ++q) w[q] = q < nr ? wgt[(size_t)i * ldg + P.sr[0] + q] : (R)0;
+    if constexpr (K2)
+#pragma unroll
+        for (int q = 0; q < NR; ++q) w1[q] = q < nr ? wgt[(size_t)i * ldg + P.sr[1] + q] : (R)0;
+    // dE/dG of the G4 terms of angular block `key` times their coefficients
+    // on the (group, eta) grid
     R C[NG * NE];
+    auto load_C = [&](int key) {
 #pragma unroll
-    for (int g = 0; g < NG; ++g)
+        for (int g = 0; g < NG; ++g)
 #pragma unroll
-        for (int e = 0; e < NE; ++e) {
-            const int t = g * kBpMaxEta + e;
-            const int q = (g < ng && e < ne) ? P.t_q[t] : -1;
-            C[g * NE + e] = q >= 0 ? wgt[(size_t)i * ldg + nr + q] * (R)P.t_coef[t] : (R)0;
-        }
+            for (int e = 0; e < NE; ++e) {
+                const int t = g * kBpMaxEta + e;
+                const int q = (g < ng && e < ne) ? P.t_q[t] : -1;
+                C[g * NE + e] = q >= 0 ? wgt[(size_t)i * ldg + P.sa[key] + q] * (R)P.t_coef[t] : (R)0;
+            }
+    };
+    load_C(0);
     const R Rs = (R)P.Rs, Rc2 = (R)P.Rc2, pirc = (R)P.pi_rc;
-    // radial part of every slot, owned by one lane
-    for (int a = lane; a < nc; a += kWave) {
+    // radial part of every slot, owned by one lane (K2: the other kind's
+    // neighbours with their block's dE/dG)
+    auto radial = [&](int a, const R* __restrict__ wr) {
         const R ax = S.x[a], ay = S.y[a], az = S.z[a], da = S.d[a], fa = S.fc[a], dfa = S.dfc[a];
         R sr = 0;
 #pragma unroll
         for (int r = 0; r < NR; ++r)
             if (LZ > 0 || r < nr) {
                 const R eta = eta_r[r];
-                sr += w[r] * dexp_(-eta * (da - Rs) * (da - Rs)) * ((R)-2 * eta * (da - Rs) * fa + dfa);
+                sr += wr[r] * dexp_(-eta * (da - Rs) * (da - Rs)) * ((R)-2 * eta * (da - Rs) * fa + dfa);
             }
         const R ia = S.id[a];
         S.g[a] = sr * ax * ia;
@@ -725,7 +826,10 @@ __global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos
             q[1] = sr * ay * ay * ia;
             q[2] = sr * az * az * ia;
         }
-    }
+    };
+    for (int a = lane; a < nc; a += kWave) radial(a, w);
+    if constexpr (K2)
+        for (int a = d0 + lane; a < cap; a += kWave) radial(a, w1);
     __builtin_amdgcn_wave_barrier();
     // angular virial of the centre, per lane: for a pair (a, b) with
     // c = r_b - r_a, per axis (alpha + beta_a) x_a^2 + (alpha + beta_b) x_b^2
@@ -751,11 +855,15 @@ __global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos
     // stays on one row a for long runs: the row's sum collects in registers
     // and goes to the LDS accumulator only when the row changes.  Twice the
     // pair arithmetic of the unordered walk, but almost no LDS atomics.
-    const int row = nc - 1;
-    const int T = nc * row;
+    // rows a in [r0, r0 + nrow), columns b in [c0, c0 + ncol) without b = a
+    // when the two ranges are one (same)
+    auto walk = [&](int r0, int nrow, int c0, int ncol, bool same) {
+    const int row = ncol - (same ? 1 : 0);
+    if (nrow <= 0 || row <= 0) return;
+    const int T = nrow * row;
     const int u0 = (int)(((long long)lane * T) / kWave), u1 = (int)(((long long)(lane + 1) * T) / kWave);
     const int steps = (T + kWave - 1) / kWave;
-    int a = row > 0 ? u0 / row : 0, r = row > 0 ? u0 - a * row : 0;
+    int a = r0 + u0 / row, r = u0 - (u0 / row) * row;
     R ax = 0, ay = 0, az = 0, da = 1, fa = 0, dfa = 0, ia = 1;
     R Ea[NE];
     int cur = -1;
@@ -781,7 +889,7 @@ __global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos
 #pragma unroll
                 for (int e = 0; e < NE; ++e) Ea[e] = e < ne ? S.E[e * cap + a] : (R)0;
             }
-            const int b = r + (r >= a ? 1 : 0);
+            const int b = c0 + r + (same && c0 + r >= a ? 1 : 0);
             const R bx = S.x[b], by = S.y[b], bz = S.z[b];
             const R cx = bx - ax, cy = by - ay, cz = bz - az;
             const R d2c = cx * cx + cy * cy + cz * cz;
@@ -824,19 +932,27 @@ __global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos
         atomicAdd(&S.g[cap + cur], hy);
         atomicAdd(&S.g[2 * cap + cur], hz);
     }
+    };
+    walk(0, nc, 0, nc, true);
+    if constexpr (K2) {
+        load_C(1);
+        walk(0, nc, d0, nd, false);
+        walk(d0, nd, 0, nc, false);
+        load_C(2);
+        walk(d0, nd, d0, nd, true);
+    }
     } else {
     // angular part, FP64: every unordered pair {a, b} once; d/dr_b is d/dr_a
     // with a and b exchanged (u_c changes sign).  Pairs are walked by
     // diagonal, d = b - a, then by a, so the lanes of one step hold
     // consecutive a and consecutive b and the LDS accumulators take both
     // sides without two lanes hitting one address.
-    for (PairWalk pw(nc, lane); pw.valid(); pw.advance()) {
-        const int a = pw.t, b = pw.t + pw.a + 1;
+    auto pair64 = [&](int a, int b) {
         const R ax = S.x[a], ay = S.y[a], az = S.z[a], da = S.d[a];
         const R bx = S.x[b], by = S.y[b], bz = S.z[b], db = S.d[b];
         const R cx = bx - ax, cy = by - ay, cz = bz - az;
         const R d2c = cx * cx + cy * cy + cz * cz;
-        if (!(d2c >= (R)1e-15 && d2c < Rc2)) continue;
+        if (!(d2c >= (R)1e-15 && d2c < Rc2)) return;
         const R fa = S.fc[a], dfa = S.dfc[a], fb = S.fc[b], dfb = S.dfc[b];
         const R ia = S.id[a], ib = S.id[b], ic = drsq_(d2c);
         const R dc = d2c * ic;
@@ -866,11 +982,31 @@ __global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos
         atomicAdd(&S.g[b], la * bx + kb * ax - cc * cx);
         atomicAdd(&S.g[cap + b], la * by + kb * ay - cc * cy);
         atomicAdd(&S.g[2 * cap + b], la * bz + kb * az - cc * cz);
+    };
+    // a triangle [lo, lo + n) by diagonal; a rectangle [la, la + na) x
+    // [lb, lb + nb) by shifted rows (b = (a + d) mod nb), so that the lanes
+    // of one step hold different a and, for na <= nb, different b
+    auto tri = [&](int lo, int n) {
+        for (PairWalk pw(n, lane); pw.valid(); pw.advance()) pair64(lo + pw.t, lo + pw.t + pw.a + 1);
+    };
+    auto rect = [&](int la, int na, int lb, int nb) {
+        for (int t = lane; t < na * nb; t += kWave) {
+            const int d = t / na, a = t - d * na;
+            pair64(la + a, lb + (a + d) % nb);
+        }
+    };
+    tri(0, nc);
+    if constexpr (K2) {
+        load_C(1);
+        rect(0, nc, d0, nd);
+        load_C(2);
+        tri(d0, nd);
     }
     }
     __builtin_amdgcn_wave_barrier();
     R ox = 0, oy = 0, oz = 0;
-    for (int a = lane; a < nc; a += kWave) {
+    for (int a = lane; a < (K2 ? cap : nc); a += kWave) {
+        if (K2 && a >= nc && a < d0) continue;
         const R gx = S.g[a], gy = S.g[cap + a], gz = S.g[2 * cap + a];
         R* q = Pout + ((size_t)i * cap + S.slot[a]) * 3;
         q[0] = -gx;
@@ -1041,7 +1177,7 @@ struct Maise {
     int nspc = 0;
     std::vector<int> NN;
     int nw = 0;
-    std::vector<double> range;  // species 0
+    std::vector<double> range;  // [nspc][inputs]
     std::vector<double> weights;
     double b2a = 1, scaling = 1;
     std::vector<std::vector<int>> map;
@@ -1080,7 +1216,7 @@ Maise read_maise(const char* path) {
             next();
             auto t = words(line);
             if (t.size() < 3) throw Error(E_FILE, "malformed MAISE input range row");
-            if (s == 0) m.range.push_back(std::stod(t[2]));
+            m.range.push_back(std::stod(t[2]));
         }
     for (int i = 0; i < 3; ++i) next();
     for (int i = 0; i < m.nw; ++i) {
@@ -1154,7 +1290,9 @@ static std::unique_ptr<BpnnParams> parse_model(const char* path, std::vector<dou
                                                std::vector<half4_t>& B) {
     Maise m = read_maise(path);
     if (m.NN.size() != 4) throw Error(E_FILE, "MaiseBPNN: only 4-layer networks are supported");
-    if (m.nspc != 1) throw Error(E_INVAL, "MaiseBPNN: one species is supported");
+    if (m.nspc < 1 || m.nspc > 2)
+        throw Error(E_INVAL, "MaiseBPNN: one or two species are supported (with three or more the reference's "
+                             "angular block of a pair depends on the order of the neighbour list)");
     if (m.NN[3] != 1) throw Error(E_FILE, "MaiseBPNN: the output layer must have one neuron");
     auto owner = std::make_unique<BpnnParams>();
     BpnnParams* bp = owner.get();
@@ -1200,8 +1338,16 @@ static std::unique_ptr<BpnnParams> parse_model(const char* path, std::vector<dou
             throw Error(E_INVAL, "only G2 radial and G4 angular descriptors are supported");
         }
     }
-    if (D.nr + D.na < nin) throw Error(E_FILE, "fewer descriptors than network inputs");
-    if (D.nr + D.na != nin) throw Error(E_INVAL, "descriptor count must equal the network input width");
+    // nspc radial and nspc (nspc + 1) / 2 angular blocks
+    D.nspc = m.nspc;
+    const int blocks = D.nspc * D.nr + D.nspc * (D.nspc + 1) / 2 * D.na;
+    if (blocks < nin) throw Error(E_FILE, "fewer descriptors than network inputs");
+    if (blocks != nin) throw Error(E_INVAL, "descriptor count must equal the network input width");
+    D.sr[0] = 0;
+    D.sr[1] = D.nr + D.na;
+    D.sa[0] = D.nr;
+    D.sa[1] = 2 * D.nr + D.na;
+    D.sa[2] = 2 * D.nr + 2 * D.na;
     D.ne = (int)etas.size();
     for (int e = 0; e < D.ne; ++e) D.eta_e[e] = etas[e];
     // G4 terms by (lambda, zeta) group and eta
@@ -1261,7 +1407,9 @@ static std::unique_ptr<BpnnParams> parse_model(const char* path, std::vector<dou
     D.h1 = h1;
     D.h2 = h2;
     D.escale = 10.0 * 1.602176634 * 6.02214076;  // maise_bpnn.cxx:317
-    // maise_ann.cxx:23-51 weight order, columns of W01 scaled by range
+    // maise_ann.cxx:23-51 weight order (species after species), columns of
+    // W01 scaled by the species' range; each species' arrays below are one
+    // block of the same size, species after species
     size_t w = 0;
     auto take = [&](size_t k) {
         if (w + k > m.weights.size()) throw Error(E_FILE, "MAISE weight count does not match the architecture");
@@ -1269,68 +1417,76 @@ static std::unique_ptr<BpnnParams> parse_model(const char* path, std::vector<dou
         w += k;
         return v;
     };
-    auto b01 = take(h1), b12 = take(h2), b23 = take(1);
-    auto W01 = take((size_t)h1 * nin), W12 = take((size_t)h2 * h1), W23 = take(h2);
-    if ((int)w != m.nw) throw Error(E_FILE, "MAISE weight count does not match the architecture");
-    for (int k = 0; k < nin; ++k)
-        for (int q = 0; q < h1; ++q) W01[(size_t)q * nin + k] *= m.range.at(k);
-    W64.clear();
-    for (auto* v : {&W01, &b01, &W12, &b12, &W23}) W64.insert(W64.end(), v->begin(), v->end());
-    W64.push_back(b23[0]);
-    // transposed copies for the FP64 network's column reads
-    for (int k = 0; k < nin; ++k)
-        for (int q = 0; q < h1; ++q) W64.push_back(W01[(size_t)q * nin + k]);
-    for (int p = 0; p < h1; ++p)
-        for (int q = 0; q < h2; ++q) W64.push_back(W12[(size_t)q * h1 + p]);
-    // B operands of the four GEMMs in the MFMA lane layout, split hi/lo
     auto pad16 = [](int x) { return (x + 15) / 16 * 16; };
     bp->nin_pad = pad16(nin);
     bp->h1_pad = pad16(h1);
     bp->h2_pad = pad16(h2);
-    // The MFMA network takes input k as G_k xs_k with xs_k the power of two
-    // nearest the file's range_k, and W01 column k divided by it (both
-    // exact): its inputs are then of order one, as the range normalises
-    // them, and the FP16 low halves of small descriptors stay normal.
-    std::vector<double> xs(bp->nin_pad, 1.0);
-    for (int k = 0; k < nin; ++k) {
-        const double r = m.range.at(k);
-        xs[k] = (r > 0 && std::isfinite(r)) ? std::ldexp(1.0, (int)std::lround(std::log2(r))) : 1.0;
-    }
+    W64.clear();
     W32.clear();
-    for (auto* v : {&b01, &b12, &W23}) W32.insert(W32.end(), v->begin(), v->end());
-    W32.push_back((float)b23[0]);
-    for (double x : xs) W32.push_back((float)x);
-    struct Gm {
-        int K, N;
-        std::function<double(int, int)> at;
-    };
-    std::vector<Gm> gm = {
-        {nin, h1, [&](int k, int n) { return W01[(size_t)n * nin + k] / xs[k]; }},  // X W01^T
-        {h1, h2, [&](int k, int n) { return W12[(size_t)n * h1 + k]; }},            // H1 W12^T
-        {h2, h1, [&](int k, int n) { return W12[(size_t)k * h1 + n]; }},            // D2 W12
-        {h1, nin, [&](int k, int n) { return W01[(size_t)k * nin + n] / xs[n]; }},  // D1 W01
-    };
     B.clear();
-    for (int g = 0; g < 4; ++g) {
-        const int kt = pad16(gm[g].K) / 16, nt = pad16(gm[g].N) / 16;
-        bp->boff[g] = (int)B.size();
-        bp->kt[g] = kt;
-        bp->nt[g] = nt;
-        for (int t = 0; t < nt; ++t)
-            for (int k = 0; k < kt; ++k)
-                for (int part = 0; part < 2; ++part)
-                    for (int lane = 0; lane < 64; ++lane) {
-                        half4_t v;
-                        for (int c = 0; c < 4; ++c) {
-                            const int kk = k * 16 + (lane >> 4) * 4 + c, nn = t * 16 + (lane & 15);
-                            const double x = (kk < gm[g].K && nn < gm[g].N) ? gm[g].at(kk, nn) : 0.0;
-                            const _Float16 hi = (_Float16)(float)x;
-                            const _Float16 lo = (_Float16)((float)x - (float)hi);
-                            v[c] = part == 0 ? hi : lo;
+    for (int sp = 0; sp < m.nspc; ++sp) {
+        auto b01 = take(h1), b12 = take(h2), b23 = take(1);
+        auto W01 = take((size_t)h1 * nin), W12 = take((size_t)h2 * h1), W23 = take(h2);
+        const double* range = &m.range.at((size_t)sp * nin);
+        for (int k = 0; k < nin; ++k)
+            for (int q = 0; q < h1; ++q) W01[(size_t)q * nin + k] *= range[k];
+        const size_t w64_0 = W64.size(), w32_0 = W32.size(), b_0 = B.size();
+        for (auto* v : {&W01, &b01, &W12, &b12, &W23}) W64.insert(W64.end(), v->begin(), v->end());
+        W64.push_back(b23[0]);
+        // transposed copies for the FP64 network's column reads
+        for (int k = 0; k < nin; ++k)
+            for (int q = 0; q < h1; ++q) W64.push_back(W01[(size_t)q * nin + k]);
+        for (int p = 0; p < h1; ++p)
+            for (int q = 0; q < h2; ++q) W64.push_back(W12[(size_t)q * h1 + p]);
+        // The MFMA network takes input k as G_k xs_k with xs_k the power of
+        // two nearest the file's range_k, and W01 column k divided by it
+        // (both exact): its inputs are then of order one, as the range
+        // normalises them, and the FP16 low halves of small descriptors stay
+        // normal.
+        std::vector<double> xs(bp->nin_pad, 1.0);
+        for (int k = 0; k < nin; ++k) {
+            const double r = range[k];
+            xs[k] = (r > 0 && std::isfinite(r)) ? std::ldexp(1.0, (int)std::lround(std::log2(r))) : 1.0;
+        }
+        for (auto* v : {&b01, &b12, &W23}) W32.insert(W32.end(), v->begin(), v->end());
+        W32.push_back((float)b23[0]);
+        for (double x : xs) W32.push_back((float)x);
+        struct Gm {
+            int K, N;
+            std::function<double(int, int)> at;
+        };
+        std::vector<Gm> gm = {
+            {nin, h1, [&](int k, int n) { return W01[(size_t)n * nin + k] / xs[k]; }},  // X W01^T
+            {h1, h2, [&](int k, int n) { return W12[(size_t)n * h1 + k]; }},            // H1 W12^T
+            {h2, h1, [&](int k, int n) { return W12[(size_t)k * h1 + n]; }},            // D2 W12
+            {h1, nin, [&](int k, int n) { return W01[(size_t)k * nin + n] / xs[n]; }},  // D1 W01
+        };
+        // B operands of the four GEMMs in the MFMA lane layout, split hi/lo
+        for (int g = 0; g < 4; ++g) {
+            const int kt = pad16(gm[g].K) / 16, nt = pad16(gm[g].N) / 16;
+            bp->boff[g] = (int)(B.size() - b_0);
+            bp->kt[g] = kt;
+            bp->nt[g] = nt;
+            for (int t = 0; t < nt; ++t)
+                for (int k = 0; k < kt; ++k)
+                    for (int part = 0; part < 2; ++part)
+                        for (int lane = 0; lane < 64; ++lane) {
+                            half4_t v;
+                            for (int c = 0; c < 4; ++c) {
+                                const int kk = k * 16 + (lane >> 4) * 4 + c, nn = t * 16 + (lane & 15);
+                                const double x = (kk < gm[g].K && nn < gm[g].N) ? gm[g].at(kk, nn) : 0.0;
+                                const _Float16 hi = (_Float16)(float)x;
+                                const _Float16 lo = (_Float16)((float)x - (float)hi);
+                                v[c] = part == 0 ? hi : lo;
+                            }
+                            B.push_back(v);
                         }
-                        B.push_back(v);
-                    }
+        }
+        bp->w64_stride = W64.size() - w64_0;
+        bp->w32_stride = W32.size() - w32_0;
+        bp->b_stride = B.size() - b_0;
     }
+    if ((int)w != m.nw) throw Error(E_FILE, "MAISE weight count does not match the architecture");
     return owner;
 }
 
@@ -1389,6 +1545,8 @@ void bpnn_describe(const char* path, int* arch, int* counts, double* rcut) {
     }
     if (rcut) *rcut = D.Rc;
 }
+
+int bpnn_species(const Pot& p) { return p.bp ? p.bp->h.nspc : 1; }
 
 void bpnn_destroy(Pot* p) {
     if (p->bp) {
@@ -1480,32 +1638,48 @@ static void bpnn_run(Batch& b, const Pot& p, const BpBufs& W, bool rev_fresh, in
                           : bp.h.lz == 5 ? 4
                           : (bp.h.nr <= 8 && bp.h.ng <= 4 && bp.h.ne <= 2) ? 0
                           : (bp.h.nr <= 8 ? 1 : 2);
-        auto kdesc = shape == 3 ? k_bp_desc<R, 8, 6, 2, 3>
-                     : shape == 4 ? k_bp_desc<R, 8, 10, 7, 5>
-                     : shape == 0 ? k_bp_desc<R, 8, 4, 2>
-                     : shape == 1 ? k_bp_desc<R, 8, kBpMaxG, kBpMaxEta>
-                                  : k_bp_desc<R, kBpMaxR, kBpMaxG, kBpMaxEta>;
+        // K2: the two-species variants
+        const bool two = bp.h.nspc == 2;
+        auto pick_desc = [&](auto k2) {
+            constexpr bool T2 = decltype(k2)::value;
+            return shape == 3 ? k_bp_desc<R, 8, 6, 2, 3, T2>
+                   : shape == 4 ? k_bp_desc<R, 8, 10, 7, 5, T2>
+                   : shape == 0 ? k_bp_desc<R, 8, 4, 2, 0, T2>
+                   : shape == 1 ? k_bp_desc<R, 8, kBpMaxG, kBpMaxEta, 0, T2>
+                                : k_bp_desc<R, kBpMaxR, kBpMaxG, kBpMaxEta, 0, T2>;
+        };
+        auto kdesc = two ? pick_desc(std::true_type{}) : pick_desc(std::false_type{});
         if (parts & 1) {
             launch_k(b, K_BP_DESC, kdesc, gw, bw, (unsigned)lds_desc, pos, b.nl, b.nn, b.tpa, b.M, b.cap, b.d_desc,
                      bp.d, (R*)W.G, ldg, n, S, b.misc);
-            if (sizeof(R) == 8) {
-                const bool narrow = std::max(bp.h.nin, std::max(bp.h.h1, bp.h.h2)) <= 32;
-                const int L = narrow ? 32 : 64;
-                const dim3 grid((unsigned)(((long long)n * L + 255) / 256));
-                launch_k(b, K_BP_MLP, narrow ? k_bp_mlp64<32> : k_bp_mlp64<64>, grid, dim3(256), 0,
-                         (const double*)W.G, ldg, bp.w64, bp.d, pos, (double*)W.w, Ep, n, S, b.misc);
-            } else
-                launch_k(b, K_BP_MLP, k_bp_mlp_mfma, dim3((n + 15) / 16), dim3(64), 0, (const float*)W.G, ldg,
-                         bp.bsplit, boff, nts, kts, bp.w32, bp.d, pos, (float*)W.w, Ep, n, S, b.misc);
+            // one network per species, each over the atoms of its kind
+            for (int sp = 0; sp < bp.h.nspc; ++sp) {
+                const int only = bp.h.nspc > 1 ? sp : -1;
+                if (sizeof(R) == 8) {
+                    const bool narrow = std::max(bp.h.nin, std::max(bp.h.h1, bp.h.h2)) <= 32;
+                    const int L = narrow ? 32 : 64;
+                    const dim3 grid((unsigned)(((long long)n * L + 255) / 256));
+                    launch_k(b, K_BP_MLP, narrow ? k_bp_mlp64<32> : k_bp_mlp64<64>, grid, dim3(256), 0,
+                             (const double*)W.G, ldg, bp.w64 + sp * bp.w64_stride, bp.d, pos, (double*)W.w, Ep, n,
+                             only, S, b.misc);
+                } else
+                    launch_k(b, K_BP_MLP, k_bp_mlp_mfma, dim3((n + 15) / 16), dim3(64), 0, (const float*)W.G, ldg,
+                             bp.bsplit + sp * bp.b_stride, boff, nts, kts, bp.w32 + sp * bp.w32_stride, bp.d, pos,
+                             (float*)W.w, Ep, n, only, S, b.misc);
+            }
         }
         if (!(parts & 2)) return;
         auto grad = [&](auto vtag) {
             constexpr bool V = decltype(vtag)::value;
-            auto kgrad = shape == 3 ? k_bp_grad<R, 8, 6, 2, V, 3>
-                         : shape == 4 ? k_bp_grad<R, 8, 10, 7, V, 5>
-                         : shape == 0 ? k_bp_grad<R, 8, 4, 2, V>
-                         : shape == 1 ? k_bp_grad<R, 8, kBpMaxG, kBpMaxEta, V>
-                                      : k_bp_grad<R, kBpMaxR, kBpMaxG, kBpMaxEta, V>;
+            auto pick_grad = [&](auto k2) {
+                constexpr bool T2 = decltype(k2)::value;
+                return shape == 3 ? k_bp_grad<R, 8, 6, 2, V, 3, T2>
+                       : shape == 4 ? k_bp_grad<R, 8, 10, 7, V, 5, T2>
+                       : shape == 0 ? k_bp_grad<R, 8, 4, 2, V, 0, T2>
+                       : shape == 1 ? k_bp_grad<R, 8, kBpMaxG, kBpMaxEta, V, 0, T2>
+                                    : k_bp_grad<R, kBpMaxR, kBpMaxG, kBpMaxEta, V, 0, T2>;
+            };
+            auto kgrad = two ? pick_grad(std::true_type{}) : pick_grad(std::false_type{});
             launch_k(b, K_BP_FORCE, kgrad, gw, bw, (unsigned)lds, pos, b.nl, b.nn, b.tpa, b.M, b.cap, b.d_desc, bp.d,
                      (const R*)W.w, ldg, (R*)W.P, (R*)W.own, (R*)W.PV, (R*)W.OV, n, S, b.misc);
         };

@@ -289,6 +289,8 @@ void timer_end(Batch& b, hipEvent_t a, int kind);
 void timer_drain(Batch& b);
 // drops the batch's captured evaluation (forces.hip)
 void graph_release(Batch& b);
+// species of a BPNN potential's model (bpnn.hip)
+int bpnn_species(const Pot& p);
 hipEvent_t timer_event(Batch& b);
 
 // Launches kernel k on the context's stream.  While timing is on, the
