@@ -110,16 +110,19 @@ mdc_status mdc_pot_alloy_create_file(mdc_ctx* ctx, const char* setfl_path, mdc_p
 mdc_status mdc_pot_multi_create(mdc_ctx* ctx, int nkinds, int npots, const int* kind_i, const int* kind_j,
                                 const mdc_pot* const* pots, mdc_pot** out);
 /* MaiseBPNNPotential(filename, domain): solver/potential/maise_bpnn.cxx:21-30;
- * G2 radial + G4 angular descriptors, one species, 4-layer tanh network
- * (limits under mdc_pot_bpnn_describe) */
+ * G2 radial + G4 angular descriptors, one or two species (kind-difference
+ * descriptor blocks, tools/bpnn/maise_model.cxx:113-131, and one network per
+ * species; an atom's kind selects its network and must be below the number
+ * of species), 4-layer tanh networks (limits under mdc_pot_bpnn_describe) */
 mdc_status mdc_pot_bpnn_create(mdc_ctx* ctx, const char* maise_path, mdc_pot** out);
 /* The model mdc_pot_bpnn_create would build from a MAISE file, read and
  * checked on the host only (no device, no context): arch[4] = (inputs,
  * hidden 1, hidden 2, 1), counts[4] = (G2 terms, G4 terms, distinct G4 eta,
  * distinct (lambda, zeta) pairs), rcut in nm (tools/bpnn/maise_model.cxx:
- * 19-50).  Any of them may be NULL.  The engine takes 4-layer one-species
- * networks of up to 64 inputs and 64 neurons per hidden layer, 16 G2 and
- * 64 G4 terms over 8 eta and 8 (lambda, zeta) pairs, whole zeta 1..32. */
+ * 19-50); G2 and G4 terms count one block.  Any of them may be NULL.  The
+ * engine takes 4-layer networks of one or two species with up to 64 inputs
+ * and 64 neurons per hidden layer, 16 G2 and 64 G4 terms per block over 8
+ * eta and 8 (lambda, zeta) pairs, whole zeta 1..32. */
 mdc_status mdc_pot_bpnn_describe(const char* maise_path, int* arch, int* counts, double* rcut);
 mdc_status mdc_pot_rcut(const mdc_pot* pot, double* rcut);
 mdc_status mdc_pot_destroy(mdc_pot* pot);
