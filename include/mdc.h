@@ -185,6 +185,16 @@ mdc_status mdc_sys_set_block(mdc_sys* sys, int threads);
  * No reference counterpart. */
 mdc_status mdc_sys_autotune_tiles(mdc_sys* sys, const mdc_pot* pot, int precision, int reps, int* best_lanes,
                                   int* best_threads, double* us);
+/* Step path of single-element EAM in mixed precision.  By default (set =
+ * 0) a step sequence whose boxes all keep a unique minimum image between
+ * rebuilds gathers neighbours as 32-bit fixed-point coordinates (a wrapped
+ * integer difference is the minimum image; 16 bytes per neighbour, FP32
+ * pair arithmetic); set = 1 keeps the FP64-difference kernels for every
+ * step; set = -1 leaves the choice as it is.  *last (may be NULL) receives
+ * 1 when the last step sequence or kernel timing of this system ran the
+ * fixed-point kernels, else 0.
+ * No reference counterpart: a choice between two device paths. */
+mdc_status mdc_sys_step_path(mdc_sys* sys, int set, int* last);
 /* Graph mode (on != 0) of mdc_compute on this system: the device work of
  * an evaluation (fills, density and force kernels, the counter readback) is
  * captured in a HIP graph for its configuration (potential, flags,

@@ -493,6 +493,10 @@ void free_batch(Batch& b) {
     dfree(b.orig[1]);
     dfree(b.rsave);
     dfree(b.posF);
+    dfree(b.uq[0]);
+    dfree(b.uq[1]);
+    dfree(b.uqF);
+    b.uq_n = 0;
     dfree(b.where);
     dfree(b.cell_count);
     dfree(b.cell_start);
@@ -1631,6 +1635,15 @@ mdc_status mdc_sys_set_block(mdc_sys* sys, int threads) {
         if (threads != 64 && threads != 128 && threads != 256)
             throw Error(E_INVAL, "threads per workgroup must be 64, 128 or 256");
         sys->b.block = threads;
+    });
+}
+
+mdc_status mdc_sys_step_path(mdc_sys* sys, int set, int* last) {
+    return guard(sys ? sys->b.ctx : nullptr, [&] {
+        if (!sys) throw Error(E_INVAL, "null argument");
+        if (set < -1 || set > 1) throw Error(E_INVAL, "set must be -1 (keep), 0 (automatic) or 1 (FP64 differences)");
+        if (set >= 0) sys->b.q_off = set == 1;
+        if (last) *last = sys->b.q_last ? 1 : 0;
     });
 }
 

@@ -41,6 +41,14 @@ struct SysDesc {
     // the box's last cell (after its nc[0] nc[1] nc[2] grid): void records
     // of the box are binned there and no list build visits it
     int void_cell;
+    // 32-bit fixed-point coordinates (set with shift_ok): u = (x - lo)
+    // qs mod 2^32 with qs = 2^32 / modulus, the modulus being the edge of
+    // a periodic axis (the wrap of the integer difference is then the
+    // minimum image) and a power of two above 4 (rns + skin) on an open
+    // axis; qf = modulus / 2^32 turns a difference back into nm
+    double qs[3];
+    float qf[3];
+    int qpad;
 };
 
 // A list entry holds the neighbour index in bits 0-25 and, per axis, the
@@ -173,6 +181,23 @@ __device__ inline void shortest3(const S& s, double& x, double& y, double& z) {
         z = fabs(z + s.L[2]) < fabs(z) ? z + s.L[2] : z;
         z = fabs(z - s.L[2]) < fabs(z) ? z - s.L[2] : z;
     }
+}
+
+// Fixed-point record of a position: three 32-bit coordinates (SysDesc qs)
+// and the low 32 bits of the tag word.  A listed pair's difference is then
+// (int)(u_j - u_i) * qf per axis: the integer wrap takes the minimum image
+// on periodic axes, exact to one unit (about 1e-8 of the edge).
+__device__ inline unsigned fixq1(double x, double lo, double qs) {
+    return (unsigned)(long long)rint((x - lo) * qs);
+}
+__device__ inline uint4 fixq4(double x, double y, double z, double w, const SysDesc& s) {
+    return make_uint4(fixq1(x, s.lo[0], s.qs[0]), fixq1(y, s.lo[1], s.qs[1]), fixq1(z, s.lo[2], s.qs[2]),
+                      (unsigned)(long long)w);
+}
+__device__ inline float fixq_d(unsigned uj, unsigned ui, float qf) { return (float)(int)(uj - ui) * qf; }
+__device__ inline uint4 buf_u4(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    const u32x4_t a = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+    return make_uint4(a[0], a[1], a[2], a[3]);
 }
 
 // |d|^2 summed left to right with no contraction, as the reference's

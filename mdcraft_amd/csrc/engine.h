@@ -182,6 +182,18 @@ struct Batch {
     // single-element EAM: each atom's position with F'(nc) in w, written by
     // the density pass so the force pass gathers both in one record
     double4* posF = nullptr;
+    // Fixed-point step path (single-element EAM in mixed precision, every
+    // box with a unique minimum image): uq[c] holds the fixed-point record
+    // (SysDesc qs) of pos[c], written with every drift and after every
+    // sort; uqF the same coordinates with F'(nc) as FP32 in w, written by
+    // the density pass.  Allocated on first use (uq_n records).
+    uint4* uq[2] = {nullptr, nullptr};
+    uint4* uqF = nullptr;
+    int uq_n = 0;
+    bool qfast = false;    // set while a step sequence or a kernel timing runs on this path
+    bool q_off = false;    // mdc_sys_step_path(1): never take it
+    bool q_last = false;   // the last step sequence or timing took it
+    bool all_shift = false;  // every box's shift_ok (set_trigger)
     int* where = nullptr;  // original index -> sorted index
     // cells
     int ncells = 0;

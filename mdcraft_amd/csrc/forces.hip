@@ -570,17 +570,214 @@ __global__ void __launch_bounds__(256) k_eam_force(const double4* __restrict__ p
     finalize_atom<R>(i, pi, (double)fx, (double)fy, (double)fz, A, S, pos_next, fin, s, misc);
 }
 
+// --------------------------------------------------------------------------
+// Fixed-point step path: single-element EAM in mixed precision, every box
+// with a unique minimum image between rebuilds (shift_ok).  Neighbours are
+// gathered as one 16-byte fixed-point record (SysDesc qs) instead of a
+// 32-byte FP64 position, and a pair's difference is the wrapped integer
+// difference times qf: no image shift, no FP64 arithmetic per pair.  The
+// formulas are those of k_eam_rho / k_eam_force (EAM.cxx:499-560 and
+// 356-497) on the same FP32 interval records.
+// --------------------------------------------------------------------------
+
+// Pair geometry of the fixed-point path: difference, distance and table
+// interval; ok is false for a pair beyond the cutoff or the table.
+struct QGeo {
+    float dx, dy, dz, r2, R1, RN;
+    int N1;
+    bool ok;
+};
+__device__ inline QGeo q_geo(const uint4& uj, const uint4& ui, bool listed, float qx, float qy, float qz, float rc2,
+                             float dr1, int nr) {
+    QGeo g;
+    g.dx = fixq_d(uj.x, ui.x, qx);
+    g.dy = fixq_d(uj.y, ui.y, qy);
+    g.dz = fixq_d(uj.z, ui.z, qz);
+    g.r2 = g.dx * g.dx + g.dy * g.dy + g.dz * g.dz;
+    g.ok = listed && !(g.r2 > rc2 || g.r2 < 1e-15f);
+    g.R1 = __builtin_amdgcn_sqrtf(g.r2);
+    g.RN = g.R1 * dr1;
+    g.N1 = (int)g.RN;
+    g.ok = g.ok && g.N1 < nr;
+    return g;
+}
+
+// Fixed-point records of pos [0, live): after a sort, before a timing.
+__global__ void __launch_bounds__(256) k_fixq(const double4* __restrict__ pos, uint4* __restrict__ uq,
+                                              const SysDesc* __restrict__ desc, int live) {
+    const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= live) return;
+    const double4 p = pos[i];
+    uq[i] = fixq4(p.x, p.y, p.z, p.w, desc[w_sys(p.w)]);
+}
+
+// Density pass: rho_i from the rho records, then F(rho), F'(rho) in FP64
+// as k_eam_rho; writes the atom's record with F' in w for the force pass.
+template <int TPA>
+__global__ void __launch_bounds__(256) k_eam_rho_q(const uint4* __restrict__ uq, uint4* __restrict__ uqF,
+                                                   const int* __restrict__ nl, const int* __restrict__ nn, int M,
+                                                   int cap, const SysDesc* __restrict__ desc, EamK P, AtomArrays A,
+                                                   StepArgs S, int* __restrict__ misc) {
+    const AtomSlots as = atom_slots<TPA>(M);
+    const int i = as.i;
+    const bool valid = i < A.live;
+    const int ab = abort_flag(S, misc);
+    const uint4 ui = valid ? uq[i] : make_uint4(0, 0, 0, 0);
+    const int nraw = valid ? nn[i] : 0;
+    int j0[kU];
+    first_group<TPA, kU>(as, valid, M, nl, j0);
+    if (ab <= S.step) return;
+    const bool active = valid && !((ui.w >> 8) & 1u);
+    const SysDesc& s = desc[valid ? (int)(ui.w >> 10) : 0];
+    const float qx = s.qf[0], qy = s.qf[1], qz = s.qf[2];
+    const int cnt = active ? min(nraw, cap) : 0;
+    const float rc2 = (float)P.rc2, dr1 = (float)P.dr1;
+    const int nr = P.nr;
+    double rho = 0.0;
+    const __amdgpu_buffer_rsrc_t ub = make_buf(uq, (unsigned)A.n * 16u);
+    const __amdgpu_buffer_rsrc_t tabb = make_buf(P.r32, (unsigned)nr * 16u);
+    int jn[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) jn[u] = j0[u];
+    const int nch = wave_chunks<TPA>(cnt);
+    for (int m0 = 0; m0 < nch; m0 += kU) {
+        bool okk[kU];
+        uint4 uj[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            okk[u] = (m0 + u) * TPA + as.sub < cnt;
+            uj[u] = buf_u4(ub, okk[u] ? (unsigned)ent_j(jn[u]) * 16u : kSkip);
+        }
+        next_group_clamped<kU>(as, m0, M, nl, jn);
+        QGeo g[kU];
+        float4 rec[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            g[u] = q_geo(uj[u], ui, okk[u], qx, qy, qz, rc2, dr1, nr);
+            rec[u] = buf_f4(tabb, g[u].ok ? (unsigned)g[u].N1 * 16u : kSkip);
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const float t = g[u].RN - (float)g[u].N1;
+            const float4 c = rec[u];
+            rho += (double)(c.x + t * (c.y + t * (c.z + t * c.w)));
+        }
+    }
+    rho = wave_sum_group(rho, TPA);
+    if (!valid || as.sub != 0) return;
+    const int n = A.n;
+    // tag == 0: neighbours evaluate F'(nc) of its stored density
+    const double rr = !active ? A.d[(size_t)D_NC * n + i] : rho;
+    double F, dF;
+    embed_at(P.F64, 0, P.nrho, P.drho, P.drho1, rr, F, dF, misc);
+    A.d[(size_t)D_FP * n + i] = dF;
+    uqF[i] = make_uint4(ui.x, ui.y, ui.z, __float_as_uint((float)dF));
+    if (active && S.store_aux && (S.flags & W_ENERGY)) A.d[(size_t)D_EM * n + i] = F;
+    if (active && S.store_aux) A.d[(size_t)D_NC * n + i] = rho;
+}
+
+// Force pass with the fused velocity-Verlet finish; also writes the drifted
+// position's fixed-point record.
+template <int TPA>
+__global__ void __launch_bounds__(256) k_eam_force_q(const double4* __restrict__ pos, double4* __restrict__ pos_next,
+                                                     const uint4* __restrict__ uqF, uint4* __restrict__ uq_next,
+                                                     const int* __restrict__ nl, const int* __restrict__ nn, int M,
+                                                     int cap, const SysDesc* __restrict__ desc, EamK P, AtomArrays A,
+                                                     StepArgs S, const double4* __restrict__ rsave,
+                                                     int* __restrict__ misc) {
+    const AtomSlots as = atom_slots<TPA>(M);
+    const int i = as.i;
+    const bool valid = i < A.live;
+    const int n = A.n;
+    const int ab = abort_flag(S, misc);
+    const double4 pi = valid ? pos[i] : make_double4(0, 0, 0, 0);
+    const uint4 ui = valid ? uqF[i] : make_uint4(0, 0, 0, 0);
+    const int nraw = valid ? nn[i] : 0;
+    int j0[kU];
+    first_group<TPA, kU>(as, valid, M, nl, j0);
+    if (ab <= S.step) return;
+    const bool active = valid && !w_inactive(pi.w);
+    const SysDesc& s = desc[valid ? w_sys(pi.w) : 0];
+    const float qx = s.qf[0], qy = s.qf[1], qz = s.qf[2];
+    const int cnt = active ? min(nraw, cap) : 0;
+    const float rc2 = (float)P.rc2, dr1 = (float)P.dr1;
+    const int nr = P.nr;
+    const float Fpi = active ? __uint_as_float(ui.w) : 0.f;
+    float fx = 0, fy = 0, fz = 0;
+    const __amdgpu_buffer_rsrc_t ub = make_buf(uqF, (unsigned)n * 16u);
+    // each interval's rho and r*phi records side by side (32 bytes)
+    const __amdgpu_buffer_rsrc_t tabpb = make_buf(P.r32p, (unsigned)nr * 32u);
+    int jn[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) jn[u] = j0[u];
+    const int nch = wave_chunks<TPA>(cnt);
+    for (int m0 = 0; m0 < nch; m0 += kU) {
+        bool okk[kU];
+        uint4 uj[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            okk[u] = (m0 + u) * TPA + as.sub < cnt;
+            uj[u] = buf_u4(ub, okk[u] ? (unsigned)ent_j(jn[u]) * 16u : kSkip);
+        }
+        next_group_clamped<kU>(as, m0, M, nl, jn);
+        QGeo g[kU];
+        float4 rr[kU], rp[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            g[u] = q_geo(uj[u], ui, okk[u], qx, qy, qz, rc2, dr1, nr);
+            const unsigned off = g[u].ok ? (unsigned)g[u].N1 * 32u : kSkip;
+            rr[u] = buf_f4(tabpb, off);
+            rp[u] = buf_f4(tabpb, off + 16u);
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const float t = g[u].RN - (float)g[u].N1;
+            const float4 a = rr[u], c = rp[u];
+            // rho'(r), r*phi and (r*phi)'(r) of the interval cubic
+            const float nd = (a.y + t * (2.f * a.z + 3.f * t * a.w)) * dr1;
+            const float phv = c.x + t * (c.y + t * (c.z + t * c.w));
+            const float phd = (c.y + t * (2.f * c.z + 3.f * t * c.w)) * dr1;
+            const float i1 = rcp_(g[u].R1), i2 = i1 * i1;
+            const float AdF = Fpi + __uint_as_float(uj[u].w);
+            // computed for every lane (the records of unused pairs read as
+            // zeros), then selected: a conditional expression would become
+            // a branch with the table reads sunk into it, one at a time
+            float FF = (AdF * nd + (phd * g[u].R1 - phv) * i2) * i1;
+            FF = g[u].ok ? FF : 0.f;
+            fx += FF * g[u].dx;
+            fy += FF * g[u].dy;
+            fz += FF * g[u].dz;
+        }
+    }
+    fx = wave_sum_group(fx, TPA);
+    fy = wave_sum_group(fy, TPA);
+    fz = wave_sum_group(fz, TPA);
+    if (!valid || as.sub != 0) return;
+    if (!active) {
+        if (S.mode == MODE_FINAL_DRIFT) {
+            pos_next[i] = pi;
+            uq_next[i] = fixq4(pi.x, pi.y, pi.z, pi.w, s);
+        }
+        return;
+    }
+    const FinIn fin = load_fin(i, A, S, rsave, true);
+    finalize_atom<float>(i, pi, (double)fx, (double)fy, (double)fz, A, S, pos_next, fin, s, misc, uq_next);
+}
+
 // Opening half kick + drift of the first step of a sequence from the stored
 // force (verlet.cxx:71-78); the trigger marks step 0 as needing a rebuild.
+// uq_next (fixed-point path, else null): the drifted position's
+// fixed-point record.
 __global__ void k_predrift(const double4* __restrict__ pos, double4* __restrict__ pos_next, AtomArrays A, int f_prec,
                            double dt, const SysDesc* __restrict__ desc, const double4* __restrict__ rsave,
-                           int* __restrict__ misc) {
+                           int* __restrict__ misc, uint4* __restrict__ uq_next) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = A.n;
     if (i >= A.live) return;
     const double4 pi = pos[i];
     if (w_inactive(pi.w)) {
         pos_next[i] = pi;
+        if (uq_next) uq_next[i] = fixq4(pi.x, pi.y, pi.z, pi.w, desc[w_sys(pi.w)]);
         return;
     }
     double F[3];
@@ -603,6 +800,7 @@ __global__ void k_predrift(const double4* __restrict__ pos, double4* __restrict_
     }
     pos_next[i] = make_double4(r[0], r[1], r[2], pi.w);
     const SysDesc& s = desc[w_sys(pi.w)];
+    if (uq_next) uq_next[i] = fixq4(r[0], r[1], r[2], pi.w, s);
     const double4 q = rsave[i];
     double dx = r[0] - q.x, dy = r[1] - q.y, dz = r[2] - q.z;
     shortest3(s, dx, dy, dz);
@@ -715,6 +913,7 @@ void timer_drain(Batch& b) {
 
 void set_trigger(Batch& b, double rcut, bool allow_shift) {
     bool changed = false;
+    b.all_shift = b.nsys > 0;
     for (int s = 0; s < b.nsys; ++s) {
         SysDesc& d = b.desc[s];
         double skin = b.rns_min[s] - rcut;
@@ -736,6 +935,21 @@ void set_trigger(Batch& b, double rcut, bool allow_shift) {
             d.trig2 = t;
             d.shift_ok = sh;
             changed = true;
+        }
+        if (!sh) b.all_shift = false;
+        // fixed-point scales: listed pairs stay within rns_max + skin, below
+        // half of every periodic edge (shift_ok) and below a quarter of the
+        // power-of-two modulus of an open axis
+        for (int a = 0; a < 3 && sh; ++a) {
+            double mod = d.per[a] ? d.L[a] : std::exp2(std::ceil(std::log2(4.0 * (b.rns_max[s] + skin))));
+            if (!(mod > 0.0 && mod < 1e300)) mod = 1.0;  // a box with no atoms
+            const double qs = std::ldexp(1.0, 32) / mod;
+            const float qf = (float)(mod * std::ldexp(1.0, -32));
+            if (d.qs[a] != qs || d.qf[a] != qf) {
+                d.qs[a] = qs;
+                d.qf[a] = qf;
+                changed = true;
+            }
         }
     }
     if (changed)
@@ -830,6 +1044,18 @@ static void launch_pair_t(Batch& b, const Pot& p, const StepArgs& S, const doubl
                  lj_args(p), AA, S, b.rsave, b.misc);
     } else {
         EamK E = eam_args(p, b);
+        if (sizeof(R) == 4 && b.qfast) {
+            // fixed-point step path: uq[c] holds the records of pos = pos[c]
+            const int c = pos == b.pos[0] ? 0 : 1;
+            if (which & 1)
+                launch_k(b, K_DENSITY, k_eam_rho_q<TPA>, dim3(grid), dim3(b.block), 0, (const uint4*)b.uq[c], b.uqF,
+                         b.nl, b.nn, b.M, b.cap, b.d_desc, E, AA, S, b.misc);
+            if (which & 2)
+                launch_k(b, K_FORCE, k_eam_force_q<TPA>, dim3(grid), dim3(b.block), 0, pos, pos_next,
+                         (const uint4*)b.uqF, pos_next ? b.uq[c ^ 1] : (uint4*)nullptr, b.nl, b.nn, b.M, b.cap,
+                         b.d_desc, E, AA, S, b.rsave, b.misc);
+            return;
+        }
         if (which & 1) {
             if (E.nk > 1)
                 launch_k(b, K_DENSITY, k_eam_rho<R, TPA, true>, dim3(grid), dim3(b.block), 0, pos, b.nl, b.nn, b.M, b.cap,
@@ -1219,6 +1445,34 @@ void stage_compute(Batch& b, const Pot& p, int stage, int prec) {
     check_errors(b);
 }
 
+// The fixed-point step path applies to single-element EAM in mixed
+// precision without thermostats, with every box's minimum image unique
+// between rebuilds (set_trigger has run); the box index must fit the 22
+// bits the record's tag word keeps.
+static bool qfast_ok(const Batch& b, const Pot& p, int prec) {
+    return prec == P_MIXED && p.type == PT_EAM && p.eam.nk == 1 && !p.eam.alloy && p.eam.r32p &&
+           b.therm.count == 0 && b.all_shift && !b.manual_rebuild && b.live > 0 && b.nsys < (1 << 22) && !b.q_off;
+}
+
+static void qfast_alloc(Batch& b) {
+    if (b.uq_n >= b.n && b.uq[0]) return;
+    for (uint4** a : {&b.uq[0], &b.uq[1], &b.uqF}) {
+        if (*a) HIPCHECK(hipFree(*a));
+        *a = nullptr;
+    }
+    b.uq_n = 0;
+    for (uint4** a : {&b.uq[0], &b.uq[1], &b.uqF})
+        if (hipMalloc(a, sizeof(uint4) * std::max(b.n, 1)) != hipSuccess)
+            throw Error(E_NOMEM, "out of device memory for the fixed-point positions");
+    b.uq_n = b.n;
+}
+
+// fixed-point records of the current positions (after a sort)
+static void qfast_refresh(Batch& b) {
+    hipLaunchKernelGGL(k_fixq, dim3(nblk(b.live, 256)), dim3(256), 0, b.ctx->stream, b.pos[b.cur], b.uq[b.cur],
+                       b.d_desc, b.live);
+}
+
 void run_steps(Batch& b, const Pot& p, double dt, int nsteps, int prec, int chunk) {
     if (b.n == 0 || nsteps <= 0) return;
     hipStream_t st = b.ctx->stream;
@@ -1229,11 +1483,19 @@ void run_steps(Batch& b, const Pot& p, double dt, int nsteps, int prec, int chun
     const int grid = nblk(b.live, 256);
     int abort_init = INT_MAX;
     HIPCHECK(hipMemcpyAsync(b.misc + M_ABORT, &abort_init, sizeof(int), hipMemcpyHostToDevice, st));
+    const bool qf = qfast_ok(b, p, prec);
+    if (qf) qfast_alloc(b);
+    struct QOff {
+        Batch& b;
+        ~QOff() { b.qfast = false; }
+    } qoff{b};
+    b.qfast = qf;
+    b.q_last = qf;
     int c0 = b.cur;
     {
         AtomArrays AA = arrays(b);
         launch_k(b, K_PREDRIFT, k_predrift, dim3(grid), dim3(256), 0, b.pos[c0], b.pos[c0 ^ 1], AA, b.f_prec, dt,
-                 b.d_desc, b.rsave, b.misc);
+                 b.d_desc, b.rsave, b.misc, b.qfast ? b.uq[c0 ^ 1] : (uint4*)nullptr);
     }
     auto buf = [&](int t) { return c0 ^ 1 ^ (t & 1); };
     // Steps go out in chunks, two chunks in flight: while one runs, the
@@ -1316,6 +1578,7 @@ void run_steps(Batch& b, const Pot& p, double dt, int nsteps, int prec, int chun
             b.since_rebuild = 0;
             b.cur = buf(ab);
             nlist_rebuild(b, false);
+            if (qf) qfast_refresh(b);
             timer_drain(b);
             c0 = b.cur ^ 1 ^ (ab & 1);
             s = launch_s = ab;  // the rebuild cleared the trigger
@@ -1340,9 +1603,8 @@ void run_steps(Batch& b, const Pot& p, double dt, int nsteps, int prec, int chun
 // Mean duration of the step-mode density and force kernels (the
 // configuration of every step but the last in a sequence), each launched
 // reps times back to back between two events on the context stream.  The
-// velocities, which the force kernel updates in place, are saved in the
-// idle half of the double-buffered state and restored; forces and F' are
-// rewritten with the values they already hold.
+// velocities, which the force kernel updates in place, forces and F' are
+// saved in the idle half of the double-buffered state and restored.
 void time_step_kernels(Batch& b, const Pot& p, int prec, double dt, int reps, double* us) {
     us[0] = us[1] = 0.0;
     if (b.n == 0) return;
@@ -1355,11 +1617,30 @@ void time_step_kernels(Batch& b, const Pot& p, int prec, double dt, int reps, do
     const int idle = b.sc ^ 1;
     const size_t vbytes = sizeof(double) * 3 * (size_t)b.n;
     HIPCHECK(hipMemcpyAsync(b.d(idle, D_VX), b.d(b.sc, D_VX), vbytes, hipMemcpyDeviceToDevice, st));
+    // forces and F' too: the kernels a step sequence runs may round them
+    // differently from the evaluation that stored them
+    const size_t fbytes = sizeof(float) * 3 * (size_t)b.n;
+    HIPCHECK(hipMemcpyAsync(b.fl(idle, F_FX), b.fl(b.sc, F_FX), fbytes, hipMemcpyDeviceToDevice, st));
+    HIPCHECK(hipMemcpyAsync(b.d(idle, D_FX), b.d(b.sc, D_FX), vbytes, hipMemcpyDeviceToDevice, st));
+    HIPCHECK(hipMemcpyAsync(b.d(idle, D_FP), b.d(b.sc, D_FP), sizeof(double) * (size_t)b.n,
+                            hipMemcpyDeviceToDevice, st));
     int abort_init = INT_MAX;
     HIPCHECK(hipMemcpyAsync(b.misc + M_ABORT, &abort_init, sizeof(int), hipMemcpyHostToDevice, st));
     const StepArgs S{dt, 0, MODE_FINAL_DRIFT, 0, 0};
     const double4* pos = b.pos[b.cur];
     double4* pos_next = b.pos[b.cur ^ 1];
+    // the kernels a step sequence of this configuration would run
+    const bool qf = qfast_ok(b, p, prec);
+    if (qf) {
+        qfast_alloc(b);
+        qfast_refresh(b);
+    }
+    struct QOff {
+        Batch& b;
+        ~QOff() { b.qfast = false; }
+    } qoff{b};
+    b.qfast = qf;
+    b.q_last = qf;
     hipEvent_t e[3];
     for (auto& x : e) HIPCHECK(hipEventCreate(&x));
     const bool timing = b.timing;
@@ -1390,6 +1671,10 @@ void time_step_kernels(Batch& b, const Pot& p, int prec, double dt, int reps, do
     us[0] = p.type == PT_EAM ? 1e3 * m0 / reps : 0.0;
     us[1] = 1e3 * m1 / reps;
     HIPCHECK(hipMemcpyAsync(b.d(b.sc, D_VX), b.d(idle, D_VX), vbytes, hipMemcpyDeviceToDevice, st));
+    HIPCHECK(hipMemcpyAsync(b.fl(b.sc, F_FX), b.fl(idle, F_FX), fbytes, hipMemcpyDeviceToDevice, st));
+    HIPCHECK(hipMemcpyAsync(b.d(b.sc, D_FX), b.d(idle, D_FX), vbytes, hipMemcpyDeviceToDevice, st));
+    HIPCHECK(hipMemcpyAsync(b.d(b.sc, D_FP), b.d(idle, D_FP), sizeof(double) * (size_t)b.n,
+                            hipMemcpyDeviceToDevice, st));
     HIPCHECK(hipMemcpyAsync(b.misc + M_ABORT, &abort_init, sizeof(int), hipMemcpyHostToDevice, st));
     read_misc(b);
     check_errors(b);

@@ -115,10 +115,12 @@ __device__ inline FinIn load_fin(int i, const AtomArrays& A, const StepArgs& S, 
 // step follows, the opening half kick and drift (verlet.cxx:71-78) into the
 // other position buffer together with the displacement trigger of
 // verlet-list.cxx:149-154.
+// uq_next (fixed-point path, else null): the drifted position's
+// fixed-point record as well.
 template <typename R>
 __device__ inline void finalize_atom(int i, const double4& pi, double fx, double fy, double fz, const AtomArrays& A,
                                      const StepArgs& S, double4* __restrict__ pos_next, const FinIn& in,
-                                     const SysDesc& s, int* __restrict__ misc) {
+                                     const SysDesc& s, int* __restrict__ misc, uint4* __restrict__ uq_next = nullptr) {
     const int n = A.n;
     if (sizeof(R) == 4) {
         A.f[(size_t)F_FX * n + i] = (float)fx;
@@ -145,6 +147,7 @@ __device__ inline void finalize_atom(int i, const double4& pi, double fx, double
             r[c] += v[c] * S.dt;
         }
         pos_next[i] = make_double4(r[0], r[1], r[2], pi.w);
+        if (uq_next) uq_next[i] = fixq4(r[0], r[1], r[2], pi.w, s);
         const double4 q = in.rs;
         double dx = r[0] - q.x, dy = r[1] - q.y, dz = r[2] - q.z;
         shortest3(s, dx, dy, dz);
@@ -183,6 +186,15 @@ __device__ inline void next_group(const AtomSlots& as, int m0, int M, const int*
         const int c = m0 + U + u;
         en[u] = c < M ? nl[as.base + (long long)c * kWave] : 0;
     }
+}
+
+// The same with every read issued (chunks past M read the last chunk, whose
+// entries the loop then ignores), so that the compiler can count the loads
+// in flight and wait for the current group's gathers alone.
+template <int U>
+__device__ inline void next_group_clamped(const AtomSlots& as, int m0, int M, const int* __restrict__ nl, int* en) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) en[u] = nl[as.base + (long long)min(m0 + U + u, M - 1) * kWave];
 }
 
 // Difference r_j - r_i of a listed pair under the reference's image rule

@@ -37,7 +37,7 @@ EXPORTS = [
     "mdc_sys_graph_stats", "mdc_ctx_set_spin", "mdc_sys_time_kernels", "mdc_sys_set_rebuild", "mdc_sys_set_boundary", "mdc_sys_set_thermostat",
     "mdc_nlist_build", "mdc_nlist_decide", "mdc_nlist_optimize_rns", "mdc_nlist_export", "mdc_nlist_stats",
     "mdc_compute", "mdc_compute_noalloc", "mdc_stage", "mdc_step", "mdc_thermo", "mdc_sync", "mdc_timing", "mdc_bpnn_inspect",
-    "mdc_copy_calibration", "mdc_sys_refill_box",
+    "mdc_copy_calibration", "mdc_sys_refill_box", "mdc_sys_step_path",
     "mdc_serve_create", "mdc_serve_add_potential", "mdc_serve_submit", "mdc_serve_wait", "mdc_serve_take",
     "mdc_serve_stats", "mdc_serve_destroy",
 ]
@@ -104,6 +104,7 @@ def load_library(path=LIB_PATH):
         "mdc_timing": (I, [P, I, DP, C.POINTER(C.c_int64)]),
         "mdc_bpnn_inspect": (I, [P, P, DP, DP, C.POINTER(I)]),
         "mdc_sys_refill_box": (I, [P, I, U64, P, DP, DP, C.POINTER(C.c_uint8)]),
+        "mdc_sys_step_path": (I, [P, I, C.POINTER(I)]),
         "mdc_serve_create": (I, [I, D, I, I, PP]),
         "mdc_serve_add_potential": (I, [P, P, I, C.POINTER(I)]),
         "mdc_serve_submit": (I, [P, I, P, U64, DP, DP, C.POINTER(C.c_uint8), D, I, C.POINTER(U64)]),
@@ -467,6 +468,16 @@ class System:
         dE = np.zeros((self.n, nin.value))
         _check(lib().mdc_bpnn_inspect(self.h, pot.h, _dp(G), _dp(dE), None))
         return G, dE
+
+    def step_path(self, set=-1):
+        """Step path of single-element EAM in mixed precision: set 0 lets
+        step sequences take the fixed-point kernels where every box has a
+        unique minimum image, 1 keeps the FP64-difference kernels, -1
+        leaves it.  Returns 1 when the last step sequence (or kernel
+        timing) ran the fixed-point kernels."""
+        last = C.c_int(0)
+        _check(lib().mdc_sys_step_path(self.h, set, C.byref(last)))
+        return last.value
 
     def timing(self, enable=-1):
         ms = np.zeros(8)
