@@ -167,9 +167,11 @@ mdc_status mdc_sys_natoms(const mdc_sys* sys, uint64_t* n);
 mdc_status mdc_sys_set_lanes(mdc_sys* sys, int lanes_per_atom);
 /* Times 1, 2, 4, 8 and 16 lanes per atom and keeps the fastest, leaving a
  * fresh list and forces for it: for LJ and EAM the kernels of a step
- * (density and force in step mode, max(reps, 20) launches back to back),
- * for BPNN one force evaluation (MDC_FORCE, reps repetitions after a
- * warm-up), at the given precision.
+ * (density and force in step mode, max(reps, 20) launches back to back;
+ * where the fixed-point step path applies, with the tables read from
+ * global memory and staged in LDS, keeping the faster, see
+ * mdc_sys_step_path), for BPNN one force evaluation (MDC_FORCE, reps
+ * repetitions after a warm-up), at the given precision.
  * us_per_eval (may be NULL) receives the 5 mean times in microseconds.
  * No reference counterpart: the reference has no device launch shape. */
 mdc_status mdc_sys_autotune(mdc_sys* sys, const mdc_pot* pot, int precision, int reps, int* best_lanes,
@@ -185,15 +187,19 @@ mdc_status mdc_sys_set_block(mdc_sys* sys, int threads);
  * No reference counterpart. */
 mdc_status mdc_sys_autotune_tiles(mdc_sys* sys, const mdc_pot* pot, int precision, int reps, int* best_lanes,
                                   int* best_threads, double* us);
-/* Step path of single-element EAM in mixed precision.  By default (set =
- * 0) a step sequence whose boxes all keep a unique minimum image between
- * rebuilds gathers neighbours as 32-bit fixed-point coordinates (a wrapped
- * integer difference is the minimum image; 16 bytes per neighbour, FP32
- * pair arithmetic); set = 1 keeps the FP64-difference kernels for every
- * step; set = -1 leaves the choice as it is.  *last (may be NULL) receives
- * 1 when the last step sequence or kernel timing of this system ran the
- * fixed-point kernels, else 0.
- * No reference counterpart: a choice between two device paths. */
+/* Step path of single-element EAM in mixed precision.  A step sequence
+ * whose boxes all keep a unique minimum image between rebuilds gathers
+ * neighbours as 32-bit fixed-point coordinates (a wrapped integer
+ * difference is the minimum image; 16 bytes per neighbour, FP32 pair
+ * arithmetic), reading the interval records of the tables from global
+ * memory (set = 0) or from a copy staged in LDS by persistent workgroups
+ * (set = 2; the default for systems of 65 536 records and more, and
+ * mdc_sys_autotune picks between the two); set = 1 keeps the
+ * FP64-difference kernels for every step; set = -1 leaves the choice as it
+ * is.  *last (may be NULL) receives what the last step sequence or kernel
+ * timing of this system ran: 0 the FP64-difference kernels, 1 the
+ * fixed-point kernels with global tables, 2 with tables in LDS.
+ * No reference counterpart: a choice between device paths. */
 mdc_status mdc_sys_step_path(mdc_sys* sys, int set, int* last);
 /* Graph mode (on != 0) of mdc_compute on this system: the device work of
  * an evaluation (fills, density and force kernels, the counter readback) is

@@ -482,6 +482,13 @@ __global__ void k_set_inactive(double4* __restrict__ pos, const int* __restrict_
     pos[p] = v;
 }
 
+// New positions: the least pair distance the fixed-point passes met is
+// forgotten.
+static void reset_rmin(Batch& b) {
+    static const int none = kRminNone;
+    HIPCHECK(hipMemcpyAsync(b.misc + M_RMIN, &none, sizeof(int), hipMemcpyHostToDevice, b.ctx->stream));
+}
+
 void free_batch(Batch& b) {
     dfree(b.pos[0]);
     dfree(b.pos[1]);
@@ -680,7 +687,10 @@ void upload(Batch& b, const AtomRec* a, uint32_t fields) {
     HIPCHECK(hipStreamSynchronize(st));
     if (fields & MDC_F_F) b.f_prec = P_FP64;
     if (fields & MDC_F_RCUT) b.last_rcut = -1;
-    if (fields & MDC_F_R) b.shifts_stale = true;
+    if (fields & MDC_F_R) {
+        b.shifts_stale = true;
+        reset_rmin(b);
+    }
     // every record now holds an atom (the tag words were written afresh):
     // records of the void tail became live, so the next build sorts all
     long long held = 0;
@@ -790,6 +800,7 @@ void refill_box(Batch& b, int box, int count, const AtomRec* a, const double* lo
     b.full_sort = true;
     b.shifts_stale = true;
     b.last_rcut = -1;
+    reset_rmin(b);
 }
 
 // records [first, first + count) of the caller's order into a[0, count)
@@ -1264,6 +1275,8 @@ mdc_status mdc_sys_create(mdc_ctx* ctx, int nsys, const uint64_t* natoms, const 
             b.misc = dalloc<int>(M_COUNT);
             b.thermo = dalloc<double>((size_t)9 * nsys + 2);
             HIPCHECK(hipMemset(b.misc, 0, sizeof(int) * M_COUNT));
+            const int rmin_none = kRminNone;
+            HIPCHECK(hipMemcpy(b.misc + M_RMIN, &rmin_none, sizeof(int), hipMemcpyHostToDevice));
             HIPCHECK(hipMemset(b.pos[0], 0, sizeof(double4) * std::max(n, 1)));
             HIPCHECK(hipMemset(b.rsave, 0, sizeof(double4) * std::max(n, 1)));
             HIPCHECK(hipMemset(b.nn, 0, sizeof(int) * std::max(n, 1)));
@@ -1272,6 +1285,9 @@ mdc_status mdc_sys_create(mdc_ctx* ctx, int nsys, const uint64_t* natoms, const 
             HIPCHECK(hipMemcpy(b.where, iota.data(), sizeof(int) * n, hipMemcpyHostToDevice));
             HIPCHECK(hipMemcpy(b.orig[0], iota.data(), sizeof(int) * n, hipMemcpyHostToDevice));
             b.tpa = n >= 200000 ? 4 : 8;
+            // the staged tables pay off once a launch has thousands of
+            // wavefronts (mdc_sys_autotune measures it)
+            b.q_lds = n >= 65536;
             layout_cells(b);
             size_list(b);
         } catch (...) {
@@ -1587,6 +1603,8 @@ mdc_status mdc_sys_autotune(mdc_sys* sys, const mdc_pot* pot, int precision, int
         HIPCHECK(hipEventCreate(&e1));
         double best = 1e300;
         int best_l = b.tpa;
+        const bool lds0 = b.q_lds;
+        bool best_lds = lds0, lds_l = lds0;
         try {
             for (int c = 0; c < 5; ++c) {
                 const int l = cand[c];
@@ -1604,24 +1622,38 @@ mdc_status mdc_sys_autotune(mdc_sys* sys, const mdc_pot* pot, int precision, int
                     us = 1e3 * ms / reps;
                 } else {
                     // pair potentials: the kernels a step runs (density and
-                    // force in step mode), back to back, without host syncs
+                    // force in step mode), back to back, without host syncs;
+                    // on the fixed-point path both table placements
                     double t[2];
+                    b.q_lds = false;
                     time_step_kernels(b, pot->p, precision, 1e-6, std::max(reps, 20), t);
                     us = t[0] + t[1];
+                    lds_l = false;
+                    if (b.q_last) {
+                        b.q_lds = true;
+                        time_step_kernels(b, pot->p, precision, 1e-6, std::max(reps, 20), t);
+                        if (t[0] + t[1] < us) {
+                            us = t[0] + t[1];
+                            lds_l = true;
+                        }
+                    }
                 }
                 if (us_per_eval) us_per_eval[c] = us;
                 if (us < best) {
                     best = us;
                     best_l = l;
+                    best_lds = lds_l;
                 }
             }
         } catch (...) {
+            b.q_lds = lds0;
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
             throw;
         }
         HIPCHECK(hipEventDestroy(e0));
         HIPCHECK(hipEventDestroy(e1));
+        b.q_lds = best_lds;
         set_lanes(b, best_l);
         nlist_rebuild(b);
         force_compute(b, pot->p, MDC_FORCE, precision);
@@ -1641,9 +1673,14 @@ mdc_status mdc_sys_set_block(mdc_sys* sys, int threads) {
 mdc_status mdc_sys_step_path(mdc_sys* sys, int set, int* last) {
     return guard(sys ? sys->b.ctx : nullptr, [&] {
         if (!sys) throw Error(E_INVAL, "null argument");
-        if (set < -1 || set > 1) throw Error(E_INVAL, "set must be -1 (keep), 0 (automatic) or 1 (FP64 differences)");
-        if (set >= 0) sys->b.q_off = set == 1;
-        if (last) *last = sys->b.q_last ? 1 : 0;
+        if (set < -1 || set > 2)
+            throw Error(E_INVAL, "set must be -1 (keep), 0 (fixed point, global tables), 1 (FP64 differences) or "
+                                 "2 (fixed point, tables in LDS)");
+        if (set >= 0) {
+            sys->b.q_off = set == 1;
+            if (set != 1) sys->b.q_lds = set == 2;
+        }
+        if (last) *last = sys->b.q_last;
     });
 }
 

@@ -42,7 +42,11 @@ enum What { W_FORCE = 1, W_ENERGY = 2, W_VIRIAL = 4, W_NC_STORED = 8 };
 enum SolverStage { ST_PREPARE = 0, ST_CHARGE = 1, ST_FORCES = 2, ST_VIRIALS = 3 };
 enum PotType { PT_LJ = 1, PT_EAM = 2, PT_BPNN = 4 };
 // misc device counters
-enum Misc { M_MAXNN = 0, M_ABORT = 1, M_ERR = 2, M_DECIDE = 3, M_COUNT = 8 };
+// M_RMIN: the least squared pair distance (FP32 bits) the fixed-point LDS
+// passes have met since positions were last uploaded; it sets the part of
+// the table they stage
+enum Misc { M_MAXNN = 0, M_ABORT = 1, M_ERR = 2, M_DECIDE = 3, M_RMIN = 4, M_COUNT = 8 };
+constexpr int kRminNone = 0x7f800000;  // +inf: nothing met yet
 enum ErrBits { EB_RHO_RANGE = 1, EB_NONFINITE = 2, EB_R_RANGE = 4 };
 
 struct Ctx {
@@ -192,7 +196,8 @@ struct Batch {
     int uq_n = 0;
     bool qfast = false;    // set while a step sequence or a kernel timing runs on this path
     bool q_off = false;    // mdc_sys_step_path(1): never take it
-    bool q_last = false;   // the last step sequence or timing took it
+    bool q_lds = true;     // the variants that stage the table in LDS
+    int q_last = 0;        // the last step sequence or timing: 0 not taken, 1 global tables, 2 tables in LDS
     bool all_shift = false;  // every box's shift_ok (set_trigger)
     int* where = nullptr;  // original index -> sorted index
     // cells

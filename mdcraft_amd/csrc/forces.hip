@@ -580,6 +580,10 @@ __global__ void __launch_bounds__(256) k_eam_force(const double4* __restrict__ p
 // 356-497) on the same FP32 interval records.
 // --------------------------------------------------------------------------
 
+// Slots of one lane per group of the fixed-point kernels (four measured no
+// faster at 32 000 to a million atoms).
+constexpr int kUq = 2;
+
 // Pair geometry of the fixed-point path: difference, distance and table
 // interval; ok is false for a pair beyond the cutoff or the table.
 struct QGeo {
@@ -589,17 +593,30 @@ struct QGeo {
 };
 __device__ inline QGeo q_geo(const uint4& uj, const uint4& ui, bool listed, float qx, float qy, float qz, float rc2,
                              float dr1, int nr) {
+#pragma clang fp contract(off)
     QGeo g;
     g.dx = fixq_d(uj.x, ui.x, qx);
     g.dy = fixq_d(uj.y, ui.y, qy);
     g.dz = fixq_d(uj.z, ui.z, qz);
-    g.r2 = g.dx * g.dx + g.dy * g.dy + g.dz * g.dz;
+    g.r2 = fmaf(g.dz, g.dz, fmaf(g.dy, g.dy, g.dx * g.dx));
     g.ok = listed && !(g.r2 > rc2 || g.r2 < 1e-15f);
     g.R1 = __builtin_amdgcn_sqrtf(g.r2);
     g.RN = g.R1 * dr1;
     g.N1 = (int)g.RN;
     g.ok = g.ok && g.N1 < nr;
     return g;
+}
+
+// Threads of the persistent LDS-table workgroups (one per CU).
+constexpr int kQlThreads = 1024;
+
+// Zeros the compiler cannot see through: the value of a masked-off table
+// read.  With literal zeros it would fold the arithmetic on the record into
+// the branch that reads it, and wait there for the read.
+__device__ inline float4 opaque_zero4() {
+    float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+    asm volatile("" : "+v"(z.x), "+v"(z.y), "+v"(z.z), "+v"(z.w));
+    return z;
 }
 
 // Fixed-point records of pos [0, live): after a sort, before a timing.
@@ -611,24 +628,71 @@ __global__ void __launch_bounds__(256) k_fixq(const double4* __restrict__ pos, u
     uq[i] = fixq4(p.x, p.y, p.z, p.w, desc[w_sys(p.w)]);
 }
 
-// Density pass: rho_i from the rho records, then F(rho), F'(rho) in FP64
-// as k_eam_rho; writes the atom's record with F' in w for the force pass.
+// The wavefront gw of a list packed with TPA lanes per atom (atom_slots for
+// kernels that pick their wavefronts themselves).
 template <int TPA>
-__global__ void __launch_bounds__(256) k_eam_rho_q(const uint4* __restrict__ uq, uint4* __restrict__ uqF,
-                                                   const int* __restrict__ nl, const int* __restrict__ nn, int M,
-                                                   int cap, const SysDesc* __restrict__ desc, EamK P, AtomArrays A,
-                                                   StepArgs S, int* __restrict__ misc) {
-    const AtomSlots as = atom_slots<TPA>(M);
+__device__ inline AtomSlots atom_slots_w(long long gw, int M) {
+    const int lane = threadIdx.x & (kWave - 1);
+    AtomSlots a;
+    a.i = (int)(gw * (kWave / TPA) + lane / TPA);
+    a.sub = lane % TPA;
+    a.base = gw * (long long)M * kWave + lane;
+    return a;
+}
+
+// Table records of the fixed-point passes: from a copy of records [lo, nr)
+// staged in LDS (LDS variants) or from global memory.  RPN: records per
+// interval (1: rho; 2: rho and r*phi side by side).
+template <bool LDS, int RPN>
+__device__ inline void q_tab(__amdgpu_buffer_rsrc_t tb, const float4* s_tab, int lo, int N1, float4* rec) {
+    if (LDS && N1 >= lo) {
+#pragma unroll
+        for (int k = 0; k < RPN; ++k) rec[k] = s_tab[RPN * (N1 - lo) + k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < RPN; ++k) rec[k] = buf_f4(tb, (unsigned)(RPN * N1 + k) * 16u);
+    }
+}
+
+// The workgroup's least squared pair distance into misc[M_RMIN] when it is
+// below the value the workgroup read (was) at its start: one atomic per
+// workgroup, and none once the least distance is known (atomics of every
+// wavefront on the one address serialised the pass).  Positive FP32 values
+// order as their bit patterns do.
+__device__ inline void q_note_rmin(float rmin, int was, int* __restrict__ misc) {
+    __shared__ float s_min[kQlThreads / kWave];
+    for (int o = kWave / 2; o > 0; o >>= 1) rmin = fminf(rmin, __shfl_xor(rmin, o, kWave));
+    if (__lane_id() == 0) s_min[threadIdx.x / kWave] = rmin;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x / kWave); ++w) rmin = fminf(rmin, s_min[w]);
+        if (__float_as_int(rmin) < was) atomicMin(&misc[M_RMIN], __float_as_int(rmin));
+    }
+}
+
+// Density pass of one wavefront: rho_i from the rho records, then F(rho),
+// F'(rho) in FP64 as k_eam_rho; writes the atom's record with F' in w for
+// the force pass.
+template <int TPA, bool ONE, bool LDS>
+__device__ inline void rho_q_wave(const AtomSlots& as, const uint4* __restrict__ uq, uint4* __restrict__ uqF,
+                                  const int* __restrict__ nl, const int* __restrict__ nn, int M, int cap,
+                                  const SysDesc* __restrict__ desc, const EamK& P, const AtomArrays& A,
+                                  const StepArgs& S, int* __restrict__ misc, const float4* s_tab, int lo, int ab,
+                                  float& rmin) {
+    // every fused multiply-add written out: the LDS and global variants then
+    // round alike, whatever the compiler would contract in each
+#pragma clang fp contract(off)
     const int i = as.i;
     const bool valid = i < A.live;
-    const int ab = abort_flag(S, misc);
     const uint4 ui = valid ? uq[i] : make_uint4(0, 0, 0, 0);
     const int nraw = valid ? nn[i] : 0;
-    int j0[kU];
-    first_group<TPA, kU>(as, valid, M, nl, j0);
+    int j0[kUq];
+    first_group<TPA, kUq>(as, valid, M, nl, j0);
+    // the step's abort flag (read by the caller) is tested after the
+    // prologue reads are issued, so that their latencies overlap
     if (ab <= S.step) return;
     const bool active = valid && !((ui.w >> 8) & 1u);
-    const SysDesc& s = desc[valid ? (int)(ui.w >> 10) : 0];
+    const SysDesc& s = desc[ONE || !valid ? 0 : (int)(ui.w >> 10)];
     const float qx = s.qf[0], qy = s.qf[1], qz = s.qf[2];
     const int cnt = active ? min(nraw, cap) : 0;
     const float rc2 = (float)P.rc2, dr1 = (float)P.dr1;
@@ -636,31 +700,36 @@ __global__ void __launch_bounds__(256) k_eam_rho_q(const uint4* __restrict__ uq,
     double rho = 0.0;
     const __amdgpu_buffer_rsrc_t ub = make_buf(uq, (unsigned)A.n * 16u);
     const __amdgpu_buffer_rsrc_t tabb = make_buf(P.r32, (unsigned)nr * 16u);
-    int jn[kU];
+    const float4 z4 = opaque_zero4();
+    int jn[kUq];
 #pragma unroll
-    for (int u = 0; u < kU; ++u) jn[u] = j0[u];
+    for (int u = 0; u < kUq; ++u) jn[u] = j0[u];
     const int nch = wave_chunks<TPA>(cnt);
-    for (int m0 = 0; m0 < nch; m0 += kU) {
-        bool okk[kU];
-        uint4 uj[kU];
+    for (int m0 = 0; m0 < nch; m0 += kUq) {
+        bool okk[kUq];
+        uint4 uj[kUq];
+        // the gathers of unused slots are masked off
 #pragma unroll
-        for (int u = 0; u < kU; ++u) {
+        for (int u = 0; u < kUq; ++u) {
             okk[u] = (m0 + u) * TPA + as.sub < cnt;
-            uj[u] = buf_u4(ub, okk[u] ? (unsigned)ent_j(jn[u]) * 16u : kSkip);
+            uj[u] = ui;
+            if (okk[u]) uj[u] = buf_u4(ub, (unsigned)ent_j(jn[u]) * 16u);
         }
-        next_group_clamped<kU>(as, m0, M, nl, jn);
-        QGeo g[kU];
-        float4 rec[kU];
+        next_group_clamped<kUq>(as, m0, M, nl, jn);
+        QGeo g[kUq];
+        float4 rec[kUq];
 #pragma unroll
-        for (int u = 0; u < kU; ++u) {
+        for (int u = 0; u < kUq; ++u) {
             g[u] = q_geo(uj[u], ui, okk[u], qx, qy, qz, rc2, dr1, nr);
-            rec[u] = buf_f4(tabb, g[u].ok ? (unsigned)g[u].N1 * 16u : kSkip);
+            rec[u] = z4;
+            if (g[u].ok) q_tab<LDS, 1>(tabb, s_tab, lo, g[u].N1, &rec[u]);
+            if (LDS) rmin = fminf(rmin, g[u].ok ? g[u].r2 : rmin);
         }
 #pragma unroll
-        for (int u = 0; u < kU; ++u) {
+        for (int u = 0; u < kUq; ++u) {
             const float t = g[u].RN - (float)g[u].N1;
             const float4 c = rec[u];
-            rho += (double)(c.x + t * (c.y + t * (c.z + t * c.w)));
+            rho += (double)fmaf(t, fmaf(t, fmaf(t, c.w, c.z), c.y), c.x);
         }
     }
     rho = wave_sum_group(rho, TPA);
@@ -676,77 +745,83 @@ __global__ void __launch_bounds__(256) k_eam_rho_q(const uint4* __restrict__ uq,
     if (active && S.store_aux) A.d[(size_t)D_NC * n + i] = rho;
 }
 
-// Force pass with the fused velocity-Verlet finish; also writes the drifted
-// position's fixed-point record.
-template <int TPA>
-__global__ void __launch_bounds__(256) k_eam_force_q(const double4* __restrict__ pos, double4* __restrict__ pos_next,
-                                                     const uint4* __restrict__ uqF, uint4* __restrict__ uq_next,
-                                                     const int* __restrict__ nl, const int* __restrict__ nn, int M,
-                                                     int cap, const SysDesc* __restrict__ desc, EamK P, AtomArrays A,
-                                                     StepArgs S, const double4* __restrict__ rsave,
-                                                     int* __restrict__ misc) {
-    const AtomSlots as = atom_slots<TPA>(M);
+// Force pass of one wavefront with the fused velocity-Verlet finish; also
+// writes the drifted position's fixed-point record.
+template <int TPA, bool ONE, bool LDS>
+__device__ inline void force_q_wave(const AtomSlots& as, const double4* __restrict__ pos,
+                                    double4* __restrict__ pos_next, const uint4* __restrict__ uqF,
+                                    uint4* __restrict__ uq_next, const int* __restrict__ nl,
+                                    const int* __restrict__ nn, int M, int cap, const SysDesc* __restrict__ desc,
+                                    const EamK& P, const AtomArrays& A, const StepArgs& S,
+                                    const double4* __restrict__ rsave, int* __restrict__ misc, const float4* s_tab,
+                                    int lo, int ab, float& rmin) {
+#pragma clang fp contract(off)
     const int i = as.i;
     const bool valid = i < A.live;
     const int n = A.n;
-    const int ab = abort_flag(S, misc);
     const double4 pi = valid ? pos[i] : make_double4(0, 0, 0, 0);
     const uint4 ui = valid ? uqF[i] : make_uint4(0, 0, 0, 0);
     const int nraw = valid ? nn[i] : 0;
-    int j0[kU];
-    first_group<TPA, kU>(as, valid, M, nl, j0);
+    int j0[kUq];
+    first_group<TPA, kUq>(as, valid, M, nl, j0);
     if (ab <= S.step) return;
     const bool active = valid && !w_inactive(pi.w);
-    const SysDesc& s = desc[valid ? w_sys(pi.w) : 0];
+    const SysDesc& s = desc[ONE || !valid ? 0 : w_sys(pi.w)];
     const float qx = s.qf[0], qy = s.qf[1], qz = s.qf[2];
     const int cnt = active ? min(nraw, cap) : 0;
     const float rc2 = (float)P.rc2, dr1 = (float)P.dr1;
     const int nr = P.nr;
     const float Fpi = active ? __uint_as_float(ui.w) : 0.f;
+    // the finish's inputs are read now, so that their latency overlaps the
+    // neighbour loop
+    const FinIn fin = load_fin(i, A, S, rsave, valid);
     float fx = 0, fy = 0, fz = 0;
     const __amdgpu_buffer_rsrc_t ub = make_buf(uqF, (unsigned)n * 16u);
     // each interval's rho and r*phi records side by side (32 bytes)
     const __amdgpu_buffer_rsrc_t tabpb = make_buf(P.r32p, (unsigned)nr * 32u);
-    int jn[kU];
+    const float4 z4 = opaque_zero4();
+    int jn[kUq];
 #pragma unroll
-    for (int u = 0; u < kU; ++u) jn[u] = j0[u];
+    for (int u = 0; u < kUq; ++u) jn[u] = j0[u];
     const int nch = wave_chunks<TPA>(cnt);
-    for (int m0 = 0; m0 < nch; m0 += kU) {
-        bool okk[kU];
-        uint4 uj[kU];
+    for (int m0 = 0; m0 < nch; m0 += kUq) {
+        bool okk[kUq];
+        uint4 uj[kUq];
+        // the gathers of unused slots are masked off
 #pragma unroll
-        for (int u = 0; u < kU; ++u) {
+        for (int u = 0; u < kUq; ++u) {
             okk[u] = (m0 + u) * TPA + as.sub < cnt;
-            uj[u] = buf_u4(ub, okk[u] ? (unsigned)ent_j(jn[u]) * 16u : kSkip);
+            uj[u] = ui;
+            if (okk[u]) uj[u] = buf_u4(ub, (unsigned)ent_j(jn[u]) * 16u);
         }
-        next_group_clamped<kU>(as, m0, M, nl, jn);
-        QGeo g[kU];
-        float4 rr[kU], rp[kU];
+        next_group_clamped<kUq>(as, m0, M, nl, jn);
+        QGeo g[kUq];
+        float4 rec[kUq][2];
 #pragma unroll
-        for (int u = 0; u < kU; ++u) {
+        for (int u = 0; u < kUq; ++u) {
             g[u] = q_geo(uj[u], ui, okk[u], qx, qy, qz, rc2, dr1, nr);
-            const unsigned off = g[u].ok ? (unsigned)g[u].N1 * 32u : kSkip;
-            rr[u] = buf_f4(tabpb, off);
-            rp[u] = buf_f4(tabpb, off + 16u);
+            rec[u][0] = rec[u][1] = z4;
+            if (g[u].ok) q_tab<LDS, 2>(tabpb, s_tab, lo, g[u].N1, rec[u]);
+            if (LDS) rmin = fminf(rmin, g[u].ok ? g[u].r2 : rmin);
         }
 #pragma unroll
-        for (int u = 0; u < kU; ++u) {
-            const float t = g[u].RN - (float)g[u].N1;
-            const float4 a = rr[u], c = rp[u];
+        for (int u = 0; u < kUq; ++u) {
+            const float t = g[u].RN - (float)g[u].N1, t3 = 3.f * t;
+            const float4 a = rec[u][0], c = rec[u][1];
             // rho'(r), r*phi and (r*phi)'(r) of the interval cubic
-            const float nd = (a.y + t * (2.f * a.z + 3.f * t * a.w)) * dr1;
-            const float phv = c.x + t * (c.y + t * (c.z + t * c.w));
-            const float phd = (c.y + t * (2.f * c.z + 3.f * t * c.w)) * dr1;
+            const float nd = fmaf(t, fmaf(t3, a.w, 2.f * a.z), a.y) * dr1;
+            const float phv = fmaf(t, fmaf(t, fmaf(t, c.w, c.z), c.y), c.x);
+            const float phd = fmaf(t, fmaf(t3, c.w, 2.f * c.z), c.y) * dr1;
             const float i1 = rcp_(g[u].R1), i2 = i1 * i1;
             const float AdF = Fpi + __uint_as_float(uj[u].w);
             // computed for every lane (the records of unused pairs read as
             // zeros), then selected: a conditional expression would become
             // a branch with the table reads sunk into it, one at a time
-            float FF = (AdF * nd + (phd * g[u].R1 - phv) * i2) * i1;
+            float FF = fmaf(AdF, nd, fmaf(phd, g[u].R1, -phv) * i2) * i1;
             FF = g[u].ok ? FF : 0.f;
-            fx += FF * g[u].dx;
-            fy += FF * g[u].dy;
-            fz += FF * g[u].dz;
+            fx = fmaf(FF, g[u].dx, fx);
+            fy = fmaf(FF, g[u].dy, fy);
+            fz = fmaf(FF, g[u].dz, fz);
         }
     }
     fx = wave_sum_group(fx, TPA);
@@ -760,8 +835,107 @@ __global__ void __launch_bounds__(256) k_eam_force_q(const double4* __restrict__
         }
         return;
     }
-    const FinIn fin = load_fin(i, A, S, rsave, true);
     finalize_atom<float>(i, pi, (double)fx, (double)fy, (double)fz, A, S, pos_next, fin, s, misc, uq_next);
+}
+
+// One wavefront per 64 / TPA atoms, tables read from global memory.
+template <int TPA, bool ONE>
+__global__ void __launch_bounds__(256) k_eam_rho_q(const uint4* __restrict__ uq, uint4* __restrict__ uqF,
+                                                   const int* __restrict__ nl, const int* __restrict__ nn, int M,
+                                                   int cap, const SysDesc* __restrict__ desc, EamK P, AtomArrays A,
+                                                   StepArgs S, int* __restrict__ misc) {
+    float rmin = 0.f;  // not tracked without staged tables
+    rho_q_wave<TPA, ONE, false>(atom_slots<TPA>(M), uq, uqF, nl, nn, M, cap, desc, P, A, S, misc, nullptr, 0,
+                                abort_flag(S, misc), rmin);
+}
+
+template <int TPA, bool ONE>
+__global__ void __launch_bounds__(256) k_eam_force_q(const double4* __restrict__ pos, double4* __restrict__ pos_next,
+                                                     const uint4* __restrict__ uqF, uint4* __restrict__ uq_next,
+                                                     const int* __restrict__ nl, const int* __restrict__ nn, int M,
+                                                     int cap, const SysDesc* __restrict__ desc, EamK P, AtomArrays A,
+                                                     StepArgs S, const double4* __restrict__ rsave,
+                                                     int* __restrict__ misc) {
+    float rmin = 0.f;
+    force_q_wave<TPA, ONE, false>(atom_slots<TPA>(M), pos, pos_next, uqF, uq_next, nl, nn, M, cap, desc, P, A, S,
+                                  rsave, misc, nullptr, 0, abort_flag(S, misc), rmin);
+}
+
+// LDS variants: persistent workgroups of kQlThreads, one per CU, each of
+// which stages the records [lo, nr) of the table once and then takes
+// wavefronts of atoms in turn.  The neighbour gathers then leave the
+// texture addresser, which bounds the passes with global table reads, to
+// the list and position records.  Wavefronts are dealt per XCD: the
+// workgroups of XCD x (blockIdx.x mod 8) take the x-th eighth of the
+// cell-ordered wavefronts, whose neighbours stay in that XCD's L2.
+__device__ inline void ql_range(long long nwaves, long long& first, long long& end, long long& step) {
+    const int x = blockIdx.x % kXcd, k = blockIdx.x / kXcd, per = gridDim.x / kXcd;
+    const int wpb = blockDim.x / kWave;
+    first = nwaves * x / kXcd + (long long)k * wpb + threadIdx.x / kWave;
+    end = nwaves * (x + 1) / kXcd;
+    step = (long long)per * wpb;
+}
+
+// Stages the records [lo, nr) of the table and returns lo: the least
+// interval a pair has met so far (misc[M_RMIN], less 2 %), but no lower than
+// lo_cap, the least one the LDS can hold.  Pairs below lo read the table in
+// global memory, so any lo gives the same results.
+template <int RPN>
+__device__ inline int ql_stage(float4* s_tab, const float4* __restrict__ tab, int lo_cap, int nr, float dr1,
+                               const int* __restrict__ misc, int& was) {
+    // one read for the workgroup: other workgroups lower the value meanwhile
+    __shared__ int s_lo, s_was;
+    if (threadIdx.x == 0) {
+        const int rb = __atomic_load_n(&misc[M_RMIN], __ATOMIC_RELAXED);
+        const float f = fminf(0.98f * __builtin_amdgcn_sqrtf(__int_as_float(rb)) * dr1, (float)nr);
+        s_lo = max(lo_cap, (int)f);
+        s_was = rb;
+    }
+    __syncthreads();
+    const int lo = s_lo;
+    was = s_was;
+    const int nt = RPN * (nr - lo);
+    for (int q = threadIdx.x; q < nt; q += blockDim.x) s_tab[q] = tab[RPN * lo + q];
+    __syncthreads();
+    return lo;
+}
+
+template <int TPA, bool ONE>
+__global__ void __launch_bounds__(kQlThreads) k_eam_rho_ql(const uint4* __restrict__ uq, uint4* __restrict__ uqF,
+                                                          const int* __restrict__ nl, const int* __restrict__ nn,
+                                                          int M, int cap, const SysDesc* __restrict__ desc, EamK P,
+                                                          AtomArrays A, StepArgs S, int* __restrict__ misc, int lo,
+                                                          long long nwaves) {
+    extern __shared__ float4 s_tab[];
+    if (abort_flag(S, misc) <= S.step) return;
+    int was;
+    lo = ql_stage<1>(s_tab, P.r32, lo, P.nr, (float)P.dr1, misc, was);
+    long long gw, end, step;
+    ql_range(nwaves, gw, end, step);
+    float rmin = __int_as_float(kRminNone);
+    for (; gw < end; gw += step)
+        rho_q_wave<TPA, ONE, true>(atom_slots_w<TPA>(gw, M), uq, uqF, nl, nn, M, cap, desc, P, A, S, misc, s_tab,
+                                   lo, INT_MAX, rmin);
+    q_note_rmin(rmin, was, misc);
+}
+
+template <int TPA, bool ONE>
+__global__ void __launch_bounds__(kQlThreads) k_eam_force_ql(
+    const double4* __restrict__ pos, double4* __restrict__ pos_next, const uint4* __restrict__ uqF,
+    uint4* __restrict__ uq_next, const int* __restrict__ nl, const int* __restrict__ nn, int M, int cap,
+    const SysDesc* __restrict__ desc, EamK P, AtomArrays A, StepArgs S, const double4* __restrict__ rsave,
+    int* __restrict__ misc, int lo, long long nwaves) {
+    extern __shared__ float4 s_tab[];
+    if (abort_flag(S, misc) <= S.step) return;
+    int was;
+    lo = ql_stage<2>(s_tab, P.r32p, lo, P.nr, (float)P.dr1, misc, was);
+    long long gw, end, step;
+    ql_range(nwaves, gw, end, step);
+    float rmin = __int_as_float(kRminNone);
+    for (; gw < end; gw += step)
+        force_q_wave<TPA, ONE, true>(atom_slots_w<TPA>(gw, M), pos, pos_next, uqF, uq_next, nl, nn, M, cap, desc,
+                                     P, A, S, rsave, misc, s_tab, lo, INT_MAX, rmin);
+    q_note_rmin(rmin, was, misc);
 }
 
 // Opening half kick + drift of the first step of a sequence from the stored
@@ -1047,12 +1221,36 @@ static void launch_pair_t(Batch& b, const Pot& p, const StepArgs& S, const doubl
         if (sizeof(R) == 4 && b.qfast) {
             // fixed-point step path: uq[c] holds the records of pos = pos[c]
             const int c = pos == b.pos[0] ? 0 : 1;
+            // one box: its descriptor by uniform loads at the kernel's start
+            const bool one = b.nsys == 1;
+            uint4* uq_next = pos_next ? b.uq[c ^ 1] : (uint4*)nullptr;
+            if (b.q_lds) {
+                // persistent workgroups, one per CU, each staging the table
+                // records [lo, nr) that the LDS holds
+                // (less a little for the kernels' own LDS variables)
+                const size_t lmax = std::min<size_t>(b.ctx->lds_max, 160 * 1024) - 256;
+                const int nr = p.eam.nr;
+                const int lo1 = std::max(0, nr - (int)(lmax / 16)), lo2 = std::max(0, nr - (int)(lmax / 32));
+                const int wpb = kQlThreads / kWave;
+                int G = (int)std::min<long long>(b.ctx->cu_count, (waves + wpb - 1) / wpb);
+                G = std::max(kXcd, (G + kXcd - 1) / kXcd * kXcd);
+                if (which & 1)
+                    launch_k(b, K_DENSITY, one ? k_eam_rho_ql<TPA, true> : k_eam_rho_ql<TPA, false>, dim3(G),
+                             dim3(kQlThreads), (unsigned)(16 * (nr - lo1)), (const uint4*)b.uq[c], b.uqF, b.nl, b.nn,
+                             b.M, b.cap, b.d_desc, E, AA, S, b.misc, lo1, waves);
+                if (which & 2)
+                    launch_k(b, K_FORCE, one ? k_eam_force_ql<TPA, true> : k_eam_force_ql<TPA, false>, dim3(G),
+                             dim3(kQlThreads), (unsigned)(32 * (nr - lo2)), pos, pos_next, (const uint4*)b.uqF,
+                             uq_next, b.nl, b.nn, b.M, b.cap, b.d_desc, E, AA, S, b.rsave, b.misc, lo2, waves);
+                return;
+            }
             if (which & 1)
-                launch_k(b, K_DENSITY, k_eam_rho_q<TPA>, dim3(grid), dim3(b.block), 0, (const uint4*)b.uq[c], b.uqF,
-                         b.nl, b.nn, b.M, b.cap, b.d_desc, E, AA, S, b.misc);
+                launch_k(b, K_DENSITY, one ? k_eam_rho_q<TPA, true> : k_eam_rho_q<TPA, false>, dim3(grid),
+                         dim3(b.block), 0, (const uint4*)b.uq[c], b.uqF, b.nl, b.nn, b.M, b.cap, b.d_desc, E, AA, S,
+                         b.misc);
             if (which & 2)
-                launch_k(b, K_FORCE, k_eam_force_q<TPA>, dim3(grid), dim3(b.block), 0, pos, pos_next,
-                         (const uint4*)b.uqF, pos_next ? b.uq[c ^ 1] : (uint4*)nullptr, b.nl, b.nn, b.M, b.cap,
+                launch_k(b, K_FORCE, one ? k_eam_force_q<TPA, true> : k_eam_force_q<TPA, false>, dim3(grid),
+                         dim3(b.block), 0, pos, pos_next, (const uint4*)b.uqF, uq_next, b.nl, b.nn, b.M, b.cap,
                          b.d_desc, E, AA, S, b.rsave, b.misc);
             return;
         }
@@ -1490,7 +1688,7 @@ void run_steps(Batch& b, const Pot& p, double dt, int nsteps, int prec, int chun
         ~QOff() { b.qfast = false; }
     } qoff{b};
     b.qfast = qf;
-    b.q_last = qf;
+    b.q_last = qf ? (b.q_lds ? 2 : 1) : 0;
     int c0 = b.cur;
     {
         AtomArrays AA = arrays(b);
@@ -1640,7 +1838,7 @@ void time_step_kernels(Batch& b, const Pot& p, int prec, double dt, int reps, do
         ~QOff() { b.qfast = false; }
     } qoff{b};
     b.qfast = qf;
-    b.q_last = qf;
+    b.q_last = qf ? (b.q_lds ? 2 : 1) : 0;
     hipEvent_t e[3];
     for (auto& x : e) HIPCHECK(hipEventCreate(&x));
     const bool timing = b.timing;

@@ -470,11 +470,13 @@ class System:
         return G, dE
 
     def step_path(self, set=-1):
-        """Step path of single-element EAM in mixed precision: set 0 lets
-        step sequences take the fixed-point kernels where every box has a
-        unique minimum image, 1 keeps the FP64-difference kernels, -1
-        leaves it.  Returns 1 when the last step sequence (or kernel
-        timing) ran the fixed-point kernels."""
+        """Step path of single-element EAM in mixed precision: where every
+        box has a unique minimum image, step sequences take the fixed-point
+        kernels with tables read from global memory (set 0) or staged in
+        LDS (set 2); 1 keeps the FP64-difference kernels; -1 leaves the
+        choice.  Returns what the last step sequence (or kernel timing) ran:
+        0 the FP64-difference kernels, 1 fixed point with global tables, 2
+        fixed point with tables in LDS."""
         last = C.c_int(0)
         _check(lib().mdc_sys_step_path(self.h, set, C.byref(last)))
         return last.value

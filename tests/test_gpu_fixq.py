@@ -1,6 +1,8 @@
 """The fixed-point step path of single-element EAM in mixed precision
 against the oracle's FP64 make_step loop with decide()/update() rebuilds,
-and against the FP64-difference mixed kernels (mdc_sys_step_path(1)).
+and against the FP64-difference mixed kernels (mdc_sys_step_path(1)); its
+two table placements (records read from global memory, path 0, or staged
+in LDS by persistent workgroups, path 2) against each other, bit for bit.
 
 On this path a step sequence keeps every atom's position also as three
 32-bit fixed-point coordinates (scale 2^32 / edge on a periodic axis, so
@@ -100,24 +102,43 @@ def eam_slab(reps=6):
     return w["atoms"], (lo, hi, (True, True, False)), w["pot"], w["dt"]
 
 
+@pytest.mark.parametrize("path", [0, 2])
 @pytest.mark.parametrize("lanes", [1, 4, 16])
-def test_bulk_nve_against_oracle(ctx, lanes):
+def test_bulk_nve_against_oracle(ctx, lanes, path):
     w = W.eam_cu(reps=6)
     steps = 40
     ref, st = oracle_nve(w["atoms"], w["box"], w["pot"], w["dt"], steps)
-    s, got, used = engine_nve(ctx, [w["box"]], [w["atoms"]], w["pot"], w["dt"], steps, lanes=lanes)
-    assert used == 1
-    check(got, s.thermo()[0], ref, st, w["box"], "bulk lanes %d" % lanes)
+    s, got, used = engine_nve(ctx, [w["box"]], [w["atoms"]], w["pot"], w["dt"], steps, lanes=lanes, path=path)
+    assert used == (1 if path == 0 else 2)
+    check(got, s.thermo()[0], ref, st, w["box"], "bulk lanes %d path %d" % (lanes, path))
 
 
+@pytest.mark.parametrize("path", [0, 2])
 @pytest.mark.parametrize("case", ["slab", "cluster"])
-def test_open_axes_against_oracle(ctx, case):
+def test_open_axes_against_oracle(ctx, case, path):
     at, box, pot, dt = eam_slab() if case == "slab" else eam_cluster()
     steps = 40
     ref, st = oracle_nve(at, box, pot, dt, steps)
-    s, got, used = engine_nve(ctx, [box], [at], pot, dt, steps, lanes=8)
-    assert used == 1
+    s, got, used = engine_nve(ctx, [box], [at], pot, dt, steps, lanes=8, path=path)
+    assert used == (1 if path == 0 else 2)
     check(got, s.thermo()[0], ref, st, box, case)
+
+
+@pytest.mark.parametrize("nr", [5000, 12000])
+def test_lds_tables_bit_identical_to_global_tables(ctx, nr):
+    """nr 12000: 384 KB of records, more than the LDS holds, so the pairs
+    below the staged part read the table in global memory."""
+    w = W.eam_cu(reps=10)
+    pot = dict(kind="eam", **W.eam_cu_tables(nr=nr))
+    steps = 40
+    runs = [engine_nve(ctx, [w["box"]], [w["atoms"]], pot, w["dt"], steps, lanes=4, path=p) for p in (0, 2)]
+    assert (runs[0][2], runs[1][2]) == (1, 2)
+    a, b = runs[0][1], runs[1][1]
+    for k in ("r", "v", "f"):
+        assert np.array_equal(vec(a, k), vec(b, k)), k
+    if nr == 12000:
+        ref, st = oracle_nve(w["atoms"], w["box"], pot, w["dt"], steps)
+        check(b, runs[1][0].thermo()[0], ref, st, w["box"], "nr 12000")
 
 
 def test_inactive_atoms_stay(ctx):
@@ -130,21 +151,22 @@ def test_inactive_atoms_stay(ctx):
     at["tag"][::13] = 0
     steps = 30
     ref, st = oracle_nve(at, w["box"], w["pot"], w["dt"], steps)
-    s, got, used = engine_nve(ctx, [w["box"]], [at], w["pot"], w["dt"], steps, lanes=4)
-    assert used == 1
+    s, got, used = engine_nve(ctx, [w["box"]], [at], w["pot"], w["dt"], steps, lanes=4, path=2)
+    assert used == 2
     frozen = at["tag"] == 0
     assert np.array_equal(vec(got, "r")[frozen], vec(at, "r")[frozen])
     check(got, s.thermo()[0], ref, st, w["box"], "inactive")
 
 
-def test_batch_of_boxes_against_each_oracle(ctx):
+@pytest.mark.parametrize("path", [0, 2])
+def test_batch_of_boxes_against_each_oracle(ctx, path):
     cases = [(W.eam_cu(reps=5)["atoms"], W.eam_cu(reps=5)["box"]), eam_slab(5)[:2],
              (W.eam_cu(reps=7, seed_index=3)["atoms"], W.eam_cu(reps=7)["box"]), eam_cluster(2.6)[:2]]
     w = W.eam_cu(reps=5)
     steps = 40
     s, got, used = engine_nve(ctx, [c[1] for c in cases], [c[0] for c in cases], w["pot"], w["dt"], steps,
-                              lanes=4)
-    assert used == 1
+                              lanes=4, path=path)
+    assert used == (1 if path == 0 else 2)
     th = s.thermo()
     o = 0
     for k, (at, box) in enumerate(cases):
@@ -153,25 +175,40 @@ def test_batch_of_boxes_against_each_oracle(ctx):
         o += len(at)
 
 
-def test_matches_fp64_difference_path(ctx):
+@pytest.mark.parametrize("path", [0, 2])
+def test_matches_fp64_difference_path(ctx, path):
     w = W.eam_cu(reps=6)
     steps = 40
-    s1, a, u1 = engine_nve(ctx, [w["box"]], [w["atoms"]], w["pot"], w["dt"], steps, lanes=4, path=0)
+    s1, a, u1 = engine_nve(ctx, [w["box"]], [w["atoms"]], w["pot"], w["dt"], steps, lanes=4, path=path)
     s2, b, u2 = engine_nve(ctx, [w["box"]], [w["atoms"]], w["pot"], w["dt"], steps, lanes=4, path=1)
-    assert (u1, u2) == (1, 0)
+    assert (u1, u2) == (1 if path == 0 else 2, 0)
     d = wrapped_diff(vec(a, "r"), vec(b, "r"), w["box"])
     assert np.max(np.abs(d)) < 1e-8, np.max(np.abs(d))
     assert force_rms_rel(vec(a, "f"), vec(b, "f")) <= 1e-5
 
 
-def test_headline_size_against_oracle(ctx):
+@pytest.mark.parametrize("path", [0, 2])
+def test_headline_size_against_oracle(ctx, path):
     """The 32 000-atom EAM Cu workload, 20 steps."""
     w = W.eam_cu()
     steps = 20
     ref, st = oracle_nve(w["atoms"], w["box"], w["pot"], w["dt"], steps)
-    s, got, used = engine_nve(ctx, [w["box"]], [w["atoms"]], w["pot"], w["dt"], steps)
-    assert used == 1
+    s, got, used = engine_nve(ctx, [w["box"]], [w["atoms"]], w["pot"], w["dt"], steps, path=path)
+    assert used == (1 if path == 0 else 2)
     check(got, s.thermo()[0], ref, st, w["box"], "32000")
+
+
+def test_autotune_keeps_the_faster_placement(ctx):
+    eng = E()
+    w = W.eam_cu()
+    s = eng.System(ctx, [len(w["atoms"])], [w["box"]])
+    s.upload(w["atoms"])
+    p = engine_potential(eng, ctx, w["pot"])
+    s.build()
+    s.compute(p, 1, MIXED)
+    s.autotune(p, MIXED, reps=5)
+    s.step(p, w["dt"], 5, MIXED)
+    assert s.step_path() in (1, 2)
 
 
 def test_manual_rebuild_and_fp64_keep_fp64_differences(ctx):
@@ -189,6 +226,6 @@ def test_manual_rebuild_and_fp64_keep_fp64_differences(ctx):
     assert s.step_path() == 0
     s.set_rebuild(True)
     s.step(p, w["dt"], 3, MIXED)
-    assert s.step_path() == 1
+    assert s.step_path() == 1  # 500 records: global tables by default
     s.step(p, w["dt"], 3, 0)
     assert s.step_path() == 0
