@@ -26,6 +26,11 @@ HBM_PEAK_GBS = 8000.0
 EV_KJMOL = 96.48533212331
 
 
+# mdc_sys_step_path: what a step sequence of single-element EAM ran
+STEP_PATHS = {0: "FP64 differences (k_eam_force)", 1: "fixed point, global tables (k_eam_force_q)",
+              2: "fixed point, tables in LDS (k_eam_force_ql)"}
+
+
 def algorithmic_bytes(kind, nbar):
     """Compulsory HBM bytes per atom for one launch of the dominant kernel
     (the pair force kernel): position read (24), list (4 per neighbour),
@@ -338,12 +343,13 @@ def run_gpu(E, ctx, w, precision, steps, warmup, lanes=None):
     s.step(p, w["dt"], min(steps, 200), precision)
     s.sync()
     ms, nl = s.timing(0)
+    path = s.step_path(-1) if pot["kind"] == "eam" else 0
     n = len(at)
     per_kernel = {KINDS[k]: round(ms[k] / nl[k] * 1e3, 2) for k in range(8) if nl[k] > 0}
     return {
         "n": n, "steps": steps, "seconds": t1 - t0, "value": n * steps / (t1 - t0), "kernel_us": per_kernel,
         "ms_per_step": 1e3 * (t1 - t0) / steps, "kernel_ms": ms.tolist(), "kernel_launches": nl.tolist(),
-        "nbar": st["mean"], "rebuilds": reb, "lanes": lanes,
+        "nbar": st["mean"], "rebuilds": reb, "lanes": lanes, "path": STEP_PATHS[path],
         "autotune_us": tune, "b2b_us": {"density": round(b2b[0], 2), "force": round(b2b[1], 2)},
         "nbar_b2b": nbar_b2b,
     }
@@ -448,6 +454,25 @@ def summarise_pmc(fetch_dir, write_dir, kernel):
         if cal_torch:
             out.append("%s_factor_torch_elementwise: %.5f (cross-check)" % (tag, COPY_MIB * 1024.0 / max(cal_torch)))
     out.append("hbm_bytes_per_launch: %.0f" % (res["fetch"] + res["write"]))
+    return "\n".join(out)
+
+
+def summarise_counters(d, kernel):
+    """Median per dispatch of every counter of a rocprofv3 --pmc run for the
+    dispatches of `kernel` (full-work dispatches only: launches queued ahead
+    of a list rebuild exit at once and read a fraction of the rest)."""
+    rows = [r for r in _csv_rows(d, "counter_collection.csv") if kernel in r["Kernel_Name"]]
+    if not rows:
+        raise SystemExit("no rows for %s under %s" % (kernel, d))
+    by = {}
+    for r in rows:
+        by.setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+    names = sorted({r["Kernel_Name"].split("(")[0] for r in rows})
+    out = ["kernel: " + ", ".join(names)]
+    for c in sorted(by):
+        v = np.sort(np.array(by[c]))
+        full = v[v >= 0.5 * np.median(v)] if len(v) else v
+        out.append("%-34s dispatches %5d  median %.1f" % (c, len(full), float(np.median(full))))
     return "\n".join(out)
 
 
@@ -612,7 +637,7 @@ def eam_sizes(E, ctx, W, lanes=None, reps=(20, 40, 64, 101)):
         gbs = per_launch / (force_us * 1e-6) / 1e9
         out[str(g["n"])] = {
             "value": round(g["value"], 1), "ms_per_step": round(g["ms_per_step"], 5), "steps": steps,
-            "rebuilds": g["rebuilds"], "lanes": g["lanes"], "nbar": round(g["nbar"], 2),
+            "rebuilds": g["rebuilds"], "lanes": g["lanes"], "path": g["path"], "nbar": round(g["nbar"], 2),
             "b2b_us": g["b2b_us"], "kernel_us": g["kernel_us"],
             "force_roofline": {"achieved_GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
                                "algorithmic_bytes_per_launch": int(per_launch)},
@@ -634,7 +659,9 @@ def main():
                     help="also run the 256 MiB calibration kernel (for rocprofv3 --pmc runs)")
     ap.add_argument("--summarise-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
     ap.add_argument("--summarise-stats", metavar="STATS_DIR")
-    ap.add_argument("--kernel", default="k_eam_force<float")
+    ap.add_argument("--summarise-counters", metavar="PMC_DIR",
+                    help="median per full-work dispatch of every counter of a rocprofv3 --pmc run for --kernel")
+    ap.add_argument("--kernel", default="k_eam_force_q<")
     ap.add_argument("--eam-reps", type=int, nargs="+", default=None,
                     help="--only eam_sizes: fcc cells per edge of the sizes to run (default 20 40 64 101)")
     ap.add_argument("--precision", choices=("mixed", "fp64", "both"), default="both",
@@ -647,6 +674,9 @@ def main():
         return
     if args.summarise_stats:
         print(summarise_stats(args.summarise_stats, args.kernel))
+        return
+    if args.summarise_counters:
+        print(summarise_counters(args.summarise_counters, args.kernel))
         return
     if args.gpus != 1:
         print("bench.py measures one GPU holding one domain; multi-GPU runs are not part of this build "
@@ -687,12 +717,12 @@ def main():
     achieved = per_launch / (avg_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic("eam_force_mixed"),
-            "kernel": "k_eam_force<float>", "avg_launch_us": round(avg_ms * 1e3, 2),
+            "kernel": head["path"], "avg_launch_us": round(avg_ms * 1e3, 2),
             "avg_launch_us_in_steps": round(head["kernel_ms"][0] / max(head["kernel_launches"][0], 1) * 1e3, 2),
             "algorithmic_bytes_per_launch": int(per_launch)}
     k = args.steps / max(head["rebuilds"], 1)
     step_gbs = step_bytes("eam", nbar, k) * head["value"] / 1e9
-    keys = ("value", "ms_per_step", "nbar", "rebuilds", "kernel_us", "b2b_us", "lanes")
+    keys = ("value", "ms_per_step", "nbar", "rebuilds", "kernel_us", "b2b_us", "lanes", "path")
     extra = {"eam_cu_mixed": {kk: head[kk] for kk in keys},
              "eam_cu_mixed_step_GBs": round(step_gbs, 1)}
     # NVE energy drift: 1 ps of each workload from its initial state, slope

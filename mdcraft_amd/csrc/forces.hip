@@ -670,10 +670,24 @@ __device__ inline void q_note_rmin(float rmin, int was, int* __restrict__ misc) 
     }
 }
 
+// Neighbour records of the group of chunks [m0, m0 + kUq): entries jn,
+// gathers of slots past the atom's count masked off (their records read as
+// the atom's own).
+template <int TPA>
+__device__ inline void q_records(const AtomSlots& as, int m0, int cnt, __amdgpu_buffer_rsrc_t ub, const uint4& ui,
+                                 const int* jn, bool* ok, uint4* uj) {
+#pragma unroll
+    for (int u = 0; u < kUq; ++u) {
+        ok[u] = (m0 + u) * TPA + as.sub < cnt;
+        uj[u] = ui;
+        if (ok[u]) uj[u] = buf_u4(ub, (unsigned)ent_j(jn[u]) * 16u);
+    }
+}
+
 // Density pass of one wavefront: rho_i from the rho records, then F(rho),
 // F'(rho) in FP64 as k_eam_rho; writes the atom's record with F' in w for
 // the force pass.
-template <int TPA, bool ONE, bool LDS>
+template <int TPA, bool ONE, bool LDS, bool PIPE>
 __device__ inline void rho_q_wave(const AtomSlots& as, const uint4* __restrict__ uq, uint4* __restrict__ uqF,
                                   const int* __restrict__ nl, const int* __restrict__ nn, int M, int cap,
                                   const SysDesc* __restrict__ desc, const EamK& P, const AtomArrays& A,
@@ -705,17 +719,20 @@ __device__ inline void rho_q_wave(const AtomSlots& as, const uint4* __restrict__
 #pragma unroll
     for (int u = 0; u < kUq; ++u) jn[u] = j0[u];
     const int nch = wave_chunks<TPA>(cnt);
+    // neighbour records of the first group, then of each next group one
+    // group ahead (PIPE), so that their latency overlaps the table reads
+    bool okk[kUq];
+    uint4 uj[kUq];
+    if (PIPE) {
+        q_records<TPA>(as, 0, cnt, ub, ui, jn, okk, uj);
+        next_group_clamped<kUq>(as, 0, M, nl, jn);
+    }
     for (int m0 = 0; m0 < nch; m0 += kUq) {
-        bool okk[kUq];
-        uint4 uj[kUq];
+        bool okn[kUq];
+        uint4 ujn[kUq];
         // the gathers of unused slots are masked off
-#pragma unroll
-        for (int u = 0; u < kUq; ++u) {
-            okk[u] = (m0 + u) * TPA + as.sub < cnt;
-            uj[u] = ui;
-            if (okk[u]) uj[u] = buf_u4(ub, (unsigned)ent_j(jn[u]) * 16u);
-        }
-        next_group_clamped<kUq>(as, m0, M, nl, jn);
+        q_records<TPA>(as, PIPE ? m0 + kUq : m0, cnt, ub, ui, jn, PIPE ? okn : okk, PIPE ? ujn : uj);
+        next_group_clamped<kUq>(as, PIPE ? m0 + kUq : m0, M, nl, jn);
         QGeo g[kUq];
         float4 rec[kUq];
 #pragma unroll
@@ -730,6 +747,13 @@ __device__ inline void rho_q_wave(const AtomSlots& as, const uint4* __restrict__
             const float t = g[u].RN - (float)g[u].N1;
             const float4 c = rec[u];
             rho += (double)fmaf(t, fmaf(t, fmaf(t, c.w, c.z), c.y), c.x);
+        }
+        if (PIPE) {
+#pragma unroll
+            for (int u = 0; u < kUq; ++u) {
+                okk[u] = okn[u];
+                uj[u] = ujn[u];
+            }
         }
     }
     rho = wave_sum_group(rho, TPA);
@@ -747,7 +771,7 @@ __device__ inline void rho_q_wave(const AtomSlots& as, const uint4* __restrict__
 
 // Force pass of one wavefront with the fused velocity-Verlet finish; also
 // writes the drifted position's fixed-point record.
-template <int TPA, bool ONE, bool LDS>
+template <int TPA, bool ONE, bool LDS, bool PIPE>
 __device__ inline void force_q_wave(const AtomSlots& as, const double4* __restrict__ pos,
                                     double4* __restrict__ pos_next, const uint4* __restrict__ uqF,
                                     uint4* __restrict__ uq_next, const int* __restrict__ nl,
@@ -784,17 +808,17 @@ __device__ inline void force_q_wave(const AtomSlots& as, const double4* __restri
 #pragma unroll
     for (int u = 0; u < kUq; ++u) jn[u] = j0[u];
     const int nch = wave_chunks<TPA>(cnt);
+    bool okk[kUq];
+    uint4 uj[kUq];
+    if (PIPE) {
+        q_records<TPA>(as, 0, cnt, ub, ui, jn, okk, uj);
+        next_group_clamped<kUq>(as, 0, M, nl, jn);
+    }
     for (int m0 = 0; m0 < nch; m0 += kUq) {
-        bool okk[kUq];
-        uint4 uj[kUq];
-        // the gathers of unused slots are masked off
-#pragma unroll
-        for (int u = 0; u < kUq; ++u) {
-            okk[u] = (m0 + u) * TPA + as.sub < cnt;
-            uj[u] = ui;
-            if (okk[u]) uj[u] = buf_u4(ub, (unsigned)ent_j(jn[u]) * 16u);
-        }
-        next_group_clamped<kUq>(as, m0, M, nl, jn);
+        bool okn[kUq];
+        uint4 ujn[kUq];
+        q_records<TPA>(as, PIPE ? m0 + kUq : m0, cnt, ub, ui, jn, PIPE ? okn : okk, PIPE ? ujn : uj);
+        next_group_clamped<kUq>(as, PIPE ? m0 + kUq : m0, M, nl, jn);
         QGeo g[kUq];
         float4 rec[kUq][2];
 #pragma unroll
@@ -823,6 +847,13 @@ __device__ inline void force_q_wave(const AtomSlots& as, const double4* __restri
             fy = fmaf(FF, g[u].dy, fy);
             fz = fmaf(FF, g[u].dz, fz);
         }
+        if (PIPE) {
+#pragma unroll
+            for (int u = 0; u < kUq; ++u) {
+                okk[u] = okn[u];
+                uj[u] = ujn[u];
+            }
+        }
     }
     fx = wave_sum_group(fx, TPA);
     fy = wave_sum_group(fy, TPA);
@@ -845,7 +876,7 @@ __global__ void __launch_bounds__(256) k_eam_rho_q(const uint4* __restrict__ uq,
                                                    int cap, const SysDesc* __restrict__ desc, EamK P, AtomArrays A,
                                                    StepArgs S, int* __restrict__ misc) {
     float rmin = 0.f;  // not tracked without staged tables
-    rho_q_wave<TPA, ONE, false>(atom_slots<TPA>(M), uq, uqF, nl, nn, M, cap, desc, P, A, S, misc, nullptr, 0,
+    rho_q_wave<TPA, ONE, false, true>(atom_slots<TPA>(M), uq, uqF, nl, nn, M, cap, desc, P, A, S, misc, nullptr, 0,
                                 abort_flag(S, misc), rmin);
 }
 
@@ -857,7 +888,7 @@ __global__ void __launch_bounds__(256) k_eam_force_q(const double4* __restrict__
                                                      StepArgs S, const double4* __restrict__ rsave,
                                                      int* __restrict__ misc) {
     float rmin = 0.f;
-    force_q_wave<TPA, ONE, false>(atom_slots<TPA>(M), pos, pos_next, uqF, uq_next, nl, nn, M, cap, desc, P, A, S,
+    force_q_wave<TPA, ONE, false, true>(atom_slots<TPA>(M), pos, pos_next, uqF, uq_next, nl, nn, M, cap, desc, P, A, S,
                                   rsave, misc, nullptr, 0, abort_flag(S, misc), rmin);
 }
 
@@ -914,7 +945,7 @@ __global__ void __launch_bounds__(kQlThreads) k_eam_rho_ql(const uint4* __restri
     ql_range(nwaves, gw, end, step);
     float rmin = __int_as_float(kRminNone);
     for (; gw < end; gw += step)
-        rho_q_wave<TPA, ONE, true>(atom_slots_w<TPA>(gw, M), uq, uqF, nl, nn, M, cap, desc, P, A, S, misc, s_tab,
+        rho_q_wave<TPA, ONE, true, false>(atom_slots_w<TPA>(gw, M), uq, uqF, nl, nn, M, cap, desc, P, A, S, misc, s_tab,
                                    lo, INT_MAX, rmin);
     q_note_rmin(rmin, was, misc);
 }
@@ -933,7 +964,7 @@ __global__ void __launch_bounds__(kQlThreads) k_eam_force_ql(
     ql_range(nwaves, gw, end, step);
     float rmin = __int_as_float(kRminNone);
     for (; gw < end; gw += step)
-        force_q_wave<TPA, ONE, true>(atom_slots_w<TPA>(gw, M), pos, pos_next, uqF, uq_next, nl, nn, M, cap, desc,
+        force_q_wave<TPA, ONE, true, false>(atom_slots_w<TPA>(gw, M), pos, pos_next, uqF, uq_next, nl, nn, M, cap, desc,
                                      P, A, S, rsave, misc, s_tab, lo, INT_MAX, rmin);
     q_note_rmin(rmin, was, misc);
 }
