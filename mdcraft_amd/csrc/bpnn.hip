@@ -750,6 +750,28 @@ __device__ inline void group_sums(R cs, const R* C, const R* Ex, const R* Exe, c
     }
 }
 
+// Sums over 8-lane groups of a wavefront for the tiled pair walk (every
+// lane active): sum_lanes8 over lanes 8 g .. 8 g + 7 by exchanges on the
+// vector unit, sum_stride8 over the lanes of one residue mod 8 (the
+// exchange across rows of 16 lanes goes through the LDS crossbar); every
+// lane of a group receives the sum.
+template <int C>
+__device__ inline float dpp_f(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), C, 0xF, 0xF, false));
+}
+__device__ inline float sum_lanes8(float v) {
+    v += dpp_f<0xB1>(v);   // quad_perm [1, 0, 3, 2]
+    v += dpp_f<0x4E>(v);   // quad_perm [2, 3, 0, 1]
+    v += dpp_f<0x141>(v);  // row_half_mirror: the other quad of the 8 lanes
+    return v;
+}
+__device__ inline float sum_stride8(float v) {
+    v += dpp_f<0x128>(v);  // row_ror:8: the lane 8 away in the row of 16
+    v += __shfl_xor(v, 16, kWave);
+    v += __shfl_xor(v, 32, kWave);
+    return v;
+}
+
 template <typename R, int NR, int NG, int NE, bool VIR, int LZ = 0, bool K2 = false>
 __global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos, const int* __restrict__ nl,
                                                  const int* __restrict__ nn, int tpa, int M, int cap,
@@ -847,97 +869,93 @@ __global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos
         vz += h * (ba * az_ * az_ + bb * bz_ * bz_ + gc * cz_ * cz_ + de * az_ * bz_);
     };
     if constexpr (sizeof(R) == 4) {
-    // angular part, FP32: ordered pairs (a, b), a != b: with A_q = c_q pz_q,
-    // Z_q = c_q zeta lam pz1_q (c_q = w_q coef_q Ex_e(q)) summed over q,
+    // angular part, FP32: every unordered pair {a, b} once, in tiles of 8
+    // rows a by 8 columns b, one pair per lane (lane = 8 column + row); with
+    // A_q = c_q pz_q, Z_q = c_q zeta lam pz1_q (c_q = w_q coef_q Ex_e(q))
+    // summed over q,
     // d/dr_a = (-Z F cs / da - 2 F da SAe + fb dfa fcc SA) u_a
-    //          + Z F / da u_b + (2 F dc SAe - fa fb dfcc SA) u_c, c = r_b - r_a.
-    // Lane l takes the l-th contiguous 64th of the flattened pairs, so it
-    // stays on one row a for long runs: the row's sum collects in registers
-    // and goes to the LDS accumulator only when the row changes.  Twice the
-    // pair arithmetic of the unordered walk, but almost no LDS atomics.
-    // rows a in [r0, r0 + nrow), columns b in [c0, c0 + ncol) without b = a
-    // when the two ranges are one (same)
+    //          + Z F / da u_b + (2 F dc SAe - fa fb dfcc SA) u_c, c = r_b - r_a,
+    // and d/dr_b the same with a and b exchanged (u_c changes sign).  A
+    // lane keeps its row a through a row of tiles, so the a-sides sum in
+    // registers and meet across lanes once per row of tiles; the b-sides of
+    // a tile sum over its 8 adjacent lanes of one column by lane exchanges
+    // on the vector unit.  Neighbours are staged in list order, which keeps
+    // those of one stencil cell together, so that whole tiles of pairs lie
+    // beyond the cutoff and are skipped.  Rows a in [r0, r0 + nrow),
+    // columns b in [c0, c0 + ncol); same: the two ranges are one and only
+    // pairs a < b are taken.
     auto walk = [&](int r0, int nrow, int c0, int ncol, bool same) {
-    const int row = ncol - (same ? 1 : 0);
-    if (nrow <= 0 || row <= 0) return;
-    const int T = nrow * row;
-    const int u0 = (int)(((long long)lane * T) / kWave), u1 = (int)(((long long)(lane + 1) * T) / kWave);
-    const int steps = (T + kWave - 1) / kWave;
-    int a = r0 + u0 / row, r = u0 - (u0 / row) * row;
-    R ax = 0, ay = 0, az = 0, da = 1, fa = 0, dfa = 0, ia = 1;
-    R Ea[NE];
-    int cur = -1;
-    R hx = 0, hy = 0, hz = 0;
-    for (int k = 0; k < steps; ++k) {
-        const int u = u0 + k;
-        if (u < u1) {
-            if (a != cur) {
-                if (cur >= 0) {
-                    atomicAdd(&S.g[cur], hx);
-                    atomicAdd(&S.g[cap + cur], hy);
-                    atomicAdd(&S.g[2 * cap + cur], hz);
-                }
-                cur = a;
-                hx = hy = hz = 0;
-                ax = S.x[a];
-                ay = S.y[a];
-                az = S.z[a];
-                da = S.d[a];
-                ia = S.id[a];
-                fa = S.fc[a];
-                dfa = S.dfc[a];
+    if (nrow <= 0 || ncol <= 0) return;
+    const int ar = lane & 7, bc = lane >> 3;
+    const int tr = (nrow + 7) >> 3, tc = (ncol + 7) >> 3;
+    for (int ti = 0; ti < tr; ++ti) {
+        const int ra = ti * 8 + ar;
+        const bool va = ra < nrow;
+        const int a = r0 + (va ? ra : 0);
+        const R ax = S.x[a], ay = S.y[a], az = S.z[a], da = S.d[a], ia = S.id[a], fa = S.fc[a], dfa = S.dfc[a];
+        R Ea[NE];
 #pragma unroll
-                for (int e = 0; e < NE; ++e) Ea[e] = e < ne ? S.E[e * cap + a] : (R)0;
-            }
-            const int b = c0 + r + (same && c0 + r >= a ? 1 : 0);
+        for (int e = 0; e < NE; ++e) Ea[e] = (LZ > 0 || e < ne) ? S.E[e * cap + a] : (R)0;
+        R hx = 0, hy = 0, hz = 0;
+        for (int tj = same ? ti : 0; tj < tc; ++tj) {
+            const int cb = tj * 8 + bc;
+            const bool vb = cb < ncol;
+            const int b = c0 + (vb ? cb : 0);
             const R bx = S.x[b], by = S.y[b], bz = S.z[b];
             const R cx = bx - ax, cy = by - ay, cz = bz - az;
             const R d2c = cx * cx + cy * cy + cz * cz;
-            if (d2c >= (R)1e-15 && d2c < Rc2) {
-                const R ib = S.id[b], fb = S.fc[b];
-                const R ic = drsq_(d2c);
-                const R dc = d2c * ic;
-                const R fcc = (R)0.5 * (dcos_(pirc * dc) + (R)1);
-                const R dfcc = (R)-0.5 * dsin_(pirc * dc) * pirc;
-                const R cs = (ax * bx + ay * by + az * bz) * ia * ib;
-                const R F = fa * fb * fcc;
-                R Ex[NE], Exe[NE];
+            const bool ok = va && vb && (!same || ra < cb) && d2c >= (R)1e-15 && d2c < Rc2;
+            if (!__ballot(ok)) continue;  // the whole tile beyond the cutoff
+            const R db = S.d[b], ib = S.id[b], fb = S.fc[b], dfb = S.dfc[b];
+            const R ic = drsq_(d2c);
+            const R dc = d2c * ic;
+            const R fcc = (R)0.5 * (dcos_(pirc * dc) + (R)1);
+            const R dfcc = (R)-0.5 * dsin_(pirc * dc) * pirc;
+            const R cs = (ax * bx + ay * by + az * bz) * ia * ib;
+            const R F = fa * fb * fcc;
+            R Ex[NE], Exe[NE];
 #pragma unroll
-                for (int e = 0; e < NE; ++e) {
-                    Ex[e] = (LZ > 0 || e < ne) ? Ea[e] * S.E[e * cap + b] * dexp_(-eta_e[e] * d2c) : (R)0;
-                    Exe[e] = Ex[e] * eta_e[e];
-                }
-                R SA = 0, SAe = 0, SZ = 0;
-                group_sums<R, NG, NE, LZ>(cs, C, Ex, Exe, g_lam, g_zl, g_iz, ng, SA, SAe, SZ);
-                SZ *= F;
-                const R cc = ((R)2 * F * dc * SAe - fa * fb * dfcc * SA) * ic;
-                const R ka = (-SZ * cs * ia - (R)2 * F * da * SAe + fb * dfa * fcc * SA) * ia;
-                const R kb = SZ * ia * ib;
-                hx += ka * ax + kb * bx + cc * cx;
-                hy += ka * ay + kb * by + cc * cy;
-                hz += ka * az + kb * bz + cc * cz;
-                // each unordered pair is visited twice here
-                if constexpr (VIR)
-                    vir_pair((R)0.5, ax, ay, az, bx, by, bz, cx, cy, cz, ia, ib, ic, fa, fb, dfa, S.dfc[b], fcc, dfcc,
-                             cs, F, SA, SAe, SZ);
+            for (int e = 0; e < NE; ++e) {
+                Ex[e] = (LZ > 0 || e < ne) ? Ea[e] * S.E[e * cap + b] * dexp_(-eta_e[e] * d2c) : (R)0;
+                Exe[e] = Ex[e] * eta_e[e];
             }
-            if (++r == row) {
-                r = 0;
-                ++a;
+            R SA = 0, SAe = 0, SZ = 0;
+            group_sums<R, NG, NE, LZ>(cs, C, Ex, Exe, g_lam, g_zl, g_iz, ng, SA, SAe, SZ);
+            SZ *= F;
+            const R cc = ((R)2 * F * dc * SAe - fa * fb * dfcc * SA) * ic;
+            const R ka = (-SZ * cs * ia - (R)2 * F * da * SAe + fb * dfa * fcc * SA) * ia;
+            const R kb = SZ * ia * ib;
+            const R la = (-SZ * cs * ib - (R)2 * F * db * SAe + fa * dfb * fcc * SA) * ib;
+            if constexpr (VIR)
+                if (ok)
+                    vir_pair((R)1, ax, ay, az, bx, by, bz, cx, cy, cz, ia, ib, ic, fa, fb, dfa, dfb, fcc, dfcc, cs, F,
+                             SA, SAe, SZ);
+            hx += ok ? ka * ax + kb * bx + cc * cx : (R)0;
+            hy += ok ? ka * ay + kb * by + cc * cy : (R)0;
+            hz += ok ? ka * az + kb * bz + cc * cz : (R)0;
+            const R gx = sum_lanes8(ok ? la * bx + kb * ax - cc * cx : (R)0);
+            const R gy = sum_lanes8(ok ? la * by + kb * ay - cc * cy : (R)0);
+            const R gz = sum_lanes8(ok ? la * bz + kb * az - cc * cz : (R)0);
+            if (ar == 0 && vb) {
+                atomicAdd(&S.g[b], gx);
+                atomicAdd(&S.g[cap + b], gy);
+                atomicAdd(&S.g[2 * cap + b], gz);
             }
         }
-    }
-    if (cur >= 0) {
-        atomicAdd(&S.g[cur], hx);
-        atomicAdd(&S.g[cap + cur], hy);
-        atomicAdd(&S.g[2 * cap + cur], hz);
+        hx = sum_stride8(hx);
+        hy = sum_stride8(hy);
+        hz = sum_stride8(hz);
+        if (bc == 0 && va) {
+            atomicAdd(&S.g[a], hx);
+            atomicAdd(&S.g[cap + a], hy);
+            atomicAdd(&S.g[2 * cap + a], hz);
+        }
     }
     };
     walk(0, nc, 0, nc, true);
     if constexpr (K2) {
         load_C(1);
         walk(0, nc, d0, nd, false);
-        walk(d0, nd, 0, nc, false);
         load_C(2);
         walk(d0, nd, d0, nd, true);
     }
