@@ -343,13 +343,13 @@ def run_gpu(E, ctx, w, precision, steps, warmup, lanes=None):
     s.step(p, w["dt"], min(steps, 200), precision)
     s.sync()
     ms, nl = s.timing(0)
-    path = s.step_path(-1) if pot["kind"] == "eam" else 0
+    path = s.step_path(-1) if pot["kind"] == "eam" else None
     n = len(at)
     per_kernel = {KINDS[k]: round(ms[k] / nl[k] * 1e3, 2) for k in range(8) if nl[k] > 0}
     return {
         "n": n, "steps": steps, "seconds": t1 - t0, "value": n * steps / (t1 - t0), "kernel_us": per_kernel,
         "ms_per_step": 1e3 * (t1 - t0) / steps, "kernel_ms": ms.tolist(), "kernel_launches": nl.tolist(),
-        "nbar": st["mean"], "rebuilds": reb, "lanes": lanes, "path": STEP_PATHS[path],
+        "nbar": st["mean"], "rebuilds": reb, "lanes": lanes, "path": STEP_PATHS.get(path),
         "autotune_us": tune, "b2b_us": {"density": round(b2b[0], 2), "force": round(b2b[1], 2)},
         "nbar_b2b": nbar_b2b,
     }
