@@ -56,10 +56,11 @@ def step_bytes(kind, nbar, k):
     return (160 + 8 * nbar + reb) if kind.startswith("eam") else (120 + 4 * nbar + reb)
 
 
-def read_traffic(kernel_tag):
+def read_traffic(name):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE,
-    WRITE_SIZE), scaled by a copy-kernel calibration, from profiles/."""
-    path = os.path.join(ROOT, "profiles", "traffic_%s.txt" % kernel_tag)
+    WRITE_SIZE), scaled by a copy-kernel calibration, from the summary
+    profiles/<name>.txt."""
+    path = os.path.join(ROOT, "profiles", name + ".txt")
     if not os.path.exists(path):
         return None
     vals = {}
@@ -228,6 +229,9 @@ def run_batch_eval(E, ctx, cells, pot, precision, reps, lanes=None):
                            "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": round(gbs / HBM_PEAK_GBS, 4), "avg_launch_us": round(us, 2),
                            "algorithmic_bytes_per_launch": int(per_launch), "traffic": None}
+        if pot["kind"] == "eam_multi" and precision == E.MIXED:
+            # measured for this configuration (Multi, mixed precision)
+            out["roofline"]["traffic"] = read_traffic("k_eam_force_nial_batch512")
     return out
 
 
@@ -716,7 +720,7 @@ def main():
     per_launch = algorithmic_bytes("eam_mixed", head["nbar_b2b"]) * head["n"]
     achieved = per_launch / (avg_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic("eam_force_mixed"),
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic("traffic_eam_force_mixed"),
             "kernel": head["path"], "avg_launch_us": round(avg_ms * 1e3, 2),
             "avg_launch_us_in_steps": round(head["kernel_ms"][0] / max(head["kernel_launches"][0], 1) * 1e3, 2),
             "algorithmic_bytes_per_launch": int(per_launch)}
