@@ -241,6 +241,48 @@ struct PairWalk {
     }
 };
 
+// Every unordered pair of n staged neighbours once, as (a, b = a + d mod
+// n): d = 1 .. (n - 1) / 2 for every a, and d = n / 2 for a < n / 2 when n
+// is even.  Lane l takes the l-th contiguous 64th of these pairs in order
+// of a, then d, so it stays on one a for long runs and steps to the next
+// without a search; row(a) is called when the lane reaches a new row a (to
+// keep that neighbour's values in registers), fn(a, b) for each pair.
+template <class Row, class Fn>
+__device__ inline void circ_walk(int n, int lane, Row&& row, Fn&& fn) {
+    if (n < 2) return;
+    const int hd = (n - 1) / 2;
+    const int ex = (n % 2 == 0) ? n / 2 : 0;  // rows with one more partner (d = n / 2)
+    const int T = n * hd + ex;
+    const int u0 = (int)(((long long)lane * T) / kWave), u1 = (int)(((long long)(lane + 1) * T) / kWave);
+    const int steps = (T + kWave - 1) / kWave;
+    // row and partner of the first pair of the lane
+    int a, d;
+    if (u0 < ex * (hd + 1)) {
+        a = u0 / (hd + 1);
+        d = u0 - a * (hd + 1) + 1;
+    } else {
+        const int v = u0 - ex * (hd + 1);
+        a = ex + v / max(hd, 1);
+        d = v - (a - ex) * max(hd, 1) + 1;
+    }
+    int cur = -1;
+    for (int k = 0; k < steps; ++k) {
+        if (u0 + k < u1) {
+            if (a != cur) {
+                row(a);
+                cur = a;
+            }
+            int b = a + d;
+            if (b >= n) b -= n;
+            fn(a, b);
+            if (++d > hd + (a < ex ? 1 : 0)) {
+                d = 1;
+                ++a;
+            }
+        }
+    }
+}
+
 // Stage the in-range neighbours of centre i (1e-15 <= r^2 < Rc^2 under
 // Domain::shortest) into LDS, compacted in list order; out-of-range slots
 // get zero force shares in Pzero and zero virial shares in PVzero when
@@ -377,17 +419,8 @@ __global__ void __launch_bounds__(256) k_bp_desc(const double4* __restrict__ pos
     // range are queued (ballot and prefix count) and taken 64 at a time, so
     // that every lane of the arithmetic has a pair
     const R Rc2 = (R)P.Rc2, pirc = (R)P.pi_rc;
-    auto pair_terms = [&](int a, int b) {
-        const R cx = S.x[b] - S.x[a], cy = S.y[b] - S.y[a], cz = S.z[b] - S.z[a];
-        const R d2c = cx * cx + cy * cy + cz * cz;
-        const R dc = d2c * drsq_(d2c);
-        const R fcc = (R)0.5 * (dcos_(pirc * dc) + (R)1);
-        const R cs = (S.x[a] * S.x[b] + S.y[a] * S.y[b] + S.z[a] * S.z[b]) * S.id[a] * S.id[b];
-        const R F = S.fc[a] * S.fc[b] * fcc;
-        R Ex[NE];
-#pragma unroll
-        for (int e = 0; e < NE; ++e)
-            Ex[e] = (LZ > 0 || e < ne) ? S.E[e * cap + a] * S.E[e * cap + b] * dexp_(-eta_e[e] * d2c) * F : (R)0;
+    // a pair's angular terms from its cosine and its exp / cutoff factors
+    auto ang_terms = [&](R cs, const R* Ex) {
         if constexpr (LZ > 0) {
             // rung b: (1 + cos)^(2^b) and (1 - cos)^(2^b) by squaring
             R pp = (R)1 + cs, pm = (R)1 - cs;
@@ -412,6 +445,48 @@ __global__ void __launch_bounds__(256) k_bp_desc(const double4* __restrict__ pos
                     for (int e = 0; e < NE; ++e) acc[NR + g * NE + e] += pz * Ex[e];
                 }
         }
+    };
+    auto pair_terms = [&](int a, int b) {
+        const R cx = S.x[b] - S.x[a], cy = S.y[b] - S.y[a], cz = S.z[b] - S.z[a];
+        const R d2c = cx * cx + cy * cy + cz * cz;
+        const R dc = d2c * drsq_(d2c);
+        const R fcc = (R)0.5 * (dcos_(pirc * dc) + (R)1);
+        const R cs = (S.x[a] * S.x[b] + S.y[a] * S.y[b] + S.z[a] * S.z[b]) * S.id[a] * S.id[b];
+        const R F = S.fc[a] * S.fc[b] * fcc;
+        R Ex[NE];
+#pragma unroll
+        for (int e = 0; e < NE; ++e)
+            Ex[e] = (LZ > 0 || e < ne) ? S.E[e * cap + a] * S.E[e * cap + b] * dexp_(-eta_e[e] * d2c) * F : (R)0;
+        ang_terms(cs, Ex);
+    };
+    // FP32 walks: the values of the lane's current row a in registers, the
+    // pair's terms from b's staged values
+    R rx = 0, ry = 0, rz = 0, rid = 0, rfc = 0, rE[NE];
+#pragma unroll
+    for (int e = 0; e < NE; ++e) rE[e] = (R)0;
+    auto load_row = [&](int a) {
+        rx = S.x[a];
+        ry = S.y[a];
+        rz = S.z[a];
+        rid = S.id[a];
+        rfc = S.fc[a];
+#pragma unroll
+        for (int e = 0; e < NE; ++e) rE[e] = (LZ > 0 || e < ne) ? S.E[e * cap + a] : (R)0;
+    };
+    auto pair_terms_row = [&](int b) {
+        const R bx = S.x[b], by = S.y[b], bz = S.z[b];
+        const R cx = bx - rx, cy = by - ry, cz = bz - rz;
+        const R d2c = cx * cx + cy * cy + cz * cz;
+        if (!(d2c >= (R)1e-15 && d2c < Rc2)) return;
+        const R dc = d2c * drsq_(d2c);
+        const R fcc = (R)0.5 * (dcos_(pirc * dc) + (R)1);
+        const R cs = (rx * bx + ry * by + rz * bz) * rid * S.id[b];
+        const R F = rfc * S.fc[b] * fcc;
+        R Ex[NE];
+#pragma unroll
+        for (int e = 0; e < NE; ++e)
+            Ex[e] = (LZ > 0 || e < ne) ? rE[e] * S.E[e * cap + b] * dexp_(-eta_e[e] * d2c) * F : (R)0;
+        ang_terms(cs, Ex);
     };
     if constexpr (sizeof(R) == 8) {
         int* Q = S.queue;
@@ -445,12 +520,7 @@ __global__ void __launch_bounds__(256) k_bp_desc(const double4* __restrict__ pos
             if (!more) break;
         }
     } else {
-        for (PairWalk pw(nc, lane); pw.valid(); pw.advance()) {
-            const int a = pw.a, b = pw.b();
-            const R cx = S.x[b] - S.x[a], cy = S.y[b] - S.y[a], cz = S.z[b] - S.z[a];
-            const R d2c = cx * cx + cy * cy + cz * cz;
-            if (d2c >= (R)1e-15 && d2c < Rc2) pair_terms(a, b);
-        }
+        circ_walk(nc, lane, load_row, [&](int, int b) { pair_terms_row(b); });
     }
     if constexpr (K2) {
         // the angular sums of the pairs walked so far into block `key`
@@ -482,9 +552,13 @@ __global__ void __launch_bounds__(256) k_bp_desc(const double4* __restrict__ pos
             if (in_range(a, b)) pair_terms(a, b);
         }
         store_ang(1);
-        for (PairWalk pw(nd, lane); pw.valid(); pw.advance()) {
-            const int a = d0 + pw.a, b = d0 + pw.b();
-            if (in_range(a, b)) pair_terms(a, b);
+        if constexpr (sizeof(R) == 8) {
+            for (PairWalk pw(nd, lane); pw.valid(); pw.advance()) {
+                const int a = d0 + pw.a, b = d0 + pw.b();
+                if (in_range(a, b)) pair_terms(a, b);
+            }
+        } else {
+            circ_walk(nd, lane, [&](int a) { load_row(d0 + a); }, [&](int, int b) { pair_terms_row(d0 + b); });
         }
         store_ang(2);
 #pragma unroll
