@@ -90,7 +90,7 @@ def make_pot(E, ctx, pot):
     raise ValueError(pot["kind"])
 
 
-KINDS = ["force", "density", "predrift", "bp_desc", "bp_mlp", "bp_force", "bp_gather", "rebuild"]
+KINDS = ["force", "density", "predrift", "bp_desc", "bp_mlp", "bp_force", "bp_gather", "rebuild", "bp_fused"]
 MLP_FLOP_PER_ATOM = 2 * (16 * 30 + 30 * 30 + 30) + 2 * (30 + 30 * 30 + 30 * 16)  # forward + input gradient
 MFMA_FP16_PEAK_TFLOPS = 2500.0
 # FP32 vector peak (published), and the rate without packed FP32 math,
@@ -99,16 +99,19 @@ FP32_VALU_PEAK_TFLOPS = 157.3
 FP32_VALU_NONPACKED_TFLOPS = 78.65
 
 
-def bpnn_flops(nr=8, ne=2, lz=3):
+def bpnn_flops(nr=8, ne=2, lz=3, lz_used=2):
     """FP32 FLOPs of the descriptor and force passes per unit, counted from
     their formulas (a transcendental counts one, an FMA two): per neighbour
     in the cutoff (radial terms) and per neighbour pair in the cutoff of the
     centre and of each other (unordered in the descriptor pass, ordered in
     the force pass).  The ladder kernels: nr radial terms, ne G4 etas, lz
-    zeta rungs."""
+    zeta rungs of which lz_used hold G4 terms (zeta 1 and 4: rungs 1 and 4
+    of 1, 2, 4); the powers of every rung below the top are formed."""
     geo = 14  # r_b - r_a, |r_c|^2, 1/|r_c|, the cutoff factor
-    desc_pair = geo + 9 + 5 * ne + 2 + 2 * (lz - 1) + 4 * lz * ne
-    grad_pair = geo + 3 + 9 + 6 * ne + lz * (8 * ne + 16) - 4 + 1 + 6 + 9 + 2 + 15
+    desc_pair = geo + 9 + 5 * ne + 2 + 2 * (lz - 1) + 4 * lz_used * ne
+    # per used rung and eta the A and Z sums (two FMAs each), then SA, SAe
+    # and SZ from them (three FMAs per eta)
+    grad_pair = geo + 3 + 9 + 6 * ne + 4 * (lz - 1) + 8 * ne * lz_used + 6 * ne + 1 + 6 + 9 + 2 + 15
     return {"desc_neighbour": 6 * nr, "desc_pair": desc_pair, "grad_neighbour": 10 * nr + 6, "grad_pair": grad_pair}
 
 
@@ -133,13 +136,15 @@ def bpnn_units(atoms, box, rc):
     return nrad, npair
 
 
-def bpnn_valu(kernel_us, units):
-    """The descriptor and force kernels against the FP32 vector peak."""
+def bpnn_valu(kernel_us, units, atoms):
+    """The descriptor and force kernels (or the fused pass, which adds the
+    network) against the FP32 vector peak."""
     nrad, npair = units
     f = bpnn_flops()
+    desc = f["desc_neighbour"] * nrad + f["desc_pair"] * npair
+    grad = f["grad_neighbour"] * nrad + f["grad_pair"] * 2 * npair
     out = {}
-    for kind, flop in (("bp_desc", f["desc_neighbour"] * nrad + f["desc_pair"] * npair),
-                       ("bp_force", f["grad_neighbour"] * nrad + f["grad_pair"] * 2 * npair)):
+    for kind, flop in (("bp_desc", desc), ("bp_force", grad), ("bp_fused", desc + grad + MLP_FLOP_PER_ATOM * atoms)):
         us = kernel_us.get(kind)
         if not us:
             continue
@@ -175,7 +180,7 @@ def run_batch(E, ctx, cells, pot, dt, precision, steps, warmup, lanes=None):
     n = s.n
     return {"value": n * steps / (t1 - t0), "ms_per_step": 1e3 * (t1 - t0) / steps, "atoms": n,
             "systems": len(cells), "rebuilds": reb, "nbar": s.stats()["mean"], "lanes": lanes,
-            "kernel_us": {KINDS[k]: round(ms[k] / nl[k] * 1e3, 2) for k in range(8) if nl[k] > 0}}
+            "kernel_us": {KINDS[k]: round(ms[k] / nl[k] * 1e3, 2) for k in range(len(KINDS)) if nl[k] > 0}}
 
 
 def run_batch_eval(E, ctx, cells, pot, precision, reps, lanes=None):
@@ -218,7 +223,7 @@ def run_batch_eval(E, ctx, cells, pot, precision, reps, lanes=None):
            "mode": "graph-captured evaluation, replayed", "graph": gstats,
            "eager": {"value": n * reps / eager, "ms_per_eval": 1e3 * eager / reps},
            "atoms": n, "systems": len(cells), "nbar": nbar, "lanes": lanes, "threads_per_workgroup": threads,
-           "kernel_us": {KINDS[k]: round(ms[k] / nl[k] * 1e3, 2) for k in range(8) if nl[k] > 0}}
+           "kernel_us": {KINDS[k]: round(ms[k] / nl[k] * 1e3, 2) for k in range(len(KINDS)) if nl[k] > 0}}
     if nl[0] > 0 and pot["kind"] in ("eam_multi", "alloy", "eam"):
         nk = pot.get("nkinds", 1) if pot["kind"] == "eam_multi" else (
             len(pot["F"]) if pot["kind"] == "alloy" and np.ndim(pot["F"]) == 2 else 1)
@@ -349,7 +354,7 @@ def run_gpu(E, ctx, w, precision, steps, warmup, lanes=None):
     ms, nl = s.timing(0)
     path = s.step_path(-1) if pot["kind"] == "eam" else None
     n = len(at)
-    per_kernel = {KINDS[k]: round(ms[k] / nl[k] * 1e3, 2) for k in range(8) if nl[k] > 0}
+    per_kernel = {KINDS[k]: round(ms[k] / nl[k] * 1e3, 2) for k in range(len(KINDS)) if nl[k] > 0}
     return {
         "n": n, "steps": steps, "seconds": t1 - t0, "value": n * steps / (t1 - t0), "kernel_us": per_kernel,
         "ms_per_step": 1e3 * (t1 - t0) / steps, "kernel_ms": ms.tolist(), "kernel_launches": nl.tolist(),
@@ -698,8 +703,8 @@ def main():
             out = {p: run_batch(E, ctx, [(wb["atoms"], wb["box"])], wb["pot"], wb["dt"], pr, min(args.steps, 200),
                                 10, args.lanes) for p, pr in precs}
             if "mixed" in out:
-                out["mixed"]["valu"] = bpnn_valu(out["mixed"]["kernel_us"], bpnn_units(wb["atoms"], wb["box"],
-                                                                                       wb["pot"]["rc"]))
+                out["mixed"]["valu"] = bpnn_valu(out["mixed"]["kernel_us"],
+                                                 bpnn_units(wb["atoms"], wb["box"], wb["pot"]["rc"]), len(wb["atoms"]))
         elif args.only == "eam_sizes":
             out = eam_sizes(E, ctx, W, args.lanes, **({"reps": tuple(args.eam_reps)} if args.eam_reps else {}))
         elif args.only in ("nial_batch", "nial_alloy"):
@@ -765,7 +770,12 @@ def main():
                 tf = MLP_FLOP_PER_ATOM * r["atoms"] / (mlp_us * 1e-6) / 1e12
                 r["mlp_mfma"] = {"algorithmic_TFLOPs": round(tf, 2), "peak_fp16_dense_TFLOPs": MFMA_FP16_PEAK_TFLOPS,
                                  "frac": round(tf / MFMA_FP16_PEAK_TFLOPS, 5), "flop_per_atom": MLP_FLOP_PER_ATOM}
-                r["valu"] = bpnn_valu(r["kernel_us"], bpnn_units(wb["atoms"], wb["box"], wb["pot"]["rc"]))
+            if prec == E.MIXED:
+                if "bp_fused" in r["kernel_us"]:
+                    r["mlp"] = ("in the fused pass: FP32 on the vector unit, one wavefront per atom, "
+                                f"{MLP_FLOP_PER_ATOM} FLOP per atom")
+                r["valu"] = bpnn_valu(r["kernel_us"], bpnn_units(wb["atoms"], wb["box"], wb["pot"]["rc"]),
+                                      r["atoms"])
             extra[name] = r
         extra["served_stream"] = served_stream(E, ctx, W)
         # the headline step at 32 000 to 4.1 million atoms

@@ -363,7 +363,9 @@ mdc_status mdc_bpnn_inspect(mdc_sys* sys, const mdc_pot* pot, double* G, double*
 mdc_status mdc_copy_calibration(int device, uint64_t bytes, int reps, double* us_per_copy);
 /* kernel timing with hipEvents on the launch stream: kind 0 pair force,
  * 1 EAM density, 2 pre-drift, 3 BPNN descriptors, 4 BPNN MLP, 5 BPNN force,
- * 6 BPNN gather, 7 rebuild */
+ * 6 BPNN gather, 7 rebuild, 8 BPNN fused pass (descriptors, network and
+ * force shares in one kernel: FP32, one species).  total_ms and launches
+ * hold 9 entries each. */
 mdc_status mdc_timing(mdc_sys* sys, int enable, double* total_ms, int64_t* launches);
 
 #ifdef __cplusplus

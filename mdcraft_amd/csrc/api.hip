@@ -1798,13 +1798,13 @@ mdc_status mdc_timing(mdc_sys* sys, int enable, double* total_ms, int64_t* launc
         Batch& b = sys->b;
         HIPCHECK(hipStreamSynchronize(b.ctx->stream));
         timer_drain(b);
-        for (int k = 0; k < 8; ++k) {
+        for (int k = 0; k < kKinds; ++k) {
             if (total_ms) total_ms[k] = b.kt_ms[k];
             if (launches) launches[k] = b.kt_n[k];
         }
         if (enable >= 0) {
             b.timing = enable != 0;
-            for (int k = 0; k < 8; ++k) {
+            for (int k = 0; k < kKinds; ++k) {
                 b.kt_ms[k] = 0;
                 b.kt_n[k] = 0;
             }

@@ -90,6 +90,7 @@ struct BpnnParams {
     BpDesc* d = nullptr;
     double* w64 = nullptr;  // W01 [h1][nin] (range-scaled), b01, W12 [h2][h1], b12, W23 [h2], b23
     float* w32 = nullptr;   // b01, b12, W23, b23 as float for the MFMA epilogues
+    float* w64f = nullptr;  // w64 in FP32 (same layout and stride): the fused pass's network
     half4_t* bsplit = nullptr;
     int boff[4] = {0, 0, 0, 0};
     int nt[4] = {0, 0, 0, 0}, kt[4] = {0, 0, 0, 0};
@@ -246,9 +247,11 @@ struct PairWalk {
 // is even.  Lane l takes the l-th contiguous 64th of these pairs in order
 // of a, then d, so it stays on one a for long runs and steps to the next
 // without a search; row(a) is called when the lane reaches a new row a (to
-// keep that neighbour's values in registers), fn(a, b) for each pair.
-template <class Row, class Fn>
-__device__ inline void circ_walk(int n, int lane, Row&& row, Fn&& fn) {
+// keep that neighbour's values in registers), fn(a, b, v) for each pair
+// with v = load(b), which is issued one pair ahead so that its latency
+// overlaps the current pair's arithmetic.
+template <class Row, class Load, class Fn>
+__device__ inline void circ_walk(int n, int lane, Row&& row, Load&& load, Fn&& fn) {
     if (n < 2) return;
     const int hd = (n - 1) / 2;
     const int ex = (n % 2 == 0) ? n / 2 : 0;  // rows with one more partner (d = n / 2)
@@ -265,6 +268,11 @@ __device__ inline void circ_walk(int n, int lane, Row&& row, Fn&& fn) {
         a = ex + v / max(hd, 1);
         d = v - (a - ex) * max(hd, 1) + 1;
     }
+    a = min(a, n - 1);  // lanes without pairs (T < 64) stay in range
+    int b = a + d;
+    if (b >= n) b -= n;
+    b = min(b, n - 1);
+    auto v = load(b);
     int cur = -1;
     for (int k = 0; k < steps; ++k) {
         if (u0 + k < u1) {
@@ -272,13 +280,20 @@ __device__ inline void circ_walk(int n, int lane, Row&& row, Fn&& fn) {
                 row(a);
                 cur = a;
             }
-            int b = a + d;
-            if (b >= n) b -= n;
-            fn(a, b);
-            if (++d > hd + (a < ex ? 1 : 0)) {
-                d = 1;
-                ++a;
+            int a2 = a, d2 = d + 1;
+            if (d2 > hd + (a < ex ? 1 : 0)) {
+                d2 = 1;
+                ++a2;
             }
+            int b2 = a2 + d2;
+            if (b2 >= n) b2 -= n;
+            const bool more = u0 + k + 1 < u1;
+            auto v2 = load(more ? b2 : b);
+            fn(a, b, v);
+            a = a2;
+            d = d2;
+            b = b2;
+            v = v2;
         }
     }
 }
@@ -355,33 +370,19 @@ __device__ int bp_stage(int i, const double4& pi, const SysDesc& s, const int* _
 // kind difference and the angular terms of the same/same, same/other and
 // other/other pairs to blocks 0, 1 and 2 (maise_bpnn.cxx:141-257 for two
 // kinds, whose differences are 0 or 1).
-template <typename R, int NR, int NG, int NE, int LZ = 0, bool K2 = false>
-__global__ void __launch_bounds__(256) k_bp_desc(const double4* __restrict__ pos, const int* __restrict__ nl,
-                                                 const int* __restrict__ nn, int tpa, int M, int cap,
-                                                 const SysDesc* __restrict__ desc, const BpDesc* __restrict__ Pd,
-                                                 R* __restrict__ G, int ldg, int n, StepArgs SA,
-                                                 int* __restrict__ misc) {
-    if (skip_step(SA, misc)) return;
-    extern __shared__ char smem[];
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int i = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + wv;
-    if (i >= n) return;
-    const BpDesc& P = *Pd;
-    const Stage<R> S = carve<R>(smem + wv * stage_bytes(cap, P.ne, sizeof(R), false), cap, P.ne, false);
-    const double4 pi = pos[i];
-    const SysDesc& s = desc[w_sys(pi.w)];
-    const int cnt = min(nn[i], cap);
-    int nd = 0;
-    const int nc = bp_stage<R>(i, pi, s, nl, cnt, tpa, M, pos, P, S, (R*)nullptr, (R*)nullptr, cap,
-                               K2 ? (int)w_kind(pi.w) : -1, nd);
+// The descriptors of one centre from its staged neighbours, by one
+// wavefront: the radial and angular sums in acc (every lane holds the
+// totals on return, single species), written to grow[0, ldg) when grow is
+// given (two species: always, block by block).
+template <typename R, int NR, int NG, int NE, int LZ, bool K2, int ZM = -1>
+__device__ inline void desc_wave(const BpDesc& P, const Stage<R>& S, int nc, int nd, int cap, int lane, R* grow,
+                                 int ldg, R (&acc)[NR + NG * NE]) {
     const int d0 = cap - nd;  // the other kind's neighbours: [d0, cap)
-    R acc[NR + NG * NE];
     R acc1[K2 ? NR : 1];  // radial block of the other kind
 #pragma unroll
     for (int q = 0; q < NR + NG * NE; ++q) acc[q] = (R)0;
 #pragma unroll
     for (int q = 0; q < (K2 ? NR : 1); ++q) acc1[q] = (R)0;
-    R* grow = G + (size_t)i * ldg;
     if constexpr (K2) {
         // blocks are stored as they complete
         for (int q = lane; q < ldg; q += kWave) grow[q] = (R)0;
@@ -426,11 +427,12 @@ __global__ void __launch_bounds__(256) k_bp_desc(const double4* __restrict__ pos
             R pp = (R)1 + cs, pm = (R)1 - cs;
 #pragma unroll
             for (int b = 0; b < LZ; ++b) {
+                if ((ZM >> b) & 1)
 #pragma unroll
-                for (int e = 0; e < NE; ++e) {
-                    acc[NR + (2 * b) * NE + e] += pp * Ex[e];
-                    acc[NR + (2 * b + 1) * NE + e] += pm * Ex[e];
-                }
+                    for (int e = 0; e < NE; ++e) {
+                        acc[NR + (2 * b) * NE + e] += pp * Ex[e];
+                        acc[NR + (2 * b + 1) * NE + e] += pm * Ex[e];
+                    }
                 if (b + 1 < LZ) {
                     pp *= pp;
                     pm *= pm;
@@ -473,19 +475,32 @@ __global__ void __launch_bounds__(256) k_bp_desc(const double4* __restrict__ pos
 #pragma unroll
         for (int e = 0; e < NE; ++e) rE[e] = (LZ > 0 || e < ne) ? S.E[e * cap + a] : (R)0;
     };
-    auto pair_terms_row = [&](int b) {
-        const R bx = S.x[b], by = S.y[b], bz = S.z[b];
-        const R cx = bx - rx, cy = by - ry, cz = bz - rz;
+    struct BVals {
+        R x, y, z, id, fc, E[NE];
+    };
+    auto load_b = [&](int b) {
+        BVals v;
+        v.x = S.x[b];
+        v.y = S.y[b];
+        v.z = S.z[b];
+        v.id = S.id[b];
+        v.fc = S.fc[b];
+#pragma unroll
+        for (int e = 0; e < NE; ++e) v.E[e] = (LZ > 0 || e < ne) ? S.E[e * cap + b] : (R)0;
+        return v;
+    };
+    auto pair_terms_row = [&](const BVals& v) {
+        const R cx = v.x - rx, cy = v.y - ry, cz = v.z - rz;
         const R d2c = cx * cx + cy * cy + cz * cz;
         if (!(d2c >= (R)1e-15 && d2c < Rc2)) return;
         const R dc = d2c * drsq_(d2c);
         const R fcc = (R)0.5 * (dcos_(pirc * dc) + (R)1);
-        const R cs = (rx * bx + ry * by + rz * bz) * rid * S.id[b];
-        const R F = rfc * S.fc[b] * fcc;
+        const R cs = (rx * v.x + ry * v.y + rz * v.z) * rid * v.id;
+        const R F = rfc * v.fc * fcc;
         R Ex[NE];
 #pragma unroll
         for (int e = 0; e < NE; ++e)
-            Ex[e] = (LZ > 0 || e < ne) ? rE[e] * S.E[e * cap + b] * dexp_(-eta_e[e] * d2c) * F : (R)0;
+            Ex[e] = (LZ > 0 || e < ne) ? rE[e] * v.E[e] * dexp_(-eta_e[e] * d2c) * F : (R)0;
         ang_terms(cs, Ex);
     };
     if constexpr (sizeof(R) == 8) {
@@ -520,7 +535,7 @@ __global__ void __launch_bounds__(256) k_bp_desc(const double4* __restrict__ pos
             if (!more) break;
         }
     } else {
-        circ_walk(nc, lane, load_row, [&](int, int b) { pair_terms_row(b); });
+        circ_walk(nc, lane, load_row, load_b, [&](int, int, const BVals& v) { pair_terms_row(v); });
     }
     if constexpr (K2) {
         // the angular sums of the pairs walked so far into block `key`
@@ -558,7 +573,8 @@ __global__ void __launch_bounds__(256) k_bp_desc(const double4* __restrict__ pos
                 if (in_range(a, b)) pair_terms(a, b);
             }
         } else {
-            circ_walk(nd, lane, [&](int a) { load_row(d0 + a); }, [&](int, int b) { pair_terms_row(d0 + b); });
+            circ_walk(nd, lane, [&](int a) { load_row(d0 + a); }, [&](int b) { return load_b(d0 + b); },
+                      [&](int, int, const BVals& v) { pair_terms_row(v); });
         }
         store_ang(2);
 #pragma unroll
@@ -577,7 +593,7 @@ __global__ void __launch_bounds__(256) k_bp_desc(const double4* __restrict__ pos
     }
 #pragma unroll
     for (int q = 0; q < NR + NG * NE; ++q) acc[q] = wave_sum(acc[q]);
-    if (lane == 0) {
+    if (grow && lane == 0) {
         R* g = grow;
         for (int q = 0; q < ldg; ++q) g[q] = (R)0;
 #pragma unroll
@@ -591,6 +607,29 @@ __global__ void __launch_bounds__(256) k_bp_desc(const double4* __restrict__ pos
                 if (gg < P.ng && e < P.ne && P.t_q[t] >= 0) g[P.nr + P.t_q[t]] = (R)P.t_coef[t] * acc[NR + gg * NE + e];
             }
     }
+}
+
+template <typename R, int NR, int NG, int NE, int LZ = 0, bool K2 = false>
+__global__ void __launch_bounds__(256) k_bp_desc(const double4* __restrict__ pos, const int* __restrict__ nl,
+                                                 const int* __restrict__ nn, int tpa, int M, int cap,
+                                                 const SysDesc* __restrict__ desc, const BpDesc* __restrict__ Pd,
+                                                 R* __restrict__ G, int ldg, int n, StepArgs SA,
+                                                 int* __restrict__ misc) {
+    if (skip_step(SA, misc)) return;
+    extern __shared__ char smem[];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + wv;
+    if (i >= n) return;
+    const BpDesc& P = *Pd;
+    const Stage<R> S = carve<R>(smem + wv * stage_bytes(cap, P.ne, sizeof(R), false), cap, P.ne, false);
+    const double4 pi = pos[i];
+    const SysDesc& s = desc[w_sys(pi.w)];
+    const int cnt = min(nn[i], cap);
+    int nd = 0;
+    const int nc = bp_stage<R>(i, pi, s, nl, cnt, tpa, M, pos, P, S, (R*)nullptr, (R*)nullptr, cap,
+                               K2 ? (int)w_kind(pi.w) : -1, nd);
+    R acc[NR + NG * NE];
+    desc_wave<R, NR, NG, NE, LZ, K2>(P, S, nc, nd, cap, lane, G + (size_t)i * ldg, ldg, acc);
 }
 
 // pass 2 (FP64): L lanes per atom (32 when the input and hidden widths fit,
@@ -778,31 +817,41 @@ __global__ void __launch_bounds__(64) k_bp_mlp_mfma(const float* __restrict__ G,
 // SA = sum_q c_q Ex pz_q, SAe = sum_q c_q eta Ex pz_q and
 // SZ = sum_q c_q Ex zeta lam pz1_q, where pz = (1 + lam cos)^zeta and pz1 =
 // (1 + lam cos)^(zeta - 1).  LZ > 0 (ladder layout): rung b holds
-// (1 +- cos)^(2^b) and (1 +- cos)^(2^b - 1) by squaring and running products.
-template <typename R, int NG, int NE, int LZ>
-__device__ inline void group_sums(R cs, const R* C, const R* Ex, const R* Exe, const R* g_lam, const R* g_zl,
-                                  const int* g_iz, int ng, R& SA, R& SAe, R& SZ) {
+// (1 +- cos)^(2^b) and (1 +- cos)^(2^b - 1) by squaring and running
+// products; the rungs are summed per eta first, A_e = sum_b (c+ P+ + c- P-)
+// and Z_e = sum_b (cz+ Q+ + cz- Q-) with Cz the coefficients times 2^b and
+// -2^b, then SA = sum_e Ex_e A_e, SAe = sum_e eta_e Ex_e A_e, SZ = sum_e
+// Ex_e Z_e.  Rungs outside the mask ZM have no terms and are skipped (their
+// powers are still formed for the rungs above).
+template <typename R, int NG, int NE, int LZ, int ZM = -1>
+__device__ inline void group_sums(R cs, const R* C, const R* Cz, const R* Ex, const R* Exe, const R* g_lam,
+                                  const R* g_zl, const int* g_iz, int ng, R& SA, R& SAe, R& SZ) {
     if constexpr (LZ > 0) {
         R Pp = (R)1 + cs, Pm = (R)1 - cs, Qp = (R)1, Qm = (R)1;
+        R A[NE], Z[NE];
+#pragma unroll
+        for (int e = 0; e < NE; ++e) A[e] = Z[e] = (R)0;
 #pragma unroll
         for (int b = 0; b < LZ; ++b) {
-            R Kp = 0, Kpe = 0, Km = 0, Kme = 0;
+            if ((ZM >> b) & 1) {
 #pragma unroll
-            for (int e = 0; e < NE; ++e) {
-                Kp += C[(2 * b) * NE + e] * Ex[e];
-                Kpe += C[(2 * b) * NE + e] * Exe[e];
-                Km += C[(2 * b + 1) * NE + e] * Ex[e];
-                Kme += C[(2 * b + 1) * NE + e] * Exe[e];
+                for (int e = 0; e < NE; ++e) {
+                    A[e] += C[(2 * b) * NE + e] * Pp + C[(2 * b + 1) * NE + e] * Pm;
+                    Z[e] += Cz[(2 * b) * NE + e] * Qp + Cz[(2 * b + 1) * NE + e] * Qm;
+                }
             }
-            SA += Kp * Pp + Km * Pm;
-            SAe += Kpe * Pp + Kme * Pm;
-            SZ += (R)(1 << b) * (Kp * Qp - Km * Qm);
             if (b + 1 < LZ) {
                 Qp *= Pp;
                 Pp *= Pp;
                 Qm *= Pm;
                 Pm *= Pm;
             }
+        }
+#pragma unroll
+        for (int e = 0; e < NE; ++e) {
+            SA += Ex[e] * A[e];
+            SAe += Exe[e] * A[e];
+            SZ += Ex[e] * Z[e];
         }
     } else {
 #pragma unroll
@@ -846,26 +895,14 @@ __device__ inline float sum_stride8(float v) {
     return v;
 }
 
-template <typename R, int NR, int NG, int NE, bool VIR, int LZ = 0, bool K2 = false>
-__global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos, const int* __restrict__ nl,
-                                                 const int* __restrict__ nn, int tpa, int M, int cap,
-                                                 const SysDesc* __restrict__ desc, const BpDesc* __restrict__ Pd,
-                                                 const R* __restrict__ wgt, int ldg, R* __restrict__ Pout,
-                                                 R* __restrict__ own, R* __restrict__ PV, R* __restrict__ OV, int n,
-                                                 StepArgs SA, int* __restrict__ misc) {
-    if (skip_step(SA, misc)) return;
-    extern __shared__ char smem[];
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int i = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + wv;
-    if (i >= n) return;
-    const BpDesc& P = *Pd;
-    const Stage<R> S = carve<R>(smem + wv * stage_bytes(cap, P.ne, sizeof(R), true), cap, P.ne, true);
-    const double4 pi = pos[i];
-    const SysDesc& s = desc[w_sys(pi.w)];
-    const int cnt = min(nn[i], cap);
-    int nd = 0;
-    const int nc = bp_stage<R>(i, pi, s, nl, cnt, tpa, M, pos, P, S, Pout, VIR ? PV : (R*)nullptr, cap,
-                               K2 ? (int)w_kind(pi.w) : -1, nd);
+// The force shares of one centre from its staged neighbours and its dE/dG,
+// by one wavefront: wsrc(q) is element q of the centre's dE/dG (input
+// order); shares go to Pout [i][slot], the centre's own to own [i], and
+// with VIR the virial shares to PV and OV.
+template <typename R, int NR, int NG, int NE, bool VIR, int LZ, bool K2, int ZM = -1, class WF>
+__device__ inline void grad_wave(const BpDesc& P, const Stage<R>& S, int i, int nc, int nd, int cap, int lane,
+                                 WF&& wsrc, R* __restrict__ Pout, R* __restrict__ own, R* __restrict__ PV,
+                                 R* __restrict__ OV) {
     const int d0 = cap - nd;  // K2: the other kind's neighbours, [d0, cap)
     const int nr = P.nr, ng = P.ng, ne = P.ne;
     R eta_r[NR], g_lam[NG], g_zl[NG], eta_e[NE];
@@ -882,13 +919,15 @@ __global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos
     for (int e = 0; e < NE; ++e) eta_e[e] = e < ne ? (R)P.eta_e[e] : (R)0;
     R w[NR], w1[K2 ? NR : 1];
 #pragma unroll
-    for (int q = 0; q < NR; ++q) w[q] = q < nr ? wgt[(size_t)i * ldg + P.sr[0] + q] : (R)0;
+    for (int q = 0; q < NR; ++q) w[q] = q < nr ? wsrc(P.sr[0] + q) : (R)0;
     if constexpr (K2)
 #pragma unroll
-        for (int q = 0; q < NR; ++q) w1[q] = q < nr ? wgt[(size_t)i * ldg + P.sr[1] + q] : (R)0;
+        for (int q = 0; q < NR; ++q) w1[q] = q < nr ? wsrc(P.sr[1] + q) : (R)0;
     // dE/dG of the G4 terms of angular block `key` times their coefficients
     // on the (group, eta) grid
-    R C[NG * NE];
+    // (ladder layout: Cz, the same times 2^b and -2^b for group rows 2b and
+    // 2b + 1, the zeta lam factors of rung b)
+    R C[NG * NE], Cz[NG * NE];
     auto load_C = [&](int key) {
 #pragma unroll
         for (int g = 0; g < NG; ++g)
@@ -896,7 +935,8 @@ __global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos
             for (int e = 0; e < NE; ++e) {
                 const int t = g * kBpMaxEta + e;
                 const int q = (g < ng && e < ne) ? P.t_q[t] : -1;
-                C[g * NE + e] = q >= 0 ? wgt[(size_t)i * ldg + P.sa[key] + q] * (R)P.t_coef[t] : (R)0;
+                C[g * NE + e] = q >= 0 ? wsrc(P.sa[key] + q) * (R)P.t_coef[t] : (R)0;
+                Cz[g * NE + e] = (R)((g & 1) ? -(1 << (g >> 1)) : (1 << (g >> 1))) * C[g * NE + e];
             }
     };
     load_C(0);
@@ -953,15 +993,34 @@ __global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos
     // lane keeps its row a through a row of tiles, so the a-sides sum in
     // registers and meet across lanes once per row of tiles; the b-sides of
     // a tile sum over its 8 adjacent lanes of one column by lane exchanges
-    // on the vector unit.  Neighbours are staged in list order, which keeps
-    // those of one stencil cell together, so that whole tiles of pairs lie
-    // beyond the cutoff and are skipped.  Rows a in [r0, r0 + nrow),
-    // columns b in [c0, c0 + ncol); same: the two ranges are one and only
-    // pairs a < b are taken.
+    // on the vector unit.  Nearly every tile holds a pair in range (of the
+    // 55 tiles of 77 neighbours, about 52), so none is skipped, and all the
+    // column values of a tile are read at once.
+    // Rows a in [r0, r0 + nrow), columns b in [c0, c0 + ncol); same: the two
+    // ranges are one and only pairs a < b are taken.
+    struct GVals {
+        R x, y, z, d, id, fc, dfc, E[NE];
+    };
     auto walk = [&](int r0, int nrow, int c0, int ncol, bool same) {
     if (nrow <= 0 || ncol <= 0) return;
     const int ar = lane & 7, bc = lane >> 3;
     const int tr = (nrow + 7) >> 3, tc = (ncol + 7) >> 3;
+    // column values of tile tj for this lane's column
+    auto load_g = [&](int tj) {
+        const int cb = tj * 8 + bc;
+        const int b = c0 + (cb < ncol ? cb : 0);
+        GVals v;
+        v.x = S.x[b];
+        v.y = S.y[b];
+        v.z = S.z[b];
+        v.d = S.d[b];
+        v.id = S.id[b];
+        v.fc = S.fc[b];
+        v.dfc = S.dfc[b];
+#pragma unroll
+        for (int e = 0; e < NE; ++e) v.E[e] = (LZ > 0 || e < ne) ? S.E[e * cap + b] : (R)0;
+        return v;
+    };
     for (int ti = 0; ti < tr; ++ti) {
         const int ra = ti * 8 + ar;
         const bool va = ra < nrow;
@@ -972,15 +1031,14 @@ __global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos
         for (int e = 0; e < NE; ++e) Ea[e] = (LZ > 0 || e < ne) ? S.E[e * cap + a] : (R)0;
         R hx = 0, hy = 0, hz = 0;
         for (int tj = same ? ti : 0; tj < tc; ++tj) {
+            const GVals bv = load_g(tj);
             const int cb = tj * 8 + bc;
             const bool vb = cb < ncol;
             const int b = c0 + (vb ? cb : 0);
-            const R bx = S.x[b], by = S.y[b], bz = S.z[b];
+            const R bx = bv.x, by = bv.y, bz = bv.z, db = bv.d, ib = bv.id, fb = bv.fc, dfb = bv.dfc;
             const R cx = bx - ax, cy = by - ay, cz = bz - az;
             const R d2c = cx * cx + cy * cy + cz * cz;
             const bool ok = va && vb && (!same || ra < cb) && d2c >= (R)1e-15 && d2c < Rc2;
-            if (!__ballot(ok)) continue;  // the whole tile beyond the cutoff
-            const R db = S.d[b], ib = S.id[b], fb = S.fc[b], dfb = S.dfc[b];
             const R ic = drsq_(d2c);
             const R dc = d2c * ic;
             const R fcc = (R)0.5 * (dcos_(pirc * dc) + (R)1);
@@ -990,11 +1048,11 @@ __global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos
             R Ex[NE], Exe[NE];
 #pragma unroll
             for (int e = 0; e < NE; ++e) {
-                Ex[e] = (LZ > 0 || e < ne) ? Ea[e] * S.E[e * cap + b] * dexp_(-eta_e[e] * d2c) : (R)0;
+                Ex[e] = (LZ > 0 || e < ne) ? Ea[e] * bv.E[e] * dexp_(-eta_e[e] * d2c) : (R)0;
                 Exe[e] = Ex[e] * eta_e[e];
             }
             R SA = 0, SAe = 0, SZ = 0;
-            group_sums<R, NG, NE, LZ>(cs, C, Ex, Exe, g_lam, g_zl, g_iz, ng, SA, SAe, SZ);
+            group_sums<R, NG, NE, LZ, ZM>(cs, C, Cz, Ex, Exe, g_lam, g_zl, g_iz, ng, SA, SAe, SZ);
             SZ *= F;
             const R cc = ((R)2 * F * dc * SAe - fa * fb * dfcc * SA) * ic;
             const R ka = (-SZ * cs * ia - (R)2 * F * da * SAe + fb * dfa * fcc * SA) * ia;
@@ -1004,12 +1062,15 @@ __global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos
                 if (ok)
                     vir_pair((R)1, ax, ay, az, bx, by, bz, cx, cy, cz, ia, ib, ic, fa, fb, dfa, dfb, fcc, dfcc, cs, F,
                              SA, SAe, SZ);
-            hx += ok ? ka * ax + kb * bx + cc * cx : (R)0;
-            hy += ok ? ka * ay + kb * by + cc * cy : (R)0;
-            hz += ok ? ka * az + kb * bz + cc * cz : (R)0;
-            const R gx = sum_lanes8(ok ? la * bx + kb * ax - cc * cx : (R)0);
-            const R gy = sum_lanes8(ok ? la * by + kb * ay - cc * cy : (R)0);
-            const R gz = sum_lanes8(ok ? la * bz + kb * az - cc * cz : (R)0);
+            // with c = b - a: d/dr_a = (ka - cc) a + (kb + cc) b and
+            // d/dr_b = (kb + cc) a + (la - cc) b
+            const R pa = ok ? ka - cc : (R)0, pb = ok ? kb + cc : (R)0, qb = ok ? la - cc : (R)0;
+            hx += pa * ax + pb * bx;
+            hy += pa * ay + pb * by;
+            hz += pa * az + pb * bz;
+            const R gx = sum_lanes8(pb * ax + qb * bx);
+            const R gy = sum_lanes8(pb * ay + qb * by);
+            const R gz = sum_lanes8(pb * az + qb * bz);
             if (ar == 0 && vb) {
                 atomicAdd(&S.g[b], gx);
                 atomicAdd(&S.g[cap + b], gy);
@@ -1059,7 +1120,7 @@ __global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos
             Exe[e] = Ex[e] * eta_e[e];
         }
         R SA = 0, SAe = 0, SZ = 0;
-        group_sums<R, NG, NE, LZ>(cs, C, Ex, Exe, g_lam, g_zl, g_iz, ng, SA, SAe, SZ);
+        group_sums<R, NG, NE, LZ, ZM>(cs, C, Cz, Ex, Exe, g_lam, g_zl, g_iz, ng, SA, SAe, SZ);
         SZ *= F;
         const R cc = ((R)2 * F * dc * SAe - fa * fb * dfcc * SA) * ic;
         const R ka = (-SZ * cs * ia - (R)2 * F * da * SAe + fb * dfa * fcc * SA) * ia;
@@ -1126,6 +1187,124 @@ __global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos
             OV[(size_t)i * 3 + 2] = vz;
         }
     }
+}
+
+template <typename R, int NR, int NG, int NE, bool VIR, int LZ = 0, bool K2 = false>
+__global__ void __launch_bounds__(256) k_bp_grad(const double4* __restrict__ pos, const int* __restrict__ nl,
+                                                 const int* __restrict__ nn, int tpa, int M, int cap,
+                                                 const SysDesc* __restrict__ desc, const BpDesc* __restrict__ Pd,
+                                                 const R* __restrict__ wgt, int ldg, R* __restrict__ Pout,
+                                                 R* __restrict__ own, R* __restrict__ PV, R* __restrict__ OV, int n,
+                                                 StepArgs SA, int* __restrict__ misc) {
+    if (skip_step(SA, misc)) return;
+    extern __shared__ char smem[];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + wv;
+    if (i >= n) return;
+    const BpDesc& P = *Pd;
+    const Stage<R> S = carve<R>(smem + wv * stage_bytes(cap, P.ne, sizeof(R), true), cap, P.ne, true);
+    const double4 pi = pos[i];
+    const SysDesc& s = desc[w_sys(pi.w)];
+    const int cnt = min(nn[i], cap);
+    int nd = 0;
+    const int nc = bp_stage<R>(i, pi, s, nl, cnt, tpa, M, pos, P, S, Pout, VIR ? PV : (R*)nullptr, cap,
+                               K2 ? (int)w_kind(pi.w) : -1, nd);
+    grad_wave<R, NR, NG, NE, VIR, LZ, K2>(P, S, i, nc, nd, cap, lane,
+                                          [&](int q) { return wgt[(size_t)i * ldg + q]; }, Pout, own, PV, OV);
+}
+
+// lane p's value of v, p uniform over the wavefront
+__device__ inline float readlane_f(float v, int p) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), p));
+}
+
+// The network of one centre by one wavefront in FP32 (maise_ann.cxx:8-55,
+// maise_bpnn.cxx:275-325): lane j holds unit j of each layer (widths up to
+// 64) and takes the previous layer's outputs one at a time from the lane
+// that holds them.  The input vector comes from the descriptor sums acc
+// (every lane holds them) in input order: radial term r at r, angular grid
+// term t at nr + t_q[t] with its coefficient.  w: the FP64 block's layout
+// in FP32 (W01 [h1][nin] range-scaled, b01, W12 [h2][h1], b12, W23, b23,
+// W01T [nin][h1], W12T [h1][h2]).  E = b23 + W23 . h2; returns dE/dx_k in
+// lane k (0 in lanes k >= nin).
+template <int NR, int NG, int NE>
+__device__ inline float mlp_wave(const BpDesc& P, const float* __restrict__ w, const float (&acc)[NR + NG * NE],
+                                 int lane, float& E) {
+    const int nin = P.nin, h1 = P.h1, h2 = P.h2, nr = P.nr;
+    const float* W01 = w;
+    const float* b01 = W01 + h1 * nin;
+    const float* W12 = b01 + h1;
+    const float* b12 = W12 + h2 * h1;
+    const float* W23 = b12 + h2;
+    const float b23 = W23[h2];
+    const float* W01T = W23 + h2 + 1;
+    const float* W12T = W01T + nin * h1;
+    // lanes past a layer's width read its last unit and discard the result
+    const int j1 = min(lane, h1 - 1), j2 = min(lane, h2 - 1), k = min(lane, nin - 1);
+    float s1 = b01[j1];
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+        if (r < nr) s1 = fmaf(W01T[r * h1 + j1], acc[r], s1);
+#pragma unroll
+    for (int g = 0; g < NG; ++g)
+#pragma unroll
+        for (int e = 0; e < NE; ++e) {
+            const int t = g * kBpMaxEta + e;
+            const int q = (g < P.ng && e < P.ne) ? P.t_q[t] : -1;
+            if (q >= 0) s1 = fmaf(W01T[(nr + q) * h1 + j1], (float)P.t_coef[t] * acc[NR + g * NE + e], s1);
+        }
+    const float y1 = lane < h1 ? tanhf(s1) : 0.f;
+    float s2 = b12[j2];
+    for (int p = 0; p < h1; ++p) s2 = fmaf(W12T[p * h2 + j2], readlane_f(y1, p), s2);
+    const float y2 = lane < h2 ? tanhf(s2) : 0.f;
+    const float e2 = lane < h2 ? W23[j2] * y2 : 0.f;
+    const float t2 = lane < h2 ? W23[j2] * (1.f - y2 * y2) : 0.f;
+    E = b23 + wave_sum(e2);
+    float s3 = 0.f;
+    for (int q = 0; q < h2; ++q) s3 = fmaf(W12[q * h1 + j1], readlane_f(t2, q), s3);
+    const float d1 = lane < h1 ? s3 * (1.f - y1 * y1) : 0.f;
+    float so = 0.f;
+    for (int p = 0; p < h1; ++p) so = fmaf(W01[p * nin + k], readlane_f(d1, p), so);
+    return lane < nin ? so : 0.f;
+}
+
+// passes 1-3 in one kernel (FP32, one species): one wavefront per centre
+// stages the neighbours once, forms the descriptors, runs the network with
+// dE/dG kept in registers and writes the force shares, as k_bp_desc,
+// k_bp_mlp_mfma and k_bp_grad do one after the other (maise_bpnn.cxx:51-96,
+// 122-325).  G and dE/dG are written as well, for inspection.
+template <int NR, int NG, int NE, bool VIR, int LZ, int ZM = -1>
+__global__ void __launch_bounds__(256) k_bp_fused(const double4* __restrict__ pos, const int* __restrict__ nl,
+                                                  const int* __restrict__ nn, int tpa, int M, int cap,
+                                                  const SysDesc* __restrict__ desc, const BpDesc* __restrict__ Pd,
+                                                  const float* __restrict__ mw, float* __restrict__ G,
+                                                  float* __restrict__ wout, int ldg, double* __restrict__ Ep,
+                                                  float* __restrict__ Pout, float* __restrict__ own,
+                                                  float* __restrict__ PV, float* __restrict__ OV, int n, StepArgs SA,
+                                                  int* __restrict__ misc) {
+    if (skip_step(SA, misc)) return;
+    extern __shared__ char smem[];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + wv;
+    if (i >= n) return;
+    const BpDesc& P = *Pd;
+    const Stage<float> S = carve<float>(smem + wv * stage_bytes(cap, P.ne, sizeof(float), true), cap, P.ne, true);
+    const double4 pi = pos[i];
+    const SysDesc& s = desc[w_sys(pi.w)];
+    const int cnt = min(nn[i], cap);
+    int nd = 0;
+    const int nc = bp_stage<float>(i, pi, s, nl, cnt, tpa, M, pos, P, S, Pout, VIR ? PV : (float*)nullptr, cap, -1,
+                                   nd);
+    float acc[NR + NG * NE];
+    desc_wave<float, NR, NG, NE, LZ, false, ZM>(P, S, nc, nd, cap, lane, G + (size_t)i * ldg, ldg, acc);
+    // inactive centres: no energy and zero dE/dG
+    const bool act = !w_inactive(pi.w);
+    float E = 0.f;
+    const float dx = act ? mlp_wave<NR, NG, NE>(P, mw, acc, lane, E) * (float)P.escale : 0.f;
+    if (lane < ldg) wout[(size_t)i * ldg + lane] = dx;
+    if (act && lane == 0) Ep[i] = (double)E * P.escale;
+    grad_wave<float, NR, NG, NE, VIR, LZ, false, ZM>(P, S, i, nc, nd, cap, lane,
+                                                     [&](int q) { return readlane_f(dx, q); }, Pout, own, PV, OV);
 }
 
 // reverse slot index: rev(m, k) = slot of m in the list of its k-th
@@ -1598,10 +1777,13 @@ Pot* bpnn_create(Ctx* ctx, const char* path) {
         bp->d = dnew<BpDesc>(1);
         bp->w64 = dnew<double>(W64.size());
         bp->w32 = dnew<float>(W32.size());
+        bp->w64f = dnew<float>(W64.size());
         bp->bsplit = dnew<half4_t>(B.size());
         HIPCHECK(hipMemcpy(bp->d, &D, sizeof(D), hipMemcpyHostToDevice));
         HIPCHECK(hipMemcpy(bp->w64, W64.data(), W64.size() * sizeof(double), hipMemcpyHostToDevice));
         HIPCHECK(hipMemcpy(bp->w32, W32.data(), W32.size() * sizeof(float), hipMemcpyHostToDevice));
+        const std::vector<float> W64f(W64.begin(), W64.end());
+        HIPCHECK(hipMemcpy(bp->w64f, W64f.data(), W64f.size() * sizeof(float), hipMemcpyHostToDevice));
         HIPCHECK(hipMemcpy(bp->bsplit, B.data(), B.size() * sizeof(half4_t), hipMemcpyHostToDevice));
     } catch (...) {
         bpnn_destroy(p);
@@ -1643,7 +1825,7 @@ int bpnn_species(const Pot& p) { return p.bp ? p.bp->h.nspc : 1; }
 void bpnn_destroy(Pot* p) {
     if (p->bp) {
         BpnnParams* bp = p->bp;
-        for (void* q : {(void*)bp->d, (void*)bp->w64, (void*)bp->w32, (void*)bp->bsplit})
+        for (void* q : {(void*)bp->d, (void*)bp->w64, (void*)bp->w32, (void*)bp->w64f, (void*)bp->bsplit})
             if (q) (void)hipFree(q);
         delete bp;
     }
@@ -1741,6 +1923,33 @@ static void bpnn_run(Batch& b, const Pot& p, const BpBufs& W, bool rev_fresh, in
                                 : k_bp_desc<R, kBpMaxR, kBpMaxG, kBpMaxEta, 0, T2>;
         };
         auto kdesc = two ? pick_desc(std::true_type{}) : pick_desc(std::false_type{});
+        // FP32, one species, descriptors through forces: the fused pass
+        if constexpr (sizeof(R) == 4)
+            if (!two && (parts & 3) == 3 && !(parts & 4)) {
+                // ladder rungs with G4 terms: zeta in {1, 4} (rungs 0 and 2 of
+                // three) and {1, 2, 4, 16} (all but rung 3 of five) have
+                // kernels that skip the empty rungs
+                int used = 0;
+                for (int g = 0; g < kBpGridG; ++g)
+                    for (int e = 0; e < kBpMaxEta; ++e)
+                        if (bp.h.t_q[g * kBpMaxEta + e] >= 0) used |= 1 << (g >> 1);
+                const bool z3 = (used & ~0b101) == 0, z5 = (used & ~0b10111) == 0;
+                auto fused = [&](auto vtag) {
+                    constexpr bool V = decltype(vtag)::value;
+                    auto kf = shape == 3   ? (z3 ? k_bp_fused<8, 6, 2, V, 3, 0b101> : k_bp_fused<8, 6, 2, V, 3>)
+                              : shape == 4 ? (z5 ? k_bp_fused<8, 10, 7, V, 5, 0b10111> : k_bp_fused<8, 10, 7, V, 5>)
+                              : shape == 0 ? k_bp_fused<8, 4, 2, V, 0>
+                              : shape == 1 ? k_bp_fused<8, kBpMaxG, kBpMaxEta, V, 0>
+                                           : k_bp_fused<kBpMaxR, kBpMaxG, kBpMaxEta, V, 0>;
+                    launch_k(b, K_BP_FUSED, kf, gw, bw, (unsigned)lds, pos, b.nl, b.nn, b.tpa, b.M, b.cap, b.d_desc,
+                             bp.d, (const float*)bp.w64f, (float*)W.G, (float*)W.w, ldg, Ep, (float*)W.P,
+                             (float*)W.own, (float*)W.PV, (float*)W.OV, n, S, b.misc);
+                };
+                if (vir) fused(std::true_type{});
+                else fused(std::false_type{});
+                parts &= ~3;
+                parts |= 8;  // gather still to do
+            }
         if (parts & 1) {
             launch_k(b, K_BP_DESC, kdesc, gw, bw, (unsigned)lds_desc, pos, b.nl, b.nn, b.tpa, b.M, b.cap, b.d_desc,
                      bp.d, (R*)W.G, ldg, n, S, b.misc);
@@ -1760,7 +1969,7 @@ static void bpnn_run(Batch& b, const Pot& p, const BpBufs& W, bool rev_fresh, in
                              (float*)W.w, Ep, n, only, S, b.misc);
             }
         }
-        if (!(parts & 2)) return;
+        if (!(parts & 2) && !(parts & 8)) return;
         auto grad = [&](auto vtag) {
             constexpr bool V = decltype(vtag)::value;
             auto pick_grad = [&](auto k2) {
@@ -1775,8 +1984,10 @@ static void bpnn_run(Batch& b, const Pot& p, const BpBufs& W, bool rev_fresh, in
             launch_k(b, K_BP_FORCE, kgrad, gw, bw, (unsigned)lds, pos, b.nl, b.nn, b.tpa, b.M, b.cap, b.d_desc, bp.d,
                      (const R*)W.w, ldg, (R*)W.P, (R*)W.own, (R*)W.PV, (R*)W.OV, n, S, b.misc);
         };
-        if (vir) grad(std::true_type{});
-        else grad(std::false_type{});
+        if (parts & 2) {
+            if (vir) grad(std::true_type{});
+            else grad(std::false_type{});
+        }
         auto gather = [&](auto tpa_tag, auto vtag) {
             constexpr int T = decltype(tpa_tag)::value;
             constexpr bool V = decltype(vtag)::value;

@@ -292,15 +292,20 @@ struct Batch {
     std::vector<hipEvent_t> ev_free;
     struct Timed { hipEvent_t a, b; int kind; };
     std::vector<Timed> ev_pending;
-    double kt_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    long kt_n[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double kt_ms[9] = {};  // per timed kernel kind (KKind)
+    long kt_n[9] = {};
 
     double* d(int s, int f) const { return dst[s] + (size_t)f * n; }
     float* fl(int s, int f) const { return fst[s] + (size_t)f * n; }
 };
 
 // timed kernel kinds
-enum KKind { K_FORCE = 0, K_DENSITY = 1, K_PREDRIFT = 2, K_BP_DESC = 3, K_BP_MLP = 4, K_BP_FORCE = 5, K_BP_GATHER = 6, K_REBUILD = 7 };
+enum KKind {
+    K_FORCE = 0, K_DENSITY = 1, K_PREDRIFT = 2, K_BP_DESC = 3, K_BP_MLP = 4, K_BP_FORCE = 5, K_BP_GATHER = 6, K_REBUILD = 7,
+    K_BP_FUSED = 8
+};
+constexpr int kKinds = 9;
+static_assert(sizeof(Batch::kt_n) / sizeof(long) == kKinds, "one timing slot per kernel kind");
 hipEvent_t timer_begin(Batch& b);
 void timer_end(Batch& b, hipEvent_t a, int kind);
 void timer_drain(Batch& b);

@@ -482,7 +482,7 @@ class System:
         return last.value
 
     def timing(self, enable=-1):
-        ms = np.zeros(8)
-        n = np.zeros(8, dtype=np.int64)
+        ms = np.zeros(9)
+        n = np.zeros(9, dtype=np.int64)
         _check(lib().mdc_timing(self.h, enable, _dp(ms), n.ctypes.data_as(C.POINTER(C.c_int64))))
         return ms, n
