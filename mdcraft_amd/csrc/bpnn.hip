@@ -247,11 +247,12 @@ struct PairWalk {
 // is even.  Lane l takes the l-th contiguous 64th of these pairs in order
 // of a, then d, so it stays on one a for long runs and steps to the next
 // without a search; row(a) is called when the lane reaches a new row a (to
-// keep that neighbour's values in registers), fn(a, b, v) for each pair
-// with v = load(b), which is issued one pair ahead so that its latency
-// overlaps the current pair's arithmetic.
-template <class Row, class Load, class Fn>
-__device__ inline void circ_walk(int n, int lane, Row&& row, Load&& load, Fn&& fn) {
+// keep that neighbour's values in registers), fn(a, b, live) at every step:
+// live is false for a lane past the end of its run, whose (a, b) is then a
+// pair of staged neighbours to be ignored.  The body runs in every lane,
+// so the pair arithmetic sits in no divergent branch.
+template <class Row, class Fn>
+__device__ inline void circ_walk(int n, int lane, Row&& row, Fn&& fn) {
     if (n < 2) return;
     const int hd = (n - 1) / 2;
     const int ex = (n % 2 == 0) ? n / 2 : 0;  // rows with one more partner (d = n / 2)
@@ -268,32 +269,21 @@ __device__ inline void circ_walk(int n, int lane, Row&& row, Load&& load, Fn&& f
         a = ex + v / max(hd, 1);
         d = v - (a - ex) * max(hd, 1) + 1;
     }
-    a = min(a, n - 1);  // lanes without pairs (T < 64) stay in range
-    int b = a + d;
-    if (b >= n) b -= n;
-    b = min(b, n - 1);
-    auto v = load(b);
     int cur = -1;
     for (int k = 0; k < steps; ++k) {
-        if (u0 + k < u1) {
-            if (a != cur) {
-                row(a);
-                cur = a;
-            }
-            int a2 = a, d2 = d + 1;
-            if (d2 > hd + (a < ex ? 1 : 0)) {
-                d2 = 1;
-                ++a2;
-            }
-            int b2 = a2 + d2;
-            if (b2 >= n) b2 -= n;
-            const bool more = u0 + k + 1 < u1;
-            auto v2 = load(more ? b2 : b);
-            fn(a, b, v);
-            a = a2;
-            d = d2;
-            b = b2;
-            v = v2;
+        const bool live = u0 + k < u1;
+        if (live && a != cur) {
+            row(a);
+            cur = a;
+        }
+        // past its run a lane steps on by at most one row (a <= n), so b
+        // stays below n
+        int b = a + d;
+        if (b >= n) b -= n;
+        fn(a, b, live);
+        if (++d > hd + (a < ex ? 1 : 0)) {
+            d = 1;
+            ++a;
         }
     }
 }
@@ -475,32 +465,28 @@ __device__ inline void desc_wave(const BpDesc& P, const Stage<R>& S, int nc, int
 #pragma unroll
         for (int e = 0; e < NE; ++e) rE[e] = (LZ > 0 || e < ne) ? S.E[e * cap + a] : (R)0;
     };
-    struct BVals {
-        R x, y, z, id, fc, E[NE];
-    };
-    auto load_b = [&](int b) {
-        BVals v;
-        v.x = S.x[b];
-        v.y = S.y[b];
-        v.z = S.z[b];
-        v.id = S.id[b];
-        v.fc = S.fc[b];
+    // all of b's values are read at once, and a pair out of range (or a
+    // lane past its run) gets F = 0 by a 0/1 factor rather than a branch:
+    // the lanes of a step run the arithmetic whenever any of them has a
+    // pair in range, and a branch only adds the copies that merge its two
+    // sides.  1/|r_c| is taken of at least 1e-30 so that every factor is
+    // finite.
+    auto pair_terms_row = [&](int b, bool live) {
+        const R bx = S.x[b], by = S.y[b], bz = S.z[b], bid = S.id[b], bfc = S.fc[b];
+        R bE[NE];
 #pragma unroll
-        for (int e = 0; e < NE; ++e) v.E[e] = (LZ > 0 || e < ne) ? S.E[e * cap + b] : (R)0;
-        return v;
-    };
-    auto pair_terms_row = [&](const BVals& v) {
-        const R cx = v.x - rx, cy = v.y - ry, cz = v.z - rz;
+        for (int e = 0; e < NE; ++e) bE[e] = (LZ > 0 || e < ne) ? S.E[e * cap + b] : (R)0;
+        const R cx = bx - rx, cy = by - ry, cz = bz - rz;
         const R d2c = cx * cx + cy * cy + cz * cz;
-        if (!(d2c >= (R)1e-15 && d2c < Rc2)) return;
-        const R dc = d2c * drsq_(d2c);
+        const R ok = (live && d2c >= (R)1e-15 && d2c < Rc2) ? (R)1 : (R)0;
+        const R dc = d2c * drsq_(fmax(d2c, (R)1e-30));
         const R fcc = (R)0.5 * (dcos_(pirc * dc) + (R)1);
-        const R cs = (rx * v.x + ry * v.y + rz * v.z) * rid * v.id;
-        const R F = rfc * v.fc * fcc;
+        const R cs = (rx * bx + ry * by + rz * bz) * rid * bid;
+        const R F = ok * rfc * bfc * fcc;
         R Ex[NE];
 #pragma unroll
         for (int e = 0; e < NE; ++e)
-            Ex[e] = (LZ > 0 || e < ne) ? rE[e] * v.E[e] * dexp_(-eta_e[e] * d2c) * F : (R)0;
+            Ex[e] = (LZ > 0 || e < ne) ? rE[e] * bE[e] * dexp_(-eta_e[e] * d2c) * F : (R)0;
         ang_terms(cs, Ex);
     };
     if constexpr (sizeof(R) == 8) {
@@ -535,7 +521,7 @@ __device__ inline void desc_wave(const BpDesc& P, const Stage<R>& S, int nc, int
             if (!more) break;
         }
     } else {
-        circ_walk(nc, lane, load_row, load_b, [&](int, int, const BVals& v) { pair_terms_row(v); });
+        circ_walk(nc, lane, load_row, [&](int, int b, bool live) { pair_terms_row(b, live); });
     }
     if constexpr (K2) {
         // the angular sums of the pairs walked so far into block `key`
@@ -573,8 +559,8 @@ __device__ inline void desc_wave(const BpDesc& P, const Stage<R>& S, int nc, int
                 if (in_range(a, b)) pair_terms(a, b);
             }
         } else {
-            circ_walk(nd, lane, [&](int a) { load_row(d0 + a); }, [&](int b) { return load_b(d0 + b); },
-                      [&](int, int, const BVals& v) { pair_terms_row(v); });
+            circ_walk(nd, lane, [&](int a) { load_row(d0 + a); },
+                      [&](int, int b, bool live) { pair_terms_row(d0 + b, live); });
         }
         store_ang(2);
 #pragma unroll
@@ -880,7 +866,8 @@ __device__ inline void group_sums(R cs, const R* C, const R* Cz, const R* Ex, co
 // lane of a group receives the sum.
 template <int C>
 __device__ inline float dpp_f(float v) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), C, 0xF, 0xF, false));
+    // every lane reads a lane of its own row, so no lane keeps an old value
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), C, 0xF, 0xF, true));
 }
 __device__ inline float sum_lanes8(float v) {
     v += dpp_f<0xB1>(v);   // quad_perm [1, 0, 3, 2]
