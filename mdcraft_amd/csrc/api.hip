@@ -92,45 +92,49 @@ char* stage_area(Batch& b, size_t bytes) {
     return b.stage;
 }
 
-// Cubic spline slopes of the reference's tables (EAM.cxx:603-686): sweep
-// solve of the tridiagonal system with end condition ib (1: z[N-1] = 0,
-// 2: second-derivative end), including the reference's back-substitution
-// z[i] = t[i] for i < N-1.
-void spline_slopes(const double* y, double* z, double H1, size_t N, int ib) {
-    std::vector<double> a(N, 2.0), b(N), c(N), d(N), s(N + 1), t(N + 1), u(N + 1), v(N + 1), w(N + 1);
-    b[0] = 1.0;
-    c[0] = 0.0;
-    d[0] = 3.0 * H1 * (y[1] - y[0]);
-    b[N - 1] = 0.0;
-    if (ib == 1) {
-        c[N - 1] = 0.0;
-        d[N - 1] = 0.0;
-    } else {
-        c[N - 1] = 1.0;
-        d[N - 1] = 3.0 * H1 * (y[N - 1] - y[N - 2]);
+// Knot slopes of the tables' cubic Hermite splines (EAM.cxx:603-686).  With
+// h the knot spacing (H1 = 1/h) the rows are
+//   2 z[0] + z[1] = 3 (y[1] - y[0]) / h,
+//   z[k-1] / 2 + 2 z[k] + z[k+1] / 2 = 3 (y[k+1] - y[k-1]) / (2 h), 0 < k < N-1,
+//   end 1: z[N-1] = 0;  end 2: z[N-2] + 2 z[N-1] = 3 (y[N-1] - y[N-2]) / h.
+// The reference's sweep leaves z[0 .. N-2] as the solution of the first N-1
+// rows with z[N-1] taken as zero, and z[N-1] as the solution of the end row
+// with z[N-2] written through it; with end 1 that is the exact solution,
+// with end 2 the slopes below the end differ from it, and this does the
+// same.  Rows 0 .. N-2 are reduced from the top (z[k] = m[k] z[k+1] + q[k])
+// and solved from z[N-1] = 0 upwards.
+void spline_slopes(const double* y, double* z, double H1, size_t N, int end) {
+    std::vector<double> m(N), q(N);
+    auto row = [&](size_t k, double& lo, double& di, double& up, double& r) {
+        if (k == 0) {
+            lo = 0.0, di = 2.0, up = 1.0, r = 3.0 * H1 * (y[1] - y[0]);
+        } else if (k + 1 < N) {
+            lo = 0.5, di = 2.0, up = 0.5, r = 1.5 * H1 * (y[k + 1] - y[k - 1]);
+        } else if (end == 1) {
+            lo = 0.0, di = 2.0, up = 0.0, r = 0.0;
+        } else {
+            lo = 1.0, di = 2.0, up = 0.0, r = 3.0 * H1 * (y[N - 1] - y[N - 2]);
+        }
+    };
+    double mp = 0.0, qp = 0.0;  // m[k-1], q[k-1]
+    for (size_t k = 0; k + 1 < N; ++k) {
+        double lo, di, up, r;
+        row(k, lo, di, up, r);
+        const double piv = di + lo * mp;
+        m[k] = -up / piv;
+        q[k] = (r - lo * qp) / piv;
+        mp = m[k];
+        qp = q[k];
     }
-    for (size_t i = 1; i + 1 < N; ++i) {
-        c[i] = 0.5;
-        b[i] = 0.5;
-        d[i] = 1.5 * H1 * (y[i + 1] - y[i - 1]);
+    double zn = 0.0;  // z[N-1] as the upper rows see it
+    for (size_t k = N - 1; k-- > 0;) {
+        z[k] = m[k] * zn + q[k];
+        zn = z[k];
     }
-    u[0] = 0.0;
-    v[0] = 0.0;
-    w[0] = 1.0;
-    for (size_t i = 0; i < N; ++i) {
-        double rp = -1.0 / (a[i] + c[i] * v[i]);
-        v[i + 1] = rp * b[i];
-        u[i + 1] = rp * (c[i] * u[i] - d[i]);
-        w[i + 1] = rp * c[i] * w[i];
-    }
-    s[N - 1] = 1.0;
-    t[N - 1] = 0.0;
-    for (long i = (long)N - 2; i >= 0; --i) {
-        s[i] = v[i + 1] * s[i + 1] + w[i + 1];
-        t[i] = v[i + 1] * t[i + 1] + u[i + 1];
-    }
-    z[N - 1] = (d[N - 1] - b[N - 1] * t[0] - c[N - 1] * t[N - 2]) / (a[N - 1] + b[N - 1] * s[0] + c[N - 1] * s[N - 2]);
-    for (size_t i = 0; i + 1 < N; ++i) z[i] = t[i];
+    double lo, di, up, r;
+    row(N - 1, lo, di, up, r);
+    // z[N-2] = m[N-2] z[N-1] + q[N-2] in the end row
+    z[N - 1] = N > 1 ? (r - lo * q[N - 2]) / (di + lo * m[N - 2]) : 0.0;
 }
 
 // interval records (y0, z0, y1, z1); the last record repeats the last knot

@@ -198,28 +198,30 @@ __device__ inline const double4* rtab<double>(const EamK& P) { return P.r64; }
 template <>
 __device__ inline const float4* rtab<float>(const EamK& P) { return P.r32; }
 
-// EAM.cxx:562-589 embedding function and derivative (FP64) on the records
-// F64[off, off + nrho); flags a density outside the table, where the
-// reference's branch is undefined.
+// Embedding F(rho) and F'(rho) in FP64 (EAM.cxx:562-589) on the interval
+// records (y0, y0', y1, y1') of F at spacing h = drho: the cubic Hermite
+// polynomial of the interval.  With x = rho - k h, t = x / h and the secant
+// slope s = (y1 - y0) / h, and A = y0' + y1' - 2 s, B = 2 y0' + y1' - 3 s,
+//   F = y0 + x (y0' + t (A t - B)),  F' = y0' + t (3 A t - 2 B).
+// A density outside the table is flagged (the reference's branch for it is
+// undefined) and yields NaN.
 __device__ inline void embed_at(const double4* __restrict__ F64, int off, int nrho, double drho, double drho1,
                                 double rho, double& F, double& dF, int* misc) {
-    double RN = rho * drho1;
-    int N1 = (int)trunc(RN);
-    if (!(RN >= 0.0 && N1 < nrho)) {
+    const double u = rho * drho1;
+    const int k = (int)trunc(u);
+    if (!(u >= 0.0 && k < nrho)) {
         atomicOr(&misc[M_ERR], EB_RHO_RANGE);
         F = dF = nan("");
         return;
     }
-    const double4 c = F64[off + N1];
-    double VR = N1;
-    double DV = RN - VR;
-    RN = rho - VR * drho;
-    VR = (c.z - c.x) * drho1;
-    double aa = (c.w - VR) - (VR - c.y);
-    double bb = aa - (VR - c.y);
-    aa = aa * DV;
-    F = c.x + RN * (c.y + DV * (aa - bb));
-    dF = c.y + 2.0 * DV * (1.5 * aa - bb);
+    const double4 rec = F64[off + k];  // y0, y0', y1, y1'
+    const double t = u - (double)k;
+    const double x = rho - (double)k * drho;
+    const double sec = (rec.z - rec.x) * drho1;
+    const double A = (rec.y - sec) + (rec.w - sec);
+    const double B = A + (rec.y - sec);
+    F = rec.x + x * (rec.y + t * (A * t - B));
+    dF = rec.y + t * (3.0 * A * t - 2.0 * B);
 }
 
 // Pairs of one lane are processed kU at a time in predicated phases
