@@ -270,21 +270,27 @@ __device__ inline void build_tile(const int* c_j, const double* c_x, const doubl
 __global__ void __launch_bounds__(kBuildThreads) k_build(const double4* __restrict__ pos, const double* __restrict__ rns,
                                                const int* __restrict__ start, const SysDesc* __restrict__ desc,
                                                int* __restrict__ nl, int* __restrict__ nn, int* __restrict__ misc,
-                                               int tpa, int M, int cap) {
+                                               int tpa, int M, int cap, int split) {
     __shared__ int s_beg[32], s_pre[32];
     __shared__ unsigned s_img[32];  // image bits of each stencil cell (fast path)
     __shared__ int s_anyimg;        // some stencil cell lies across a periodic face
     __shared__ int c_j[kTile];
     __shared__ double c_x[kTile], c_y[kTile], c_z[kTile], c_r[kTile];
     __shared__ int s_cnt[64];
-    const int c = xcd_block(blockIdx.x, gridDim.x);
+    // split workgroups per cell, each with a slice of the cell's atoms (a
+    // multiple of the 8 atoms its waves take per pass) and the whole stencil
+    const int w = xcd_block(blockIdx.x, gridDim.x);
+    const int c = w / split, part = w - c * split;
     const int a0 = start[c], a1 = start[c + 1];
-    if (a0 == a1) return;  // uniform over the workgroup
+    const int slice = (((a1 - a0) + split - 1) / split + kGroup * kBuildWaves - 1) / (kGroup * kBuildWaves) *
+                      (kGroup * kBuildWaves);
+    const int b0 = a0 + part * slice, b1 = min(a1, b0 + slice);
+    if (b0 >= b1) return;  // uniform over the workgroup
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const SysDesc& s = desc[w_sys(pos[a0].w)];
     if (c == s.void_cell) {
         // void records of empty slot space: no list (uniform over the workgroup)
-        for (int a = a0 + tid; a < a1; a += kBuildThreads) nn[a] = 0;
+        for (int a = b0 + tid; a < b1; a += kBuildThreads) nn[a] = 0;
         return;
     }
     if (wv == 0) {
@@ -348,8 +354,8 @@ __global__ void __launch_bounds__(kBuildThreads) k_build(const double4* __restri
     int lt = 0;
     while ((1 << lt) < tpa) ++lt;
     const int lA = 6 - lt;  // log2 of the atoms per wavefront
-    for (int g0 = a0; g0 < a1; g0 += 64) {
-        const int ga = min(64, a1 - g0);
+    for (int g0 = b0; g0 < b1; g0 += 64) {
+        const int ga = min(64, b1 - g0);
         if (tid < 64) s_cnt[tid] = 0;
         for (int t0 = 0; t0 < total; t0 += kTile) {
             const int tn = min(kTile, total - t0);
@@ -498,9 +504,22 @@ static void grow_list(Batch& b, int maxnn) {
     b.alloc_gen++;
 }
 
+// One workgroup per cell, or, when the cells are too few to fill the
+// device (six resident build workgroups per CU), up to one per 8 of a
+// cell's atoms on average: each stages the whole stencil and lists its
+// slice of the cell's atoms, which shortens the longest workgroup's chain.
+static int build_split(const Batch& b) {
+    const long long room = 6LL * std::max(b.ctx->cu_count, 1);
+    const int by_room = (int)std::min<long long>(room / std::max(b.ncells, 1), 8);
+    const int by_atoms = (b.live + 8LL * b.ncells - 1) / (8LL * b.ncells);
+    return std::max(1, std::min(by_room, by_atoms));
+}
+
 static void launch_build(Batch& b) {
-    hipLaunchKernelGGL(k_build, dim3(b.ncells), dim3(kBuildThreads), 0, b.ctx->stream, b.pos[b.cur], b.d(b.sc, D_RNS),
-                       b.cell_start, b.d_desc, b.nl, b.nn, b.misc, b.tpa, b.M, b.cap);
+    const int split = build_split(b);
+    hipLaunchKernelGGL(k_build, dim3((unsigned)((long long)b.ncells * split)), dim3(kBuildThreads), 0, b.ctx->stream,
+                       b.pos[b.cur], b.d(b.sc, D_RNS), b.cell_start, b.d_desc, b.nl, b.nn, b.misc, b.tpa, b.M, b.cap,
+                       split);
 }
 
 // Build the list again after an overflow: grow it until every atom fits,
