@@ -244,46 +244,32 @@ struct PairWalk {
 
 // Every unordered pair of n staged neighbours once, as (a, b = a + d mod
 // n): d = 1 .. (n - 1) / 2 for every a, and d = n / 2 for a < n / 2 when n
-// is even.  Lane l takes the l-th contiguous 64th of these pairs in order
-// of a, then d, so it stays on one a for long runs and steps to the next
-// without a search; row(a) is called when the lane reaches a new row a (to
-// keep that neighbour's values in registers), fn(a, b, live) at every step:
-// live is false for a lane past the end of its run, whose (a, b) is then a
-// pair of staged neighbours to be ignored.  The body runs in every lane,
-// so the pair arithmetic sits in no divergent branch.
-template <class Row, class Fn>
-__device__ inline void circ_walk(int n, int lane, Row&& row, Fn&& fn) {
+// is even.  The pairs are taken in order of d, then a (index t = (d - 1) n
+// + a), 64 at a time, lane l at step k on index 64 k + l: the lanes of a
+// step hold consecutive a and consecutive b, so their LDS reads of both
+// neighbours' values fall in different banks, and a lane moves on by adding
+// 64 to its index (a += 64 mod n, d += 64 div n and the carry), with no
+// search.  fn(a, b, live) runs at every step in every lane; live is false
+// past the last pair, where (a, b) is a pair of staged neighbours to be
+// ignored.
+template <class Fn>
+__device__ inline void pair_walk(int n, int lane, Fn&& fn) {
     if (n < 2) return;
     const int hd = (n - 1) / 2;
     const int ex = (n % 2 == 0) ? n / 2 : 0;  // rows with one more partner (d = n / 2)
     const int T = n * hd + ex;
-    const int u0 = (int)(((long long)lane * T) / kWave), u1 = (int)(((long long)(lane + 1) * T) / kWave);
     const int steps = (T + kWave - 1) / kWave;
-    // row and partner of the first pair of the lane
-    int a, d;
-    if (u0 < ex * (hd + 1)) {
-        a = u0 / (hd + 1);
-        d = u0 - a * (hd + 1) + 1;
-    } else {
-        const int v = u0 - ex * (hd + 1);
-        a = ex + v / max(hd, 1);
-        d = v - (a - ex) * max(hd, 1) + 1;
-    }
-    int cur = -1;
-    for (int k = 0; k < steps; ++k) {
-        const bool live = u0 + k < u1;
-        if (live && a != cur) {
-            row(a);
-            cur = a;
-        }
-        // past its run a lane steps on by at most one row (a <= n), so b
-        // stays below n
+    const int q = kWave / n, r = kWave - q * n;  // 64 = q n + r
+    int a = lane % n, d = lane / n + 1;
+    for (int k = 0, t = lane; k < steps; ++k, t += kWave) {
         int b = a + d;
         if (b >= n) b -= n;
-        fn(a, b, live);
-        if (++d > hd + (a < ex ? 1 : 0)) {
-            d = 1;
-            ++a;
+        fn(a, min(b, n - 1), t < T);
+        a += r;
+        d += q;
+        if (a >= n) {
+            a -= n;
+            ++d;
         }
     }
 }
@@ -451,8 +437,7 @@ __device__ inline void desc_wave(const BpDesc& P, const Stage<R>& S, int nc, int
             Ex[e] = (LZ > 0 || e < ne) ? S.E[e * cap + a] * S.E[e * cap + b] * dexp_(-eta_e[e] * d2c) * F : (R)0;
         ang_terms(cs, Ex);
     };
-    // FP32 walks: the values of the lane's current row a in registers, the
-    // pair's terms from b's staged values
+    // FP32 walks: the values of the pair's row a and of b, read at each step
     R rx = 0, ry = 0, rz = 0, rid = 0, rfc = 0, rE[NE];
 #pragma unroll
     for (int e = 0; e < NE; ++e) rE[e] = (R)0;
@@ -521,7 +506,10 @@ __device__ inline void desc_wave(const BpDesc& P, const Stage<R>& S, int nc, int
             if (!more) break;
         }
     } else {
-        circ_walk(nc, lane, load_row, [&](int, int b, bool live) { pair_terms_row(b, live); });
+        pair_walk(nc, lane, [&](int a, int b, bool live) {
+            load_row(a);
+            pair_terms_row(b, live);
+        });
     }
     if constexpr (K2) {
         // the angular sums of the pairs walked so far into block `key`
@@ -559,8 +547,10 @@ __device__ inline void desc_wave(const BpDesc& P, const Stage<R>& S, int nc, int
                 if (in_range(a, b)) pair_terms(a, b);
             }
         } else {
-            circ_walk(nd, lane, [&](int a) { load_row(d0 + a); },
-                      [&](int, int b, bool live) { pair_terms_row(d0 + b, live); });
+            pair_walk(nd, lane, [&](int a, int b, bool live) {
+                load_row(d0 + a);
+                pair_terms_row(d0 + b, live);
+            });
         }
         store_ang(2);
 #pragma unroll
