@@ -686,7 +686,7 @@ def main():
                     help="--only eam_sizes: fcc cells per edge of the sizes to run (default 20 40 64 101)")
     ap.add_argument("--precision", choices=("mixed", "fp64", "both"), default="both",
                     help="--only bpnn_cu / nial_batch: precisions to run")
-    ap.add_argument("--only", choices=("bpnn_cu", "nial_batch", "nial_alloy", "served", "eam_sizes"),
+    ap.add_argument("--only", choices=("bpnn_cu", "nial_batch", "nial_alloy", "served", "eam_sizes", "lj_argon"),
                     help="run one of the extra workloads alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.summarise_pmc:
@@ -716,6 +716,11 @@ def main():
             if "mixed" in out:
                 out["mixed"]["valu"] = bpnn_valu(out["mixed"]["kernel_us_full_work"],
                                                  bpnn_units(wb["atoms"], wb["box"], wb["pot"]["rc"]), len(wb["atoms"]))
+        elif args.only == "lj_argon":
+            wl = W.lj_argon()
+            r = run_gpu(E, ctx, wl, E.FP64, 1000, args.warmup, args.lanes)
+            out = {"fp64": {kk: r[kk] for kk in ("value", "ms_per_step", "nbar", "rebuilds", "kernel_us", "b2b_us",
+                                                 "lanes")}}
         elif args.only == "eam_sizes":
             out = eam_sizes(E, ctx, W, args.lanes, **({"reps": tuple(args.eam_reps)} if args.eam_reps else {}))
         elif args.only in ("nial_batch", "nial_alloy"):
