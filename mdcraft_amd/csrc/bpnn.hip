@@ -1195,6 +1195,25 @@ __device__ inline float readlane_f(float v, int p) {
     return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), p));
 }
 
+// s + sum over p < cnt of w(p) times lane p's v, in order of p: the loop
+// runs to the widest layer in chunks of 8 (a loop with a run-time count and
+// the lane reads would not be unrolled), and each chunk's 8 weight reads are
+// issued before its products, so they are in flight together
+template <class W>
+__device__ inline float lane_dot(float s, float v, int cnt, W&& w) {
+#pragma unroll
+    for (int p0 = 0; p0 < kBpMaxHidden; p0 += 8) {
+        if (p0 < cnt) {
+            float wv[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) wv[j] = p0 + j < cnt ? w(p0 + j) : 0.f;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) s = fmaf(wv[j], readlane_f(v, p0 + j), s);
+        }
+    }
+    return s;
+}
+
 // The network of one centre by one wavefront in FP32 (maise_ann.cxx:8-55,
 // maise_bpnn.cxx:275-325): lane j holds unit j of each layer (widths up to
 // 64) and takes the previous layer's outputs one at a time from the lane
@@ -1231,17 +1250,14 @@ __device__ inline float mlp_wave(const BpDesc& P, const float* __restrict__ w, c
             if (q >= 0) s1 = fmaf(W01T[(nr + q) * h1 + j1], (float)P.t_coef[t] * acc[NR + g * NE + e], s1);
         }
     const float y1 = lane < h1 ? tanhf(s1) : 0.f;
-    float s2 = b12[j2];
-    for (int p = 0; p < h1; ++p) s2 = fmaf(W12T[p * h2 + j2], readlane_f(y1, p), s2);
+    const float s2 = lane_dot(b12[j2], y1, h1, [&](int p) { return W12T[p * h2 + j2]; });
     const float y2 = lane < h2 ? tanhf(s2) : 0.f;
     const float e2 = lane < h2 ? W23[j2] * y2 : 0.f;
     const float t2 = lane < h2 ? W23[j2] * (1.f - y2 * y2) : 0.f;
     E = b23 + wave_sum(e2);
-    float s3 = 0.f;
-    for (int q = 0; q < h2; ++q) s3 = fmaf(W12[q * h1 + j1], readlane_f(t2, q), s3);
+    const float s3 = lane_dot(0.f, t2, h2, [&](int q) { return W12[q * h1 + j1]; });
     const float d1 = lane < h1 ? s3 * (1.f - y1 * y1) : 0.f;
-    float so = 0.f;
-    for (int p = 0; p < h1; ++p) so = fmaf(W01[p * nin + k], readlane_f(d1, p), so);
+    const float so = lane_dot(0.f, d1, h1, [&](int p) { return W01[p * nin + k]; });
     return lane < nin ? so : 0.f;
 }
 
