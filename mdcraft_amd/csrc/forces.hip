@@ -1808,8 +1808,7 @@ void run_steps(Batch& b, const Pot& p, double dt, int nsteps, int prec, int chun
                                                   : (double)b.since_rebuild;
             b.since_rebuild = 0;
             b.cur = buf(ab);
-            nlist_rebuild(b, false);
-            if (qf) qfast_refresh(b);
+            nlist_rebuild(b, false);  // with b.qfast it also writes the sorted records
             timer_drain(b);
             c0 = b.cur ^ 1 ^ (ab & 1);
             s = launch_s = ab;  // the rebuild cleared the trigger

@@ -150,7 +150,8 @@ __global__ void k_scatter(const int* __restrict__ atom_cell, const int* __restri
 // shows.  Void records (the void cell) keep their scatter slot: their order
 // carries nothing, and ranking a cell of thousands of them would cost the
 // square of its size.  Also records the cell of each sorted atom and the
-// displacement reference positions.
+// displacement reference positions, and (uq_out given) the fixed-point
+// records of the sorted positions.
 __global__ void __launch_bounds__(256) k_permute(const int* __restrict__ src, const int* __restrict__ start,
                                                  const double4* __restrict__ pos_in, double4* __restrict__ pos_out,
                                                  const double* __restrict__ din, double* __restrict__ dout,
@@ -158,7 +159,8 @@ __global__ void __launch_bounds__(256) k_permute(const int* __restrict__ src, co
                                                  const int* __restrict__ oin, int* __restrict__ oout,
                                                  const int* __restrict__ cell_in, int* __restrict__ cell_out,
                                                  double4* __restrict__ rsave, int* __restrict__ where, int count, int n,
-                                                 int void_cell) {
+                                                 int void_cell, uint4* __restrict__ uq_out,
+                                                 const SysDesc* __restrict__ desc) {
     const int sp = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (sp >= count) return;
     const int s = src[sp];
@@ -173,6 +175,8 @@ __global__ void __launch_bounds__(256) k_permute(const int* __restrict__ src, co
     const double4 q = pos_in[s];
     pos_out[p] = q;
     rsave[p] = q;
+    // during a fixed-point step sequence: the sorted positions' records
+    if (uq_out) uq_out[p] = fixq4(q.x, q.y, q.z, q.w, desc[w_sys(q.w)]);
     for (int f = 0; f < D_NFIELDS; ++f) dout[(size_t)f * n + p] = din[(size_t)f * n + s];
     for (int f = 0; f < F_NFIELDS; ++f) fout[(size_t)f * n + p] = fin[(size_t)f * n + s];
     const int o = oin[s];
@@ -561,7 +565,8 @@ void nlist_rebuild(Batch& b, bool sync) {
     // cell_fill is free after the scatter: reuse it for the sorted cell ids
     hipLaunchKernelGGL(k_permute, dim3(nblk(count, 256)), dim3(256), 0, st, b.sorted_src, b.cell_start, b.pos[b.cur],
                        b.pos[b.cur ^ 1], b.dst[s0], b.dst[s1], b.fst[s0], b.fst[s1], b.orig[s0], b.orig[s1],
-                       b.atom_cell, b.cell_fill, b.rsave, b.where, count, n, b.ncells - 1);
+                       b.atom_cell, b.cell_fill, b.rsave, b.where, count, n, b.ncells - 1,
+                       b.qfast ? b.uq[b.cur ^ 1] : nullptr, b.d_desc);
     b.cur ^= 1;
     b.sc = s1;
     if (b.full_sort) {
