@@ -725,8 +725,8 @@ def main():
         print(summarise_counters(args.summarise_counters, args.kernel))
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU holding one domain; multi-GPU runs are not part of this build "
-              "(see the sharding notes in DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU holding one domain; runs on several GPUs are not supported "
+              "(the engine is one process on one GPU)", file=sys.stderr)
         sys.exit(2)
     cal = copy_calibration_buffers() if args.copy_calibration else None
     from mdcraft_amd import engine as E
