@@ -1,6 +1,6 @@
 // Pair (LJsCut) and two-pass EAM force kernels with the velocity-Verlet
 // update fused in, the standalone pre-drift kernel, the FP64 thermo
-// reduction and the multi-step driver.
+// reduction and the multi-step loop.
 //
 // Semantics restated from the reference:
 //   LJsCut force/virial  solver/potential/LJs-cut.cxx:22-131
